@@ -1,0 +1,299 @@
+/*
+ * pico_hip.h — C ABI of the MI355X (gfx950) rendering hot path of Pico.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, never
+ * throws, and returns an int status (PICO_OK == 0, negative on error; the
+ * text of the last error of a context is returned by pico_hip_last_error).
+ *
+ * Which reference interface each entry point replaces (reference files are
+ * cited relative to the Pico source tree):
+ *
+ *   pico_hip_upload_scene   Scene::Scene ctor + UpperLevelBVH::build
+ *                           (Source/Core/Scene.cpp:36-84, Source/Core/UpperLevelBVH.cpp:27-57,
+ *                           Source/Core/LowerLevelMeshBVH.cpp:11-79)
+ *   pico_hip_render         Scene::Scene::render_scene_to_memory
+ *                           (Source/Core/Scene.hpp:68, Source/Core/Scene.cpp:143-235)
+ *   pico_hip_gbuffer        Scene::generate_denoiser_inputs (Source/Core/Scene.cpp:248-294)
+ *   pico_hip_atrous         Util::atrous_denoise (Source/Util/Denoisers.hpp:9,
+ *                           Source/Util/Denoisers.cpp:107-126)
+ *   pico_hip_tonemap        Util::reinhard_tone_mapping (Source/Util/ToneMappers.cpp:12-44)
+ *   pico_hip_closest_hit    UpperLevelBVH::get_closest_intersection
+ *                           (Source/Core/UpperLevelBVH.cpp:12-20) for a batch of rays
+ *
+ * Buffers passed as `float*` RGB/vec3 images use the reference layout: packed
+ * 3 x f32 per pixel (glm::vec3, 12 bytes, no padding), pixel (x, y) at y*W+x.
+ * Entry points whose name ends in `_d` take device pointers (memory obtained
+ * from pico_hip_malloc); all others take host pointers and stage internally.
+ */
+#ifndef PICO_HIP_H
+#define PICO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PICO_OK              0
+#define PICO_ERR_ARG        -1   /* invalid argument                        */
+#define PICO_ERR_HIP        -2   /* HIP runtime error                       */
+#define PICO_ERR_STATE      -3   /* call out of order (e.g. no scene)       */
+#define PICO_ERR_CANCELLED  -4   /* cancel flag observed                    */
+#define PICO_ERR_LIMIT      -5   /* size beyond a documented limit          */
+#define PICO_ERR_NOMEM      -6   /* host or device allocation failed        */
+
+/* ---- scene description ------------------------------------------------- */
+/* BSDF kinds, Source/Render/BSRDF.hpp:33-40 and the "BSRDF" keys of
+ * Source/Core/Scene.cpp:367-417. */
+#define PICO_BSDF_DIFFUSE     0u
+#define PICO_BSDF_SPECULAR    1u
+#define PICO_BSDF_DIELECTRIC  2u
+#define PICO_BSDF_LIGHT       3u
+#define PICO_BSDF_DELTA       4u
+
+/* Microfacet distribution used by every specular lobe.  BECKMANN_ALL is the
+ * reference's Beckmann_All_Microfacet_Distribution
+ * (Source/Render/Distributions.cpp:31-78); GGX is a new Distribution subclass
+ * (Walter et al. 2007) plugged into the same interface
+ * (Source/Render/Distributions.hpp:10-21). */
+#define PICO_MICROFACET_BECKMANN_ALL 0u
+#define PICO_MICROFACET_GGX          1u
+
+/* Core::EvaluatedMaterial (Source/Core/MaterialManager.hpp:17-46) of a
+ * constant material, without the unused normal field. */
+typedef struct pico_material {
+    float diffuse[3];
+    float specular[3];
+    float roughness;
+    float emissive[3];
+} pico_material;
+
+/* Flattened scene.  Meshes are object-space triangle soups with shared
+ * vertices; instances reference a mesh, a material and a BSDF kind with a 4x4
+ * column-major transform (glm::mat4 layout).  Lights are instances whose BSDF
+ * is PICO_BSDF_LIGHT, listed in light-selection order (Scene::m_lights,
+ * Source/Core/Scene.cpp:407-411). */
+typedef struct pico_scene_desc {
+    uint32_t        n_meshes;
+    const uint32_t* mesh_vertex_offset;    /* n_meshes + 1 */
+    const uint32_t* mesh_triangle_offset;  /* n_meshes + 1 */
+    const float*    positions;             /* 3 x f32 per vertex */
+    const float*    normals;               /* 3 x f32 per vertex */
+    const uint32_t* indices;               /* 3 x u32 per triangle, mesh-local */
+    uint32_t        n_instances;
+    const float*    transforms;            /* 16 x f32 per instance */
+    const uint32_t* instance_mesh;
+    const uint32_t* instance_material;
+    const uint32_t* instance_bsdf;
+    uint32_t        n_materials;
+    const pico_material* materials;
+    uint32_t        n_lights;
+    const uint32_t* light_instances;
+    float           sky_rgb[3];            /* constant sky (1x1 cube map) */
+    uint32_t        microfacet;            /* PICO_MICROFACET_* */
+} pico_scene_desc;
+
+/* Scene::Camera (Source/Core/Camera.hpp:14-35): pinhole with
+ * tan(half-FOV) = 0.5, up = (0,1,0); the far plane is the miss distance of
+ * G-buffer rays. */
+typedef struct pico_camera {
+    float position[3];
+    float direction[3];
+    float up[3];
+    float aspect;
+    float far_plane;
+} pico_camera;
+
+/* Scene::RenderParams (Source/Core/Scene.hpp:30-43) minus the buffers, plus
+ * the frame seed of the per-pixel PCG32 streams and the tile size used by the
+ * tile-local denoiser constants. */
+typedef struct pico_render_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;            /* m_maxSamples                               */
+    uint32_t max_depth;      /* m_maxRayDepth                              */
+    uint32_t tile_w;         /* tile size (Util::Tiler)                    */
+    uint32_t tile_h;
+    uint32_t frame_seed;     /* seeds PCG32: state=mix(frame,sample), seq=pixel */
+    uint32_t view_id;        /* batched views: folded into the seed         */
+    float    max_variance;   /* m_maxVariance (early-out)                  */
+    uint32_t flags;          /* PICO_RENDER_*                              */
+} pico_render_params;
+
+#define PICO_RENDER_RESUME  1u  /* accumulate onto the caller's buffers (sample counts > 0) */
+#define PICO_RENDER_COUNT   2u  /* count BVH node visits and triangle tests (slower) */
+
+typedef struct pico_stats {
+    uint64_t camera_rays;
+    uint64_t extension_rays;
+    uint64_t shadow_rays;
+    uint64_t gbuffer_rays;
+    uint64_t node_visits;    /* PICO_RENDER_COUNT only */
+    uint64_t tri_tests;      /* PICO_RENDER_COUNT only */
+    uint64_t instance_visits;/* PICO_RENDER_COUNT only */
+    uint64_t samples;        /* (pixel, sample) pairs accumulated */
+    uint64_t accumulated_rays; /* rays of the accumulated samples only */
+    double   render_ms;      /* device time of the render, HIP events */
+    int32_t  cancelled;
+    int32_t  pad;
+} pico_stats;
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct pico_hip_ctx pico_hip_ctx;
+
+int  pico_hip_create(int device, pico_hip_ctx** out);
+void pico_hip_destroy(pico_hip_ctx* ctx);
+const char* pico_hip_last_error(const pico_hip_ctx* ctx);
+int  pico_hip_synchronize(pico_hip_ctx* ctx);
+int  pico_hip_device_name(pico_hip_ctx* ctx, char* buf, size_t n);
+
+/* Device memory owned by the caller, used by the `_d` entry points. */
+int  pico_hip_malloc(pico_hip_ctx* ctx, size_t bytes, void** dptr);
+int  pico_hip_free(pico_hip_ctx* ctx, void* dptr);
+int  pico_hip_memcpy_h2d(pico_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int  pico_hip_memcpy_d2h(pico_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int  pico_hip_memset_d(pico_hip_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* Per-kernel timing with HIP events on the context's stream.  When enabled,
+ * every launch of every kernel is bracketed by events; pico_hip_kernel_time
+ * returns the launch count and mean duration (ms) of the named kernel since
+ * the last reset. */
+int  pico_hip_enable_kernel_timing(pico_hip_ctx* ctx, int enable);
+int  pico_hip_kernel_time(pico_hip_ctx* ctx, const char* kernel, uint64_t* launches, double* mean_ms);
+
+/* ---- scene ------------------------------------------------------------- */
+/* Builds the BLAS (one per mesh, 16-bin binned SAH) and the TLAS over
+ * instances on the host, lays them out for the device and uploads them. */
+int  pico_hip_upload_scene(pico_hip_ctx* ctx, const pico_scene_desc* scene);
+/* Node/triangle/instance byte sizes and tree depths of the uploaded scene. */
+int  pico_hip_scene_info(pico_hip_ctx* ctx, uint64_t* bytes_nodes, uint64_t* bytes_tris,
+                         uint64_t* bytes_instances, uint32_t* tlas_depth, uint32_t* blas_depth);
+
+/* Closest hit of n world-space rays (origins/dirs: 3 x f32 each).  Outputs
+ * t (world distance), instance, mesh-local primitive and barycentrics (u, v);
+ * misses give t = +inf and instance = prim = 0xffffffff. */
+int  pico_hip_closest_hit(pico_hip_ctx* ctx, uint32_t n, const float* origins, const float* dirs,
+                          float* t, uint32_t* instance, uint32_t* prim, float* uv);
+
+/* Primary-hit ids of every pixel for a fixed sub-pixel sample xi (the G-buffer
+ * pass uses xi = (0,0), Source/Core/Scene.cpp:271). */
+int  pico_hip_primary_ids(pico_hip_ctx* ctx, const pico_camera* cam, uint32_t width, uint32_t height,
+                          float xi_x, float xi_y, uint32_t* instance, uint32_t* prim);
+
+/* ---- render ------------------------------------------------------------ */
+/* Path-traces the frame and accumulates into the reference's buffers:
+ * pixels / variance 3 x f32 per pixel, sample_count u32 per pixel (running
+ * mean/variance update and early-out of Source/Core/Scene.cpp:163-197).
+ * cancel (may be NULL) is polled between wavefront passes. */
+int  pico_hip_render(pico_hip_ctx* ctx, const pico_camera* cam, const pico_render_params* params,
+                     float* pixels, uint32_t* sample_count, float* variance,
+                     const volatile int32_t* cancel, pico_stats* stats);
+int  pico_hip_render_d(pico_hip_ctx* ctx, const pico_camera* cam, const pico_render_params* params,
+                       float* d_pixels, uint32_t* d_sample_count, float* d_variance,
+                       const volatile int32_t* cancel, pico_stats* stats);
+
+/* Radiance of every (pixel, sample) with sample index in
+ * [sample_base, sample_base + params->spp), without accumulation: out is
+ * spp x H x W x 3 f32 (sample-major).  Used for per-path parity checks. */
+int  pico_hip_render_samples(pico_hip_ctx* ctx, const pico_camera* cam, const pico_render_params* params,
+                             uint32_t sample_base, float* out);
+
+/* FP64 accumulation of spp samples per pixel (no early-out) into
+ * d_sum (3 x f64 per pixel, added to), for the converged-reference oracle. */
+int  pico_hip_render_accum64_d(pico_hip_ctx* ctx, const pico_camera* cam, const pico_render_params* params,
+                               uint32_t sample_base, double* d_sum, pico_stats* stats);
+
+/* G-buffer of Source/Core/Scene.cpp:248-294: world normal, view-space
+ * position (glm::lookAt view matrix) and diffuse albedo, 3 x f32 each. */
+int  pico_hip_gbuffer(pico_hip_ctx* ctx, const pico_camera* cam, uint32_t width, uint32_t height,
+                      float* normal, float* position, float* albedo);
+int  pico_hip_gbuffer_d(pico_hip_ctx* ctx, const pico_camera* cam, uint32_t width, uint32_t height,
+                        float* d_normal, float* d_position, float* d_albedo);
+
+/* ---- denoise / post ---------------------------------------------------- */
+/* A-trous wavelet filter of Source/Util/Denoisers.cpp:18-126: `levels`
+ * passes at dilation 2^l, edge-stopping on normal, position and albedo with
+ * tile-local range constants over tiles of tile_w x tile_h. */
+int  pico_hip_atrous(pico_hip_ctx* ctx, const float* rgb, const float* normal, const float* position,
+                     const float* albedo, uint32_t width, uint32_t height, uint32_t tile_w,
+                     uint32_t tile_h, uint32_t levels, float* out_rgb);
+/* Device variant; d_scratch must hold width*height*3 f32 (ping-pong). */
+int  pico_hip_atrous_d(pico_hip_ctx* ctx, const float* d_rgb, const float* d_normal, const float* d_position,
+                       const float* d_albedo, uint32_t width, uint32_t height, uint32_t tile_w,
+                       uint32_t tile_h, uint32_t levels, float* d_scratch, float* d_out);
+
+/* Extended Reinhard with a global luminance white point, in place. */
+int  pico_hip_tonemap(pico_hip_ctx* ctx, float* rgb, uint32_t width, uint32_t height);
+
+/* relMSE = mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2) with the
+ * reference r = d_ref_sum / ref_count (FP64), reduced in FP64. */
+int  pico_hip_relmse_d(pico_hip_ctx* ctx, const float* d_img, const double* d_ref_sum, double ref_count,
+                       uint32_t n_pixels, double* out);
+
+/* ---- learned denoiser (kernel-predicting U-Net) ------------------------ */
+/* Input: 10 channels per pixel (radiance 3, albedo 3, normal 3, depth 1),
+ * planar layout [10][H][W] f32.  Network: 3-level U-Net 32/64/128 with 3x3
+ * convolutions (BF16 weights/activations on MFMA, FP32 accumulation) and a
+ * 1x1 head predicting a 5x5 per-pixel kernel applied to the radiance. */
+typedef struct pico_cnn pico_cnn;
+int  pico_hip_cnn_create(pico_hip_ctx* ctx, uint32_t max_width, uint32_t max_height, uint32_t max_batch,
+                         pico_cnn** out);
+void pico_hip_cnn_destroy(pico_cnn* cnn);
+uint32_t pico_hip_cnn_num_params(void);
+int  pico_hip_cnn_init(pico_cnn* cnn, uint32_t seed);
+int  pico_hip_cnn_set_params(pico_cnn* cnn, const float* params);
+int  pico_hip_cnn_get_params(pico_cnn* cnn, float* params);
+int  pico_hip_cnn_denoise(pico_cnn* cnn, const float* inputs, uint32_t width, uint32_t height, float* out_rgb);
+int  pico_hip_cnn_denoise_d(pico_cnn* cnn, const float* d_inputs, uint32_t width, uint32_t height, float* d_out_rgb);
+/* Loss (mean relMSE of the filtered radiance vs target, FP64 reduction) and
+ * FP32 gradients of every parameter for one batch (host buffers). */
+int  pico_hip_cnn_loss_grad(pico_cnn* cnn, const float* inputs, const float* targets, uint32_t batch,
+                            uint32_t width, uint32_t height, double* loss, float* grads);
+/* One Adam step on a device-resident batch. */
+int  pico_hip_cnn_train_step_d(pico_cnn* cnn, const float* d_inputs, const float* d_targets, uint32_t batch,
+                               uint32_t width, uint32_t height, float lr, double* loss);
+
+/* ---- batched uneven views ---------------------------------------------- */
+typedef struct pico_view {
+    pico_camera        camera;
+    pico_render_params params;
+    float*    d_pixels;         /* device, W*H*3 */
+    uint32_t* d_sample_count;   /* device, W*H   */
+    float*    d_variance;       /* device, W*H*3 */
+} pico_view;
+
+/* Renders n views in one wavefront: every pass of the path tracer covers the
+ * live paths of all views.  stats may be NULL or hold n entries.  queue_paths
+ * bounds the number of paths in flight (0 = automatic). */
+int  pico_hip_render_views_d(pico_hip_ctx* ctx, const pico_view* views, uint32_t n, uint32_t queue_paths,
+                             pico_stats* stats);
+
+/* Launch-shape knobs chosen by the autotuner: paths in flight per wavefront
+ * pass and workgroup size of the traversal kernels. */
+int  pico_hip_set_tuning(pico_hip_ctx* ctx, uint32_t queue_paths, uint32_t trace_block);
+int  pico_hip_get_tuning(pico_hip_ctx* ctx, uint32_t* queue_paths, uint32_t* trace_block);
+
+/* ---- procedural scenes (inputs of the configurations) ------------------ */
+typedef struct pico_scene pico_scene;
+/* Cornell box: 5 walls x 2 tris, two boxes of 12 tris, one quad light,
+ * Dielectric walls/boxes with albedo U[0.2,0.8], roughness U[0.1,0.9]. */
+int  pico_scene_cornell(uint32_t seed, uint32_t microfacet, pico_scene** out);
+/* Instanced scene: n_mesh_types meshes (icospheres and noise-displaced tori
+ * of min_tris..max_tris triangles, geometric), n_instances random affine
+ * instances in a box of side `extent`, 3 quad lights and a constant sky. */
+int  pico_scene_instanced(uint32_t seed, uint32_t n_mesh_types, uint32_t n_instances, uint32_t min_tris,
+                          uint32_t max_tris, float extent, uint32_t microfacet, pico_scene** out);
+/* Small randomised scenes used to train the learned denoiser. */
+int  pico_scene_training(uint32_t seed, uint32_t microfacet, pico_scene** out);
+int  pico_scene_desc_get(const pico_scene* scene, pico_scene_desc* out);
+/* A camera that frames the scene as its generator intends. */
+int  pico_scene_default_camera(const pico_scene* scene, float aspect, pico_camera* out);
+void pico_scene_destroy(pico_scene* scene);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PICO_HIP_H */

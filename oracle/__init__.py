@@ -1,0 +1,259 @@
+"""TEST INFRASTRUCTURE ONLY — the oracle.
+
+ctypes wrappers of
+  * ``libcpu_ref.so``: the CPU restatement of Pico's rendering hot path
+    (``oracle/cpu_ref.cpp``, ``oracle/cnn_ref.cpp``), and
+  * ``_ref/libpico_ref.so``: Pico's own hot-path sources compiled from the
+    reference tree plus ``oracle/ref_harness.cpp`` (present only where it was
+    built; the tests that need it skip otherwise).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this package.  Parity status of every oracle function
+(what pins it) is listed in DESIGN.md and exercised in tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from pico_amd import Camera, RenderParams, SceneDesc, Stats, Material  # plain ctypes structs of the C header
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+CPU_REF_PATH = os.path.join(_HERE, "libcpu_ref.so")
+REF_PATH = os.path.join(_HERE, "_ref", "libpico_ref.so")
+
+RNG_REFERENCE, RNG_PCG = 0, 1
+DIST_COS, DIST_BECKMANN, DIST_GGX = 0, 1, 2
+
+_cr = None
+_rf = None
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f32(a, shape):
+    return np.ascontiguousarray(a, dtype=np.float32).reshape(shape)
+
+
+def cpu_lib():
+    global _cr
+    if _cr is None:
+        if not os.path.exists(CPU_REF_PATH):
+            raise RuntimeError(f"{CPU_REF_PATH} missing: build the oracle (make -C oracle)")
+        _cr = C.CDLL(CPU_REF_PATH)
+        vp, u32, u64, f32, f64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_double, C.c_int32
+        P = C.POINTER
+        sigs = {
+            "cpuref_scene_create": (vp, [P(SceneDesc), u32]),
+            "cpuref_scene_destroy": (None, [vp]),
+            "cpuref_scene_node_count": (u32, [vp, P(u32), P(u32)]),
+            "cpuref_closest_hit": (None, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "cpuref_closest_hit_brute": (None, [vp, u32, vp, vp, vp, vp, vp]),
+            "cpuref_primary_ids": (None, [vp, P(Camera), u32, u32, f32, f32, vp, vp]),
+            "cpuref_camera_rays": (None, [P(Camera), u32, u32, u32, vp, vp, vp, vp]),
+            "cpuref_pixel_seed": (u32, [u32, u32]),
+            "cpuref_render": (i32, [vp, P(Camera), P(RenderParams), u32, u32, vp, vp, vp, P(Stats)]),
+            "cpuref_render_samples": (i32, [vp, P(Camera), P(RenderParams), u32, u32, u32, vp, vp]),
+            "cpuref_gbuffer": (None, [vp, P(Camera), u32, u32, vp, vp, vp]),
+            "cpuref_atrous": (None, [vp, vp, vp, vp, u32, u32, u32, u32, u32, vp]),
+            "cpuref_tonemap": (None, [vp, u32, u32]),
+            "cpuref_relmse": (f64, [vp, vp, f64, u32]),
+            "cpuref_dist_sample": (None, [u32, u32, vp, vp, vp, vp]),
+            "cpuref_dist_pdf": (None, [u32, u32, vp, vp, vp, vp]),
+            "cpuref_tangent_frame": (None, [u32, vp, vp, vp]),
+            "cpuref_xorshift": (None, [u32, u32, vp]),
+            "cpuref_hammersley": (None, [u32, vp, vp]),
+            "cpuref_pcg32": (None, [u64, u64, u32, vp]),
+            "cpuref_pcg32_advance": (None, [u64, u64, u64, u32, vp]),
+            "cpuref_alias_build": (None, [u32, vp, vp, vp]),
+            "cpuref_alias_sample": (u32, [u32, vp, vp, u32, u32, vp]),
+            "cpuref_triangle_areas": (None, [u32, vp, vp, vp]),
+            "cpuref_bsdf_sample": (None, [u32, u32, P(Material), u32, vp, vp, u32, u64, vp, vp, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            if hasattr(_cr, name):
+                fn = getattr(_cr, name)
+                fn.restype = res
+                fn.argtypes = args
+    return _cr
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref_lib():
+    global _rf
+    if _rf is None:
+        if not ref_available():
+            raise RuntimeError(f"{REF_PATH} missing (reference tree not available when the oracle was built)")
+        _rf = C.CDLL(REF_PATH)
+        vp, u32, f32, i32 = C.c_void_p, C.c_uint32, C.c_float, C.c_int32
+        P = C.POINTER
+        sigs = {
+            "ref_scene_create": (vp, [P(SceneDesc)]),
+            "ref_scene_destroy": (None, [vp]),
+            "ref_closest_hit": (None, [vp, u32, vp, vp, vp, vp, vp, vp, vp]),
+            "ref_camera_rays": (None, [P(Camera), u32, u32, u32, vp, vp, vp, vp]),
+            "ref_primary_ids": (None, [vp, P(Camera), u32, u32, f32, f32, vp, vp]),
+            "ref_render_samples": (None, [vp, P(Camera), P(RenderParams), vp]),
+            "ref_render": (i32, [vp, P(Camera), P(RenderParams), u32, vp, vp, vp]),
+            "ref_gbuffer": (None, [vp, P(Camera), u32, u32, vp, vp, vp]),
+            "ref_atrous": (None, [vp, vp, vp, vp, u32, u32, u32, u32, vp]),
+            "ref_tonemap": (None, [vp, u32, u32]),
+            "ref_dist_sample": (None, [u32, u32, vp, vp, vp, vp]),
+            "ref_dist_pdf": (None, [u32, u32, vp, vp, vp, vp]),
+            "ref_tangent_frame": (None, [u32, vp, vp, vp]),
+            "ref_xorshift": (None, [u32, u32, vp]),
+            "ref_hammersley": (None, [u32, vp, vp]),
+            "ref_alias_sample": (None, [u32, vp, u32, u32, vp]),
+            "ref_bsdf_sample": (None, [u32, P(Material), u32, vp, vp, u32, vp, vp, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(_rf, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _rf
+
+
+class CpuScene:
+    """The restatement's scene (reference-style BVH2, `buckets`-bucket SAH)."""
+
+    def __init__(self, scene, buckets=12):
+        self.scene = scene  # keeps the description's arrays alive
+        self.h = C.c_void_p(cpu_lib().cpuref_scene_create(C.byref(scene.desc), buckets))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _cr is not None:
+            _cr.cpuref_scene_destroy(self.h)
+            self.h = None
+
+    def closest_hit(self, origins, dirs, brute=False):
+        o, d = _f32(origins, (-1, 3)), _f32(dirs, (-1, 3))
+        n = o.shape[0]
+        t = np.empty(n, np.float32)
+        inst = np.empty(n, np.uint32)
+        prim = np.empty(n, np.uint32)
+        if brute:
+            cpu_lib().cpuref_closest_hit_brute(self.h, n, _p(o), _p(d), _p(t), _p(inst), _p(prim))
+            return t, inst, prim
+        uv = np.empty((n, 2), np.float32)
+        pos = np.empty((n, 3), np.float32)
+        nrm = np.empty((n, 3), np.float32)
+        cpu_lib().cpuref_closest_hit(self.h, n, _p(o), _p(d), _p(t), _p(inst), _p(prim), _p(uv), _p(pos), _p(nrm))
+        return t, inst, prim, uv, pos, nrm
+
+    def primary_ids(self, cam, w, h, xi=(0.0, 0.0)):
+        inst = np.empty(w * h, np.uint32)
+        prim = np.empty(w * h, np.uint32)
+        cpu_lib().cpuref_primary_ids(self.h, C.byref(cam), w, h, xi[0], xi[1], _p(inst), _p(prim))
+        return inst, prim
+
+    def render(self, cam, p, rng=RNG_PCG, threads=0):
+        n = p.width * p.height
+        pix = np.zeros((n, 3), np.float32)
+        cnt = np.zeros(n, np.uint32)
+        var = np.zeros((n, 3), np.float32)
+        st = Stats()
+        rc = cpu_lib().cpuref_render(self.h, C.byref(cam), C.byref(p), rng, threads, _p(pix), _p(cnt), _p(var), C.byref(st))
+        assert rc == 0
+        return pix, cnt, var, st
+
+    def render_samples(self, cam, p, rng=RNG_PCG, sample_base=0, threads=0):
+        out = np.empty((p.spp, p.width * p.height, 3), np.float32)
+        rays = np.empty((p.spp, p.width * p.height), np.uint32)
+        cpu_lib().cpuref_render_samples(self.h, C.byref(cam), C.byref(p), rng, threads, sample_base, _p(out), _p(rays))
+        return out, rays
+
+    def gbuffer(self, cam, w, h):
+        n = np.empty((w * h, 3), np.float32)
+        p = np.empty((w * h, 3), np.float32)
+        a = np.empty((w * h, 3), np.float32)
+        cpu_lib().cpuref_gbuffer(self.h, C.byref(cam), w, h, _p(n), _p(p), _p(a))
+        return n, p, a
+
+
+def atrous(rgb, normal, position, albedo, w, h, tile=(16, 16), levels=5):
+    args = [_f32(x, (-1, 3)) for x in (rgb, normal, position, albedo)]
+    out = np.empty((w * h, 3), np.float32)
+    cpu_lib().cpuref_atrous(*[_p(a) for a in args], w, h, tile[0], tile[1], levels, _p(out))
+    return out
+
+
+def tonemap(rgb, w, h):
+    out = _f32(rgb, (-1, 3)).copy()
+    cpu_lib().cpuref_tonemap(_p(out), w, h)
+    return out
+
+
+def relmse(img, ref_sum, ref_count):
+    img = _f32(img, (-1, 3))
+    ref = np.ascontiguousarray(ref_sum, dtype=np.float64).reshape(-1, 3)
+    return cpu_lib().cpuref_relmse(_p(img), _p(ref), float(ref_count), img.shape[0])
+
+
+class RefScene:
+    """Pico's own hot path (compiled reference subset) on the same scene."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self.h = C.c_void_p(ref_lib().ref_scene_create(C.byref(scene.desc)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _rf is not None:
+            _rf.ref_scene_destroy(self.h)
+            self.h = None
+
+    def closest_hit(self, origins, dirs):
+        o, d = _f32(origins, (-1, 3)), _f32(dirs, (-1, 3))
+        n = o.shape[0]
+        t = np.empty(n, np.float32)
+        inst = np.empty(n, np.uint32)
+        prim = np.empty(n, np.uint32)
+        pos = np.empty((n, 3), np.float32)
+        nrm = np.empty((n, 3), np.float32)
+        ref_lib().ref_closest_hit(self.h, n, _p(o), _p(d), _p(t), _p(inst), _p(prim), _p(pos), _p(nrm))
+        return t, inst, prim, pos, nrm
+
+    def primary_ids(self, cam, w, h, xi=(0.0, 0.0)):
+        inst = np.empty(w * h, np.uint32)
+        prim = np.empty(w * h, np.uint32)
+        ref_lib().ref_primary_ids(self.h, C.byref(cam), w, h, xi[0], xi[1], _p(inst), _p(prim))
+        return inst, prim
+
+    def render_samples(self, cam, p):
+        out = np.empty((p.spp, p.width * p.height, 3), np.float32)
+        ref_lib().ref_render_samples(self.h, C.byref(cam), C.byref(p), _p(out))
+        return out
+
+    def render(self, cam, p, threads=0):
+        n = p.width * p.height
+        pix = np.zeros((n, 3), np.float32)
+        cnt = np.zeros(n, np.uint32)
+        var = np.zeros((n, 3), np.float32)
+        ref_lib().ref_render(self.h, C.byref(cam), C.byref(p), threads, _p(pix), _p(cnt), _p(var))
+        return pix, cnt, var
+
+    def gbuffer(self, cam, w, h):
+        n = np.empty((w * h, 3), np.float32)
+        p = np.empty((w * h, 3), np.float32)
+        a = np.empty((w * h, 3), np.float32)
+        ref_lib().ref_gbuffer(self.h, C.byref(cam), w, h, _p(n), _p(p), _p(a))
+        return n, p, a
+
+
+def ref_atrous(rgb, normal, position, albedo, w, h, tile=(16, 16)):
+    args = [_f32(x, (-1, 3)) for x in (rgb, normal, position, albedo)]
+    out = np.empty((w * h, 3), np.float32)
+    ref_lib().ref_atrous(*[_p(a) for a in args], w, h, tile[0], tile[1], _p(out))
+    return out
+
+
+def ref_tonemap(rgb, w, h):
+    out = _f32(rgb, (-1, 3)).copy()
+    ref_lib().ref_tonemap(_p(out), w, h)
+    return out

@@ -1,0 +1,411 @@
+"""pico_amd — Python binding of the MI355X (gfx950) rendering hot path of Pico.
+
+The compute path is the C ABI of ``libpico_hip.so`` (declared in
+``include/pico_hip.h``); this module only wraps it with ctypes for tests,
+benchmarks and scripts.  There is no fallback: if the library is missing, or
+no GPU is visible, the calls raise.
+
+Reference interfaces mirrored here (Pico source tree):
+  * ``Context.render``   -> Scene::Scene::render_scene_to_memory (Source/Core/Scene.cpp:143-235)
+  * ``Context.gbuffer``  -> Scene::generate_denoiser_inputs (Source/Core/Scene.cpp:248-294)
+  * ``Context.atrous``   -> Util::atrous_denoise (Source/Util/Denoisers.cpp:107-126)
+  * ``Context.tonemap``  -> Util::reinhard_tone_mapping (Source/Util/ToneMappers.cpp:12-44)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpico_hip.so")
+
+PICO_OK = 0
+PICO_ERR_ARG = -1
+PICO_ERR_HIP = -2
+PICO_ERR_STATE = -3
+PICO_ERR_CANCELLED = -4
+PICO_ERR_LIMIT = -5
+PICO_ERR_NOMEM = -6
+
+BSDF_DIFFUSE, BSDF_SPECULAR, BSDF_DIELECTRIC, BSDF_LIGHT, BSDF_DELTA = range(5)
+MICROFACET_BECKMANN_ALL, MICROFACET_GGX = 0, 1
+RENDER_RESUME, RENDER_COUNT = 1, 2
+
+
+class PicoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pico error {code}: {msg}")
+        self.code = code
+
+
+class Material(C.Structure):
+    _fields_ = [("diffuse", C.c_float * 3), ("specular", C.c_float * 3), ("roughness", C.c_float),
+                ("emissive", C.c_float * 3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("n_meshes", C.c_uint32), ("mesh_vertex_offset", C.c_void_p), ("mesh_triangle_offset", C.c_void_p),
+                ("positions", C.c_void_p), ("normals", C.c_void_p), ("indices", C.c_void_p),
+                ("n_instances", C.c_uint32), ("transforms", C.c_void_p), ("instance_mesh", C.c_void_p),
+                ("instance_material", C.c_void_p), ("instance_bsdf", C.c_void_p), ("n_materials", C.c_uint32),
+                ("materials", C.c_void_p), ("n_lights", C.c_uint32), ("light_instances", C.c_void_p),
+                ("sky_rgb", C.c_float * 3), ("microfacet", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("up", C.c_float * 3),
+                ("aspect", C.c_float), ("far_plane", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
+                ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("frame_seed", C.c_uint32),
+                ("view_id", C.c_uint32), ("max_variance", C.c_float), ("flags", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("gbuffer_rays", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("instance_visits", C.c_uint64), ("samples", C.c_uint64), ("accumulated_rays", C.c_uint64),
+                ("render_ms", C.c_double), ("cancelled", C.c_int32), ("pad", C.c_int32)]
+
+    @property
+    def rays(self):
+        return self.camera_rays + self.extension_rays + self.shadow_rays + self.gbuffer_rays
+
+
+class View(C.Structure):
+    _fields_ = [("camera", Camera), ("params", RenderParams), ("d_pixels", C.c_void_p),
+                ("d_sample_count", C.c_void_p), ("d_variance", C.c_void_p)]
+
+
+def params(width, height, spp, max_depth, tile=16, frame_seed=0, view_id=0, max_variance=0.0, flags=0):
+    return RenderParams(width, height, spp, max_depth, tile, tile, frame_seed, view_id, max_variance, flags)
+
+
+_lib = None
+
+
+def _declare(lib):
+    vp, u32, i32, f32, f64, u64, sz = C.c_void_p, C.c_uint32, C.c_int32, C.c_float, C.c_double, C.c_uint64, C.c_size_t
+    P = C.POINTER
+    sig = {
+        "pico_hip_create": (i32, [i32, P(vp)]),
+        "pico_hip_destroy": (None, [vp]),
+        "pico_hip_last_error": (C.c_char_p, [vp]),
+        "pico_hip_synchronize": (i32, [vp]),
+        "pico_hip_device_name": (i32, [vp, C.c_char_p, sz]),
+        "pico_hip_malloc": (i32, [vp, sz, P(vp)]),
+        "pico_hip_free": (i32, [vp, vp]),
+        "pico_hip_memcpy_h2d": (i32, [vp, vp, vp, sz]),
+        "pico_hip_memcpy_d2h": (i32, [vp, vp, vp, sz]),
+        "pico_hip_memset_d": (i32, [vp, vp, i32, sz]),
+        "pico_hip_enable_kernel_timing": (i32, [vp, i32]),
+        "pico_hip_kernel_time": (i32, [vp, C.c_char_p, P(u64), P(f64)]),
+        "pico_hip_upload_scene": (i32, [vp, P(SceneDesc)]),
+        "pico_hip_scene_info": (i32, [vp, P(u64), P(u64), P(u64), P(u32), P(u32)]),
+        "pico_hip_closest_hit": (i32, [vp, u32, vp, vp, vp, vp, vp, vp]),
+        "pico_hip_primary_ids": (i32, [vp, P(Camera), u32, u32, f32, f32, vp, vp]),
+        "pico_hip_render": (i32, [vp, P(Camera), P(RenderParams), vp, vp, vp, vp, P(Stats)]),
+        "pico_hip_render_d": (i32, [vp, P(Camera), P(RenderParams), vp, vp, vp, vp, P(Stats)]),
+        "pico_hip_render_samples": (i32, [vp, P(Camera), P(RenderParams), u32, vp]),
+        "pico_hip_render_accum64_d": (i32, [vp, P(Camera), P(RenderParams), u32, vp, P(Stats)]),
+        "pico_hip_gbuffer": (i32, [vp, P(Camera), u32, u32, vp, vp, vp]),
+        "pico_hip_gbuffer_d": (i32, [vp, P(Camera), u32, u32, vp, vp, vp]),
+        "pico_hip_atrous": (i32, [vp, vp, vp, vp, vp, u32, u32, u32, u32, u32, vp]),
+        "pico_hip_atrous_d": (i32, [vp, vp, vp, vp, vp, u32, u32, u32, u32, u32, vp, vp]),
+        "pico_hip_tonemap": (i32, [vp, vp, u32, u32]),
+        "pico_hip_relmse_d": (i32, [vp, vp, vp, f64, u32, P(f64)]),
+        "pico_hip_render_views_d": (i32, [vp, P(View), u32, u32, P(Stats)]),
+        "pico_hip_set_tuning": (i32, [vp, u32, u32]),
+        "pico_hip_get_tuning": (i32, [vp, P(u32), P(u32)]),
+        "pico_hip_cnn_create": (i32, [vp, u32, u32, u32, P(vp)]),
+        "pico_hip_cnn_destroy": (None, [vp]),
+        "pico_hip_cnn_num_params": (u32, []),
+        "pico_hip_cnn_init": (i32, [vp, u32]),
+        "pico_hip_cnn_set_params": (i32, [vp, vp]),
+        "pico_hip_cnn_get_params": (i32, [vp, vp]),
+        "pico_hip_cnn_denoise": (i32, [vp, vp, u32, u32, vp]),
+        "pico_hip_cnn_denoise_d": (i32, [vp, vp, u32, u32, vp]),
+        "pico_hip_cnn_loss_grad": (i32, [vp, vp, vp, u32, u32, u32, P(f64), vp]),
+        "pico_hip_cnn_train_step_d": (i32, [vp, vp, vp, u32, u32, u32, f32, P(f64)]),
+        "pico_scene_cornell": (i32, [u32, u32, P(vp)]),
+        "pico_scene_instanced": (i32, [u32, u32, u32, u32, u32, f32, u32, P(vp)]),
+        "pico_scene_training": (i32, [u32, u32, P(vp)]),
+        "pico_scene_desc_get": (i32, [vp, P(SceneDesc)]),
+        "pico_scene_default_camera": (i32, [vp, f32, P(Camera)]),
+        "pico_scene_destroy": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        if not hasattr(lib, name):
+            continue
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def exported_symbols():
+    """Every function name the C header declares (for the load check)."""
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "pico_hip.h")
+    import re
+    names = []
+    with open(hdr) as f:
+        for m in re.finditer(r"^\s*(?:int|void|uint32_t|const char\*)\s+\*?\s*(pico_\w+)\s*\(", f.read(), re.M):
+            names.append(m.group(1))
+    return names
+
+
+def lib():
+    """The loaded C-ABI library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PicoError(PICO_ERR_STATE, f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+class Scene:
+    """A procedural scene (flattened pico_scene_desc owned by the library)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        self.desc = SceneDesc()
+        rc = lib().pico_scene_desc_get(self._h, C.byref(self.desc))
+        if rc:
+            raise PicoError(rc, "pico_scene_desc_get")
+
+    @staticmethod
+    def cornell(seed=1, microfacet=MICROFACET_GGX):
+        h = C.c_void_p()
+        rc = lib().pico_scene_cornell(seed, microfacet, C.byref(h))
+        if rc:
+            raise PicoError(rc, "pico_scene_cornell")
+        return Scene(h.value)
+
+    @staticmethod
+    def instanced(seed=2, n_types=16, n_instances=1024, min_tris=2000, max_tris=20000, extent=100.0,
+                  microfacet=MICROFACET_GGX):
+        h = C.c_void_p()
+        rc = lib().pico_scene_instanced(seed, n_types, n_instances, min_tris, max_tris, extent, microfacet, C.byref(h))
+        if rc:
+            raise PicoError(rc, "pico_scene_instanced")
+        return Scene(h.value)
+
+    @staticmethod
+    def training(seed, microfacet=MICROFACET_GGX):
+        h = C.c_void_p()
+        rc = lib().pico_scene_training(seed, microfacet, C.byref(h))
+        if rc:
+            raise PicoError(rc, "pico_scene_training")
+        return Scene(h.value)
+
+    def camera(self, aspect=1.0):
+        cam = Camera()
+        rc = lib().pico_scene_default_camera(self._h, aspect, C.byref(cam))
+        if rc:
+            raise PicoError(rc, "pico_scene_default_camera")
+        return cam
+
+    @property
+    def n_triangles(self):
+        d = self.desc
+        offs = np.ctypeslib.as_array(C.cast(d.mesh_triangle_offset, C.POINTER(C.c_uint32)), (d.n_meshes + 1,))
+        mesh = np.ctypeslib.as_array(C.cast(d.instance_mesh, C.POINTER(C.c_uint32)), (d.n_instances,))
+        per = np.diff(offs.astype(np.int64))
+        return int(per.sum()), int(per[mesh].sum())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.pico_scene_destroy(self._h)
+            self._h = None
+
+
+class DeviceArray:
+    """Device memory owned by a Context (for the `_d` entry points)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx._check(lib().pico_hip_malloc(ctx._h, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.ctx._check(lib().pico_hip_memcpy_h2d(self.ctx._h, self.ptr, _ptr(a), a.nbytes))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx._check(lib().pico_hip_memcpy_d2h(self.ctx._h, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        self.ctx._check(lib().pico_hip_memset_d(self.ctx._h, self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().pico_hip_free(self.ctx._h, self.ptr)
+        self.ptr = None
+
+
+class Context:
+    """One device context (stream, scene, work buffers)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().pico_hip_create(device, C.byref(h))
+        if rc:
+            raise PicoError(rc, "pico_hip_create failed (no GPU visible?)")
+        self._h = h
+
+    def _check(self, rc):
+        if rc:
+            raise PicoError(rc, lib().pico_hip_last_error(self._h).decode())
+        return rc
+
+    def close(self):
+        if self._h:
+            lib().pico_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        self._check(lib().pico_hip_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    def sync(self):
+        self._check(lib().pico_hip_synchronize(self._h))
+
+    def alloc(self, nbytes):
+        return DeviceArray(self, nbytes)
+
+    def upload_scene(self, scene):
+        self._check(lib().pico_hip_upload_scene(self._h, C.byref(scene.desc)))
+
+    def scene_info(self):
+        bn, bt, bi = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        td, bd = C.c_uint32(), C.c_uint32()
+        self._check(lib().pico_hip_scene_info(self._h, C.byref(bn), C.byref(bt), C.byref(bi), C.byref(td), C.byref(bd)))
+        return dict(bytes_nodes=bn.value, bytes_tris=bt.value, bytes_instances=bi.value, tlas_depth=td.value,
+                    blas_depth=bd.value)
+
+    def closest_hit(self, origins, dirs):
+        o, d = _f32(origins, (-1, 3)), _f32(dirs, (-1, 3))
+        n = o.shape[0]
+        t = np.empty(n, np.float32)
+        inst = np.empty(n, np.uint32)
+        prim = np.empty(n, np.uint32)
+        uv = np.empty((n, 2), np.float32)
+        self._check(lib().pico_hip_closest_hit(self._h, n, _ptr(o), _ptr(d), _ptr(t), _ptr(inst), _ptr(prim), _ptr(uv)))
+        return t, inst, prim, uv
+
+    def primary_ids(self, cam, w, h, xi=(0.0, 0.0)):
+        inst = np.empty(w * h, np.uint32)
+        prim = np.empty(w * h, np.uint32)
+        self._check(lib().pico_hip_primary_ids(self._h, C.byref(cam), w, h, xi[0], xi[1], _ptr(inst), _ptr(prim)))
+        return inst, prim
+
+    def render(self, cam, p, pixels=None, counts=None, variance=None, cancel=None):
+        n = p.width * p.height
+        pixels = np.zeros((n, 3), np.float32) if pixels is None else pixels
+        counts = np.zeros(n, np.uint32) if counts is None else counts
+        variance = np.zeros((n, 3), np.float32) if variance is None else variance
+        st = Stats()
+        cptr = None if cancel is None else C.c_void_p(C.addressof(cancel))
+        rc = lib().pico_hip_render(self._h, C.byref(cam), C.byref(p), _ptr(pixels), _ptr(counts), _ptr(variance), cptr,
+                                   C.byref(st))
+        if rc and rc != PICO_ERR_CANCELLED:
+            self._check(rc)
+        return pixels, counts, variance, st
+
+    def render_d(self, cam, p, d_pixels, d_counts, d_variance):
+        st = Stats()
+        self._check(lib().pico_hip_render_d(self._h, C.byref(cam), C.byref(p), d_pixels.ptr, d_counts.ptr,
+                                            d_variance.ptr, None, C.byref(st)))
+        return st
+
+    def render_samples(self, cam, p, sample_base=0):
+        out = np.empty((p.spp, p.height * p.width, 3), np.float32)
+        self._check(lib().pico_hip_render_samples(self._h, C.byref(cam), C.byref(p), sample_base, _ptr(out)))
+        return out
+
+    def render_accum64_d(self, cam, p, sample_base, d_sum):
+        st = Stats()
+        self._check(lib().pico_hip_render_accum64_d(self._h, C.byref(cam), C.byref(p), sample_base, d_sum.ptr,
+                                                    C.byref(st)))
+        return st
+
+    def gbuffer(self, cam, w, h):
+        n = np.empty((w * h, 3), np.float32)
+        p = np.empty((w * h, 3), np.float32)
+        a = np.empty((w * h, 3), np.float32)
+        self._check(lib().pico_hip_gbuffer(self._h, C.byref(cam), w, h, _ptr(n), _ptr(p), _ptr(a)))
+        return n, p, a
+
+    def gbuffer_d(self, cam, w, h, dn, dp, da):
+        self._check(lib().pico_hip_gbuffer_d(self._h, C.byref(cam), w, h, dn.ptr, dp.ptr, da.ptr))
+
+    def atrous(self, rgb, normal, position, albedo, w, h, tile=(16, 16), levels=5):
+        args = [_f32(x, (-1, 3)) for x in (rgb, normal, position, albedo)]
+        out = np.empty((w * h, 3), np.float32)
+        self._check(lib().pico_hip_atrous(self._h, *[_ptr(a) for a in args], w, h, tile[0], tile[1], levels, _ptr(out)))
+        return out
+
+    def atrous_d(self, d_rgb, d_n, d_p, d_a, w, h, d_scratch, d_out, tile=(16, 16), levels=5):
+        self._check(lib().pico_hip_atrous_d(self._h, d_rgb.ptr, d_n.ptr, d_p.ptr, d_a.ptr, w, h, tile[0], tile[1], levels,
+                                            d_scratch.ptr, d_out.ptr))
+
+    def tonemap(self, rgb, w, h):
+        out = _f32(rgb, (-1, 3)).copy()
+        self._check(lib().pico_hip_tonemap(self._h, _ptr(out), w, h))
+        return out
+
+    def relmse_d(self, d_img, d_ref_sum, ref_count, n_pixels):
+        out = C.c_double()
+        self._check(lib().pico_hip_relmse_d(self._h, d_img.ptr, d_ref_sum.ptr, float(ref_count), n_pixels, C.byref(out)))
+        return out.value
+
+    def render_views_d(self, views, queue_paths=0):
+        arr = (View * len(views))(*views)
+        st = (Stats * len(views))()
+        self._check(lib().pico_hip_render_views_d(self._h, arr, len(views), queue_paths, st))
+        return list(st)
+
+    def set_tuning(self, queue_paths=0, trace_block=0):
+        self._check(lib().pico_hip_set_tuning(self._h, queue_paths, trace_block))
+
+    def get_tuning(self):
+        q, b = C.c_uint32(), C.c_uint32()
+        self._check(lib().pico_hip_get_tuning(self._h, C.byref(q), C.byref(b)))
+        return q.value, b.value
+
+    def enable_kernel_timing(self, on=True):
+        self._check(lib().pico_hip_enable_kernel_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        n, ms = C.c_uint64(), C.c_double()
+        self._check(lib().pico_hip_kernel_time(self._h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value

@@ -1,0 +1,385 @@
+// Host-side scene builder: binned-SAH BVH2 per mesh (BLAS) and over instances
+// (TLAS), emitted as 64-byte child-pair nodes, plus the light-sampling tables.
+//
+// Reference counterparts: BVHFactory/SAH_parition_Scheme (Source/Core/BVH.cpp:65-197)
+// build the same kind of tree with 12 buckets on one axis; here 16 bins are
+// evaluated on all three axes and leaves hold up to 4 triangles.  Closest-hit
+// results do not depend on the topology except for exact ties.  Instance
+// bounds follow AABB * mat4 (Source/Core/AABB.cpp:253-270); light tables
+// follow LowerLevelMeshBVH::generate_sampling_data (Source/Core/LowerLevelMeshBVH.cpp:92-117)
+// and AliasTable::build_table (Source/Util/AliasTable.cpp:12-87).
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <set>
+
+namespace pico {
+
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float* l, const float* h) {
+        for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], l[k]); hi[k] = std::max(hi[k], h[k]); }
+    }
+    void grow_pt(const float* p) { grow(p, p); }
+    float area() const {
+        float d[3];
+        for (int k = 0; k < 3; ++k) d[k] = std::max(0.0f, hi[k] - lo[k]);
+        return 2.0f * (d[0] * d[1] + d[0] * d[2] + d[1] * d[2]);
+    }
+    bool empty() const { return !(lo[0] <= hi[0]); }
+};
+
+struct Builder {
+    const std::vector<float>& lo;
+    const std::vector<float>& hi;
+    std::vector<uint32_t>& order;
+    std::vector<float> cen;
+    uint32_t bins, max_leaf, prim_base;
+    std::vector<DNode>& nodes;
+    uint32_t max_depth = 0;
+
+    Box range_box(uint32_t b, uint32_t e) const {
+        Box bx;
+        for (uint32_t i = b; i < e; ++i) bx.grow(&lo[3 * order[i]], &hi[3 * order[i]]);
+        return bx;
+    }
+
+    // Splits [b, e) and returns the split position, or e for "make a leaf".
+    uint32_t split(uint32_t b, uint32_t e, const Box& bounds) {
+        const uint32_t n = e - b;
+        if (n == 1) return e;
+        Box cb;
+        for (uint32_t i = b; i < e; ++i) cb.grow_pt(&cen[3 * order[i]]);
+        float best_cost = INFINITY;
+        int best_axis = -1;
+        uint32_t best_bin = 0;
+        std::vector<Box> bb(bins);
+        std::vector<uint32_t> cnt(bins);
+        std::vector<float> rarea(bins);
+        std::vector<uint32_t> rcnt(bins);
+        for (int axis = 0; axis < 3; ++axis) {
+            const float ext = cb.hi[axis] - cb.lo[axis];
+            if (!(ext > 0)) continue;
+            const float sc = (float)bins / ext;
+            std::fill(bb.begin(), bb.end(), Box());
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (uint32_t i = b; i < e; ++i) {
+                const uint32_t p = order[i];
+                uint32_t k = (uint32_t)std::min<float>((float)(bins - 1), (cen[3 * p + axis] - cb.lo[axis]) * sc);
+                cnt[k]++;
+                bb[k].grow(&lo[3 * p], &hi[3 * p]);
+            }
+            Box acc;
+            uint32_t c = 0;
+            for (int k = (int)bins - 1; k > 0; --k) {
+                acc.grow(bb[k].lo, bb[k].hi);
+                c += cnt[k];
+                rarea[k] = acc.empty() ? 0.0f : acc.area();
+                rcnt[k] = c;
+            }
+            acc = Box();
+            c = 0;
+            for (uint32_t k = 0; k + 1 < bins; ++k) {
+                acc.grow(bb[k].lo, bb[k].hi);
+                c += cnt[k];
+                if (c == 0 || rcnt[k + 1] == 0) continue;
+                const float cost = (float)c * acc.area() + (float)rcnt[k + 1] * rarea[k + 1];
+                if (cost < best_cost) { best_cost = cost; best_axis = axis; best_bin = k; }
+            }
+        }
+        const float parent = bounds.area();
+        const float leaf_cost = (float)n * parent;
+        const float split_cost = parent + best_cost;  // traversal cost 1, intersection cost 1
+        if (best_axis < 0) {
+            // all centroids coincide: leaf if it fits, else split the index range
+            return n <= max_leaf ? e : b + n / 2;
+        }
+        if (n <= max_leaf && leaf_cost <= split_cost) return e;
+        const int axis = best_axis;
+        const float sc = (float)bins / (cb.hi[axis] - cb.lo[axis]);
+        auto mid = std::partition(order.begin() + b, order.begin() + e, [&](uint32_t p) {
+            uint32_t k = (uint32_t)std::min<float>((float)(bins - 1), (cen[3 * p + axis] - cb.lo[axis]) * sc);
+            return k <= best_bin;
+        });
+        uint32_t m = (uint32_t)(mid - order.begin());
+        if (m == b || m == e) m = b + n / 2;
+        return m;
+    }
+
+    // Returns a child reference for the range [b, e); emits nodes in DFS order.
+    uint32_t build(uint32_t b, uint32_t e, uint32_t depth) {
+        const Box bounds = range_box(b, e);
+        const uint32_t s = split(b, e, bounds);
+        if (s == e) return make_leaf(prim_base + b, e - b);
+        return emit_internal(b, s, e, depth);
+    }
+
+    uint32_t emit_internal(uint32_t b, uint32_t s, uint32_t e, uint32_t depth) {
+        max_depth = std::max(max_depth, depth + 1);
+        const uint32_t id = (uint32_t)nodes.size();
+        nodes.emplace_back();
+        const Box l = range_box(b, s), r = range_box(s, e);
+        const uint32_t rl = build(b, s, depth + 1);
+        const uint32_t rr = build(s, e, depth + 1);
+        DNode& nd = nodes[id];
+        set_child(nd, 0, l, rl);
+        set_child(nd, 1, r, rr);
+        return id;
+    }
+
+    static void set_child(DNode& nd, int c, const Box& bx, uint32_t ref) {
+        float* x = c == 0 ? nd.c0x : nd.c1x;
+        float* y = c == 0 ? nd.c0y : nd.c1y;
+        float* z = c == 0 ? nd.c0z : nd.c1z;
+        x[0] = bx.lo[0]; x[1] = bx.hi[0];
+        y[0] = bx.lo[1]; y[1] = bx.hi[1];
+        z[0] = bx.lo[2]; z[1] = bx.hi[2];
+        nd.ref[c] = ref;
+    }
+};
+
+// Inverse of an affine column-major 4x4 (bottom row 0 0 0 1), in double.
+bool affine_inverse_rows(const float* m, float inv[3][4]) {
+    double a[3][3], t[3];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) a[r][c] = m[c * 4 + r];
+        t[r] = m[12 + r];
+    }
+    const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                       a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    if (!(std::fabs(det) > 1e-30)) return false;
+    double ia[3][3];
+    ia[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    ia[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    ia[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    ia[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    ia[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    ia[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    ia[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    ia[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    ia[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) inv[r][c] = (float)ia[r][c];
+        inv[r][3] = (float)(-(ia[r][0] * t[0] + ia[r][1] * t[1] + ia[r][2] * t[2]));
+    }
+    return true;
+}
+
+inline float fdot(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Triangle "area" of the reference's light sampler (the sine takes the angle
+// scaled by 180/pi), Source/Core/LowerLevelMeshBVH.cpp:98-112, in its float/double mix.
+float light_tri_weight(const float* a, const float* b, const float* c) {
+    float ac[3], bc[3];
+    for (int k = 0; k < 3; ++k) { ac[k] = a[k] - c[k]; bc[k] = b[k] - c[k]; }
+    const float bc_len = std::sqrt(fdot(bc, bc));
+    const float base = std::sqrt(fdot(ac, ac));
+    float nac[3], nbc[3];
+    const float ia = 1.0f / std::sqrt(fdot(ac, ac)), ib = 1.0f / std::sqrt(fdot(bc, bc));
+    for (int k = 0; k < 3; ++k) { nac[k] = ac[k] * ia; nbc[k] = bc[k] * ib; }
+    const float theta = (float)std::sin((double)std::acos(fdot(nac, nbc)) * (180.0f / M_PI));
+    const float height = theta * bc_len;
+    return (base * height) / 2.0f;
+}
+
+// Vose alias table as AliasTable::build_table builds it (leftovers alias 0, weight 1).
+void alias_table(const std::vector<float>& w, std::vector<float>& bw, std::vector<uint32_t>& ba) {
+    const uint32_t n = (uint32_t)w.size();
+    bw.assign(n, 0.0f);
+    ba.assign(n, 0u);
+    const float total = (float)std::accumulate(w.begin(), w.end(), 0.0);
+    for (uint32_t i = 0; i < n; ++i) bw[i] = w[i] / total;
+    struct O { float p; uint32_t i; };
+    std::vector<O> over, under;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float ph = bw[i] * (float)n;
+        (ph < 1.0f ? under : over).push_back({ph, i});
+    }
+    while (!over.empty() && !under.empty()) {
+        const O ov = over.back(); over.pop_back();
+        const O un = under.back(); under.pop_back();
+        bw[un.i] = un.p;
+        ba[un.i] = ov.i;
+        const float excess = un.p + ov.p - 1.0f;
+        (excess < 1.0f ? under : over).push_back({excess, ov.i});
+    }
+    for (auto& u : under) { ba[u.i] = 0; bw[u.i] = 1.0f; }
+    for (auto& o : over) { ba[o.i] = 0; bw[o.i] = 1.0f; }
+}
+
+}  // namespace
+
+uint32_t build_bvh(const std::vector<float>& box_lo, const std::vector<float>& box_hi, uint32_t bins, uint32_t max_leaf,
+                   uint32_t prim_base, std::vector<DNode>& nodes, std::vector<uint32_t>& order, uint32_t& depth) {
+    const uint32_t n = (uint32_t)(box_lo.size() / 3);
+    order.resize(n);
+    std::iota(order.begin(), order.end(), 0u);
+    Builder b{box_lo, box_hi, order, {}, bins, max_leaf, prim_base, nodes};
+    b.cen.resize(3 * n);
+    for (uint32_t i = 0; i < 3 * n; ++i) b.cen[i] = 0.5f * (box_lo[i] + box_hi[i]);
+    const Box all = b.range_box(0, n);
+    uint32_t root;
+    const uint32_t s = n ? b.split(0, n, all) : 0;
+    if (n == 0 || s == n) {
+        // a single leaf: wrap it in a node whose second child is empty
+        root = (uint32_t)nodes.size();
+        nodes.emplace_back();
+        DNode& nd = nodes[root];
+        Builder::set_child(nd, 0, all, n ? make_leaf(prim_base, n) : kEmptyRef);
+        Builder::set_child(nd, 1, Box(), kEmptyRef);
+        b.max_depth = 1;
+    } else {
+        root = b.emit_internal(0, s, n, 0);
+    }
+    depth = b.max_depth;
+    return root;
+}
+
+std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out) {
+    if (d.n_meshes == 0 || !d.mesh_vertex_offset || !d.mesh_triangle_offset || !d.positions || !d.normals || !d.indices)
+        return "scene has no meshes";
+    if (d.n_instances == 0 || !d.transforms || !d.instance_mesh || !d.instance_material || !d.instance_bsdf)
+        return "scene has no instances";
+    if (d.n_materials == 0 || !d.materials) return "scene has no materials";
+    if (d.microfacet > PICO_MICROFACET_GGX) return "unknown microfacet distribution";
+    if (d.n_lights && !d.light_instances) return "light list missing";
+    if (d.mesh_vertex_offset[0] != 0 || d.mesh_triangle_offset[0] != 0) return "mesh offsets must start at 0";
+    for (uint32_t m = 0; m < d.n_meshes; ++m)
+        if (d.mesh_vertex_offset[m + 1] < d.mesh_vertex_offset[m] || d.mesh_triangle_offset[m + 1] < d.mesh_triangle_offset[m])
+            return "mesh offsets must be non-decreasing";
+    const uint32_t nv = d.mesh_vertex_offset[d.n_meshes], nt = d.mesh_triangle_offset[d.n_meshes];
+    if (nt > kPrimMask || d.n_instances > kPrimMask) return "too many triangles or instances";
+    out = HostScene();
+    out.positions.assign(d.positions, d.positions + 3 * (size_t)nv);
+    out.normals.assign(d.normals, d.normals + 3 * (size_t)nv);
+    out.indices.resize(3 * (size_t)nt);
+    out.mesh_tri_offset.assign(d.mesh_triangle_offset, d.mesh_triangle_offset + d.n_meshes + 1);
+    std::vector<uint32_t> mesh_root(d.n_meshes);
+    std::vector<Box> mesh_box(d.n_meshes);
+    for (uint32_t m = 0; m < d.n_meshes; ++m) {
+        const uint32_t v0 = d.mesh_vertex_offset[m], v1 = d.mesh_vertex_offset[m + 1];
+        const uint32_t t0 = d.mesh_triangle_offset[m], t1 = d.mesh_triangle_offset[m + 1];
+        std::vector<float> lo, hi;
+        for (uint32_t t = t0; t < t1; ++t) {
+            Box b;
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t li = d.indices[3 * (size_t)t + k];
+                if (li >= v1 - v0) return "triangle index out of range";
+                out.indices[3 * (size_t)t + k] = v0 + li;
+                b.grow_pt(&d.positions[3 * (size_t)(v0 + li)]);
+            }
+            lo.insert(lo.end(), b.lo, b.lo + 3);
+            hi.insert(hi.end(), b.hi, b.hi + 3);
+        }
+        for (uint32_t v = v0; v < v1; ++v) mesh_box[m].grow_pt(&d.positions[3 * (size_t)v]);
+        std::vector<uint32_t> order;
+        uint32_t depth = 0;
+        const uint32_t base = (uint32_t)out.tris.size();
+        mesh_root[m] = build_bvh(lo, hi, bins, 4, base, out.nodes, order, depth);
+        out.blas_depth = std::max(out.blas_depth, depth);
+        for (uint32_t k = 0; k < order.size(); ++k) {
+            const uint32_t t = t0 + order[k];
+            const float* p0 = &out.positions[3 * (size_t)out.indices[3 * (size_t)t]];
+            const float* p1 = &out.positions[3 * (size_t)out.indices[3 * (size_t)t + 1]];
+            const float* p2 = &out.positions[3 * (size_t)out.indices[3 * (size_t)t + 2]];
+            DTri tr{};
+            for (int c = 0; c < 3; ++c) { tr.v0[c] = p0[c]; tr.e1[c] = p1[c] - p0[c]; tr.e2[c] = p2[c] - p0[c]; }
+            tr.prim = order[k];
+            out.tris.push_back(tr);
+        }
+    }
+    // materials
+    for (uint32_t i = 0; i < d.n_materials; ++i) {
+        const pico_material& pm = d.materials[i];
+        DMaterial m{};
+        for (int c = 0; c < 3; ++c) { m.diffuse[c] = pm.diffuse[c]; m.specular[c] = pm.specular[c]; m.emissive[c] = pm.emissive[c]; }
+        const float s = std::sqrt(fdot(pm.specular, pm.specular));
+        m.reflectance = s / (s + std::sqrt(fdot(pm.diffuse, pm.diffuse)));
+        m.roughness = pm.roughness;
+        out.materials.push_back(m);
+    }
+    // instances + TLAS
+    std::vector<float> lo, hi;
+    out.n_instances = d.n_instances;
+    for (uint32_t i = 0; i < d.n_instances; ++i) {
+        const uint32_t m = d.instance_mesh[i];
+        if (m >= d.n_meshes) return "instance mesh out of range";
+        if (d.instance_material[i] >= d.n_materials) return "instance material out of range";
+        if (d.instance_bsdf[i] > PICO_BSDF_DELTA) return "unknown BSDF kind";
+        const float* M = &d.transforms[16 * (size_t)i];
+        if (M[3] != 0.0f || M[7] != 0.0f || M[11] != 0.0f || M[15] != 1.0f) return "instance transform must be affine";
+        DInstance di{};
+        if (!affine_inverse_rows(M, di.inv)) return "instance transform is singular";
+        di.blas_root = mesh_root[m];
+        di.tri_offset = d.mesh_triangle_offset[m];
+        di.material = d.instance_material[i];
+        di.bsdf = d.instance_bsdf[i];
+        out.instances.push_back(di);
+        DXform xf;
+        std::memcpy(xf.m, M, sizeof(xf.m));
+        out.xforms.push_back(xf);
+        // 8-corner world box (AABB * mat4)
+        Box wb;
+        const Box& mb = mesh_box[m];
+        for (int c = 0; c < 8; ++c) {
+            const float p[3] = {(c & 1) ? mb.hi[0] : mb.lo[0], (c & 2) ? mb.hi[1] : mb.lo[1], (c & 4) ? mb.hi[2] : mb.lo[2]};
+            float w[3];
+            for (int r = 0; r < 3; ++r) w[r] = M[r] * p[0] + M[4 + r] * p[1] + M[8 + r] * p[2] + M[12 + r];
+            wb.grow_pt(w);
+        }
+        // widen by a relative epsilon so float rounding of the transform never culls a hit
+        for (int k = 0; k < 3; ++k) {
+            const float e = 1e-5f * (std::fabs(wb.lo[k]) + std::fabs(wb.hi[k]) + 1.0f);
+            wb.lo[k] -= e; wb.hi[k] += e;
+        }
+        lo.insert(lo.end(), wb.lo, wb.lo + 3);
+        hi.insert(hi.end(), wb.hi, wb.hi + 3);
+    }
+    std::vector<uint32_t> order;
+    uint32_t depth = 0;
+    out.tlas_root = build_bvh(lo, hi, bins, 1, 0, out.nodes, order, depth);
+    out.tlas_depth = depth;
+    // TLAS leaves reference positions in `order`; remap them to instance ids.
+    {
+        const uint32_t first_tlas = out.tlas_root;
+        for (size_t k = first_tlas; k < out.nodes.size(); ++k)
+            for (int c = 0; c < 2; ++c) {
+                uint32_t& r = out.nodes[k].ref[c];
+                if (r != kEmptyRef && (r & kLeafBit)) r = make_leaf(order[r & kPrimMask], 1);
+            }
+    }
+    // lights
+    std::set<uint32_t> light_meshes;
+    for (uint32_t l = 0; l < d.n_lights; ++l) {
+        const uint32_t ii = d.light_instances[l];
+        if (ii >= d.n_instances) return "light instance out of range";
+        if (d.instance_bsdf[ii] != PICO_BSDF_LIGHT) return "light instance must use the Light BSDF";
+        const uint32_t m = d.instance_mesh[ii];
+        if (!light_meshes.insert(m).second) return "lights must not share a mesh";
+        const uint32_t t0 = d.mesh_triangle_offset[m], t1 = d.mesh_triangle_offset[m + 1];
+        if (t1 == t0) return "light mesh has no triangles";
+        std::vector<float> w;
+        for (uint32_t t = t0; t < t1; ++t)
+            w.push_back(light_tri_weight(&out.positions[3 * (size_t)out.indices[3 * (size_t)t]],
+                                         &out.positions[3 * (size_t)out.indices[3 * (size_t)t + 1]],
+                                         &out.positions[3 * (size_t)out.indices[3 * (size_t)t + 2]]));
+        std::vector<float> bw;
+        std::vector<uint32_t> ba;
+        alias_table(w, bw, ba);
+        DLight dl{ii, t1 - t0, (uint32_t)out.alias_weight.size(), t0};
+        out.alias_weight.insert(out.alias_weight.end(), bw.begin(), bw.end());
+        out.alias_index.insert(out.alias_index.end(), ba.begin(), ba.end());
+        out.lights.push_back(dl);
+    }
+    std::memcpy(out.sky, d.sky_rgb, sizeof(out.sky));
+    out.microfacet = d.microfacet;
+    return "";
+}
+
+}  // namespace pico

@@ -1,0 +1,43 @@
+// Host-side builder of the device scene (BLAS per mesh, TLAS over instances).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pico_hip.h"
+#include "device_types.h"
+
+namespace pico {
+
+struct HostScene {
+    std::vector<DNode> nodes;          // BLAS nodes of every mesh, then the TLAS
+    std::vector<DTri> tris;            // triangle records in BLAS leaf order
+    std::vector<DInstance> instances;  // original instance order
+    std::vector<DXform> xforms;
+    std::vector<DMaterial> materials;
+    std::vector<DLight> lights;
+    std::vector<float> alias_weight;
+    std::vector<uint32_t> alias_index;
+    std::vector<float> positions;      // 3 x f32 per vertex (global)
+    std::vector<float> normals;        // 3 x f32 per vertex (global)
+    std::vector<uint32_t> indices;     // 3 x u32 per triangle, global vertex ids
+    std::vector<uint32_t> mesh_tri_offset;
+    uint32_t tlas_root = 0;
+    uint32_t tlas_depth = 0;
+    uint32_t blas_depth = 0;           // deepest BLAS
+    uint32_t n_instances = 0;
+    float sky[3] = {0, 0, 0};
+    uint32_t microfacet = 0;
+};
+
+// Binned SAH (bins per axis, all three axes) BVH2 over boxes; returns the
+// child-pair root node index.  Leaves hold at most max_leaf primitives.
+// `order` receives the primitive order of the leaves.
+uint32_t build_bvh(const std::vector<float>& box_lo, const std::vector<float>& box_hi, uint32_t bins,
+                   uint32_t max_leaf, uint32_t prim_base, std::vector<DNode>& nodes, std::vector<uint32_t>& order,
+                   uint32_t& depth);
+
+// Validates the description and builds everything; returns an error text or "".
+std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out);
+
+}  // namespace pico

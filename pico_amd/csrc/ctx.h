@@ -1,0 +1,93 @@
+// Library context: device, stream, uploaded scene, work buffers, error text
+// and per-kernel event timing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "bvh_build.h"
+#include "trace.h"
+
+struct KernelTimes {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    uint64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct pico_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 256;
+    // scene
+    bool has_scene = false;
+    pico::HostScene host;
+    pico::DevScene dev{};
+    std::vector<void*> scene_allocs;
+    uint32_t stack_need = 0;
+    // named scratch buffers (grown on demand, freed at destroy)
+    std::map<std::string, DevBuf> bufs;
+    // tuning
+    uint32_t queue_paths = 16u << 20;
+    uint32_t trace_block = 128;
+    // timing
+    bool timing = false;
+    std::map<std::string, KernelTimes> timers;
+};
+
+namespace pico {
+
+inline int fail(pico_hip_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define PICO_HIP(ctx, expr)                                                                        \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return ::pico::fail((ctx), PICO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Returns a device buffer of at least `bytes` under `name` (contents undefined).
+void* scratch(pico_hip_ctx* c, const std::string& name, size_t bytes);
+
+// Bracket a launch with events when timing is enabled; checks the launch.
+struct LaunchScope {
+    pico_hip_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    LaunchScope(pico_hip_ctx* ctx, const char* n) : c(ctx), name(n) {
+        if (c->timing) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    int done() {
+        hipError_t e = hipGetLastError();
+        if (c->timing) {
+            hipEventRecord(b, c->stream);
+            c->timers[name].pending.push_back({a, b});
+        }
+        if (e != hipSuccess) return fail(c, PICO_ERR_HIP, std::string("launch of ") + name + ": " + hipGetErrorString(e));
+        return PICO_OK;
+    }
+};
+
+#define PICO_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                               \
+    do {                                                                                        \
+        ::pico::LaunchScope ls_(ctx, name);                                                     \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, (ctx)->stream, __VA_ARGS__); \
+        int rc_ = ls_.done();                                                                   \
+        if (rc_) return rc_;                                                                    \
+    } while (0)
+
+}  // namespace pico

@@ -1,0 +1,230 @@
+// Post-render filters: the a-trous denoiser, Reinhard tone mapping and the
+// FP64 relMSE reduction.
+//
+// A-trous (Source/Util/Denoisers.cpp:18-126): one workgroup per tile and level.
+// The workgroup first reduces the tile-local constants over exactly the set of
+// pixels the tile's 25-tap kernels read (white point = max luminance of the
+// level input, range = max - min of |position|), then filters its pixels.
+// The tapped set is reduced tap by tap, not as a dense dilated rectangle: for
+// tiles narrower than the dilation the two differ.  Per-pixel |position| is
+// computed once per call.  HBM-bound: 4 x 12 B read + 12 B written per pixel
+// and level, the tile reduce re-reads through L2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/pico_hip.h"
+#include "ctx.h"
+
+using namespace pico;
+
+namespace {
+
+constexpr int kTileThreads = 256;
+
+__constant__ float kAtrousW[25] = {0.00292f, 0.01306f, 0.02154f, 0.01306f, 0.00292f, 0.01306f, 0.05855f,
+                                   0.09653f, 0.05855f, 0.01306f, 0.02154f, 0.09653f, 0.15915f, 0.09653f,
+                                   0.02154f, 0.01306f, 0.05855f, 0.09563f, 0.05855f, 0.01306f, 0.00292f,
+                                   0.01306f, 0.02154f, 0.01306f, 0.00292f};
+
+__global__ void k_plen(const float* __restrict__ pos, float* __restrict__ plen, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        plen[i] = length(ld3(pos + 3 * (size_t)i));
+}
+
+__device__ __forceinline__ uint32_t tap_index(int x, int y, int u, int scale, int w, int h) {
+    const int ox = u % 5 - 2, oy = u / 5 - 2;
+    const int ix = min(max(x + ox * scale, 0), w - 1);
+    const int iy = min(max(y + oy * scale, 0), h - 1);
+    return (uint32_t)iy * (uint32_t)w + (uint32_t)ix;
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_atrous_level(const float* __restrict__ src, const float* __restrict__ nrm,
+                                                               const float* __restrict__ pos, const float* __restrict__ alb,
+                                                               const float* __restrict__ plen, float* __restrict__ dst,
+                                                               int w, int h, int tw, int th, int tiles_y, int scale) {
+    __shared__ float red[3][kTileThreads / 64];
+    const int tx0 = (blockIdx.x / tiles_y) * tw, ty0 = (blockIdx.x % tiles_y) * th;
+    const int cw = min(tw, w - tx0), ch = min(th, h - ty0);
+    const int npx = cw * ch;
+    float white = 0.0f, dmin = INFINITY, dmax = -INFINITY;
+    for (int q = threadIdx.x; q < npx * 25; q += kTileThreads) {
+        const int pidx = q / 25, u = q - pidx * 25;
+        const int x = tx0 + pidx % cw, y = ty0 + pidx / cw;
+        const uint32_t fi = tap_index(x, y, u, scale, w, h);
+        const f3 c = ld3(src + 3 * (size_t)fi);
+        const float lum = c.x * 0.2126f + c.y * 0.587f + c.z * 0.114f;
+        white = fmaxf(white, lum);
+        const float l = plen[fi];
+        dmin = fminf(dmin, l);
+        dmax = fmaxf(dmax, l);
+    }
+    // wave reduce, then across the workgroup's waves
+    for (int o = 32; o > 0; o >>= 1) {
+        white = fmaxf(white, __shfl_xor(white, o));
+        dmin = fminf(dmin, __shfl_xor(dmin, o));
+        dmax = fmaxf(dmax, __shfl_xor(dmax, o));
+    }
+    const int wave = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) { red[0][wave] = white; red[1][wave] = dmin; red[2][wave] = dmax; }
+    __syncthreads();
+    white = red[0][0]; dmin = red[1][0]; dmax = red[2][0];
+    for (int k = 1; k < kTileThreads / 64; ++k) {
+        white = fmaxf(white, red[0][k]);
+        dmin = fminf(dmin, red[1][k]);
+        dmax = fmaxf(dmax, red[2][k]);
+    }
+    const float range = dmax - dmin;
+    const float r2 = range * range, w2 = white * white;
+    for (int pidx = threadIdx.x; pidx < npx; pidx += kTileThreads) {
+        const int x = tx0 + pidx % cw, y = ty0 + pidx / cw;
+        const size_t ci = (size_t)y * w + x;
+        const f3 cn = ld3(nrm + 3 * ci), cp = ld3(pos + 3 * ci), ca = ld3(alb + 3 * ci);
+        f3 res = mk(0.0f, 0.0f, 0.0f);
+        float wsum = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 25; ++u) {
+            const size_t fi = tap_index(x, y, u, scale, w, h);
+            f3 t = cn - ld3(nrm + 3 * fi);
+            float dist = dot(t, t);
+            const float nw = smin(expf(-dist / 0.25f), 1.0f);
+            t = cp - ld3(pos + 3 * fi);
+            dist = dot(t, t);
+            const float pw = smin(expf(-dist / r2), 1.0f);
+            t = ca - ld3(alb + 3 * fi);
+            dist = dot(t, t);
+            const float aw = smin(expf(-dist / w2), 1.0f);
+            const float tw_ = kAtrousW[u] * nw * pw * aw;
+            res = res + ld3(src + 3 * fi) * tw_;
+            wsum += tw_;
+        }
+        const f3 o = res / wsum;
+        dst[3 * ci] = o.x; dst[3 * ci + 1] = o.y; dst[3 * ci + 2] = o.z;
+    }
+}
+
+__global__ void k_lum_max(const float* __restrict__ rgb, uint32_t n, uint32_t* white_bits) {
+    float m = 0.0f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float l = rgb[3 * i] * 0.2126f + rgb[3 * i + 1] * 0.587f + rgb[3 * i + 2] * 0.114f;
+        m = smax(m, l);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(white_bits, __float_as_uint(m));
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Util::reinhard_tone_mapping, Source/Util/ToneMappers.cpp:12-44.
+__global__ void k_reinhard(float* rgb, uint32_t n3, const uint32_t* white_bits) {
+    const float white = __uint_as_float(*white_bits);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += gridDim.x * blockDim.x) {
+        const float p = rgb[i];
+        const float num = p * (1.0f + (p / (white * white)));
+        rgb[i] = num / (1.0f + p);
+    }
+}
+
+__global__ void k_relmse(const float* __restrict__ img, const double* __restrict__ ref, double inv_count, uint32_t n3,
+                         double* out) {
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += gridDim.x * blockDim.x) {
+        const double r = ref[i] * inv_count;
+        const double d = (double)img[i] - r;
+        acc += d * d / (r * r + 1e-2);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+__global__ void k_zero_f64(double* p) { *p = 0.0; }
+
+}  // namespace
+
+extern "C" {
+
+int pico_hip_atrous_d(pico_hip_ctx* c, const float* rgb, const float* nrm, const float* pos, const float* alb, uint32_t w,
+                      uint32_t h, uint32_t tw, uint32_t th, uint32_t levels, float* scratch_buf, float* out) {
+    if (!c) return PICO_ERR_ARG;
+    if (!rgb || !nrm || !pos || !alb || !scratch_buf || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if (w == 0 || h == 0 || tw == 0 || th == 0) return fail(c, PICO_ERR_ARG, "sizes must be positive");
+    if ((uint64_t)w * h > (1ull << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
+    if (levels > 16) return fail(c, PICO_ERR_LIMIT, "at most 16 levels");
+    const size_t n = (size_t)w * h;
+    if (levels == 0) {
+        PICO_HIP(c, hipMemcpyAsync(out, rgb, 12 * n, hipMemcpyDeviceToDevice, c->stream));
+        return PICO_OK;
+    }
+    float* plen = (float*)scratch(c, "atrous_plen", 4 * n + 64);
+    if (!plen) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    PICO_LAUNCH(c, "atrous_plen", k_plen, std::min<size_t>((n + 255) / 256, 8192), 256, 0, pos, plen, (uint32_t)n);
+    const int tiles_x = (int)((w + tw - 1) / tw), tiles_y = (int)((h + th - 1) / th);
+    const float* src = rgb;
+    for (uint32_t l = 0; l < levels; ++l) {
+        float* dst = ((levels - 1 - l) % 2 == 0) ? out : scratch_buf;
+        PICO_LAUNCH(c, "atrous", k_atrous_level, tiles_x * tiles_y, kTileThreads, 0, src, nrm, pos, alb, plen, dst, (int)w,
+                    (int)h, (int)tw, (int)th, tiles_y, 1 << l);
+        src = dst;
+    }
+    return PICO_OK;
+}
+
+int pico_hip_atrous(pico_hip_ctx* c, const float* rgb, const float* nrm, const float* pos, const float* alb, uint32_t w,
+                    uint32_t h, uint32_t tw, uint32_t th, uint32_t levels, float* out) {
+    if (!c) return PICO_ERR_ARG;
+    if (!rgb || !nrm || !pos || !alb || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    const size_t n = (size_t)w * h;
+    float* b = (float*)scratch(c, "atrous_io", 72 * n + 64);
+    if (!b) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    float* d_rgb = b;
+    float* d_n = b + 3 * n;
+    float* d_p = b + 6 * n;
+    float* d_a = b + 9 * n;
+    float* d_s = b + 12 * n;
+    float* d_o = b + 15 * n;
+    PICO_HIP(c, hipMemcpyAsync(d_rgb, rgb, 12 * n, hipMemcpyHostToDevice, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(d_n, nrm, 12 * n, hipMemcpyHostToDevice, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(d_p, pos, 12 * n, hipMemcpyHostToDevice, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(d_a, alb, 12 * n, hipMemcpyHostToDevice, c->stream));
+    int rc = pico_hip_atrous_d(c, d_rgb, d_n, d_p, d_a, w, h, tw, th, levels, d_s, d_o);
+    if (rc) return rc;
+    PICO_HIP(c, hipMemcpyAsync(out, d_o, 12 * n, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_tonemap(pico_hip_ctx* c, float* rgb, uint32_t w, uint32_t h) {
+    if (!c) return PICO_ERR_ARG;
+    if (!rgb) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    const size_t n = (size_t)w * h;
+    float* d = (float*)scratch(c, "tonemap", 12 * n + 64);
+    if (!d) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    uint32_t* wb = (uint32_t*)(d + 3 * n);
+    PICO_HIP(c, hipMemcpyAsync(d, rgb, 12 * n, hipMemcpyHostToDevice, c->stream));
+    PICO_LAUNCH(c, "fill", k_fill_u32, 1, 64, 0, wb, 0u, 1u);
+    PICO_LAUNCH(c, "lum_max", k_lum_max, std::min<size_t>((n + 255) / 256, 4096), 256, 0, d, (uint32_t)n, wb);
+    PICO_LAUNCH(c, "reinhard", k_reinhard, std::min<size_t>((3 * n + 255) / 256, 8192), 256, 0, d, (uint32_t)(3 * n), wb);
+    PICO_HIP(c, hipMemcpyAsync(rgb, d, 12 * n, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_relmse_d(pico_hip_ctx* c, const float* img, const double* ref, double count, uint32_t n_pixels, double* out) {
+    if (!c) return PICO_ERR_ARG;
+    if (!img || !ref || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if (!(count > 0)) return fail(c, PICO_ERR_ARG, "reference count must be positive");
+    double* acc = (double*)scratch(c, "relmse", 64);
+    if (!acc) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    const uint32_t n3 = 3 * n_pixels;
+    PICO_LAUNCH(c, "zero", k_zero_f64, 1, 1, 0, acc);
+    PICO_LAUNCH(c, "relmse", k_relmse, std::min<uint32_t>((n3 + 255) / 256, 4096), 256, 0, img, ref, 1.0 / count, n3, acc);
+    double h = 0.0;
+    PICO_HIP(c, hipMemcpyAsync(&h, acc, 8, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    *out = n_pixels ? h / (double)n3 : 0.0;
+    return PICO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,735 @@
+// Wavefront path tracer: the MI355X replacement of the reference's tiled
+// thread pool, per-pixel sample loop and recursive integrator
+// (Source/Core/Scene.cpp:143-201, Source/Render/Integrators.cpp:46-209).
+//
+// A render runs in chunks.  A chunk takes every still-active (view, pixel)
+// item and S samples of it, i.e. P = n_active * S paths, and runs them
+// through persistent kernels that pull work from device-resident queues:
+//   generate  camera rays, per-(pixel, sample) PCG32 streams   -> extend queue 0
+//   extend    closest hit (TLAS -> BLAS)                        -> hit records
+//   shade     emission, BSDF sample, NEE light sample, RR       -> shadow queue, extend queue b+1
+//   shadow    closest hit of the NEE ray; light -> add radiance
+//   finalize  per-path radiance (NaN -> pink), ray count
+//   accumulate  the reference's running mean / variance and early-out, in sample order
+//   compact   surviving items -> next chunk's active list
+// Every count lives in device memory; the host never synchronises inside a
+// render.  Queue appends are wave-aggregated (ballot + popcount, one atomic per
+// wave); persistent waves fetch 64 queue entries per atomic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/pico_hip.h"
+#include "ctx.h"
+
+using namespace pico;
+
+namespace pico {
+DCam make_dcam(const pico_camera& c, uint32_t w, uint32_t h);
+int check_camera(pico_hip_ctx* c, const pico_camera* cam, uint32_t w, uint32_t h);
+}
+
+namespace {
+
+constexpr uint32_t kMaxViews = 64;
+constexpr uint32_t kViewShift = 26;
+constexpr uint32_t kPixMask = (1u << kViewShift) - 1u;
+constexpr uint32_t kMaxBounce = 64;
+constexpr int kBlock = 128;
+
+enum : uint32_t { MODE_ACCUM = 0, MODE_SAMPLES = 1, MODE_SUM64 = 2 };
+
+struct DView {
+    DCam cam;
+    uint32_t spp, max_depth, frame_seed, view_id;
+    float max_variance;
+    uint32_t mode, resume, sample_base;
+    float* pixels;
+    uint32_t* counts;
+    float* variance;
+    double* sum64;
+    float* samples_out;
+    uint32_t* count0;   // per-pixel sample count at render start
+    uint8_t* done;      // per-pixel
+    unsigned long long* stats;  // 4 counters: rays, samples, camera, (pad)
+};
+
+struct Ctl {
+    uint32_t n_active;
+    uint32_t n_next;
+    uint32_t gen_head;
+    uint32_t overflow;
+    uint32_t ext_count[kMaxBounce + 2];
+    uint32_t ext_head[kMaxBounce + 2];
+    uint32_t shade_head[kMaxBounce + 2];
+    uint32_t sh_count[kMaxBounce + 2];
+    uint32_t sh_head[kMaxBounce + 2];
+    unsigned long long rays_cam, rays_ext, rays_sh, rays_acc, samples_acc;
+    unsigned long long nodes, tris, insts;
+};
+
+struct Pool {
+    float4* ray_o;   // origin, w: 1 when the camera ray missed
+    float4* ray_d;
+    float4* thr;
+    float4* rad;     // radiance, w: rays traced by the path
+    uint4* rng;      // hs state, ms state
+    uint32_t* key;   // (view << 26) | pixel
+    float4* hit;     // t, u, v, instance bits
+    uint32_t* hprim;
+    float4* sh_o;
+    float4* sh_d;
+    float4* sh_tp;   // throughput * direct pdf
+    float4* sh_en;   // BSDF sample energy
+    uint32_t* q0;
+    uint32_t* q1;
+    uint32_t* shq;
+    float4* samp;    // finalized radiance + rays
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated append: every active lane calls it; lanes with pred get
+// consecutive slots.  Returns the slot (valid only where pred).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    const uint64_t m = __ballot(pred);
+    const uint32_t n = (uint32_t)__popcll(m);
+    const uint64_t active = __ballot(1);
+    const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)active) - 1);
+    uint32_t base = 0;
+    if (lane_id() == leader && n) base = atomicAdd(counter, n);
+    base = __shfl(base, leader);
+    const uint64_t below = m & ((1ull << lane_id()) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+// Persistent work loop helper: each wave claims 64 consecutive entries.
+__device__ __forceinline__ uint32_t wave_claim(uint32_t* head) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(head, 64u);
+    return __shfl(base, 0);
+}
+
+__global__ void k_reset_chunk(Ctl* ctl, uint32_t first) {
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        ctl->n_active = first ? ctl->n_active : ctl->n_next;
+        ctl->n_next = 0;
+        ctl->gen_head = 0;
+    }
+    for (uint32_t b = t; b < kMaxBounce + 2; b += blockDim.x) {
+        ctl->ext_count[b] = 0; ctl->ext_head[b] = 0; ctl->shade_head[b] = 0;
+        ctl->sh_count[b] = 0; ctl->sh_head[b] = 0;
+    }
+}
+
+// Per-view set-up: sample-count snapshot, cleared outputs (unless resuming),
+// done flags, and the initial active list.
+__global__ void k_init_view(DView* views, uint32_t vi, uint32_t* active, Ctl* ctl) {
+    const DView& V = views[vi];
+    const uint32_t n = V.cam.w * V.cam.h;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t p = base + threadIdx.x;
+        bool keep = false;
+        if (p < n) {
+            uint32_t c0 = 0;
+            if (V.mode == MODE_ACCUM) {
+                if (V.resume) {
+                    c0 = V.counts[p];
+                } else {
+                    V.counts[p] = 0;
+                    for (int k = 0; k < 3; ++k) { V.pixels[3 * p + k] = 0.0f; V.variance[3 * p + k] = 0.0f; }
+                }
+            }
+            V.count0[p] = c0;
+            keep = V.mode != MODE_ACCUM || c0 < V.spp;
+            V.done[p] = keep ? 0 : 1;
+        }
+        const uint32_t slot = wave_append(&ctl->n_active, keep);
+        if (keep) active[slot] = (vi << kViewShift) | p;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ views, const uint32_t* __restrict__ active,
+                                                     Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
+    const uint32_t na = ctl->n_active;
+    const uint32_t total = na * S;
+    for (;;) {
+        const uint32_t base = wave_claim(&ctl->gen_head);
+        if (base >= total) break;
+        const uint32_t i = base + lane_id();
+        bool live = false;
+        if (i < total) {
+            const uint32_t j = i / na, k = i - j * na;
+            const uint32_t key = active[k];
+            const DView& V = views[key >> kViewShift];
+            const uint32_t p = key & kPixMask;
+            const uint32_t rel = off + j;
+            if (rel < V.spp) {
+                live = true;
+                const uint32_t sample = (V.mode == MODE_ACCUM ? V.count0[p] : V.sample_base) + rel;
+                Rng rng;
+                rng.seed(V.frame_seed, V.view_id, sample, p);
+                const float2 xi = rng.next2d();
+                const f3 d = camera_dir(V.cam, xi, p % V.cam.w, p / V.cam.w);
+                P.ray_o[i] = f4(V.cam.pos, 0.0f);
+                P.ray_d[i] = f4(d, 0.0f);
+                P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                P.rng[i] = make_uint4((uint32_t)rng.hs, (uint32_t)(rng.hs >> 32), (uint32_t)rng.ms, (uint32_t)(rng.ms >> 32));
+                P.key[i] = key;
+            }
+        }
+        const uint32_t slot = wave_append(&ctl->ext_count[0], live);
+        if (live) P.q0[slot] = i;
+    }
+}
+
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene s, Ctl* ctl, uint32_t b, Pool P) {
+    __shared__ uint32_t stk[STACK * kBlock];
+    const uint32_t* q = (b & 1) ? P.q1 : P.q0;
+    const uint32_t n = ctl->ext_count[b];
+    TraceCounters tc{0, 0, 0};
+    for (;;) {
+        const uint32_t base = wave_claim(&ctl->ext_head[b]);
+        if (base >= n) break;
+        const uint32_t e = base + lane_id();
+        if (e < n) {
+            const uint32_t i = q[e];
+            const float4 o = P.ray_o[i], d = P.ray_d[i];
+            const Hit h = trace_closest<STACK, COUNT>(s, xyz(o), xyz(d), stk + threadIdx.x, kBlock, &tc, &ctl->overflow);
+            P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.inst));
+            P.hprim[i] = h.prim;
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&ctl->nodes, (unsigned long long)tc.nodes);
+        atomicAdd(&ctl->tris, (unsigned long long)tc.tris);
+        atomicAdd(&ctl->insts, (unsigned long long)tc.insts);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __restrict__ views, Ctl* ctl, uint32_t b, Pool P) {
+    const uint32_t* q = (b & 1) ? P.q1 : P.q0;
+    uint32_t* qn = (b & 1) ? P.q0 : P.q1;
+    const uint32_t n = ctl->ext_count[b];
+    for (;;) {
+        const uint32_t base = wave_claim(&ctl->shade_head[b]);
+        if (base >= n) break;
+        const uint32_t e = base + lane_id();
+        bool cont = false, shadow = false;
+        uint32_t i = 0;
+        if (e < n) {
+            i = q[e];
+            const float4 h4 = P.hit[i];
+            const uint32_t inst = __float_as_uint(h4.w);
+            const uint32_t key = P.key[i];
+            const DView& V = views[key >> kViewShift];
+            const f3 dir = xyz(P.ray_d[i]);
+            float4 r4 = P.rad[i];
+            f3 rad = xyz(r4);
+            f3 thr = xyz(P.thr[i]);
+            const f3 sky = mk(s.sky[0], s.sky[1], s.sky[2]);
+            if (inst == 0xffffffffu) {
+                if (b == 0) {
+                    rad = sky;  // Integrators.cpp:152 (no NaN rule on this path)
+                    P.ray_o[i].w = 1.0f;
+                } else {
+                    rad = rad + thr * sky;  // Integrators.cpp:208
+                }
+            } else if (b < V.max_depth) {
+                Hit h;
+                h.t = h4.x; h.u = h4.y; h.v = h4.z; h.inst = inst; h.prim = P.hprim[i];
+                const Frag f = make_frag(s, h);
+                const DInstance& I = s.inst[inst];
+                const uint32_t kind = I.bsdf;
+                const DMaterial& M = s.mats[I.material];
+                const uint4 rs = P.rng[i];
+                Rng rng;
+                rng.set_streams(key & kPixMask);
+                rng.hs = (uint64_t)rs.x | ((uint64_t)rs.y << 32);
+                rng.ms = (uint64_t)rs.z | ((uint64_t)rs.w << 32);
+                const BSample smp = bsdf_sample(kind, s.microfacet, M, f.n, dir, rng);
+                if (kind == PICO_BSDF_LIGHT) rad = rad + thr * smp.energy;  // Integrators.cpp:164-168
+                // next-event estimation, Integrators.cpp:46-128
+                if (kind != PICO_BSDF_LIGHT) {
+                    const float u = rng.uniform();
+                    if (s.n_lights) {
+                        uint32_t li = (uint32_t)(u * (float)s.n_lights);
+                        if (li >= s.n_lights) li = s.n_lights - 1;
+                        const DLight L = s.lights[li];
+                        // AliasTable::sample, Source/Util/AliasTable.cpp:89-104
+                        const uint32_t bucket = rng.next_u32() % L.n_tris;
+                        const float coin = (float)rng.next_u32() / 4294967296.0f;
+                        const uint32_t tri = coin <= s.alias_w[L.alias_offset + bucket] ? bucket : s.alias_i[L.alias_offset + bucket];
+                        // LowerLevelMeshBVH::sample_geometry, LowerLevelMeshBVH.cpp:119-132
+                        const float2 xi = rng.next2d();
+                        const float su0 = sqrtf(xi.x);
+                        const float bx = 1 - su0, by = xi.y * su0;
+                        const uint32_t t = L.tri_offset + tri;
+                        const f3 p0 = ld3(s.pos + 3 * s.idx[3 * t]), p1 = ld3(s.pos + 3 * s.idx[3 * t + 1]),
+                                 p2 = ld3(s.pos + 3 * s.idx[3 * t + 2]);
+                        const f3 sp = (((1.0f - bx - by) * p0) + (bx * p1)) + (by * p2);
+                        const float* LM = s.xf[L.instance].m;
+                        const f3 wp = mk((LM[0] * sp.x + LM[4] * sp.y) + (LM[8] * sp.z + LM[12]),
+                                         (LM[1] * sp.x + LM[5] * sp.y) + (LM[9] * sp.z + LM[13]),
+                                         (LM[2] * sp.x + LM[6] * sp.y) + (LM[10] * sp.z + LM[14]));
+                        const f3 to_light = normalize(wp - f.p);
+                        if (!(dot(to_light, f.n) < 0.0f)) {
+                            const float pdf = direct_pdf(kind, s.microfacet, M, f.n, -dir, to_light);
+                            P.sh_o[i] = f4(f.p + 0.01f * to_light, 0.0f);
+                            P.sh_d[i] = f4(to_light, 0.0f);
+                            P.sh_tp[i] = f4(thr * pdf, 0.0f);
+                            P.sh_en[i] = f4(smp.energy, 0.0f);
+                            shadow = true;
+                            r4.w += 1.0f;
+                        }
+                    }
+                }
+                if (smp.P != 0.0f) {
+                    // Russian roulette then throughput update, Integrators.cpp:185-197
+                    const float qk = smax(smax(thr.x, thr.y), thr.z);
+                    if (!(rng.uniform() > qk)) {
+                        thr = thr * (1.0f / qk);
+                        thr = thr * (smp.P * smp.energy);
+                        P.ray_o[i] = f4(f.p + 0.01f * f.n, 0.0f);
+                        P.ray_d[i] = f4(smp.L, 0.0f);
+                        P.thr[i] = f4(thr, 0.0f);
+                        cont = true;
+                        r4.w += 1.0f;
+                    }
+                }
+                P.rng[i] = make_uint4((uint32_t)rng.hs, (uint32_t)(rng.hs >> 32), (uint32_t)rng.ms, (uint32_t)(rng.ms >> 32));
+            }
+            P.rad[i] = make_float4(rad.x, rad.y, rad.z, r4.w);
+        }
+        const uint32_t s1 = wave_append(&ctl->ext_count[b + 1], cont);
+        if (cont) qn[s1] = i;
+        const uint32_t s2 = wave_append(&ctl->sh_count[b], shadow);
+        if (shadow) P.shq[s2] = i;
+    }
+}
+
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_t b, Pool P) {
+    __shared__ uint32_t stk[STACK * kBlock];
+    const uint32_t n = ctl->sh_count[b];
+    TraceCounters tc{0, 0, 0};
+    for (;;) {
+        const uint32_t base = wave_claim(&ctl->sh_head[b]);
+        if (base >= n) break;
+        const uint32_t e = base + lane_id();
+        if (e < n) {
+            const uint32_t i = P.shq[e];
+            const Hit h = trace_closest<STACK, COUNT>(s, xyz(P.sh_o[i]), xyz(P.sh_d[i]), stk + threadIdx.x, kBlock, &tc,
+                                                      &ctl->overflow);
+            if (h.inst != 0xffffffffu) {
+                const DInstance& I = s.inst[h.inst];
+                if (I.bsdf == PICO_BSDF_LIGHT) {
+                    const f3 em = ld3(s.mats[I.material].emissive);
+                    float4 r = P.rad[i];
+                    const f3 add = (xyz(P.sh_tp[i]) * em) * xyz(P.sh_en[i]);
+                    P.rad[i] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
+                }
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&ctl->nodes, (unsigned long long)tc.nodes);
+        atomicAdd(&ctl->tris, (unsigned long long)tc.tris);
+        atomicAdd(&ctl->insts, (unsigned long long)tc.insts);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize(const DView* __restrict__ views, const uint32_t* __restrict__ active,
+                                                     const Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
+    const uint32_t na = ctl->n_active;
+    const uint32_t total = na * S;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const uint32_t j = i / na, k = i - j * na;
+        const uint32_t key = active[k];
+        const DView& V = views[key >> kViewShift];
+        const uint32_t rel = off + j;
+        if (rel >= V.spp) continue;
+        const float4 r = P.rad[i];
+        f3 res = xyz(r);
+        if (P.ray_o[i].w == 0.0f && bad(res)) res = mk(1.0f, 0.4f, 0.7f);  // Integrators.cpp:145-146
+        if (V.mode == MODE_SAMPLES) {
+            const uint32_t p = key & kPixMask;
+            float* o = V.samples_out + ((size_t)rel * V.cam.w * V.cam.h + p) * 3;
+            o[0] = res.x; o[1] = res.y; o[2] = res.z;
+            atomicAdd(&V.stats[0], (unsigned long long)r.w);
+            atomicAdd(&V.stats[1], 1ull);
+        } else {
+            P.samp[i] = make_float4(res.x, res.y, res.z, r.w);
+        }
+    }
+}
+
+// Running mean / variance in sample order with the early-out,
+// Source/Core/Scene.cpp:173-197.
+__global__ __launch_bounds__(kBlock) void k_accumulate(const DView* __restrict__ views, const uint32_t* __restrict__ active,
+                                                       Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
+    const uint32_t na = ctl->n_active;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
+        const uint32_t key = active[k];
+        const DView& V = views[key >> kViewShift];
+        const uint32_t p = key & kPixMask;
+        if (V.mode == MODE_SAMPLES) {
+            if (off + S >= V.spp) V.done[p] = 1;
+            continue;
+        }
+        unsigned long long rays = 0, ns = 0;
+        bool stop = false;
+        if (V.mode == MODE_SUM64) {
+            double sx = V.sum64[3 * p], sy = V.sum64[3 * p + 1], sz = V.sum64[3 * p + 2];
+            for (uint32_t j = 0; j < S && off + j < V.spp; ++j) {
+                const float4 x = P.samp[(size_t)j * na + k];
+                sx += (double)x.x; sy += (double)x.y; sz += (double)x.z;
+                rays += (unsigned long long)x.w;
+                ++ns;
+            }
+            V.sum64[3 * p] = sx; V.sum64[3 * p + 1] = sy; V.sum64[3 * p + 2] = sz;
+        } else {
+            uint32_t cnt = V.counts[p];
+            f3 pix = ld3(V.pixels + 3 * p), var = ld3(V.variance + 3 * p);
+            for (uint32_t j = 0; j < S && off + j < V.spp; ++j) {
+                const float4 x4 = P.samp[(size_t)j * na + k];
+                f3 x = xyz(x4);
+                const uint32_t prev = cnt;
+                if (prev < 1) {
+                    cnt = 1;
+                    var = mk(0.0f, 0.0f, 0.0f);
+                } else {
+                    const f3 np = pix + ((x - pix) * (1.0f / (float)prev));
+                    var = ((var * (float)prev) + ((x - pix) * (x - np))) / (float)(prev + 1);
+                    cnt = prev + 1;
+                    x = np;
+                }
+                pix = x;
+                rays += (unsigned long long)x4.w;
+                ++ns;
+                if (prev > 4 && var.x <= V.max_variance && var.y <= V.max_variance && var.z <= V.max_variance) {
+                    stop = true;
+                    break;
+                }
+            }
+            V.counts[p] = cnt;
+            V.pixels[3 * p] = pix.x; V.pixels[3 * p + 1] = pix.y; V.pixels[3 * p + 2] = pix.z;
+            V.variance[3 * p] = var.x; V.variance[3 * p + 1] = var.y; V.variance[3 * p + 2] = var.z;
+        }
+        if (stop || off + S >= V.spp) V.done[p] = 1;
+        atomicAdd(&V.stats[0], rays);
+        atomicAdd(&V.stats[1], ns);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact(const DView* __restrict__ views, const uint32_t* __restrict__ active,
+                                                    uint32_t* __restrict__ next, Ctl* ctl) {
+    const uint32_t na = ctl->n_active;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < na; base += gridDim.x * blockDim.x) {
+        const uint32_t k = base + threadIdx.x;
+        bool keep = false;
+        uint32_t key = 0;
+        if (k < na) {
+            key = active[k];
+            keep = views[key >> kViewShift].done[key & kPixMask] == 0;
+        }
+        const uint32_t slot = wave_append(&ctl->n_next, keep);
+        if (keep) next[slot] = key;
+    }
+}
+
+__global__ void k_sum_rays(Ctl* ctl, uint32_t bounces) {
+    if (threadIdx.x == 0) {
+        unsigned long long e = 0, sh = 0;
+        for (uint32_t b = 1; b <= bounces; ++b) e += ctl->ext_count[b];
+        for (uint32_t b = 0; b <= bounces; ++b) sh += ctl->sh_count[b];
+        ctl->rays_cam += ctl->ext_count[0];
+        ctl->rays_ext += e;
+        ctl->rays_sh += sh;
+    }
+}
+
+struct Plan {
+    uint32_t max_spp = 0, max_depth = 0, total_items = 0;
+};
+
+template <int STACK>
+int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, const Pool& P, const DView* dviews, bool count,
+                          uint32_t grid_trace, uint32_t grid_shade, bool with_shadow) {
+    if (count) PICO_LAUNCH(c, "extend", (k_extend<STACK, true>), grid_trace, kBlock, 0, c->dev, ctl, b, P);
+    else PICO_LAUNCH(c, "extend", (k_extend<STACK, false>), grid_trace, kBlock, 0, c->dev, ctl, b, P);
+    PICO_LAUNCH(c, "shade", k_shade, grid_shade, kBlock, 0, c->dev, dviews, ctl, b, P);
+    if (with_shadow) {
+        if (count) PICO_LAUNCH(c, "shadow", (k_shadow<STACK, true>), grid_trace, kBlock, 0, c->dev, ctl, b, P);
+        else PICO_LAUNCH(c, "shadow", (k_shadow<STACK, false>), grid_trace, kBlock, 0, c->dev, ctl, b, P);
+    }
+    return PICO_OK;
+}
+
+// Runs the chunked wavefront over `nv` views already described on the host.
+int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, const volatile int32_t* cancel, bool count,
+              pico_stats* stats, uint32_t n_stats) {
+    if (!c->has_scene) return fail(c, PICO_ERR_STATE, "no scene uploaded");
+    const uint32_t nv = (uint32_t)hv.size();
+    if (nv == 0 || nv > kMaxViews) return fail(c, PICO_ERR_LIMIT, "between 1 and 64 views per launch");
+    Plan pl;
+    size_t pix_total = 0;
+    for (auto& v : hv) {
+        pl.max_spp = std::max(pl.max_spp, v.spp);
+        pl.max_depth = std::max(pl.max_depth, v.max_depth);
+        pix_total += (size_t)v.cam.w * v.cam.h;
+    }
+    if (pix_total > (1ull << 31)) return fail(c, PICO_ERR_LIMIT, "too many pixels in one launch");
+    if (pl.max_depth > kMaxBounce - 1) return fail(c, PICO_ERR_LIMIT, "max_depth above 63");
+    if (pl.max_spp == 0) return PICO_OK;
+    uint32_t cap = queue_paths ? queue_paths : c->queue_paths;
+    cap = std::max<uint32_t>(cap, 64);
+    // samples per chunk: the first chunk ends right after the sample at which
+    // the reference's early-out is first tested (prev_count > 4)
+    const uint32_t s_max = std::max<uint32_t>(1, (uint32_t)std::min<size_t>(pl.max_spp, cap / pix_total));
+    const size_t paths = std::max<size_t>((size_t)s_max * pix_total, 64);
+    // buffers
+    char* pb = (char*)scratch(c, "pool", paths * (16 * 11 + 4 * 6) + 4096);
+    uint32_t* act = (uint32_t*)scratch(c, "active", pix_total * 8 + 64);
+    Ctl* ctl = (Ctl*)scratch(c, "ctl", sizeof(Ctl));
+    uint32_t* count0 = (uint32_t*)scratch(c, "count0", pix_total * 4 + 64);
+    uint8_t* done = (uint8_t*)scratch(c, "done", pix_total + 64);
+    unsigned long long* vstats = (unsigned long long*)scratch(c, "vstats", nv * 32 + 64);
+    DView* dv = (DView*)scratch(c, "views", sizeof(DView) * nv);
+    if (!pb || !act || !ctl || !count0 || !done || !vstats || !dv) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    Pool P;
+    {
+        char* q = pb;
+        auto take = [&](size_t bytes) { char* r = q; q += (bytes + 255) & ~(size_t)255; return r; };
+        P.ray_o = (float4*)take(paths * 16); P.ray_d = (float4*)take(paths * 16);
+        P.thr = (float4*)take(paths * 16); P.rad = (float4*)take(paths * 16);
+        P.rng = (uint4*)take(paths * 16); P.hit = (float4*)take(paths * 16);
+        P.sh_o = (float4*)take(paths * 16); P.sh_d = (float4*)take(paths * 16);
+        P.sh_tp = (float4*)take(paths * 16); P.sh_en = (float4*)take(paths * 16);
+        P.samp = (float4*)take(paths * 16);
+        P.key = (uint32_t*)take(paths * 4); P.hprim = (uint32_t*)take(paths * 4);
+        P.q0 = (uint32_t*)take(paths * 4); P.q1 = (uint32_t*)take(paths * 4); P.shq = (uint32_t*)take(paths * 4);
+    }
+    uint32_t* act_a = act;
+    uint32_t* act_b = act + pix_total;
+    size_t off_px = 0;
+    for (uint32_t v = 0; v < nv; ++v) {
+        hv[v].count0 = count0 + off_px;
+        hv[v].done = done + off_px;
+        hv[v].stats = vstats + 4 * v;
+        off_px += (size_t)hv[v].cam.w * hv[v].cam.h;
+    }
+    PICO_HIP(c, hipMemcpyAsync(dv, hv.data(), sizeof(DView) * nv, hipMemcpyHostToDevice, c->stream));
+    PICO_HIP(c, hipMemsetAsync(ctl, 0, sizeof(Ctl), c->stream));
+    PICO_HIP(c, hipMemsetAsync(vstats, 0, nv * 32, c->stream));
+    for (uint32_t v = 0; v < nv; ++v) {
+        const uint32_t n = hv[v].cam.w * hv[v].cam.h;
+        const uint32_t g = std::min<uint32_t>((n + 255) / 256, 4096);
+        PICO_LAUNCH(c, "init_view", k_init_view, g, 256, 0, dv, v, act_a, ctl);
+    }
+    const uint32_t grid_trace = (uint32_t)c->num_cus * 8;
+    const uint32_t grid_shade = (uint32_t)c->num_cus * 8;
+    const uint32_t grid_lin = (uint32_t)c->num_cus * 16;
+    hipEvent_t e0, e1;
+    PICO_HIP(c, hipEventCreate(&e0));
+    PICO_HIP(c, hipEventCreate(&e1));
+    PICO_HIP(c, hipEventRecord(e0, c->stream));
+    bool first = true;
+    bool cancelled = false;
+    const bool accum_mode = hv[0].mode == MODE_ACCUM;
+    for (uint32_t off = 0; off < pl.max_spp;) {
+        if (cancel && *cancel) { cancelled = true; break; }
+        uint32_t S = s_max;
+        if (off == 0 && accum_mode) S = std::min<uint32_t>(S, 6);
+        S = std::min(S, pl.max_spp - off);
+        PICO_LAUNCH(c, "reset_chunk", k_reset_chunk, 1, 256, 0, ctl, first ? 1u : 0u);
+        first = false;
+        PICO_LAUNCH(c, "generate", k_generate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
+        for (uint32_t b = 0; b <= pl.max_depth; ++b) {
+            int rc;
+            if (c->stack_need <= 32) rc = launch_bounce_kernels<32>(c, ctl, b, P, dv, count, grid_trace, grid_shade, b < pl.max_depth);
+            else rc = launch_bounce_kernels<64>(c, ctl, b, P, dv, count, grid_trace, grid_shade, b < pl.max_depth);
+            if (rc) return rc;
+        }
+        PICO_LAUNCH(c, "sum_rays", k_sum_rays, 1, 64, 0, ctl, pl.max_depth);
+        PICO_LAUNCH(c, "finalize", k_finalize, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
+        PICO_LAUNCH(c, "accumulate", k_accumulate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
+        PICO_LAUNCH(c, "compact", k_compact, grid_lin, kBlock, 0, dv, act_a, act_b, ctl);
+        std::swap(act_a, act_b);
+        off += S;
+    }
+    PICO_HIP(c, hipEventRecord(e1, c->stream));
+    Ctl hc;
+    std::vector<unsigned long long> hs(4 * nv);
+    PICO_HIP(c, hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(hs.data(), vstats, nv * 32, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (hc.overflow) return fail(c, PICO_ERR_LIMIT, "traversal stack overflow");
+    if (stats) {
+        for (uint32_t v = 0; v < n_stats; ++v) {
+            pico_stats& st = stats[v];
+            std::memset(&st, 0, sizeof(st));
+            st.render_ms = ms;
+            st.cancelled = cancelled ? 1 : 0;
+            if (n_stats == 1) {
+                // the whole launch: rays of every pass (including samples the
+                // early-out discarded), and rays of the accumulated samples
+                st.camera_rays = hc.rays_cam;
+                st.extension_rays = hc.rays_ext;
+                st.shadow_rays = hc.rays_sh;
+                st.node_visits = hc.nodes;
+                st.tri_tests = hc.tris;
+                st.instance_visits = hc.insts;
+                for (uint32_t k = 0; k < nv; ++k) {
+                    st.accumulated_rays += hs[4 * k];
+                    st.samples += hs[4 * k + 1];
+                }
+            } else {
+                st.accumulated_rays = hs[4 * v];
+                st.samples = hs[4 * v + 1];
+            }
+        }
+    }
+    return cancelled ? fail(c, PICO_ERR_CANCELLED, "cancelled") : PICO_OK;
+}
+
+DView base_view(const pico_camera* cam, const pico_render_params* p) {
+    DView v{};
+    v.cam = make_dcam(*cam, p->width, p->height);
+    v.spp = p->spp;
+    v.max_depth = p->max_depth;
+    v.frame_seed = p->frame_seed;
+    v.view_id = p->view_id;
+    v.max_variance = p->max_variance;
+    v.resume = (p->flags & PICO_RENDER_RESUME) ? 1u : 0u;
+    return v;
+}
+
+int check_params(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p) {
+    if (!p) return fail(c, PICO_ERR_ARG, "params is NULL");
+    int rc = check_camera(c, cam, p->width, p->height);
+    if (rc) return rc;
+    if (p->max_depth > kMaxBounce - 1) return fail(c, PICO_ERR_LIMIT, "max_depth above 63");
+    return PICO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pico_hip_render_d(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p, float* dpix, uint32_t* dcnt,
+                      float* dvar, const volatile int32_t* cancel, pico_stats* stats) {
+    if (!c) return PICO_ERR_ARG;
+    int rc = check_params(c, cam, p);
+    if (rc) return rc;
+    if (!dpix || !dcnt || !dvar) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    std::vector<DView> v(1, base_view(cam, p));
+    v[0].mode = MODE_ACCUM;
+    v[0].pixels = dpix;
+    v[0].counts = dcnt;
+    v[0].variance = dvar;
+    return run_views(c, v, 0, cancel, (p->flags & PICO_RENDER_COUNT) != 0, stats, stats ? 1 : 0);
+}
+
+int pico_hip_render(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p, float* pix, uint32_t* cnt,
+                    float* var, const volatile int32_t* cancel, pico_stats* stats) {
+    if (!c) return PICO_ERR_ARG;
+    int rc = check_params(c, cam, p);
+    if (rc) return rc;
+    if (!pix || !cnt || !var) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    const size_t n = (size_t)p->width * p->height;
+    char* b = (char*)scratch(c, "frame", n * 28 + 64);
+    if (!b) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    float* dp = (float*)b;
+    float* dv = dp + 3 * n;
+    uint32_t* dc = (uint32_t*)(dv + 3 * n);
+    if (p->flags & PICO_RENDER_RESUME) {
+        PICO_HIP(c, hipMemcpyAsync(dp, pix, 12 * n, hipMemcpyHostToDevice, c->stream));
+        PICO_HIP(c, hipMemcpyAsync(dv, var, 12 * n, hipMemcpyHostToDevice, c->stream));
+        PICO_HIP(c, hipMemcpyAsync(dc, cnt, 4 * n, hipMemcpyHostToDevice, c->stream));
+    }
+    rc = pico_hip_render_d(c, cam, p, dp, dc, dv, cancel, stats);
+    if (rc && rc != PICO_ERR_CANCELLED) return rc;
+    PICO_HIP(c, hipMemcpyAsync(pix, dp, 12 * n, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(var, dv, 12 * n, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(cnt, dc, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+int pico_hip_render_samples(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p, uint32_t sample_base,
+                            float* out) {
+    if (!c) return PICO_ERR_ARG;
+    int rc = check_params(c, cam, p);
+    if (rc) return rc;
+    if (!out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    const size_t n = (size_t)p->width * p->height * p->spp * 3;
+    float* d = (float*)scratch(c, "samples_out", n * 4 + 64);
+    if (!d) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    std::vector<DView> v(1, base_view(cam, p));
+    v[0].mode = MODE_SAMPLES;
+    v[0].sample_base = sample_base;
+    v[0].samples_out = d;
+    rc = run_views(c, v, 0, nullptr, false, nullptr, 0);
+    if (rc) return rc;
+    PICO_HIP(c, hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_render_accum64_d(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p, uint32_t sample_base,
+                              double* d_sum, pico_stats* stats) {
+    if (!c) return PICO_ERR_ARG;
+    int rc = check_params(c, cam, p);
+    if (rc) return rc;
+    if (!d_sum) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    std::vector<DView> v(1, base_view(cam, p));
+    v[0].mode = MODE_SUM64;
+    v[0].sample_base = sample_base;
+    v[0].sum64 = d_sum;
+    return run_views(c, v, 0, nullptr, false, stats, stats ? 1 : 0);
+}
+
+int pico_hip_render_views_d(pico_hip_ctx* c, const pico_view* views, uint32_t n, uint32_t queue_paths, pico_stats* stats) {
+    if (!c) return PICO_ERR_ARG;
+    if (!views || n == 0) return fail(c, PICO_ERR_ARG, "no views");
+    if (n > kMaxViews) return fail(c, PICO_ERR_LIMIT, "at most 64 views per launch");
+    std::vector<DView> hv;
+    for (uint32_t i = 0; i < n; ++i) {
+        int rc = check_params(c, &views[i].camera, &views[i].params);
+        if (rc) return rc;
+        if (!views[i].d_pixels || !views[i].d_sample_count || !views[i].d_variance) return fail(c, PICO_ERR_ARG, "NULL buffer");
+        DView v = base_view(&views[i].camera, &views[i].params);
+        v.mode = MODE_ACCUM;
+        v.pixels = views[i].d_pixels;
+        v.counts = views[i].d_sample_count;
+        v.variance = views[i].d_variance;
+        hv.push_back(v);
+    }
+    return run_views(c, hv, queue_paths, nullptr, false, stats, stats ? n : 0);
+}
+
+int pico_hip_set_tuning(pico_hip_ctx* c, uint32_t queue_paths, uint32_t trace_block) {
+    if (!c) return PICO_ERR_ARG;
+    if (queue_paths) c->queue_paths = std::max<uint32_t>(queue_paths, 1024);
+    if (trace_block) c->trace_block = trace_block;
+    return PICO_OK;
+}
+
+int pico_hip_get_tuning(pico_hip_ctx* c, uint32_t* queue_paths, uint32_t* trace_block) {
+    if (!c) return PICO_ERR_ARG;
+    if (queue_paths) *queue_paths = c->queue_paths;
+    if (trace_block) *trace_block = c->trace_block;
+    return PICO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,440 @@
+// Device-side building blocks of the path tracer: vector math, PCG32 streams,
+// the two-level closest-hit traversal and the shading functions.
+//
+// Shading follows the reference's arithmetic order (glm's dot products are
+// (x*x + y*y) + z*z, matrix-vector products go column by column) so that the
+// device and the CPU restatement consume random numbers identically and agree
+// per path to rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "device_types.h"
+
+namespace pico {
+
+#define PICO_PI_D 3.14159265358979323846
+
+// ------------------------------------------------------------------ math --
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ f3 reflect(f3 i, f3 n) { return i - (n * dot(n, i)) * 2.0f; }
+// std::max / std::min semantics
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ bool bad(f3 a) {
+    return isnan(a.x) || isnan(a.y) || isnan(a.z) || isinf(a.x) || isinf(a.y) || isinf(a.z);
+}
+__device__ __forceinline__ f3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// --------------------------------------------------------------- random ---
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+// PCG32 output step (O'Neill 2014); `inc` is derived from the stream id.
+__device__ __forceinline__ uint32_t pcg_next(uint64_t& state, uint64_t inc) {
+    const uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+__device__ __forceinline__ uint64_t pcg_seed(uint64_t initstate, uint64_t initseq) {
+    const uint64_t inc = (initseq << 1u) | 1u;
+    uint64_t st = 0;
+    pcg_next(st, inc);
+    st += initstate;
+    pcg_next(st, inc);
+    return st;
+}
+
+// Per-path streams: hs replaces the reference's xorshift/Hammersley generator,
+// ms its mt19937_64 uniform float.  Stream ids 2p and 2p+1 of pixel p.
+struct Rng {
+    uint64_t hs, ms;
+    uint64_t hinc, minc;
+    __device__ void seed(uint32_t frame, uint32_t view, uint32_t sample, uint32_t pixel) {
+        const uint64_t base = mix64(((uint64_t)frame << 32) | sample) + (uint64_t)view * 0xD1B54A32D192ED03ULL;
+        hinc = ((2ull * pixel) << 1) | 1u;
+        minc = ((2ull * pixel + 1ull) << 1) | 1u;
+        hs = pcg_seed(base, 2ull * pixel);
+        ms = pcg_seed(base ^ 0xA0761D6478BD642FULL, 2ull * pixel + 1ull);
+    }
+    __device__ void set_streams(uint32_t pixel) {
+        hinc = ((2ull * pixel) << 1) | 1u;
+        minc = ((2ull * pixel + 1ull) << 1) | 1u;
+    }
+    __device__ uint32_t next_u32() { return pcg_next(hs, hinc); }
+    // Hammersley_Generator::next (Source/Core/RandUtils.cpp:23-27,87-90): one
+    // draw gives (i / 2^32, radical_inverse(i)).
+    __device__ float2 next2d() {
+        const uint32_t i = next_u32();
+        return make_float2((float)i * 0x1p-32f, (float)__builtin_bitreverse32(i) * 0x1p-32f);
+    }
+    __device__ float uniform() { return (float)(pcg_next(ms, minc) >> 8) * 0x1p-24f; }
+};
+
+// --------------------------------------------------------------- scene ----
+struct DevScene {
+    const DNode* nodes;
+    const DTri* tris;
+    const DInstance* inst;
+    const DXform* xf;
+    const DMaterial* mats;
+    const DLight* lights;
+    const float* alias_w;
+    const uint32_t* alias_i;
+    const float* pos;
+    const float* nrm;
+    const uint32_t* idx;
+    uint32_t tlas_root;
+    uint32_t n_lights;
+    float sky[3];
+    uint32_t microfacet;
+};
+
+struct Hit { float t, u, v; uint32_t inst, prim; };
+
+struct TraceCounters { uint32_t nodes, tris, insts; };
+
+constexpr uint32_t kSentinel = 0xfffffffeu;
+
+__device__ __forceinline__ float4 ldg4(const void* p) { return *reinterpret_cast<const float4*>(p); }
+
+// Closest hit of a world-space ray against the TLAS and the instanced BLASes.
+// The traversal stack lives in LDS (`stk`, STACK entries at stride `stride`);
+// near child first, far child pushed, an instance boundary pushes a sentinel
+// that restores the world-space ray when popped.  t is the ray parameter of
+// the world ray (directions are unit length, so t is the hit distance).
+// Box tests widen tmax by 2^-21 relative so that rounding never culls a
+// triangle whose bound the ray only grazes.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint32_t* stk, uint32_t stride,
+                                             TraceCounters* cnt, uint32_t* overflow) {
+#pragma clang fp contract(fast)
+    Hit h;
+    h.t = INFINITY; h.u = 0.0f; h.v = 0.0f; h.inst = 0xffffffffu; h.prim = 0xffffffffu;
+    f3 ro = o, rd = d;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    f3 oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
+    uint32_t ref = s.tlas_root;
+    uint32_t sp = 0;
+    uint32_t cur = 0xffffffffu;
+    bool in_blas = false;
+    const float kWide = 1.0000005f;
+    for (;;) {
+        if (!(ref & kLeafBit)) {
+            const DNode* n = s.nodes + ref;
+            const float4 a = ldg4(&n->c0x[0]);
+            const float4 b = ldg4(&n->c1x[0]);
+            const float4 c = ldg4(&n->c0z[0]);
+            const uint2 r = *reinterpret_cast<const uint2*>(&n->ref[0]);
+            if (COUNT) cnt->nodes++;
+            // child 0
+            const float x00 = fmaf(a.x, inv.x, oi.x), x01 = fmaf(a.y, inv.x, oi.x);
+            const float y00 = fmaf(a.z, inv.y, oi.y), y01 = fmaf(a.w, inv.y, oi.y);
+            const float z00 = fmaf(c.x, inv.z, oi.z), z01 = fmaf(c.y, inv.z, oi.z);
+            const float t0n = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fminf(z00, z01));
+            const float t0f = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fmaxf(z00, z01)) * kWide;
+            // child 1
+            const float x10 = fmaf(b.x, inv.x, oi.x), x11 = fmaf(b.y, inv.x, oi.x);
+            const float y10 = fmaf(b.z, inv.y, oi.y), y11 = fmaf(b.w, inv.y, oi.y);
+            const float z10 = fmaf(c.z, inv.z, oi.z), z11 = fmaf(c.w, inv.z, oi.z);
+            const float t1n = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fminf(z10, z11));
+            const float t1f = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fmaxf(z10, z11)) * kWide;
+            const bool h0 = r.x != kEmptyRef && t0f >= 0.0f && t0n <= t0f && t0n < h.t;
+            const bool h1 = r.y != kEmptyRef && t1f >= 0.0f && t1n <= t1f && t1n < h.t;
+            if (h0 && h1) {
+                const bool first0 = t0n <= t1n;
+                ref = first0 ? r.x : r.y;
+                if (sp < STACK) stk[sp * stride] = first0 ? r.y : r.x, ++sp;
+                else *overflow = 1u;
+                continue;
+            }
+            if (h0) { ref = r.x; continue; }
+            if (h1) { ref = r.y; continue; }
+        } else if (in_blas) {
+            const uint32_t first = ref & kPrimMask, count = (ref >> 27) & 15u;
+            for (uint32_t k = 0; k < count; ++k) {
+                const DTri* tr = s.tris + first + k;
+                const float4 p0 = ldg4(&tr->v0[0]);
+                const float4 p1 = ldg4(&tr->e1[0]);
+                const float4 p2 = ldg4(&tr->e2[0]);
+                if (COUNT) cnt->tris++;
+                // Moller-Trumbore with the reference's epsilons,
+                // Source/Core/LowerLevelMeshBVH.cpp:165-207
+                const f3 e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z);
+                const f3 hh = cross(rd, e2);
+                const float det = dot(e1, hh);
+                if (det > -0.0000001f && det < 0.0000001f) continue;
+                const float f = 1.0f / det;
+                const f3 sv = ro - mk(p0.x, p0.y, p0.z);
+                const float u = f * dot(sv, hh);
+                if (u < 0.0f || u > 1.0f) continue;
+                const f3 q = cross(sv, e1);
+                const float v = f * dot(rd, q);
+                if (v < 0.0f || u + v > 1.0f) continue;
+                const float t = f * dot(e2, q);
+                if (t > 0.0000001f && t < h.t) {
+                    h.t = t; h.u = u; h.v = v; h.inst = cur; h.prim = __float_as_uint(p0.w);
+                }
+            }
+        } else {
+            // TLAS leaf: one instance; move the ray into object space
+            cur = ref & kPrimMask;
+            const DInstance* I = s.inst + cur;
+            const float4 m0 = ldg4(&I->inv[0][0]);
+            const float4 m1 = ldg4(&I->inv[1][0]);
+            const float4 m2 = ldg4(&I->inv[2][0]);
+            const uint32_t root = I->blas_root;
+            if (COUNT) cnt->insts++;
+            if (sp < STACK) stk[sp * stride] = kSentinel, ++sp;
+            else *overflow = 1u;
+            ro = mk(m0.x * o.x + m0.y * o.y + m0.z * o.z + m0.w, m1.x * o.x + m1.y * o.y + m1.z * o.z + m1.w,
+                    m2.x * o.x + m2.y * o.y + m2.z * o.z + m2.w);
+            rd = mk(m0.x * d.x + m0.y * d.y + m0.z * d.z, m1.x * d.x + m1.y * d.y + m1.z * d.z,
+                    m2.x * d.x + m2.y * d.y + m2.z * d.z);
+            inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+            oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
+            in_blas = true;
+            ref = root;
+            continue;
+        }
+        // pop
+        for (;;) {
+            if (sp == 0) return h;
+            --sp;
+            ref = stk[sp * stride];
+            if (ref != kSentinel) break;
+            ro = o; rd = d;
+            inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+            oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
+            in_blas = false;
+        }
+    }
+}
+
+// ------------------------------------------------------------- shading ----
+// Interpolated hit point in world space: interpolate_fragment
+// (Source/Core/LowerLevelMeshBVH.cpp:134-163) then the instance transform
+// (Source/Core/UpperLevelBVH.cpp:66-71): position M*p, normal
+// normalize(mat3(M)*n).
+struct Frag { f3 p; f3 n; };
+
+__device__ __forceinline__ Frag make_frag(const DevScene& s, const Hit& h) {
+    const DInstance& I = s.inst[h.inst];
+    const uint32_t t = I.tri_offset + h.prim;
+    const uint32_t i0 = s.idx[3 * t], i1 = s.idx[3 * t + 1], i2 = s.idx[3 * t + 2];
+    const float w = 1.0f - h.v - h.u;
+    const f3 P = ((w * ld3(s.pos + 3 * i0)) + (h.u * ld3(s.pos + 3 * i1))) + (h.v * ld3(s.pos + 3 * i2));
+    const f3 N = normalize(((w * ld3(s.nrm + 3 * i0)) + (h.u * ld3(s.nrm + 3 * i1))) + (h.v * ld3(s.nrm + 3 * i2)));
+    const float* M = s.xf[h.inst].m;
+    Frag f;
+    // (c0*x + c1*y) + (c2*z + c3*1)
+    f.p = mk((M[0] * P.x + M[4] * P.y) + (M[8] * P.z + M[12]), (M[1] * P.x + M[5] * P.y) + (M[9] * P.z + M[13]),
+             (M[2] * P.x + M[6] * P.y) + (M[10] * P.z + M[14]));
+    f.n = normalize(mk(M[0] * N.x + M[4] * N.y + M[8] * N.z, M[1] * N.x + M[5] * N.y + M[9] * N.z,
+                       M[2] * N.x + M[6] * N.y + M[10] * N.z));
+    return f;
+}
+
+// construct_world_to_tangent_transform, Source/Core/vectorUtils.cpp:9-35: rows (T, B, N).
+struct Frame { f3 t, b, n; };
+__device__ __forceinline__ Frame tangent_frame(f3 V, f3 N) {
+    f3 t;
+    if (fabsf(dot(V, N)) > 0.95f) {
+        const f3 vv = fabsf(N.z) > 0.99f ? mk(1, 0, 0) : mk(0, 0, 1);
+        t = normalize(cross(N, vv));
+    } else {
+        t = normalize(cross(V, N));
+    }
+    const f3 b = normalize(cross(t, N));
+    return {t, b, N};
+}
+__device__ __forceinline__ f3 to_tangent(const Frame& f, f3 v) {
+    return {f.t.x * v.x + f.t.y * v.y + f.t.z * v.z, f.b.x * v.x + f.b.y * v.y + f.b.z * v.z,
+            f.n.x * v.x + f.n.y * v.y + f.n.z * v.z};
+}
+// Inverse of the frame matrix (adjugate / determinant), as the restatement does.
+__device__ __forceinline__ f3 to_world(const Frame& f, f3 v) {
+    const f3 c0 = cross(f.b, f.n), c1 = cross(f.n, f.t), c2 = cross(f.t, f.b);
+    const float id = 1.0f / dot(f.t, c0);
+    return {(c0.x * id) * v.x + (c1.x * id) * v.y + (c2.x * id) * v.z,
+            (c0.y * id) * v.x + (c1.y * id) * v.y + (c2.y * id) * v.z,
+            (c0.z * id) * v.x + (c1.z * id) * v.y + (c2.z * id) * v.z};
+}
+
+__device__ __forceinline__ float cos2t(f3 w) { return w.z * w.z; }
+__device__ __forceinline__ float sin2t(f3 w) { return smax(0.0f, 1.0f - cos2t(w)); }
+__device__ __forceinline__ float sint(f3 w) { return sqrtf(sin2t(w)); }
+__device__ __forceinline__ float tan2t(f3 w) { return sin2t(w) / cos2t(w); }
+__device__ __forceinline__ float clamp1(float x) { return fminf(fmaxf(x, -1.0f), 1.0f); }
+__device__ __forceinline__ float cosp(f3 w) { const float s = sint(w); return (s == 0) ? 1.0f : clamp1(w.x / s); }
+__device__ __forceinline__ float sinp(f3 w) { const float s = sint(w); return (s == 0) ? 0.0f : clamp1(w.y / s); }
+
+// Cos_Weighted_Hemisphere_Distribution, Source/Render/Distributions.cpp:14-29.
+__device__ __forceinline__ f3 cos_sample(float2 xi) {
+    const float r = sqrtf(xi.x);
+    const float phi = (float)((double)xi.y * PICO_PI_D * 2.0);
+    return normalize(mk(r * cosf(phi), r * sinf(phi), sqrtf(smax(0.0f, 1.0f - xi.x))));
+}
+__device__ __forceinline__ float cos_pdf(f3 h) { return smax(0.0f, h.z / (float)PICO_PI_D); }
+
+// Beckmann_All_Microfacet_Distribution, Source/Render/Distributions.cpp:31-78.
+__device__ __forceinline__ f3 beckmann_sample(float2 xi, f3 V, float R) {
+    const float alpha = R * R;
+    float ls = logf(1.0f - xi.x);
+    if (isinf(ls)) ls = 0.0f;
+    const float tan2 = -alpha * alpha * ls;
+    const float phi = (float)((double)(xi.y * 2.0f) * PICO_PI_D);
+    const float ct = 1.0f / sqrtf(1.0f + tan2);
+    const float st = sqrtf(smax(0.0f, 1.0f - ct * ct));
+    f3 H = mk(st * cosf(phi), st * sinf(phi), ct);
+    if (!(V.z * H.z > 0.0f)) H = -H;
+    return normalize(H);
+}
+__device__ __forceinline__ float beckmann_pdf(f3 wh, float R) {
+    const float r = smax(R, 1e-3f);
+    const float x = logf(r);
+    const float a = 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+    const float tan2 = tan2t(wh);
+    float d;
+    if (isinf(tan2)) {
+        d = 0.0f;
+    } else {
+        const float cos4 = cos2t(wh) * cos2t(wh);
+        const float c = cosp(wh), sn = sinp(wh);
+        const float num = expf(-tan2 * ((c * c) / (a * a) + (sn * sn) / (a * a)));
+        d = (float)((double)num / (PICO_PI_D * a * a * cos4));
+    }
+    d = fminf(fmaxf(d, 0.0f), 1.0f);
+    return d * fabsf(wh.z);
+}
+
+// GGX (Walter et al. 2007) with alpha = R^2: D = a^2 / (pi cos^4 (a^2 + tan^2)^2),
+// sampled by tan^2 = a^2 u / (1 - u); pdf_h = D |cos|.
+__device__ __forceinline__ f3 ggx_sample(float2 xi, f3 V, float R) {
+    const float a = R * R;
+    const float tan2 = a * a * xi.x / (1.0f - xi.x);
+    const float phi = (float)((double)(xi.y * 2.0f) * PICO_PI_D);
+    const float ct = 1.0f / sqrtf(1.0f + tan2);
+    const float st = sqrtf(smax(0.0f, 1.0f - ct * ct));
+    f3 H = mk(st * cosf(phi), st * sinf(phi), ct);
+    if (!(V.z * H.z > 0.0f)) H = -H;
+    return normalize(H);
+}
+__device__ __forceinline__ float ggx_pdf(f3 wh, float R) {
+    const float a = R * R;
+    const float a2 = a * a;
+    const float c2 = cos2t(wh);
+    const float tan2 = tan2t(wh);
+    if (isinf(tan2)) return 0.0f;
+    const float den = a2 + tan2;
+    const float D = a2 / ((float)PICO_PI_D * (c2 * c2) * (den * den));
+    return D * fabsf(wh.z);
+}
+
+__device__ __forceinline__ f3 spec_sample(uint32_t mf, float2 xi, f3 V, float R) {
+    return mf == 1u ? ggx_sample(xi, V, R) : beckmann_sample(xi, V, R);
+}
+__device__ __forceinline__ float spec_pdf(uint32_t mf, f3 H, float R) {
+    return mf == 1u ? ggx_pdf(H, R) : beckmann_pdf(H, R);
+}
+
+struct BSample { f3 L; float P; f3 energy; };
+
+// BSRDF::sample for the Diffuse, Specular, Dielectric, Light and Delta BSDFs,
+// Source/Render/BSRDF.cpp:11-238.
+__device__ __forceinline__ BSample bsdf_sample(uint32_t kind, uint32_t mf, const DMaterial& m, f3 N, f3 raydir, Rng& rng) {
+    const f3 V = -raydir;
+    BSample s;
+    if (kind == PICO_BSDF_DELTA) {
+        s.energy = ld3(m.specular);
+        s.L = reflect(-V, N);
+        s.P = 1.0f;
+        return s;
+    }
+    const Frame f = tangent_frame(V, N);
+    const f3 vt = normalize(to_tangent(f, V));
+    if (kind == PICO_BSDF_DIFFUSE || kind == PICO_BSDF_LIGHT) {
+        const float2 xi = rng.next2d();
+        const f3 H = cos_sample(xi);
+        s.P = cos_pdf(H);
+        s.L = normalize(to_world(f, H));
+        s.energy = kind == PICO_BSDF_LIGHT ? ld3(m.emissive) : ld3(m.diffuse);
+        return s;
+    }
+    if (kind == PICO_BSDF_SPECULAR) {
+        const float2 xi = rng.next2d();
+        const f3 H = spec_sample(mf, xi, vt, m.roughness);
+        const f3 L = normalize(reflect(-vt, H));
+        s.L = normalize(to_world(f, L));
+        s.P = spec_pdf(mf, H, m.roughness) / (2.0f * dot(vt, H));
+        s.energy = ld3(m.specular);
+        return s;
+    }
+    float2 xi = rng.next2d();
+    if (xi.y >= m.reflectance) {
+        const float2 x2 = rng.next2d();
+        const f3 H = cos_sample(x2);
+        s.L = normalize(to_world(f, H));
+        s.P = cos_pdf(H);
+        s.energy = ld3(m.diffuse);
+    } else {
+        xi = rng.next2d();
+        const f3 H = spec_sample(mf, xi, vt, m.roughness);
+        const f3 L = normalize(reflect(-vt, H));
+        s.L = normalize(to_world(f, L));
+        s.P = spec_pdf(mf, H, m.roughness) / (2.0f * dot(vt, H));
+        s.energy = ld3(m.specular);
+    }
+    return s;
+}
+
+// BSRDF::pdf (Source/Render/BSRDF.cpp:42-45,87-91,151-155,229-232) at the
+// half vector, as direct_lighting_pdf calls it (Source/Render/Integrators.cpp:13-22).
+__device__ __forceinline__ float direct_pdf(uint32_t kind, uint32_t mf, const DMaterial& m, f3 N, f3 wi, f3 wo) {
+    const Frame f = tangent_frame(wi, N);
+    const f3 two = to_tangent(f, wo);
+    f3 H = normalize(wi + wo);
+    H = to_tangent(f, H);
+    switch (kind) {
+        case PICO_BSDF_DIFFUSE:
+        case PICO_BSDF_LIGHT: return cos_pdf(H);
+        case PICO_BSDF_SPECULAR: return spec_pdf(mf, H, m.roughness) / (2.0f * dot(two, H));
+        case PICO_BSDF_DIELECTRIC: {
+            const float r = m.reflectance;
+            return (cos_pdf(H) * (1 - r)) + ((spec_pdf(mf, H, m.roughness) / (2.0f * dot(two, H))) * r);
+        }
+        default: return 0.0f;
+    }
+}
+
+// Camera::generate_ray, Source/Core/Camera.cpp:11-28 (right = cross(normalize(D), up)).
+struct DCam { f3 pos, dir, up, right; float aspect, far_plane; uint32_t w, h; };
+__device__ __forceinline__ f3 camera_dir(const DCam& c, float2 rnd, uint32_t px, uint32_t py) {
+    const float rx = rnd.x / 2.0f, ry = rnd.y / 2.0f;
+    const f3 d = mk((((float)px + rx) / (float)c.w - 0.5f) * c.aspect, ((float)py + ry) / (float)c.h - 0.5f, 1.0f);
+    return normalize(((d.z * c.dir) + (d.y * c.up)) + (d.x * c.right));
+}
+
+}  // namespace pico
