@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the rendering hot path on one MI355X.
+
+One step = one frame of the instanced scene (1024 instances of 16 seeded
+meshes with 2k-20k triangles, ~8.3M instanced triangles, two-level BVH,
+3 quad lights + constant sky) at 1920x1080, 64 spp, max depth 8, rendered by
+the wavefront path tracer into device-resident accumulation buffers (the
+reference's running mean / variance with its early-out).  The scene is built
+and uploaded before timing.
+
+Printed: one JSON line with Mrays/s (rays of the accumulated samples / wall
+time of the timed steps), the roofline of the traversal kernel (`extend`) and
+the reference CPU renderer's Mrays/s on the host cores.
+
+Ray = one closest-hit scene query (camera, extension or shadow ray).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
+METRIC = "Mrays/s at 1080p; relMSE vs FP64 oracle at equal ms/frame (CNN vs à-trous)"
+
+# algorithmic bytes per traversal unit: 48 B ray I/O (32 B ray in, 16 B hit out),
+# 64 B per child-pair node, 48 B per triangle record, 64 B per instance record
+B_RAY, B_NODE, B_TRI, B_INST = 48, 64, 48, 64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traversal_bytes(st, shadow=False):
+    if shadow:
+        rays = st.shadow_rays
+        return rays * B_RAY + st.shadow_node_visits * B_NODE + st.shadow_tri_tests * B_TRI + \
+            st.shadow_instance_visits * B_INST
+    rays = st.camera_rays + st.extension_rays
+    return rays * B_RAY + st.node_visits * B_NODE + st.tri_tests * B_TRI + st.instance_visits * B_INST
+
+
+def cpu_baseline(scene, cam, w, h, spp, depth, seed):
+    """Pico's own CPU renderer (compiled reference subset) on the host cores,
+    on a bounded sample of the same workload; rays counted by the oracle on the
+    identical paths (same generators and seeds)."""
+    import oracle
+    import pico_amd as pa
+    threads = min(16, os.cpu_count() or 1)
+    p = pa.params(w, h, spp, depth, tile=64, frame_seed=seed)
+    cs = oracle.CpuScene(scene)
+    _, _, _, st = cs.render(cam, p, rng=oracle.RNG_REFERENCE, threads=threads)
+    rays = st.camera_rays + st.extension_rays + st.shadow_rays
+    if oracle.ref_available():
+        rs = oracle.RefScene(scene)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            rs.render(cam, p, threads=threads)
+            times.append(time.perf_counter() - t0)
+        kind = "reference"
+    else:
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            cs.render(cam, p, rng=oracle.RNG_REFERENCE, threads=threads)
+            times.append(time.perf_counter() - t0)
+        kind = "port"
+    t = sorted(times)[1]
+    return {"value": round(rays / t / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": kind,
+            "sample": f"same scene and camera at {w}x{h}, {spp} spp, depth {depth}, tiles 64x64; "
+                      f"{rays} rays, median of 3 runs {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline bytes)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard by "
+              "image bands or views, see DESIGN.md)", file=sys.stderr)
+        sys.exit(1)
+
+    import pico_amd as pa
+    ctx = pa.Context(0)
+    log(f"[bench] device {ctx.device_name}")
+    t0 = time.time()
+    scene = pa.Scene.instanced(2, 16, 1024, 2000, 20000, 100.0, pa.MICROFACET_GGX)
+    uniq, inst_tris = scene.n_triangles
+    ctx.upload_scene(scene)
+    info = ctx.scene_info()
+    log(f"[bench] scene: {uniq} unique / {inst_tris} instanced triangles, upload+build {time.time() - t0:.1f}s, {info}")
+    W, H = args.width, args.height
+    cam = scene.camera(W / H)
+    n = W * H
+    d_pix, d_cnt, d_var = ctx.alloc(12 * n), ctx.alloc(4 * n), ctx.alloc(12 * n)
+    p = pa.params(W, H, args.spp, args.depth, tile=16, frame_seed=1)
+
+    count = None
+    if not args.no_count:
+        pc = pa.params(W, H, args.spp, args.depth, tile=16, frame_seed=1, flags=pa.RENDER_COUNT)
+        count = ctx.render_d(cam, pc, d_pix, d_cnt, d_var)
+        log(f"[bench] counting run: {count.camera_rays + count.extension_rays} extend rays, "
+            f"{count.shadow_rays} shadow rays, {count.node_visits} node visits, {count.tri_tests} tri tests")
+    for i in range(args.warmup):
+        st = ctx.render_d(cam, p, d_pix, d_cnt, d_var)
+        log(f"[bench] warmup {i}: {st.render_ms:.1f} ms, {st.accumulated_rays / st.render_ms / 1e3:.1f} Mrays/s")
+    ctx.sync()
+    ctx.enable_kernel_timing(True)
+    rays = 0
+    traced = 0
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = ctx.render_d(cam, p, d_pix, d_cnt, d_var)
+        rays += st.accumulated_rays
+        traced += st.rays
+        log(f"[bench] step {i}: {st.render_ms:.1f} ms device")
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ext_n, ext_ms = ctx.kernel_time("extend")
+    sh_n, sh_ms = ctx.kernel_time("shadow")
+    ctx.enable_kernel_timing(False)
+    ms_step = dt * 1e3 / args.steps
+    mrays = rays / dt / 1e6
+
+    roof = None
+    if count is not None and ext_n:
+        launches_per_frame = ext_n / args.steps
+        bytes_per_launch = traversal_bytes(count) / launches_per_frame
+        achieved = bytes_per_launch / (ext_ms * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic_extend_instanced_1080p.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "kernel": "extend", "launches": ext_n, "mean_launch_ms": round(ext_ms, 4),
+                "algorithmic_bytes_per_launch": round(bytes_per_launch),
+                "bytes_per_ray": round(traversal_bytes(count) / max(1, count.camera_rays + count.extension_rays), 1),
+                "shadow_kernel_mean_launch_ms": round(sh_ms, 4),
+                "shadow_achieved_GBps": round(traversal_bytes(count, True) / (sh_n / args.steps) / (sh_ms * 1e-3) / 1e9, 1)
+                if sh_n else None}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        cpu = cpu_baseline(scene, cam, 960, 540, 2, args.depth, 1)
+        log(f"[bench] CPU baseline {cpu}")
+
+    out = {"metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
+           "config": {"workload": f"instanced scene, 1024 instances of 16 meshes (2k-20k tris), {inst_tris} instanced "
+                                  f"triangles, {W}x{H}, {args.spp} spp, depth {args.depth}, GGX, wavefront path tracer",
+                      "width": W, "height": H, "spp": args.spp, "max_depth": args.depth, "scene_seed": 2,
+                      "unique_triangles": uniq, "instanced_triangles": inst_tris,
+                      "rays_per_step": rays // args.steps, "traced_rays_per_step": traced // args.steps},
+           "roofline": roof, "cpu_baseline": cpu}
+    if cpu:
+        out["gpu_over_cpu"] = round(mrays / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

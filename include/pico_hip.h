@@ -130,9 +130,12 @@ typedef struct pico_stats {
     uint64_t extension_rays;
     uint64_t shadow_rays;
     uint64_t gbuffer_rays;
-    uint64_t node_visits;    /* PICO_RENDER_COUNT only */
+    uint64_t node_visits;    /* PICO_RENDER_COUNT only: extension + camera rays */
     uint64_t tri_tests;      /* PICO_RENDER_COUNT only */
     uint64_t instance_visits;/* PICO_RENDER_COUNT only */
+    uint64_t shadow_node_visits;     /* PICO_RENDER_COUNT only: shadow rays */
+    uint64_t shadow_tri_tests;       /* PICO_RENDER_COUNT only */
+    uint64_t shadow_instance_visits; /* PICO_RENDER_COUNT only */
     uint64_t samples;        /* (pixel, sample) pairs accumulated */
     uint64_t accumulated_rays; /* rays of the accumulated samples only */
     double   render_ms;      /* device time of the render, HIP events */
@@ -162,6 +165,11 @@ int  pico_hip_memset_d(pico_hip_ctx* ctx, void* dst, int value, size_t bytes);
  * the last reset. */
 int  pico_hip_enable_kernel_timing(pico_hip_ctx* ctx, int enable);
 int  pico_hip_kernel_time(pico_hip_ctx* ctx, const char* kernel, uint64_t* launches, double* mean_ms);
+
+/* Measurement aid with no counterpart in the reference: a streaming copy of
+ * `bytes` (multiple of 16) with 16-byte loads and stores (kernel "copy"), used
+ * to measure achievable HBM bandwidth and to calibrate byte counters. */
+int  pico_hip_copy_d(pico_hip_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 
 /* ---- scene ------------------------------------------------------------- */
 /* Builds the BLAS (one per mesh, 16-bin binned SAH) and the TLAS over

@@ -69,7 +69,9 @@ class RenderParams(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("gbuffer_rays", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
-                ("instance_visits", C.c_uint64), ("samples", C.c_uint64), ("accumulated_rays", C.c_uint64),
+                ("instance_visits", C.c_uint64), ("shadow_node_visits", C.c_uint64),
+                ("shadow_tri_tests", C.c_uint64), ("shadow_instance_visits", C.c_uint64),
+                ("samples", C.c_uint64), ("accumulated_rays", C.c_uint64),
                 ("render_ms", C.c_double), ("cancelled", C.c_int32), ("pad", C.c_int32)]
 
     @property
@@ -105,6 +107,7 @@ def _declare(lib):
         "pico_hip_memset_d": (i32, [vp, vp, i32, sz]),
         "pico_hip_enable_kernel_timing": (i32, [vp, i32]),
         "pico_hip_kernel_time": (i32, [vp, C.c_char_p, P(u64), P(f64)]),
+        "pico_hip_copy_d": (i32, [vp, vp, vp, sz]),
         "pico_hip_upload_scene": (i32, [vp, P(SceneDesc)]),
         "pico_hip_scene_info": (i32, [vp, P(u64), P(u64), P(u64), P(u32), P(u32)]),
         "pico_hip_closest_hit": (i32, [vp, u32, vp, vp, vp, vp, vp, vp]),
@@ -404,6 +407,9 @@ class Context:
 
     def enable_kernel_timing(self, on=True):
         self._check(lib().pico_hip_enable_kernel_timing(self._h, 1 if on else 0))
+
+    def copy_d(self, dst, src, nbytes):
+        self._check(lib().pico_hip_copy_d(self._h, dst.ptr, src.ptr, nbytes))
 
     def kernel_time(self, name):
         n, ms = C.c_uint64(), C.c_double()

@@ -141,9 +141,22 @@ __global__ void k_relmse(const float* __restrict__ img, const double* __restrict
 
 __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
+__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+int pico_hip_copy_d(pico_hip_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return PICO_ERR_ARG;
+    if (!dst || !src || (bytes & 15)) return fail(c, PICO_ERR_ARG, "copy needs non-NULL buffers and a multiple of 16 bytes");
+    const size_t n = bytes / 16;
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)c->num_cus * 32);
+    if (n) PICO_LAUNCH(c, "copy", k_copy, grid, 256, 0, (float4*)dst, (const float4*)src, n);
+    return PICO_OK;
+}
 
 int pico_hip_atrous_d(pico_hip_ctx* c, const float* rgb, const float* nrm, const float* pos, const float* alb, uint32_t w,
                       uint32_t h, uint32_t tw, uint32_t th, uint32_t levels, float* scratch_buf, float* out) {

@@ -67,7 +67,8 @@ struct Ctl {
     uint32_t sh_count[kMaxBounce + 2];
     uint32_t sh_head[kMaxBounce + 2];
     unsigned long long rays_cam, rays_ext, rays_sh, rays_acc, samples_acc;
-    unsigned long long nodes, tris, insts;
+    unsigned long long nodes, tris, insts;           // extend (camera + extension rays)
+    unsigned long long nodes_sh, tris_sh, insts_sh;  // shadow rays
 };
 
 struct Pool {
@@ -338,9 +339,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_
         }
     }
     if (COUNT) {
-        atomicAdd(&ctl->nodes, (unsigned long long)tc.nodes);
-        atomicAdd(&ctl->tris, (unsigned long long)tc.tris);
-        atomicAdd(&ctl->insts, (unsigned long long)tc.insts);
+        atomicAdd(&ctl->nodes_sh, (unsigned long long)tc.nodes);
+        atomicAdd(&ctl->tris_sh, (unsigned long long)tc.tris);
+        atomicAdd(&ctl->insts_sh, (unsigned long long)tc.insts);
     }
 }
 
@@ -589,6 +590,9 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
                 st.node_visits = hc.nodes;
                 st.tri_tests = hc.tris;
                 st.instance_visits = hc.insts;
+                st.shadow_node_visits = hc.nodes_sh;
+                st.shadow_tri_tests = hc.tris_sh;
+                st.shadow_instance_visits = hc.insts_sh;
                 for (uint32_t k = 0; k < nv; ++k) {
                     st.accumulated_rays += hs[4 * k];
                     st.samples += hs[4 * k + 1];

@@ -279,9 +279,9 @@ static void cornell_into(pico_scene* s, Rng& rng, bool jitter) {
     // two boxes rotated +-18 degrees about y
     const float jx = jitter ? rng.uniform(-0.15f, 0.15f) : 0.0f, jz = jitter ? rng.uniform(-0.15f, 0.15f) : 0.0f;
     const float r1 = jitter ? rng.uniform(-40.0f, 40.0f) : 18.0f, r2 = jitter ? rng.uniform(-40.0f, 40.0f) : -18.0f;
-    s->add_instance(box, mul(translate({-0.35f + jx, -0.7f, 0.3f + jz}), mul(rot_axis({0, 1, 0}, r1), scale({0.6f, 0.6f, 0.6f}))),
+    s->add_instance(box, mul(translate({-0.35f + jx, -0.698f, 0.3f + jz}), mul(rot_axis({0, 1, 0}, r1), scale({0.6f, 0.6f, 0.6f}))),
                     rand_mat(), PICO_BSDF_DIELECTRIC);
-    s->add_instance(box, mul(translate({0.35f - jx, -0.4f, -0.2f - jz}), mul(rot_axis({0, 1, 0}, r2), scale({0.6f, 1.2f, 0.6f}))),
+    s->add_instance(box, mul(translate({0.35f - jx, -0.398f, -0.2f - jz}), mul(rot_axis({0, 1, 0}, r2), scale({0.6f, 1.2f, 0.6f}))),
                     rand_mat(), PICO_BSDF_DIELECTRIC);
     // quad light 0.5 x 0.5 at y = 0.99 facing down
     const float le = jitter ? rng.uniform(5.0f, 20.0f) : 10.0f;
@@ -329,7 +329,8 @@ static void instanced_into(pico_scene* s, Rng& rng, uint32_t n_types, uint32_t n
     }
     s->sky[0] = s->sky[1] = s->sky[2] = sky;
     const float c = 0.5f * extent;
-    s->cam = pico_camera{{c, c, -1.2f * extent}, {0.0f, 0.0f, 1.0f}, {0.0f, 1.0f, 0.0f}, 1.0f, 10000.0f};
+    // outside the box, looking at its centre; the view is filled by the box
+    s->cam = pico_camera{{c, c, -0.5f * extent}, {0.0f, 0.0f, 1.0f}, {0.0f, 1.0f, 0.0f}, 1.0f, 10000.0f};
 }
 
 extern "C" {
