@@ -113,16 +113,31 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t* head) {
     return __shfl(base, 0);
 }
 
-__global__ void k_reset_chunk(Ctl* ctl, uint32_t first) {
+// Sum over the wave, added once per wave (all 64 lanes must call it).
+__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0 && v) atomicAdd(dst, v);
+}
+
+constexpr uint32_t kDeadPath = 0xffffffffu;
+
+__global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
     const uint32_t t = threadIdx.x;
-    if (t == 0) {
-        ctl->n_active = first ? ctl->n_active : ctl->n_next;
-        ctl->n_next = 0;
-        ctl->gen_head = 0;
-    }
+    __syncthreads();
+    const uint32_t na = first ? ctl->n_active : ctl->n_next;
+    __syncthreads();
     for (uint32_t b = t; b < kMaxBounce + 2; b += blockDim.x) {
         ctl->ext_count[b] = 0; ctl->ext_head[b] = 0; ctl->shade_head[b] = 0;
         ctl->sh_count[b] = 0; ctl->sh_head[b] = 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        ctl->n_active = na;
+        ctl->n_next = 0;
+        ctl->gen_head = 0;
+        // bounce 0 uses identity slots: path i sits at queue entry i (dead
+        // paths of views with fewer samples are marked kDeadPath)
+        ctl->ext_count[0] = na * S;
     }
 }
 
@@ -157,12 +172,9 @@ __global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ v
                                                      Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
     const uint32_t na = ctl->n_active;
     const uint32_t total = na * S;
-    for (;;) {
-        const uint32_t base = wave_claim(&ctl->gen_head);
-        if (base >= total) break;
-        const uint32_t i = base + lane_id();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
         bool live = false;
-        if (i < total) {
+        {
             const uint32_t j = i / na, k = i - j * na;
             const uint32_t key = active[k];
             const DView& V = views[key >> kViewShift];
@@ -183,8 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ v
                 P.key[i] = key;
             }
         }
-        const uint32_t slot = wave_append(&ctl->ext_count[0], live);
-        if (live) P.q0[slot] = i;
+        P.q0[i] = live ? i : kDeadPath;
     }
 }
 
@@ -198,8 +209,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene s, Ctl* ctl, uint32_
         const uint32_t base = wave_claim(&ctl->ext_head[b]);
         if (base >= n) break;
         const uint32_t e = base + lane_id();
-        if (e < n) {
-            const uint32_t i = q[e];
+        const uint32_t i = e < n ? q[e] : kDeadPath;
+        if (i != kDeadPath) {
             const float4 o = P.ray_o[i], d = P.ray_d[i];
             const Hit h = trace_closest<STACK, COUNT>(s, xyz(o), xyz(d), stk + threadIdx.x, kBlock, &tc, &ctl->overflow);
             P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.inst));
@@ -222,9 +233,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
         if (base >= n) break;
         const uint32_t e = base + lane_id();
         bool cont = false, shadow = false;
-        uint32_t i = 0;
-        if (e < n) {
-            i = q[e];
+        const uint32_t i = e < n ? q[e] : kDeadPath;
+        if (i != kDeadPath) {
             const float4 h4 = P.hit[i];
             const uint32_t inst = __float_as_uint(h4.w);
             const uint32_t key = P.key[i];
@@ -345,28 +355,59 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_
     }
 }
 
+// Adds (rays, samples) to the view's counters: one atomic pair per wave when
+// every lane of the wave is on the same view, per lane otherwise.
+__device__ __forceinline__ void add_view_stats(const DView* views, uint32_t view, unsigned long long rays,
+                                               unsigned long long ns) {
+    const uint32_t v0 = __shfl(view, 0);
+    const bool uniform = __ballot(view != v0 && view != 0xffffffffu) == 0ull;
+    if (uniform) {
+        if (v0 == 0xffffffffu) {
+            // lane 0 idle: find any active view
+            const uint64_t act = __ballot(view != 0xffffffffu);
+            if (!act) return;
+            const uint32_t vv = __shfl(view, (int)(__ffsll((unsigned long long)act) - 1));
+            for (int o = 32; o > 0; o >>= 1) { rays += __shfl_xor(rays, o); ns += __shfl_xor(ns, o); }
+            if (lane_id() == 0) { atomicAdd(&views[vv].stats[0], rays); atomicAdd(&views[vv].stats[1], ns); }
+            return;
+        }
+        for (int o = 32; o > 0; o >>= 1) { rays += __shfl_xor(rays, o); ns += __shfl_xor(ns, o); }
+        if (lane_id() == 0) { atomicAdd(&views[v0].stats[0], rays); atomicAdd(&views[v0].stats[1], ns); }
+    } else if (view != 0xffffffffu) {
+        atomicAdd(&views[view].stats[0], rays);
+        atomicAdd(&views[view].stats[1], ns);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_finalize(const DView* __restrict__ views, const uint32_t* __restrict__ active,
                                                      const Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
     const uint32_t na = ctl->n_active;
     const uint32_t total = na * S;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        const uint32_t j = i / na, k = i - j * na;
-        const uint32_t key = active[k];
-        const DView& V = views[key >> kViewShift];
-        const uint32_t rel = off + j;
-        if (rel >= V.spp) continue;
-        const float4 r = P.rad[i];
-        f3 res = xyz(r);
-        if (P.ray_o[i].w == 0.0f && bad(res)) res = mk(1.0f, 0.4f, 0.7f);  // Integrators.cpp:145-146
-        if (V.mode == MODE_SAMPLES) {
-            const uint32_t p = key & kPixMask;
-            float* o = V.samples_out + ((size_t)rel * V.cam.w * V.cam.h + p) * 3;
-            o[0] = res.x; o[1] = res.y; o[2] = res.z;
-            atomicAdd(&V.stats[0], (unsigned long long)r.w);
-            atomicAdd(&V.stats[1], 1ull);
-        } else {
-            P.samp[i] = make_float4(res.x, res.y, res.z, r.w);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t vstat = 0xffffffffu;
+        unsigned long long rays = 0;
+        if (i < total) {
+            const uint32_t j = i / na, k = i - j * na;
+            const uint32_t key = active[k];
+            const DView& V = views[key >> kViewShift];
+            const uint32_t rel = off + j;
+            if (rel < V.spp) {
+                const float4 r = P.rad[i];
+                f3 res = xyz(r);
+                if (P.ray_o[i].w == 0.0f && bad(res)) res = mk(1.0f, 0.4f, 0.7f);  // Integrators.cpp:145-146
+                if (V.mode == MODE_SAMPLES) {
+                    const uint32_t p = key & kPixMask;
+                    float* o = V.samples_out + ((size_t)rel * V.cam.w * V.cam.h + p) * 3;
+                    o[0] = res.x; o[1] = res.y; o[2] = res.z;
+                    vstat = key >> kViewShift;
+                    rays = (unsigned long long)r.w;
+                } else {
+                    P.samp[i] = make_float4(res.x, res.y, res.z, r.w);
+                }
+            }
         }
+        add_view_stats(views, vstat, rays, vstat != 0xffffffffu ? 1ull : 0ull);
     }
 }
 
@@ -375,56 +416,60 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DView* __restrict__ v
 __global__ __launch_bounds__(kBlock) void k_accumulate(const DView* __restrict__ views, const uint32_t* __restrict__ active,
                                                        Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
     const uint32_t na = ctl->n_active;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na; k += gridDim.x * blockDim.x) {
-        const uint32_t key = active[k];
-        const DView& V = views[key >> kViewShift];
-        const uint32_t p = key & kPixMask;
-        if (V.mode == MODE_SAMPLES) {
-            if (off + S >= V.spp) V.done[p] = 1;
-            continue;
-        }
+    for (uint32_t base = blockIdx.x * blockDim.x; base < na; base += gridDim.x * blockDim.x) {
+        const uint32_t k = base + threadIdx.x;
+        uint32_t vstat = 0xffffffffu;
         unsigned long long rays = 0, ns = 0;
-        bool stop = false;
-        if (V.mode == MODE_SUM64) {
-            double sx = V.sum64[3 * p], sy = V.sum64[3 * p + 1], sz = V.sum64[3 * p + 2];
-            for (uint32_t j = 0; j < S && off + j < V.spp; ++j) {
-                const float4 x = P.samp[(size_t)j * na + k];
-                sx += (double)x.x; sy += (double)x.y; sz += (double)x.z;
-                rays += (unsigned long long)x.w;
-                ++ns;
-            }
-            V.sum64[3 * p] = sx; V.sum64[3 * p + 1] = sy; V.sum64[3 * p + 2] = sz;
-        } else {
-            uint32_t cnt = V.counts[p];
-            f3 pix = ld3(V.pixels + 3 * p), var = ld3(V.variance + 3 * p);
-            for (uint32_t j = 0; j < S && off + j < V.spp; ++j) {
-                const float4 x4 = P.samp[(size_t)j * na + k];
-                f3 x = xyz(x4);
-                const uint32_t prev = cnt;
-                if (prev < 1) {
-                    cnt = 1;
-                    var = mk(0.0f, 0.0f, 0.0f);
+        if (k < na) {
+            const uint32_t key = active[k];
+            const DView& V = views[key >> kViewShift];
+            const uint32_t p = key & kPixMask;
+            if (V.mode == MODE_SAMPLES) {
+                if (off + S >= V.spp) V.done[p] = 1;
+            } else {
+                vstat = key >> kViewShift;
+                bool stop = false;
+                if (V.mode == MODE_SUM64) {
+                    double sx = V.sum64[3 * p], sy = V.sum64[3 * p + 1], sz = V.sum64[3 * p + 2];
+                    for (uint32_t j = 0; j < S && off + j < V.spp; ++j) {
+                        const float4 x = P.samp[(size_t)j * na + k];
+                        sx += (double)x.x; sy += (double)x.y; sz += (double)x.z;
+                        rays += (unsigned long long)x.w;
+                        ++ns;
+                    }
+                    V.sum64[3 * p] = sx; V.sum64[3 * p + 1] = sy; V.sum64[3 * p + 2] = sz;
                 } else {
-                    const f3 np = pix + ((x - pix) * (1.0f / (float)prev));
-                    var = ((var * (float)prev) + ((x - pix) * (x - np))) / (float)(prev + 1);
-                    cnt = prev + 1;
-                    x = np;
+                    uint32_t cnt = V.counts[p];
+                    f3 pix = ld3(V.pixels + 3 * p), var = ld3(V.variance + 3 * p);
+                    for (uint32_t j = 0; j < S && off + j < V.spp; ++j) {
+                        const float4 x4 = P.samp[(size_t)j * na + k];
+                        f3 x = xyz(x4);
+                        const uint32_t prev = cnt;
+                        if (prev < 1) {
+                            cnt = 1;
+                            var = mk(0.0f, 0.0f, 0.0f);
+                        } else {
+                            const f3 np = pix + ((x - pix) * (1.0f / (float)prev));
+                            var = ((var * (float)prev) + ((x - pix) * (x - np))) / (float)(prev + 1);
+                            cnt = prev + 1;
+                            x = np;
+                        }
+                        pix = x;
+                        rays += (unsigned long long)x4.w;
+                        ++ns;
+                        if (prev > 4 && var.x <= V.max_variance && var.y <= V.max_variance && var.z <= V.max_variance) {
+                            stop = true;
+                            break;
+                        }
+                    }
+                    V.counts[p] = cnt;
+                    V.pixels[3 * p] = pix.x; V.pixels[3 * p + 1] = pix.y; V.pixels[3 * p + 2] = pix.z;
+                    V.variance[3 * p] = var.x; V.variance[3 * p + 1] = var.y; V.variance[3 * p + 2] = var.z;
                 }
-                pix = x;
-                rays += (unsigned long long)x4.w;
-                ++ns;
-                if (prev > 4 && var.x <= V.max_variance && var.y <= V.max_variance && var.z <= V.max_variance) {
-                    stop = true;
-                    break;
-                }
+                if (stop || off + S >= V.spp) V.done[p] = 1;
             }
-            V.counts[p] = cnt;
-            V.pixels[3 * p] = pix.x; V.pixels[3 * p + 1] = pix.y; V.pixels[3 * p + 2] = pix.z;
-            V.variance[3 * p] = var.x; V.variance[3 * p + 1] = var.y; V.variance[3 * p + 2] = var.z;
         }
-        if (stop || off + S >= V.spp) V.done[p] = 1;
-        atomicAdd(&V.stats[0], rays);
-        atomicAdd(&V.stats[1], ns);
+        add_view_stats(views, vstat, rays, ns);
     }
 }
 
@@ -533,8 +578,13 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
         const uint32_t g = std::min<uint32_t>((n + 255) / 256, 4096);
         PICO_LAUNCH(c, "init_view", k_init_view, g, 256, 0, dv, v, act_a, ctl);
     }
-    const uint32_t grid_trace = (uint32_t)c->num_cus * 8;
-    const uint32_t grid_shade = (uint32_t)c->num_cus * 8;
+    // persistent grids: as many workgroups as stay resident at once
+    int occ_t = 0, occ_s = 0;
+    if (c->stack_need <= 32) hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, k_extend<32, false>, kBlock, 0);
+    else hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, k_extend<64, false>, kBlock, 0);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_shade, kBlock, 0);
+    const uint32_t grid_trace = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_t, 16));
+    const uint32_t grid_shade = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_s, 16));
     const uint32_t grid_lin = (uint32_t)c->num_cus * 16;
     hipEvent_t e0, e1;
     PICO_HIP(c, hipEventCreate(&e0));
@@ -548,7 +598,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
         uint32_t S = s_max;
         if (off == 0 && accum_mode) S = std::min<uint32_t>(S, 6);
         S = std::min(S, pl.max_spp - off);
-        PICO_LAUNCH(c, "reset_chunk", k_reset_chunk, 1, 256, 0, ctl, first ? 1u : 0u);
+        PICO_LAUNCH(c, "reset_chunk", k_reset_chunk, 1, 256, 0, ctl, first ? 1u : 0u, S);
         first = false;
         PICO_LAUNCH(c, "generate", k_generate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         for (uint32_t b = 0; b <= pl.max_depth; ++b) {

@@ -13,6 +13,11 @@ def cornell():
     return pa.Scene.cornell(1, pa.MICROFACET_GGX)
 
 
+@pytest.fixture(scope="module")
+def instanced():
+    return pa.Scene.instanced(2, 16, 1024, 2000, 20000, 100.0, pa.MICROFACET_GGX)
+
+
 def _random_rays(n, seed, lo, hi):
     rng = np.random.default_rng(seed)
     o = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
@@ -32,12 +37,38 @@ def test_closest_hit_cornell(ctx, cornell):
     np.testing.assert_allclose(t[m], ct[m], rtol=1e-4, atol=1e-5)
 
 
-def test_primary_ids_cornell(ctx, cornell):
+def test_closest_hit_instanced_full_scene(ctx, instanced):
+    ctx.upload_scene(instanced)
+    o, d = _random_rays(50000, 2, 0.0, 100.0)
+    t, inst, prim, uv = ctx.closest_hit(o, d)
+    ct, ci, cp, *_ = oracle.CpuScene(instanced).closest_hit(o, d)
+    assert (ci != 0xffffffff).mean() > 0.3
+    agree = (inst == ci) & (prim == cp)
+    assert agree.mean() >= 0.9999, agree.mean()
+    m = agree & np.isfinite(ct)
+    np.testing.assert_allclose(t[m], ct[m], rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("xi", [(0.37, 0.61), (0.9, 0.1)])
+def test_primary_ids_jittered(ctx, cornell, instanced, xi):
+    for scene, w, h in [(cornell, 256, 256), (instanced, 480, 270)]:
+        ctx.upload_scene(scene)
+        cam = scene.camera(w / h)
+        gi, gp = ctx.primary_ids(cam, w, h, xi)
+        ci, cp = oracle.CpuScene(scene).primary_ids(cam, w, h, xi)
+        assert ((gi == ci) & (gp == cp)).mean() >= 0.9999
+
+
+def test_primary_ids_pixel_corners_differ_only_on_shared_edges(ctx, cornell):
+    """xi = (0,0) (the G-buffer rays) puts rays of the Cornell box exactly on the
+    shared diagonals of its quads; both triangles hit at the same distance and
+    traversal order picks one.  The instance always agrees."""
     ctx.upload_scene(cornell)
     cam = cornell.camera(1.0)
     gi, gp = ctx.primary_ids(cam, 256, 256, (0.0, 0.0))
     ci, cp = oracle.CpuScene(cornell).primary_ids(cam, 256, 256, (0.0, 0.0))
-    assert ((gi == ci) & (gp == cp)).mean() >= 0.9999
+    assert (gi == ci).mean() >= 0.9999
+    assert ((gi == ci) & (gp == cp)).mean() >= 0.99
 
 
 def test_render_samples_match_oracle_pathwise(ctx, cornell):

@@ -179,8 +179,9 @@ def test_gbuffer(scenes, name):
     cam = s.camera(1.0)
     a = oracle.CpuScene(s).gbuffer(cam, 64, 48)
     b = oracle.RefScene(s).gbuffer(cam, 64, 48)
+    # instance transforms are inverted in float by different cofactor orders
     for x, y in zip(a, b):
-        np.testing.assert_allclose(x, y, rtol=1e-5, atol=1e-4)
+        np.testing.assert_allclose(x, y, rtol=1e-5, atol=5e-4)
 
 
 @pytest.mark.parametrize("tile", [(16, 16), (64, 64), (5, 7)])
