@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline bytes)")
+    ap.add_argument("--copy-mb", type=int, default=1024, help="size of the streaming-copy calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard by "
@@ -102,6 +103,20 @@ def main():
     ctx.upload_scene(scene)
     info = ctx.scene_info()
     log(f"[bench] scene: {uniq} unique / {inst_tris} instanced triangles, upload+build {time.time() - t0:.1f}s, {info}")
+    # streaming copy (16 B loads/stores) of a known size: the achievable HBM
+    # bandwidth on this box, and the calibration point of the byte counters
+    copy_bytes = args.copy_mb << 20
+    src, dst = ctx.alloc(copy_bytes), ctx.alloc(copy_bytes)
+    src.zero()
+    ctx.enable_kernel_timing(True)
+    for _ in range(5):
+        ctx.copy_d(dst, src, copy_bytes)
+    cn, cms = ctx.kernel_time("copy")
+    ctx.enable_kernel_timing(False)
+    copy_gbps = 2 * copy_bytes / (cms * 1e-3) / 1e9
+    src.free()
+    dst.free()
+    log(f"[bench] copy {args.copy_mb} MiB: {cms:.3f} ms, {copy_gbps:.0f} GB/s (read + write)")
     W, H = args.width, args.height
     cam = scene.camera(W / H)
     n = W * H
@@ -148,6 +163,7 @@ def main():
                 traffic = json.load(f).get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "measured_copy_GBps": round(copy_gbps, 1),
                 "kernel": "extend", "launches": ext_n, "mean_launch_ms": round(ext_ms, 4),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch),
                 "bytes_per_ray": round(traversal_bytes(count) / max(1, count.camera_rays + count.extension_rays), 1),

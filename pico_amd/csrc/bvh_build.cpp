@@ -144,30 +144,30 @@ struct Builder {
     }
 };
 
-// Inverse of an affine column-major 4x4 (bottom row 0 0 0 1), in double.
+// Inverse of a column-major 4x4 by cofactor expansion in single precision
+// (the reference inverts instance transforms in float, Source/Core/UpperLevelBVH.cpp:24);
+// rows 0..2 of the inverse are returned.  The expansion order is fixed so that
+// every consumer of the scene (device traversal, CPU checks) sees the same
+// object-space rays bit for bit.
 bool affine_inverse_rows(const float* m, float inv[3][4]) {
-    double a[3][3], t[3];
-    for (int r = 0; r < 3; ++r) {
-        for (int c = 0; c < 3; ++c) a[r][c] = m[c * 4 + r];
-        t[r] = m[12 + r];
-    }
-    const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
-                       a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
-    if (!(std::fabs(det) > 1e-30)) return false;
-    double ia[3][3];
-    ia[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
-    ia[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
-    ia[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
-    ia[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
-    ia[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
-    ia[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
-    ia[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
-    ia[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
-    ia[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
-    for (int r = 0; r < 3; ++r) {
-        for (int c = 0; c < 3; ++c) inv[r][c] = (float)ia[r][c];
-        inv[r][3] = (float)(-(ia[r][0] * t[0] + ia[r][1] * t[1] + ia[r][2] * t[2]));
-    }
+    auto e = [&](int r, int c) { return m[c * 4 + r]; };
+    auto minor3 = [&](int r0, int r1, int r2, int c0, int c1, int c2) {
+        return e(r0, c0) * (e(r1, c1) * e(r2, c2) - e(r2, c1) * e(r1, c2)) -
+               e(r0, c1) * (e(r1, c0) * e(r2, c2) - e(r2, c0) * e(r1, c2)) +
+               e(r0, c2) * (e(r1, c0) * e(r2, c1) - e(r2, c0) * e(r1, c1));
+    };
+    const int rs[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    float cof[4][4];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            const float mn = minor3(rs[r][0], rs[r][1], rs[r][2], rs[c][0], rs[c][1], rs[c][2]);
+            cof[r][c] = ((r + c) & 1) ? -mn : mn;
+        }
+    const float det = e(0, 0) * cof[0][0] + e(0, 1) * cof[0][1] + e(0, 2) * cof[0][2] + e(0, 3) * cof[0][3];
+    if (!(std::fabs(det) > 1e-30f) || !std::isfinite(det)) return false;
+    const float id = 1.0f / det;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) inv[r][c] = cof[c][r] * id;
     return true;
 }
 

@@ -130,7 +130,10 @@ __device__ __forceinline__ float4 ldg4(const void* p) { return *reinterpret_cast
 template <int STACK, bool COUNT>
 __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint32_t* stk, uint32_t stride,
                                              TraceCounters* cnt, uint32_t* overflow) {
-#pragma clang fp contract(fast)
+    // No FP contraction here (the library is built without it): the
+    // object-space transform and Moller-Trumbore round exactly as the CPU
+    // restatement does; only the box tests use explicit fmaf (conservative,
+    // they only cull).
     Hit h;
     h.t = INFINITY; h.u = 0.0f; h.v = 0.0f; h.inst = 0xffffffffu; h.prim = 0xffffffffu;
     f3 ro = o, rd = d;
@@ -209,10 +212,12 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint
             if (COUNT) cnt->insts++;
             if (sp < STACK) stk[sp * stride] = kSentinel, ++sp;
             else *overflow = 1u;
-            ro = mk(m0.x * o.x + m0.y * o.y + m0.z * o.z + m0.w, m1.x * o.x + m1.y * o.y + m1.z * o.z + m1.w,
-                    m2.x * o.x + m2.y * o.y + m2.z * o.z + m2.w);
-            rd = mk(m0.x * d.x + m0.y * d.y + m0.z * d.z, m1.x * d.x + m1.y * d.y + m1.z * d.z,
-                    m2.x * d.x + m2.y * d.y + m2.z * d.z);
+            // transform_ray (Source/Core/AABB.cpp:43-52): M^-1 (o, 1) as
+            // (c0*x + c1*y) + (c2*z + c3), mat3(M^-1) d as (c0*x + c1*y) + c2*z
+            ro = mk((m0.x * o.x + m0.y * o.y) + (m0.z * o.z + m0.w), (m1.x * o.x + m1.y * o.y) + (m1.z * o.z + m1.w),
+                    (m2.x * o.x + m2.y * o.y) + (m2.z * o.z + m2.w));
+            rd = mk((m0.x * d.x + m0.y * d.y) + m0.z * d.z, (m1.x * d.x + m1.y * d.y) + m1.z * d.z,
+                    (m2.x * d.x + m2.y * d.y) + m2.z * d.z);
             inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
             oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
             in_blas = true;
