@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "../../include/pico_hip.h"
 #include "device_types.h"
 
 namespace pico {
@@ -120,6 +121,16 @@ constexpr uint32_t kSentinel = 0xfffffffeu;
 
 __device__ __forceinline__ float4 ldg4(const void* p) { return *reinterpret_cast<const float4*>(p); }
 
+// Reciprocal direction for the fused slab test lo * inv + (-o * inv).  A zero
+// component would give inf * lo - inf * o = NaN there (the reference's
+// (lo - o) * inv gives +-inf); it is replaced by +-1e-20 so the slab bounds
+// stay finite and keep their sign.
+__device__ __forceinline__ f3 slab_rcp(f3 d) {
+    const float e = 1e-20f;
+    return mk(1.0f / (fabsf(d.x) > e ? d.x : copysignf(e, d.x)), 1.0f / (fabsf(d.y) > e ? d.y : copysignf(e, d.y)),
+              1.0f / (fabsf(d.z) > e ? d.z : copysignf(e, d.z)));
+}
+
 // Closest hit of a world-space ray against the TLAS and the instanced BLASes.
 // The traversal stack lives in LDS (`stk`, STACK entries at stride `stride`);
 // near child first, far child pushed, an instance boundary pushes a sentinel
@@ -137,7 +148,7 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint
     Hit h;
     h.t = INFINITY; h.u = 0.0f; h.v = 0.0f; h.inst = 0xffffffffu; h.prim = 0xffffffffu;
     f3 ro = o, rd = d;
-    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    f3 inv = slab_rcp(d);
     f3 oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
     uint32_t ref = s.tlas_root;
     uint32_t sp = 0;
@@ -218,7 +229,7 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint
                     (m2.x * o.x + m2.y * o.y) + (m2.z * o.z + m2.w));
             rd = mk((m0.x * d.x + m0.y * d.y) + m0.z * d.z, (m1.x * d.x + m1.y * d.y) + m1.z * d.z,
                     (m2.x * d.x + m2.y * d.y) + m2.z * d.z);
-            inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+            inv = slab_rcp(rd);
             oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
             in_blas = true;
             ref = root;
@@ -231,7 +242,7 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint
             ref = stk[sp * stride];
             if (ref != kSentinel) break;
             ro = o; rd = d;
-            inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+            inv = slab_rcp(d);
             oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
             in_blas = false;
         }
