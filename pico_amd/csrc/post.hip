@@ -142,7 +142,8 @@ __global__ void k_relmse(const float* __restrict__ img, const double* __restrict
 __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
 // 4 independent 16-byte loads in flight per lane, non-temporal stores.
-__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(v4f* __restrict__ dst, const v4f* __restrict__ src, size_t n) {
     const size_t stride = (size_t)gridDim.x * 256;
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     for (; i + 3 * stride < n; i += 4 * stride) {
@@ -164,7 +165,7 @@ int pico_hip_copy_d(pico_hip_ctx* c, void* dst, const void* src, size_t bytes) {
     if (!dst || !src || (bytes & 15)) return fail(c, PICO_ERR_ARG, "copy needs non-NULL buffers and a multiple of 16 bytes");
     const size_t n = bytes / 16;
     const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)c->num_cus * 32);
-    if (n) PICO_LAUNCH(c, "copy", k_copy, grid, 256, 0, (float4*)dst, (const float4*)src, n);
+    if (n) PICO_LAUNCH(c, "copy", k_copy, grid, 256, 0, (v4f*)dst, (const v4f*)src, n);
     return PICO_OK;
 }
 
