@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_copy(v4f* __restrict__ dst, const v4f* 
     const size_t stride = (size_t)gridDim.x * 256;
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     for (; i + 3 * stride < n; i += 4 * stride) {
-        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        const v4f a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
         __builtin_nontemporal_store(a, dst + i);
         __builtin_nontemporal_store(b, dst + i + stride);
         __builtin_nontemporal_store(c, dst + i + 2 * stride);
