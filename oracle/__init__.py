@@ -257,3 +257,48 @@ def ref_tonemap(rgb, w, h):
     out = _f32(rgb, (-1, 3)).copy()
     ref_lib().ref_tonemap(_p(out), w, h)
     return out
+
+
+def cnn_num_params():
+    f = cpu_lib().cnnref_num_params
+    f.restype = C.c_uint32
+    return f()
+
+
+def cnn_forward(params, in10, h, w):
+    lib = cpu_lib()
+    lib.cnnref_forward.restype = None
+    lib.cnnref_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(in10, np.float32)
+    out = np.empty((h * w, 3), np.float64)
+    lib.cnnref_forward(_p(p), _p(x), h, w, _p(out))
+    return out
+
+
+def cnn_loss_grad(params, in10, targets, batch, h, w, grads=True):
+    lib = cpu_lib()
+    lib.cnnref_loss_grad.restype = C.c_double
+    lib.cnnref_loss_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(in10, np.float32)
+    t = np.ascontiguousarray(targets, np.float32)
+    g = np.zeros(p.size, np.float64) if grads else None
+    loss = lib.cnnref_loss_grad(_p(p), _p(x), _p(t), batch, h, w, _p(g))
+    return loss, g
+
+
+CNN_LAYERS = [(10, 32, 9), (32, 32, 9), (32, 64, 9), (64, 64, 9), (64, 128, 9), (128, 128, 9), (192, 64, 9), (64, 64, 9),
+              (96, 32, 9), (32, 32, 9), (32, 25, 1)]
+
+
+def cnn_layer_slices():
+    """(weights slice, bias slice) of every layer in the flat parameter vector."""
+    out, o = [], 0
+    for cin, cout, taps in CNN_LAYERS:
+        w = slice(o, o + taps * cin * cout)
+        o += taps * cin * cout
+        b = slice(o, o + cout)
+        o += cout
+        out.append((w, b))
+    return out

@@ -415,3 +415,60 @@ class Context:
         n, ms = C.c_uint64(), C.c_double()
         self._check(lib().pico_hip_kernel_time(self._h, name.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+
+class CNN:
+    """The learned kernel-predicting denoiser on the device (BF16 MFMA)."""
+
+    def __init__(self, ctx, max_width, max_height, max_batch=1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._check(lib().pico_hip_cnn_create(ctx._h, max_width, max_height, max_batch, C.byref(h)))
+        self._h = h
+        self.n_params = lib().pico_hip_cnn_num_params()
+
+    def close(self):
+        if self._h:
+            lib().pico_hip_cnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, seed):
+        self.ctx._check(lib().pico_hip_cnn_init(self._h, seed))
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        self.ctx._check(lib().pico_hip_cnn_set_params(self._h, _ptr(p)))
+
+    def get_params(self):
+        p = np.empty(self.n_params, np.float32)
+        self.ctx._check(lib().pico_hip_cnn_get_params(self._h, _ptr(p)))
+        return p
+
+    def denoise(self, inputs10, w, h):
+        x = _f32(inputs10).reshape(10, h * w)
+        out = np.empty((w * h, 3), np.float32)
+        self.ctx._check(lib().pico_hip_cnn_denoise(self._h, _ptr(x), w, h, _ptr(out)))
+        return out
+
+    def denoise_d(self, d_in, w, h, d_out):
+        self.ctx._check(lib().pico_hip_cnn_denoise_d(self._h, d_in.ptr, w, h, d_out.ptr))
+
+    def loss_grad(self, inputs, targets, batch, w, h):
+        x = _f32(inputs).reshape(batch, 10, h * w)
+        t = _f32(targets).reshape(batch, h * w, 3)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_double()
+        self.ctx._check(lib().pico_hip_cnn_loss_grad(self._h, _ptr(x), _ptr(t), batch, w, h, C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train_step_d(self, d_in, d_tgt, batch, w, h, lr):
+        loss = C.c_double()
+        self.ctx._check(lib().pico_hip_cnn_train_step_d(self._h, d_in.ptr, d_tgt.ptr, batch, w, h, lr, C.byref(loss)))
+        return loss.value

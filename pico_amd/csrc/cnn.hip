@@ -1,6 +1,690 @@
-// placeholder replaced by the learned denoiser
+// Learned denoiser: kernel-predicting 3-level U-Net (32/64/128 channels, 3x3
+// convolutions, 1x1 head predicting a 5x5 per-pixel kernel), trained on relMSE.
+// No counterpart in the reference; it stands beside the a-trous filter.
+//
+// Device layout: activations are bf16 NHWC with the channel count padded to a
+// multiple of 16 (the input's 10 channels live in 16).  A 3x3 convolution is
+// an implicit GEMM (M = pixels, N = output channels, K = 9 x input channels)
+// on v_mfma_f32_16x16x16_bf16 with FP32 accumulation: a wave owns 16
+// consecutive pixels of one row and 32 output channels; each K step loads one
+// 8-byte A fragment (4 channels of one pixel at the tap's offset, zero outside
+// the image) and two 8-byte B fragments (weights stored [tap][out][in], K
+// contiguous).  Backward-data is the same kernel with the taps mirrored and
+// the weights stored [tap][in][out]; backward-weight stages 64-pixel tiles of
+// the input and of the output gradient in LDS and reduces over pixels with
+// MFMA (split over pixel chunks, FP32 atomics into the FP32 gradient).
+// The softmax/kernel application, the relMSE loss (FP64 reduction) and Adam
+// run in FP32/FP64.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "../../include/pico_hip.h"
-extern "C" {
-uint32_t pico_hip_cnn_num_params(void) { return 0; }
+#include "ctx.h"
+
+using namespace pico;
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+struct LayerDef { int cin, cout, taps, kpad, npad; };
+// e0a e0b e1a e1b ba bb d1a d1b d0a d0b head
+constexpr LayerDef kL[11] = {{10, 32, 9, 16, 32},   {32, 32, 9, 32, 32},    {32, 64, 9, 32, 64},  {64, 64, 9, 64, 64},
+                             {64, 128, 9, 64, 128}, {128, 128, 9, 128, 128}, {192, 64, 9, 192, 64}, {64, 64, 9, 64, 64},
+                             {96, 32, 9, 96, 32},   {32, 32, 9, 32, 32},    {32, 25, 1, 32, 32}};
+
+__host__ __device__ inline bf16 f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)(u >> 16);  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
 }
+__device__ inline float bf2f(bf16 b) { return __uint_as_float((uint32_t)b << 16); }
+
+struct Tensor { bf16* p; int stride, off; };
+
+struct ConvArgs {
+    const bf16* in; int in_stride, in_off;
+    const bf16* w;          // [taps][npad][kpad]
+    const float* bias;      // [npad] or null
+    bf16* out; int out_stride, out_off;
+    float* out_f32;         // [P][npad] (head logits) or null
+    const bf16* mask; int mask_stride, mask_off;  // multiply by (mask > 0)
+    int B, H, W, kpad, npad, taps, flip, relu;
+};
+
+// Implicit-GEMM convolution on MFMA 16x16x16 bf16.  grid.x: row tiles of 64
+// pixels (4 waves x 16), grid.y: blocks of 32 output channels.
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_x = (a.W + 63) / 64;
+    const int row = blockIdx.x / tiles_x;         // b*H + y
+    const int x0 = (blockIdx.x % tiles_x) * 64 + wave * 16;
+    if (x0 >= a.W) return;
+    const int b = row / a.H, y = row % a.H;
+    const int n0 = blockIdx.y * 32;
+    const int mi = lane & 15, kq = lane >> 4;
+    f4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+    const bf16* w0 = a.w + (size_t)(n0 + mi) * a.kpad + 4 * kq;
+    const bf16* w1 = w0 + (size_t)16 * a.kpad;
+    for (int t = 0; t < a.taps; ++t) {
+        int dy = a.taps == 9 ? t / 3 - 1 : 0, dx = a.taps == 9 ? t % 3 - 1 : 0;
+        if (a.flip) { dy = -dy; dx = -dx; }
+        const int yy = y + dy, xx = x0 + mi + dx;
+        const bool valid = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+        const bf16* ap = a.in + ((size_t)(b * a.H + (valid ? yy : 0)) * a.W + (valid ? xx : 0)) * a.in_stride + a.in_off + 4 * kq;
+        const size_t wt = (size_t)t * a.npad * a.kpad;
+        for (int k0 = 0; k0 < a.kpad; k0 += 16) {
+            s4 av = {0, 0, 0, 0};
+            if (valid) av = *reinterpret_cast<const s4*>(ap + k0);
+            const s4 b0 = *reinterpret_cast<const s4*>(w0 + wt + k0);
+            const s4 b1 = *reinterpret_cast<const s4*>(w1 + wt + k0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b1, acc1, 0, 0, 0);
+        }
+    }
+    // D[i][j]: pixel i = 4*kq + r, channel j = mi
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int n = n0 + nt * 16 + mi;
+        const float bias = a.bias ? a.bias[n] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int x = x0 + 4 * kq + r;
+            if (x >= a.W) continue;
+            float v = (nt ? acc1[r] : acc0[r]) + bias;
+            if (a.relu) v = v > 0.0f ? v : 0.0f;
+            const size_t p = (size_t)(b * a.H + y) * a.W + x;
+            if (a.mask && !(bf2f(a.mask[p * a.mask_stride + a.mask_off + n]) > 0.0f)) v = 0.0f;
+            if (a.out_f32) a.out_f32[p * a.npad + n] = v;
+            else a.out[p * a.out_stride + a.out_off + n] = f2bf(v);
+        }
+    }
+}
+
+struct WgradArgs {
+    const bf16* x; int x_stride, x_off;   // layer input (kpad channels)
+    const bf16* g; int g_stride, g_off;   // gradient of the pre-activation (npad channels)
+    float* dw;   // [taps][cin][cout] FP32, logical
+    float* db;   // [cout]
+    int B, H, W, kpad, npad, cin, cout, taps, chunk;
+};
+
+constexpr int kWgTile = 64;
+constexpr int kWgSlots = 8;  // 16x16 output tiles per wave (8 waves cover up to 64 tiles)
+
+// dW[t][ci][co] = sum_p x[p + off_t][ci] g[p][co]; grid.x pixel chunks, grid.y taps.
+__global__ __launch_bounds__(512) void k_conv_wgrad(WgradArgs a) {
+    extern __shared__ bf16 lds[];
+    bf16* xs = lds;                       // [64][kpad]
+    bf16* gs = lds + kWgTile * a.kpad;    // [64][npad]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = blockIdx.y;
+    const int dy = a.taps == 9 ? t / 3 - 1 : 0, dx = a.taps == 9 ? t % 3 - 1 : 0;
+    const int P = a.B * a.H * a.W;
+    const int p_begin = blockIdx.x * a.chunk, p_end = min(P, p_begin + a.chunk);
+    const int tiles_n = a.npad / 16, tiles = (a.kpad / 16) * tiles_n;
+    f4v acc[kWgSlots];
+#pragma unroll
+    for (int i = 0; i < kWgSlots; ++i) acc[i] = {0, 0, 0, 0};
+    float bsum = 0.0f;
+    const int mi = lane & 15, kq = lane >> 4;
+    for (int p0 = p_begin; p0 < p_end; p0 += kWgTile) {
+        __syncthreads();
+        const int xv = a.kpad / 8, gv = a.npad / 8;  // 16-byte vectors per pixel
+        for (int idx = threadIdx.x; idx < kWgTile * xv; idx += 512) {
+            const int q = idx / xv, part = idx - q * xv;
+            const int p = p0 + q;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (p < p_end) {
+                const int b = p / (a.H * a.W), rem = p - b * a.H * a.W, yy = rem / a.W + dy, xx = rem % a.W + dx;
+                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                    v = *reinterpret_cast<const uint4*>(a.x + ((size_t)(b * a.H + yy) * a.W + xx) * a.x_stride + a.x_off + 8 * part);
+            }
+            *reinterpret_cast<uint4*>(xs + q * a.kpad + 8 * part) = v;
+        }
+        for (int idx = threadIdx.x; idx < kWgTile * gv; idx += 512) {
+            const int q = idx / gv, part = idx - q * gv;
+            const int p = p0 + q;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (p < p_end) v = *reinterpret_cast<const uint4*>(a.g + (size_t)p * a.g_stride + a.g_off + 8 * part);
+            *reinterpret_cast<uint4*>(gs + q * a.npad + 8 * part) = v;
+        }
+        __syncthreads();
+        if (t == 0 && threadIdx.x < a.npad)
+            for (int q = 0; q < kWgTile; ++q) bsum += bf2f(gs[q * a.npad + threadIdx.x]);
+#pragma unroll 1
+        for (int kk = 0; kk < kWgTile; kk += 16) {
+#pragma unroll
+            for (int slot = 0; slot < kWgSlots; ++slot) {
+                const int ti = wave + 8 * slot;
+                if (ti >= tiles) continue;
+                const int ci = (ti / tiles_n) * 16 + mi, co = (ti % tiles_n) * 16 + mi;
+                s4 av, bv;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int q = kk + 4 * kq + e;
+                    av[e] = (short)xs[q * a.kpad + ci];
+                    bv[e] = (short)gs[q * a.npad + co];
+                }
+                acc[slot] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[slot], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int slot = 0; slot < kWgSlots; ++slot) {
+        const int ti = wave + 8 * slot;
+        if (ti >= tiles) continue;
+        const int co = (ti % tiles_n) * 16 + mi;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int ci = (ti / tiles_n) * 16 + 4 * kq + r;
+            if (ci < a.cin && co < a.cout) atomicAdd(&a.dw[((size_t)t * a.cin + ci) * a.cout + co], acc[slot][r]);
+        }
+    }
+    if (t == 0 && threadIdx.x < a.cout) atomicAdd(&a.db[threadIdx.x], bsum);
+}
+
+// planar f32 [B][10][H][W] -> bf16 NHWC16 at the padded size (replicated edge)
+__global__ void k_pack(const float* __restrict__ in, bf16* __restrict__ x, int B, int H, int W, int Hp, int Wp) {
+    const int P = B * Hp * Wp;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const int b = p / (Hp * Wp), rem = p - b * Hp * Wp;
+        const int y = min(rem / Wp, H - 1), xx = min(rem % Wp, W - 1);
+        const float* src = in + (size_t)b * 10 * H * W + (size_t)y * W + xx;
+        bf16 v[16];
+        for (int k = 0; k < 10; ++k) {
+            float f = src[(size_t)k * H * W];
+            if (k < 3) f = log1pf(f > 0.0f ? f : 0.0f);
+            if (k == 9) f = log1pf(f);
+            v[k] = f2bf(f);
+        }
+        for (int k = 10; k < 16; ++k) v[k] = 0;
+        uint4* dst = reinterpret_cast<uint4*>(x + (size_t)p * 16);
+        dst[0] = *reinterpret_cast<uint4*>(&v[0]);
+        dst[1] = *reinterpret_cast<uint4*>(&v[8]);
+    }
+}
+
+// 2x2 average pool: src level (H, W) slice -> dst (H/2, W/2) with C channels
+__global__ void k_pool(const bf16* __restrict__ s, int s_stride, int s_off, bf16* __restrict__ d, int C, int B, int H, int W) {
+    const int h = H / 2, w = W / 2;
+    const size_t n = (size_t)B * h * w * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        const size_t p = i / C;
+        const int b = p / (h * w), rem = p - b * h * w, y = rem / w, x = rem % w;
+        auto at = [&](int yy, int xx) { return bf2f(s[((size_t)(b * H + yy) * W + xx) * s_stride + s_off + c]); };
+        d[i] = f2bf(0.25f * (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1) + at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1)));
+    }
+}
+
+// nearest 2x upsample of src (H/2, W/2, C) into the dst slice at (H, W)
+__global__ void k_up(const bf16* __restrict__ s, bf16* __restrict__ d, int d_stride, int d_off, int C, int B, int H, int W) {
+    const size_t n = (size_t)B * H * W * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        const size_t p = i / C;
+        const int b = p / (H * W), rem = p - b * H * W, y = rem / W, x = rem % W;
+        d[p * d_stride + d_off + c] = s[((size_t)(b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+    }
+}
+
+// backward of the upsample: dst (H/2, W/2, C) = sum of the 2x2 children of the slice
+__global__ void k_up_bwd(const bf16* __restrict__ g, int g_stride, int g_off, bf16* __restrict__ d, int C, int B, int H, int W) {
+    const int h = H / 2, w = W / 2;
+    const size_t n = (size_t)B * h * w * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        const size_t p = i / C;
+        const int b = p / (h * w), rem = p - b * h * w, y = rem / w, x = rem % w;
+        auto at = [&](int yy, int xx) { return bf2f(g[((size_t)(b * H + yy) * W + xx) * g_stride + g_off + c]); };
+        d[i] = f2bf(at(2 * y, 2 * x) + at(2 * y, 2 * x + 1) + at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
+    }
+}
+
+// d[p][c] = (pool_bwd(gp)[p][c] + skip[p][c]) * (act[p][c] > 0)
+__global__ void k_pool_bwd_add_mask(const bf16* __restrict__ gp, const bf16* __restrict__ sk, int sk_stride, int sk_off,
+                                    const bf16* __restrict__ act, int act_stride, int act_off, bf16* __restrict__ d, int C,
+                                    int B, int H, int W) {
+    const size_t n = (size_t)B * H * W * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        const size_t p = i / C;
+        const int b = p / (H * W), rem = p - b * H * W, y = rem / W, x = rem % W;
+        float v = 0.25f * bf2f(gp[((size_t)(b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]) +
+                  bf2f(sk[p * sk_stride + sk_off + c]);
+        if (!(bf2f(act[p * act_stride + act_off + c]) > 0.0f)) v = 0.0f;
+        d[i] = f2bf(v);
+    }
+}
+
+// softmax over the 25 logits and the 5x5 kernel applied to the radiance
+// (planar input channels 0..2, clamped neighbours); out is packed RGB.
+__global__ void k_kpn_apply(const float* __restrict__ logits, const float* __restrict__ in, float* __restrict__ wk,
+                            float* __restrict__ out, int B, int H, int W, int Hp, int Wp) {
+    const int P = B * H * W;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const int b = p / (H * W), rem = p - b * H * W, y = rem / W, x = rem % W;
+        const float* lg = logits + ((size_t)(b * Hp + y) * Wp + x) * 32;
+        float m = lg[0];
+        for (int k = 1; k < 25; ++k) m = fmaxf(m, lg[k]);
+        float e[25], s = 0.0f;
+        for (int k = 0; k < 25; ++k) { e[k] = expf(lg[k] - m); s += e[k]; }
+        const float is = 1.0f / s;
+        const float* rad = in + (size_t)b * 10 * H * W;
+        float r0 = 0, r1 = 0, r2 = 0;
+        for (int k = 0; k < 25; ++k) {
+            const float wgt = e[k] * is;
+            if (wk) wk[(size_t)p * 25 + k] = wgt;
+            const int yy = min(max(y + k / 5 - 2, 0), H - 1), xx = min(max(x + k % 5 - 2, 0), W - 1);
+            const size_t q = (size_t)yy * W + xx;
+            r0 += wgt * rad[q];
+            r1 += wgt * rad[(size_t)H * W + q];
+            r2 += wgt * rad[(size_t)2 * H * W + q];
+        }
+        out[(size_t)p * 3] = r0; out[(size_t)p * 3 + 1] = r1; out[(size_t)p * 3 + 2] = r2;
+    }
+}
+
+// relMSE loss (FP64 reduction) and its gradient with respect to the output.
+__global__ void k_loss(const float* __restrict__ out, const float* __restrict__ tgt, float* __restrict__ dout, size_t n3,
+                       double inv_count, double* loss) {
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        const double r = tgt[i], d = (double)out[i] - r, den = r * r + 1e-2;
+        acc += d * d / den;
+        dout[i] = (float)(2.0 * d / den * inv_count);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, acc);
+}
+
+// backward of the kernel application and softmax -> bf16 logit gradient [Pp][32]
+__global__ void k_kpn_bwd(const float* __restrict__ dout, const float* __restrict__ wk, const float* __restrict__ in,
+                          bf16* __restrict__ g, int B, int H, int W, int Hp, int Wp) {
+    const int Pp = B * Hp * Wp;
+    for (int pp = blockIdx.x * blockDim.x + threadIdx.x; pp < Pp; pp += gridDim.x * blockDim.x) {
+        const int b = pp / (Hp * Wp), rem = pp - b * Hp * Wp, y = rem / Wp, x = rem % Wp;
+        bf16* gd = g + (size_t)pp * 32;
+        if (y >= H || x >= W) {
+            for (int k = 0; k < 32; ++k) gd[k] = 0;
+            continue;
+        }
+        const size_t p = (size_t)(b * H + y) * W + x;
+        const float d0 = dout[p * 3], d1 = dout[p * 3 + 1], d2 = dout[p * 3 + 2];
+        const float* rad = in + (size_t)b * 10 * H * W;
+        float dw[25], dot = 0.0f;
+        for (int k = 0; k < 25; ++k) {
+            const int yy = min(max(y + k / 5 - 2, 0), H - 1), xx = min(max(x + k % 5 - 2, 0), W - 1);
+            const size_t q = (size_t)yy * W + xx;
+            dw[k] = d0 * rad[q] + d1 * rad[(size_t)H * W + q] + d2 * rad[(size_t)2 * H * W + q];
+            dot += wk[p * 25 + k] * dw[k];
+        }
+        for (int k = 0; k < 25; ++k) gd[k] = f2bf(wk[p * 25 + k] * (dw[k] - dot));
+        for (int k = 25; k < 32; ++k) gd[k] = 0;
+    }
+}
+
+// fp32 logical parameters -> bf16 [taps][npad][kpad] (forward) and
+// [taps][kpad][npad] (backward data) plus padded bias
+__global__ void k_relayout(const float* __restrict__ w, const float* __restrict__ bias, bf16* wf, bf16* wb, float* bpad,
+                           int cin, int cout, int taps, int kpad, int npad) {
+    const int n = taps * kpad * npad;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int t = i / (kpad * npad), rem = i - t * kpad * npad, co = rem / kpad, ci = rem % kpad;
+        const float v = (ci < cin && co < cout) ? w[((size_t)t * cin + ci) * cout + co] : 0.0f;
+        const bf16 b = f2bf(v);
+        wf[i] = b;
+        wb[((size_t)t * kpad + ci) * npad + co] = b;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npad; i += gridDim.x * blockDim.x) bpad[i] = i < cout ? bias[i] : 0.0f;
+}
+
+__global__ void k_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float c1, float c2,
+                       float eps) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+__global__ void k_zero_f32(float* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+__global__ void k_zero1(double* p) { *p = 0.0; }
+
+uint32_t grid_for(size_t n, int block = 256) { return (uint32_t)std::min<size_t>((n + block - 1) / block, 16384); }
+
+}  // namespace
+
+struct pico_cnn {
+    pico_hip_ctx* c = nullptr;
+    int maxW = 0, maxH = 0, maxB = 0, Hp = 0, Wp = 0;
+    size_t nparams = 0;
+    size_t off[11][2];
+    float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
+    bf16 *wf[11], *wb[11];
+    float* bpad[11];
+    int step = 0;
+    // activations (bf16 NHWC)
+    bf16 *x0, *e0a, *c0, *p0, *e1a, *c1, *p1, *ba, *bb, *d1a, *d1b, *d0a, *d0b;
+    float *head, *wk, *out, *dout;
+    double* loss;
+    // gradients
+    bf16 *g_head, *g_d0b, *g_d0a, *g_c0, *g_d1b, *g_d1a, *g_c1, *g_bb, *g_ba, *g_p1, *g_e1b, *g_e1a, *g_p0, *g_e0b, *g_e0a;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <class T>
+T* dalloc(pico_cnn* n, size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count * sizeof(T), 64)) != hipSuccess) return nullptr;
+    n->allocs.push_back(p);
+    return (T*)p;
+}
+
+int conv(pico_cnn* n, int l, const bf16* in, int in_stride, int in_off, bf16* out, int out_stride, int out_off, int B,
+         int H, int W, bool relu, float* out_f32 = nullptr) {
+    const LayerDef& L = kL[l];
+    ConvArgs a{in, in_stride, in_off, n->wf[l], n->bpad[l], out, out_stride, out_off, out_f32, nullptr, 0, 0,
+               B, H, W, L.kpad, L.npad, L.taps, 0, relu ? 1 : 0};
+    dim3 grid((unsigned)(B * H * ((W + 63) / 64)), (unsigned)(L.npad / 32));
+    PICO_LAUNCH(n->c, "cnn_conv_fwd", k_conv, grid, 256, 0, a);
+    return PICO_OK;
+}
+
+// dX of layer l: gradient wrt its input (kpad channels), masked by `mask`
+int conv_dx(pico_cnn* n, int l, const bf16* g, int g_stride, bf16* dx, int dx_stride, int dx_off, const bf16* mask,
+            int mask_stride, int mask_off, int B, int H, int W) {
+    const LayerDef& L = kL[l];
+    ConvArgs a{g, g_stride, 0, n->wb[l], nullptr, dx, dx_stride, dx_off, nullptr, mask, mask_stride, mask_off,
+               B, H, W, L.npad, L.kpad, L.taps, 1, 0};
+    dim3 grid((unsigned)(B * H * ((W + 63) / 64)), (unsigned)(L.kpad / 32));
+    PICO_LAUNCH(n->c, "cnn_conv_bwd_data", k_conv, grid, 256, 0, a);
+    return PICO_OK;
+}
+
+int conv_dw(pico_cnn* n, int l, const bf16* x, int x_stride, int x_off, const bf16* g, int g_stride, int B, int H, int W) {
+    const LayerDef& L = kL[l];
+    const int P = B * H * W;
+    const int chunk = 2048;
+    WgradArgs a{x, x_stride, x_off, g, g_stride, 0, n->grads + n->off[l][0], n->grads + n->off[l][1],
+                B, H, W, L.kpad, L.npad, L.cin, L.cout, L.taps, chunk};
+    dim3 grid((unsigned)((P + chunk - 1) / chunk), (unsigned)L.taps);
+    const size_t lds = (size_t)kWgTile * (L.kpad + L.npad) * sizeof(bf16);
+    PICO_LAUNCH(n->c, "cnn_conv_bwd_weight", k_conv_wgrad, grid, 512, lds, a);
+    return PICO_OK;
+}
+
+int relayout(pico_cnn* n) {
+    for (int l = 0; l < 11; ++l) {
+        const LayerDef& L = kL[l];
+        PICO_LAUNCH(n->c, "cnn_relayout", k_relayout, grid_for((size_t)L.taps * L.kpad * L.npad), 256, 0,
+                    n->params + n->off[l][0], n->params + n->off[l][1], n->wf[l], n->wb[l], n->bpad[l], L.cin, L.cout, L.taps,
+                    L.kpad, L.npad);
+    }
+    return PICO_OK;
+}
+
+int check_dims(pico_cnn* n, uint32_t B, uint32_t W, uint32_t H) {
+    if (B == 0 || W == 0 || H == 0) return fail(n->c, PICO_ERR_ARG, "batch and image size must be positive");
+    const int Hp = (int)((H + 3) / 4 * 4), Wp = (int)((W + 3) / 4 * 4);
+    if ((int)B > n->maxB || Hp > n->Hp || Wp > n->Wp || (size_t)B * Hp * Wp > (size_t)n->maxB * n->Hp * n->Wp)
+        return fail(n->c, PICO_ERR_LIMIT, "batch or image larger than the sizes the network was created for");
+    return PICO_OK;
+}
+
+// forward of a device-resident batch; the filtered radiance lands in n->out
+int forward(pico_cnn* n, const float* d_in, int B, int H, int W, bool keep_wk) {
+    int rc;
+    const int Hp = (H + 3) / 4 * 4, Wp = (W + 3) / 4 * 4;
+    const int H1 = Hp / 2, W1 = Wp / 2, H2 = Hp / 4, W2 = Wp / 4;
+    const size_t P0 = (size_t)B * Hp * Wp;
+    PICO_LAUNCH(n->c, "cnn_pack", k_pack, grid_for(P0), 256, 0, d_in, n->x0, B, H, W, Hp, Wp);
+    if ((rc = conv(n, 0, n->x0, 16, 0, n->e0a, 32, 0, B, Hp, Wp, true))) return rc;
+    if ((rc = conv(n, 1, n->e0a, 32, 0, n->c0, 96, 64, B, Hp, Wp, true))) return rc;  // e0b -> skip slice of c0
+    PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 4 * 32), 256, 0, n->c0, 96, 64, n->p0, 32, B, Hp, Wp);
+    if ((rc = conv(n, 2, n->p0, 32, 0, n->e1a, 64, 0, B, H1, W1, true))) return rc;
+    if ((rc = conv(n, 3, n->e1a, 64, 0, n->c1, 192, 128, B, H1, W1, true))) return rc;  // e1b -> skip slice of c1
+    PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 16 * 64), 256, 0, n->c1, 192, 128, n->p1, 64, B, H1, W1);
+    if ((rc = conv(n, 4, n->p1, 64, 0, n->ba, 128, 0, B, H2, W2, true))) return rc;
+    if ((rc = conv(n, 5, n->ba, 128, 0, n->bb, 128, 0, B, H2, W2, true))) return rc;
+    PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 / 4 * 128), 256, 0, n->bb, n->c1, 192, 0, 128, B, H1, W1);
+    if ((rc = conv(n, 6, n->c1, 192, 0, n->d1a, 64, 0, B, H1, W1, true))) return rc;
+    if ((rc = conv(n, 7, n->d1a, 64, 0, n->d1b, 64, 0, B, H1, W1, true))) return rc;
+    PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 * 64), 256, 0, n->d1b, n->c0, 96, 0, 64, B, Hp, Wp);
+    if ((rc = conv(n, 8, n->c0, 96, 0, n->d0a, 32, 0, B, Hp, Wp, true))) return rc;
+    if ((rc = conv(n, 9, n->d0a, 32, 0, n->d0b, 32, 0, B, Hp, Wp, true))) return rc;
+    if ((rc = conv(n, 10, n->d0b, 32, 0, nullptr, 32, 0, B, Hp, Wp, false, n->head))) return rc;
+    PICO_LAUNCH(n->c, "cnn_kpn_apply", k_kpn_apply, grid_for((size_t)B * H * W), 256, 0, n->head, d_in,
+                keep_wk ? n->wk : nullptr, n->out, B, H, W, Hp, Wp);
+    return PICO_OK;
+}
+
+// loss + backward (gradients accumulated into n->grads, zeroed first)
+int backward(pico_cnn* n, const float* d_in, const float* d_tgt, int B, int H, int W) {
+    int rc;
+    const int Hp = (H + 3) / 4 * 4, Wp = (W + 3) / 4 * 4;
+    const int H1 = Hp / 2, W1 = Wp / 2, H2 = Hp / 4, W2 = Wp / 4;
+    const size_t P0 = (size_t)B * Hp * Wp;
+    const size_t n3 = (size_t)B * H * W * 3;
+    PICO_LAUNCH(n->c, "cnn_zero", k_zero_f32, grid_for(n->nparams), 256, 0, n->grads, n->nparams);
+    PICO_LAUNCH(n->c, "cnn_zero", k_zero1, 1, 1, 0, n->loss);
+    PICO_LAUNCH(n->c, "cnn_loss", k_loss, grid_for(n3), 256, 0, n->out, d_tgt, n->dout, n3, 1.0 / (double)n3, n->loss);
+    PICO_LAUNCH(n->c, "cnn_kpn_bwd", k_kpn_bwd, grid_for(P0), 256, 0, n->dout, n->wk, d_in, n->g_head, B, H, W, Hp, Wp);
+    // head
+    if ((rc = conv_dw(n, 10, n->d0b, 32, 0, n->g_head, 32, B, Hp, Wp))) return rc;
+    if ((rc = conv_dx(n, 10, n->g_head, 32, n->g_d0b, 32, 0, n->d0b, 32, 0, B, Hp, Wp))) return rc;
+    // d0b, d0a
+    if ((rc = conv_dw(n, 9, n->d0a, 32, 0, n->g_d0b, 32, B, Hp, Wp))) return rc;
+    if ((rc = conv_dx(n, 9, n->g_d0b, 32, n->g_d0a, 32, 0, n->d0a, 32, 0, B, Hp, Wp))) return rc;
+    if ((rc = conv_dw(n, 8, n->c0, 96, 0, n->g_d0a, 32, B, Hp, Wp))) return rc;
+    if ((rc = conv_dx(n, 8, n->g_d0a, 32, n->g_c0, 96, 0, n->c0, 96, 0, B, Hp, Wp))) return rc;
+    PICO_LAUNCH(n->c, "cnn_up_bwd", k_up_bwd, grid_for(P0 / 4 * 64), 256, 0, n->g_c0, 96, 0, n->g_d1b, 64, B, Hp, Wp);
+    // d1b, d1a
+    if ((rc = conv_dw(n, 7, n->d1a, 64, 0, n->g_d1b, 64, B, H1, W1))) return rc;
+    if ((rc = conv_dx(n, 7, n->g_d1b, 64, n->g_d1a, 64, 0, n->d1a, 64, 0, B, H1, W1))) return rc;
+    if ((rc = conv_dw(n, 6, n->c1, 192, 0, n->g_d1a, 64, B, H1, W1))) return rc;
+    if ((rc = conv_dx(n, 6, n->g_d1a, 64, n->g_c1, 192, 0, n->c1, 192, 0, B, H1, W1))) return rc;
+    PICO_LAUNCH(n->c, "cnn_up_bwd", k_up_bwd, grid_for(P0 / 16 * 128), 256, 0, n->g_c1, 192, 0, n->g_bb, 128, B, H1, W1);
+    // bb, ba
+    if ((rc = conv_dw(n, 5, n->ba, 128, 0, n->g_bb, 128, B, H2, W2))) return rc;
+    if ((rc = conv_dx(n, 5, n->g_bb, 128, n->g_ba, 128, 0, n->ba, 128, 0, B, H2, W2))) return rc;
+    if ((rc = conv_dw(n, 4, n->p1, 64, 0, n->g_ba, 128, B, H2, W2))) return rc;
+    if ((rc = conv_dx(n, 4, n->g_ba, 128, n->g_p1, 64, 0, nullptr, 0, 0, B, H2, W2))) return rc;
+    PICO_LAUNCH(n->c, "cnn_pool_bwd", k_pool_bwd_add_mask, grid_for(P0 / 4 * 64), 256, 0, n->g_p1, n->g_c1, 192, 128, n->c1,
+                192, 128, n->g_e1b, 64, B, H1, W1);
+    // e1b, e1a
+    if ((rc = conv_dw(n, 3, n->e1a, 64, 0, n->g_e1b, 64, B, H1, W1))) return rc;
+    if ((rc = conv_dx(n, 3, n->g_e1b, 64, n->g_e1a, 64, 0, n->e1a, 64, 0, B, H1, W1))) return rc;
+    if ((rc = conv_dw(n, 2, n->p0, 32, 0, n->g_e1a, 64, B, H1, W1))) return rc;
+    if ((rc = conv_dx(n, 2, n->g_e1a, 64, n->g_p0, 32, 0, nullptr, 0, 0, B, H1, W1))) return rc;
+    PICO_LAUNCH(n->c, "cnn_pool_bwd", k_pool_bwd_add_mask, grid_for(P0 * 32), 256, 0, n->g_p0, n->g_c0, 96, 64, n->c0, 96, 64,
+                n->g_e0b, 32, B, Hp, Wp);
+    // e0b, e0a
+    if ((rc = conv_dw(n, 1, n->e0a, 32, 0, n->g_e0b, 32, B, Hp, Wp))) return rc;
+    if ((rc = conv_dx(n, 1, n->g_e0b, 32, n->g_e0a, 32, 0, n->e0a, 32, 0, B, Hp, Wp))) return rc;
+    if ((rc = conv_dw(n, 0, n->x0, 16, 0, n->g_e0a, 32, B, Hp, Wp))) return rc;
+    return PICO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t pico_hip_cnn_num_params(void) {
+    size_t s = 0;
+    for (auto& L : kL) s += (size_t)L.taps * L.cin * L.cout + L.cout;
+    return (uint32_t)s;
+}
+
+int pico_hip_cnn_create(pico_hip_ctx* c, uint32_t mw, uint32_t mh, uint32_t mb, pico_cnn** out) {
+    if (!c || !out) return PICO_ERR_ARG;
+    if (mw == 0 || mh == 0 || mb == 0) return fail(c, PICO_ERR_ARG, "sizes must be positive");
+    if ((uint64_t)mw * mh * mb > (1ull << 27)) return fail(c, PICO_ERR_LIMIT, "at most 2^27 pixels per batch");
+    auto* n = new pico_cnn();
+    n->c = c;
+    n->maxW = (int)mw; n->maxH = (int)mh; n->maxB = (int)mb;
+    n->Hp = ((int)mh + 3) / 4 * 4; n->Wp = ((int)mw + 3) / 4 * 4;
+    size_t o = 0;
+    for (int l = 0; l < 11; ++l) {
+        n->off[l][0] = o;
+        o += (size_t)kL[l].taps * kL[l].cin * kL[l].cout;
+        n->off[l][1] = o;
+        o += kL[l].cout;
+    }
+    n->nparams = o;
+    const size_t P0 = (size_t)mb * n->Hp * n->Wp, P1 = P0 / 4, P2 = P0 / 16;
+    const size_t n3 = (size_t)mb * mw * mh * 3;
+    bool ok = (n->params = dalloc<float>(n, o)) && (n->grads = dalloc<float>(n, o)) && (n->m = dalloc<float>(n, o)) &&
+              (n->v = dalloc<float>(n, o));
+    for (int l = 0; l < 11 && ok; ++l) {
+        const size_t wsz = (size_t)kL[l].taps * kL[l].kpad * kL[l].npad;
+        ok = (n->wf[l] = dalloc<bf16>(n, wsz)) && (n->wb[l] = dalloc<bf16>(n, wsz)) && (n->bpad[l] = dalloc<float>(n, kL[l].npad));
+    }
+    ok = ok && (n->x0 = dalloc<bf16>(n, P0 * 16)) && (n->e0a = dalloc<bf16>(n, P0 * 32)) && (n->c0 = dalloc<bf16>(n, P0 * 96)) &&
+         (n->p0 = dalloc<bf16>(n, P1 * 32)) && (n->e1a = dalloc<bf16>(n, P1 * 64)) && (n->c1 = dalloc<bf16>(n, P1 * 192)) &&
+         (n->p1 = dalloc<bf16>(n, P2 * 64)) && (n->ba = dalloc<bf16>(n, P2 * 128)) && (n->bb = dalloc<bf16>(n, P2 * 128)) &&
+         (n->d1a = dalloc<bf16>(n, P1 * 64)) && (n->d1b = dalloc<bf16>(n, P1 * 64)) && (n->d0a = dalloc<bf16>(n, P0 * 32)) &&
+         (n->d0b = dalloc<bf16>(n, P0 * 32)) && (n->head = dalloc<float>(n, P0 * 32)) && (n->wk = dalloc<float>(n, P0 * 25)) &&
+         (n->out = dalloc<float>(n, n3)) && (n->dout = dalloc<float>(n, n3)) && (n->loss = dalloc<double>(n, 1)) &&
+         (n->g_head = dalloc<bf16>(n, P0 * 32)) && (n->g_d0b = dalloc<bf16>(n, P0 * 32)) && (n->g_d0a = dalloc<bf16>(n, P0 * 32)) &&
+         (n->g_c0 = dalloc<bf16>(n, P0 * 96)) && (n->g_d1b = dalloc<bf16>(n, P1 * 64)) && (n->g_d1a = dalloc<bf16>(n, P1 * 64)) &&
+         (n->g_c1 = dalloc<bf16>(n, P1 * 192)) && (n->g_bb = dalloc<bf16>(n, P2 * 128)) && (n->g_ba = dalloc<bf16>(n, P2 * 128)) &&
+         (n->g_p1 = dalloc<bf16>(n, P2 * 64)) && (n->g_e1b = dalloc<bf16>(n, P1 * 64)) && (n->g_e1a = dalloc<bf16>(n, P1 * 64)) &&
+         (n->g_p0 = dalloc<bf16>(n, P1 * 32)) && (n->g_e0b = dalloc<bf16>(n, P0 * 32)) && (n->g_e0a = dalloc<bf16>(n, P0 * 32));
+    if (!ok) {
+        pico_hip_cnn_destroy(n);
+        return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    }
+    *out = n;
+    return pico_hip_cnn_init(n, 1);
+}
+
+void pico_hip_cnn_destroy(pico_cnn* n) {
+    if (!n) return;
+    hipStreamSynchronize(n->c->stream);
+    for (void* p : n->allocs) hipFree(p);
+    delete n;
+}
+
+// He-normal initialisation (std sqrt(2 / fan_in)), zero biases; the head
+// starts small so the first kernels are near-uniform 5x5 averages.
+int pico_hip_cnn_init(pico_cnn* n, uint32_t seed) {
+    if (!n) return PICO_ERR_ARG;
+    std::vector<float> p(n->nparams, 0.0f);
+    uint64_t s = mix64(seed ^ 0xC0FFEEull);
+    auto u01 = [&]() { s = mix64(s); return ((s >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+    for (int l = 0; l < 11; ++l) {
+        const double sd = l == 10 ? 0.01 : std::sqrt(2.0 / (kL[l].taps * kL[l].cin));
+        for (size_t i = n->off[l][0]; i < n->off[l][1]; ++i)
+            p[i] = (float)(sd * std::sqrt(-2.0 * std::log(u01())) * std::cos(6.283185307179586 * u01()));
+    }
+    n->step = 0;
+    int rc = pico_hip_cnn_set_params(n, p.data());
+    if (rc) return rc;
+    PICO_HIP(n->c, hipMemsetAsync(n->m, 0, n->nparams * 4, n->c->stream));
+    PICO_HIP(n->c, hipMemsetAsync(n->v, 0, n->nparams * 4, n->c->stream));
+    PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_cnn_set_params(pico_cnn* n, const float* host) {
+    if (!n || !host) return PICO_ERR_ARG;
+    PICO_HIP(n->c, hipMemcpyAsync(n->params, host, n->nparams * 4, hipMemcpyHostToDevice, n->c->stream));
+    int rc = relayout(n);
+    if (rc) return rc;
+    PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_cnn_get_params(pico_cnn* n, float* host) {
+    if (!n || !host) return PICO_ERR_ARG;
+    PICO_HIP(n->c, hipMemcpyAsync(host, n->params, n->nparams * 4, hipMemcpyDeviceToHost, n->c->stream));
+    PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_cnn_denoise_d(pico_cnn* n, const float* d_in, uint32_t w, uint32_t h, float* d_out) {
+    if (!n) return PICO_ERR_ARG;
+    if (!d_in || !d_out) return fail(n->c, PICO_ERR_ARG, "NULL buffer");
+    int rc = check_dims(n, 1, w, h);
+    if (rc) return rc;
+    if ((rc = forward(n, d_in, 1, (int)h, (int)w, false))) return rc;
+    PICO_HIP(n->c, hipMemcpyAsync(d_out, n->out, (size_t)w * h * 12, hipMemcpyDeviceToDevice, n->c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_cnn_denoise(pico_cnn* n, const float* in, uint32_t w, uint32_t h, float* out) {
+    if (!n) return PICO_ERR_ARG;
+    if (!in || !out) return fail(n->c, PICO_ERR_ARG, "NULL buffer");
+    int rc = check_dims(n, 1, w, h);
+    if (rc) return rc;
+    const size_t px = (size_t)w * h;
+    float* d = (float*)scratch(n->c, "cnn_io", px * 52 + 64);
+    if (!d) return fail(n->c, PICO_ERR_NOMEM, "device allocation failed");
+    PICO_HIP(n->c, hipMemcpyAsync(d, in, px * 40, hipMemcpyHostToDevice, n->c->stream));
+    if ((rc = pico_hip_cnn_denoise_d(n, d, w, h, d + 10 * px))) return rc;
+    PICO_HIP(n->c, hipMemcpyAsync(out, d + 10 * px, px * 12, hipMemcpyDeviceToHost, n->c->stream));
+    PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_cnn_loss_grad(pico_cnn* n, const float* in, const float* tgt, uint32_t B, uint32_t w, uint32_t h, double* loss,
+                           float* grads) {
+    if (!n) return PICO_ERR_ARG;
+    if (!in || !tgt) return fail(n->c, PICO_ERR_ARG, "NULL buffer");
+    int rc = check_dims(n, B, w, h);
+    if (rc) return rc;
+    const size_t px = (size_t)B * w * h;
+    float* d = (float*)scratch(n->c, "cnn_train_io", px * 52 + 64);
+    if (!d) return fail(n->c, PICO_ERR_NOMEM, "device allocation failed");
+    PICO_HIP(n->c, hipMemcpyAsync(d, in, px * 40, hipMemcpyHostToDevice, n->c->stream));
+    PICO_HIP(n->c, hipMemcpyAsync(d + 10 * px, tgt, px * 12, hipMemcpyHostToDevice, n->c->stream));
+    if ((rc = forward(n, d, (int)B, (int)h, (int)w, true))) return rc;
+    if ((rc = backward(n, d, d + 10 * px, (int)B, (int)h, (int)w))) return rc;
+    double l = 0.0;
+    PICO_HIP(n->c, hipMemcpyAsync(&l, n->loss, 8, hipMemcpyDeviceToHost, n->c->stream));
+    if (grads) PICO_HIP(n->c, hipMemcpyAsync(grads, n->grads, n->nparams * 4, hipMemcpyDeviceToHost, n->c->stream));
+    PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+    if (loss) *loss = l / (double)(px * 3);
+    return PICO_OK;
+}
+
+int pico_hip_cnn_train_step_d(pico_cnn* n, const float* d_in, const float* d_tgt, uint32_t B, uint32_t w, uint32_t h,
+                              float lr, double* loss) {
+    if (!n) return PICO_ERR_ARG;
+    if (!d_in || !d_tgt) return fail(n->c, PICO_ERR_ARG, "NULL buffer");
+    int rc = check_dims(n, B, w, h);
+    if (rc) return rc;
+    if ((rc = forward(n, d_in, (int)B, (int)h, (int)w, true))) return rc;
+    if ((rc = backward(n, d_in, d_tgt, (int)B, (int)h, (int)w))) return rc;
+    n->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
+    PICO_LAUNCH(n->c, "cnn_adam", k_adam, grid_for(n->nparams), 256, 0, n->params, n->grads, n->m, n->v, (int)n->nparams, lr,
+                b1, b2, c1, c2, 1e-8f);
+    if ((rc = relayout(n))) return rc;
+    if (loss) {
+        double l = 0.0;
+        PICO_HIP(n->c, hipMemcpyAsync(&l, n->loss, 8, hipMemcpyDeviceToHost, n->c->stream));
+        PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+        *loss = l / ((double)B * w * h * 3);
+    }
+    return PICO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+"""The learned denoiser on the device (BF16 MFMA, FP32 accumulation) against
+its FP64 CPU oracle: forward output, loss, every parameter gradient, and a
+training run that lowers the loss."""
+import numpy as np
+import pytest
+
+import oracle
+import pico_amd as pa
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(rng, b, h, w):
+    x = np.empty((b, 10, h, w), np.float32)
+    x[:, 0:3] = rng.gamma(1.0, 0.5, (b, 3, h, w))
+    x[:, 3:6] = rng.uniform(0.2, 0.8, (b, 3, h, w))
+    n = rng.normal(size=(b, 3, h, w))
+    x[:, 6:9] = n / np.linalg.norm(n, axis=1, keepdims=True)
+    x[:, 9] = rng.uniform(1, 10, (b, h, w))
+    t = rng.gamma(2.0, 0.3, (b, h * w, 3)).astype(np.float32)
+    return x, t
+
+
+def _params(rng):
+    p = np.zeros(oracle.cnn_num_params(), np.float32)
+    for (cin, cout, taps), (ws, bs) in zip(oracle.CNN_LAYERS, oracle.cnn_layer_slices()):
+        p[ws] = rng.normal(0, np.sqrt(2.0 / (taps * cin)), ws.stop - ws.start)
+        p[bs] = rng.normal(0, 0.05, cout)
+    # a head with some spread so the predicted kernels are not uniform
+    ws, bs = oracle.cnn_layer_slices()[-1]
+    p[ws] *= 20
+    return p
+
+
+@pytest.mark.parametrize("h,w", [(16, 16), (24, 40)])
+def test_forward_matches_oracle(ctx, h, w):
+    rng = np.random.default_rng(h + w)
+    x, _ = _data(rng, 1, h, w)
+    p = _params(rng)
+    net = pa.CNN(ctx, w, h, 1)
+    net.set_params(p)
+    g = net.denoise(x[0], w, h)
+    c = oracle.cnn_forward(p.astype(np.float64), x[0], h, w)
+    err = np.abs(g - c) / (np.abs(c) + 1e-2)
+    # bf16 activations and weights: ~3 significant digits per layer
+    assert np.median(err) < 5e-3 and err.max() < 0.1, (np.median(err), err.max())
+
+
+def test_gradients_match_oracle(ctx):
+    rng = np.random.default_rng(3)
+    b, h, w = 2, 16, 16
+    x, t = _data(rng, b, h, w)
+    p = _params(rng)
+    net = pa.CNN(ctx, w, h, b)
+    net.set_params(p)
+    gl, gg = net.loss_grad(x, t, b, w, h)
+    cl, cg = oracle.cnn_loss_grad(p.astype(np.float64), x, t, b, h, w)
+    assert abs(gl - cl) <= 2e-2 * cl
+    for i, (ws, bs) in enumerate(oracle.cnn_layer_slices()):
+        for s in (ws, bs):
+            a, r = gg[s].astype(np.float64), cg[s]
+            cos = a @ r / (np.linalg.norm(a) * np.linalg.norm(r) + 1e-30)
+            rel = np.linalg.norm(a - r) / (np.linalg.norm(r) + 1e-30)
+            assert cos > 0.99 and rel < 0.15, (i, s, cos, rel)
+
+
+def test_training_lowers_the_loss(ctx):
+    rng = np.random.default_rng(4)
+    b, h, w = 4, 32, 32
+    x, _ = _data(rng, b, h, w)
+    # target: a smoothed version of the noisy radiance (a denoising task)
+    rad = x[:, 0:3]
+    sm = rad.copy()
+    for ax in (2, 3):
+        sm = (np.roll(sm, 1, ax) + sm + np.roll(sm, -1, ax)) / 3
+    t = np.ascontiguousarray(sm.reshape(b, 3, h * w).transpose(0, 2, 1))
+    net = pa.CNN(ctx, w, h, b)
+    net.init(7)
+    d_x, d_t = ctx.alloc(x.nbytes), ctx.alloc(t.nbytes)
+    d_x.upload(x)
+    d_t.upload(t)
+    losses = [net.train_step_d(d_x, d_t, b, w, h, 2e-3) for _ in range(60)]
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
