@@ -9,8 +9,11 @@ reference's running mean / variance with its early-out).  The scene is built
 and uploaded before timing.
 
 Printed: one JSON line with Mrays/s (rays of the accumulated samples / wall
-time of the timed steps), the roofline of the traversal kernel (`extend`) and
-the reference CPU renderer's Mrays/s on the host cores.
+time of the timed steps), the roofline of the traversal kernel (`extend`),
+the reference CPU renderer's Mrays/s on the host cores, and the quality leg:
+relMSE against a 4096-spp FP64-accumulated render of the same view for the
+64-spp frame, 4 and 16 spp with the a-trous filter and 4 spp with the learned
+denoiser, each with its ms/frame (render + G-buffer + filter).
 
 Ray = one closest-hit scene query (camera, extension or shadow ray).
 """
@@ -76,6 +79,99 @@ def cpu_baseline(scene, cam, w, h, spp, depth, seed):
                       f"{rays} rays, median of 3 runs {t:.2f} s"}
 
 
+def _median_ms(fn, reps, sync):
+    times = []
+    for _ in range(reps):
+        sync()
+        t0 = time.perf_counter()
+        fn()
+        sync()
+        times.append((time.perf_counter() - t0) * 1e3)
+    return sorted(times)[len(times) // 2]
+
+
+def quality_leg(ctx, pa, scene, cam, W, H, depth, oracle_spp, weights, d_frame):
+    """relMSE vs an FP64-accumulated oracle (frame seed 1000) and ms/frame of
+    the a-trous and learned filters at 4 and 16 spp (frame seed 7)."""
+    n = W * H
+    d_sum = ctx.alloc(24 * n)
+    d_sum.zero()
+    t0 = time.time()
+    chunk = 256
+    for base in range(0, oracle_spp, chunk):
+        p = pa.params(W, H, min(chunk, oracle_spp - base), depth, tile=16, frame_seed=1000)
+        ctx.render_accum64_d(cam, p, base, d_sum)
+        ctx.sync()
+        log(f"[bench] oracle {base + chunk}/{oracle_spp} spp, {time.time() - t0:.1f} s")
+    t_oracle = time.time() - t0
+    out = {"oracle_spp": oracle_spp, "oracle_s": round(t_oracle, 1), "oracle_frame_seed": 1000,
+           "frame_64spp_relmse": round(ctx.relmse_d(d_frame, d_sum, oracle_spp, n), 6)}
+    d_pix, d_cnt, d_var = ctx.alloc(12 * n), ctx.alloc(4 * n), ctx.alloc(12 * n)
+    d_n, d_p, d_a = ctx.alloc(12 * n), ctx.alloc(12 * n), ctx.alloc(12 * n)
+    d_out, d_scr = ctx.alloc(12 * n), ctx.alloc(12 * n)
+    net = None
+    if weights and os.path.exists(weights):
+        net = pa.CNN(ctx, W, H, 1)
+        net.load(weights)
+
+    def frame(spp, filt):
+        p = pa.params(W, H, spp, depth, tile=16, frame_seed=7)
+        ctx.render_d(cam, p, d_pix, d_cnt, d_var)
+        if filt is None:
+            return d_pix
+        ctx.gbuffer_d(cam, W, H, d_n, d_p, d_a)
+        if filt == "atrous":
+            ctx.atrous_d(d_pix, d_n, d_p, d_a, W, H, d_scr, d_out)
+        else:
+            net.denoise_gbuffer_d(d_pix, d_n, d_p, d_a, W, H, d_out)
+        return d_out
+
+    rows = {}
+    for name, spp, filt in (("raw_4spp", 4, None), ("raw_16spp", 16, None), ("atrous_4spp", 4, "atrous"),
+                            ("atrous_16spp", 16, "atrous"), ("cnn_4spp", 4, "cnn")):
+        if filt == "cnn" and net is None:
+            rows[name] = None
+            continue
+        frame(spp, filt)  # warm
+        ms = _median_ms(lambda: frame(spp, filt), 5, ctx.sync)
+        img = frame(spp, filt)
+        rows[name] = {"relmse": round(ctx.relmse_d(img, d_sum, oracle_spp, n), 6), "ms_per_frame": round(ms, 2)}
+        log(f"[bench] {name}: {rows[name]}")
+    out.update(rows)
+    if net is not None:
+        # MFMA fraction of the network's forward at this size: logical FLOPs
+        # (3x3 convolutions + 1x1 head) over the summed kernel time
+        net.denoise_gbuffer_d(d_pix, d_n, d_p, d_a, W, H, d_out)
+        ctx.sync()
+        ctx.enable_kernel_timing(True)
+        reps = 5
+        for _ in range(reps):
+            net.denoise_gbuffer_d(d_pix, d_n, d_p, d_a, W, H, d_out)
+        ctx.sync()
+        conv_n, conv_ms = ctx.kernel_time("cnn_conv_fwd")
+        tot = 0.0
+        for k in ("cnn_inputs", "cnn_pack", "cnn_conv_fwd", "cnn_pool", "cnn_up", "cnn_kpn_apply"):
+            kn, kms = ctx.kernel_time(k)
+            tot += kn * kms
+        ctx.enable_kernel_timing(False)
+        flops = pa.cnn_flops_per_pixel() * n
+        conv_total_ms = conv_n * conv_ms / reps
+        out["cnn_forward"] = {"flops_per_frame": flops, "conv_ms_per_frame": round(conv_total_ms, 3),
+                              "kernels_ms_per_frame": round(tot / reps, 3),
+                              "mfma": {"bound": "mfma", "achieved": round(flops / (conv_total_ms * 1e-3) / 1e12, 1),
+                                       "peak": 2500.0, "unit": "TFLOP/s",
+                                       "frac": round(flops / (conv_total_ms * 1e-3) / 1e12 / 2500.0, 4)}}
+        out["cnn_weights"] = os.path.relpath(weights, ROOT)
+    if out.get("cnn_4spp") and out.get("atrous_16spp"):
+        c, a = out["cnn_4spp"], out["atrous_16spp"]
+        out["cnn4_beats_atrous16"] = bool(c["relmse"] < a["relmse"] and c["ms_per_frame"] <= a["ms_per_frame"])
+    for d in (d_sum, d_pix, d_cnt, d_var, d_n, d_p, d_a, d_out, d_scr):
+        d.free()
+    if net is not None:
+        net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,11 +183,15 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-count", action="store_true", help="skip the counting run (roofline bytes)")
+    ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline's sample")
+    ap.add_argument("--oracle-spp", type=int, default=4096, help="samples of the FP64 quality oracle")
+    ap.add_argument("--no-quality", action="store_true", help="skip the relMSE / ms-per-frame leg")
+    ap.add_argument("--weights", default=None, help="trained denoiser parameters (.npy; default: the shipped ones)")
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the streaming-copy calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard by "
-              "image bands or views, see DESIGN.md)", file=sys.stderr)
+              "image bands or by views, one scene replica per GPU)", file=sys.stderr)
         sys.exit(1)
 
     import pico_amd as pa
@@ -171,10 +271,15 @@ def main():
                 "shadow_achieved_GBps": round(traversal_bytes(count, True) / (sh_n / args.steps) / (sh_ms * 1e-3) / 1e9, 1)
                 if sh_n else None}
 
+    quality = None
+    if not args.no_quality:
+        quality = quality_leg(ctx, pa, scene, cam, W, H, args.depth, args.oracle_spp, args.weights or pa.weights_path(),
+                              d_pix)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
-        cpu = cpu_baseline(scene, cam, 960, 540, 2, args.depth, 1)
+        cpu = cpu_baseline(scene, cam, W, H, args.cpu_spp, args.depth, 1)
         log(f"[bench] CPU baseline {cpu}")
 
     out = {"metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -185,7 +290,7 @@ def main():
                       "width": W, "height": H, "spp": args.spp, "max_depth": args.depth, "scene_seed": 2,
                       "unique_triangles": uniq, "instanced_triangles": inst_tris,
                       "rays_per_step": rays // args.steps, "traced_rays_per_step": traced // args.steps},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu, "quality": quality}
     if cpu:
         out["gpu_over_cpu"] = round(mrays / cpu["value"], 1)
     print(json.dumps(out), flush=True)

@@ -255,6 +255,18 @@ int  pico_hip_cnn_set_params(pico_cnn* cnn, const float* params);
 int  pico_hip_cnn_get_params(pico_cnn* cnn, float* params);
 int  pico_hip_cnn_denoise(pico_cnn* cnn, const float* inputs, uint32_t width, uint32_t height, float* out_rgb);
 int  pico_hip_cnn_denoise_d(pico_cnn* cnn, const float* d_inputs, uint32_t width, uint32_t height, float* d_out_rgb);
+/* Network input from a G-buffer: radiance, albedo, normal (packed RGB per
+ * pixel, as pico_hip_gbuffer writes them) and depth = |view-space position|,
+ * written planar [10][H][W]. */
+int  pico_hip_cnn_inputs_d(pico_hip_ctx* ctx, const float* d_rgb, const float* d_normal, const float* d_position,
+                           const float* d_albedo, uint32_t width, uint32_t height, float* d_inputs);
+/* The learned filter with the argument list of Util::atrous_denoise
+ * (Source/Util/Denoisers.hpp:9: pixels, normals, positions, diffuse), so it
+ * drops in where the a-trous filter is called (Source/Core/Scene.cpp:214-223). */
+int  pico_hip_cnn_denoise_gbuffer(pico_cnn* cnn, const float* rgb, const float* normal, const float* position,
+                                  const float* albedo, uint32_t width, uint32_t height, float* out_rgb);
+int  pico_hip_cnn_denoise_gbuffer_d(pico_cnn* cnn, const float* d_rgb, const float* d_normal, const float* d_position,
+                                    const float* d_albedo, uint32_t width, uint32_t height, float* d_out_rgb);
 /* Loss (mean relMSE of the filtered radiance vs target, FP64 reduction) and
  * FP32 gradients of every parameter for one batch (host buffers). */
 int  pico_hip_cnn_loss_grad(pico_cnn* cnn, const float* inputs, const float* targets, uint32_t batch,
@@ -293,7 +305,9 @@ int  pico_scene_cornell(uint32_t seed, uint32_t microfacet, pico_scene** out);
  * instances in a box of side `extent`, 3 quad lights and a constant sky. */
 int  pico_scene_instanced(uint32_t seed, uint32_t n_mesh_types, uint32_t n_instances, uint32_t min_tris,
                           uint32_t max_tris, float extent, uint32_t microfacet, pico_scene** out);
-/* Small randomised scenes used to train the learned denoiser. */
+/* Randomised scenes used to train the learned denoiser, by seed % 3: a
+ * jittered Cornell box, a small instanced cluster, or a 100^3 instanced field
+ * like the benchmark scene seen from a random outside viewpoint. */
 int  pico_scene_training(uint32_t seed, uint32_t microfacet, pico_scene** out);
 int  pico_scene_desc_get(const pico_scene* scene, pico_scene_desc* out);
 /* A camera that frames the scene as its generator intends. */

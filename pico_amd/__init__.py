@@ -133,6 +133,9 @@ def _declare(lib):
         "pico_hip_cnn_get_params": (i32, [vp, vp]),
         "pico_hip_cnn_denoise": (i32, [vp, vp, u32, u32, vp]),
         "pico_hip_cnn_denoise_d": (i32, [vp, vp, u32, u32, vp]),
+        "pico_hip_cnn_inputs_d": (i32, [vp, vp, vp, vp, vp, u32, u32, vp]),
+        "pico_hip_cnn_denoise_gbuffer": (i32, [vp, vp, vp, vp, vp, u32, u32, vp]),
+        "pico_hip_cnn_denoise_gbuffer_d": (i32, [vp, vp, vp, vp, vp, u32, u32, vp]),
         "pico_hip_cnn_loss_grad": (i32, [vp, vp, vp, u32, u32, u32, P(f64), vp]),
         "pico_hip_cnn_train_step_d": (i32, [vp, vp, vp, u32, u32, u32, f32, P(f64)]),
         "pico_scene_cornell": (i32, [u32, u32, P(vp)]),
@@ -261,7 +264,18 @@ class DeviceArray:
     def zero(self):
         self.ctx._check(lib().pico_hip_memset_d(self.ctx._h, self.ptr, 0, self.nbytes))
 
+    def view(self, offset, nbytes):
+        """A window [offset, offset + nbytes) of this allocation (not owned)."""
+        assert 0 <= offset and offset + nbytes <= self.nbytes
+        v = DeviceArray.__new__(DeviceArray)
+        v.ctx, v.nbytes, v._owner = self.ctx, int(nbytes), self
+        v.ptr = C.c_void_p(self.ptr.value + int(offset))
+        return v
+
     def free(self):
+        if getattr(self, "_owner", None) is not None:
+            self.ptr = None
+            return
         if self.ptr is not None and self.ptr.value:
             lib().pico_hip_free(self.ctx._h, self.ptr)
         self.ptr = None
@@ -386,6 +400,9 @@ class Context:
         self._check(lib().pico_hip_tonemap(self._h, _ptr(out), w, h))
         return out
 
+    def cnn_inputs_d(self, d_rgb, d_n, d_p, d_a, w, h, d_inputs):
+        self._check(lib().pico_hip_cnn_inputs_d(self._h, d_rgb.ptr, d_n.ptr, d_p.ptr, d_a.ptr, w, h, d_inputs.ptr))
+
     def relmse_d(self, d_img, d_ref_sum, ref_count, n_pixels):
         out = C.c_double()
         self._check(lib().pico_hip_relmse_d(self._h, d_img.ptr, d_ref_sum.ptr, float(ref_count), n_pixels, C.byref(out)))
@@ -460,6 +477,24 @@ class CNN:
     def denoise_d(self, d_in, w, h, d_out):
         self.ctx._check(lib().pico_hip_cnn_denoise_d(self._h, d_in.ptr, w, h, d_out.ptr))
 
+    def denoise_gbuffer(self, rgb, normal, position, albedo, w, h):
+        args = [_f32(x, (-1, 3)) for x in (rgb, normal, position, albedo)]
+        out = np.empty((w * h, 3), np.float32)
+        self.ctx._check(lib().pico_hip_cnn_denoise_gbuffer(self._h, *[_ptr(a) for a in args], w, h, _ptr(out)))
+        return out
+
+    def denoise_gbuffer_d(self, d_rgb, d_n, d_p, d_a, w, h, d_out):
+        self.ctx._check(lib().pico_hip_cnn_denoise_gbuffer_d(self._h, d_rgb.ptr, d_n.ptr, d_p.ptr, d_a.ptr, w, h,
+                                                            d_out.ptr))
+
+    def load(self, path=None):
+        """Parameters from a .npy file (the trained weights by default)."""
+        p = np.load(path or weights_path(), allow_pickle=False)
+        self.set_params(p)
+
+    def save(self, path):
+        np.save(path, self.get_params(), allow_pickle=False)
+
     def loss_grad(self, inputs, targets, batch, w, h):
         x = _f32(inputs).reshape(batch, 10, h * w)
         t = _f32(targets).reshape(batch, h * w, 3)
@@ -468,7 +503,24 @@ class CNN:
         self.ctx._check(lib().pico_hip_cnn_loss_grad(self._h, _ptr(x), _ptr(t), batch, w, h, C.byref(loss), _ptr(g)))
         return loss.value, g
 
-    def train_step_d(self, d_in, d_tgt, batch, w, h, lr):
+    def train_step_d(self, d_in, d_tgt, batch, w, h, lr, want_loss=True):
         loss = C.c_double()
-        self.ctx._check(lib().pico_hip_cnn_train_step_d(self._h, d_in.ptr, d_tgt.ptr, batch, w, h, lr, C.byref(loss)))
-        return loss.value
+        self.ctx._check(lib().pico_hip_cnn_train_step_d(self._h, d_in.ptr, d_tgt.ptr, batch, w, h, lr,
+                                                       C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+
+# the network's convolutions: (input channels, output channels, taps, U-Net
+# level); level l runs at 1/4^l of the pixels
+CNN_LAYERS = [(10, 32, 9, 0), (32, 32, 9, 0), (32, 64, 9, 1), (64, 64, 9, 1), (64, 128, 9, 2), (128, 128, 9, 2),
+              (192, 64, 9, 1), (64, 64, 9, 1), (96, 32, 9, 0), (32, 32, 9, 0), (32, 25, 1, 0)]
+
+
+def cnn_flops_per_pixel():
+    """Multiply-add FLOPs of one forward pass per full-resolution pixel."""
+    return sum(2 * taps * cin * cout / 4 ** lvl for cin, cout, taps, lvl in CNN_LAYERS)
+
+
+def weights_path():
+    """The trained denoiser's parameters shipped with the package."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "kpn.npy")

@@ -213,6 +213,20 @@ __global__ void k_pack(const float* __restrict__ in, bf16* __restrict__ x, int B
     }
 }
 
+// G-buffer (packed RGB per pixel: radiance, world normal, view-space
+// position, albedo) -> planar [10][H][W] network input: radiance, albedo,
+// normal, distance to the camera (|position|; the far plane for misses)
+__global__ void k_inputs(const float* __restrict__ rgb, const float* __restrict__ nrm, const float* __restrict__ pos,
+                         const float* __restrict__ alb, float* __restrict__ out, size_t px) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < px; i += (size_t)gridDim.x * blockDim.x) {
+        const float* src[3] = {rgb, alb, nrm};
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) out[(3 * k + c) * px + i] = src[k][3 * i + c];
+        const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        out[9 * px + i] = sqrtf(x * x + y * y + z * z);
+    }
+}
+
 // 2x2 average pool: src level (H, W) slice -> dst (H/2, W/2) with C channels
 __global__ void k_pool(const bf16* __restrict__ s, int s_stride, int s_off, bf16* __restrict__ d, int C, int B, int H, int W) {
     const int h = H / 2, w = W / 2;
@@ -639,6 +653,44 @@ int pico_hip_cnn_denoise(pico_cnn* n, const float* in, uint32_t w, uint32_t h, f
     PICO_HIP(n->c, hipMemcpyAsync(d, in, px * 40, hipMemcpyHostToDevice, n->c->stream));
     if ((rc = pico_hip_cnn_denoise_d(n, d, w, h, d + 10 * px))) return rc;
     PICO_HIP(n->c, hipMemcpyAsync(out, d + 10 * px, px * 12, hipMemcpyDeviceToHost, n->c->stream));
+    PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_cnn_inputs_d(pico_hip_ctx* c, const float* d_rgb, const float* d_normal, const float* d_position,
+                          const float* d_albedo, uint32_t w, uint32_t h, float* d_inputs) {
+    if (!c) return PICO_ERR_ARG;
+    if (!d_rgb || !d_normal || !d_position || !d_albedo || !d_inputs) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    const size_t px = (size_t)w * h;
+    if (px) PICO_LAUNCH(c, "cnn_inputs", k_inputs, grid_for(px), 256, 0, d_rgb, d_normal, d_position, d_albedo, d_inputs, px);
+    return PICO_OK;
+}
+
+int pico_hip_cnn_denoise_gbuffer_d(pico_cnn* n, const float* d_rgb, const float* d_normal, const float* d_position,
+                                   const float* d_albedo, uint32_t w, uint32_t h, float* d_out) {
+    if (!n) return PICO_ERR_ARG;
+    int rc = check_dims(n, 1, w, h);
+    if (rc) return rc;
+    float* d = (float*)scratch(n->c, "cnn_frame_in", (size_t)w * h * 40 + 64);
+    if (!d) return fail(n->c, PICO_ERR_NOMEM, "device allocation failed");
+    if ((rc = pico_hip_cnn_inputs_d(n->c, d_rgb, d_normal, d_position, d_albedo, w, h, d))) return rc;
+    return pico_hip_cnn_denoise_d(n, d, w, h, d_out);
+}
+
+int pico_hip_cnn_denoise_gbuffer(pico_cnn* n, const float* rgb, const float* normal, const float* position,
+                                 const float* albedo, uint32_t w, uint32_t h, float* out) {
+    if (!n) return PICO_ERR_ARG;
+    if (!rgb || !normal || !position || !albedo || !out) return fail(n->c, PICO_ERR_ARG, "NULL buffer");
+    int rc = check_dims(n, 1, w, h);
+    if (rc) return rc;
+    const size_t px = (size_t)w * h;
+    float* d = (float*)scratch(n->c, "cnn_frame_io", px * 60 + 64);
+    if (!d) return fail(n->c, PICO_ERR_NOMEM, "device allocation failed");
+    const float* src[4] = {rgb, normal, position, albedo};
+    for (int k = 0; k < 4; ++k)
+        PICO_HIP(n->c, hipMemcpyAsync(d + 3 * k * px, src[k], px * 12, hipMemcpyHostToDevice, n->c->stream));
+    if ((rc = pico_hip_cnn_denoise_gbuffer_d(n, d, d + 3 * px, d + 6 * px, d + 9 * px, w, h, d + 12 * px))) return rc;
+    PICO_HIP(n->c, hipMemcpyAsync(out, d + 12 * px, px * 12, hipMemcpyDeviceToHost, n->c->stream));
     PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
     return PICO_OK;
 }

@@ -361,15 +361,28 @@ int pico_scene_training(uint32_t seed, uint32_t microfacet, pico_scene** out) {
     if (!out || microfacet > PICO_MICROFACET_GGX) return PICO_ERR_ARG;
     auto* s = new pico_scene();
     Rng rng(seed, 3);
-    if (seed % 2 == 0) {
+    if (seed % 3 == 0) {
         cornell_into(s, rng, true);
-    } else {
+    } else if (seed % 3 == 1) {
         instanced_into(s, rng, 4, 24 + rng.next() % 40, 200, 2000, 12.0f, 1 + rng.next() % 3, rng.uniform(0.0f, 0.3f));
-        // camera inside the cluster's bounding box front, looking roughly at the centre
+        // camera in front of the cluster, looking roughly at its centre
         const float a = rng.uniform(-0.5f, 0.5f);
         s->cam.position[0] = 6.0f + 10.0f * std::sin(a);
         s->cam.position[2] = 6.0f - 10.0f * std::cos(a);
         F3 d = normalize(F3{6.0f - s->cam.position[0], 0.0f, 6.0f - s->cam.position[2]});
+        s->cam.direction[0] = d.x; s->cam.direction[1] = d.y; s->cam.direction[2] = d.z;
+    } else {
+        // a field of the size of the benchmark scene (100^3 box, 2k-20k
+        // triangle meshes, three quad lights, dim sky) seen from a random
+        // point 60-110 units from its centre, looking near the centre
+        instanced_into(s, rng, 8, 256 + rng.next() % 769, 2000, 20000, 100.0f, 3, rng.uniform(0.05f, 0.2f));
+        F3 dir = normalize(F3{rng.gauss(), 0.5f * rng.gauss(), rng.gauss()});
+        const float r = rng.uniform(60.0f, 110.0f);
+        const F3 c = {50.0f, 50.0f, 50.0f};
+        const F3 p = c + dir * r;
+        const F3 t = c + F3{rng.uniform(-15.0f, 15.0f), rng.uniform(-15.0f, 15.0f), rng.uniform(-15.0f, 15.0f)};
+        const F3 d = normalize(t - p);
+        s->cam.position[0] = p.x; s->cam.position[1] = p.y; s->cam.position[2] = p.z;
         s->cam.direction[0] = d.x; s->cam.direction[1] = d.y; s->cam.direction[2] = d.z;
     }
     s->microfacet = microfacet;
