@@ -23,6 +23,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -121,22 +123,40 @@ def quality_leg(ctx, pa, scene, cam, W, H, depth, oracle_spp, weights, d_frame):
             return d_pix
         ctx.gbuffer_d(cam, W, H, d_n, d_p, d_a)
         if filt == "atrous":
+            ctx.atrous_fixed_d(d_pix, d_n, d_p, d_a, W, H, d_scr, d_out)
+        elif filt == "atrous_ref":
             ctx.atrous_d(d_pix, d_n, d_p, d_a, W, H, d_scr, d_out)
         else:
             net.denoise_gbuffer_d(d_pix, d_n, d_p, d_a, W, H, d_out)
         return d_out
 
+    ref = None
     rows = {}
     for name, spp, filt in (("raw_4spp", 4, None), ("raw_16spp", 16, None), ("atrous_4spp", 4, "atrous"),
-                            ("atrous_16spp", 16, "atrous"), ("cnn_4spp", 4, "cnn")):
+                            ("atrous_16spp", 16, "atrous"), ("atrous_ref_4spp", 4, "atrous_ref"),
+                            ("atrous_ref_16spp", 16, "atrous_ref"), ("cnn_4spp", 4, "cnn")):
         if filt == "cnn" and net is None:
             rows[name] = None
             continue
         frame(spp, filt)  # warm
         ms = _median_ms(lambda: frame(spp, filt), 5, ctx.sync)
         img = frame(spp, filt)
-        rows[name] = {"relmse": round(ctx.relmse_d(img, d_sum, oracle_spp, n), 6), "ms_per_frame": round(ms, 2)}
+        if filt == "atrous_ref":
+            # the reference's tile-local widths divide by zero on tiles with
+            # no luminance or no depth range (NaN, as in the reference): the
+            # error is taken over the finite pixels and the rest are counted
+            if ref is None:
+                ref = d_sum.download(np.float64, 3 * n).reshape(n, 3) / oracle_spp
+            x = img.download(np.float32, 3 * n).reshape(n, 3).astype(np.float64)
+            ok = np.isfinite(x).all(axis=1)
+            e = (x[ok] - ref[ok]) ** 2 / (ref[ok] ** 2 + 1e-2)
+            rows[name] = {"relmse_finite": round(float(e.mean()), 6), "nonfinite_pixels": int((~ok).sum()),
+                          "ms_per_frame": round(ms, 2)}
+        else:
+            rows[name] = {"relmse": round(ctx.relmse_d(img, d_sum, oracle_spp, n), 6), "ms_per_frame": round(ms, 2)}
         log(f"[bench] {name}: {rows[name]}")
+    out["atrous_fixed_sigmas"] = {"normal": pa.ATROUS_FIXED_SIGMAS[0], "depth_rel": pa.ATROUS_FIXED_SIGMAS[1],
+                                  "albedo": pa.ATROUS_FIXED_SIGMAS[2], "colour": pa.ATROUS_FIXED_SIGMAS[3]}
     out.update(rows)
     if net is not None:
         # MFMA fraction of the network's forward at this size: logical FLOPs

@@ -233,6 +233,17 @@ int  pico_hip_atrous_d(pico_hip_ctx* ctx, const float* d_rgb, const float* d_nor
                        uint32_t tile_h, uint32_t levels, float* d_scratch, float* d_out);
 
 /* Extended Reinhard with a global luminance white point, in place. */
+/* Labelled mode with no counterpart in the reference: edge-avoiding a-trous
+ * (Dammertz et al., HPG 2010) with FIXED edge-stopping widths in place of the
+ * reference's tile-local ones (Source/Util/Denoisers.cpp:45-63), B3-spline
+ * taps, `levels` passes at dilation 2^l.  Widths: normal distance, depth
+ * difference relative to the centre's |position|, albedo distance, and
+ * colour distance (halved every level); 0 drops a term.  The pixel's own tap
+ * always has weight > 0, so no tile can divide by zero. */
+int  pico_hip_atrous_fixed_d(pico_hip_ctx* ctx, const float* d_rgb, const float* d_normal, const float* d_position,
+                             const float* d_albedo, uint32_t width, uint32_t height, uint32_t levels,
+                             float sigma_normal, float sigma_depth, float sigma_albedo, float sigma_color,
+                             float* d_scratch, float* d_out);
 int  pico_hip_tonemap(pico_hip_ctx* ctx, float* rgb, uint32_t width, uint32_t height);
 
 /* relMSE = mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2) with the

@@ -60,6 +60,7 @@ def cpu_lib():
             "cpuref_render_samples": (i32, [vp, P(Camera), P(RenderParams), u32, u32, u32, vp, vp]),
             "cpuref_gbuffer": (None, [vp, P(Camera), u32, u32, vp, vp, vp]),
             "cpuref_atrous": (None, [vp, vp, vp, vp, u32, u32, u32, u32, u32, vp]),
+            "cpuref_atrous_fixed": (None, [vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, f32, vp]),
             "cpuref_tonemap": (None, [vp, u32, u32]),
             "cpuref_relmse": (f64, [vp, vp, f64, u32]),
             "cpuref_dist_sample": (None, [u32, u32, vp, vp, vp, vp]),
@@ -181,6 +182,13 @@ def atrous(rgb, normal, position, albedo, w, h, tile=(16, 16), levels=5):
     args = [_f32(x, (-1, 3)) for x in (rgb, normal, position, albedo)]
     out = np.empty((w * h, 3), np.float32)
     cpu_lib().cpuref_atrous(*[_p(a) for a in args], w, h, tile[0], tile[1], levels, _p(out))
+    return out
+
+
+def atrous_fixed(rgb, normal, position, albedo, w, h, levels=5, sigmas=(0.5, 0.05, 0.1, 0.0)):
+    args = [_f32(x, (-1, 3)) for x in (rgb, normal, position, albedo)]
+    out = np.empty((w * h, 3), np.float32)
+    cpu_lib().cpuref_atrous_fixed(*[_p(a) for a in args], w, h, levels, *[C.c_float(s) for s in sigmas], _p(out))
     return out
 
 

@@ -1181,6 +1181,55 @@ void cpuref_tonemap(float* rgb, uint32_t w, uint32_t h) {
         }
 }
 
+// Fixed-width edge-avoiding a-trous (Dammertz et al., HPG 2010): no
+// counterpart in the reference, which uses tile-local widths instead
+// (Source/Util/Denoisers.cpp:45-63).  B3-spline taps (1,4,6,4,1)/16 per axis
+// at dilation 2^l, neighbours clamped to the image, tap weight
+// h(u)h(v) exp(-(|dn|^2/sn^2 + (dd/d_c)^2/sd^2 + |da|^2/sa^2 + |dc|^2/(sc 2^-l)^2)),
+// d = |position|, a width of 0 dropping its term.  Sums in FP64.
+void cpuref_atrous_fixed(const float* rgb, const float* normal, const float* position, const float* albedo, uint32_t w,
+                         uint32_t h, uint32_t levels, float sn, float sd, float sa, float sc, float* out) {
+    const size_t n = (size_t)w * h;
+    std::vector<float> a(rgb, rgb + 3 * n), b(3 * n), dist(n);
+    for (size_t i = 0; i < n; ++i) {
+        const double x = position[3 * i], y = position[3 * i + 1], z = position[3 * i + 2];
+        dist[i] = (float)std::sqrt(x * x + y * y + z * z);
+    }
+    const double hb[5] = {1.0 / 16, 1.0 / 4, 3.0 / 8, 1.0 / 4, 1.0 / 16};
+    auto inv2 = [](double s) { return s > 0.0 ? 1.0 / (s * s) : 0.0; };
+    for (uint32_t l = 0; l < levels; ++l) {
+        const int step = 1 << l;
+        const double isn2 = inv2(sn), isd2 = inv2(sd), isa2 = inv2(sa), isc2 = inv2((double)sc / step);
+        for (int y = 0; y < (int)h; ++y)
+            for (int x = 0; x < (int)w; ++x) {
+                const size_t i = (size_t)y * w + x;
+                double acc[3] = {0, 0, 0}, ws = 0.0;
+                const double dc2 = (double)dist[i] * dist[i];
+                for (int v = 0; v < 5; ++v)
+                    for (int u = 0; u < 5; ++u) {
+                        const int xx = std::min(std::max(x + (u - 2) * step, 0), (int)w - 1);
+                        const int yy = std::min(std::max(y + (v - 2) * step, 0), (int)h - 1);
+                        const size_t j = (size_t)yy * w + xx;
+                        double en = 0, ea = 0, ec = 0;
+                        for (int k = 0; k < 3; ++k) {
+                            const double dn = (double)normal[3 * i + k] - normal[3 * j + k];
+                            const double da = (double)albedo[3 * i + k] - albedo[3 * j + k];
+                            const double dcl = (double)a[3 * i + k] - a[3 * j + k];
+                            en += dn * dn; ea += da * da; ec += dcl * dcl;
+                        }
+                        const double dd = (double)dist[i] - dist[j];
+                        const double e = en * isn2 + (dc2 > 0 ? dd * dd / dc2 : 0.0) * isd2 + ea * isa2 + ec * isc2;
+                        const double wt = hb[u] * hb[v] * std::exp(-e);
+                        for (int k = 0; k < 3; ++k) acc[k] += wt * a[3 * j + k];
+                        ws += wt;
+                    }
+                for (int k = 0; k < 3; ++k) b[3 * i + k] = (float)(acc[k] / ws);
+            }
+        std::swap(a, b);
+    }
+    std::memcpy(out, a.data(), 12 * n);
+}
+
 double cpuref_relmse(const float* img, const double* ref_sum, double ref_count, uint32_t n_pixels) {
     double acc = 0.0;
     for (size_t i = 0; i < (size_t)n_pixels * 3; ++i) {

@@ -58,6 +58,9 @@ void cpuref_gbuffer(const cpuref_scene* s, const pico_camera* cam, uint32_t w, u
 
 void cpuref_atrous(const float* rgb, const float* normal, const float* position, const float* albedo,
                    uint32_t w, uint32_t h, uint32_t tile_w, uint32_t tile_h, uint32_t levels, float* out);
+void cpuref_atrous_fixed(const float* rgb, const float* normal, const float* position, const float* albedo,
+                         uint32_t w, uint32_t h, uint32_t levels, float sigma_normal, float sigma_depth,
+                         float sigma_albedo, float sigma_color, float* out);
 void cpuref_tonemap(float* rgb, uint32_t w, uint32_t h);
 double cpuref_relmse(const float* img, const double* ref_sum, double ref_count, uint32_t n_pixels);
 

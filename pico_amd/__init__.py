@@ -33,6 +33,8 @@ PICO_ERR_NOMEM = -6
 BSDF_DIFFUSE, BSDF_SPECULAR, BSDF_DIELECTRIC, BSDF_LIGHT, BSDF_DELTA = range(5)
 MICROFACET_BECKMANN_ALL, MICROFACET_GGX = 0, 1
 RENDER_RESUME, RENDER_COUNT = 1, 2
+# fixed-width a-trous: normal, relative depth, albedo, colour (0 = term off)
+ATROUS_FIXED_SIGMAS = (0.5, 0.05, 0.1, 0.0)
 
 
 class PicoError(RuntimeError):
@@ -121,6 +123,7 @@ def _declare(lib):
         "pico_hip_atrous": (i32, [vp, vp, vp, vp, vp, u32, u32, u32, u32, u32, vp]),
         "pico_hip_atrous_d": (i32, [vp, vp, vp, vp, vp, u32, u32, u32, u32, u32, vp, vp]),
         "pico_hip_tonemap": (i32, [vp, vp, u32, u32]),
+        "pico_hip_atrous_fixed_d": (i32, [vp, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, f32, vp, vp]),
         "pico_hip_relmse_d": (i32, [vp, vp, vp, f64, u32, P(f64)]),
         "pico_hip_render_views_d": (i32, [vp, P(View), u32, u32, P(Stats)]),
         "pico_hip_set_tuning": (i32, [vp, u32, u32]),
@@ -395,6 +398,10 @@ class Context:
         self._check(lib().pico_hip_atrous_d(self._h, d_rgb.ptr, d_n.ptr, d_p.ptr, d_a.ptr, w, h, tile[0], tile[1], levels,
                                             d_scratch.ptr, d_out.ptr))
 
+    def atrous_fixed_d(self, d_rgb, d_n, d_p, d_a, w, h, d_scratch, d_out, levels=5, sigmas=ATROUS_FIXED_SIGMAS):
+        self._check(lib().pico_hip_atrous_fixed_d(self._h, d_rgb.ptr, d_n.ptr, d_p.ptr, d_a.ptr, w, h, levels, *sigmas,
+                                                  d_scratch.ptr, d_out.ptr))
+
     def tonemap(self, rgb, w, h):
         out = _f32(rgb, (-1, 3)).copy()
         self._check(lib().pico_hip_tonemap(self._h, _ptr(out), w, h))
@@ -445,9 +452,10 @@ class CNN:
         self.n_params = lib().pico_hip_cnn_num_params()
 
     def close(self):
-        if self._h:
+        # a closed context has already destroyed the instances created on it
+        if self._h and self.ctx._h:
             lib().pico_hip_cnn_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:

@@ -167,6 +167,8 @@ void pico_hip_destroy(pico_hip_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    const std::set<pico_cnn*> cnns = c->cnns;
+    for (pico_cnn* n : cnns) pico_hip_cnn_destroy(n);
     free_scene(c);
     for (auto& kv : c->bufs)
         if (kv.second.p) hipFree(kv.second.p);

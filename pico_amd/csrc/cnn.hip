@@ -584,6 +584,7 @@ int pico_hip_cnn_create(pico_hip_ctx* c, uint32_t mw, uint32_t mh, uint32_t mb, 
         pico_hip_cnn_destroy(n);
         return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     }
+    c->cnns.insert(n);
     *out = n;
     return pico_hip_cnn_init(n, 1);
 }
@@ -592,6 +593,7 @@ void pico_hip_cnn_destroy(pico_cnn* n) {
     if (!n) return;
     hipStreamSynchronize(n->c->stream);
     for (void* p : n->allocs) hipFree(p);
+    n->c->cnns.erase(n);
     delete n;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,8 @@ struct pico_hip_ctx {
     // timing
     bool timing = false;
     std::map<std::string, KernelTimes> timers;
+    // learned-denoiser instances created on this context (destroyed with it)
+    std::set<pico_cnn*> cnns;
 };
 
 namespace pico {

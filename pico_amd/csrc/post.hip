@@ -103,6 +103,55 @@ __global__ __launch_bounds__(kTileThreads) void k_atrous_level(const float* __re
     }
 }
 
+// Labelled mode with no counterpart in the reference: the edge-avoiding
+// a-trous transform of Dammertz et al. (HPG 2010) with FIXED edge-stopping
+// widths in place of the reference's tile-local constants, and the B3-spline
+// taps (1, 4, 6, 4, 1) / 16 per axis.  Tap weight:
+//   h(u) h(v) exp(-( |dn|^2 / sn^2 + (dd / d_c)^2 / sd^2 + |da|^2 / sa^2
+//                   + |dc|^2 / (sc 2^-l)^2 ))
+// with n the normal, d = |view-space position| (relative difference), a the
+// albedo and c the level's input colour; a width of 0 drops its term.
+// Neighbours are clamped to the image like the reference's taps.
+__constant__ float kB3[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+
+__global__ void k_guide(const float* __restrict__ nrm, const float* __restrict__ pos, float4* __restrict__ g, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const f3 nn = ld3(nrm + 3 * (size_t)i), p = ld3(pos + 3 * (size_t)i);
+        g[i] = make_float4(nn.x, nn.y, nn.z, sqrtf(p.x * p.x + p.y * p.y + p.z * p.z));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_atrous_fixed(const float* __restrict__ src, const float4* __restrict__ guide,
+                                                      const float* __restrict__ alb, float* __restrict__ dst, int w, int h,
+                                                      int scale, float isn2, float isd2, float isa2, float isc2) {
+    const uint32_t n = (uint32_t)w * (uint32_t)h;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int x = (int)(i % (uint32_t)w), y = (int)(i / (uint32_t)w);
+        const f3 cc = ld3(src + 3 * (size_t)i), ca = ld3(alb + 3 * (size_t)i);
+        const float4 cg = guide[i];
+        const float idd2 = isd2 / (cg.w * cg.w + 1e-30f);
+        f3 res = mk(0.0f, 0.0f, 0.0f);
+        float wsum = 0.0f;
+        for (int v = 0; v < 5; ++v) {
+            const int yy = min(max(y + (v - 2) * scale, 0), h - 1);
+            for (int u = 0; u < 5; ++u) {
+                const int xx = min(max(x + (u - 2) * scale, 0), w - 1);
+                const size_t j = (size_t)yy * w + xx;
+                const float4 g = guide[j];
+                const f3 a = ld3(alb + 3 * j), c = ld3(src + 3 * j);
+                const f3 dn = mk(cg.x - g.x, cg.y - g.y, cg.z - g.z), da = ca - a, dc = cc - c;
+                const float dd = cg.w - g.w;
+                const float e = dot(dn, dn) * isn2 + dd * dd * idd2 + dot(da, da) * isa2 + dot(dc, dc) * isc2;
+                const float wt = kB3[u] * kB3[v] * expf(-e);
+                res = res + c * wt;
+                wsum += wt;
+            }
+        }
+        const f3 o = res / wsum;
+        dst[3 * (size_t)i] = o.x; dst[3 * (size_t)i + 1] = o.y; dst[3 * (size_t)i + 2] = o.z;
+    }
+}
+
 __global__ void k_lum_max(const float* __restrict__ rgb, uint32_t n, uint32_t* white_bits) {
     float m = 0.0f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -190,6 +239,38 @@ int pico_hip_atrous_d(pico_hip_ctx* c, const float* rgb, const float* nrm, const
         float* dst = ((levels - 1 - l) % 2 == 0) ? out : scratch_buf;
         PICO_LAUNCH(c, "atrous", k_atrous_level, tiles_x * tiles_y, kTileThreads, 0, src, nrm, pos, alb, plen, dst, (int)w,
                     (int)h, (int)tw, (int)th, tiles_y, 1 << l);
+        src = dst;
+    }
+    return PICO_OK;
+}
+
+int pico_hip_atrous_fixed_d(pico_hip_ctx* c, const float* rgb, const float* nrm, const float* pos, const float* alb,
+                            uint32_t w, uint32_t h, uint32_t levels, float sigma_normal, float sigma_depth,
+                            float sigma_albedo, float sigma_color, float* scratch_buf, float* out) {
+    if (!c) return PICO_ERR_ARG;
+    if (!rgb || !nrm || !pos || !alb || !scratch_buf || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if (w == 0 || h == 0) return fail(c, PICO_ERR_ARG, "sizes must be positive");
+    if ((uint64_t)w * h > (1ull << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
+    if (levels > 16) return fail(c, PICO_ERR_LIMIT, "at most 16 levels");
+    if (!(sigma_normal >= 0.0f && sigma_depth >= 0.0f && sigma_albedo >= 0.0f && sigma_color >= 0.0f))
+        return fail(c, PICO_ERR_ARG, "edge-stopping widths must be >= 0");
+    const size_t n = (size_t)w * h;
+    if (levels == 0) {
+        PICO_HIP(c, hipMemcpyAsync(out, rgb, 12 * n, hipMemcpyDeviceToDevice, c->stream));
+        return PICO_OK;
+    }
+    float4* guide = (float4*)scratch(c, "atrous_guide", 16 * n + 64);
+    if (!guide) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)c->num_cus * 16);
+    PICO_LAUNCH(c, "atrous_guide", k_guide, grid, 256, 0, nrm, pos, guide, (uint32_t)n);
+    auto inv2 = [](float s) { return s > 0.0f ? 1.0f / (s * s) : 0.0f; };
+    const float isn2 = inv2(sigma_normal), isd2 = inv2(sigma_depth), isa2 = inv2(sigma_albedo);
+    const float* src = rgb;
+    for (uint32_t l = 0; l < levels; ++l) {
+        float* dst = ((levels - 1 - l) % 2 == 0) ? out : scratch_buf;
+        const float sc = sigma_color / (float)(1u << l);
+        PICO_LAUNCH(c, "atrous_fixed", k_atrous_fixed, grid, 256, 0, src, guide, alb, dst, (int)w, (int)h, 1 << l, isn2,
+                    isd2, isa2, inv2(sc));
         src = dst;
     }
     return PICO_OK;

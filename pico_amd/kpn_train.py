@@ -127,8 +127,10 @@ def main(argv=None):
     t_train = time.time() - t1
     net.save(args.out)
 
-    # held-out pairs: raw 4 spp, a-trous and the network against the target
-    res = {"raw": [], "atrous": [], "cnn": []}
+    # held-out pairs: raw 4 spp, the two a-trous modes and the network
+    # against the target (the reference-mode filter's NaN pixels, from its
+    # tile-local widths, are left out of its error and counted)
+    res = {"raw": [], "atrous_fixed": [], "atrous_ref_finite": [], "atrous_ref_nonfinite_frac": [], "cnn": []}
     d_out, d_scr = ctx.alloc(ts), ctx.alloc(ts)
     for k in range(ntrain, N):
         x = X.view(k * xs, xs)
@@ -138,14 +140,20 @@ def main(argv=None):
         res["raw"].append(relmse(rgb, tgt))
         net.denoise_d(x, S, S, d_out)
         res["cnn"].append(relmse(d_out.download(np.float32, 3 * px).reshape(px, 3), tgt))
-        # a-trous on the same G-buffer (positions re-rendered: the input keeps only the distance)
         scene = pa.Scene.training(args.seed0 + k, pa.MICROFACET_GGX)
         ctx.upload_scene(scene)
         cam = scene.camera(1.0)
-        n, p, a = ctx.gbuffer(cam, S, S)
-        res["atrous"].append(relmse(ctx.atrous(rgb, n, p, a, S, S), tgt))
+        ctx.gbuffer_d(cam, S, S, pr.d_n, pr.d_p, pr.d_a)
+        pr.d_pix.upload(rgb)
+        ctx.atrous_fixed_d(pr.d_pix, pr.d_n, pr.d_p, pr.d_a, S, S, d_scr, d_out)
+        res["atrous_fixed"].append(relmse(d_out.download(np.float32, 3 * px).reshape(px, 3), tgt))
+        ctx.atrous_d(pr.d_pix, pr.d_n, pr.d_p, pr.d_a, S, S, d_scr, d_out)
+        ar = d_out.download(np.float32, 3 * px).reshape(px, 3)
+        ok = np.isfinite(ar).all(axis=1)
+        res["atrous_ref_finite"].append(relmse(ar[ok], tgt[ok]) if ok.any() else float("nan"))
+        res["atrous_ref_nonfinite_frac"].append(float((~ok).mean()))
         del scene
-    summary = {k: float(np.mean(v)) for k, v in res.items()}
+    summary = {k: float(np.nanmean(v)) for k, v in res.items()}
     out = {"scenes": N, "val": args.val, "size": S, "spp_in": args.spp_in, "spp_target": args.spp_target,
            "steps": args.steps, "batch": args.batch, "lr": args.lr, "data_s": round(t_data, 1),
            "train_s": round(t_train, 1), "ms_per_step": round(1e3 * t_train / max(1, args.steps), 2),

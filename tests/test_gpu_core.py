@@ -109,3 +109,21 @@ def test_atrous_matches_oracle(ctx):
         g = ctx.atrous(rgb, n, pos, alb, w, h, tile, 5)
         c = oracle.atrous(rgb, n, pos, alb, w, h, tile, 5)
         np.testing.assert_allclose(g, c, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("sigmas", [(0.5, 0.05, 0.1, 0.0), (0.3, 0.1, 0.2, 0.8), (0.0, 0.0, 0.0, 0.0)])
+def test_atrous_fixed_matches_oracle(ctx, sigmas):
+    rng = np.random.default_rng(6)
+    w, h = 100, 75
+    rgb = rng.gamma(2.0, 0.5, size=(w * h, 3)).astype(np.float32)
+    n = rng.normal(size=(w * h, 3)).astype(np.float32)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    pos = rng.normal(size=(w * h, 3)).astype(np.float32) * 3
+    alb = rng.uniform(0, 1, size=(w * h, 3)).astype(np.float32)
+    d = [ctx.alloc(12 * w * h) for _ in range(6)]
+    for arr, src in zip(d, (rgb, n, pos, alb)):
+        arr.upload(src)
+    ctx.atrous_fixed_d(*d[:4], w, h, d[4], d[5], levels=5, sigmas=sigmas)
+    g = d[5].download(np.float32, 3 * w * h).reshape(-1, 3)
+    c = oracle.atrous_fixed(rgb, n, pos, alb, w, h, 5, sigmas)
+    np.testing.assert_allclose(g, c, rtol=1e-4, atol=1e-6)
