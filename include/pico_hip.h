@@ -147,6 +147,7 @@ typedef struct pico_stats {
 typedef struct pico_hip_ctx pico_hip_ctx;
 
 int  pico_hip_create(int device, pico_hip_ctx** out);
+/* Frees the context's device memory, its scene and every pico_cnn created on it. */
 void pico_hip_destroy(pico_hip_ctx* ctx);
 const char* pico_hip_last_error(const pico_hip_ctx* ctx);
 int  pico_hip_synchronize(pico_hip_ctx* ctx);
@@ -194,7 +195,10 @@ int  pico_hip_primary_ids(pico_hip_ctx* ctx, const pico_camera* cam, uint32_t wi
 /* Path-traces the frame and accumulates into the reference's buffers:
  * pixels / variance 3 x f32 per pixel, sample_count u32 per pixel (running
  * mean/variance update and early-out of Source/Core/Scene.cpp:163-197).
- * cancel (may be NULL) is polled between wavefront passes. */
+ * cancel (may be NULL; may be written by another thread) is read before each
+ * chunk of samples, after the previous chunk has completed on the device:
+ * a non-zero value stops the render with PICO_ERR_CANCELLED, the pixels
+ * accumulated so far kept (Scene.cpp:158 polls `should_quit` per pixel). */
 int  pico_hip_render(pico_hip_ctx* ctx, const pico_camera* cam, const pico_render_params* params,
                      float* pixels, uint32_t* sample_count, float* variance,
                      const volatile int32_t* cancel, pico_stats* stats);
@@ -282,6 +286,10 @@ int  pico_hip_cnn_denoise_gbuffer_d(pico_cnn* cnn, const float* d_rgb, const flo
  * FP32 gradients of every parameter for one batch (host buffers). */
 int  pico_hip_cnn_loss_grad(pico_cnn* cnn, const float* inputs, const float* targets, uint32_t batch,
                             uint32_t width, uint32_t height, double* loss, float* grads);
+/* Global gradient-norm clipping of the training step (default 1.0, 0 = off)
+ * and the pre-clipping norm of the last step. */
+int  pico_hip_cnn_set_clip(pico_cnn* cnn, float max_grad_norm);
+int  pico_hip_cnn_last_grad_norm(pico_cnn* cnn, double* out);
 /* One Adam step on a device-resident batch. */
 int  pico_hip_cnn_train_step_d(pico_cnn* cnn, const float* d_inputs, const float* d_targets, uint32_t batch,
                                uint32_t width, uint32_t height, float lr, double* loss);

@@ -139,6 +139,8 @@ def _declare(lib):
         "pico_hip_cnn_inputs_d": (i32, [vp, vp, vp, vp, vp, u32, u32, vp]),
         "pico_hip_cnn_denoise_gbuffer": (i32, [vp, vp, vp, vp, vp, u32, u32, vp]),
         "pico_hip_cnn_denoise_gbuffer_d": (i32, [vp, vp, vp, vp, vp, u32, u32, vp]),
+        "pico_hip_cnn_set_clip": (i32, [vp, f32]),
+        "pico_hip_cnn_last_grad_norm": (i32, [vp, P(f64)]),
         "pico_hip_cnn_loss_grad": (i32, [vp, vp, vp, u32, u32, u32, P(f64), vp]),
         "pico_hip_cnn_train_step_d": (i32, [vp, vp, vp, u32, u32, u32, f32, P(f64)]),
         "pico_scene_cornell": (i32, [u32, u32, P(vp)]),
@@ -510,6 +512,14 @@ class CNN:
         loss = C.c_double()
         self.ctx._check(lib().pico_hip_cnn_loss_grad(self._h, _ptr(x), _ptr(t), batch, w, h, C.byref(loss), _ptr(g)))
         return loss.value, g
+
+    def set_clip(self, max_grad_norm):
+        self.ctx._check(lib().pico_hip_cnn_set_clip(self._h, max_grad_norm))
+
+    def last_grad_norm(self):
+        out = C.c_double()
+        self.ctx._check(lib().pico_hip_cnn_last_grad_norm(self._h, C.byref(out)))
+        return out.value
 
     def train_step_d(self, d_in, d_tgt, batch, w, h, lr, want_loss=True):
         loss = C.c_double()

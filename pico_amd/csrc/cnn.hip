@@ -362,10 +362,21 @@ __global__ void k_relayout(const float* __restrict__ w, const float* __restrict_
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npad; i += gridDim.x * blockDim.x) bpad[i] = i < cout ? bias[i] : 0.0f;
 }
 
+// squared L2 norm of the gradient (FP64 reduction)
+__global__ void k_sumsq(const float* __restrict__ g, int n, double* out) {
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += (double)g[i] * g[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+// Adam on the gradient scaled to a global L2 norm of at most `clip` (0: no clipping)
 __global__ void k_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float c1, float c2,
-                       float eps) {
+                       float eps, const double* sumsq, float clip) {
+    const double norm = sqrt(*sumsq);
+    const float scale = (clip > 0.0f && norm > (double)clip) ? (float)((double)clip / norm) : 1.0f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float gi = g[i];
+        const float gi = g[i] * scale;
         const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
         m[i] = mi;
         v[i] = vi;
@@ -391,6 +402,8 @@ struct pico_cnn {
     bf16 *wf[11], *wb[11];
     float* bpad[11];
     int step = 0;
+    float clip = 1.0f;          // global gradient-norm clip of the Adam step (0: off)
+    double* gnorm2 = nullptr;   // squared gradient norm of the last step
     // activations (bf16 NHWC)
     bf16 *x0, *e0a, *c0, *p0, *e1a, *c1, *p1, *ba, *bb, *d1a, *d1b, *d0a, *d0b;
     float *head, *wk, *out, *dout;
@@ -575,6 +588,7 @@ int pico_hip_cnn_create(pico_hip_ctx* c, uint32_t mw, uint32_t mh, uint32_t mb, 
          (n->d1a = dalloc<bf16>(n, P1 * 64)) && (n->d1b = dalloc<bf16>(n, P1 * 64)) && (n->d0a = dalloc<bf16>(n, P0 * 32)) &&
          (n->d0b = dalloc<bf16>(n, P0 * 32)) && (n->head = dalloc<float>(n, P0 * 32)) && (n->wk = dalloc<float>(n, P0 * 25)) &&
          (n->out = dalloc<float>(n, n3)) && (n->dout = dalloc<float>(n, n3)) && (n->loss = dalloc<double>(n, 1)) &&
+         (n->gnorm2 = dalloc<double>(n, 1)) &&
          (n->g_head = dalloc<bf16>(n, P0 * 32)) && (n->g_d0b = dalloc<bf16>(n, P0 * 32)) && (n->g_d0a = dalloc<bf16>(n, P0 * 32)) &&
          (n->g_c0 = dalloc<bf16>(n, P0 * 96)) && (n->g_d1b = dalloc<bf16>(n, P1 * 64)) && (n->g_d1a = dalloc<bf16>(n, P1 * 64)) &&
          (n->g_c1 = dalloc<bf16>(n, P1 * 192)) && (n->g_bb = dalloc<bf16>(n, P2 * 128)) && (n->g_ba = dalloc<bf16>(n, P2 * 128)) &&
@@ -718,6 +732,22 @@ int pico_hip_cnn_loss_grad(pico_cnn* n, const float* in, const float* tgt, uint3
     return PICO_OK;
 }
 
+int pico_hip_cnn_set_clip(pico_cnn* n, float max_grad_norm) {
+    if (!n) return PICO_ERR_ARG;
+    if (!(max_grad_norm >= 0.0f)) return fail(n->c, PICO_ERR_ARG, "clip norm must be >= 0");
+    n->clip = max_grad_norm;
+    return PICO_OK;
+}
+
+int pico_hip_cnn_last_grad_norm(pico_cnn* n, double* out) {
+    if (!n || !out) return PICO_ERR_ARG;
+    double h = 0.0;
+    PICO_HIP(n->c, hipMemcpyAsync(&h, n->gnorm2, 8, hipMemcpyDeviceToHost, n->c->stream));
+    PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+    *out = std::sqrt(h);
+    return PICO_OK;
+}
+
 int pico_hip_cnn_train_step_d(pico_cnn* n, const float* d_in, const float* d_tgt, uint32_t B, uint32_t w, uint32_t h,
                               float lr, double* loss) {
     if (!n) return PICO_ERR_ARG;
@@ -729,8 +759,10 @@ int pico_hip_cnn_train_step_d(pico_cnn* n, const float* d_in, const float* d_tgt
     n->step++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
+    PICO_LAUNCH(n->c, "cnn_zero", k_zero1, 1, 1, 0, n->gnorm2);
+    PICO_LAUNCH(n->c, "cnn_sumsq", k_sumsq, grid_for(n->nparams), 256, 0, n->grads, (int)n->nparams, n->gnorm2);
     PICO_LAUNCH(n->c, "cnn_adam", k_adam, grid_for(n->nparams), 256, 0, n->params, n->grads, n->m, n->v, (int)n->nparams, lr,
-                b1, b2, c1, c2, 1e-8f);
+                b1, b2, c1, c2, 1e-8f, n->gnorm2, n->clip);
     if ((rc = relayout(n))) return rc;
     if (loss) {
         double l = 0.0;

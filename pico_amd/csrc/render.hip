@@ -594,7 +594,12 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
     bool cancelled = false;
     const bool accum_mode = hv[0].mode == MODE_ACCUM;
     for (uint32_t off = 0; off < pl.max_spp;) {
-        if (cancel && *cancel) { cancelled = true; break; }
+        if (cancel) {
+            // the flag is host memory written by another thread: look at it once
+            // the previous chunk has finished on the device
+            if (off) PICO_HIP(c, hipStreamSynchronize(c->stream));
+            if (*cancel) { cancelled = true; break; }
+        }
         uint32_t S = s_max;
         if (off == 0 && accum_mode) S = std::min<uint32_t>(S, 6);
         S = std::min(S, pl.max_spp - off);

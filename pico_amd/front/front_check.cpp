@@ -1,0 +1,53 @@
+// Drives the C++ front the way Pico's main() drives Scene::render_scene_to_file
+// (Source/main.cpp:97): one Cornell-box frame with denoising and tone mapping,
+// then the a-trous entry point on its own.  Exit status 0 on success; without
+// a visible GPU it reports the device error and exits 0 as well ("nodevice").
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "pico_front.hpp"
+
+struct vec3 { float x, y, z; };
+
+extern "C" int pico_scene_cornell(uint32_t seed, uint32_t microfacet, struct pico_scene** out);
+extern "C" int pico_scene_desc_get(const struct pico_scene* s, pico_scene_desc* d);
+extern "C" int pico_scene_default_camera(const struct pico_scene* s, float aspect, pico_camera* out);
+extern "C" void pico_scene_destroy(struct pico_scene* s);
+
+int main() {
+    pico_front::Renderer r(0);
+    if (r.status() != PICO_OK) {
+        std::printf("nodevice status=%d\n", r.status());
+        return 0;
+    }
+    pico_scene* s = nullptr;
+    pico_scene_desc d;
+    pico_camera cam;
+    if (pico_scene_cornell(1, PICO_MICROFACET_GGX, &s) || pico_scene_desc_get(s, &d) ||
+        pico_scene_default_camera(s, 1.0f, &cam))
+        return 2;
+    if (int rc = r.upload(d)) { std::printf("upload %d %s\n", rc, r.last_error().c_str()); return 3; }
+    const uint32_t W = 64, H = 64;
+    std::vector<vec3> pix(W * H), var(W * H);
+    std::vector<uint32_t> cnt(W * H, 0);
+    pico_front::RenderParams<vec3> p{5, 8, 0.0f, H, W, true, true, pix.data(), cnt.data(), var.data()};
+    bool quit = false;
+    if (int rc = r.render_scene_to_memory(cam, p, &quit)) { std::printf("render %d %s\n", rc, r.last_error().c_str()); return 4; }
+    for (uint32_t i = 0; i < W * H; ++i) {
+        if (cnt[i] == 0 || !std::isfinite(pix[i].x) || pix[i].x < 0.0f || pix[i].x > 1.0f) {
+            std::printf("bad pixel %u: count %u value %g\n", i, cnt[i], pix[i].x);
+            return 5;
+        }
+    }
+    vec3* den = pico_front::atrous_denoise(r.context(), pix.data(), var.data(), var.data(), pix.data(), W, H);
+    if (!den) return 6;
+    delete[] den;
+    quit = true;  // a set flag stops the next render before it starts
+    std::vector<uint32_t> cnt2(W * H, 0);
+    pico_front::RenderParams<vec3> p2{5, 8, 0.0f, H, W, false, false, pix.data(), cnt2.data(), var.data()};
+    if (r.render_scene_to_memory(cam, p2, &quit) != PICO_ERR_CANCELLED) return 7;
+    pico_scene_destroy(s);
+    std::printf("ok\n");
+    return 0;
+}

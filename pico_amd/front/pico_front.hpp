@@ -1,0 +1,150 @@
+// C++ front of the MI355X path with the call shapes of Pico's own renderer:
+//
+//   Scene::Scene::render_scene_to_memory(const Camera&, const RenderParams&, const bool* should_quit)
+//       (Source/Core/Scene.hpp:68, Source/Core/Scene.cpp:143-235)
+//   glm::vec3* Util::atrous_denoise(pixels, normals, position, diffuse, Tiler&)
+//       (Source/Util/Denoisers.hpp:9; returns a new[] buffer the caller delete[]s)
+//
+// Header-only; it calls the C ABI of libpico_hip.so (include/pico_hip.h) and
+// throws nothing: failures are reported through the int status of each call,
+// with the text in last_error().  Vec3 is any 12-byte {x, y, z} float type
+// (glm::vec3 in Pico).
+#pragma once
+
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <thread>
+
+#include "../../include/pico_hip.h"
+
+namespace pico_front {
+
+// Scene::RenderParams (Source/Core/Scene.hpp:30-43), same members and meaning.
+template <class Vec3>
+struct RenderParams {
+    uint32_t m_maxRayDepth;
+    uint32_t m_maxSamples;
+    float m_maxVariance;
+    uint32_t m_Height;
+    uint32_t m_Width;
+    bool m_tonemap;
+    bool m_denoise;
+    Vec3* m_Pixels;
+    uint32_t* m_SampleCount;
+    Vec3* m_variance;
+};
+
+class Renderer {
+public:
+    explicit Renderer(int device = 0) { status_ = pico_hip_create(device, &ctx_); }
+    ~Renderer() { pico_hip_destroy(ctx_); }
+    Renderer(const Renderer&) = delete;
+    Renderer& operator=(const Renderer&) = delete;
+
+    int status() const { return status_; }
+    pico_hip_ctx* context() const { return ctx_; }
+    std::string last_error() const { return ctx_ ? pico_hip_last_error(ctx_) : "no device context"; }
+
+    // Scene::Scene's BVH build (Source/Core/Scene.cpp:36-141): a flattened scene.
+    int upload(const pico_scene_desc& scene) { return ctx_ ? pico_hip_upload_scene(ctx_, &scene) : status_; }
+
+    // Scene::render_scene_to_memory: render (resuming from the pixels'
+    // sample counts like the reference does), then, as the reference orders
+    // them, a-trous on the G-buffer and Reinhard tone mapping, in place.
+    template <class Vec3>
+    int render_scene_to_memory(const pico_camera& camera, const RenderParams<Vec3>& params, const bool* should_quit,
+                               uint32_t frame_seed = 0) {
+        static_assert(sizeof(Vec3) == 12, "packed 3 x f32 per pixel");
+        if (!ctx_) return status_;
+        pico_render_params p{};
+        p.width = params.m_Width;
+        p.height = params.m_Height;
+        p.spp = params.m_maxSamples;
+        p.max_depth = params.m_maxRayDepth;
+        p.tile_w = p.tile_h = 64;  // Scene.cpp:206
+        p.frame_seed = frame_seed;
+        p.max_variance = params.m_maxVariance;
+        p.flags = PICO_RENDER_RESUME;
+        // the C ABI reads an int32 flag; mirror the caller's bool into it
+        // while the render runs
+        volatile int32_t cancel = should_quit && *should_quit ? 1 : 0;
+        std::atomic<bool> done{false};
+        std::thread watch;
+        if (should_quit)
+            watch = std::thread([&] {
+                while (!done.load(std::memory_order_acquire)) {
+                    if (*reinterpret_cast<const volatile bool*>(should_quit)) cancel = 1;
+                    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+                }
+            });
+        int rc = pico_hip_render(ctx_, &camera, &p, reinterpret_cast<float*>(params.m_Pixels), params.m_SampleCount,
+                                 reinterpret_cast<float*>(params.m_variance), &cancel, nullptr);
+        done.store(true, std::memory_order_release);
+        if (watch.joinable()) watch.join();
+        if (rc == PICO_ERR_CANCELLED) return rc;  // the reference returns early too
+        if (rc) return rc;
+        const size_t n = (size_t)params.m_Width * params.m_Height;
+        if (params.m_denoise) {
+            Vec3* nrm = new Vec3[n];
+            Vec3* pos = new Vec3[n];
+            Vec3* alb = new Vec3[n];
+            rc = pico_hip_gbuffer(ctx_, &camera, params.m_Width, params.m_Height, reinterpret_cast<float*>(nrm),
+                                  reinterpret_cast<float*>(pos), reinterpret_cast<float*>(alb));
+            if (!rc)
+                rc = pico_hip_atrous(ctx_, reinterpret_cast<const float*>(params.m_Pixels), reinterpret_cast<float*>(nrm),
+                                     reinterpret_cast<float*>(pos), reinterpret_cast<float*>(alb), params.m_Width,
+                                     params.m_Height, 64, 64, 5, reinterpret_cast<float*>(params.m_Pixels));
+            delete[] nrm;
+            delete[] pos;
+            delete[] alb;
+            if (rc) return rc;
+        }
+        if (params.m_tonemap)
+            rc = pico_hip_tonemap(ctx_, reinterpret_cast<float*>(params.m_Pixels), params.m_Width, params.m_Height);
+        return rc;
+    }
+
+private:
+    pico_hip_ctx* ctx_ = nullptr;
+    int status_ = PICO_OK;
+};
+
+// Util::atrous_denoise: returns a new[] buffer of width * height pixels (the
+// caller delete[]s it), or nullptr on failure.  tile is the Tiler's tile size.
+template <class Vec3>
+Vec3* atrous_denoise(pico_hip_ctx* ctx, const Vec3* pixels, const Vec3* normals, const Vec3* position, const Vec3* diffuse,
+                     uint32_t width, uint32_t height, uint32_t tile = 64) {
+    static_assert(sizeof(Vec3) == 12, "packed 3 x f32 per pixel");
+    Vec3* out = new Vec3[(size_t)width * height];
+    const int rc = pico_hip_atrous(ctx, reinterpret_cast<const float*>(pixels), reinterpret_cast<const float*>(normals),
+                                   reinterpret_cast<const float*>(position), reinterpret_cast<const float*>(diffuse), width,
+                                   height, tile, tile, 5, reinterpret_cast<float*>(out));
+    if (rc) {
+        delete[] out;
+        return nullptr;
+    }
+    return out;
+}
+
+// The learned filter in the same shape (no counterpart in the reference).
+template <class Vec3>
+Vec3* cnn_denoise(pico_cnn* cnn, const Vec3* pixels, const Vec3* normals, const Vec3* position, const Vec3* diffuse,
+                  uint32_t width, uint32_t height) {
+    static_assert(sizeof(Vec3) == 12, "packed 3 x f32 per pixel");
+    Vec3* out = new Vec3[(size_t)width * height];
+    const int rc = pico_hip_cnn_denoise_gbuffer(cnn, reinterpret_cast<const float*>(pixels),
+                                                reinterpret_cast<const float*>(normals),
+                                                reinterpret_cast<const float*>(position),
+                                                reinterpret_cast<const float*>(diffuse), width, height,
+                                                reinterpret_cast<float*>(out));
+    if (rc) {
+        delete[] out;
+        return nullptr;
+    }
+    return out;
+}
+
+}  // namespace pico_front

@@ -82,7 +82,8 @@ def main(argv=None):
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--clip", type=float, default=1.0, help="global gradient-norm clip (0: off)")
     ap.add_argument("--val", type=int, default=48, help="held-out pairs (the last seeds)")
     ap.add_argument("--init-seed", type=int, default=1)
     ap.add_argument("--out", required=True, help=".npy file for the trained parameters")
@@ -107,6 +108,7 @@ def main(argv=None):
 
     net = pa.CNN(ctx, S, S, args.batch)
     net.init(args.init_seed)
+    net.set_clip(args.clip)
     bx, bt = ctx.alloc(args.batch * xs), ctx.alloc(args.batch * ts)
     ntrain = N - args.val
     rng = np.random.default_rng(args.init_seed)
@@ -120,9 +122,10 @@ def main(argv=None):
         want = step % 50 == 0 or step == args.steps - 1
         loss = net.train_step_d(bx, bt, args.batch, S, S, lr_at(step, args.steps, args.lr), want_loss=want)
         if want:
-            history.append((step, loss))
+            gn = net.last_grad_norm()
+            history.append((step, loss, gn))
             if step % 250 == 0 or step == args.steps - 1:
-                log(f"[train] step {step}: loss {loss:.5f}, {time.time() - t1:.1f} s")
+                log(f"[train] step {step}: loss {loss:.5f}, grad norm {gn:.4g}, {time.time() - t1:.1f} s")
     ctx.sync()
     t_train = time.time() - t1
     net.save(args.out)

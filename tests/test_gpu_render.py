@@ -177,3 +177,26 @@ def test_errors_are_reported(ctx, cornell):
         ctx.render(cornell.camera(1.0), pa.params(0, 16, 1, 1))
     with pytest.raises(pa.PicoError):
         ctx.render(cornell.camera(1.0), pa.params(16, 16, 1, 200))
+
+
+@pytest.mark.parametrize("w,h,queue", [(160, 90, 8 * 160 * 90), (1920, 1080, 0)])
+def test_multi_chunk_render_matches_oracle(ctx, w, h, queue):
+    """Renders that run in several chunks of samples (6, then 8 at a time,
+    with early-outed pixels compacted away between chunks) against the oracle
+    with the identical PCG streams: the benchmark scene, at a reduced size
+    with the queue depth forcing the 1080p chunking, and at 1080p itself."""
+    scene = pa.Scene.instanced(2, 16, 1024, 2000, 20000, 100.0, pa.MICROFACET_GGX)
+    ctx.upload_scene(scene)
+    cam = scene.camera(w / h)
+    old = ctx.get_tuning()
+    try:
+        if queue:
+            ctx.set_tuning(queue_paths=queue)
+        p = pa.params(w, h, 16, 8, frame_seed=7)
+        gp, gc, gv, _ = ctx.render(cam, p)
+    finally:
+        ctx.set_tuning(queue_paths=old[0])
+    cp, cc, cv, _ = oracle.CpuScene(scene).render(cam, p, rng=oracle.RNG_PCG, threads=16)
+    assert (gc == cc).mean() >= 0.999, (gc == cc).mean()
+    rel = (np.abs(gp - cp) / (np.abs(cp) + 1e-3)).max(axis=1)
+    assert (rel < 1e-3).mean() >= 0.99, (rel < 1e-3).mean()
