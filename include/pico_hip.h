@@ -138,9 +138,13 @@ typedef struct pico_stats {
     uint64_t shadow_instance_visits; /* PICO_RENDER_COUNT only */
     uint64_t samples;        /* (pixel, sample) pairs accumulated */
     uint64_t accumulated_rays; /* rays of the accumulated samples only */
-    double   render_ms;      /* device time of the render, HIP events */
+    double   render_ms;      /* device time of the render, HIP events; for each view
+                                of a batched launch: launch start -> end of the chunk
+                                holding the view's last sample */
     int32_t  cancelled;
     int32_t  pad;
+    double   done_ms;        /* batched launches: launch start -> the view's output is
+                                final (after its denoiser when one is requested) */
 } pico_stats;
 
 /* ---- context ----------------------------------------------------------- */
@@ -303,16 +307,46 @@ typedef struct pico_view {
     float*    d_variance;       /* device, W*H*3 */
 } pico_view;
 
+/* Launch shape of the wavefront, chosen by the autotuner:
+ *   queue_paths      paths in flight per pass (samples per chunk = queue_paths / pixels); 0 = 16M
+ *   tile_w, tile_h   order of the pixels in the active list, i.e. which pixels
+ *                    neighbouring lanes trace: tiles of tile_w x tile_h in
+ *                    row-major tile order, row-major inside a tile; 0 x 0 =
+ *                    plain row-major image order.
+ * The output never depends on the tuning: every (pixel, sample) has its own
+ * random stream and samples are accumulated in sample order. */
+typedef struct pico_tuning {
+    uint32_t queue_paths;
+    uint32_t tile_w;
+    uint32_t tile_h;
+    uint32_t reserved;
+} pico_tuning;
+int  pico_hip_set_tuning(pico_hip_ctx* ctx, const pico_tuning* tuning);
+int  pico_hip_get_tuning(pico_hip_ctx* ctx, pico_tuning* tuning);
+
 /* Renders n views in one wavefront: every pass of the path tracer covers the
- * live paths of all views.  stats may be NULL or hold n entries.  queue_paths
- * bounds the number of paths in flight (0 = automatic). */
+ * live paths of all views.  stats may be NULL or hold n entries (per-view
+ * completion times in render_ms / done_ms).  queue_paths overrides the
+ * tuning's (0 = use the tuning). */
 int  pico_hip_render_views_d(pico_hip_ctx* ctx, const pico_view* views, uint32_t n, uint32_t queue_paths,
                              pico_stats* stats);
 
-/* Launch-shape knobs chosen by the autotuner: paths in flight per wavefront
- * pass and workgroup size of the traversal kernels. */
-int  pico_hip_set_tuning(pico_hip_ctx* ctx, uint32_t queue_paths, uint32_t trace_block);
-int  pico_hip_get_tuning(pico_hip_ctx* ctx, uint32_t* queue_paths, uint32_t* trace_block);
+/* The same launch, each view followed by the learned denoiser as soon as its
+ * last chunk of samples is done: G-buffer + network on a second stream,
+ * overlapping the remaining passes.  d_denoised[i] receives view i's
+ * filtered image (W*H*3 f32, device); views must fit the network's size. */
+int  pico_hip_render_views_denoise_d(pico_hip_ctx* ctx, const pico_view* views, uint32_t n, pico_cnn* cnn,
+                                     float* const* d_denoised, pico_stats* stats);
+
+/* Autotuner: times the batch at reduced samples per pixel (the largest view
+ * at calib_spp, the others scaled alike) for a fixed set of launch shapes
+ * (queue depth x pixel-tile order, the default first), keeps the fastest
+ * (ties: the earlier) as the context's tuning and caches the choice per
+ * (scene, view sizes, spp, depth), so a repeated call returns it without
+ * measuring.  Renders into the views' buffers.  chosen_ms / default_ms (may
+ * be NULL) are the calibration times of the choice and of the default. */
+int  pico_hip_autotune_views(pico_hip_ctx* ctx, const pico_view* views, uint32_t n, uint32_t calib_spp,
+                             pico_tuning* chosen, float* chosen_ms, float* default_ms);
 
 /* ---- procedural scenes (inputs of the configurations) ------------------ */
 typedef struct pico_scene pico_scene;

@@ -74,11 +74,18 @@ class Stats(C.Structure):
                 ("instance_visits", C.c_uint64), ("shadow_node_visits", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("shadow_instance_visits", C.c_uint64),
                 ("samples", C.c_uint64), ("accumulated_rays", C.c_uint64),
-                ("render_ms", C.c_double), ("cancelled", C.c_int32), ("pad", C.c_int32)]
+                ("render_ms", C.c_double), ("cancelled", C.c_int32), ("pad", C.c_int32), ("done_ms", C.c_double)]
 
     @property
     def rays(self):
         return self.camera_rays + self.extension_rays + self.shadow_rays + self.gbuffer_rays
+
+
+class Tuning(C.Structure):
+    _fields_ = [("queue_paths", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("reserved", C.c_uint32)]
+
+    def as_tuple(self):
+        return (self.queue_paths, self.tile_w, self.tile_h)
 
 
 class View(C.Structure):
@@ -126,8 +133,10 @@ def _declare(lib):
         "pico_hip_atrous_fixed_d": (i32, [vp, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, f32, vp, vp]),
         "pico_hip_relmse_d": (i32, [vp, vp, vp, f64, u32, P(f64)]),
         "pico_hip_render_views_d": (i32, [vp, P(View), u32, u32, P(Stats)]),
-        "pico_hip_set_tuning": (i32, [vp, u32, u32]),
-        "pico_hip_get_tuning": (i32, [vp, P(u32), P(u32)]),
+        "pico_hip_set_tuning": (i32, [vp, P(Tuning)]),
+        "pico_hip_get_tuning": (i32, [vp, P(Tuning)]),
+        "pico_hip_render_views_denoise_d": (i32, [vp, P(View), u32, vp, P(vp), P(Stats)]),
+        "pico_hip_autotune_views": (i32, [vp, P(View), u32, u32, P(Tuning), P(f32), P(f32)]),
         "pico_hip_cnn_create": (i32, [vp, u32, u32, u32, P(vp)]),
         "pico_hip_cnn_destroy": (None, [vp]),
         "pico_hip_cnn_num_params": (u32, []),
@@ -423,13 +432,30 @@ class Context:
         self._check(lib().pico_hip_render_views_d(self._h, arr, len(views), queue_paths, st))
         return list(st)
 
-    def set_tuning(self, queue_paths=0, trace_block=0):
-        self._check(lib().pico_hip_set_tuning(self._h, queue_paths, trace_block))
+    def set_tuning(self, queue_paths=16 << 20, tile=(0, 0)):
+        t = Tuning(queue_paths, tile[0], tile[1], 0)
+        self._check(lib().pico_hip_set_tuning(self._h, C.byref(t)))
 
     def get_tuning(self):
-        q, b = C.c_uint32(), C.c_uint32()
-        self._check(lib().pico_hip_get_tuning(self._h, C.byref(q), C.byref(b)))
-        return q.value, b.value
+        t = Tuning()
+        self._check(lib().pico_hip_get_tuning(self._h, C.byref(t)))
+        return t.as_tuple()
+
+    def render_views_denoise_d(self, views, cnn, d_outs):
+        arr = (View * len(views))(*views)
+        outs = (C.c_void_p * len(views))(*[d.ptr.value for d in d_outs])
+        st = (Stats * len(views))()
+        self._check(lib().pico_hip_render_views_denoise_d(self._h, arr, len(views), cnn._h, outs, st))
+        return list(st)
+
+    def autotune_views(self, views, calib_spp=4):
+        """Chosen (queue_paths, tile_w, tile_h) and the calibration ms of the
+        choice and of the default launch shape."""
+        arr = (View * len(views))(*views)
+        t, cm, dm = Tuning(), C.c_float(), C.c_float()
+        self._check(lib().pico_hip_autotune_views(self._h, arr, len(views), calib_spp, C.byref(t), C.byref(cm),
+                                                  C.byref(dm)))
+        return t.as_tuple(), cm.value, dm.value
 
     def enable_kernel_timing(self, on=True):
         self._check(lib().pico_hip_enable_kernel_timing(self._h, 1 if on else 0))

@@ -174,6 +174,10 @@ void pico_hip_destroy(pico_hip_ctx* c) {
         if (kv.second.p) hipFree(kv.second.p);
     for (auto& kv : c->timers)
         for (auto& e : kv.second.pending) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    if (c->side) {
+        hipStreamSynchronize(c->side);
+        hipStreamDestroy(c->side);
+    }
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -288,6 +292,7 @@ int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) {
     c->stack_need = need;
     c->host = std::move(hs);
     c->has_scene = true;
+    ++c->scene_serial;
     return PICO_OK;
 }
 

@@ -550,6 +550,11 @@ int backward(pico_cnn* n, const float* d_in, const float* d_tgt, int B, int H, i
 
 }  // namespace
 
+namespace pico {
+bool cnn_belongs_to(const pico_cnn* n, const pico_hip_ctx* c) { return n && n->c == c; }
+int cnn_check_size(pico_cnn* n, uint32_t w, uint32_t h) { return check_dims(n, 1, w, h); }
+}  // namespace pico
+
 extern "C" {
 
 uint32_t pico_hip_cnn_num_params(void) {

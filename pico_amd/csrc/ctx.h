@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/pico_hip.h"
 #include "bvh_build.h"
 #include "trace.h"
 
@@ -22,22 +23,29 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+struct TuneEntry {
+    pico_tuning tuning;
+    float chosen_ms, default_ms;
+};
+
 struct pico_hip_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // every launch goes here
+    hipStream_t side = nullptr;     // second stream of batched render + denoise (created on first use)
     std::string err;
     int num_cus = 256;
     // scene
     bool has_scene = false;
+    uint64_t scene_serial = 0;      // bumped by every upload (keys the tuning cache)
     pico::HostScene host;
     pico::DevScene dev{};
     std::vector<void*> scene_allocs;
     uint32_t stack_need = 0;
     // named scratch buffers (grown on demand, freed at destroy)
     std::map<std::string, DevBuf> bufs;
-    // tuning
-    uint32_t queue_paths = 16u << 20;
-    uint32_t trace_block = 128;
+    // launch shape of the wavefront and the autotuner's cache
+    pico_tuning tuning{16u << 20, 0, 0, 0};
+    std::map<uint64_t, TuneEntry> tune_cache;
     // timing
     bool timing = false;
     std::map<std::string, KernelTimes> timers;
@@ -58,6 +66,19 @@ inline int fail(pico_hip_ctx* c, int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                                      \
             return ::pico::fail((ctx), PICO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// Points the context's launches at another stream for a scope (the second
+// stream of the batched render + denoise); restores the main one after.
+struct StreamSwap {
+    pico_hip_ctx* c;
+    hipStream_t saved;
+    StreamSwap(pico_hip_ctx* ctx, hipStream_t s) : c(ctx), saved(ctx->stream) { c->stream = s; }
+    ~StreamSwap() { c->stream = saved; }
+};
+
+// Learned-denoiser queries used by the batched render (defined with the network).
+bool cnn_belongs_to(const pico_cnn* n, const pico_hip_ctx* c);
+int cnn_check_size(pico_cnn* n, uint32_t w, uint32_t h);
 
 // Returns a device buffer of at least `bytes` under `name` (contents undefined).
 void* scratch(pico_hip_ctx* c, const std::string& name, size_t bytes);

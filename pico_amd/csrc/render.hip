@@ -142,26 +142,37 @@ __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
 }
 
 // Per-view set-up: sample-count snapshot, cleared outputs (unless resuming),
-// done flags, and the initial active list.
-__global__ void k_init_view(DView* views, uint32_t vi, uint32_t* active, Ctl* ctl) {
+// done flags, and the initial active list.  The active list enumerates the
+// pixels in tiles of tw x th (row-major tile order, row-major inside a tile),
+// which decides which pixels neighbouring lanes trace; tw = width, th = 1 is
+// plain image order.
+__global__ void k_init_view(DView* views, uint32_t vi, uint32_t* active, Ctl* ctl, uint32_t tw, uint32_t th) {
     const DView& V = views[vi];
-    const uint32_t n = V.cam.w * V.cam.h;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t p = base + threadIdx.x;
+    const uint32_t w = V.cam.w, h = V.cam.h;
+    const uint32_t tiles_x = (w + tw - 1) / tw, tiles_y = (h + th - 1) / th, tsz = tw * th;
+    const uint64_t total = (uint64_t)tiles_x * tiles_y * tsz;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = base + threadIdx.x;
         bool keep = false;
-        if (p < n) {
-            uint32_t c0 = 0;
-            if (V.mode == MODE_ACCUM) {
-                if (V.resume) {
-                    c0 = V.counts[p];
-                } else {
-                    V.counts[p] = 0;
-                    for (int k = 0; k < 3; ++k) { V.pixels[3 * p + k] = 0.0f; V.variance[3 * p + k] = 0.0f; }
+        uint32_t p = 0;
+        if (t < total) {
+            const uint32_t tile = (uint32_t)(t / tsz), r = (uint32_t)(t - (uint64_t)tile * tsz);
+            const uint32_t x = (tile % tiles_x) * tw + r % tw, y = (tile / tiles_x) * th + r / tw;
+            if (x < w && y < h) {
+                p = y * w + x;
+                uint32_t c0 = 0;
+                if (V.mode == MODE_ACCUM) {
+                    if (V.resume) {
+                        c0 = V.counts[p];
+                    } else {
+                        V.counts[p] = 0;
+                        for (int k = 0; k < 3; ++k) { V.pixels[3 * p + k] = 0.0f; V.variance[3 * p + k] = 0.0f; }
+                    }
                 }
+                V.count0[p] = c0;
+                keep = V.mode != MODE_ACCUM || c0 < V.spp;
+                V.done[p] = keep ? 0 : 1;
             }
-            V.count0[p] = c0;
-            keep = V.mode != MODE_ACCUM || c0 < V.spp;
-            V.done[p] = keep ? 0 : 1;
         }
         const uint32_t slot = wave_append(&ctl->n_active, keep);
         if (keep) active[slot] = (vi << kViewShift) | p;
@@ -517,9 +528,17 @@ int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, const Pool& P, 
     return PICO_OK;
 }
 
+// Optional per-view follow-up of a batched launch: the learned denoiser on
+// the second stream once the view's last chunk is done.
+struct ViewHooks {
+    pico_cnn* cnn = nullptr;
+    float* const* d_out = nullptr;
+    const pico_view* views = nullptr;
+};
+
 // Runs the chunked wavefront over `nv` views already described on the host.
-int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, const volatile int32_t* cancel, bool count,
-              pico_stats* stats, uint32_t n_stats) {
+int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, const volatile int32_t* cancel,
+              bool count, pico_stats* stats, uint32_t n_stats, const ViewHooks* hooks = nullptr) {
     if (!c->has_scene) return fail(c, PICO_ERR_STATE, "no scene uploaded");
     const uint32_t nv = (uint32_t)hv.size();
     if (nv == 0 || nv > kMaxViews) return fail(c, PICO_ERR_LIMIT, "between 1 and 64 views per launch");
@@ -533,7 +552,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
     if (pix_total > (1ull << 31)) return fail(c, PICO_ERR_LIMIT, "too many pixels in one launch");
     if (pl.max_depth > kMaxBounce - 1) return fail(c, PICO_ERR_LIMIT, "max_depth above 63");
     if (pl.max_spp == 0) return PICO_OK;
-    uint32_t cap = queue_paths ? queue_paths : c->queue_paths;
+    uint32_t cap = tune.queue_paths ? tune.queue_paths : (16u << 20);
     cap = std::max<uint32_t>(cap, 64);
     // samples per chunk: the first chunk ends right after the sample at which
     // the reference's early-out is first tested (prev_count > 4)
@@ -548,6 +567,18 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
     unsigned long long* vstats = (unsigned long long*)scratch(c, "vstats", nv * 32 + 64);
     DView* dv = (DView*)scratch(c, "views", sizeof(DView) * nv);
     if (!pb || !act || !ctl || !count0 || !done || !vstats || !dv) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    // the follow-up's buffers, sized before the first launch so that no
+    // allocation happens while the two streams run
+    float *gn = nullptr, *gp = nullptr, *ga = nullptr;
+    if (hooks && hooks->cnn) {
+        size_t maxpx = 0;
+        for (auto& v : hv) maxpx = std::max(maxpx, (size_t)v.cam.w * v.cam.h);
+        float* g = (float*)scratch(c, "views_gbuffer", maxpx * 36 + 64);
+        if (!g || !scratch(c, "cnn_frame_in", maxpx * 40 + 64) || !scratch(c, "gb_ovf", 64))
+            return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+        gn = g; gp = g + 3 * maxpx; ga = g + 6 * maxpx;
+        if (!c->side) PICO_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    }
     Pool P;
     {
         char* q = pb;
@@ -573,10 +604,27 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
     PICO_HIP(c, hipMemcpyAsync(dv, hv.data(), sizeof(DView) * nv, hipMemcpyHostToDevice, c->stream));
     PICO_HIP(c, hipMemsetAsync(ctl, 0, sizeof(Ctl), c->stream));
     PICO_HIP(c, hipMemsetAsync(vstats, 0, nv * 32, c->stream));
+    // events: launch start/end, the end of every chunk (batched launches) and
+    // each view's final output (follow-up on the second stream)
+    std::vector<hipEvent_t> evs;
+    auto new_event = [&](hipEvent_t* e) -> bool {
+        if (hipEventCreate(e) != hipSuccess) return false;
+        evs.push_back(*e);
+        return true;
+    };
+    struct EventsGuard {
+        std::vector<hipEvent_t>& v;
+        ~EventsGuard() { for (hipEvent_t e : v) hipEventDestroy(e); }
+    } guard{evs};
+    hipEvent_t e0, e1;
+    if (!new_event(&e0) || !new_event(&e1)) return fail(c, PICO_ERR_HIP, "hipEventCreate failed");
+    PICO_HIP(c, hipEventRecord(e0, c->stream));
     for (uint32_t v = 0; v < nv; ++v) {
         const uint32_t n = hv[v].cam.w * hv[v].cam.h;
+        uint32_t tw = tune.tile_w ? std::min(tune.tile_w, hv[v].cam.w) : hv[v].cam.w;
+        uint32_t th = tune.tile_w ? std::max<uint32_t>(1, std::min(tune.tile_h, hv[v].cam.h)) : 1;
         const uint32_t g = std::min<uint32_t>((n + 255) / 256, 4096);
-        PICO_LAUNCH(c, "init_view", k_init_view, g, 256, 0, dv, v, act_a, ctl);
+        PICO_LAUNCH(c, "init_view", k_init_view, g, 256, 0, dv, v, act_a, ctl, tw, th);
     }
     // persistent grids: as many workgroups as stay resident at once
     int occ_t = 0, occ_s = 0;
@@ -586,10 +634,8 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
     const uint32_t grid_trace = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_t, 16));
     const uint32_t grid_shade = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_s, 16));
     const uint32_t grid_lin = (uint32_t)c->num_cus * 16;
-    hipEvent_t e0, e1;
-    PICO_HIP(c, hipEventCreate(&e0));
-    PICO_HIP(c, hipEventCreate(&e1));
-    PICO_HIP(c, hipEventRecord(e0, c->stream));
+    const bool per_view_times = n_stats > 1 || hooks;
+    std::vector<hipEvent_t> view_render(nv, nullptr), view_done(nv, nullptr);
     bool first = true;
     bool cancelled = false;
     const bool accum_mode = hv[0].mode == MODE_ACCUM;
@@ -617,7 +663,39 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
         PICO_LAUNCH(c, "accumulate", k_accumulate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         PICO_LAUNCH(c, "compact", k_compact, grid_lin, kBlock, 0, dv, act_a, act_b, ctl);
         std::swap(act_a, act_b);
+        if (per_view_times) {
+            hipEvent_t ce = nullptr;
+            for (uint32_t v = 0; v < nv; ++v) {
+                if (view_render[v] || off + S < hv[v].spp) continue;
+                if (!ce) {
+                    if (!new_event(&ce)) return fail(c, PICO_ERR_HIP, "hipEventCreate failed");
+                    PICO_HIP(c, hipEventRecord(ce, c->stream));
+                }
+                view_render[v] = ce;
+                if (hooks && hooks->cnn) {
+                    // G-buffer and network of this view on the second stream,
+                    // overlapping the passes still to come
+                    PICO_HIP(c, hipStreamWaitEvent(c->side, ce, 0));
+                    StreamSwap sw(c, c->side);
+                    const uint32_t w = hv[v].cam.w, h = hv[v].cam.h;
+                    int rc = pico_hip_gbuffer_d(c, &hooks->views[v].camera, w, h, gn, gp, ga);
+                    if (!rc) rc = pico_hip_cnn_denoise_gbuffer_d(hooks->cnn, hv[v].pixels, gn, gp, ga, w, h, hooks->d_out[v]);
+                    if (rc) return rc;
+                    if (!new_event(&view_done[v])) return fail(c, PICO_ERR_HIP, "hipEventCreate failed");
+                    PICO_HIP(c, hipEventRecord(view_done[v], c->side));
+                } else {
+                    view_done[v] = ce;
+                }
+            }
+        }
         off += S;
+    }
+    if (hooks && hooks->cnn) {
+        // the caller's stream sees the follow-up's outputs
+        hipEvent_t se;
+        if (!new_event(&se)) return fail(c, PICO_ERR_HIP, "hipEventCreate failed");
+        PICO_HIP(c, hipEventRecord(se, c->side));
+        PICO_HIP(c, hipStreamWaitEvent(c->stream, se, 0));
     }
     PICO_HIP(c, hipEventRecord(e1, c->stream));
     Ctl hc;
@@ -627,14 +705,13 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
     PICO_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0.0f;
     hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     if (hc.overflow) return fail(c, PICO_ERR_LIMIT, "traversal stack overflow");
     if (stats) {
         for (uint32_t v = 0; v < n_stats; ++v) {
             pico_stats& st = stats[v];
             std::memset(&st, 0, sizeof(st));
             st.render_ms = ms;
+            st.done_ms = ms;
             st.cancelled = cancelled ? 1 : 0;
             if (n_stats == 1) {
                 // the whole launch: rays of every pass (including samples the
@@ -656,6 +733,13 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, uint32_t queue_paths, con
                 st.accumulated_rays = hs[4 * v];
                 st.samples = hs[4 * v + 1];
             }
+            if (per_view_times && view_render[v]) {
+                float a = 0.0f, b = 0.0f;
+                hipEventElapsedTime(&a, e0, view_render[v]);
+                st.render_ms = a;
+                if (view_done[v] && hipEventElapsedTime(&b, e0, view_done[v]) == hipSuccess) st.done_ms = b;
+                else st.done_ms = a;
+            }
         }
     }
     return cancelled ? fail(c, PICO_ERR_CANCELLED, "cancelled") : PICO_OK;
@@ -671,6 +755,26 @@ DView base_view(const pico_camera* cam, const pico_render_params* p) {
     v.max_variance = p->max_variance;
     v.resume = (p->flags & PICO_RENDER_RESUME) ? 1u : 0u;
     return v;
+}
+
+int check_params(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p);
+
+// Host descriptions of a caller's views (accumulating into its buffers).
+int make_views(pico_hip_ctx* c, const pico_view* views, uint32_t n, std::vector<DView>& hv) {
+    if (!views || n == 0) return fail(c, PICO_ERR_ARG, "no views");
+    if (n > kMaxViews) return fail(c, PICO_ERR_LIMIT, "at most 64 views per launch");
+    for (uint32_t i = 0; i < n; ++i) {
+        int rc = check_params(c, &views[i].camera, &views[i].params);
+        if (rc) return rc;
+        if (!views[i].d_pixels || !views[i].d_sample_count || !views[i].d_variance) return fail(c, PICO_ERR_ARG, "NULL buffer");
+        DView v = base_view(&views[i].camera, &views[i].params);
+        v.mode = MODE_ACCUM;
+        v.pixels = views[i].d_pixels;
+        v.counts = views[i].d_sample_count;
+        v.variance = views[i].d_variance;
+        hv.push_back(v);
+    }
+    return PICO_OK;
 }
 
 int check_params(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p) {
@@ -696,7 +800,7 @@ int pico_hip_render_d(pico_hip_ctx* c, const pico_camera* cam, const pico_render
     v[0].pixels = dpix;
     v[0].counts = dcnt;
     v[0].variance = dvar;
-    return run_views(c, v, 0, cancel, (p->flags & PICO_RENDER_COUNT) != 0, stats, stats ? 1 : 0);
+    return run_views(c, v, c->tuning, cancel, (p->flags & PICO_RENDER_COUNT) != 0, stats, stats ? 1 : 0);
 }
 
 int pico_hip_render(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p, float* pix, uint32_t* cnt,
@@ -738,7 +842,7 @@ int pico_hip_render_samples(pico_hip_ctx* c, const pico_camera* cam, const pico_
     v[0].mode = MODE_SAMPLES;
     v[0].sample_base = sample_base;
     v[0].samples_out = d;
-    rc = run_views(c, v, 0, nullptr, false, nullptr, 0);
+    rc = run_views(c, v, c->tuning, nullptr, false, nullptr, 0);
     if (rc) return rc;
     PICO_HIP(c, hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
@@ -755,39 +859,108 @@ int pico_hip_render_accum64_d(pico_hip_ctx* c, const pico_camera* cam, const pic
     v[0].mode = MODE_SUM64;
     v[0].sample_base = sample_base;
     v[0].sum64 = d_sum;
-    return run_views(c, v, 0, nullptr, false, stats, stats ? 1 : 0);
+    return run_views(c, v, c->tuning, nullptr, false, stats, stats ? 1 : 0);
 }
 
 int pico_hip_render_views_d(pico_hip_ctx* c, const pico_view* views, uint32_t n, uint32_t queue_paths, pico_stats* stats) {
     if (!c) return PICO_ERR_ARG;
-    if (!views || n == 0) return fail(c, PICO_ERR_ARG, "no views");
-    if (n > kMaxViews) return fail(c, PICO_ERR_LIMIT, "at most 64 views per launch");
     std::vector<DView> hv;
-    for (uint32_t i = 0; i < n; ++i) {
-        int rc = check_params(c, &views[i].camera, &views[i].params);
-        if (rc) return rc;
-        if (!views[i].d_pixels || !views[i].d_sample_count || !views[i].d_variance) return fail(c, PICO_ERR_ARG, "NULL buffer");
-        DView v = base_view(&views[i].camera, &views[i].params);
-        v.mode = MODE_ACCUM;
-        v.pixels = views[i].d_pixels;
-        v.counts = views[i].d_sample_count;
-        v.variance = views[i].d_variance;
-        hv.push_back(v);
-    }
-    return run_views(c, hv, queue_paths, nullptr, false, stats, stats ? n : 0);
+    int rc = make_views(c, views, n, hv);
+    if (rc) return rc;
+    pico_tuning t = c->tuning;
+    if (queue_paths) t.queue_paths = queue_paths;
+    return run_views(c, hv, t, nullptr, false, stats, stats ? n : 0);
 }
 
-int pico_hip_set_tuning(pico_hip_ctx* c, uint32_t queue_paths, uint32_t trace_block) {
+int pico_hip_render_views_denoise_d(pico_hip_ctx* c, const pico_view* views, uint32_t n, pico_cnn* cnn,
+                                    float* const* d_denoised, pico_stats* stats) {
     if (!c) return PICO_ERR_ARG;
-    if (queue_paths) c->queue_paths = std::max<uint32_t>(queue_paths, 1024);
-    if (trace_block) c->trace_block = trace_block;
+    if (!cnn || !d_denoised) return fail(c, PICO_ERR_ARG, "NULL network or output list");
+    if (!cnn_belongs_to(cnn, c)) return fail(c, PICO_ERR_ARG, "the network was created on another context");
+    std::vector<DView> hv;
+    int rc = make_views(c, views, n, hv);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!d_denoised[i]) return fail(c, PICO_ERR_ARG, "NULL output buffer");
+        if ((rc = cnn_check_size(cnn, views[i].params.width, views[i].params.height))) return rc;
+    }
+    ViewHooks hk;
+    hk.cnn = cnn;
+    hk.d_out = d_denoised;
+    hk.views = views;
+    return run_views(c, hv, c->tuning, nullptr, false, stats, stats ? n : 0, &hk);
+}
+
+int pico_hip_set_tuning(pico_hip_ctx* c, const pico_tuning* t) {
+    if (!c) return PICO_ERR_ARG;
+    if (!t) return fail(c, PICO_ERR_ARG, "NULL tuning");
+    if ((t->tile_w == 0) != (t->tile_h == 0)) return fail(c, PICO_ERR_ARG, "tile_w and tile_h are both 0 or both > 0");
+    if ((uint64_t)t->tile_w * t->tile_h > (1u << 20)) return fail(c, PICO_ERR_LIMIT, "tile of more than 2^20 pixels");
+    c->tuning = *t;
+    if (c->tuning.queue_paths) c->tuning.queue_paths = std::max<uint32_t>(c->tuning.queue_paths, 1024);
     return PICO_OK;
 }
 
-int pico_hip_get_tuning(pico_hip_ctx* c, uint32_t* queue_paths, uint32_t* trace_block) {
+int pico_hip_get_tuning(pico_hip_ctx* c, pico_tuning* t) {
+    if (!c || !t) return PICO_ERR_ARG;
+    *t = c->tuning;
+    return PICO_OK;
+}
+
+int pico_hip_autotune_views(pico_hip_ctx* c, const pico_view* views, uint32_t n, uint32_t calib_spp, pico_tuning* chosen,
+                            float* chosen_ms, float* default_ms) {
     if (!c) return PICO_ERR_ARG;
-    if (queue_paths) *queue_paths = c->queue_paths;
-    if (trace_block) *trace_block = c->trace_block;
+    if (calib_spp == 0) return fail(c, PICO_ERR_ARG, "calib_spp must be positive");
+    std::vector<DView> hv;
+    int rc = make_views(c, views, n, hv);
+    if (rc) return rc;
+    // key: scene upload, view sizes, samples, depths
+    uint64_t key = 0xcbf29ce484222325ull;
+    auto mixin = [&](uint64_t x) { key = (key ^ x) * 0x100000001b3ull; };
+    mixin(c->scene_serial);
+    mixin(n);
+    mixin(calib_spp);
+    uint32_t max_spp = 1;
+    for (uint32_t i = 0; i < n; ++i) {
+        mixin(views[i].params.width);
+        mixin(views[i].params.height);
+        mixin(views[i].params.spp);
+        mixin(views[i].params.max_depth);
+        max_spp = std::max(max_spp, views[i].params.spp);
+    }
+    auto it = c->tune_cache.find(key);
+    if (it == c->tune_cache.end()) {
+        for (uint32_t i = 0; i < n; ++i)
+            hv[i].spp = std::max<uint32_t>(1, (uint32_t)(((uint64_t)views[i].params.spp * calib_spp + max_spp - 1) / max_spp));
+        const pico_tuning defaults{16u << 20, 0, 0, 0};
+        std::vector<pico_tuning> cand{defaults};
+        const uint32_t queues[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20};
+        const uint32_t tiles[][2] = {{0, 0}, {8, 8}, {16, 16}, {32, 4}};
+        for (uint32_t q : queues)
+            for (auto& t : tiles) {
+                pico_tuning x{q, t[0], t[1], 0};
+                if (!(x.queue_paths == defaults.queue_paths && x.tile_w == 0)) cand.push_back(x);
+            }
+        // warm-up at the deepest queue: the work buffers reach their largest size
+        std::vector<DView> w = hv;
+        if ((rc = run_views(c, w, pico_tuning{32u << 20, 0, 0, 0}, nullptr, false, nullptr, 0))) return rc;
+        std::vector<float> best(cand.size(), 1e30f);
+        for (int rep = 0; rep < 2; ++rep)
+            for (size_t k = 0; k < cand.size(); ++k) {
+                std::vector<DView> v = hv;
+                pico_stats st;
+                if ((rc = run_views(c, v, cand[k], nullptr, false, &st, 1))) return rc;
+                best[k] = std::min(best[k], (float)st.render_ms);
+            }
+        size_t kb = 0;
+        for (size_t k = 1; k < cand.size(); ++k)
+            if (best[k] < best[kb]) kb = k;
+        it = c->tune_cache.emplace(key, TuneEntry{cand[kb], best[kb], best[0]}).first;
+    }
+    c->tuning = it->second.tuning;
+    if (chosen) *chosen = it->second.tuning;
+    if (chosen_ms) *chosen_ms = it->second.chosen_ms;
+    if (default_ms) *default_ms = it->second.default_ms;
     return PICO_OK;
 }
 

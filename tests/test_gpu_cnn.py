@@ -108,3 +108,33 @@ def test_gbuffer_entry_point_equals_planar_inputs(ctx):
     np.testing.assert_array_equal(xi[:9], x[:9])
     np.testing.assert_allclose(xi[9], x[9], rtol=1e-6)
     np.testing.assert_array_equal(b, net.denoise(xi, w, h))
+
+
+def test_trained_network_gains_over_the_methods_without_it(ctx):
+    """Held-out procedural scenes (seeds outside the training range), 4-spp
+    input, 512-spp FP64 target: the shipped trained network beats the raw
+    input, the fixed-width a-trous filter and the untrained network."""
+    from pico_amd.kpn_train import PairRenderer, relmse
+    S = 128
+    pr = PairRenderer(ctx, S, S, spp_in=4, spp_target=512, depth=8)
+    trained, untrained = pa.CNN(ctx, S, S, 1), pa.CNN(ctx, S, S, 1)
+    trained.load()
+    untrained.init(1)
+    d_x, d_t, d_o, d_s = ctx.alloc(40 * S * S), ctx.alloc(12 * S * S), ctx.alloc(12 * S * S), ctx.alloc(12 * S * S)
+    err = {"raw": [], "atrous": [], "untrained": [], "trained": []}
+    for seed in (20000, 20001, 20002, 20003, 20004, 20005):
+        scene = pa.Scene.training(seed)
+        pr.render(scene, seed, d_x, d_t)
+        tgt = d_t.download(np.float32, 3 * S * S).reshape(-1, 3)
+        rgb = np.ascontiguousarray(d_x.download(np.float32, 3 * S * S).reshape(3, -1).T)
+        err["raw"].append(relmse(rgb, tgt))
+        pr.d_pix.upload(rgb)
+        ctx.gbuffer_d(scene.camera(1.0), S, S, pr.d_n, pr.d_p, pr.d_a)
+        ctx.atrous_fixed_d(pr.d_pix, pr.d_n, pr.d_p, pr.d_a, S, S, d_s, d_o)
+        err["atrous"].append(relmse(d_o.download(np.float32, 3 * S * S).reshape(-1, 3), tgt))
+        for name, net in (("untrained", untrained), ("trained", trained)):
+            net.denoise_d(d_x, S, S, d_o)
+            err[name].append(relmse(d_o.download(np.float32, 3 * S * S).reshape(-1, 3), tgt))
+    m = {k: float(np.mean(v)) for k, v in err.items()}
+    print("held-out relMSE", m)
+    assert m["trained"] < m["atrous"] and m["trained"] < m["raw"] and m["trained"] < m["untrained"], m
