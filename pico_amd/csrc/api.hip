@@ -37,9 +37,10 @@ __global__ __launch_bounds__(kRayBlock) void k_closest_hit(DevScene s, uint32_t 
                                                             const float* __restrict__ d, float* t, uint32_t* inst,
                                                             uint32_t* prim, float* uv, uint32_t* overflow) {
     __shared__ uint32_t stk[STACK * kRayBlock];
+    TraceStack<STACK> st{stk + threadIdx.x, kRayBlock, nullptr, 0, 0};
     const uint32_t i = blockIdx.x * kRayBlock + threadIdx.x;
     if (i >= n) return;
-    Hit h = trace_closest<STACK, false>(s, ld3(o + 3 * i), ld3(d + 3 * i), stk + threadIdx.x, kRayBlock, nullptr, overflow);
+    Hit h = trace_closest<STACK, false>(s, ld3(o + 3 * i), ld3(d + 3 * i), st, nullptr, overflow);
     t[i] = h.t;
     inst[i] = h.inst;
     prim[i] = h.prim;
@@ -51,10 +52,11 @@ template <int STACK>
 __global__ __launch_bounds__(kRayBlock) void k_primary_ids(DevScene s, DCam cam, float2 xi, uint32_t* inst, uint32_t* prim,
                                                             uint32_t* overflow) {
     __shared__ uint32_t stk[STACK * kRayBlock];
+    TraceStack<STACK> st{stk + threadIdx.x, kRayBlock, nullptr, 0, 0};
     const uint32_t i = blockIdx.x * kRayBlock + threadIdx.x;
     if (i >= cam.w * cam.h) return;
     const f3 d = camera_dir(cam, xi, i % cam.w, i / cam.w);
-    Hit h = trace_closest<STACK, false>(s, cam.pos, d, stk + threadIdx.x, kRayBlock, nullptr, overflow);
+    Hit h = trace_closest<STACK, false>(s, cam.pos, d, st, nullptr, overflow);
     inst[i] = h.inst;
     prim[i] = h.prim;
 }
@@ -64,10 +66,11 @@ template <int STACK>
 __global__ __launch_bounds__(kRayBlock) void k_gbuffer(DevScene s, DCam cam, float4 v0, float4 v1, float4 v2, float4 v3,
                                                         float* normal, float* position, float* albedo, uint32_t* overflow) {
     __shared__ uint32_t stk[STACK * kRayBlock];
+    TraceStack<STACK> st{stk + threadIdx.x, kRayBlock, nullptr, 0, 0};
     const uint32_t i = blockIdx.x * kRayBlock + threadIdx.x;
     if (i >= cam.w * cam.h) return;
     const f3 d = camera_dir(cam, make_float2(0.0f, 0.0f), i % cam.w, i / cam.w);
-    Hit h = trace_closest<STACK, false>(s, cam.pos, d, stk + threadIdx.x, kRayBlock, nullptr, overflow);
+    Hit h = trace_closest<STACK, false>(s, cam.pos, d, st, nullptr, overflow);
     f3 n, p, a;
     if (h.inst != 0xffffffffu) {
         const Frag f = make_frag(s, h);
@@ -268,11 +271,11 @@ int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) {
     HostScene hs;
     std::string e = build_scene(*d, 16, hs);
     if (!e.empty()) return fail(c, PICO_ERR_ARG, e);
-    const uint32_t need = hs.tlas_depth + hs.blas_depth + 1;
+    const uint32_t need = hs.stack_bound;
     if (need > 64) return fail(c, PICO_ERR_LIMIT, "BVH too deep for the 64-entry traversal stack");
     DevScene s{};
     int rc;
-    if ((rc = upload_vec(c, hs.nodes, &s.nodes))) return rc;
+    if ((rc = upload_vec(c, hs.nodes4, &s.nodes))) return rc;
     if ((rc = upload_vec(c, hs.tris, &s.tris))) return rc;
     if ((rc = upload_vec(c, hs.instances, &s.inst))) return rc;
     if ((rc = upload_vec(c, hs.xforms, &s.xf))) return rc;
@@ -299,7 +302,7 @@ int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) {
 int pico_hip_scene_info(pico_hip_ctx* c, uint64_t* bn, uint64_t* bt, uint64_t* bi, uint32_t* td, uint32_t* bd) {
     if (!c) return PICO_ERR_ARG;
     if (!c->has_scene) return fail(c, PICO_ERR_STATE, "no scene uploaded");
-    if (bn) *bn = c->host.nodes.size() * sizeof(DNode);
+    if (bn) *bn = c->host.nodes4.size() * sizeof(DNode4);
     if (bt) *bt = c->host.tris.size() * sizeof(DTri);
     if (bi) *bi = c->host.instances.size() * sizeof(DInstance);
     if (td) *td = c->host.tlas_depth;

@@ -215,6 +215,111 @@ void alias_table(const std::vector<float>& w, std::vector<float>& bw, std::vecto
 
 }  // namespace
 
+void quantise_children(const float (*lo)[3], const float (*hi)[3], int nc, DNode4& nd) {
+    std::memset(&nd, 0, sizeof(nd));
+    for (int i = 0; i < 4; ++i) nd.ref[i] = kEmptyRef;
+    uint8_t* q[3] = {nd.qx, nd.qy, nd.qz};
+    uint32_t exps = 0;
+    for (int k = 0; k < 3; ++k) {
+        float o = INFINITY, m = -INFINITY;
+        for (int i = 0; i < nc; ++i) { o = std::min(o, lo[i][k]); m = std::max(m, hi[i][k]); }
+        if (nc == 0 || !(o <= m)) { o = 0.0f; m = 0.0f; }
+        nd.origin[k] = o;
+        const double ext = (double)m - (double)o;
+        int e = ext > 0.0 ? (int)std::ceil(std::log2(ext / 255.0)) : -126;
+        e = std::min(std::max(e, -126), 126);
+        for (;; ++e) {
+            // decoded plane = fma(q, 2^e, origin) (one rounding, as on the device)
+            const float sc = std::ldexp(1.0f, e);
+            bool ok = true;
+            for (int i = 0; i < nc && ok; ++i) {
+                int ql = (int)std::floor(((double)lo[i][k] - o) / sc);
+                ql = std::min(std::max(ql, 0), 255);
+                while (ql > 0 && std::fma((float)ql, sc, o) > lo[i][k]) --ql;
+                int qh = (int)std::ceil(((double)hi[i][k] - o) / sc);
+                qh = std::min(std::max(qh, 0), 255);
+                while (qh < 255 && std::fma((float)qh, sc, o) < hi[i][k]) ++qh;
+                if (std::fma((float)ql, sc, o) > lo[i][k] || std::fma((float)qh, sc, o) < hi[i][k]) ok = false;
+                q[k][i] = (uint8_t)ql;
+                q[k][4 + i] = (uint8_t)qh;
+            }
+            if (ok || e >= 126) break;
+        }
+        exps |= (uint32_t)(uint8_t)(int8_t)e << (8 * k);
+    }
+    nd.exps = exps;
+}
+
+namespace {
+struct WideBuilder {
+    const std::vector<DNode>& n2;
+    std::vector<DNode4>& out;
+
+    static void child_box(const DNode& n, int c, float lo[3], float hi[3]) {
+        const float* x = c == 0 ? n.c0x : n.c1x;
+        const float* y = c == 0 ? n.c0y : n.c1y;
+        const float* z = c == 0 ? n.c0z : n.c1z;
+        lo[0] = x[0]; hi[0] = x[1]; lo[1] = y[0]; hi[1] = y[1]; lo[2] = z[0]; hi[2] = z[1];
+    }
+    static float area(const float lo[3], const float hi[3]) {
+        const float dx = std::max(0.0f, hi[0] - lo[0]), dy = std::max(0.0f, hi[1] - lo[1]), dz = std::max(0.0f, hi[2] - lo[2]);
+        return dx * dy + dx * dz + dy * dz;
+    }
+
+    uint32_t collapse(uint32_t id2, uint32_t& bound) {
+        struct Item { uint32_t ref; float lo[3], hi[3]; };
+        Item ch[4];
+        int nc = 0;
+        auto add_children = [&](uint32_t id) {
+            for (int c = 0; c < 2; ++c) {
+                if (n2[id].ref[c] == kEmptyRef) continue;
+                Item it;
+                it.ref = n2[id].ref[c];
+                child_box(n2[id], c, it.lo, it.hi);
+                ch[nc++] = it;
+            }
+        };
+        add_children(id2);
+        while (nc < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int i = 0; i < nc; ++i)
+                if (!(ch[i].ref & kLeafBit) && area(ch[i].lo, ch[i].hi) > ba) { ba = area(ch[i].lo, ch[i].hi); best = i; }
+            if (best < 0) break;
+            const uint32_t open = ch[best].ref;
+            ch[best] = ch[--nc];
+            add_children(open);
+        }
+        const uint32_t me = (uint32_t)out.size();
+        out.emplace_back();
+        DNode4 nd;
+        float lo[4][3], hi[4][3];
+        for (int i = 0; i < nc; ++i)
+            for (int k = 0; k < 3; ++k) { lo[i][k] = ch[i].lo[k]; hi[i][k] = ch[i].hi[k]; }
+        quantise_children(lo, hi, nc, nd);
+        uint32_t deepest = 0;
+        for (int i = 0; i < nc; ++i) {
+            if (ch[i].ref & kLeafBit) {
+                nd.ref[i] = ch[i].ref;
+            } else {
+                uint32_t b = 0;
+                nd.ref[i] = collapse(ch[i].ref, b);
+                deepest = std::max(deepest, b);
+            }
+        }
+        // entering any child leaves at most nc - 1 siblings on the stack
+        bound = (nc > 0 ? (uint32_t)nc - 1 : 0) + deepest;
+        out[me] = nd;
+        return me;
+    }
+};
+}  // namespace
+
+uint32_t collapse_bvh4(const std::vector<DNode>& n2, uint32_t root2, std::vector<DNode4>& out, uint32_t& bound) {
+    WideBuilder wb{n2, out};
+    return wb.collapse(root2, bound);
+}
+
 uint32_t build_bvh(const std::vector<float>& box_lo, const std::vector<float>& box_hi, uint32_t bins, uint32_t max_leaf,
                    uint32_t prim_base, std::vector<DNode>& nodes, std::vector<uint32_t>& order, uint32_t& depth) {
     const uint32_t n = (uint32_t)(box_lo.size() / 3);
@@ -316,7 +421,7 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
         if (M[3] != 0.0f || M[7] != 0.0f || M[11] != 0.0f || M[15] != 1.0f) return "instance transform must be affine";
         DInstance di{};
         if (!affine_inverse_rows(M, di.inv)) return "instance transform is singular";
-        di.blas_root = mesh_root[m];
+        di.blas_root = mesh_root[m];  // binary root for now; the 4-wide root below
         di.tri_offset = d.mesh_triangle_offset[m];
         di.material = d.instance_material[i];
         di.bsdf = d.instance_bsdf[i];
@@ -353,6 +458,20 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
                 uint32_t& r = out.nodes[k].ref[c];
                 if (r != kEmptyRef && (r & kLeafBit)) r = make_leaf(order[r & kPrimMask], 1);
             }
+    }
+    // 4-wide nodes: every BLAS, then the TLAS
+    {
+        std::vector<uint32_t> root4(d.n_meshes);
+        uint32_t blas_bound = 0;
+        for (uint32_t m = 0; m < d.n_meshes; ++m) {
+            uint32_t b = 0;
+            root4[m] = collapse_bvh4(out.nodes, mesh_root[m], out.nodes4, b);
+            blas_bound = std::max(blas_bound, b);
+        }
+        for (uint32_t i = 0; i < d.n_instances; ++i) out.instances[i].blas_root = root4[d.instance_mesh[i]];
+        uint32_t tb = 0;
+        out.tlas_root = collapse_bvh4(out.nodes, out.tlas_root, out.nodes4, tb);
+        out.stack_bound = tb + 1 + blas_bound;  // TLAS entries, the instance sentinel, BLAS entries
     }
     // lights
     std::set<uint32_t> light_meshes;

@@ -10,7 +10,8 @@
 namespace pico {
 
 struct HostScene {
-    std::vector<DNode> nodes;          // BLAS nodes of every mesh, then the TLAS
+    std::vector<DNode> nodes;          // binary build: BLAS nodes of every mesh, then the TLAS
+    std::vector<DNode4> nodes4;        // the 4-wide nodes the device traverses
     std::vector<DTri> tris;            // triangle records in BLAS leaf order
     std::vector<DInstance> instances;  // original instance order
     std::vector<DXform> xforms;
@@ -22,9 +23,10 @@ struct HostScene {
     std::vector<float> normals;        // 3 x f32 per vertex (global)
     std::vector<uint32_t> indices;     // 3 x u32 per triangle, global vertex ids
     std::vector<uint32_t> mesh_tri_offset;
-    uint32_t tlas_root = 0;
+    uint32_t tlas_root = 0;            // DNode4 index
     uint32_t tlas_depth = 0;
-    uint32_t blas_depth = 0;           // deepest BLAS
+    uint32_t blas_depth = 0;           // deepest BLAS (binary levels)
+    uint32_t stack_bound = 0;          // most stack entries any 4-wide traversal can hold
     uint32_t n_instances = 0;
     float sky[3] = {0, 0, 0};
     uint32_t microfacet = 0;
@@ -36,6 +38,16 @@ struct HostScene {
 uint32_t build_bvh(const std::vector<float>& box_lo, const std::vector<float>& box_hi, uint32_t bins,
                    uint32_t max_leaf, uint32_t prim_base, std::vector<DNode>& nodes, std::vector<uint32_t>& order,
                    uint32_t& depth);
+
+// Collapses the binary tree under child-pair node `root2` into 4-wide nodes
+// appended to `out` (children chosen by opening the largest-area internal
+// child until four); returns the DNode4 index of the root and, in `bound`,
+// the most traversal-stack entries a ray can hold in this subtree.
+uint32_t collapse_bvh4(const std::vector<DNode>& n2, uint32_t root2, std::vector<DNode4>& out, uint32_t& bound);
+
+// Quantises `nc` child boxes (lo/hi, 3 floats each) into a DNode4 whose
+// decoded boxes fma(q, 2^e, origin) contain the originals.
+void quantise_children(const float (*lo)[3], const float (*hi)[3], int nc, DNode4& nd);
 
 // Validates the description and builds everything; returns an error text or "".
 std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out);

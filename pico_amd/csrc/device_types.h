@@ -7,6 +7,10 @@
 //   bit 31 set    -> leaf: bits 30..27 primitive count (1..15), bits 26..0 first
 //                    primitive (BLAS: triangle record, TLAS: instance)
 //   0xffffffff    -> empty child (its box is empty and never hit)
+// The device traverses 4-wide nodes (DNode4) made by collapsing that binary
+// tree: one 64-byte record per node with the four child boxes quantised to
+// 8 bits per plane against the node's own origin and power-of-two scales
+// (rounded outwards, so a child box only grows), and four child references.
 // Triangles are stored in leaf order as (v0, v1 - v0, v2 - v0) with the
 // mesh-local primitive id in v0.w, so Moller-Trumbore needs 48 bytes per test.
 #pragma once
@@ -29,6 +33,17 @@ struct alignas(16) DNode {
     uint32_t pad[2];
 };
 static_assert(sizeof(DNode) == 64, "node is one 64-byte record");
+
+struct alignas(16) DNode4 {
+    float origin[3];        // union lower corner of the children
+    uint32_t exps;          // int8 scale exponents: x | y << 8 | z << 16
+    uint8_t qx[8];          // lo[4], hi[4]: child box = origin + q * 2^e
+    uint8_t qy[8];
+    uint8_t qz[8];
+    uint32_t pad[2];
+    uint32_t ref[4];        // child references (kEmptyRef: no child)
+};
+static_assert(sizeof(DNode4) == 64, "wide node is one 64-byte record");
 
 struct alignas(16) DTri {
     float v0[3]; uint32_t prim;

@@ -113,12 +113,6 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t* head) {
     return __shfl(base, 0);
 }
 
-// Sum over the wave, added once per wave (all 64 lanes must call it).
-__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane_id() == 0 && v) atomicAdd(dst, v);
-}
-
 constexpr uint32_t kDeadPath = 0xffffffffu;
 
 __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
@@ -210,9 +204,16 @@ __global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ v
     }
 }
 
-template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene s, Ctl* ctl, uint32_t b, Pool P) {
-    __shared__ uint32_t stk[STACK * kBlock];
+// Traversal stacks of the persistent trace kernels: kTraceLds entries per
+// lane in LDS, up to kTraceSpill more in device memory (rarely touched).
+constexpr int kTraceLds = 32;
+constexpr uint32_t kTraceSpill = 32;
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
+    __shared__ uint32_t stk[kTraceLds * kBlock];
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     const uint32_t n = ctl->ext_count[b];
     TraceCounters tc{0, 0, 0};
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene s, Ctl* ctl, uint32_
         const uint32_t i = e < n ? q[e] : kDeadPath;
         if (i != kDeadPath) {
             const float4 o = P.ray_o[i], d = P.ray_d[i];
-            const Hit h = trace_closest<STACK, COUNT>(s, xyz(o), xyz(d), stk + threadIdx.x, kBlock, &tc, &ctl->overflow);
+            const Hit h = trace_closest<kTraceLds, COUNT>(s, xyz(o), xyz(d), st, &tc, &ctl->overflow);
             P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.inst));
             P.hprim[i] = h.prim;
         }
@@ -335,9 +336,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
     }
 }
 
-template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_t b, Pool P) {
-    __shared__ uint32_t stk[STACK * kBlock];
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
+    __shared__ uint32_t stk[kTraceLds * kBlock];
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
     const uint32_t n = ctl->sh_count[b];
     TraceCounters tc{0, 0, 0};
     for (;;) {
@@ -346,8 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_
         const uint32_t e = base + lane_id();
         if (e < n) {
             const uint32_t i = P.shq[e];
-            const Hit h = trace_closest<STACK, COUNT>(s, xyz(P.sh_o[i]), xyz(P.sh_d[i]), stk + threadIdx.x, kBlock, &tc,
-                                                      &ctl->overflow);
+            const Hit h = trace_closest<kTraceLds, COUNT>(s, xyz(P.sh_o[i]), xyz(P.sh_d[i]), st, &tc, &ctl->overflow);
             if (h.inst != 0xffffffffu) {
                 const DInstance& I = s.inst[h.inst];
                 if (I.bsdf == PICO_BSDF_LIGHT) {
@@ -515,15 +517,14 @@ struct Plan {
     uint32_t max_spp = 0, max_depth = 0, total_items = 0;
 };
 
-template <int STACK>
 int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, const Pool& P, const DView* dviews, bool count,
-                          uint32_t grid_trace, uint32_t grid_shade, bool with_shadow) {
-    if (count) PICO_LAUNCH(c, "extend", (k_extend<STACK, true>), grid_trace, kBlock, 0, c->dev, ctl, b, P);
-    else PICO_LAUNCH(c, "extend", (k_extend<STACK, false>), grid_trace, kBlock, 0, c->dev, ctl, b, P);
+                          uint32_t grid_trace, uint32_t grid_shade, bool with_shadow, uint32_t* spill, uint32_t spill_cap) {
+    if (count) PICO_LAUNCH(c, "extend", (k_extend<true>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
+    else PICO_LAUNCH(c, "extend", (k_extend<false>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
     PICO_LAUNCH(c, "shade", k_shade, grid_shade, kBlock, 0, c->dev, dviews, ctl, b, P);
     if (with_shadow) {
-        if (count) PICO_LAUNCH(c, "shadow", (k_shadow<STACK, true>), grid_trace, kBlock, 0, c->dev, ctl, b, P);
-        else PICO_LAUNCH(c, "shadow", (k_shadow<STACK, false>), grid_trace, kBlock, 0, c->dev, ctl, b, P);
+        if (count) PICO_LAUNCH(c, "shadow", (k_shadow<true>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
+        else PICO_LAUNCH(c, "shadow", (k_shadow<false>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
     }
     return PICO_OK;
 }
@@ -554,10 +555,25 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     if (pl.max_spp == 0) return PICO_OK;
     uint32_t cap = tune.queue_paths ? tune.queue_paths : (16u << 20);
     cap = std::max<uint32_t>(cap, 64);
-    // samples per chunk: the first chunk ends right after the sample at which
-    // the reference's early-out is first tested (prev_count > 4)
-    const uint32_t s_max = std::max<uint32_t>(1, (uint32_t)std::min<size_t>(pl.max_spp, cap / pix_total));
-    const size_t paths = std::max<size_t>((size_t)s_max * pix_total, 64);
+    const bool accum_mode = hv[0].mode == MODE_ACCUM;
+    // Chunk schedule, fixed on the host: a chunk gives every pixel of the
+    // views still short of their sample count S more samples, S = path budget
+    // / those pixels (views with fewer samples drop out, so later chunks take
+    // more samples at once).  The first chunk of an accumulating render ends
+    // right after the sample at which the reference's early-out is first
+    // tested (prev_count > 4).
+    std::vector<std::pair<uint32_t, uint32_t>> sched;  // (first sample, samples)
+    size_t paths = 64;
+    for (uint32_t off = 0; off < pl.max_spp;) {
+        size_t live = 0;
+        for (auto& v : hv)
+            if (v.spp > off) live += (size_t)v.cam.w * v.cam.h;
+        uint32_t S = (uint32_t)std::max<size_t>(1, std::min<size_t>(cap / live, pl.max_spp - off));
+        if (off == 0 && accum_mode) S = std::min<uint32_t>(S, 6);
+        sched.emplace_back(off, S);
+        paths = std::max(paths, (size_t)S * live);
+        off += S;
+    }
     // buffers
     char* pb = (char*)scratch(c, "pool", paths * (16 * 11 + 4 * 6) + 4096);
     uint32_t* act = (uint32_t*)scratch(c, "active", pix_total * 8 + 64);
@@ -628,34 +644,33 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     }
     // persistent grids: as many workgroups as stay resident at once
     int occ_t = 0, occ_s = 0;
-    if (c->stack_need <= 32) hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, k_extend<32, false>, kBlock, 0);
-    else hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, k_extend<64, false>, kBlock, 0);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, k_extend<false>, kBlock, 0);
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_shade, kBlock, 0);
     const uint32_t grid_trace = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_t, 16));
+    // stack entries beyond the LDS part, per trace lane, in device memory
+    const uint32_t spill_cap = c->stack_need > (uint32_t)kTraceLds ? std::min(c->stack_need - kTraceLds, kTraceSpill) : 0;
+    uint32_t* spill = (uint32_t*)scratch(c, "trace_spill", (size_t)grid_trace * kBlock * kTraceSpill * 4 + 64);
+    if (!spill) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     const uint32_t grid_shade = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_s, 16));
     const uint32_t grid_lin = (uint32_t)c->num_cus * 16;
     const bool per_view_times = n_stats > 1 || hooks;
     std::vector<hipEvent_t> view_render(nv, nullptr), view_done(nv, nullptr);
     bool first = true;
     bool cancelled = false;
-    const bool accum_mode = hv[0].mode == MODE_ACCUM;
-    for (uint32_t off = 0; off < pl.max_spp;) {
+    for (const auto& chunk : sched) {
+        const uint32_t off = chunk.first, S = chunk.second;
         if (cancel) {
             // the flag is host memory written by another thread: look at it once
             // the previous chunk has finished on the device
             if (off) PICO_HIP(c, hipStreamSynchronize(c->stream));
             if (*cancel) { cancelled = true; break; }
         }
-        uint32_t S = s_max;
-        if (off == 0 && accum_mode) S = std::min<uint32_t>(S, 6);
-        S = std::min(S, pl.max_spp - off);
         PICO_LAUNCH(c, "reset_chunk", k_reset_chunk, 1, 256, 0, ctl, first ? 1u : 0u, S);
         first = false;
         PICO_LAUNCH(c, "generate", k_generate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         for (uint32_t b = 0; b <= pl.max_depth; ++b) {
             int rc;
-            if (c->stack_need <= 32) rc = launch_bounce_kernels<32>(c, ctl, b, P, dv, count, grid_trace, grid_shade, b < pl.max_depth);
-            else rc = launch_bounce_kernels<64>(c, ctl, b, P, dv, count, grid_trace, grid_shade, b < pl.max_depth);
+            rc = launch_bounce_kernels(c, ctl, b, P, dv, count, grid_trace, grid_shade, b < pl.max_depth, spill, spill_cap);
             if (rc) return rc;
         }
         PICO_LAUNCH(c, "sum_rays", k_sum_rays, 1, 64, 0, ctl, pl.max_depth);
@@ -688,7 +703,6 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
                 }
             }
         }
-        off += S;
     }
     if (hooks && hooks->cnn) {
         // the caller's stream sees the follow-up's outputs
@@ -934,7 +948,7 @@ int pico_hip_autotune_views(pico_hip_ctx* c, const pico_view* views, uint32_t n,
             hv[i].spp = std::max<uint32_t>(1, (uint32_t)(((uint64_t)views[i].params.spp * calib_spp + max_spp - 1) / max_spp));
         const pico_tuning defaults{16u << 20, 0, 0, 0};
         std::vector<pico_tuning> cand{defaults};
-        const uint32_t queues[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20};
+        const uint32_t queues[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20};
         const uint32_t tiles[][2] = {{0, 0}, {8, 8}, {16, 16}, {32, 4}};
         for (uint32_t q : queues)
             for (auto& t : tiles) {
@@ -943,7 +957,7 @@ int pico_hip_autotune_views(pico_hip_ctx* c, const pico_view* views, uint32_t n,
             }
         // warm-up at the deepest queue: the work buffers reach their largest size
         std::vector<DView> w = hv;
-        if ((rc = run_views(c, w, pico_tuning{32u << 20, 0, 0, 0}, nullptr, false, nullptr, 0))) return rc;
+        if ((rc = run_views(c, w, pico_tuning{64u << 20, 0, 0, 0}, nullptr, false, nullptr, 0))) return rc;
         std::vector<float> best(cand.size(), 1e30f);
         for (int rep = 0; rep < 2; ++rep)
             for (size_t k = 0; k < cand.size(); ++k) {

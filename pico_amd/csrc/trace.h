@@ -96,7 +96,7 @@ struct Rng {
 
 // --------------------------------------------------------------- scene ----
 struct DevScene {
-    const DNode* nodes;
+    const DNode4* nodes;
     const DTri* tris;
     const DInstance* inst;
     const DXform* xf;
@@ -131,16 +131,45 @@ __device__ __forceinline__ f3 slab_rcp(f3 d) {
               1.0f / (fabsf(d.z) > e ? d.z : copysignf(e, d.z)));
 }
 
-// Closest hit of a world-space ray against the TLAS and the instanced BLASes.
-// The traversal stack lives in LDS (`stk`, STACK entries at stride `stride`);
-// near child first, far child pushed, an instance boundary pushes a sentinel
-// that restores the world-space ray when popped.  t is the ray parameter of
-// the world ray (directions are unit length, so t is the hit distance).
-// Box tests widen tmax by 2^-21 relative so that rounding never culls a
-// triangle whose bound the ray only grazes.
+// Traversal stack of one lane: the first STACK entries in LDS (`lds`, stride
+// `lds_stride`), the next `spill_cap` in device memory (`spill`, stride
+// `spill_stride`, private to the lane); pushing beyond both sets `overflow`.
+template <int STACK>
+struct TraceStack {
+    uint32_t* lds;
+    uint32_t lds_stride;
+    uint32_t* spill;
+    uint32_t spill_stride;
+    uint32_t spill_cap;
+    uint32_t sp = 0;
+    __device__ __forceinline__ void push(uint32_t v, uint32_t* overflow) {
+        if (sp < STACK) lds[sp * lds_stride] = v;
+        else if (sp < STACK + spill_cap) spill[(sp - STACK) * spill_stride] = v;
+        else { *overflow = 1u; return; }
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        --sp;
+        return sp < STACK ? lds[sp * lds_stride] : spill[(sp - STACK) * spill_stride];
+    }
+};
+
+__device__ __forceinline__ float exp2_scale(uint32_t bits) {
+    // int8 exponent -> 2^e as a float (e in [-126, 126])
+    return __int_as_float(((int)(int8_t)(bits & 0xffu) + 127) << 23);
+}
+
+// Closest hit of a world-space ray against the TLAS and the instanced BLASes
+// over 4-wide nodes.  Each node decodes its four quantised child boxes
+// (fma(q, 2^e, origin), boxes only grow), tests them with the slab test,
+// continues with the nearest hit child and pushes the others farthest first;
+// an instance boundary pushes a sentinel that restores the world-space ray
+// when popped.  t is the ray parameter of the world ray (directions are unit
+// length, so t is the hit distance).  Box tests widen tmax by 2^-21 relative
+// so that rounding never culls a triangle whose bound the ray only grazes.
 template <int STACK, bool COUNT>
-__device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint32_t* stk, uint32_t stride,
-                                             TraceCounters* cnt, uint32_t* overflow) {
+__device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, TraceStack<STACK>& st, TraceCounters* cnt,
+                                             uint32_t* overflow) {
     // No FP contraction here (the library is built without it): the
     // object-space transform and Moller-Trumbore round exactly as the CPU
     // restatement does; only the box tests use explicit fmaf (conservative,
@@ -151,41 +180,55 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint
     f3 inv = slab_rcp(d);
     f3 oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
     uint32_t ref = s.tlas_root;
-    uint32_t sp = 0;
     uint32_t cur = 0xffffffffu;
     bool in_blas = false;
     const float kWide = 1.0000005f;
+    st.sp = 0;
     for (;;) {
         if (!(ref & kLeafBit)) {
-            const DNode* n = s.nodes + ref;
-            const float4 a = ldg4(&n->c0x[0]);
-            const float4 b = ldg4(&n->c1x[0]);
-            const float4 c = ldg4(&n->c0z[0]);
-            const uint2 r = *reinterpret_cast<const uint2*>(&n->ref[0]);
+            const DNode4* n = s.nodes + ref;
+            const float4 a = ldg4(&n->origin[0]);
+            const uint4 qxy = *reinterpret_cast<const uint4*>(&n->qx[0]);
+            const uint2 qz = *reinterpret_cast<const uint2*>(&n->qz[0]);
+            const uint4 r = *reinterpret_cast<const uint4*>(&n->ref[0]);
             if (COUNT) cnt->nodes++;
-            // child 0
-            const float x00 = fmaf(a.x, inv.x, oi.x), x01 = fmaf(a.y, inv.x, oi.x);
-            const float y00 = fmaf(a.z, inv.y, oi.y), y01 = fmaf(a.w, inv.y, oi.y);
-            const float z00 = fmaf(c.x, inv.z, oi.z), z01 = fmaf(c.y, inv.z, oi.z);
-            const float t0n = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fminf(z00, z01));
-            const float t0f = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fmaxf(z00, z01)) * kWide;
-            // child 1
-            const float x10 = fmaf(b.x, inv.x, oi.x), x11 = fmaf(b.y, inv.x, oi.x);
-            const float y10 = fmaf(b.z, inv.y, oi.y), y11 = fmaf(b.w, inv.y, oi.y);
-            const float z10 = fmaf(c.z, inv.z, oi.z), z11 = fmaf(c.w, inv.z, oi.z);
-            const float t1n = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fminf(z10, z11));
-            const float t1f = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fmaxf(z10, z11)) * kWide;
-            const bool h0 = r.x != kEmptyRef && t0f >= 0.0f && t0n <= t0f && t0n < h.t;
-            const bool h1 = r.y != kEmptyRef && t1f >= 0.0f && t1n <= t1f && t1n < h.t;
-            if (h0 && h1) {
-                const bool first0 = t0n <= t1n;
-                ref = first0 ? r.x : r.y;
-                if (sp < STACK) stk[sp * stride] = first0 ? r.y : r.x, ++sp;
-                else *overflow = 1u;
+            const uint32_t eb = __float_as_uint(a.w);
+            const float sx = exp2_scale(eb), sy = exp2_scale(eb >> 8), sz = exp2_scale(eb >> 16);
+            float key[4];
+            uint32_t cref[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int sh = 8 * i;
+                const float lx = fmaf((float)((qxy.x >> sh) & 0xffu), sx, a.x);
+                const float hx = fmaf((float)((qxy.y >> sh) & 0xffu), sx, a.x);
+                const float ly = fmaf((float)((qxy.z >> sh) & 0xffu), sy, a.y);
+                const float hy = fmaf((float)((qxy.w >> sh) & 0xffu), sy, a.y);
+                const float lz = fmaf((float)((qz.x >> sh) & 0xffu), sz, a.z);
+                const float hz = fmaf((float)((qz.y >> sh) & 0xffu), sz, a.z);
+                const float x0 = fmaf(lx, inv.x, oi.x), x1 = fmaf(hx, inv.x, oi.x);
+                const float y0 = fmaf(ly, inv.y, oi.y), y1 = fmaf(hy, inv.y, oi.y);
+                const float z0 = fmaf(lz, inv.z, oi.z), z1 = fmaf(hz, inv.z, oi.z);
+                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kWide;
+                const bool hit = cref[i] != kEmptyRef && tf >= 0.0f && tn <= tf && tn < h.t;
+                key[i] = hit ? tn : INFINITY;
+            }
+            // sort the four (distance, child) pairs, nearest first
+#define PICO_CSWAP(i, j)                                                  \
+            if (key[j] < key[i]) {                                        \
+                const float tk = key[i]; key[i] = key[j]; key[j] = tk;    \
+                const uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
+            }
+            PICO_CSWAP(0, 1) PICO_CSWAP(2, 3) PICO_CSWAP(0, 2) PICO_CSWAP(1, 3) PICO_CSWAP(1, 2)
+#undef PICO_CSWAP
+            const int nh = (key[0] != INFINITY) + (key[1] != INFINITY) + (key[2] != INFINITY) + (key[3] != INFINITY);
+            if (nh > 0) {
+                if (nh > 3) st.push(cref[3], overflow);
+                if (nh > 2) st.push(cref[2], overflow);
+                if (nh > 1) st.push(cref[1], overflow);
+                ref = cref[0];
                 continue;
             }
-            if (h0) { ref = r.x; continue; }
-            if (h1) { ref = r.y; continue; }
         } else if (in_blas) {
             const uint32_t first = ref & kPrimMask, count = (ref >> 27) & 15u;
             for (uint32_t k = 0; k < count; ++k) {
@@ -221,8 +264,7 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint
             const float4 m2 = ldg4(&I->inv[2][0]);
             const uint32_t root = I->blas_root;
             if (COUNT) cnt->insts++;
-            if (sp < STACK) stk[sp * stride] = kSentinel, ++sp;
-            else *overflow = 1u;
+            st.push(kSentinel, overflow);
             // transform_ray (Source/Core/AABB.cpp:43-52): M^-1 (o, 1) as
             // (c0*x + c1*y) + (c2*z + c3), mat3(M^-1) d as (c0*x + c1*y) + c2*z
             ro = mk((m0.x * o.x + m0.y * o.y) + (m0.z * o.z + m0.w), (m1.x * o.x + m1.y * o.y) + (m1.z * o.z + m1.w),
@@ -237,9 +279,8 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, uint
         }
         // pop
         for (;;) {
-            if (sp == 0) return h;
-            --sp;
-            ref = stk[sp * stride];
+            if (st.sp == 0) return h;
+            ref = st.pop();
             if (ref != kSentinel) break;
             ro = o; rd = d;
             inv = slab_rcp(d);
