@@ -60,52 +60,90 @@ struct ConvArgs {
     int B, H, W, kpad, npad, taps, flip, relu;
 };
 
-// Implicit-GEMM convolution on MFMA 16x16x16 bf16.  grid.x: row tiles of 64
-// pixels (4 waves x 16), grid.y: blocks of 32 output channels.
+// Implicit-GEMM convolution on MFMA 16x16x16 bf16.  A workgroup of 4 waves
+// owns a run of 4 * MT * 16 pixels of one image row and NT * 16 output
+// channels; each wave computes MT x NT 16x16 tiles (MT * NT = 8 accumulators,
+// so every A fragment feeds NT instructions and every B fragment MT).  Per
+// tap the workgroup stages that tap's weights for its channels in LDS (rows
+// padded by 4 bf16 against bank conflicts); A fragments (4 channels of one
+// pixel at the tap's offset, zero outside the image) come from memory.
+// grid.x: row runs, grid.y: channel blocks of NT * 16.
+template <int MT, int NT>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+    extern __shared__ bf16 wl[];  // [NT * 16][kpad + 4]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tiles_x = (a.W + 63) / 64;
-    const int row = blockIdx.x / tiles_x;         // b*H + y
-    const int x0 = (blockIdx.x % tiles_x) * 64 + wave * 16;
-    if (x0 >= a.W) return;
+    constexpr int kRun = 4 * MT * 16;
+    const int runs_x = (a.W + kRun - 1) / kRun;
+    const int row = blockIdx.x / runs_x;  // b*H + y
+    const int xw = (blockIdx.x % runs_x) * kRun + wave * MT * 16;
     const int b = row / a.H, y = row % a.H;
-    const int n0 = blockIdx.y * 32;
+    const int n0 = blockIdx.y * NT * 16;
     const int mi = lane & 15, kq = lane >> 4;
-    f4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-    const bf16* w0 = a.w + (size_t)(n0 + mi) * a.kpad + 4 * kq;
-    const bf16* w1 = w0 + (size_t)16 * a.kpad;
-    for (int t = 0; t < a.taps; ++t) {
-        int dy = a.taps == 9 ? t / 3 - 1 : 0, dx = a.taps == 9 ? t % 3 - 1 : 0;
+    const int ldk = a.kpad + 4;
+    f4v acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    const int kq4 = a.kpad / 4;
+    for (int tap = 0; tap < a.taps; ++tap) {
+        int dy = a.taps == 9 ? tap / 3 - 1 : 0, dx = a.taps == 9 ? tap % 3 - 1 : 0;
         if (a.flip) { dy = -dy; dx = -dx; }
-        const int yy = y + dy, xx = x0 + mi + dx;
-        const bool valid = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-        const bf16* ap = a.in + ((size_t)(b * a.H + (valid ? yy : 0)) * a.W + (valid ? xx : 0)) * a.in_stride + a.in_off + 4 * kq;
-        const size_t wt = (size_t)t * a.npad * a.kpad;
+        __syncthreads();
+        {
+            const bf16* wt = a.w + ((size_t)tap * a.npad + n0) * a.kpad;
+            for (int c = threadIdx.x; c < NT * 16 * kq4; c += 256) {
+                const int nn = c / kq4, k4 = c - nn * kq4;
+                *reinterpret_cast<s4*>(wl + nn * ldk + 4 * k4) = *reinterpret_cast<const s4*>(wt + (size_t)nn * a.kpad + 4 * k4);
+            }
+        }
+        __syncthreads();
+        if (xw >= a.W) continue;
+        const int yy = y + dy;
+        const bool rok = yy >= 0 && yy < a.H;
+        const bf16* ap[MT];
+        bool ok[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int xx = xw + m * 16 + mi + dx;
+            ok[m] = rok && xx >= 0 && xx < a.W;
+            ap[m] = a.in + ((size_t)(b * a.H + (ok[m] ? yy : 0)) * a.W + (ok[m] ? xx : 0)) * a.in_stride + a.in_off + 4 * kq;
+        }
+        const bf16* bp = wl + mi * ldk + 4 * kq;
         for (int k0 = 0; k0 < a.kpad; k0 += 16) {
-            s4 av = {0, 0, 0, 0};
-            if (valid) av = *reinterpret_cast<const s4*>(ap + k0);
-            const s4 b0 = *reinterpret_cast<const s4*>(w0 + wt + k0);
-            const s4 b1 = *reinterpret_cast<const s4*>(w1 + wt + k0);
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b1, acc1, 0, 0, 0);
+            s4 av[MT], bv[NT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                av[m] = s4{0, 0, 0, 0};
+                if (ok[m]) av[m] = *reinterpret_cast<const s4*>(ap[m] + k0);
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const s4*>(bp + t * 16 * ldk + k0);
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[t], acc[m][t], 0, 0, 0);
         }
     }
+    if (xw >= a.W) return;
     // D[i][j]: pixel i = 4*kq + r, channel j = mi
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) {
-        const int n = n0 + nt * 16 + mi;
+    for (int t = 0; t < NT; ++t) {
+        const int n = n0 + t * 16 + mi;
         const float bias = a.bias ? a.bias[n] : 0.0f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int x = x0 + 4 * kq + r;
-            if (x >= a.W) continue;
-            float v = (nt ? acc1[r] : acc0[r]) + bias;
-            if (a.relu) v = v > 0.0f ? v : 0.0f;
-            const size_t p = (size_t)(b * a.H + y) * a.W + x;
-            if (a.mask && !(bf2f(a.mask[p * a.mask_stride + a.mask_off + n]) > 0.0f)) v = 0.0f;
-            if (a.out_f32) a.out_f32[p * a.npad + n] = v;
-            else a.out[p * a.out_stride + a.out_off + n] = f2bf(v);
-        }
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = xw + m * 16 + 4 * kq + r;
+                if (x >= a.W) continue;
+                float v = acc[m][t][r] + bias;
+                if (a.relu) v = v > 0.0f ? v : 0.0f;
+                const size_t p = (size_t)(b * a.H + y) * a.W + x;
+                if (a.mask && !(bf2f(a.mask[p * a.mask_stride + a.mask_off + n]) > 0.0f)) v = 0.0f;
+                if (a.out_f32) a.out_f32[p * a.npad + n] = v;
+                else a.out[p * a.out_stride + a.out_off + n] = f2bf(v);
+            }
     }
 }
 
@@ -423,14 +461,25 @@ T* dalloc(pico_cnn* n, size_t count) {
     return (T*)p;
 }
 
+// Launches k_conv with the widest channel block (16, 32 or 64) that divides
+// the output channels, and 8 accumulators per wave.
+int launch_conv(pico_cnn* n, const char* name, const ConvArgs& a) {
+    const int nt = (a.npad % 64 == 0) ? 4 : (a.npad % 32 == 0) ? 2 : 1;
+    const int run = 4 * (8 / nt) * 16;
+    const dim3 grid((unsigned)(a.B * a.H * ((a.W + run - 1) / run)), (unsigned)(a.npad / (nt * 16)));
+    const size_t lds = (size_t)nt * 16 * (a.kpad + 4) * sizeof(bf16);
+    if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv<2, 4>), grid, 256, lds, a);
+    else if (nt == 2) PICO_LAUNCH(n->c, name, (k_conv<4, 2>), grid, 256, lds, a);
+    else PICO_LAUNCH(n->c, name, (k_conv<8, 1>), grid, 256, lds, a);
+    return PICO_OK;
+}
+
 int conv(pico_cnn* n, int l, const bf16* in, int in_stride, int in_off, bf16* out, int out_stride, int out_off, int B,
          int H, int W, bool relu, float* out_f32 = nullptr) {
     const LayerDef& L = kL[l];
     ConvArgs a{in, in_stride, in_off, n->wf[l], n->bpad[l], out, out_stride, out_off, out_f32, nullptr, 0, 0,
                B, H, W, L.kpad, L.npad, L.taps, 0, relu ? 1 : 0};
-    dim3 grid((unsigned)(B * H * ((W + 63) / 64)), (unsigned)(L.npad / 32));
-    PICO_LAUNCH(n->c, "cnn_conv_fwd", k_conv, grid, 256, 0, a);
-    return PICO_OK;
+    return launch_conv(n, "cnn_conv_fwd", a);
 }
 
 // dX of layer l: gradient wrt its input (kpad channels), masked by `mask`
@@ -439,9 +488,7 @@ int conv_dx(pico_cnn* n, int l, const bf16* g, int g_stride, bf16* dx, int dx_st
     const LayerDef& L = kL[l];
     ConvArgs a{g, g_stride, 0, n->wb[l], nullptr, dx, dx_stride, dx_off, nullptr, mask, mask_stride, mask_off,
                B, H, W, L.npad, L.kpad, L.taps, 1, 0};
-    dim3 grid((unsigned)(B * H * ((W + 63) / 64)), (unsigned)(L.kpad / 32));
-    PICO_LAUNCH(n->c, "cnn_conv_bwd_data", k_conv, grid, 256, 0, a);
-    return PICO_OK;
+    return launch_conv(n, "cnn_conv_bwd_data", a);
 }
 
 int conv_dw(pico_cnn* n, int l, const bf16* x, int x_stride, int x_off, const bf16* g, int g_stride, int B, int H, int W) {
