@@ -251,17 +251,25 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3):
     if weights and os.path.exists(weights):
         net.load(weights)
     outs = [ctx.alloc(12 * int(sizes[i]) ** 2) for i in range(n_views)]
-    ctx.render_views_denoise_d(views, net, outs)  # warm
-    runs = []
-    for _ in range(reps):
-        ctx.sync()
-        t0 = time.perf_counter()
-        st = ctx.render_views_denoise_d(views, net, outs)
-        ctx.sync()
-        runs.append((time.perf_counter() - t0, st))
-    wall, st = sorted(runs, key=lambda r: r[0])[len(runs) // 2]
+    def timed(flags):
+        ctx.set_tuning(tuning[0], tuning[1:3], flags)
+        ctx.render_views_denoise_d(views, net, outs)  # warm
+        runs = []
+        for _ in range(reps):
+            ctx.sync()
+            t0 = time.perf_counter()
+            st = ctx.render_views_denoise_d(views, net, outs)
+            ctx.sync()
+            runs.append((time.perf_counter() - t0, st))
+        return sorted(runs, key=lambda r: r[0])[len(runs) // 2]
+
+    # the denoiser between chunks on the launch's stream, or on a second stream
+    wall, st = timed(0)
+    wall2, st2 = timed(pa.TUNE_SIDE_STREAM)
     rays = sum(s.accumulated_rays for s in st)
     lat = np.array([s.done_ms for s in st])
+    rlat = np.array([s.render_ms for s in st])
+    lat2 = np.array([s.done_ms for s in st2])
     # the same views one at a time (render, G-buffer, network), for comparison
     d_n, d_p, d_a = (ctx.alloc(12 * 1024 * 1024) for _ in range(3))
     ctx.sync()
@@ -286,6 +294,11 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3):
             "value": round(rays / wall / 1e6, 2), "unit": "Mrays/s", "ms_per_batch": round(wall * 1e3, 2),
             "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
             "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "render_latency_ms_p50": round(float(np.percentile(rlat, 50)), 2),
+            "render_latency_ms_p99": round(float(np.percentile(rlat, 99)), 2),
+            "second_stream": {"ms_per_batch": round(wall2 * 1e3, 2),
+                              "latency_ms_p50": round(float(np.percentile(lat2, 50)), 2),
+                              "latency_ms_p99": round(float(np.percentile(lat2, 99)), 2)},
             "sequential_ms": round(seq * 1e3, 2),
             "tuning": {"queue_paths": tuning[0], "tile_w": tuning[1], "tile_h": tuning[2],
                        "calib_ms": round(cal_ms, 2), "default_calib_ms": round(def_ms, 2)}}

@@ -319,8 +319,11 @@ typedef struct pico_tuning {
     uint32_t queue_paths;
     uint32_t tile_w;
     uint32_t tile_h;
-    uint32_t reserved;
+    uint32_t flags;          /* PICO_TUNE_* */
 } pico_tuning;
+/* pico_hip_render_views_denoise_d runs each view's denoiser on a second
+ * stream (overlapping the remaining passes) instead of between two chunks. */
+#define PICO_TUNE_SIDE_STREAM 1u
 int  pico_hip_set_tuning(pico_hip_ctx* ctx, const pico_tuning* tuning);
 int  pico_hip_get_tuning(pico_hip_ctx* ctx, pico_tuning* tuning);
 

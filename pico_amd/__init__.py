@@ -33,6 +33,7 @@ PICO_ERR_NOMEM = -6
 BSDF_DIFFUSE, BSDF_SPECULAR, BSDF_DIELECTRIC, BSDF_LIGHT, BSDF_DELTA = range(5)
 MICROFACET_BECKMANN_ALL, MICROFACET_GGX = 0, 1
 RENDER_RESUME, RENDER_COUNT = 1, 2
+TUNE_SIDE_STREAM = 1
 # fixed-width a-trous: normal, relative depth, albedo, colour (0 = term off)
 ATROUS_FIXED_SIGMAS = (0.5, 0.05, 0.1, 0.0)
 
@@ -82,10 +83,10 @@ class Stats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("queue_paths", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("reserved", C.c_uint32)]
+    _fields_ = [("queue_paths", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("flags", C.c_uint32)]
 
     def as_tuple(self):
-        return (self.queue_paths, self.tile_w, self.tile_h)
+        return (self.queue_paths, self.tile_w, self.tile_h, self.flags)
 
 
 class View(C.Structure):
@@ -432,8 +433,8 @@ class Context:
         self._check(lib().pico_hip_render_views_d(self._h, arr, len(views), queue_paths, st))
         return list(st)
 
-    def set_tuning(self, queue_paths=16 << 20, tile=(0, 0)):
-        t = Tuning(queue_paths, tile[0], tile[1], 0)
+    def set_tuning(self, queue_paths=16 << 20, tile=(0, 0), flags=0):
+        t = Tuning(queue_paths, tile[0], tile[1], flags)
         self._check(lib().pico_hip_set_tuning(self._h, C.byref(t)))
 
     def get_tuning(self):

@@ -32,61 +32,73 @@ namespace {
 
 constexpr int kRayBlock = 128;
 
-template <int STACK>
-__global__ __launch_bounds__(kRayBlock) void k_closest_hit(DevScene s, uint32_t n, const float* __restrict__ o,
-                                                            const float* __restrict__ d, float* t, uint32_t* inst,
-                                                            uint32_t* prim, float* uv, uint32_t* overflow) {
-    __shared__ uint32_t stk[STACK * kRayBlock];
-    TraceStack<STACK> st{stk + threadIdx.x, kRayBlock, nullptr, 0, 0};
-    const uint32_t i = blockIdx.x * kRayBlock + threadIdx.x;
-    if (i >= n) return;
-    Hit h = trace_closest<STACK, false>(s, ld3(o + 3 * i), ld3(d + 3 * i), st, nullptr, overflow);
-    t[i] = h.t;
-    inst[i] = h.inst;
-    prim[i] = h.prim;
-    uv[2 * i] = h.u;
-    uv[2 * i + 1] = h.v;
+// One-ray-per-item kernels over persistent grids (as many workgroups as stay
+// resident), each lane walking the items in grid stride.  Traversal stacks:
+// kLds entries per lane in LDS, the rest (up to kSpill) in device memory.
+constexpr int kLds = 32;
+constexpr uint32_t kSpill = 32;
+
+struct StackArgs { uint32_t* spill; uint32_t cap; };
+
+__device__ __forceinline__ TraceStack<kLds> lane_stack(uint32_t* stk, StackArgs sa) {
+    const uint32_t g = blockIdx.x * kRayBlock + threadIdx.x;
+    return TraceStack<kLds>{stk + threadIdx.x, kRayBlock, sa.spill + g, gridDim.x * kRayBlock, sa.cap};
 }
 
-template <int STACK>
+__global__ __launch_bounds__(kRayBlock) void k_closest_hit(DevScene s, uint32_t n, const float* __restrict__ o,
+                                                            const float* __restrict__ d, float* t, uint32_t* inst,
+                                                            uint32_t* prim, float* uv, uint32_t* overflow, StackArgs sa) {
+    __shared__ uint32_t stk[kLds * kRayBlock];
+    TraceStack<kLds> st = lane_stack(stk, sa);
+    for (uint32_t i = blockIdx.x * kRayBlock + threadIdx.x; i < n; i += gridDim.x * kRayBlock) {
+        Hit h = trace_closest<kLds, false>(s, ld3(o + 3 * i), ld3(d + 3 * i), st, nullptr, overflow);
+        t[i] = h.t;
+        inst[i] = h.inst;
+        prim[i] = h.prim;
+        uv[2 * i] = h.u;
+        uv[2 * i + 1] = h.v;
+    }
+}
+
 __global__ __launch_bounds__(kRayBlock) void k_primary_ids(DevScene s, DCam cam, float2 xi, uint32_t* inst, uint32_t* prim,
-                                                            uint32_t* overflow) {
-    __shared__ uint32_t stk[STACK * kRayBlock];
-    TraceStack<STACK> st{stk + threadIdx.x, kRayBlock, nullptr, 0, 0};
-    const uint32_t i = blockIdx.x * kRayBlock + threadIdx.x;
-    if (i >= cam.w * cam.h) return;
-    const f3 d = camera_dir(cam, xi, i % cam.w, i / cam.w);
-    Hit h = trace_closest<STACK, false>(s, cam.pos, d, st, nullptr, overflow);
-    inst[i] = h.inst;
-    prim[i] = h.prim;
+                                                            uint32_t* overflow, StackArgs sa) {
+    __shared__ uint32_t stk[kLds * kRayBlock];
+    TraceStack<kLds> st = lane_stack(stk, sa);
+    for (uint32_t i = blockIdx.x * kRayBlock + threadIdx.x; i < cam.w * cam.h; i += gridDim.x * kRayBlock) {
+        const f3 d = camera_dir(cam, xi, i % cam.w, i / cam.w);
+        Hit h = trace_closest<kLds, false>(s, cam.pos, d, st, nullptr, overflow);
+        inst[i] = h.inst;
+        prim[i] = h.prim;
+    }
 }
 
 // Scene::generate_denoiser_inputs, Source/Core/Scene.cpp:265-286.
-template <int STACK>
 __global__ __launch_bounds__(kRayBlock) void k_gbuffer(DevScene s, DCam cam, float4 v0, float4 v1, float4 v2, float4 v3,
-                                                        float* normal, float* position, float* albedo, uint32_t* overflow) {
-    __shared__ uint32_t stk[STACK * kRayBlock];
-    TraceStack<STACK> st{stk + threadIdx.x, kRayBlock, nullptr, 0, 0};
-    const uint32_t i = blockIdx.x * kRayBlock + threadIdx.x;
-    if (i >= cam.w * cam.h) return;
-    const f3 d = camera_dir(cam, make_float2(0.0f, 0.0f), i % cam.w, i / cam.w);
-    Hit h = trace_closest<STACK, false>(s, cam.pos, d, st, nullptr, overflow);
-    f3 n, p, a;
-    if (h.inst != 0xffffffffu) {
-        const Frag f = make_frag(s, h);
-        n = f.n;
-        // view matrix (glm::lookAt) times (p, 1): (c0*x + c1*y) + (c2*z + c3)
-        p = mk((v0.x * f.p.x + v1.x * f.p.y) + (v2.x * f.p.z + v3.x), (v0.y * f.p.x + v1.y * f.p.y) + (v2.y * f.p.z + v3.y),
-               (v0.z * f.p.x + v1.z * f.p.y) + (v2.z * f.p.z + v3.z));
-        a = ld3(s.mats[s.inst[h.inst].material].diffuse);
-    } else {
-        n = -d;
-        p = d * cam.far_plane;
-        a = mk(s.sky[0], s.sky[1], s.sky[2]);
+                                                        float* normal, float* position, float* albedo, uint32_t* overflow,
+                                                        StackArgs sa) {
+    __shared__ uint32_t stk[kLds * kRayBlock];
+    TraceStack<kLds> st = lane_stack(stk, sa);
+    for (uint32_t i = blockIdx.x * kRayBlock + threadIdx.x; i < cam.w * cam.h; i += gridDim.x * kRayBlock) {
+        const f3 d = camera_dir(cam, make_float2(0.0f, 0.0f), i % cam.w, i / cam.w);
+        Hit h = trace_closest<kLds, false>(s, cam.pos, d, st, nullptr, overflow);
+        f3 n, p, a;
+        if (h.inst != 0xffffffffu) {
+            const Frag f = make_frag(s, h);
+            n = f.n;
+            // view matrix (glm::lookAt) times (p, 1): (c0*x + c1*y) + (c2*z + c3)
+            p = mk((v0.x * f.p.x + v1.x * f.p.y) + (v2.x * f.p.z + v3.x),
+                   (v0.y * f.p.x + v1.y * f.p.y) + (v2.y * f.p.z + v3.y),
+                   (v0.z * f.p.x + v1.z * f.p.y) + (v2.z * f.p.z + v3.z));
+            a = ld3(s.mats[s.inst[h.inst].material].diffuse);
+        } else {
+            n = -d;
+            p = d * cam.far_plane;
+            a = mk(s.sky[0], s.sky[1], s.sky[2]);
+        }
+        normal[3 * i] = n.x; normal[3 * i + 1] = n.y; normal[3 * i + 2] = n.z;
+        position[3 * i] = p.x; position[3 * i + 1] = p.y; position[3 * i + 2] = p.z;
+        albedo[3 * i] = a.x; albedo[3 * i + 1] = a.y; albedo[3 * i + 2] = a.z;
     }
-    normal[3 * i] = n.x; normal[3 * i + 1] = n.y; normal[3 * i + 2] = n.z;
-    position[3 * i] = p.x; position[3 * i + 1] = p.y; position[3 * i + 2] = p.z;
-    albedo[3 * i] = a.x; albedo[3 * i + 1] = a.y; albedo[3 * i + 2] = a.z;
 }
 
 template <class T>
@@ -136,12 +148,18 @@ int check_camera(pico_hip_ctx* c, const pico_camera* cam, uint32_t w, uint32_t h
     return PICO_OK;
 }
 
-// Launch a one-ray-per-thread kernel specialised on the traversal stack depth.
-#define PICO_STACK_DISPATCH(ctx, NAME, KERNEL, N, ...)                                          \
-    do {                                                                                        \
-        const uint32_t grid_ = ((N) + kRayBlock - 1) / kRayBlock;                                \
-        if ((ctx)->stack_need <= 32) PICO_LAUNCH(ctx, NAME, KERNEL<32>, grid_, kRayBlock, 0, __VA_ARGS__); \
-        else PICO_LAUNCH(ctx, NAME, KERNEL<64>, grid_, kRayBlock, 0, __VA_ARGS__);               \
+// Launch a one-ray-per-item kernel on a persistent grid with its stack spill area.
+#define PICO_RAY_LAUNCH(ctx, NAME, KERNEL, N, ...)                                                          \
+    do {                                                                                                    \
+        int occ_ = 0;                                                                                       \
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_, KERNEL, kRayBlock, 0);                           \
+        const uint32_t want_ = ((N) + kRayBlock - 1) / kRayBlock;                                            \
+        const uint32_t grid_ = std::max<uint32_t>(1, std::min<uint32_t>(want_, (uint32_t)(ctx)->num_cus *   \
+                                                                       (uint32_t)std::max(1, std::min(occ_, 16)))); \
+        StackArgs sa_{(uint32_t*)scratch(ctx, "ray_spill", (size_t)grid_ * kRayBlock * kSpill * 4 + 64),     \
+                      (ctx)->stack_need > (uint32_t)kLds ? std::min((ctx)->stack_need - kLds, kSpill) : 0u};   \
+        if (!sa_.spill) return fail(ctx, PICO_ERR_NOMEM, "device allocation failed");                        \
+        PICO_LAUNCH(ctx, NAME, KERNEL, grid_, kRayBlock, 0, __VA_ARGS__, sa_);                               \
     } while (0)
 
 }  // namespace pico
@@ -328,7 +346,7 @@ int pico_hip_closest_hit(pico_hip_ctx* c, uint32_t n, const float* o, const floa
     PICO_HIP(c, hipMemcpyAsync(dO, o, 12 * (size_t)n, hipMemcpyHostToDevice, c->stream));
     PICO_HIP(c, hipMemcpyAsync(dD, d, 12 * (size_t)n, hipMemcpyHostToDevice, c->stream));
     PICO_HIP(c, hipMemsetAsync(ovf, 0, 4, c->stream));
-    PICO_STACK_DISPATCH(c, "closest_hit", k_closest_hit, n, c->dev, n, dO, dD, dT, dI, dP, dUV, ovf);
+    PICO_RAY_LAUNCH(c, "closest_hit", k_closest_hit, n, c->dev, n, dO, dD, dT, dI, dP, dUV, ovf);
     PICO_HIP(c, hipMemcpyAsync(t, dT, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipMemcpyAsync(inst, dI, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipMemcpyAsync(prim, dP, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
@@ -353,7 +371,7 @@ int pico_hip_primary_ids(pico_hip_ctx* c, const pico_camera* cam, uint32_t w, ui
     uint32_t* ovf = b + 2 * n;
     PICO_HIP(c, hipMemsetAsync(ovf, 0, 4, c->stream));
     const DCam dc = make_dcam(*cam, w, h);
-    PICO_STACK_DISPATCH(c, "primary_ids", k_primary_ids, (uint32_t)n, c->dev, dc, make_float2(xx, xy), b, b + n, ovf);
+    PICO_RAY_LAUNCH(c, "primary_ids", k_primary_ids, (uint32_t)n, c->dev, dc, make_float2(xx, xy), b, b + n, ovf);
     PICO_HIP(c, hipMemcpyAsync(inst, b, 4 * n, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipMemcpyAsync(prim, b + n, 4 * n, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
@@ -385,7 +403,7 @@ int pico_hip_gbuffer_d(pico_hip_ctx* c, const pico_camera* cam, uint32_t w, uint
                                   f[0] * E[0] + f[1] * E[1] + f[2] * E[2], 1.0f);
     PICO_HIP(c, hipMemsetAsync(ovf, 0, 4, c->stream));
     const DCam dc = make_dcam(*cam, w, h);
-    PICO_STACK_DISPATCH(c, "gbuffer", k_gbuffer, w * h, c->dev, dc, v0, v1, v2, v3, dn, dp, da, ovf);
+    PICO_RAY_LAUNCH(c, "gbuffer", k_gbuffer, w * h, c->dev, dc, v0, v1, v2, v3, dn, dp, da, ovf);
     return PICO_OK;
 }
 

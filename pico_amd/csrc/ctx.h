@@ -44,7 +44,7 @@ struct pico_hip_ctx {
     // named scratch buffers (grown on demand, freed at destroy)
     std::map<std::string, DevBuf> bufs;
     // launch shape of the wavefront and the autotuner's cache
-    pico_tuning tuning{16u << 20, 0, 0, 0};
+    pico_tuning tuning{16u << 20, 0, 0, 0};  // queue depth, tile, flags
     std::map<uint64_t, TuneEntry> tune_cache;
     // timing
     bool timing = false;

@@ -113,6 +113,27 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t* head) {
     return __shfl(base, 0);
 }
 
+// Workgroup-aggregated append: one atomic per workgroup and call, slots in
+// wave then lane order.  Every thread of the workgroup must call it.
+template <int NW>
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred) {
+    __shared__ uint32_t wcount[NW];
+    __shared__ uint32_t wbase;
+    const uint64_t m = __ballot(pred);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < NW; ++w) { const uint32_t c = wcount[w]; wcount[w] = tot; tot += c; }
+        wbase = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = wbase + wcount[wave] + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+    __syncthreads();  // the next call reuses wcount / wbase
+    return r;
+}
+
 constexpr uint32_t kDeadPath = 0xffffffffu;
 
 __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
@@ -168,7 +189,7 @@ __global__ void k_init_view(DView* views, uint32_t vi, uint32_t* active, Ctl* ct
                 V.done[p] = keep ? 0 : 1;
             }
         }
-        const uint32_t slot = wave_append(&ctl->n_active, keep);
+        const uint32_t slot = block_append<4>(&ctl->n_active, keep);
         if (keep) active[slot] = (vi << kViewShift) | p;
     }
 }
@@ -303,8 +324,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
                         const f3 to_light = normalize(wp - f.p);
                         if (!(dot(to_light, f.n) < 0.0f)) {
                             const float pdf = direct_pdf(kind, s.microfacet, M, f.n, -dir, to_light);
+                            // the shadow ray's closest hit lies before the sampled
+                            // light point (the light itself, or a blocker), so the
+                            // query stops a margin past it
+                            const f3 dl = wp - f.p;
                             P.sh_o[i] = f4(f.p + 0.01f * to_light, 0.0f);
-                            P.sh_d[i] = f4(to_light, 0.0f);
+                            P.sh_d[i] = f4(to_light, 1.01f * sqrtf(dot(dl, dl)) + 1e-3f);
                             P.sh_tp[i] = f4(thr * pdf, 0.0f);
                             P.sh_en[i] = f4(smp.energy, 0.0f);
                             shadow = true;
@@ -349,7 +374,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_
         const uint32_t e = base + lane_id();
         if (e < n) {
             const uint32_t i = P.shq[e];
-            const Hit h = trace_closest<kTraceLds, COUNT>(s, xyz(P.sh_o[i]), xyz(P.sh_d[i]), st, &tc, &ctl->overflow);
+            const float4 sd = P.sh_d[i];
+            const Hit h = trace_closest<kTraceLds, COUNT>(s, xyz(P.sh_o[i]), xyz(sd), st, &tc, &ctl->overflow, sd.w);
             if (h.inst != 0xffffffffu) {
                 const DInstance& I = s.inst[h.inst];
                 if (I.bsdf == PICO_BSDF_LIGHT) {
@@ -368,32 +394,41 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_
     }
 }
 
-// Adds (rays, samples) to the view's counters: one atomic pair per wave when
-// every lane of the wave is on the same view, per lane otherwise.
-__device__ __forceinline__ void add_view_stats(const DView* views, uint32_t view, unsigned long long rays,
-                                               unsigned long long ns) {
-    const uint32_t v0 = __shfl(view, 0);
-    const bool uniform = __ballot(view != v0 && view != 0xffffffffu) == 0ull;
-    if (uniform) {
-        if (v0 == 0xffffffffu) {
-            // lane 0 idle: find any active view
-            const uint64_t act = __ballot(view != 0xffffffffu);
-            if (!act) return;
-            const uint32_t vv = __shfl(view, (int)(__ffsll((unsigned long long)act) - 1));
-            for (int o = 32; o > 0; o >>= 1) { rays += __shfl_xor(rays, o); ns += __shfl_xor(ns, o); }
-            if (lane_id() == 0) { atomicAdd(&views[vv].stats[0], rays); atomicAdd(&views[vv].stats[1], ns); }
-            return;
-        }
-        for (int o = 32; o > 0; o >>= 1) { rays += __shfl_xor(rays, o); ns += __shfl_xor(ns, o); }
-        if (lane_id() == 0) { atomicAdd(&views[v0].stats[0], rays); atomicAdd(&views[v0].stats[1], ns); }
-    } else if (view != 0xffffffffu) {
-        atomicAdd(&views[view].stats[0], rays);
-        atomicAdd(&views[view].stats[1], ns);
+// Per-workgroup (rays, samples) counters of every view in LDS: items add to
+// them (one LDS atomic pair per wave when the wave is on one view), and the
+// workgroup adds its non-zero totals to the views' counters once at the end.
+struct BlockViewStats {
+    unsigned long long v[kMaxViews][2];
+    __device__ __forceinline__ void init() {
+        for (uint32_t i = threadIdx.x; i < 2 * kMaxViews; i += blockDim.x) (&v[0][0])[i] = 0ull;
+        __syncthreads();
     }
-}
+    __device__ __forceinline__ void add(uint32_t view, unsigned long long rays, unsigned long long ns) {
+        const uint64_t act = __ballot(view != 0xffffffffu);
+        if (!act) return;
+        const uint32_t vv = __shfl(view, (int)(__ffsll((unsigned long long)act) - 1));
+        if (__ballot(view != vv && view != 0xffffffffu) == 0ull) {
+            for (int o = 32; o > 0; o >>= 1) { rays += __shfl_xor(rays, o); ns += __shfl_xor(ns, o); }
+            if (lane_id() == 0) { atomicAdd(&v[vv][0], rays); atomicAdd(&v[vv][1], ns); }
+        } else if (view != 0xffffffffu) {
+            atomicAdd(&v[view][0], rays);
+            atomicAdd(&v[view][1], ns);
+        }
+    }
+    __device__ __forceinline__ void flush(const DView* views) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kMaxViews; i += blockDim.x)
+            if (v[i][0] | v[i][1]) {
+                atomicAdd(&views[i].stats[0], v[i][0]);
+                atomicAdd(&views[i].stats[1], v[i][1]);
+            }
+    }
+};
 
 __global__ __launch_bounds__(kBlock) void k_finalize(const DView* __restrict__ views, const uint32_t* __restrict__ active,
                                                      const Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
+    __shared__ BlockViewStats bvs;
+    bvs.init();
     const uint32_t na = ctl->n_active;
     const uint32_t total = na * S;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
@@ -420,14 +455,17 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DView* __restrict__ v
                 }
             }
         }
-        add_view_stats(views, vstat, rays, vstat != 0xffffffffu ? 1ull : 0ull);
+        bvs.add(vstat, rays, vstat != 0xffffffffu ? 1ull : 0ull);
     }
+    bvs.flush(views);
 }
 
 // Running mean / variance in sample order with the early-out,
 // Source/Core/Scene.cpp:173-197.
 __global__ __launch_bounds__(kBlock) void k_accumulate(const DView* __restrict__ views, const uint32_t* __restrict__ active,
                                                        Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
+    __shared__ BlockViewStats bvs;
+    bvs.init();
     const uint32_t na = ctl->n_active;
     for (uint32_t base = blockIdx.x * blockDim.x; base < na; base += gridDim.x * blockDim.x) {
         const uint32_t k = base + threadIdx.x;
@@ -482,8 +520,9 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const DView* __restrict__
                 if (stop || off + S >= V.spp) V.done[p] = 1;
             }
         }
-        add_view_stats(views, vstat, rays, ns);
+        bvs.add(vstat, rays, ns);
     }
+    bvs.flush(views);
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact(const DView* __restrict__ views, const uint32_t* __restrict__ active,
@@ -497,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(const DView* __restrict__ vi
             key = active[k];
             keep = views[key >> kViewShift].done[key & kPixMask] == 0;
         }
-        const uint32_t slot = wave_append(&ctl->n_next, keep);
+        const uint32_t slot = block_append<kBlock / 64>(&ctl->n_next, keep);
         if (keep) next[slot] = key;
     }
 }
@@ -529,8 +568,8 @@ int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, const Pool& P, 
     return PICO_OK;
 }
 
-// Optional per-view follow-up of a batched launch: the learned denoiser on
-// the second stream once the view's last chunk is done.
+// Optional per-view follow-up of a batched launch: the learned denoiser once
+// the view's last chunk is done.
 struct ViewHooks {
     pico_cnn* cnn = nullptr;
     float* const* d_out = nullptr;
@@ -688,23 +727,25 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
                 }
                 view_render[v] = ce;
                 if (hooks && hooks->cnn) {
-                    // G-buffer and network of this view on the second stream,
-                    // overlapping the passes still to come
-                    PICO_HIP(c, hipStreamWaitEvent(c->side, ce, 0));
-                    StreamSwap sw(c, c->side);
+                    // G-buffer and network of this view, either between this
+                    // chunk and the next on the main stream, or on the second
+                    // stream overlapping the passes still to come
+                    const bool side = (tune.flags & PICO_TUNE_SIDE_STREAM) != 0;
+                    if (side) PICO_HIP(c, hipStreamWaitEvent(c->side, ce, 0));
+                    StreamSwap sw(c, side ? c->side : c->stream);
                     const uint32_t w = hv[v].cam.w, h = hv[v].cam.h;
                     int rc = pico_hip_gbuffer_d(c, &hooks->views[v].camera, w, h, gn, gp, ga);
                     if (!rc) rc = pico_hip_cnn_denoise_gbuffer_d(hooks->cnn, hv[v].pixels, gn, gp, ga, w, h, hooks->d_out[v]);
                     if (rc) return rc;
                     if (!new_event(&view_done[v])) return fail(c, PICO_ERR_HIP, "hipEventCreate failed");
-                    PICO_HIP(c, hipEventRecord(view_done[v], c->side));
+                    PICO_HIP(c, hipEventRecord(view_done[v], c->stream));
                 } else {
                     view_done[v] = ce;
                 }
             }
         }
     }
-    if (hooks && hooks->cnn) {
+    if (hooks && hooks->cnn && (tune.flags & PICO_TUNE_SIDE_STREAM)) {
         // the caller's stream sees the follow-up's outputs
         hipEvent_t se;
         if (!new_event(&se)) return fail(c, PICO_ERR_HIP, "hipEventCreate failed");

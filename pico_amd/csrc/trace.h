@@ -165,17 +165,18 @@ __device__ __forceinline__ float exp2_scale(uint32_t bits) {
 // continues with the nearest hit child and pushes the others farthest first;
 // an instance boundary pushes a sentinel that restores the world-space ray
 // when popped.  t is the ray parameter of the world ray (directions are unit
-// length, so t is the hit distance).  Box tests widen tmax by 2^-21 relative
+// length, so t is the hit distance).  Hits at or beyond `tmax` are not
+// reported (infinity for the reference's unbounded query).  Box tests widen tmax by 2^-21 relative
 // so that rounding never culls a triangle whose bound the ray only grazes.
 template <int STACK, bool COUNT>
 __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, TraceStack<STACK>& st, TraceCounters* cnt,
-                                             uint32_t* overflow) {
+                                             uint32_t* overflow, float tmax = INFINITY) {
     // No FP contraction here (the library is built without it): the
     // object-space transform and Moller-Trumbore round exactly as the CPU
     // restatement does; only the box tests use explicit fmaf (conservative,
     // they only cull).
     Hit h;
-    h.t = INFINITY; h.u = 0.0f; h.v = 0.0f; h.inst = 0xffffffffu; h.prim = 0xffffffffu;
+    h.t = tmax; h.u = 0.0f; h.v = 0.0f; h.inst = 0xffffffffu; h.prim = 0xffffffffu;
     f3 ro = o, rd = d;
     f3 inv = slab_rcp(d);
     f3 oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);

@@ -195,7 +195,7 @@ def test_multi_chunk_render_matches_oracle(ctx, w, h, queue):
         p = pa.params(w, h, 16, 8, frame_seed=7)
         gp, gc, gv, _ = ctx.render(cam, p)
     finally:
-        ctx.set_tuning(old[0], old[1:])
+        ctx.set_tuning(old[0], old[1:3], old[3])
     cp, cc, cv, _ = oracle.CpuScene(scene).render(cam, p, rng=oracle.RNG_PCG, threads=16)
     assert (gc == cc).mean() >= 0.999, (gc == cc).mean()
     rel = (np.abs(gp - cp) / (np.abs(cp) + 1e-3)).max(axis=1)

@@ -43,7 +43,7 @@ def _read(bufs, specs):
 def restore_tuning(ctx):
     old = ctx.get_tuning()
     yield
-    ctx.set_tuning(old[0], old[1:])
+    ctx.set_tuning(old[0], old[1:3], old[3])
 
 
 def test_output_does_not_depend_on_the_launch_shape(ctx, scene, restore_tuning):
@@ -85,9 +85,10 @@ def test_autotuner_is_deterministic_and_never_worse_than_default(ctx, scene, res
     assert ctx.get_tuning() == t1
 
 
-def test_render_and_denoise_launch_equals_separate_calls(ctx, scene, restore_tuning):
+@pytest.mark.parametrize("flags", [0, pa.TUNE_SIDE_STREAM])
+def test_render_and_denoise_launch_equals_separate_calls(ctx, scene, restore_tuning, flags):
     ctx.upload_scene(scene)
-    ctx.set_tuning()
+    ctx.set_tuning(flags=flags)
     specs = SPECS[:3]
     views, bufs = _views(ctx, scene, specs)
     net = pa.CNN(ctx, 128, 96, 1)
