@@ -227,11 +227,13 @@ __global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ v
 
 // Traversal stacks of the persistent trace kernels: kTraceLds entries per
 // lane in LDS, up to kTraceSpill more in device memory (rarely touched).
-constexpr int kTraceLds = 32;
-constexpr uint32_t kTraceSpill = 32;
+// With 16 LDS entries the register file, not LDS, bounds the resident waves;
+// the kernels ask for 6 waves per SIMD.
+constexpr int kTraceLds = 16;
+constexpr uint32_t kTraceSpill = 48;
 
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
     __shared__ uint32_t stk[kTraceLds * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
     __shared__ uint32_t stk[kTraceLds * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};

@@ -160,8 +160,9 @@ __device__ __forceinline__ float exp2_scale(uint32_t bits) {
 }
 
 // Closest hit of a world-space ray against the TLAS and the instanced BLASes
-// over 4-wide nodes.  Each node decodes its four quantised child boxes
-// (fma(q, 2^e, origin), boxes only grow), tests them with the slab test,
+// over 4-wide nodes.  Each node tests its four quantised child boxes (planes
+// origin + q 2^e, which contain the exact boxes) with the slab test, the
+// plane distances taken as fma(q, 2^e inv, origin inv - o inv) per axis,
 // continues with the nearest hit child and pushes the others farthest first;
 // an instance boundary pushes a sentinel that restores the world-space ray
 // when popped.  t is the ray parameter of the world ray (directions are unit
@@ -197,18 +198,14 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, Trac
             const float sx = exp2_scale(eb), sy = exp2_scale(eb >> 8), sz = exp2_scale(eb >> 16);
             float key[4];
             uint32_t cref[4] = {r.x, r.y, r.z, r.w};
+            const float Ax = sx * inv.x, Ay = sy * inv.y, Az = sz * inv.z;
+            const float Bx = fmaf(a.x, inv.x, oi.x), By = fmaf(a.y, inv.y, oi.y), Bz = fmaf(a.z, inv.z, oi.z);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int sh = 8 * i;
-                const float lx = fmaf((float)((qxy.x >> sh) & 0xffu), sx, a.x);
-                const float hx = fmaf((float)((qxy.y >> sh) & 0xffu), sx, a.x);
-                const float ly = fmaf((float)((qxy.z >> sh) & 0xffu), sy, a.y);
-                const float hy = fmaf((float)((qxy.w >> sh) & 0xffu), sy, a.y);
-                const float lz = fmaf((float)((qz.x >> sh) & 0xffu), sz, a.z);
-                const float hz = fmaf((float)((qz.y >> sh) & 0xffu), sz, a.z);
-                const float x0 = fmaf(lx, inv.x, oi.x), x1 = fmaf(hx, inv.x, oi.x);
-                const float y0 = fmaf(ly, inv.y, oi.y), y1 = fmaf(hy, inv.y, oi.y);
-                const float z0 = fmaf(lz, inv.z, oi.z), z1 = fmaf(hz, inv.z, oi.z);
+                const float x0 = fmaf((float)((qxy.x >> sh) & 0xffu), Ax, Bx), x1 = fmaf((float)((qxy.y >> sh) & 0xffu), Ax, Bx);
+                const float y0 = fmaf((float)((qxy.z >> sh) & 0xffu), Ay, By), y1 = fmaf((float)((qxy.w >> sh) & 0xffu), Ay, By);
+                const float z0 = fmaf((float)((qz.x >> sh) & 0xffu), Az, Bz), z1 = fmaf((float)((qz.y >> sh) & 0xffu), Az, Bz);
                 const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
                 const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kWide;
                 const bool hit = cref[i] != kEmptyRef && tf >= 0.0f && tn <= tf && tn < h.t;
