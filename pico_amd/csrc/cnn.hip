@@ -60,26 +60,33 @@ struct ConvArgs {
     int B, H, W, kpad, npad, taps, flip, relu;
 };
 
-// Implicit-GEMM convolution on MFMA 16x16x16 bf16.  A workgroup of 4 waves
-// owns a run of 4 * MT * 16 pixels of one image row and NT * 16 output
-// channels; each wave computes MT x NT 16x16 tiles (MT * NT = 8 accumulators,
-// so every A fragment feeds NT instructions and every B fragment MT).  Per
-// tap the workgroup stages that tap's weights for its channels in LDS (rows
-// padded by 4 bf16 against bank conflicts); A fragments (4 channels of one
-// pixel at the tap's offset, zero outside the image) come from memory.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Implicit-GEMM convolution on MFMA (M = pixels, N = output channels,
+// K = taps x input channels), FP32 accumulation.  KS is the K depth of one
+// instruction: 32 (v_mfma_f32_16x16x32_bf16, 8 consecutive channels = 16
+// bytes per lane and operand) where the input has a multiple of 32 channels,
+// else 16 (v_mfma_f32_16x16x16_bf16, 8 bytes).  A workgroup of 4 waves owns a
+// run of 4 * MT * 16 pixels of one image row and NT * 16 output channels;
+// each wave computes MT x NT 16x16 tiles (8 accumulators, so every A
+// fragment feeds NT instructions and every B fragment MT).  Per tap the
+// workgroup stages that tap's weights for its channels in LDS (rows padded
+// by 8 bf16 against bank conflicts); A fragments (channels of one pixel at
+// the tap's offset, zero outside the image) come from memory.
 // grid.x: row runs, grid.y: channel blocks of NT * 16.
-template <int MT, int NT>
+template <int MT, int NT, int KS>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
-    extern __shared__ bf16 wl[];  // [NT * 16][kpad + 4]
+    extern __shared__ bf16 wl[];  // [NT * 16][kpad + 8]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int kRun = 4 * MT * 16;
+    constexpr int KL = KS / 4;  // channels per lane and fragment
     const int runs_x = (a.W + kRun - 1) / kRun;
     const int row = blockIdx.x / runs_x;  // b*H + y
     const int xw = (blockIdx.x % runs_x) * kRun + wave * MT * 16;
     const int b = row / a.H, y = row % a.H;
     const int n0 = blockIdx.y * NT * 16;
     const int mi = lane & 15, kq = lane >> 4;
-    const int ldk = a.kpad + 4;
+    const int ldk = a.kpad + 8;
     f4v acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
@@ -107,22 +114,39 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         for (int m = 0; m < MT; ++m) {
             const int xx = xw + m * 16 + mi + dx;
             ok[m] = rok && xx >= 0 && xx < a.W;
-            ap[m] = a.in + ((size_t)(b * a.H + (ok[m] ? yy : 0)) * a.W + (ok[m] ? xx : 0)) * a.in_stride + a.in_off + 4 * kq;
+            ap[m] = a.in + ((size_t)(b * a.H + (ok[m] ? yy : 0)) * a.W + (ok[m] ? xx : 0)) * a.in_stride + a.in_off + KL * kq;
         }
-        const bf16* bp = wl + mi * ldk + 4 * kq;
-        for (int k0 = 0; k0 < a.kpad; k0 += 16) {
-            s4 av[MT], bv[NT];
+        const bf16* bp = wl + mi * ldk + KL * kq;
+        for (int k0 = 0; k0 < a.kpad; k0 += KS) {
+            if constexpr (KS == 32) {
+                uint4 av[MT], bv[NT];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                av[m] = s4{0, 0, 0, 0};
-                if (ok[m]) av[m] = *reinterpret_cast<const s4*>(ap[m] + k0);
+                for (int m = 0; m < MT; ++m) {
+                    av[m] = make_uint4(0, 0, 0, 0);
+                    if (ok[m]) av[m] = *reinterpret_cast<const uint4*>(ap[m] + k0);
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const uint4*>(bp + t * 16 * ldk + k0);
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[m]),
+                                                                            __builtin_bit_cast(bf16x8, bv[t]), acc[m][t], 0, 0, 0);
+            } else {
+                s4 av[MT], bv[NT];
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    av[m] = s4{0, 0, 0, 0};
+                    if (ok[m]) av[m] = *reinterpret_cast<const s4*>(ap[m] + k0);
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const s4*>(bp + t * 16 * ldk + k0);
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[t], acc[m][t], 0, 0, 0);
             }
-#pragma unroll
-            for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const s4*>(bp + t * 16 * ldk + k0);
-#pragma unroll
-            for (int m = 0; m < MT; ++m)
-#pragma unroll
-                for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[t], acc[m][t], 0, 0, 0);
         }
     }
     if (xw >= a.W) return;
@@ -462,16 +486,22 @@ T* dalloc(pico_cnn* n, size_t count) {
 }
 
 // Launches k_conv with the widest channel block (16, 32 or 64) that divides
-// the output channels, and 8 accumulators per wave.
-int launch_conv(pico_cnn* n, const char* name, const ConvArgs& a) {
+// the output channels, 8 accumulators per wave, and the 32-deep instruction
+// where the input channels allow it.
+template <int KS>
+int launch_conv_k(pico_cnn* n, const char* name, const ConvArgs& a) {
     const int nt = (a.npad % 64 == 0) ? 4 : (a.npad % 32 == 0) ? 2 : 1;
     const int run = 4 * (8 / nt) * 16;
     const dim3 grid((unsigned)(a.B * a.H * ((a.W + run - 1) / run)), (unsigned)(a.npad / (nt * 16)));
-    const size_t lds = (size_t)nt * 16 * (a.kpad + 4) * sizeof(bf16);
-    if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv<2, 4>), grid, 256, lds, a);
-    else if (nt == 2) PICO_LAUNCH(n->c, name, (k_conv<4, 2>), grid, 256, lds, a);
-    else PICO_LAUNCH(n->c, name, (k_conv<8, 1>), grid, 256, lds, a);
+    const size_t lds = (size_t)nt * 16 * (a.kpad + 8) * sizeof(bf16);
+    if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv<2, 4, KS>), grid, 256, lds, a);
+    else if (nt == 2) PICO_LAUNCH(n->c, name, (k_conv<4, 2, KS>), grid, 256, lds, a);
+    else PICO_LAUNCH(n->c, name, (k_conv<8, 1, KS>), grid, 256, lds, a);
     return PICO_OK;
+}
+
+int launch_conv(pico_cnn* n, const char* name, const ConvArgs& a) {
+    return a.kpad % 32 == 0 ? launch_conv_k<32>(n, name, a) : launch_conv_k<16>(n, name, a);
 }
 
 int conv(pico_cnn* n, int l, const bf16* in, int in_stride, int in_off, bf16* out, int out_stride, int out_off, int B,
