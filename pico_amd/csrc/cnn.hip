@@ -291,25 +291,36 @@ __global__ void k_inputs(const float* __restrict__ rgb, const float* __restrict_
 
 // 2x2 average pool: src level (H, W) slice -> dst (H/2, W/2) with C channels
 __global__ void k_pool(const bf16* __restrict__ s, int s_stride, int s_off, bf16* __restrict__ d, int C, int B, int H, int W) {
-    const int h = H / 2, w = W / 2;
-    const size_t n = (size_t)B * h * w * C;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = i % C;
-        const size_t p = i / C;
-        const int b = p / (h * w), rem = p - b * h * w, y = rem / w, x = rem % w;
-        auto at = [&](int yy, int xx) { return bf2f(s[((size_t)(b * H + yy) * W + xx) * s_stride + s_off + c]); };
-        d[i] = f2bf(0.25f * (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1) + at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1)));
+    // one thread per (output pixel, 8 channels): 16-byte loads and stores
+    const uint32_t h = H / 2, w = W / 2, c8 = C / 8;
+    const uint32_t n = (uint32_t)B * h * w * c8;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = i % c8, p = i / c8;
+        const uint32_t x = p % w, q = p / w, y = q % h, b = q / h;
+        const bf16* s0 = s + ((size_t)(b * H + 2 * y) * W + 2 * x) * s_stride + s_off + 8 * g;
+        const uint4 v[4] = {*reinterpret_cast<const uint4*>(s0), *reinterpret_cast<const uint4*>(s0 + s_stride),
+                            *reinterpret_cast<const uint4*>(s0 + (size_t)W * s_stride),
+                            *reinterpret_cast<const uint4*>(s0 + (size_t)W * s_stride + s_stride)};
+        bf16 o[8];
+        for (int k = 0; k < 8; ++k) {
+            float acc = 0.0f;
+            for (int j = 0; j < 4; ++j) acc += bf2f(reinterpret_cast<const bf16*>(&v[j])[k]);
+            o[k] = f2bf(0.25f * acc);
+        }
+        *reinterpret_cast<uint4*>(d + (size_t)p * C + 8 * g) = *reinterpret_cast<const uint4*>(o);
     }
 }
 
 // nearest 2x upsample of src (H/2, W/2, C) into the dst slice at (H, W)
 __global__ void k_up(const bf16* __restrict__ s, bf16* __restrict__ d, int d_stride, int d_off, int C, int B, int H, int W) {
-    const size_t n = (size_t)B * H * W * C;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = i % C;
-        const size_t p = i / C;
-        const int b = p / (H * W), rem = p - b * H * W, y = rem / W, x = rem % W;
-        d[p * d_stride + d_off + c] = s[((size_t)(b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+    // one thread per (output pixel, 8 channels): 16-byte loads and stores
+    const uint32_t c8 = C / 8;
+    const uint32_t n = (uint32_t)B * H * W * c8;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = i % c8, p = i / c8;
+        const uint32_t x = p % W, q = p / W, y = q % H, b = q / H;
+        const uint4 v = *reinterpret_cast<const uint4*>(s + ((size_t)(b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + 8 * g);
+        *reinterpret_cast<uint4*>(d + (size_t)p * d_stride + d_off + 8 * g) = v;
     }
 }
 
@@ -560,16 +571,16 @@ int forward(pico_cnn* n, const float* d_in, int B, int H, int W, bool keep_wk) {
     PICO_LAUNCH(n->c, "cnn_pack", k_pack, grid_for(P0), 256, 0, d_in, n->x0, B, H, W, Hp, Wp);
     if ((rc = conv(n, 0, n->x0, 16, 0, n->e0a, 32, 0, B, Hp, Wp, true))) return rc;
     if ((rc = conv(n, 1, n->e0a, 32, 0, n->c0, 96, 64, B, Hp, Wp, true))) return rc;  // e0b -> skip slice of c0
-    PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 4 * 32), 256, 0, n->c0, 96, 64, n->p0, 32, B, Hp, Wp);
+    PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 4 * 4), 256, 0, n->c0, 96, 64, n->p0, 32, B, Hp, Wp);
     if ((rc = conv(n, 2, n->p0, 32, 0, n->e1a, 64, 0, B, H1, W1, true))) return rc;
     if ((rc = conv(n, 3, n->e1a, 64, 0, n->c1, 192, 128, B, H1, W1, true))) return rc;  // e1b -> skip slice of c1
-    PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 16 * 64), 256, 0, n->c1, 192, 128, n->p1, 64, B, H1, W1);
+    PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 16 * 8), 256, 0, n->c1, 192, 128, n->p1, 64, B, H1, W1);
     if ((rc = conv(n, 4, n->p1, 64, 0, n->ba, 128, 0, B, H2, W2, true))) return rc;
     if ((rc = conv(n, 5, n->ba, 128, 0, n->bb, 128, 0, B, H2, W2, true))) return rc;
-    PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 / 4 * 128), 256, 0, n->bb, n->c1, 192, 0, 128, B, H1, W1);
+    PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 / 4 * 16), 256, 0, n->bb, n->c1, 192, 0, 128, B, H1, W1);
     if ((rc = conv(n, 6, n->c1, 192, 0, n->d1a, 64, 0, B, H1, W1, true))) return rc;
     if ((rc = conv(n, 7, n->d1a, 64, 0, n->d1b, 64, 0, B, H1, W1, true))) return rc;
-    PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 * 64), 256, 0, n->d1b, n->c0, 96, 0, 64, B, Hp, Wp);
+    PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 * 8), 256, 0, n->d1b, n->c0, 96, 0, 64, B, Hp, Wp);
     if ((rc = conv(n, 8, n->c0, 96, 0, n->d0a, 32, 0, B, Hp, Wp, true))) return rc;
     if ((rc = conv(n, 9, n->d0a, 32, 0, n->d0b, 32, 0, B, Hp, Wp, true))) return rc;
     if ((rc = conv(n, 10, n->d0b, 32, 0, nullptr, 32, 0, B, Hp, Wp, false, n->head))) return rc;
