@@ -232,6 +232,34 @@ __global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ v
 constexpr int kTraceLds = 16;
 constexpr uint32_t kTraceSpill = 48;
 
+// Lane refill for the persistent trace kernels: a wave keeps a reservoir of
+// queue entries (claimed 64 at a time, one atomic) and hands them to idle
+// lanes between traversal steps, so that lanes whose ray finished early trace
+// the next one instead of waiting for the wave's slowest ray.  Returns the
+// queue position this lane takes, or kDeadPath; wave-uniform state in
+// res_base / res_left / drained.
+__device__ __forceinline__ uint32_t refill_slot(bool idle, uint32_t* head, uint32_t n, uint32_t& res_base,
+                                                uint32_t& res_left, bool& drained) {
+    if (drained) return kDeadPath;
+    const uint64_t im = __ballot(idle);
+    if (!im) return kDeadPath;
+    if (res_left == 0) {
+        const uint32_t base = wave_claim(head);
+        if (base >= n) {
+            drained = true;
+            return kDeadPath;
+        }
+        res_base = base;
+        res_left = min(64u, n - base);
+    }
+    const uint32_t take = min((uint32_t)__popcll(im), res_left);
+    const uint32_t rank = (uint32_t)__popcll(im & ((1ull << lane_id()) - 1ull));
+    const uint32_t e = (idle && rank < take) ? res_base + rank : kDeadPath;
+    res_base += take;
+    res_left -= take;
+    return e;
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
     __shared__ uint32_t stk[kTraceLds * kBlock];
@@ -240,16 +268,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     const uint32_t n = ctl->ext_count[b];
     TraceCounters tc{0, 0, 0};
+    TraceState t;
+    uint32_t item = kDeadPath, res_base = 0, res_left = 0;
+    bool drained = false;
     for (;;) {
-        const uint32_t base = wave_claim(&ctl->ext_head[b]);
-        if (base >= n) break;
-        const uint32_t e = base + lane_id();
-        const uint32_t i = e < n ? q[e] : kDeadPath;
-        if (i != kDeadPath) {
-            const float4 o = P.ray_o[i], d = P.ray_d[i];
-            const Hit h = trace_closest<kTraceLds, COUNT>(s, xyz(o), xyz(d), st, &tc, &ctl->overflow);
-            P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.inst));
-            P.hprim[i] = h.prim;
+        // idle lanes take queue entries until none is idle or the queue is empty
+        // (dead entries, samples a view does not need, leave the lane idle)
+        for (;;) {
+            const uint32_t e = refill_slot(item == kDeadPath, &ctl->ext_head[b], n, res_base, res_left, drained);
+            if (e != kDeadPath) {
+                const uint32_t i = q[e];
+                if (i != kDeadPath) {
+                    item = i;
+                    trace_begin(s, t, st, xyz(P.ray_o[i]), xyz(P.ray_d[i]));
+                }
+            }
+            if (drained || !__ballot(item == kDeadPath)) break;
+        }
+        if (!__ballot(item != kDeadPath)) break;
+        if (item != kDeadPath && trace_step<kTraceLds, COUNT>(s, t, st, &tc, &ctl->overflow)) {
+            P.hit[item] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.inst));
+            P.hprim[item] = t.h.prim;
+            item = kDeadPath;
         }
     }
     if (COUNT) {
@@ -370,23 +410,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
     const uint32_t n = ctl->sh_count[b];
     TraceCounters tc{0, 0, 0};
+    TraceState t;
+    uint32_t item = kDeadPath, res_base = 0, res_left = 0;
+    bool drained = false;
     for (;;) {
-        const uint32_t base = wave_claim(&ctl->sh_head[b]);
-        if (base >= n) break;
-        const uint32_t e = base + lane_id();
-        if (e < n) {
-            const uint32_t i = P.shq[e];
-            const float4 sd = P.sh_d[i];
-            const Hit h = trace_closest<kTraceLds, COUNT>(s, xyz(P.sh_o[i]), xyz(sd), st, &tc, &ctl->overflow, sd.w);
-            if (h.inst != 0xffffffffu) {
-                const DInstance& I = s.inst[h.inst];
+        for (;;) {
+            const uint32_t e = refill_slot(item == kDeadPath, &ctl->sh_head[b], n, res_base, res_left, drained);
+            if (e != kDeadPath) {
+                item = P.shq[e];
+                const float4 sd = P.sh_d[item];
+                trace_begin(s, t, st, xyz(P.sh_o[item]), xyz(sd), sd.w);
+            }
+            if (drained || !__ballot(item == kDeadPath)) break;
+        }
+        if (!__ballot(item != kDeadPath)) break;
+        if (item != kDeadPath && trace_step<kTraceLds, COUNT>(s, t, st, &tc, &ctl->overflow)) {
+            if (t.h.inst != 0xffffffffu) {
+                const DInstance& I = s.inst[t.h.inst];
                 if (I.bsdf == PICO_BSDF_LIGHT) {
                     const f3 em = ld3(s.mats[I.material].emissive);
-                    float4 r = P.rad[i];
-                    const f3 add = (xyz(P.sh_tp[i]) * em) * xyz(P.sh_en[i]);
-                    P.rad[i] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
+                    const float4 r = P.rad[item];
+                    const f3 add = (xyz(P.sh_tp[item]) * em) * xyz(P.sh_en[item]);
+                    P.rad[item] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
                 }
             }
+            item = kDeadPath;
         }
     }
     if (COUNT) {

@@ -159,133 +159,167 @@ __device__ __forceinline__ float exp2_scale(uint32_t bits) {
     return __int_as_float(((int)(int8_t)(bits & 0xffu) + 127) << 23);
 }
 
-// Closest hit of a world-space ray against the TLAS and the instanced BLASes
-// over 4-wide nodes.  Each node tests its four quantised child boxes (planes
-// origin + q 2^e, which contain the exact boxes) with the slab test, the
-// plane distances taken as fma(q, 2^e inv, origin inv - o inv) per axis,
-// continues with the nearest hit child and pushes the others farthest first;
-// an instance boundary pushes a sentinel that restores the world-space ray
-// when popped.  t is the ray parameter of the world ray (directions are unit
-// length, so t is the hit distance).  Hits at or beyond `tmax` are not
-// reported (infinity for the reference's unbounded query).  Box tests widen tmax by 2^-21 relative
-// so that rounding never culls a triangle whose bound the ray only grazes.
+// Two-level closest-hit traversal over 4-wide nodes.  Each node tests its
+// four quantised child boxes (planes origin + q 2^e, which contain the exact
+// boxes) with the slab test, the plane distances taken as
+// fma(q, 2^e inv, origin inv - o inv) per axis, continues with the nearest hit
+// child and pushes the others farthest first; an instance boundary pushes a
+// sentinel that restores the world-space ray when popped.  t is the ray
+// parameter of the world ray (directions are unit length, so t is the hit
+// distance).  Hits at or beyond `tmax` are not reported (infinity for the
+// reference's unbounded query).  Box tests widen the far distance by 2^-21
+// relative so that rounding never culls a triangle whose bound the ray only
+// grazes.
+//
+// Traversal state of one ray, advanced one node, leaf or instance at a time
+// by trace_step, so that a persistent kernel can hand a lane a new ray as
+// soon as its current one is finished.
+struct TraceState {
+    f3 o, d;      // world ray
+    f3 ro, rd;    // ray in the current space (world or the instance's object space)
+    f3 inv, oi;   // slab-test reciprocals and -ro * inv
+    Hit h;
+    uint32_t ref, cur;
+    bool in_blas;
+};
+
+template <int STACK>
+__device__ __forceinline__ void trace_begin(const DevScene& s, TraceState& t, TraceStack<STACK>& st, f3 o, f3 d,
+                                            float tmax = INFINITY) {
+    t.o = o; t.d = d; t.ro = o; t.rd = d;
+    t.inv = slab_rcp(d);
+    t.oi = mk(-o.x * t.inv.x, -o.y * t.inv.y, -o.z * t.inv.z);
+    t.h.t = tmax; t.h.u = 0.0f; t.h.v = 0.0f; t.h.inst = 0xffffffffu; t.h.prim = 0xffffffffu;
+    t.ref = s.tlas_root;
+    t.cur = 0xffffffffu;
+    t.in_blas = false;
+    st.sp = 0;
+}
+
+// One traversal step: an inner node (continue with its nearest hit child, the
+// others pushed farthest first), a BLAS leaf (its triangles) or a TLAS leaf
+// (move into the instance's object space); when that leads nowhere further the
+// next stack entry is popped.  Returns true once the stack is empty: t.h then
+// holds the closest hit.
 template <int STACK, bool COUNT>
-__device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, TraceStack<STACK>& st, TraceCounters* cnt,
-                                             uint32_t* overflow, float tmax = INFINITY) {
+__device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& t, TraceStack<STACK>& st, TraceCounters* cnt,
+                                           uint32_t* overflow) {
     // No FP contraction here (the library is built without it): the
     // object-space transform and Moller-Trumbore round exactly as the CPU
     // restatement does; only the box tests use explicit fmaf (conservative,
     // they only cull).
-    Hit h;
-    h.t = tmax; h.u = 0.0f; h.v = 0.0f; h.inst = 0xffffffffu; h.prim = 0xffffffffu;
-    f3 ro = o, rd = d;
-    f3 inv = slab_rcp(d);
-    f3 oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
-    uint32_t ref = s.tlas_root;
-    uint32_t cur = 0xffffffffu;
-    bool in_blas = false;
     const float kWide = 1.0000005f;
-    st.sp = 0;
-    for (;;) {
-        if (!(ref & kLeafBit)) {
-            const DNode4* n = s.nodes + ref;
-            const float4 a = ldg4(&n->origin[0]);
-            const uint4 qxy = *reinterpret_cast<const uint4*>(&n->qx[0]);
-            const uint2 qz = *reinterpret_cast<const uint2*>(&n->qz[0]);
-            const uint4 r = *reinterpret_cast<const uint4*>(&n->ref[0]);
-            if (COUNT) cnt->nodes++;
-            const uint32_t eb = __float_as_uint(a.w);
-            const float sx = exp2_scale(eb), sy = exp2_scale(eb >> 8), sz = exp2_scale(eb >> 16);
-            float key[4];
-            uint32_t cref[4] = {r.x, r.y, r.z, r.w};
-            const float Ax = sx * inv.x, Ay = sy * inv.y, Az = sz * inv.z;
-            const float Bx = fmaf(a.x, inv.x, oi.x), By = fmaf(a.y, inv.y, oi.y), Bz = fmaf(a.z, inv.z, oi.z);
+    if (!(t.ref & kLeafBit)) {
+        const DNode4* n = s.nodes + t.ref;
+        const float4 a = ldg4(&n->origin[0]);
+        const uint4 qxy = *reinterpret_cast<const uint4*>(&n->qx[0]);
+        const uint2 qz = *reinterpret_cast<const uint2*>(&n->qz[0]);
+        const uint4 r = *reinterpret_cast<const uint4*>(&n->ref[0]);
+        if (COUNT) cnt->nodes++;
+        const uint32_t eb = __float_as_uint(a.w);
+        const float sx = exp2_scale(eb), sy = exp2_scale(eb >> 8), sz = exp2_scale(eb >> 16);
+        float key[4];
+        uint32_t cref[4] = {r.x, r.y, r.z, r.w};
+        const float Ax = sx * t.inv.x, Ay = sy * t.inv.y, Az = sz * t.inv.z;
+        const float Bx = fmaf(a.x, t.inv.x, t.oi.x), By = fmaf(a.y, t.inv.y, t.oi.y), Bz = fmaf(a.z, t.inv.z, t.oi.z);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int sh = 8 * i;
-                const float x0 = fmaf((float)((qxy.x >> sh) & 0xffu), Ax, Bx), x1 = fmaf((float)((qxy.y >> sh) & 0xffu), Ax, Bx);
-                const float y0 = fmaf((float)((qxy.z >> sh) & 0xffu), Ay, By), y1 = fmaf((float)((qxy.w >> sh) & 0xffu), Ay, By);
-                const float z0 = fmaf((float)((qz.x >> sh) & 0xffu), Az, Bz), z1 = fmaf((float)((qz.y >> sh) & 0xffu), Az, Bz);
-                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
-                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kWide;
-                const bool hit = cref[i] != kEmptyRef && tf >= 0.0f && tn <= tf && tn < h.t;
-                key[i] = hit ? tn : INFINITY;
-            }
-            // sort the four (distance, child) pairs, nearest first
+        for (int i = 0; i < 4; ++i) {
+            const int sh = 8 * i;
+            const float x0 = fmaf((float)((qxy.x >> sh) & 0xffu), Ax, Bx), x1 = fmaf((float)((qxy.y >> sh) & 0xffu), Ax, Bx);
+            const float y0 = fmaf((float)((qxy.z >> sh) & 0xffu), Ay, By), y1 = fmaf((float)((qxy.w >> sh) & 0xffu), Ay, By);
+            const float z0 = fmaf((float)((qz.x >> sh) & 0xffu), Az, Bz), z1 = fmaf((float)((qz.y >> sh) & 0xffu), Az, Bz);
+            const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+            const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kWide;
+            const bool hit = cref[i] != kEmptyRef && tf >= 0.0f && tn <= tf && tn < t.h.t;
+            key[i] = hit ? tn : INFINITY;
+        }
+        // sort the four (distance, child) pairs, nearest first
 #define PICO_CSWAP(i, j)                                                  \
-            if (key[j] < key[i]) {                                        \
-                const float tk = key[i]; key[i] = key[j]; key[j] = tk;    \
-                const uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
-            }
-            PICO_CSWAP(0, 1) PICO_CSWAP(2, 3) PICO_CSWAP(0, 2) PICO_CSWAP(1, 3) PICO_CSWAP(1, 2)
+        if (key[j] < key[i]) {                                            \
+            const float tk = key[i]; key[i] = key[j]; key[j] = tk;        \
+            const uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
+        }
+        PICO_CSWAP(0, 1) PICO_CSWAP(2, 3) PICO_CSWAP(0, 2) PICO_CSWAP(1, 3) PICO_CSWAP(1, 2)
 #undef PICO_CSWAP
-            const int nh = (key[0] != INFINITY) + (key[1] != INFINITY) + (key[2] != INFINITY) + (key[3] != INFINITY);
-            if (nh > 0) {
-                if (nh > 3) st.push(cref[3], overflow);
-                if (nh > 2) st.push(cref[2], overflow);
-                if (nh > 1) st.push(cref[1], overflow);
-                ref = cref[0];
-                continue;
-            }
-        } else if (in_blas) {
-            const uint32_t first = ref & kPrimMask, count = (ref >> 27) & 15u;
-            for (uint32_t k = 0; k < count; ++k) {
-                const DTri* tr = s.tris + first + k;
-                const float4 p0 = ldg4(&tr->v0[0]);
-                const float4 p1 = ldg4(&tr->e1[0]);
-                const float4 p2 = ldg4(&tr->e2[0]);
-                if (COUNT) cnt->tris++;
-                // Moller-Trumbore with the reference's epsilons,
-                // Source/Core/LowerLevelMeshBVH.cpp:165-207
-                const f3 e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z);
-                const f3 hh = cross(rd, e2);
-                const float det = dot(e1, hh);
-                if (det > -0.0000001f && det < 0.0000001f) continue;
-                const float f = 1.0f / det;
-                const f3 sv = ro - mk(p0.x, p0.y, p0.z);
-                const float u = f * dot(sv, hh);
-                if (u < 0.0f || u > 1.0f) continue;
-                const f3 q = cross(sv, e1);
-                const float v = f * dot(rd, q);
-                if (v < 0.0f || u + v > 1.0f) continue;
-                const float t = f * dot(e2, q);
-                if (t > 0.0000001f && t < h.t) {
-                    h.t = t; h.u = u; h.v = v; h.inst = cur; h.prim = __float_as_uint(p0.w);
-                }
-            }
-        } else {
-            // TLAS leaf: one instance; move the ray into object space
-            cur = ref & kPrimMask;
-            const DInstance* I = s.inst + cur;
-            const float4 m0 = ldg4(&I->inv[0][0]);
-            const float4 m1 = ldg4(&I->inv[1][0]);
-            const float4 m2 = ldg4(&I->inv[2][0]);
-            const uint32_t root = I->blas_root;
-            if (COUNT) cnt->insts++;
-            st.push(kSentinel, overflow);
-            // transform_ray (Source/Core/AABB.cpp:43-52): M^-1 (o, 1) as
-            // (c0*x + c1*y) + (c2*z + c3), mat3(M^-1) d as (c0*x + c1*y) + c2*z
-            ro = mk((m0.x * o.x + m0.y * o.y) + (m0.z * o.z + m0.w), (m1.x * o.x + m1.y * o.y) + (m1.z * o.z + m1.w),
-                    (m2.x * o.x + m2.y * o.y) + (m2.z * o.z + m2.w));
-            rd = mk((m0.x * d.x + m0.y * d.y) + m0.z * d.z, (m1.x * d.x + m1.y * d.y) + m1.z * d.z,
-                    (m2.x * d.x + m2.y * d.y) + m2.z * d.z);
-            inv = slab_rcp(rd);
-            oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
-            in_blas = true;
-            ref = root;
-            continue;
+        const int nh = (key[0] != INFINITY) + (key[1] != INFINITY) + (key[2] != INFINITY) + (key[3] != INFINITY);
+        if (nh > 0) {
+            if (nh > 3) st.push(cref[3], overflow);
+            if (nh > 2) st.push(cref[2], overflow);
+            if (nh > 1) st.push(cref[1], overflow);
+            t.ref = cref[0];
+            return false;
         }
-        // pop
-        for (;;) {
-            if (st.sp == 0) return h;
-            ref = st.pop();
-            if (ref != kSentinel) break;
-            ro = o; rd = d;
-            inv = slab_rcp(d);
-            oi = mk(-ro.x * inv.x, -ro.y * inv.y, -ro.z * inv.z);
-            in_blas = false;
+    } else if (t.in_blas) {
+        const uint32_t first = t.ref & kPrimMask, count = (t.ref >> 27) & 15u;
+        for (uint32_t k = 0; k < count; ++k) {
+            const DTri* tr = s.tris + first + k;
+            const float4 p0 = ldg4(&tr->v0[0]);
+            const float4 p1 = ldg4(&tr->e1[0]);
+            const float4 p2 = ldg4(&tr->e2[0]);
+            if (COUNT) cnt->tris++;
+            // Moller-Trumbore with the reference's epsilons,
+            // Source/Core/LowerLevelMeshBVH.cpp:165-207
+            const f3 e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z);
+            const f3 hh = cross(t.rd, e2);
+            const float det = dot(e1, hh);
+            if (det > -0.0000001f && det < 0.0000001f) continue;
+            const float f = 1.0f / det;
+            const f3 sv = t.ro - mk(p0.x, p0.y, p0.z);
+            const float u = f * dot(sv, hh);
+            if (u < 0.0f || u > 1.0f) continue;
+            const f3 q = cross(sv, e1);
+            const float v = f * dot(t.rd, q);
+            if (v < 0.0f || u + v > 1.0f) continue;
+            const float tt = f * dot(e2, q);
+            if (tt > 0.0000001f && tt < t.h.t) {
+                t.h.t = tt; t.h.u = u; t.h.v = v; t.h.inst = t.cur; t.h.prim = __float_as_uint(p0.w);
+            }
         }
+    } else {
+        // TLAS leaf: one instance; move the ray into object space
+        t.cur = t.ref & kPrimMask;
+        const DInstance* I = s.inst + t.cur;
+        const float4 m0 = ldg4(&I->inv[0][0]);
+        const float4 m1 = ldg4(&I->inv[1][0]);
+        const float4 m2 = ldg4(&I->inv[2][0]);
+        const uint32_t root = I->blas_root;
+        if (COUNT) cnt->insts++;
+        st.push(kSentinel, overflow);
+        // transform_ray (Source/Core/AABB.cpp:43-52): M^-1 (o, 1) as
+        // (c0*x + c1*y) + (c2*z + c3), mat3(M^-1) d as (c0*x + c1*y) + c2*z
+        const f3 o = t.o, d = t.d;
+        t.ro = mk((m0.x * o.x + m0.y * o.y) + (m0.z * o.z + m0.w), (m1.x * o.x + m1.y * o.y) + (m1.z * o.z + m1.w),
+                  (m2.x * o.x + m2.y * o.y) + (m2.z * o.z + m2.w));
+        t.rd = mk((m0.x * d.x + m0.y * d.y) + m0.z * d.z, (m1.x * d.x + m1.y * d.y) + m1.z * d.z,
+                  (m2.x * d.x + m2.y * d.y) + m2.z * d.z);
+        t.inv = slab_rcp(t.rd);
+        t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
+        t.in_blas = true;
+        t.ref = root;
+        return false;
     }
+    // pop; an instance sentinel restores the world-space ray
+    for (;;) {
+        if (st.sp == 0) return true;
+        t.ref = st.pop();
+        if (t.ref != kSentinel) return false;
+        t.ro = t.o; t.rd = t.d;
+        t.inv = slab_rcp(t.d);
+        t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
+        t.in_blas = false;
+    }
+}
+
+// Closest hit of a world-space ray against the TLAS and the instanced
+// BLASes: trace_begin, then trace_step until the stack is empty.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, TraceStack<STACK>& st, TraceCounters* cnt,
+                                             uint32_t* overflow, float tmax = INFINITY) {
+    TraceState t;
+    trace_begin(s, t, st, o, d, tmax);
+    while (!trace_step<STACK, COUNT>(s, t, st, cnt, overflow)) {
+    }
+    return t.h;
 }
 
 // ------------------------------------------------------------- shading ----
