@@ -263,9 +263,12 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3):
             runs.append((time.perf_counter() - t0, st))
         return sorted(runs, key=lambda r: r[0])[len(runs) // 2]
 
-    # the denoiser between chunks on the launch's stream, or on a second stream
+    # the denoiser between chunks on the launch's stream, or on a second
+    # stream; and the views shortest first
     wall, st = timed(0)
     wall2, st2 = timed(pa.TUNE_SIDE_STREAM)
+    wall3, st3 = timed(pa.TUNE_SHORTEST_FIRST)
+    lat3 = np.array([s.done_ms for s in st3])
     rays = sum(s.accumulated_rays for s in st)
     lat = np.array([s.done_ms for s in st])
     rlat = np.array([s.render_ms for s in st])
@@ -296,6 +299,10 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3):
             "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
             "render_latency_ms_p50": round(float(np.percentile(rlat, 50)), 2),
             "render_latency_ms_p99": round(float(np.percentile(rlat, 99)), 2),
+            "shortest_first": {"value": round(sum(s.accumulated_rays for s in st3) / wall3 / 1e6, 2),
+                               "ms_per_batch": round(wall3 * 1e3, 2),
+                               "latency_ms_p50": round(float(np.percentile(lat3, 50)), 2),
+                               "latency_ms_p99": round(float(np.percentile(lat3, 99)), 2)},
             "second_stream": {"ms_per_batch": round(wall2 * 1e3, 2),
                               "latency_ms_p50": round(float(np.percentile(lat2, 50)), 2),
                               "latency_ms_p99": round(float(np.percentile(lat2, 99)), 2)},

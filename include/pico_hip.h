@@ -324,6 +324,10 @@ typedef struct pico_tuning {
 /* pico_hip_render_views_denoise_d runs each view's denoiser on a second
  * stream (overlapping the remaining passes) instead of between two chunks. */
 #define PICO_TUNE_SIDE_STREAM 1u
+/* Batched views run shortest first: ordered by pixels x samples, as
+ * successive launches whose cumulative work doubles (1/16, 1/8, ... of the
+ * batch), so cheap views finish early; per-view times count from the first. */
+#define PICO_TUNE_SHORTEST_FIRST 2u
 int  pico_hip_set_tuning(pico_hip_ctx* ctx, const pico_tuning* tuning);
 int  pico_hip_get_tuning(pico_hip_ctx* ctx, pico_tuning* tuning);
 

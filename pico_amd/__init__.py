@@ -34,6 +34,7 @@ BSDF_DIFFUSE, BSDF_SPECULAR, BSDF_DIELECTRIC, BSDF_LIGHT, BSDF_DELTA = range(5)
 MICROFACET_BECKMANN_ALL, MICROFACET_GGX = 0, 1
 RENDER_RESUME, RENDER_COUNT = 1, 2
 TUNE_SIDE_STREAM = 1
+TUNE_SHORTEST_FIRST = 2
 # fixed-width a-trous: normal, relative depth, albedo, colour (0 = term off)
 ATROUS_FIXED_SIGMAS = (0.5, 0.05, 0.1, 0.0)
 

@@ -628,7 +628,8 @@ struct ViewHooks {
 
 // Runs the chunked wavefront over `nv` views already described on the host.
 int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, const volatile int32_t* cancel,
-              bool count, pico_stats* stats, uint32_t n_stats, const ViewHooks* hooks = nullptr) {
+              bool count, pico_stats* stats, uint32_t n_stats, const ViewHooks* hooks = nullptr,
+              hipEvent_t epoch = nullptr) {
     if (!c->has_scene) return fail(c, PICO_ERR_STATE, "no scene uploaded");
     const uint32_t nv = (uint32_t)hv.size();
     if (nv == 0 || nv > kMaxViews) return fail(c, PICO_ERR_LIMIT, "between 1 and 64 views per launch");
@@ -840,14 +841,69 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
             }
             if (per_view_times && view_render[v]) {
                 float a = 0.0f, b = 0.0f;
-                hipEventElapsedTime(&a, e0, view_render[v]);
+                hipEventElapsedTime(&a, epoch ? epoch : e0, view_render[v]);
                 st.render_ms = a;
-                if (view_done[v] && hipEventElapsedTime(&b, e0, view_done[v]) == hipSuccess) st.done_ms = b;
+                if (view_done[v] && hipEventElapsedTime(&b, epoch ? epoch : e0, view_done[v]) == hipSuccess) st.done_ms = b;
                 else st.done_ms = a;
             }
         }
     }
     return cancelled ? fail(c, PICO_ERR_CANCELLED, "cancelled") : PICO_OK;
+}
+
+// Batched views with PICO_TUNE_SHORTEST_FIRST: the views, ordered by work
+// (pixels x samples), run as successive launches whose cumulative work
+// doubles (1/16, 1/8, 1/4, 1/2, all of it), so that the cheap views are done
+// early; per-view times count from the first launch.  Without the flag: one
+// launch.
+int run_batched(pico_hip_ctx* c, const pico_view* views, uint32_t n, std::vector<DView>& hv, const pico_tuning& tune,
+                pico_stats* stats, const ViewHooks* hooks) {
+    if (!(tune.flags & PICO_TUNE_SHORTEST_FIRST) || n < 2)
+        return run_views(c, hv, tune, nullptr, false, stats, stats ? n : 0, hooks);
+    std::vector<uint32_t> order(n);
+    std::vector<double> work(n);
+    double total = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        order[i] = i;
+        work[i] = (double)hv[i].cam.w * hv[i].cam.h * hv[i].spp;
+        total += work[i];
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return work[a] < work[b]; });
+    hipEvent_t epoch;
+    PICO_HIP(c, hipEventCreate(&epoch));
+    struct Guard { hipEvent_t e; ~Guard() { hipEventDestroy(e); } } g{epoch};
+    PICO_HIP(c, hipEventRecord(epoch, c->stream));
+    double done = 0.0, bound = total / 16.0;
+    size_t k = 0;
+    while (k < n) {
+        std::vector<uint32_t> grp;
+        while (k < n && (grp.empty() || done + work[order[k]] <= bound * (1.0 + 1e-9))) {
+            done += work[order[k]];
+            grp.push_back(order[k++]);
+        }
+        bound = std::min(total, bound * 2.0);
+        std::vector<DView> gv;
+        std::vector<pico_view> gpv;
+        std::vector<float*> gout;
+        for (uint32_t i : grp) {
+            gv.push_back(hv[i]);
+            gpv.push_back(views[i]);
+            if (hooks && hooks->d_out) gout.push_back(hooks->d_out[i]);
+        }
+        ViewHooks gh;
+        if (hooks) {
+            gh.cnn = hooks->cnn;
+            gh.d_out = gout.data();
+            gh.views = gpv.data();
+        }
+        std::vector<pico_stats> gs(grp.size());
+        int rc = run_views(c, gv, tune, nullptr, false, stats ? gs.data() : nullptr, stats ? (uint32_t)grp.size() : 0,
+                           hooks ? &gh : nullptr, epoch);
+        if (rc) return rc;
+        if (stats)
+            for (size_t j = 0; j < grp.size(); ++j) stats[grp[j]] = gs[j];
+    }
+    return PICO_OK;
 }
 
 DView base_view(const pico_camera* cam, const pico_render_params* p) {
@@ -974,7 +1030,7 @@ int pico_hip_render_views_d(pico_hip_ctx* c, const pico_view* views, uint32_t n,
     if (rc) return rc;
     pico_tuning t = c->tuning;
     if (queue_paths) t.queue_paths = queue_paths;
-    return run_views(c, hv, t, nullptr, false, stats, stats ? n : 0);
+    return run_batched(c, views, n, hv, t, stats, nullptr);
 }
 
 int pico_hip_render_views_denoise_d(pico_hip_ctx* c, const pico_view* views, uint32_t n, pico_cnn* cnn,
@@ -993,7 +1049,7 @@ int pico_hip_render_views_denoise_d(pico_hip_ctx* c, const pico_view* views, uin
     hk.cnn = cnn;
     hk.d_out = d_denoised;
     hk.views = views;
-    return run_views(c, hv, c->tuning, nullptr, false, stats, stats ? n : 0, &hk);
+    return run_batched(c, views, n, hv, c->tuning, stats, &hk);
 }
 
 int pico_hip_set_tuning(pico_hip_ctx* c, const pico_tuning* t) {

@@ -50,8 +50,9 @@ def test_output_does_not_depend_on_the_launch_shape(ctx, scene, restore_tuning):
     ctx.upload_scene(scene)
     views, bufs = _views(ctx, scene, SPECS)
     results = []
-    for q, tile in [(16 << 20, (0, 0)), (4096, (8, 8)), (20000, (32, 4)), (1 << 20, (16, 16))]:
-        ctx.set_tuning(q, tile)
+    for q, tile, flags in [(16 << 20, (0, 0), 0), (4096, (8, 8), 0), (20000, (32, 4), 0), (1 << 20, (16, 16), 0),
+                           (1 << 20, (8, 8), pa.TUNE_SHORTEST_FIRST)]:
+        ctx.set_tuning(q, tile, flags)
         ctx.render_views_d(views)
         results.append(_read(bufs, SPECS))
     for r in results[1:]:
@@ -85,7 +86,7 @@ def test_autotuner_is_deterministic_and_never_worse_than_default(ctx, scene, res
     assert ctx.get_tuning() == t1
 
 
-@pytest.mark.parametrize("flags", [0, pa.TUNE_SIDE_STREAM])
+@pytest.mark.parametrize("flags", [0, pa.TUNE_SIDE_STREAM, pa.TUNE_SHORTEST_FIRST])
 def test_render_and_denoise_launch_equals_separate_calls(ctx, scene, restore_tuning, flags):
     ctx.upload_scene(scene)
     ctx.set_tuning(flags=flags)
@@ -105,3 +106,13 @@ def test_render_and_denoise_launch_equals_separate_calls(ctx, scene, restore_tun
         net.denoise_gbuffer_d(dr, dn, dp, da, w, h, do)
         np.testing.assert_array_equal(got[k], do.download(np.float32, 3 * n))
         assert stats[k].done_ms >= stats[k].render_ms > 0
+
+
+def test_shortest_first_finishes_cheap_views_first(ctx, scene, restore_tuning):
+    ctx.upload_scene(scene)
+    ctx.set_tuning(flags=pa.TUNE_SHORTEST_FIRST)
+    views, bufs = _views(ctx, scene, SPECS)
+    stats = ctx.render_views_d(views)
+    work = [w * h * spp for (w, h, spp, _) in SPECS]
+    cheap, dear = int(np.argmin(work)), int(np.argmax(work))
+    assert stats[cheap].render_ms < stats[dear].render_ms
