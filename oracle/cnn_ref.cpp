@@ -13,13 +13,16 @@
  *     image, transformed to [log1p(max(rad,0)), albedo, normal, log1p(depth)];
  *   conv: out[y][x][co] = b[co] + sum_{dy,dx,ci} in[y+dy][x+dx][ci] W[t][ci][co],
  *     t = (dy+1)*3 + (dx+1), zero padding; ReLU after every conv but the head;
- *   pool: 2x2 average; up: nearest; concat: [upsampled, skip];
+ *   pool: 2x2 average; up: nearest; concat: [upsampled, skip]; sizes that
+ *     are not a multiple of 4 are padded up to one by repeating the last row
+ *     and column (forward only; the backward takes multiples of 4);
  *   kernel: logits -> softmax over k = (dy+2)*5 + (dx+2), applied to the
  *     radiance at the clamped neighbour;
  *   loss: mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2);
  *   parameters: per layer W[t][ci][co] then b[co], layers in the order
  *     e0a e0b e1a e1b ba bb d1a d1b d0a d0b head.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -134,11 +137,14 @@ struct Acts {
 };
 
 void forward(const Net& n, const float* in10, int h, int w, Acts& A, std::vector<double>& out_rgb) {
-    A.x = T(h, w, 10);
-    for (int y = 0; y < h; ++y)
-        for (int x = 0; x < w; ++x)
+    // the network runs on the image padded to a multiple of 4 by repeating its
+    // last row and column; the kernel is applied on the image itself
+    const int hp = (h + 3) / 4 * 4, wp = (w + 3) / 4 * 4;
+    A.x = T(hp, wp, 10);
+    for (int y = 0; y < hp; ++y)
+        for (int x = 0; x < wp; ++x)
             for (int k = 0; k < 10; ++k) {
-                double v = in10[(size_t)k * h * w + (size_t)y * w + x];
+                double v = in10[(size_t)k * h * w + (size_t)std::min(y, h - 1) * w + std::min(x, w - 1)];
                 if (k < 3) v = std::log1p(v > 0 ? v : 0.0);
                 if (k == 9) v = std::log1p(v);
                 A.x.at(y, x, k) = v;
@@ -199,6 +205,7 @@ void cnnref_forward(const double* params, const float* in10, int h, int w, doubl
  * h*w*3 per image. */
 double cnnref_loss_grad(const double* params, const float* in10, const float* targets, int batch, int h, int w,
                         double* grads) {
+    if (h % 4 || w % 4) return NAN;  // the backward is written for sizes the pooling divides
     Net n(params);
     const double count = (double)batch * h * w * 3;
     double loss = 0.0;
