@@ -355,6 +355,13 @@ int  pico_hip_render_views_denoise_d(pico_hip_ctx* ctx, const pico_view* views, 
 int  pico_hip_autotune_views(pico_hip_ctx* ctx, const pico_view* views, uint32_t n, uint32_t calib_spp,
                              pico_tuning* chosen, float* chosen_ms, float* default_ms);
 
+/* ---- file output --------------------------------------------------------- */
+/* The file half of Scene::render_scene_to_file (Source/Core/Scene.cpp:236-246):
+ * writes 3 x f32 pixels (host memory) as a Radiance .hdr file (RGBE); with
+ * `reverse` the pixel order is reversed end to end first, as the reference
+ * does before stbi_write_hdr.  Host-only; needs no context. */
+int  pico_write_hdr(const char* path, uint32_t width, uint32_t height, const float* rgb, int reverse);
+
 /* ---- procedural scenes (inputs of the configurations) ------------------ */
 typedef struct pico_scene pico_scene;
 /* Cornell box: 5 walls x 2 tris, two boxes of 12 tris, one quad light,

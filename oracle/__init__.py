@@ -113,6 +113,8 @@ def ref_lib():
             "ref_hammersley": (None, [u32, vp, vp]),
             "ref_alias_sample": (None, [u32, vp, u32, u32, vp]),
             "ref_bsdf_sample": (None, [u32, P(Material), u32, vp, vp, u32, vp, vp, vp]),
+            "ref_write_hdr": (i32, [C.c_char_p, i32, i32, vp]),
+            "ref_load_hdr": (i32, [C.c_char_p, i32, i32, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(_rf, name)
@@ -258,6 +260,17 @@ def ref_atrous(rgb, normal, position, albedo, w, h, tile=(16, 16)):
     args = [_f32(x, (-1, 3)) for x in (rgb, normal, position, albedo)]
     out = np.empty((w * h, 3), np.float32)
     ref_lib().ref_atrous(*[_p(a) for a in args], w, h, tile[0], tile[1], _p(out))
+    return out
+
+
+def ref_write_hdr(path, rgb, w, h):
+    a = _f32(rgb, (-1, 3))
+    assert ref_lib().ref_write_hdr(os.fsencode(path), w, h, _p(a)) != 0
+
+
+def ref_load_hdr(path, w, h):
+    out = np.empty((w * h, 3), np.float32)
+    assert ref_lib().ref_load_hdr(os.fsencode(path), w, h, _p(out)) == 0
     return out
 
 

@@ -441,4 +441,22 @@ void ref_bsdf_sample(uint32_t bsdf, const pico_material* pm, uint32_t n, const f
     }
 }
 
+
+// Radiance .hdr through the reference's own writer and reader (the file half
+// of Scene::render_scene_to_file, Source/Core/Scene.cpp:236-246).
+int stbi_write_hdr(char const* filename, int w, int h, int comp, const float* data);
+float* stbi_loadf(char const* filename, int* x, int* y, int* comp, int req_comp);
+void stbi_image_free(void* retval_from_stbi_load);
+
+int ref_write_hdr(const char* path, int w, int h, const float* rgb) { return stbi_write_hdr(path, w, h, 3, rgb); }
+
+int ref_load_hdr(const char* path, int w, int h, float* out) {
+    int x = 0, y = 0, comp = 0;
+    float* d = stbi_loadf(path, &x, &y, &comp, 3);
+    if (!d) return -1;
+    const int ok = (x == w && y == h) ? 0 : -2;
+    if (!ok) std::memcpy(out, d, sizeof(float) * 3 * (size_t)w * h);
+    stbi_image_free(d);
+    return ok;
+}
 }  // extern "C"

@@ -154,6 +154,7 @@ def _declare(lib):
         "pico_hip_cnn_last_grad_norm": (i32, [vp, P(f64)]),
         "pico_hip_cnn_loss_grad": (i32, [vp, vp, vp, u32, u32, u32, P(f64), vp]),
         "pico_hip_cnn_train_step_d": (i32, [vp, vp, vp, u32, u32, u32, f32, P(f64)]),
+        "pico_write_hdr": (i32, [C.c_char_p, u32, u32, vp, i32]),
         "pico_scene_cornell": (i32, [u32, u32, P(vp)]),
         "pico_scene_instanced": (i32, [u32, u32, u32, u32, u32, f32, u32, P(vp)]),
         "pico_scene_training": (i32, [u32, u32, P(vp)]),
@@ -565,6 +566,15 @@ CNN_LAYERS = [(10, 32, 9, 0), (32, 32, 9, 0), (32, 64, 9, 1), (64, 64, 9, 1), (6
 def cnn_flops_per_pixel():
     """Multiply-add FLOPs of one forward pass per full-resolution pixel."""
     return sum(2 * taps * cin * cout / 4 ** lvl for cin, cout, taps, lvl in CNN_LAYERS)
+
+
+def write_hdr(path, rgb, width, height, reverse=True):
+    """Radiance .hdr file of a packed RGB image; `reverse` reverses the pixel
+    order end to end first, as Scene::render_scene_to_file does."""
+    a = _f32(rgb, (-1, 3))
+    rc = lib().pico_write_hdr(os.fsencode(path), width, height, _ptr(a), 1 if reverse else 0)
+    if rc:
+        raise PicoError(rc, f"pico_write_hdr({path})")
 
 
 def weights_path():

@@ -4,6 +4,7 @@
 // a visible GPU it reports the device error and exits 0 as well ("nodevice").
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "pico_front.hpp"
@@ -47,6 +48,16 @@ int main() {
     std::vector<uint32_t> cnt2(W * H, 0);
     pico_front::RenderParams<vec3> p2{5, 8, 0.0f, H, W, false, false, pix.data(), cnt2.data(), var.data()};
     if (r.render_scene_to_memory(cam, p2, &quit) != PICO_ERR_CANCELLED) return 7;
+    // render_scene_to_file: a Radiance file whose first line is the magic
+    std::vector<uint32_t> cnt3(W * H, 0);
+    pico_front::RenderParams<vec3> p3{5, 4, 0.0f, H, W, false, false, pix.data(), cnt3.data(), var.data()};
+    const char* path = "front_check.hdr";
+    if (int rc = r.render_scene_to_file(cam, p3, path)) { std::printf("file %d %s\n", rc, r.last_error().c_str()); return 8; }
+    FILE* f = std::fopen(path, "rb");
+    char magic[11] = {0};
+    if (!f || std::fread(magic, 1, 10, f) != 10 || std::string(magic) != "#?RADIANCE") return 9;
+    std::fclose(f);
+    std::remove(path);
     pico_scene_destroy(s);
     std::printf("ok\n");
     return 0;

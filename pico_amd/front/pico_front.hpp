@@ -2,6 +2,8 @@
 //
 //   Scene::Scene::render_scene_to_memory(const Camera&, const RenderParams&, const bool* should_quit)
 //       (Source/Core/Scene.hpp:68, Source/Core/Scene.cpp:143-235)
+//   Scene::Scene::render_scene_to_file(const Camera&, RenderParams&, const char*)
+//       (Source/Core/Scene.cpp:236-246)
 //   glm::vec3* Util::atrous_denoise(pixels, normals, position, diffuse, Tiler&)
 //       (Source/Util/Denoisers.hpp:9; returns a new[] buffer the caller delete[]s)
 //
@@ -11,6 +13,7 @@
 // (glm::vec3 in Pico).
 #pragma once
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdint>
@@ -105,6 +108,19 @@ public:
         if (params.m_tonemap)
             rc = pico_hip_tonemap(ctx_, reinterpret_cast<float*>(params.m_Pixels), params.m_Width, params.m_Height);
         return rc;
+    }
+
+    // Scene::render_scene_to_file (Source/Core/Scene.cpp:236-246): render,
+    // reverse the pixels end to end in the caller's buffer (as the reference
+    // does), write them as a Radiance .hdr file.
+    template <class Vec3>
+    int render_scene_to_file(const pico_camera& camera, RenderParams<Vec3>& params, const char* path,
+                             uint32_t frame_seed = 0) {
+        const bool quit = false;
+        int rc = render_scene_to_memory(camera, params, &quit, frame_seed);
+        if (rc) return rc;
+        std::reverse(params.m_Pixels, params.m_Pixels + (size_t)params.m_Width * params.m_Height);
+        return pico_write_hdr(path, params.m_Width, params.m_Height, reinterpret_cast<const float*>(params.m_Pixels), 0);
     }
 
 private:

@@ -20,7 +20,7 @@ def _build(tmp_path):
 
 def test_front_compiles_and_runs(tmp_path):
     exe = _build(tmp_path)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith(("ok", "nodevice")), r.stdout
 
@@ -28,5 +28,5 @@ def test_front_compiles_and_runs(tmp_path):
 @pytest.mark.gpu
 def test_front_renders_on_device(tmp_path):
     exe = _build(tmp_path)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
