@@ -184,6 +184,29 @@ int  pico_hip_upload_scene(pico_hip_ctx* ctx, const pico_scene_desc* scene);
 int  pico_hip_scene_info(pico_hip_ctx* ctx, uint64_t* bytes_nodes, uint64_t* bytes_tris,
                          uint64_t* bytes_instances, uint32_t* tlas_depth, uint32_t* blas_depth);
 
+/* Result of a device TLAS rebuild. */
+typedef struct pico_rebuild_stats {
+    float    device_ms;     /* the rebuild's kernels, event-timed on the context's stream */
+    uint32_t binary_depth;  /* deepest internal node of the binary (Karras) hierarchy */
+    uint32_t stack_bound;   /* traversal-stack entries the rebuilt scene can need (<= 64) */
+    uint32_t wide_nodes;    /* 4-wide TLAS nodes written */
+} pico_rebuild_stats;
+
+/* New transforms for every instance of the uploaded scene (n_instances x 16
+ * f32, column-major like glm::mat4, the layout of pico_scene_desc.transforms);
+ * the instance inverses and the TLAS are rebuilt on the device (linear BVH
+ * over Morton codes of the instance world boxes, collapsed to 4-wide nodes)
+ * and the BLAS are kept.  Replaces the TLAS part of the reference's scene
+ * build, UpperLevelBVH::add_lower_level_bvh / UpperLevelBVH::build
+ * (Source/Core/UpperLevelBVH.cpp:22-57), which the reference runs once on the
+ * host.  Closest hits do not depend on the tree, so a rebuild with the
+ * upload's own transforms renders the same image.  A transform that is not
+ * affine or is singular gives PICO_ERR_ARG and a tree too deep for the
+ * traversal stack PICO_ERR_LIMIT; in both cases the scene is left unchanged.
+ * `_d`: the transforms are in device memory. */
+int  pico_hip_set_instance_transforms(pico_hip_ctx* ctx, const float* transforms, pico_rebuild_stats* stats);
+int  pico_hip_set_instance_transforms_d(pico_hip_ctx* ctx, const float* d_transforms, pico_rebuild_stats* stats);
+
 /* Closest hit of n world-space rays (origins/dirs: 3 x f32 each).  Outputs
  * t (world distance), instance, mesh-local primitive and barycentrics (u, v);
  * misses give t = +inf and instance = prim = 0xffffffff. */

@@ -90,6 +90,11 @@ class Tuning(C.Structure):
         return (self.queue_paths, self.tile_w, self.tile_h, self.flags)
 
 
+class RebuildStats(C.Structure):
+    _fields_ = [("device_ms", C.c_float), ("binary_depth", C.c_uint32), ("stack_bound", C.c_uint32),
+                ("wide_nodes", C.c_uint32)]
+
+
 class View(C.Structure):
     _fields_ = [("camera", Camera), ("params", RenderParams), ("d_pixels", C.c_void_p),
                 ("d_sample_count", C.c_void_p), ("d_variance", C.c_void_p)]
@@ -121,6 +126,8 @@ def _declare(lib):
         "pico_hip_copy_d": (i32, [vp, vp, vp, sz]),
         "pico_hip_upload_scene": (i32, [vp, P(SceneDesc)]),
         "pico_hip_scene_info": (i32, [vp, P(u64), P(u64), P(u64), P(u32), P(u32)]),
+        "pico_hip_set_instance_transforms": (i32, [vp, vp, P(RebuildStats)]),
+        "pico_hip_set_instance_transforms_d": (i32, [vp, vp, P(RebuildStats)]),
         "pico_hip_closest_hit": (i32, [vp, u32, vp, vp, vp, vp, vp, vp]),
         "pico_hip_primary_ids": (i32, [vp, P(Camera), u32, u32, f32, f32, vp, vp]),
         "pico_hip_render": (i32, [vp, P(Camera), P(RenderParams), vp, vp, vp, vp, P(Stats)]),
@@ -345,6 +352,20 @@ class Context:
         self._check(lib().pico_hip_scene_info(self._h, C.byref(bn), C.byref(bt), C.byref(bi), C.byref(td), C.byref(bd)))
         return dict(bytes_nodes=bn.value, bytes_tris=bt.value, bytes_instances=bi.value, tlas_depth=td.value,
                     blas_depth=bd.value)
+
+    def set_instance_transforms(self, transforms):
+        """New column-major 4x4 transforms (n_instances x 16 f32, the scene
+        description's layout) for every instance; the TLAS is rebuilt on the
+        device.  Returns RebuildStats."""
+        m = _f32(transforms, (-1, 16))
+        st = RebuildStats()
+        self._check(lib().pico_hip_set_instance_transforms(self._h, _ptr(m), C.byref(st)))
+        return st
+
+    def set_instance_transforms_d(self, d_transforms):
+        st = RebuildStats()
+        self._check(lib().pico_hip_set_instance_transforms_d(self._h, d_transforms.ptr, C.byref(st)))
+        return st
 
     def closest_hit(self, origins, dirs):
         o, d = _f32(origins, (-1, 3)), _f32(dirs, (-1, 3))

@@ -101,11 +101,13 @@ __global__ __launch_bounds__(kRayBlock) void k_gbuffer(DevScene s, DCam cam, flo
     }
 }
 
+// Copies v into a new device allocation of at least `capacity` elements.
 template <class T>
-int upload_vec(pico_hip_ctx* c, const std::vector<T>& v, const T** out) {
+int upload_vec(pico_hip_ctx* c, const std::vector<T>& v, const T** out, size_t capacity = 0) {
     void* p = nullptr;
     const size_t bytes = v.size() * sizeof(T);
-    PICO_HIP(c, hipMalloc(&p, bytes ? bytes : 16));
+    const size_t alloc = std::max(bytes, capacity * sizeof(T));
+    PICO_HIP(c, hipMalloc(&p, alloc ? alloc : 16));
     c->scene_allocs.push_back(p);
     if (bytes) PICO_HIP(c, hipMemcpyAsync(p, v.data(), bytes, hipMemcpyHostToDevice, c->stream));
     *out = reinterpret_cast<const T*>(p);
@@ -293,7 +295,13 @@ int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) {
     if (need > 64) return fail(c, PICO_ERR_LIMIT, "BVH too deep for the 64-entry traversal stack");
     DevScene s{};
     int rc;
-    if ((rc = upload_vec(c, hs.nodes4, &s.nodes))) return rc;
+    // a device TLAS rebuild emits at most max(n - 1, 1) nodes after tlas_base
+    const size_t node_cap = hs.tlas_base + std::max<size_t>(hs.n_instances > 1 ? hs.n_instances - 1 : 1, 1);
+    if ((rc = upload_vec(c, hs.nodes4, &s.nodes, node_cap))) return rc;
+    const float* mesh_box = nullptr;
+    const uint32_t* inst_mesh = nullptr;
+    if ((rc = upload_vec(c, hs.mesh_box, &mesh_box))) return rc;
+    if ((rc = upload_vec(c, hs.instance_mesh, &inst_mesh))) return rc;
     if ((rc = upload_vec(c, hs.tris, &s.tris))) return rc;
     if ((rc = upload_vec(c, hs.instances, &s.inst))) return rc;
     if ((rc = upload_vec(c, hs.xforms, &s.xf))) return rc;
@@ -311,6 +319,10 @@ int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) {
     s.microfacet = hs.microfacet;
     c->dev = s;
     c->stack_need = need;
+    c->d_mesh_box = mesh_box;
+    c->d_instance_mesh = inst_mesh;
+    c->tlas_base = hs.tlas_base;
+    c->blas_bound = hs.blas_bound;
     c->host = std::move(hs);
     c->has_scene = true;
     ++c->scene_serial;

@@ -9,6 +9,7 @@
 // follow LowerLevelMeshBVH::generate_sampling_data (Source/Core/LowerLevelMeshBVH.cpp:92-117)
 // and AliasTable::build_table (Source/Util/AliasTable.cpp:12-87).
 #include "bvh_build.h"
+#include "scene_math.h"
 
 #include <algorithm>
 #include <cmath>
@@ -144,33 +145,6 @@ struct Builder {
     }
 };
 
-// Inverse of a column-major 4x4 by cofactor expansion in single precision
-// (the reference inverts instance transforms in float, Source/Core/UpperLevelBVH.cpp:24);
-// rows 0..2 of the inverse are returned.  The expansion order is fixed so that
-// every consumer of the scene (device traversal, CPU checks) sees the same
-// object-space rays bit for bit.
-bool affine_inverse_rows(const float* m, float inv[3][4]) {
-    auto e = [&](int r, int c) { return m[c * 4 + r]; };
-    auto minor3 = [&](int r0, int r1, int r2, int c0, int c1, int c2) {
-        return e(r0, c0) * (e(r1, c1) * e(r2, c2) - e(r2, c1) * e(r1, c2)) -
-               e(r0, c1) * (e(r1, c0) * e(r2, c2) - e(r2, c0) * e(r1, c2)) +
-               e(r0, c2) * (e(r1, c0) * e(r2, c1) - e(r2, c0) * e(r1, c1));
-    };
-    const int rs[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
-    float cof[4][4];
-    for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) {
-            const float mn = minor3(rs[r][0], rs[r][1], rs[r][2], rs[c][0], rs[c][1], rs[c][2]);
-            cof[r][c] = ((r + c) & 1) ? -mn : mn;
-        }
-    const float det = e(0, 0) * cof[0][0] + e(0, 1) * cof[0][1] + e(0, 2) * cof[0][2] + e(0, 3) * cof[0][3];
-    if (!(std::fabs(det) > 1e-30f) || !std::isfinite(det)) return false;
-    const float id = 1.0f / det;
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 4; ++c) inv[r][c] = cof[c][r] * id;
-    return true;
-}
-
 inline float fdot(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
 // Triangle "area" of the reference's light sampler (the sine takes the angle
@@ -214,41 +188,6 @@ void alias_table(const std::vector<float>& w, std::vector<float>& bw, std::vecto
 }
 
 }  // namespace
-
-void quantise_children(const float (*lo)[3], const float (*hi)[3], int nc, DNode4& nd) {
-    std::memset(&nd, 0, sizeof(nd));
-    for (int i = 0; i < 4; ++i) nd.ref[i] = kEmptyRef;
-    uint8_t* q[3] = {nd.qx, nd.qy, nd.qz};
-    uint32_t exps = 0;
-    for (int k = 0; k < 3; ++k) {
-        float o = INFINITY, m = -INFINITY;
-        for (int i = 0; i < nc; ++i) { o = std::min(o, lo[i][k]); m = std::max(m, hi[i][k]); }
-        if (nc == 0 || !(o <= m)) { o = 0.0f; m = 0.0f; }
-        nd.origin[k] = o;
-        const double ext = (double)m - (double)o;
-        int e = ext > 0.0 ? (int)std::ceil(std::log2(ext / 255.0)) : -126;
-        e = std::min(std::max(e, -126), 126);
-        for (;; ++e) {
-            // decoded plane = fma(q, 2^e, origin) (one rounding, as on the device)
-            const float sc = std::ldexp(1.0f, e);
-            bool ok = true;
-            for (int i = 0; i < nc && ok; ++i) {
-                int ql = (int)std::floor(((double)lo[i][k] - o) / sc);
-                ql = std::min(std::max(ql, 0), 255);
-                while (ql > 0 && std::fma((float)ql, sc, o) > lo[i][k]) --ql;
-                int qh = (int)std::ceil(((double)hi[i][k] - o) / sc);
-                qh = std::min(std::max(qh, 0), 255);
-                while (qh < 255 && std::fma((float)qh, sc, o) < hi[i][k]) ++qh;
-                if (std::fma((float)ql, sc, o) > lo[i][k] || std::fma((float)qh, sc, o) < hi[i][k]) ok = false;
-                q[k][i] = (uint8_t)ql;
-                q[k][4 + i] = (uint8_t)qh;
-            }
-            if (ok || e >= 126) break;
-        }
-        exps |= (uint32_t)(uint8_t)(int8_t)e << (8 * k);
-    }
-    nd.exps = exps;
-}
 
 namespace {
 struct WideBuilder {
@@ -383,6 +322,8 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
             hi.insert(hi.end(), b.hi, b.hi + 3);
         }
         for (uint32_t v = v0; v < v1; ++v) mesh_box[m].grow_pt(&d.positions[3 * (size_t)v]);
+        out.mesh_box.insert(out.mesh_box.end(), mesh_box[m].lo, mesh_box[m].lo + 3);
+        out.mesh_box.insert(out.mesh_box.end(), mesh_box[m].hi, mesh_box[m].hi + 3);
         std::vector<uint32_t> order;
         uint32_t depth = 0;
         const uint32_t base = (uint32_t)out.tris.size();
@@ -426,23 +367,12 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
         di.material = d.instance_material[i];
         di.bsdf = d.instance_bsdf[i];
         out.instances.push_back(di);
+        out.instance_mesh.push_back(m);
         DXform xf;
         std::memcpy(xf.m, M, sizeof(xf.m));
         out.xforms.push_back(xf);
-        // 8-corner world box (AABB * mat4)
         Box wb;
-        const Box& mb = mesh_box[m];
-        for (int c = 0; c < 8; ++c) {
-            const float p[3] = {(c & 1) ? mb.hi[0] : mb.lo[0], (c & 2) ? mb.hi[1] : mb.lo[1], (c & 4) ? mb.hi[2] : mb.lo[2]};
-            float w[3];
-            for (int r = 0; r < 3; ++r) w[r] = M[r] * p[0] + M[4 + r] * p[1] + M[8 + r] * p[2] + M[12 + r];
-            wb.grow_pt(w);
-        }
-        // widen by a relative epsilon so float rounding of the transform never culls a hit
-        for (int k = 0; k < 3; ++k) {
-            const float e = 1e-5f * (std::fabs(wb.lo[k]) + std::fabs(wb.hi[k]) + 1.0f);
-            wb.lo[k] -= e; wb.hi[k] += e;
-        }
+        instance_world_box(M, mesh_box[m].lo, mesh_box[m].hi, wb.lo, wb.hi);
         lo.insert(lo.end(), wb.lo, wb.lo + 3);
         hi.insert(hi.end(), wb.hi, wb.hi + 3);
     }
@@ -470,6 +400,8 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
         }
         for (uint32_t i = 0; i < d.n_instances; ++i) out.instances[i].blas_root = root4[d.instance_mesh[i]];
         uint32_t tb = 0;
+        out.tlas_base = (uint32_t)out.nodes4.size();
+        out.blas_bound = blas_bound;
         out.tlas_root = collapse_bvh4(out.nodes, out.tlas_root, out.nodes4, tb);
         out.stack_bound = tb + 1 + blas_bound;  // TLAS entries, the instance sentinel, BLAS entries
     }

@@ -23,7 +23,11 @@ struct HostScene {
     std::vector<float> normals;        // 3 x f32 per vertex (global)
     std::vector<uint32_t> indices;     // 3 x u32 per triangle, global vertex ids
     std::vector<uint32_t> mesh_tri_offset;
+    std::vector<float> mesh_box;       // 6 x f32 per mesh: object-space vertex bounds lo, hi
+    std::vector<uint32_t> instance_mesh;
     uint32_t tlas_root = 0;            // DNode4 index
+    uint32_t tlas_base = 0;            // first TLAS DNode4 (every BLAS node comes before it)
+    uint32_t blas_bound = 0;           // most stack entries any BLAS traversal can hold
     uint32_t tlas_depth = 0;
     uint32_t blas_depth = 0;           // deepest BLAS (binary levels)
     uint32_t stack_bound = 0;          // most stack entries any 4-wide traversal can hold
@@ -44,10 +48,6 @@ uint32_t build_bvh(const std::vector<float>& box_lo, const std::vector<float>& b
 // child until four); returns the DNode4 index of the root and, in `bound`,
 // the most traversal-stack entries a ray can hold in this subtree.
 uint32_t collapse_bvh4(const std::vector<DNode>& n2, uint32_t root2, std::vector<DNode4>& out, uint32_t& bound);
-
-// Quantises `nc` child boxes (lo/hi, 3 floats each) into a DNode4 whose
-// decoded boxes fma(q, 2^e, origin) contain the originals.
-void quantise_children(const float (*lo)[3], const float (*hi)[3], int nc, DNode4& nd);
 
 // Validates the description and builds everything; returns an error text or "".
 std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out);

@@ -41,6 +41,12 @@ struct pico_hip_ctx {
     pico::DevScene dev{};
     std::vector<void*> scene_allocs;
     uint32_t stack_need = 0;
+    // kept from the upload for the device TLAS rebuild: the node array has
+    // room for the largest TLAS a rebuild can emit after tlas_base
+    const uint32_t* d_instance_mesh = nullptr;
+    const float* d_mesh_box = nullptr;
+    uint32_t tlas_base = 0;
+    uint32_t blas_bound = 0;
     // named scratch buffers (grown on demand, freed at destroy)
     std::map<std::string, DevBuf> bufs;
     // launch shape of the wavefront and the autotuner's cache
