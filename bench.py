@@ -311,6 +311,78 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3):
                        "calib_ms": round(cal_ms, 2), "default_calib_ms": round(def_ms, 2)}}
 
 
+def _turned(xf, rng, angle, shift):
+    """Column-major instance transforms, each turned about its own origin by a
+    random rotation of up to `angle` radians and moved by up to `shift`."""
+    n = xf.shape[0]
+    m = xf.astype(np.float64).reshape(n, 4, 4).transpose(0, 2, 1)
+    ax = rng.normal(size=(n, 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    a = rng.uniform(-angle, angle, n)[:, None, None]
+    k = np.zeros((n, 3, 3))
+    k[:, 0, 1], k[:, 0, 2], k[:, 1, 2] = -ax[:, 2], ax[:, 1], -ax[:, 0]
+    k[:, 1, 0], k[:, 2, 0], k[:, 2, 1] = ax[:, 2], -ax[:, 1], ax[:, 0]
+    r = np.eye(3) + np.sin(a) * k + (1 - np.cos(a)) * (k @ k)
+    t = np.tile(np.eye(4), (n, 1, 1))
+    t[:, :3, :3] = r
+    c = m[:, :3, 3]
+    t[:, :3, 3] = c - np.einsum("nij,nj->ni", r, c) + rng.uniform(-shift, shift, (n, 3))
+    out = t @ m
+    out[:, 3] = (0.0, 0.0, 0.0, 1.0)
+    return np.ascontiguousarray(out.transpose(0, 2, 1).reshape(n, 16), dtype=np.float32)
+
+
+def tlas_leg(ctx, pa, scene, cam, W, H, depth, d_pix, d_cnt, d_var, frames=8, spp=16):
+    """Device TLAS rebuild: its time per frame of animation (every instance
+    moved), the traversal cost of the linear-BVH TLAS against the host SAH
+    TLAS on the same frame, and the rebuild against the host build at 65536
+    instances."""
+    import ctypes as C
+    d = scene.desc
+    n = d.n_instances
+    base = np.ctypeslib.as_array(C.cast(d.transforms, C.POINTER(C.c_float)), (n * 16,)).copy().reshape(n, 16)
+    p = pa.params(W, H, spp, depth, tile=16, frame_seed=5)
+    host = ctx.render_d(cam, p, d_pix, d_cnt, d_var)
+    img_host = d_pix.download(np.float32, 3 * W * H)
+    st = ctx.set_instance_transforms(base)
+    dev = ctx.render_d(cam, p, d_pix, d_cnt, d_var)
+    img_dev = d_pix.download(np.float32, 3 * W * H)
+    same = float(np.all(img_host.reshape(-1, 3) == img_dev.reshape(-1, 3), axis=1).mean())
+    rng = np.random.default_rng(21)
+    buf = ctx.alloc(64 * n)
+    rebuild, render = [], []
+    for f in range(frames + 2):
+        buf.upload(_turned(base, rng, 0.05 * f, 0.5 * f))
+        s = ctx.set_instance_transforms_d(buf)
+        r = ctx.render_d(cam, p, d_pix, d_cnt, d_var)
+        if f >= 2:
+            rebuild.append(s.device_ms)
+            render.append(r.render_ms)
+    buf.free()
+    ctx.upload_scene(scene)  # back to the host-built TLAS
+    big = pa.Scene.instanced(3, 16, 65536, 20, 200, 400.0, pa.MICROFACET_GGX)
+    t0 = time.perf_counter()
+    ctx.upload_scene(big)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    nb = big.desc.n_instances
+    bx = np.ctypeslib.as_array(C.cast(big.desc.transforms, C.POINTER(C.c_float)), (nb * 16,)).copy().reshape(nb, 16)
+    big_ms = []
+    for f in range(6):
+        s = ctx.set_instance_transforms(_turned(bx, rng, 0.5, 2.0))
+        if f >= 2:
+            big_ms.append(s.device_ms)
+    ctx.upload_scene(scene)
+    out = {"instances": n, "rebuild_ms": round(float(np.median(rebuild)), 3), "wide_nodes": st.wide_nodes,
+           "binary_depth": st.binary_depth, "stack_bound": st.stack_bound,
+           "render_ms_host_sah_tlas": round(host.render_ms, 2), "render_ms_device_tlas": round(dev.render_ms, 2),
+           "identical_pixels": round(same, 5), "frame": f"{W}x{H}, {spp} spp",
+           "animated_render_ms": round(float(np.median(render)), 2),
+           "instances_65536": {"instances": nb, "rebuild_ms": round(float(np.median(big_ms)), 3),
+                               "host_upload_ms": round(host_ms, 1),
+                               "minstances_per_s": round(nb / (float(np.median(big_ms)) * 1e-3) / 1e6, 1)}}
+    return out
+
+
 def _ptr_array(ctx, ptr):
     """A non-owning device array at a raw pointer (the views' own buffers)."""
     import pico_amd as pa
@@ -335,6 +407,7 @@ def main():
     ap.add_argument("--oracle-spp", type=int, default=4096, help="samples of the FP64 quality oracle")
     ap.add_argument("--no-quality", action="store_true", help="skip the relMSE / ms-per-frame leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views leg")
+    ap.add_argument("--no-tlas", action="store_true", help="skip the device TLAS rebuild leg")
     ap.add_argument("--weights", default=None, help="trained denoiser parameters (.npy; default: the shipped ones)")
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the streaming-copy calibration")
     args = ap.parse_args()
@@ -430,6 +503,11 @@ def main():
         batched = batched_leg(ctx, pa, args.depth, args.weights or pa.weights_path())
         log(f"[bench] batched views: {batched}")
 
+    tlas = None
+    if not args.no_tlas:
+        tlas = tlas_leg(ctx, pa, scene, cam, W, H, args.depth, d_pix, d_cnt, d_var)
+        log(f"[bench] device TLAS rebuild: {tlas}")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
@@ -445,7 +523,7 @@ def main():
                       "unique_triangles": uniq, "instanced_triangles": inst_tris,
                       "rays_per_step": rays // args.steps, "traced_rays_per_step": traced // args.steps},
            "roofline": roof, "cpu_baseline": cpu, "quality": quality,
-           "batched_views": batched}
+           "batched_views": batched, "tlas_rebuild": tlas}
     if cpu:
         out["gpu_over_cpu"] = round(mrays / cpu["value"], 1)
     print(json.dumps(out), flush=True)
