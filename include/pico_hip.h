@@ -180,6 +180,36 @@ int  pico_hip_copy_d(pico_hip_ctx* ctx, void* d_dst, const void* d_src, size_t b
 /* Builds the BLAS (one per mesh, 16-bin binned SAH) and the TLAS over
  * instances on the host, lays them out for the device and uploads them. */
 int  pico_hip_upload_scene(pico_hip_ctx* ctx, const pico_scene_desc* scene);
+
+/* Where pico_hip_upload_scene_ex builds the BVHs. */
+#define PICO_BUILD_DEVICE   1u   /* BLAS and TLAS built by kernels instead of on the host */
+
+typedef struct pico_build_stats {
+    double   total_ms;      /* the whole upload: checks, tables, copies, BVH build */
+    float    bvh_ms;        /* the BVH build: host wall time, or the device kernels event-timed */
+    uint32_t binary_nodes;  /* binary BLAS nodes over every mesh */
+    uint32_t blas_nodes4;   /* 4-wide BLAS nodes over every mesh */
+    uint32_t blas_bound;    /* most traversal-stack entries any BLAS can need */
+    uint32_t stack_bound;   /* the same for the whole scene (<= 64) */
+    uint32_t levels;        /* deepest binary BLAS, in levels of internal nodes */
+} pico_build_stats;
+
+/* pico_hip_upload_scene with a choice of builder and build statistics
+ * (stats may be NULL).  With PICO_BUILD_DEVICE every mesh BVH is built on
+ * the GPU, all meshes at once, level by level: binned SAH with the host
+ * builder's split rule (16 bins on each axis, surface-area cost, leaves of
+ * at most 4 triangles), so both builders make the same binary tree except
+ * where all centroids of a node coincide; then the same greedy collapse to
+ * 4-wide nodes, and the TLAS as in pico_hip_set_instance_transforms.  The
+ * host only checks the description and builds the material, instance and
+ * light tables.  Replaces the serial host build of the reference's scene
+ * set-up: BVHFactory::generate_BVH / SAH_parition_Scheme
+ * (Source/Core/BVH.cpp:73-197), LowerLevelMeshBVH::build
+ * (Source/Core/LowerLevelMeshBVH.cpp:48-77) and UpperLevelBVH::build
+ * (Source/Core/UpperLevelBVH.cpp:27-57).  Errors as pico_hip_upload_scene;
+ * on any error no scene is left uploaded. */
+int  pico_hip_upload_scene_ex(pico_hip_ctx* ctx, const pico_scene_desc* scene, uint32_t flags,
+                              pico_build_stats* stats);
 /* Node/triangle/instance byte sizes and tree depths of the uploaded scene. */
 int  pico_hip_scene_info(pico_hip_ctx* ctx, uint64_t* bytes_nodes, uint64_t* bytes_tris,
                          uint64_t* bytes_instances, uint32_t* tlas_depth, uint32_t* blas_depth);

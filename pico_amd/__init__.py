@@ -23,6 +23,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpico_hip.so")
 
 PICO_OK = 0
+BUILD_DEVICE = 1
 PICO_ERR_ARG = -1
 PICO_ERR_HIP = -2
 PICO_ERR_STATE = -3
@@ -90,6 +91,12 @@ class Tuning(C.Structure):
         return (self.queue_paths, self.tile_w, self.tile_h, self.flags)
 
 
+class BuildStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("bvh_ms", C.c_float), ("binary_nodes", C.c_uint32),
+                ("blas_nodes4", C.c_uint32), ("blas_bound", C.c_uint32), ("stack_bound", C.c_uint32),
+                ("levels", C.c_uint32)]
+
+
 class RebuildStats(C.Structure):
     _fields_ = [("device_ms", C.c_float), ("binary_depth", C.c_uint32), ("stack_bound", C.c_uint32),
                 ("wide_nodes", C.c_uint32)]
@@ -125,6 +132,7 @@ def _declare(lib):
         "pico_hip_kernel_time": (i32, [vp, C.c_char_p, P(u64), P(f64)]),
         "pico_hip_copy_d": (i32, [vp, vp, vp, sz]),
         "pico_hip_upload_scene": (i32, [vp, P(SceneDesc)]),
+        "pico_hip_upload_scene_ex": (i32, [vp, P(SceneDesc), u32, P(BuildStats)]),
         "pico_hip_scene_info": (i32, [vp, P(u64), P(u64), P(u64), P(u32), P(u32)]),
         "pico_hip_set_instance_transforms": (i32, [vp, vp, P(RebuildStats)]),
         "pico_hip_set_instance_transforms_d": (i32, [vp, vp, P(RebuildStats)]),
@@ -343,8 +351,13 @@ class Context:
     def alloc(self, nbytes):
         return DeviceArray(self, nbytes)
 
-    def upload_scene(self, scene):
-        self._check(lib().pico_hip_upload_scene(self._h, C.byref(scene.desc)))
+    def upload_scene(self, scene, device_build=False):
+        """Builds the BVHs (on the host, or with device_build on the GPU) and
+        uploads the scene; returns BuildStats."""
+        st = BuildStats()
+        self._check(lib().pico_hip_upload_scene_ex(self._h, C.byref(scene.desc), BUILD_DEVICE if device_build else 0,
+                                                   C.byref(st)))
+        return st
 
     def scene_info(self):
         bn, bt, bi = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -3,6 +3,7 @@
 // filters in post.hip.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -284,12 +285,18 @@ int pico_hip_kernel_time(pico_hip_ctx* c, const char* kernel, uint64_t* launches
     return PICO_OK;
 }
 
-int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) {
+int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) { return pico_hip_upload_scene_ex(c, d, 0, nullptr); }
+
+int pico_hip_upload_scene_ex(pico_hip_ctx* c, const pico_scene_desc* d, uint32_t flags, pico_build_stats* stats) {
     if (!c || !d) return fail(c, PICO_ERR_ARG, "scene description is NULL");
+    if (flags & ~PICO_BUILD_DEVICE) return fail(c, PICO_ERR_ARG, "unknown build flags");
     PICO_HIP(c, hipSetDevice(c->device));
     free_scene(c);
+    if (flags & PICO_BUILD_DEVICE) return upload_scene_device(c, *d, stats);
+    const auto t0 = std::chrono::steady_clock::now();
     HostScene hs;
     std::string e = build_scene(*d, 16, hs);
+    const double build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (!e.empty()) return fail(c, PICO_ERR_ARG, e);
     const uint32_t need = hs.stack_bound;
     if (need > 64) return fail(c, PICO_ERR_LIMIT, "BVH too deep for the 64-entry traversal stack");
@@ -323,6 +330,20 @@ int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) {
     c->d_instance_mesh = inst_mesh;
     c->tlas_base = hs.tlas_base;
     c->blas_bound = hs.blas_bound;
+    c->n_nodes4 = hs.nodes4.size();
+    c->n_tris = hs.tris.size();
+    c->tlas_depth = hs.tlas_depth;
+    c->blas_depth = hs.blas_depth;
+    if (stats) {
+        *stats = pico_build_stats{};
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->bvh_ms = (float)build_ms;
+        stats->binary_nodes = hs.blas_nodes2;
+        stats->blas_nodes4 = hs.tlas_base;
+        stats->blas_bound = hs.blas_bound;
+        stats->stack_bound = hs.stack_bound;
+        stats->levels = hs.blas_depth;
+    }
     c->host = std::move(hs);
     c->has_scene = true;
     ++c->scene_serial;
@@ -332,11 +353,11 @@ int pico_hip_upload_scene(pico_hip_ctx* c, const pico_scene_desc* d) {
 int pico_hip_scene_info(pico_hip_ctx* c, uint64_t* bn, uint64_t* bt, uint64_t* bi, uint32_t* td, uint32_t* bd) {
     if (!c) return PICO_ERR_ARG;
     if (!c->has_scene) return fail(c, PICO_ERR_STATE, "no scene uploaded");
-    if (bn) *bn = c->host.nodes4.size() * sizeof(DNode4);
-    if (bt) *bt = c->host.tris.size() * sizeof(DTri);
-    if (bi) *bi = c->host.instances.size() * sizeof(DInstance);
-    if (td) *td = c->host.tlas_depth;
-    if (bd) *bd = c->host.blas_depth;
+    if (bn) *bn = c->n_nodes4 * sizeof(DNode4);
+    if (bt) *bt = c->n_tris * sizeof(DTri);
+    if (bi) *bi = (uint64_t)c->host.n_instances * sizeof(DInstance);
+    if (td) *td = c->tlas_depth;
+    if (bd) *bd = c->blas_depth;
     return PICO_OK;
 }
 

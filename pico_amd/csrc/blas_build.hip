@@ -1,0 +1,756 @@
+// Device build of a whole scene: every mesh BVH on the GPU, all meshes at
+// once, then the TLAS (tlas.hip).  The host keeps only the checks of the
+// description and the small tables (materials, instances, lights).
+//
+// Binary build, level by level (one launch per level, one workgroup per
+// node of the level):
+//   - bin the node's triangles by centroid, 16 bins on each axis, counts and
+//     triangle / centroid boxes in LDS (exact float min/max through mapped
+//     integer atomics, so the bins do not depend on the order of arrival);
+//   - one lane evaluates the surface-area cost of the 45 candidate planes in
+//     the host builder's order and arithmetic, so both builders take the same
+//     decisions and make the same tree; a node becomes a leaf when it holds
+//     one triangle, or at most 4 and the leaf is no dearer than the split;
+//   - a stable partition of the node's range of the triangle permutation
+//     (wave ballots and a workgroup prefix) writes the children's ranges into
+//     the other permutation buffer; the children's boxes come from the bins.
+// Leaves write their triangle records in place (range position = record
+// index), so leaves never move again.  The binary tree is then collapsed to
+// 4-wide nodes level by level with the host's greedy rule (open the child of
+// largest area until four), quantised with the host's function, and the
+// traversal-stack bound is the largest sum of (children - 1) down any path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "../../include/pico_hip.h"
+#include "ctx.h"
+#include "dev_atomics.h"
+#include "scene_math.h"
+
+using namespace pico;
+
+namespace {
+
+constexpr uint32_t kBins = 16;
+constexpr uint32_t kMaxLeaf = 4;
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kRootSlot = 0x80000000u;  // slot | mesh: the reference goes to the mesh's root entry
+constexpr uint32_t kMaxLevels = 1u << 16;
+
+struct BuildTask {
+    uint32_t begin, end;  // range of the triangle permutation
+    uint32_t slot;        // binary node * 2 + child that receives this node's reference, or kRootSlot | mesh
+    uint32_t mesh;
+    float box[6];         // union of the triangle boxes: lo, hi
+    float cbox[6];        // bounds of the triangle centroids: lo, hi
+};
+static_assert(sizeof(BuildTask) == 64, "task is one 64-byte record");
+
+struct WideTask {
+    uint32_t bnode;  // binary node the wide node is made from
+    uint32_t slot;   // wide node * 4 + child that receives this node's index, or kRootSlot | mesh
+    uint32_t acc;    // stack entries the wide nodes above can leave
+    uint32_t pad;
+};
+
+struct Geo {
+    const float* pos;       // vertex positions (3 per vertex, all meshes)
+    const uint32_t* lidx;   // mesh-local triangle indices as given
+    uint32_t* gidx;         // global vertex indices (written)
+    const uint32_t* vofs;   // n_meshes + 1 vertex offsets
+    const uint32_t* tofs;   // n_meshes + 1 triangle offsets
+    uint32_t n_meshes, n_tris, n_verts;
+    float* pbox;            // 6 per triangle
+    float* pcen;            // 3 per triangle
+    uint32_t* perm;         // identity permutation (written)
+    uint32_t* mbox;         // 6 per mesh, mapped: vertex bounds (the TLAS uses them)
+    uint32_t* rbox;         // 12 per mesh, mapped: triangle box union, centroid bounds
+    uint32_t* err;
+};
+
+struct Level {
+    const BuildTask* in;
+    BuildTask* out;
+    uint32_t* out_count;
+    const uint32_t* perm_in;
+    uint32_t* perm_out;
+    const float* pbox;
+    const float* pcen;
+    const float* pos;
+    const uint32_t* gidx;
+    const uint32_t* tofs;
+    DNode* bnodes;
+    uint32_t* bcount;
+    uint32_t* mesh_root2;
+    DTri* tris;
+};
+
+// host Box arithmetic (bvh_build.cpp), replayed on the device
+struct DBox {
+    float lo[3], hi[3];
+    __device__ void clear() {
+        for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+    }
+    __device__ void grow(const float* l, const float* h) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = l[k] < lo[k] ? l[k] : lo[k];  // std::min(lo, l)
+            hi[k] = hi[k] < h[k] ? h[k] : hi[k];  // std::max(hi, h)
+        }
+    }
+    __device__ float area() const {
+        float d[3];
+        for (int k = 0; k < 3; ++k) {
+            const float x = hi[k] - lo[k];
+            d[k] = 0.0f < x ? x : 0.0f;
+        }
+        return 2.0f * (d[0] * d[1] + d[0] * d[2] + d[1] * d[2]);
+    }
+    __device__ bool empty() const { return !(lo[0] <= hi[0]); }
+};
+
+__device__ __forceinline__ uint32_t find_mesh(const uint32_t* ofs, uint32_t n_meshes, uint32_t x) {
+    // the last m with ofs[m] <= x (skips empty meshes, whose offsets repeat)
+    uint32_t lo = 0, hi = n_meshes;  // ofs[lo] <= x < ofs[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (ofs[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_geo_clear(Geo g) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n_meshes; i += gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            g.mbox[6 * i + k] = kMappedPosInf;
+            g.mbox[6 * i + 3 + k] = kMappedNegInf;
+            g.rbox[12 * i + k] = kMappedPosInf;
+            g.rbox[12 * i + 3 + k] = kMappedNegInf;
+            g.rbox[12 * i + 6 + k] = kMappedPosInf;
+            g.rbox[12 * i + 9 + k] = kMappedNegInf;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *g.err = 0;
+}
+
+// Per triangle: global indices, box, centroid, identity permutation, mesh
+// box union and centroid bounds.
+__global__ void k_geo_tris(Geo g) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < g.n_tris; t += gridDim.x * blockDim.x) {
+        const uint32_t m = find_mesh(g.tofs, g.n_meshes, t);
+        const uint32_t v0 = g.vofs[m], nv = g.vofs[m + 1] - v0;
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = 0; c < 3; ++c) {
+            uint32_t li = g.lidx[3 * (size_t)t + c];
+            if (li >= nv) {
+                atomicOr(g.err, 1u);
+                li = 0;
+            }
+            const uint32_t v = v0 + li;
+            g.gidx[3 * (size_t)t + c] = v;
+            const float* p = g.pos + 3 * (size_t)v;
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = p[k] < lo[k] ? p[k] : lo[k];
+                hi[k] = hi[k] < p[k] ? p[k] : hi[k];
+            }
+        }
+        uint32_t* rb = g.rbox + 12 * (size_t)m;
+        for (int k = 0; k < 3; ++k) {
+            const float cen = 0.5f * (lo[k] + hi[k]);
+            g.pbox[6 * (size_t)t + k] = lo[k];
+            g.pbox[6 * (size_t)t + 3 + k] = hi[k];
+            g.pcen[3 * (size_t)t + k] = cen;
+            atomic_min_f(rb + k, lo[k]);
+            atomic_max_f(rb + 3 + k, hi[k]);
+            atomic_min_f(rb + 6 + k, cen);
+            atomic_max_f(rb + 9 + k, cen);
+        }
+        g.perm[t] = t;
+    }
+}
+
+// Per vertex: the mesh's vertex bounds (the host's mesh box for the TLAS).
+__global__ void k_geo_verts(Geo g) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n_verts; v += gridDim.x * blockDim.x) {
+        const uint32_t m = find_mesh(g.vofs, g.n_meshes, v);
+        const float* p = g.pos + 3 * (size_t)v;
+        for (int k = 0; k < 3; ++k) {
+            atomic_min_f(g.mbox + 6 * (size_t)m + k, p[k]);
+            atomic_max_f(g.mbox + 6 * (size_t)m + 3 + k, p[k]);
+        }
+    }
+}
+
+__device__ void set_child(DNode& nd, int c, const DBox& b, uint32_t ref) {
+    float* x = c == 0 ? nd.c0x : nd.c1x;
+    float* y = c == 0 ? nd.c0y : nd.c1y;
+    float* z = c == 0 ? nd.c0z : nd.c1z;
+    x[0] = b.lo[0]; x[1] = b.hi[0];
+    y[0] = b.lo[1]; y[1] = b.hi[1];
+    z[0] = b.lo[2]; z[1] = b.hi[2];
+    nd.ref[c] = ref;
+}
+
+// One root task per mesh with triangles; a mesh without triangles gets a
+// node with two empty children, as on the host.
+__global__ void k_roots(Geo g, BuildTask* out, uint32_t* out_count, DNode* bnodes, uint32_t* bcount, uint32_t* mesh_root2,
+                        float* mesh_box) {
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < g.n_meshes; m += gridDim.x * blockDim.x) {
+        for (int k = 0; k < 6; ++k) mesh_box[6 * (size_t)m + k] = unmap_f(g.mbox[6 * (size_t)m + k]);
+        const uint32_t b = g.tofs[m], e = g.tofs[m + 1];
+        if (b == e) {
+            const uint32_t id = atomicAdd(bcount, 1u);
+            DNode nd{};
+            DBox none;
+            none.clear();
+            set_child(nd, 0, none, kEmptyRef);
+            set_child(nd, 1, none, kEmptyRef);
+            bnodes[id] = nd;
+            mesh_root2[m] = id;
+            continue;
+        }
+        BuildTask t;
+        t.begin = b;
+        t.end = e;
+        t.slot = kRootSlot | m;
+        t.mesh = m;
+        for (int k = 0; k < 6; ++k) {
+            t.box[k] = unmap_f(g.rbox[12 * (size_t)m + k]);
+            t.cbox[k] = unmap_f(g.rbox[12 * (size_t)m + 6 + k]);
+        }
+        out[atomicAdd(out_count, 1u)] = t;
+    }
+}
+
+__device__ __forceinline__ uint32_t bin_of(float c, float lo, float sc) {
+    const float x = (c - lo) * sc;
+    return (uint32_t)(x < (float)(kBins - 1) ? x : (float)(kBins - 1));  // std::min<float>(bins - 1, x)
+}
+
+struct LevelShared {
+    uint32_t cnt[3][kBins];
+    uint32_t box[3][kBins][6];   // mapped triangle box per bin
+    uint32_t cen[3][kBins][6];   // mapped centroid bounds per bin
+    uint32_t half[2][6];         // mapped boxes of the two halves of a median split
+    uint32_t wsum[kThreads / 64];
+    // decision of lane 0
+    int kind;                    // 0 leaf, 1 binned split, 2 median split
+    int axis;
+    uint32_t bin, nleft, node;
+    float lbox[6], rbox[6], lcen[6], rcen[6];
+};
+
+__device__ void write_ref(const Level& L, uint32_t slot, uint32_t ref) {
+    if (slot & kRootSlot) L.mesh_root2[slot & ~kRootSlot] = ref;
+    else L.bnodes[slot >> 1].ref[slot & 1] = ref;
+}
+
+__global__ void __launch_bounds__(kThreads) k_build_level(Level L) {
+    __shared__ LevelShared sh;
+    const BuildTask t = L.in[blockIdx.x];
+    const uint32_t n = t.end - t.begin;
+    const uint32_t tid = threadIdx.x;
+    float ext[3], sc[3];
+    for (int a = 0; a < 3; ++a) {
+        ext[a] = t.cbox[3 + a] - t.cbox[a];
+        sc[a] = ext[a] > 0 ? (float)kBins / ext[a] : 0.0f;
+    }
+    if (n > 1) {
+        for (uint32_t i = tid; i < 3 * kBins; i += kThreads) {
+            (&sh.cnt[0][0])[i] = 0;
+            for (int k = 0; k < 3; ++k) {
+                (&sh.box[0][0][0])[6 * i + k] = kMappedPosInf;
+                (&sh.box[0][0][0])[6 * i + 3 + k] = kMappedNegInf;
+                (&sh.cen[0][0][0])[6 * i + k] = kMappedPosInf;
+                (&sh.cen[0][0][0])[6 * i + 3 + k] = kMappedNegInf;
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
+            const uint32_t p = L.perm_in[i];
+            float c[3], b[6];
+            for (int k = 0; k < 3; ++k) c[k] = L.pcen[3 * (size_t)p + k];
+            for (int k = 0; k < 6; ++k) b[k] = L.pbox[6 * (size_t)p + k];
+            for (int a = 0; a < 3; ++a) {
+                if (!(ext[a] > 0)) continue;
+                const uint32_t k = bin_of(c[a], t.cbox[a], sc[a]);
+                atomicAdd(&sh.cnt[a][k], 1u);
+                for (int j = 0; j < 3; ++j) {
+                    atomicMin(&sh.box[a][k][j], map_f(b[j]));
+                    atomicMax(&sh.box[a][k][3 + j], map_f(b[3 + j]));
+                    atomicMin(&sh.cen[a][k][j], map_f(c[j]));
+                    atomicMax(&sh.cen[a][k][3 + j], map_f(c[j]));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        // the host builder's split() on the same bins
+        int kind = 0, best_axis = -1;
+        uint32_t best_bin = 0;
+        if (n > 1) {
+            float best_cost = INFINITY;
+            for (int axis = 0; axis < 3; ++axis) {
+                if (!(ext[axis] > 0)) continue;
+                float rarea[kBins];
+                uint32_t rcnt[kBins];
+                DBox acc;
+                acc.clear();
+                uint32_t c = 0;
+                for (int k = (int)kBins - 1; k > 0; --k) {
+                    float l[3], h[3];
+                    for (int j = 0; j < 3; ++j) { l[j] = unmap_f(sh.box[axis][k][j]); h[j] = unmap_f(sh.box[axis][k][3 + j]); }
+                    acc.grow(l, h);
+                    c += sh.cnt[axis][k];
+                    rarea[k] = acc.empty() ? 0.0f : acc.area();
+                    rcnt[k] = c;
+                }
+                acc.clear();
+                c = 0;
+                for (uint32_t k = 0; k + 1 < kBins; ++k) {
+                    float l[3], h[3];
+                    for (int j = 0; j < 3; ++j) { l[j] = unmap_f(sh.box[axis][k][j]); h[j] = unmap_f(sh.box[axis][k][3 + j]); }
+                    acc.grow(l, h);
+                    c += sh.cnt[axis][k];
+                    if (c == 0 || rcnt[k + 1] == 0) continue;
+                    const float cost = (float)c * acc.area() + (float)rcnt[k + 1] * rarea[k + 1];
+                    if (cost < best_cost) { best_cost = cost; best_axis = axis; best_bin = k; }
+                }
+            }
+            DBox bounds;
+            for (int j = 0; j < 3; ++j) { bounds.lo[j] = t.box[j]; bounds.hi[j] = t.box[3 + j]; }
+            const float parent = bounds.area();
+            const float leaf_cost = (float)n * parent;
+            const float split_cost = parent + best_cost;
+            if (best_axis < 0) kind = n <= kMaxLeaf ? 0 : 2;
+            else kind = (n <= kMaxLeaf && leaf_cost <= split_cost) ? 0 : 1;
+        }
+        sh.kind = kind;
+        sh.axis = best_axis;
+        sh.bin = best_bin;
+        if (kind == 1) {
+            DBox lb, rb, lc, rc;
+            lb.clear(); rb.clear(); lc.clear(); rc.clear();
+            uint32_t nl = 0;
+            for (uint32_t k = 0; k < kBins; ++k) {
+                float l[3], h[3], cl[3], ch[3];
+                for (int j = 0; j < 3; ++j) {
+                    l[j] = unmap_f(sh.box[best_axis][k][j]); h[j] = unmap_f(sh.box[best_axis][k][3 + j]);
+                    cl[j] = unmap_f(sh.cen[best_axis][k][j]); ch[j] = unmap_f(sh.cen[best_axis][k][3 + j]);
+                }
+                if (k <= best_bin) { lb.grow(l, h); lc.grow(cl, ch); nl += sh.cnt[best_axis][k]; }
+                else { rb.grow(l, h); rc.grow(cl, ch); }
+            }
+            sh.nleft = nl;
+            for (int j = 0; j < 3; ++j) {
+                sh.lbox[j] = lb.lo[j]; sh.lbox[3 + j] = lb.hi[j];
+                sh.rbox[j] = rb.lo[j]; sh.rbox[3 + j] = rb.hi[j];
+                sh.lcen[j] = lc.lo[j]; sh.lcen[3 + j] = lc.hi[j];
+                sh.rcen[j] = rc.lo[j]; sh.rcen[3 + j] = rc.hi[j];
+            }
+        } else if (kind == 2) {
+            sh.nleft = n / 2;
+            for (int h = 0; h < 2; ++h)
+                for (int j = 0; j < 3; ++j) { sh.half[h][j] = kMappedPosInf; sh.half[h][3 + j] = kMappedNegInf; }
+        }
+    }
+    __syncthreads();
+    const int kind = sh.kind;
+    if (kind == 0) {
+        // leaf: triangle records in place, reference to the parent's slot
+        for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
+            const uint32_t p = L.perm_in[i];
+            const float* p0 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p];
+            const float* p1 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 1];
+            const float* p2 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 2];
+            DTri tr{};
+            for (int c = 0; c < 3; ++c) { tr.v0[c] = p0[c]; tr.e1[c] = p1[c] - p0[c]; tr.e2[c] = p2[c] - p0[c]; }
+            tr.prim = p - L.tofs[t.mesh];
+            L.tris[i] = tr;
+        }
+        if (tid == 0) {
+            const uint32_t ref = make_leaf(t.begin, n);
+            if (t.slot & kRootSlot) {
+                // a mesh that is one leaf: a node with the leaf and an empty child, as on the host
+                const uint32_t id = atomicAdd(L.bcount, 1u);
+                DNode nd{};
+                DBox b, none;
+                for (int j = 0; j < 3; ++j) { b.lo[j] = t.box[j]; b.hi[j] = t.box[3 + j]; }
+                none.clear();
+                set_child(nd, 0, b, ref);
+                set_child(nd, 1, none, kEmptyRef);
+                L.bnodes[id] = nd;
+                L.mesh_root2[t.slot & ~kRootSlot] = id;
+            } else {
+                write_ref(L, t.slot, ref);
+            }
+        }
+        return;
+    }
+    const uint32_t nl = sh.nleft;
+    if (kind == 1) {
+        // stable partition of the range by the chosen plane
+        const int axis = sh.axis;
+        const uint32_t bin = sh.bin;
+        const uint32_t lane = tid & 63u, wave = tid >> 6;
+        uint32_t lbase = 0, rbase = 0;
+        for (uint32_t s = t.begin; s < t.end; s += kThreads) {
+            const uint32_t i = s + tid;
+            const bool valid = i < t.end;
+            const uint32_t p = valid ? L.perm_in[i] : 0u;
+            const bool left = valid && bin_of(L.pcen[3 * (size_t)p + axis], t.cbox[axis], sc[axis]) <= bin;
+            const uint64_t mask = __ballot(left);
+            const uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            if (lane == 0) sh.wsum[wave] = (uint32_t)__popcll(mask);
+            __syncthreads();
+            uint32_t woff = 0, total = 0;
+            for (uint32_t w = 0; w < kThreads / 64; ++w) {
+                if (w < wave) woff += sh.wsum[w];
+                total += sh.wsum[w];
+            }
+            if (valid) {
+                const uint32_t lrank = woff + below;
+                if (left) L.perm_out[t.begin + lbase + lrank] = p;
+                else L.perm_out[t.begin + nl + rbase + (tid - lrank)] = p;
+            }
+            const uint32_t chunk = min(kThreads, t.end - s);
+            lbase += total;
+            rbase += chunk - total;
+            __syncthreads();
+        }
+    } else {
+        // all centroids coincide: split the range in the middle, keep the order
+        for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
+            const uint32_t p = L.perm_in[i];
+            L.perm_out[i] = p;
+            const int h = (i - t.begin) < nl ? 0 : 1;
+            for (int j = 0; j < 3; ++j) {
+                atomicMin(&sh.half[h][j], map_f(L.pbox[6 * (size_t)p + j]));
+                atomicMax(&sh.half[h][3 + j], map_f(L.pbox[6 * (size_t)p + 3 + j]));
+            }
+        }
+        __syncthreads();
+        if (tid == 0)
+            for (int j = 0; j < 6; ++j) {
+                sh.lbox[j] = unmap_f(sh.half[0][j]);
+                sh.rbox[j] = unmap_f(sh.half[1][j]);
+                sh.lcen[j] = t.cbox[j];
+                sh.rcen[j] = t.cbox[j];
+            }
+    }
+    if (tid == 0) {
+        const uint32_t id = atomicAdd(L.bcount, 1u);
+        DNode nd{};
+        DBox lb, rb;
+        for (int j = 0; j < 3; ++j) {
+            lb.lo[j] = sh.lbox[j]; lb.hi[j] = sh.lbox[3 + j];
+            rb.lo[j] = sh.rbox[j]; rb.hi[j] = sh.rbox[3 + j];
+        }
+        set_child(nd, 0, lb, kEmptyRef);  // references written by the children
+        set_child(nd, 1, rb, kEmptyRef);
+        L.bnodes[id] = nd;
+        write_ref(L, t.slot, id);
+        const uint32_t o = atomicAdd(L.out_count, 2u);
+        BuildTask c0, c1;
+        c0.begin = t.begin; c0.end = t.begin + nl; c0.slot = 2 * id; c0.mesh = t.mesh;
+        c1.begin = t.begin + nl; c1.end = t.end; c1.slot = 2 * id + 1; c1.mesh = t.mesh;
+        for (int j = 0; j < 6; ++j) {
+            c0.box[j] = sh.lbox[j]; c0.cbox[j] = sh.lcen[j];
+            c1.box[j] = sh.rbox[j]; c1.cbox[j] = sh.rcen[j];
+        }
+        L.out[o] = c0;
+        L.out[o + 1] = c1;
+    }
+}
+
+// ---- collapse to 4-wide nodes ------------------------------------------
+struct Wide {
+    const WideTask* in;
+    uint32_t n_in;
+    WideTask* out;
+    uint32_t* out_count;
+    const DNode* bnodes;
+    DNode4* nodes4;
+    uint32_t* wcount;
+    uint32_t* mesh_root4;
+    uint32_t* bound;
+};
+
+__device__ void child_box(const DNode& n, int c, float lo[3], float hi[3]) {
+    const float* x = c == 0 ? n.c0x : n.c1x;
+    const float* y = c == 0 ? n.c0y : n.c1y;
+    const float* z = c == 0 ? n.c0z : n.c1z;
+    lo[0] = x[0]; hi[0] = x[1]; lo[1] = y[0]; hi[1] = y[1]; lo[2] = z[0]; hi[2] = z[1];
+}
+
+__device__ float wide_area(const float lo[3], const float hi[3]) {
+    float d[3];
+    for (int k = 0; k < 3; ++k) {
+        const float x = hi[k] - lo[k];
+        d[k] = 0.0f < x ? x : 0.0f;  // std::max(0.0f, x)
+    }
+    return d[0] * d[1] + d[0] * d[2] + d[1] * d[2];
+}
+
+__global__ void k_roots4(const uint32_t* mesh_root2, uint32_t n_meshes, WideTask* out, uint32_t* out_count) {
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n_meshes; m += gridDim.x * blockDim.x)
+        out[m] = WideTask{mesh_root2[m], kRootSlot | m, 0u, 0u};
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = n_meshes;
+}
+
+// the host WideBuilder::collapse, one wide node per lane
+__global__ void k_wide_level(Wide W) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < W.n_in; i += gridDim.x * blockDim.x) {
+        const WideTask w = W.in[i];
+        uint32_t ref[4];
+        float lo[4][3], hi[4][3];
+        int nc = 0;
+        auto add_children = [&](uint32_t id) {
+            const DNode nd = W.bnodes[id];
+            for (int c = 0; c < 2; ++c) {
+                if (nd.ref[c] == kEmptyRef) continue;
+                ref[nc] = nd.ref[c];
+                child_box(nd, c, lo[nc], hi[nc]);
+                ++nc;
+            }
+        };
+        add_children(w.bnode);
+        while (nc < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int k = 0; k < nc; ++k)
+                if (!(ref[k] & kLeafBit) && wide_area(lo[k], hi[k]) > ba) { ba = wide_area(lo[k], hi[k]); best = k; }
+            if (best < 0) break;
+            const uint32_t open = ref[best];
+            --nc;
+            ref[best] = ref[nc];
+            for (int k = 0; k < 3; ++k) { lo[best][k] = lo[nc][k]; hi[best][k] = hi[nc][k]; }
+            add_children(open);
+        }
+        const uint32_t me = atomicAdd(W.wcount, 1u);
+        DNode4 nd;
+        quantise_children(lo, hi, nc, nd);
+        const uint32_t acc = w.acc + (nc > 0 ? (uint32_t)nc - 1u : 0u);
+        for (int k = 0; k < nc; ++k) {
+            if (ref[k] & kLeafBit) {
+                nd.ref[k] = ref[k];
+            } else {
+                nd.ref[k] = kEmptyRef;  // written by the child's task
+                W.out[atomicAdd(W.out_count, 1u)] = WideTask{ref[k], me * 4u + (uint32_t)k, acc, 0u};
+            }
+        }
+        W.nodes4[me] = nd;
+        if (w.slot & kRootSlot) W.mesh_root4[w.slot & ~kRootSlot] = me;
+        else W.nodes4[w.slot >> 2].ref[w.slot & 3] = me;
+        atomicMax(W.bound, acc);
+    }
+}
+
+__global__ void k_instance_roots(DInstance* inst, const uint32_t* inst_mesh, const uint32_t* mesh_root4, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        inst[i].blas_root = mesh_root4[inst_mesh[i]];
+}
+
+uint32_t grid_for(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
+
+// Device allocations of the build itself, freed when it ends.
+struct Temp {
+    std::vector<void*> p;
+    ~Temp() {
+        for (void* x : p) hipFree(x);
+    }
+    template <class T>
+    T* get(size_t count) {
+        void* x = nullptr;
+        if (hipMalloc(&x, std::max<size_t>(count * sizeof(T), 64)) != hipSuccess) return nullptr;
+        p.push_back(x);
+        return static_cast<T*>(x);
+    }
+};
+
+}  // namespace
+
+namespace pico {
+
+// The scene arrays live in the context's scene allocations (freed with the scene).
+template <class T>
+static T* scene_alloc(pico_hip_ctx* c, size_t count) {
+    void* x = nullptr;
+    if (hipMalloc(&x, std::max<size_t>(count * sizeof(T), 64)) != hipSuccess) return nullptr;
+    c->scene_allocs.push_back(x);
+    return static_cast<T*>(x);
+}
+
+template <class T>
+static T* scene_upload(pico_hip_ctx* c, const T* src, size_t count, hipError_t& e) {
+    T* d = scene_alloc<T>(c, count);
+    if (!d) { e = hipErrorOutOfMemory; return nullptr; }
+    if (count) e = hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream);
+    return d;
+}
+
+static void drop_scene(pico_hip_ctx* c) {
+    hipStreamSynchronize(c->stream);
+    for (void* p : c->scene_allocs) hipFree(p);
+    c->scene_allocs.clear();
+    c->has_scene = false;
+}
+
+static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_stats* stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::string e = check_scene(d);
+    if (!e.empty()) return fail(c, PICO_ERR_ARG, e);
+    HostScene hs;
+    e = scene_tables(d, hs);
+    if (!e.empty()) return fail(c, PICO_ERR_ARG, e);
+    const uint32_t nm = d.n_meshes, nv = d.mesh_vertex_offset[nm], nt = d.mesh_triangle_offset[nm];
+    const uint32_t ni = d.n_instances;
+    // scene arrays
+    DevScene s{};
+    hipError_t he = hipSuccess;
+    const size_t node_cap = (size_t)nt + nm + std::max<uint32_t>(ni > 1 ? ni - 1 : 1, 1);
+    DNode4* nodes4 = scene_alloc<DNode4>(c, node_cap);
+    DTri* tris = scene_alloc<DTri>(c, nt);
+    uint32_t* gidx = scene_alloc<uint32_t>(c, 3 * (size_t)nt);
+    float* mesh_box = scene_alloc<float>(c, 6 * (size_t)nm);
+    if (!nodes4 || !tris || !gidx || !mesh_box) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    s.nodes = nodes4;
+    s.tris = tris;
+    s.idx = gidx;
+    s.pos = scene_upload(c, d.positions, 3 * (size_t)nv, he);
+    if (he == hipSuccess) s.nrm = scene_upload(c, d.normals, 3 * (size_t)nv, he);
+    if (he == hipSuccess) s.inst = scene_upload(c, hs.instances.data(), hs.instances.size(), he);
+    if (he == hipSuccess) s.xf = scene_upload(c, hs.xforms.data(), hs.xforms.size(), he);
+    if (he == hipSuccess) s.mats = scene_upload(c, hs.materials.data(), hs.materials.size(), he);
+    if (he == hipSuccess) s.lights = scene_upload(c, hs.lights.data(), hs.lights.size(), he);
+    if (he == hipSuccess) s.alias_w = scene_upload(c, hs.alias_weight.data(), hs.alias_weight.size(), he);
+    if (he == hipSuccess) s.alias_i = scene_upload(c, hs.alias_index.data(), hs.alias_index.size(), he);
+    const uint32_t* inst_mesh = he == hipSuccess ? scene_upload(c, hs.instance_mesh.data(), hs.instance_mesh.size(), he) : nullptr;
+    if (he != hipSuccess) return fail(c, PICO_ERR_HIP, std::string("scene upload: ") + hipGetErrorString(he));
+    // build inputs and work arrays
+    Temp tmp;
+    uint32_t* lidx = tmp.get<uint32_t>(3 * (size_t)nt);
+    uint32_t* vofs = tmp.get<uint32_t>(nm + 1);
+    uint32_t* tofs = tmp.get<uint32_t>(nm + 1);
+    float* xform = tmp.get<float>(16 * (size_t)ni);
+    float* pbox = tmp.get<float>(6 * (size_t)nt);
+    float* pcen = tmp.get<float>(3 * (size_t)nt);
+    uint32_t* perm[2] = {tmp.get<uint32_t>(nt), tmp.get<uint32_t>(nt)};
+    BuildTask* tasks[2] = {tmp.get<BuildTask>(std::max<uint32_t>(nt, nm)), tmp.get<BuildTask>(std::max<uint32_t>(nt, nm))};
+    DNode* bnodes = tmp.get<DNode>((size_t)nt + nm);
+    uint32_t* mbox = tmp.get<uint32_t>(6 * (size_t)nm);
+    uint32_t* rbox = tmp.get<uint32_t>(12 * (size_t)nm);
+    uint32_t* roots = tmp.get<uint32_t>(2 * (size_t)nm);
+    WideTask* wtasks[2] = {tmp.get<WideTask>((size_t)nt + nm), tmp.get<WideTask>((size_t)nt + nm)};
+    uint32_t* counters = tmp.get<uint32_t>(16);  // [0] err [1] bnodes [2,3] task counts [4] wide [5] bound [6,7] wide task counts
+    if (!lidx || !vofs || !tofs || !xform || !pbox || !pcen || !perm[0] || !perm[1] || !tasks[0] || !tasks[1] || !bnodes ||
+        !mbox || !rbox || !roots || !wtasks[0] || !wtasks[1] || !counters)
+        return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    PICO_HIP(c, hipMemcpyAsync(lidx, d.indices, 12 * (size_t)nt, hipMemcpyHostToDevice, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(vofs, d.mesh_vertex_offset, 4 * ((size_t)nm + 1), hipMemcpyHostToDevice, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(tofs, d.mesh_triangle_offset, 4 * ((size_t)nm + 1), hipMemcpyHostToDevice, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(xform, d.transforms, 64 * (size_t)ni, hipMemcpyHostToDevice, c->stream));
+    PICO_HIP(c, hipMemsetAsync(counters, 0, 64, c->stream));
+    hipEvent_t e0, e1;
+    PICO_HIP(c, hipEventCreate(&e0));
+    PICO_HIP(c, hipEventCreate(&e1));
+    struct Ev { hipEvent_t a, b; ~Ev() { hipEventDestroy(a); hipEventDestroy(b); } } ev{e0, e1};
+    PICO_HIP(c, hipEventRecord(e0, c->stream));
+    Geo g{s.pos, lidx, gidx, vofs, tofs, nm, nt, nv, pbox, pcen, perm[0], mbox, rbox, counters};
+    PICO_LAUNCH(c, "bvh_geo_clear", k_geo_clear, grid_for(nm), 256, 0, g);
+    PICO_LAUNCH(c, "bvh_geo_tris", k_geo_tris, grid_for(nt), 256, 0, g);
+    PICO_LAUNCH(c, "bvh_geo_verts", k_geo_verts, grid_for(nv), 256, 0, g);
+    uint32_t* mesh_root2 = roots;
+    uint32_t* mesh_root4 = roots + nm;
+    PICO_LAUNCH(c, "bvh_roots", k_roots, grid_for(nm), 256, 0, g, tasks[0], counters + 2, bnodes, counters + 1, mesh_root2,
+                mesh_box);
+    uint32_t h[2] = {0, 0};
+    PICO_HIP(c, hipMemcpyAsync(h, counters, 8, hipMemcpyDeviceToHost, c->stream));
+    uint32_t n_tasks = 0;
+    PICO_HIP(c, hipMemcpyAsync(&n_tasks, counters + 2, 4, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    if (h[0]) return fail(c, PICO_ERR_ARG, "triangle index out of range");
+    // binary levels
+    uint32_t levels = 0, cur = 0;
+    while (n_tasks > 0) {
+        if (++levels > kMaxLevels) return fail(c, PICO_ERR_LIMIT, "BVH build did not converge");
+        uint32_t* next_count = counters + 2 + (1 - cur);
+        PICO_HIP(c, hipMemsetAsync(next_count, 0, 4, c->stream));
+        Level L{tasks[cur], tasks[1 - cur], next_count, perm[cur], perm[1 - cur], pbox, pcen, s.pos, gidx, tofs,
+                bnodes, counters + 1, mesh_root2, tris};
+        PICO_LAUNCH(c, "bvh_build_level", k_build_level, n_tasks, kThreads, 0, L);
+        PICO_HIP(c, hipMemcpyAsync(&n_tasks, next_count, 4, hipMemcpyDeviceToHost, c->stream));
+        PICO_HIP(c, hipStreamSynchronize(c->stream));
+        cur = 1 - cur;
+    }
+    uint32_t n_bin = 0;
+    PICO_HIP(c, hipMemcpyAsync(&n_bin, counters + 1, 4, hipMemcpyDeviceToHost, c->stream));
+    // 4-wide collapse
+    PICO_LAUNCH(c, "bvh_roots4", k_roots4, grid_for(nm), 256, 0, mesh_root2, nm, wtasks[0], counters + 6);
+    uint32_t n_wide_tasks = nm;
+    cur = 0;
+    while (n_wide_tasks > 0) {
+        uint32_t* next_count = counters + 6 + (1 - cur);
+        PICO_HIP(c, hipMemsetAsync(next_count, 0, 4, c->stream));
+        Wide W{wtasks[cur], n_wide_tasks, wtasks[1 - cur], next_count, bnodes, nodes4, counters + 4, mesh_root4,
+               counters + 5};
+        PICO_LAUNCH(c, "bvh_wide_level", k_wide_level, grid_for(n_wide_tasks), 256, 0, W);
+        PICO_HIP(c, hipMemcpyAsync(&n_wide_tasks, next_count, 4, hipMemcpyDeviceToHost, c->stream));
+        PICO_HIP(c, hipStreamSynchronize(c->stream));
+        cur = 1 - cur;
+    }
+    PICO_LAUNCH(c, "bvh_instance_roots", k_instance_roots, grid_for(ni), 256, 0, const_cast<DInstance*>(s.inst), inst_mesh,
+                mesh_root4, ni);
+    PICO_HIP(c, hipEventRecord(e1, c->stream));
+    uint32_t wb[2] = {0, 0};
+    PICO_HIP(c, hipMemcpyAsync(wb, counters + 4, 8, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    float blas_ms = 0.0f;
+    hipEventElapsedTime(&blas_ms, e0, e1);
+    const uint32_t n_blas4 = wb[0], blas_bound = wb[1];
+    if (blas_bound + 1 > 64) return fail(c, PICO_ERR_LIMIT, "BVH too deep for the 64-entry traversal stack");
+    // the TLAS, by the rebuild path, into the node array after the BLAS
+    s.n_lights = (uint32_t)hs.lights.size();
+    std::memcpy(s.sky, hs.sky, sizeof(s.sky));
+    s.microfacet = hs.microfacet;
+    c->dev = s;
+    c->d_mesh_box = mesh_box;
+    c->d_instance_mesh = inst_mesh;
+    c->tlas_base = n_blas4;
+    c->blas_bound = blas_bound;
+    pico_rebuild_stats ts{};
+    int rc = tlas_rebuild(c, ni, xform, &ts);
+    if (rc) return rc;
+    c->n_tris = nt;
+    c->blas_depth = std::max(levels, 2u) - 1;  // levels of internal nodes (the last level holds only leaves)
+    hs.n_instances = ni;
+    c->host = std::move(hs);
+    if (stats) {
+        *stats = pico_build_stats{};
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->bvh_ms = blas_ms + ts.device_ms;
+        stats->binary_nodes = n_bin;
+        stats->blas_nodes4 = n_blas4;
+        stats->blas_bound = blas_bound;
+        stats->stack_bound = ts.stack_bound;
+        stats->levels = c->blas_depth;
+    }
+    return PICO_OK;
+}
+
+int upload_scene_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_stats* stats) {
+    const int rc = build_device(c, d, stats);
+    if (rc) {
+        drop_scene(c);
+        return rc;
+    }
+    c->has_scene = true;
+    ++c->scene_serial;
+    return PICO_OK;
+}
+
+}  // namespace pico

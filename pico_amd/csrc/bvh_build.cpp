@@ -285,7 +285,7 @@ uint32_t build_bvh(const std::vector<float>& box_lo, const std::vector<float>& b
     return root;
 }
 
-std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out) {
+std::string check_scene(const pico_scene_desc& d) {
     if (d.n_meshes == 0 || !d.mesh_vertex_offset || !d.mesh_triangle_offset || !d.positions || !d.normals || !d.indices)
         return "scene has no meshes";
     if (d.n_instances == 0 || !d.transforms || !d.instance_mesh || !d.instance_material || !d.instance_bsdf)
@@ -297,9 +297,78 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
     for (uint32_t m = 0; m < d.n_meshes; ++m)
         if (d.mesh_vertex_offset[m + 1] < d.mesh_vertex_offset[m] || d.mesh_triangle_offset[m + 1] < d.mesh_triangle_offset[m])
             return "mesh offsets must be non-decreasing";
-    const uint32_t nv = d.mesh_vertex_offset[d.n_meshes], nt = d.mesh_triangle_offset[d.n_meshes];
-    if (nt > kPrimMask || d.n_instances > kPrimMask) return "too many triangles or instances";
+    if (d.mesh_triangle_offset[d.n_meshes] > kPrimMask || d.n_instances > kPrimMask) return "too many triangles or instances";
+    return "";
+}
+
+std::string scene_tables(const pico_scene_desc& d, HostScene& out) {
+    // materials
+    for (uint32_t i = 0; i < d.n_materials; ++i) {
+        const pico_material& pm = d.materials[i];
+        DMaterial m{};
+        for (int c = 0; c < 3; ++c) { m.diffuse[c] = pm.diffuse[c]; m.specular[c] = pm.specular[c]; m.emissive[c] = pm.emissive[c]; }
+        const float s = std::sqrt(fdot(pm.specular, pm.specular));
+        m.reflectance = s / (s + std::sqrt(fdot(pm.diffuse, pm.diffuse)));
+        m.roughness = pm.roughness;
+        out.materials.push_back(m);
+    }
+    // instances (the BLAS roots are filled in by the builder)
+    out.n_instances = d.n_instances;
+    for (uint32_t i = 0; i < d.n_instances; ++i) {
+        const uint32_t m = d.instance_mesh[i];
+        if (m >= d.n_meshes) return "instance mesh out of range";
+        if (d.instance_material[i] >= d.n_materials) return "instance material out of range";
+        if (d.instance_bsdf[i] > PICO_BSDF_DELTA) return "unknown BSDF kind";
+        const float* M = &d.transforms[16 * (size_t)i];
+        if (M[3] != 0.0f || M[7] != 0.0f || M[11] != 0.0f || M[15] != 1.0f) return "instance transform must be affine";
+        DInstance di{};
+        if (!affine_inverse_rows(M, di.inv)) return "instance transform is singular";
+        di.blas_root = 0;
+        di.tri_offset = d.mesh_triangle_offset[m];
+        di.material = d.instance_material[i];
+        di.bsdf = d.instance_bsdf[i];
+        out.instances.push_back(di);
+        out.instance_mesh.push_back(m);
+        DXform xf;
+        std::memcpy(xf.m, M, sizeof(xf.m));
+        out.xforms.push_back(xf);
+    }
+    // lights: alias tables over the light meshes' triangles (mesh-local indices)
+    std::set<uint32_t> light_meshes;
+    for (uint32_t l = 0; l < d.n_lights; ++l) {
+        const uint32_t ii = d.light_instances[l];
+        if (ii >= d.n_instances) return "light instance out of range";
+        if (d.instance_bsdf[ii] != PICO_BSDF_LIGHT) return "light instance must use the Light BSDF";
+        const uint32_t m = d.instance_mesh[ii];
+        if (!light_meshes.insert(m).second) return "lights must not share a mesh";
+        const uint32_t t0 = d.mesh_triangle_offset[m], t1 = d.mesh_triangle_offset[m + 1];
+        const uint32_t v0 = d.mesh_vertex_offset[m], nv = d.mesh_vertex_offset[m + 1] - v0;
+        if (t1 == t0) return "light mesh has no triangles";
+        std::vector<float> w;
+        for (uint32_t t = t0; t < t1; ++t) {
+            const uint32_t* li = &d.indices[3 * (size_t)t];
+            if (li[0] >= nv || li[1] >= nv || li[2] >= nv) return "triangle index out of range";
+            w.push_back(light_tri_weight(&d.positions[3 * (size_t)(v0 + li[0])], &d.positions[3 * (size_t)(v0 + li[1])],
+                                         &d.positions[3 * (size_t)(v0 + li[2])]));
+        }
+        std::vector<float> bw;
+        std::vector<uint32_t> ba;
+        alias_table(w, bw, ba);
+        DLight dl{ii, t1 - t0, (uint32_t)out.alias_weight.size(), t0};
+        out.alias_weight.insert(out.alias_weight.end(), bw.begin(), bw.end());
+        out.alias_index.insert(out.alias_index.end(), ba.begin(), ba.end());
+        out.lights.push_back(dl);
+    }
+    std::memcpy(out.sky, d.sky_rgb, sizeof(out.sky));
+    out.microfacet = d.microfacet;
+    return "";
+}
+
+std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out) {
     out = HostScene();
+    std::string e = check_scene(d);
+    if (!e.empty()) return e;
+    const uint32_t nv = d.mesh_vertex_offset[d.n_meshes], nt = d.mesh_triangle_offset[d.n_meshes];
     out.positions.assign(d.positions, d.positions + 3 * (size_t)nv);
     out.normals.assign(d.normals, d.normals + 3 * (size_t)nv);
     out.indices.resize(3 * (size_t)nt);
@@ -340,39 +409,15 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
             out.tris.push_back(tr);
         }
     }
-    // materials
-    for (uint32_t i = 0; i < d.n_materials; ++i) {
-        const pico_material& pm = d.materials[i];
-        DMaterial m{};
-        for (int c = 0; c < 3; ++c) { m.diffuse[c] = pm.diffuse[c]; m.specular[c] = pm.specular[c]; m.emissive[c] = pm.emissive[c]; }
-        const float s = std::sqrt(fdot(pm.specular, pm.specular));
-        m.reflectance = s / (s + std::sqrt(fdot(pm.diffuse, pm.diffuse)));
-        m.roughness = pm.roughness;
-        out.materials.push_back(m);
-    }
-    // instances + TLAS
+    out.blas_nodes2 = (uint32_t)out.nodes.size();
+    e = scene_tables(d, out);
+    if (!e.empty()) return e;
+    // TLAS over the instance world boxes
     std::vector<float> lo, hi;
-    out.n_instances = d.n_instances;
     for (uint32_t i = 0; i < d.n_instances; ++i) {
         const uint32_t m = d.instance_mesh[i];
-        if (m >= d.n_meshes) return "instance mesh out of range";
-        if (d.instance_material[i] >= d.n_materials) return "instance material out of range";
-        if (d.instance_bsdf[i] > PICO_BSDF_DELTA) return "unknown BSDF kind";
-        const float* M = &d.transforms[16 * (size_t)i];
-        if (M[3] != 0.0f || M[7] != 0.0f || M[11] != 0.0f || M[15] != 1.0f) return "instance transform must be affine";
-        DInstance di{};
-        if (!affine_inverse_rows(M, di.inv)) return "instance transform is singular";
-        di.blas_root = mesh_root[m];  // binary root for now; the 4-wide root below
-        di.tri_offset = d.mesh_triangle_offset[m];
-        di.material = d.instance_material[i];
-        di.bsdf = d.instance_bsdf[i];
-        out.instances.push_back(di);
-        out.instance_mesh.push_back(m);
-        DXform xf;
-        std::memcpy(xf.m, M, sizeof(xf.m));
-        out.xforms.push_back(xf);
         Box wb;
-        instance_world_box(M, mesh_box[m].lo, mesh_box[m].hi, wb.lo, wb.hi);
+        instance_world_box(&d.transforms[16 * (size_t)i], mesh_box[m].lo, mesh_box[m].hi, wb.lo, wb.hi);
         lo.insert(lo.end(), wb.lo, wb.lo + 3);
         hi.insert(hi.end(), wb.hi, wb.hi + 3);
     }
@@ -405,31 +450,6 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
         out.tlas_root = collapse_bvh4(out.nodes, out.tlas_root, out.nodes4, tb);
         out.stack_bound = tb + 1 + blas_bound;  // TLAS entries, the instance sentinel, BLAS entries
     }
-    // lights
-    std::set<uint32_t> light_meshes;
-    for (uint32_t l = 0; l < d.n_lights; ++l) {
-        const uint32_t ii = d.light_instances[l];
-        if (ii >= d.n_instances) return "light instance out of range";
-        if (d.instance_bsdf[ii] != PICO_BSDF_LIGHT) return "light instance must use the Light BSDF";
-        const uint32_t m = d.instance_mesh[ii];
-        if (!light_meshes.insert(m).second) return "lights must not share a mesh";
-        const uint32_t t0 = d.mesh_triangle_offset[m], t1 = d.mesh_triangle_offset[m + 1];
-        if (t1 == t0) return "light mesh has no triangles";
-        std::vector<float> w;
-        for (uint32_t t = t0; t < t1; ++t)
-            w.push_back(light_tri_weight(&out.positions[3 * (size_t)out.indices[3 * (size_t)t]],
-                                         &out.positions[3 * (size_t)out.indices[3 * (size_t)t + 1]],
-                                         &out.positions[3 * (size_t)out.indices[3 * (size_t)t + 2]]));
-        std::vector<float> bw;
-        std::vector<uint32_t> ba;
-        alias_table(w, bw, ba);
-        DLight dl{ii, t1 - t0, (uint32_t)out.alias_weight.size(), t0};
-        out.alias_weight.insert(out.alias_weight.end(), bw.begin(), bw.end());
-        out.alias_index.insert(out.alias_index.end(), ba.begin(), ba.end());
-        out.lights.push_back(dl);
-    }
-    std::memcpy(out.sky, d.sky_rgb, sizeof(out.sky));
-    out.microfacet = d.microfacet;
     return "";
 }
 

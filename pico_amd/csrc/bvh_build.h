@@ -28,6 +28,7 @@ struct HostScene {
     uint32_t tlas_root = 0;            // DNode4 index
     uint32_t tlas_base = 0;            // first TLAS DNode4 (every BLAS node comes before it)
     uint32_t blas_bound = 0;           // most stack entries any BLAS traversal can hold
+    uint32_t blas_nodes2 = 0;          // binary BLAS nodes (every mesh)
     uint32_t tlas_depth = 0;
     uint32_t blas_depth = 0;           // deepest BLAS (binary levels)
     uint32_t stack_bound = 0;          // most stack entries any 4-wide traversal can hold
@@ -51,5 +52,11 @@ uint32_t collapse_bvh4(const std::vector<DNode>& n2, uint32_t root2, std::vector
 
 // Validates the description and builds everything; returns an error text or "".
 std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out);
+
+// The parts of build_scene that do not touch the triangles: the structural
+// checks of the description, and the materials, instances (inverses, no BLAS
+// roots), transforms and light tables.  The device builder uses these.
+std::string check_scene(const pico_scene_desc& d);
+std::string scene_tables(const pico_scene_desc& d, HostScene& out);
 
 }  // namespace pico

@@ -47,6 +47,9 @@ struct pico_hip_ctx {
     const float* d_mesh_box = nullptr;
     uint32_t tlas_base = 0;
     uint32_t blas_bound = 0;
+    // reported by pico_hip_scene_info (either builder)
+    uint64_t n_nodes4 = 0, n_tris = 0;
+    uint32_t tlas_depth = 0, blas_depth = 0;
     // named scratch buffers (grown on demand, freed at destroy)
     std::map<std::string, DevBuf> bufs;
     // launch shape of the wavefront and the autotuner's cache
@@ -88,6 +91,13 @@ int cnn_check_size(pico_cnn* n, uint32_t w, uint32_t h);
 
 // Returns a device buffer of at least `bytes` under `name` (contents undefined).
 void* scratch(pico_hip_ctx* c, const std::string& name, size_t bytes);
+
+// Device build of the whole scene, BLAS and TLAS (blas_build.hip); the
+// previous scene is already freed.
+int upload_scene_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_stats* stats);
+
+// Device TLAS rebuild over the context's n instances (tlas.hip).
+int tlas_rebuild(pico_hip_ctx* c, uint32_t n, const float* d_transforms, pico_rebuild_stats* stats);
 
 // Bracket a launch with events when timing is enabled; checks the launch.
 struct LaunchScope {

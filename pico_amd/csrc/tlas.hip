@@ -28,6 +28,7 @@
 
 #include "../../include/pico_hip.h"
 #include "ctx.h"
+#include "dev_atomics.h"
 #include "scene_math.h"
 
 using namespace pico;
@@ -63,22 +64,11 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ void atomic_min_f(uint32_t* a, float v) {
-    // order-preserving map of floats onto unsigned ints
-    const uint32_t u = __float_as_uint(v);
-    atomicMin(a, (u & 0x80000000u) ? ~u : (u | 0x80000000u));
-}
-__device__ __forceinline__ void atomic_max_f(uint32_t* a, float v) {
-    const uint32_t u = __float_as_uint(v);
-    atomicMax(a, (u & 0x80000000u) ? ~u : (u | 0x80000000u));
-}
-__device__ __forceinline__ float unmap_f(uint32_t m) { return __uint_as_float((m & 0x80000000u) ? (m & 0x7fffffffu) : ~m); }
-
 __global__ void k_tlas_init(TlasBuf t) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         t.info[0] = 0;
         t.info[1] = 0;
-        for (int k = 0; k < 3; ++k) { t.info[2 + k] = 0xffffffffu; t.info[5 + k] = 0u; }
+        for (int k = 0; k < 3; ++k) { t.info[2 + k] = kMappedPosInf; t.info[5 + k] = kMappedNegInf; }
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < t.n; i += gridDim.x * blockDim.x) t.visits[i] = 0;
 }
@@ -292,13 +282,11 @@ uint32_t grid_of(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_
 
 }  // namespace
 
-extern "C" {
+namespace pico {
 
-int pico_hip_set_instance_transforms_d(pico_hip_ctx* c, const float* d_transforms, pico_rebuild_stats* stats) {
-    if (!c) return PICO_ERR_ARG;
-    if (!c->has_scene) return fail(c, PICO_ERR_STATE, "no scene uploaded");
-    if (!d_transforms) return fail(c, PICO_ERR_ARG, "NULL transforms");
-    const uint32_t n = c->host.n_instances;
+// Rebuilds the TLAS of the context's scene (n instances) from device
+// transforms; also the last step of the device scene build.
+int tlas_rebuild(pico_hip_ctx* c, uint32_t n, const float* d_transforms, pico_rebuild_stats* stats) {
     TlasBuf t{};
     t.n = n;
     t.base = c->tlas_base;
@@ -370,7 +358,20 @@ int pico_hip_set_instance_transforms_d(pico_hip_ctx* c, const float* d_transform
     if (!h[2]) return fail(c, PICO_ERR_LIMIT, "rebuilt TLAS too deep for the 64-entry traversal stack (scene unchanged)");
     c->dev.tlas_root = c->tlas_base;
     c->stack_need = h[4];
+    c->n_nodes4 = (uint64_t)c->tlas_base + h[3];
+    c->tlas_depth = h[1] + 1;  // levels of internal nodes, as the host builder counts them
     return PICO_OK;
+}
+
+}  // namespace pico
+
+extern "C" {
+
+int pico_hip_set_instance_transforms_d(pico_hip_ctx* c, const float* d_transforms, pico_rebuild_stats* stats) {
+    if (!c) return PICO_ERR_ARG;
+    if (!c->has_scene) return fail(c, PICO_ERR_STATE, "no scene uploaded");
+    if (!d_transforms) return fail(c, PICO_ERR_ARG, "NULL transforms");
+    return tlas_rebuild(c, c->host.n_instances, d_transforms, stats);
 }
 
 int pico_hip_set_instance_transforms(pico_hip_ctx* c, const float* transforms, pico_rebuild_stats* stats) {
