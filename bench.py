@@ -383,6 +383,25 @@ def tlas_leg(ctx, pa, scene, cam, W, H, depth, d_pix, d_cnt, d_var, frames=8, sp
     return out
 
 
+def build_leg(ctx, pa, scene):
+    """Scene build on the device against the host build: the bench scene
+    (16 meshes, 2k-20k triangles each) and one mesh of 2M triangles; the
+    device build's BVH kernels are event-timed, the host's is wall time.
+    Both builders make the same trees (node counts checked here too)."""
+    out = {}
+    big = pa.Scene.instanced(5, 1, 4, 2000000, 2000000, 100.0, pa.MICROFACET_GGX)
+    for name, sc in (("bench_scene", scene), ("mesh_2M", big)):
+        h = ctx.upload_scene(sc)
+        d = [ctx.upload_scene(sc, device_build=True) for _ in range(3)][-1]
+        out[name] = {"triangles": sc.n_triangles[0], "host_bvh_ms": round(h.bvh_ms, 1),
+                     "device_bvh_ms": round(d.bvh_ms, 2), "device_upload_total_ms": round(d.total_ms, 1),
+                     "speedup": round(h.bvh_ms / d.bvh_ms, 1), "levels": d.levels,
+                     "same_tree": (h.binary_nodes, h.blas_nodes4, h.blas_bound, h.levels) ==
+                                  (d.binary_nodes, d.blas_nodes4, d.blas_bound, d.levels)}
+    ctx.upload_scene(scene)
+    return out
+
+
 def _ptr_array(ctx, ptr):
     """A non-owning device array at a raw pointer (the views' own buffers)."""
     import pico_amd as pa
@@ -407,7 +426,7 @@ def main():
     ap.add_argument("--oracle-spp", type=int, default=4096, help="samples of the FP64 quality oracle")
     ap.add_argument("--no-quality", action="store_true", help="skip the relMSE / ms-per-frame leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views leg")
-    ap.add_argument("--no-tlas", action="store_true", help="skip the device TLAS rebuild leg")
+    ap.add_argument("--no-tlas", action="store_true", help="skip the device TLAS rebuild and scene build legs")
     ap.add_argument("--weights", default=None, help="trained denoiser parameters (.npy; default: the shipped ones)")
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the streaming-copy calibration")
     args = ap.parse_args()
@@ -503,10 +522,12 @@ def main():
         batched = batched_leg(ctx, pa, args.depth, args.weights or pa.weights_path())
         log(f"[bench] batched views: {batched}")
 
-    tlas = None
+    tlas = build = None
     if not args.no_tlas:
         tlas = tlas_leg(ctx, pa, scene, cam, W, H, args.depth, d_pix, d_cnt, d_var)
         log(f"[bench] device TLAS rebuild: {tlas}")
+        build = build_leg(ctx, pa, scene)
+        log(f"[bench] device scene build: {build}")
 
     cpu = None
     if not args.no_cpu:
@@ -523,7 +544,7 @@ def main():
                       "unique_triangles": uniq, "instanced_triangles": inst_tris,
                       "rays_per_step": rays // args.steps, "traced_rays_per_step": traced // args.steps},
            "roofline": roof, "cpu_baseline": cpu, "quality": quality,
-           "batched_views": batched, "tlas_rebuild": tlas}
+           "batched_views": batched, "tlas_rebuild": tlas, "scene_build": build}
     if cpu:
         out["gpu_over_cpu"] = round(mrays / cpu["value"], 1)
     print(json.dumps(out), flush=True)

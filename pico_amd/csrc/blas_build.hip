@@ -40,6 +40,8 @@ constexpr uint32_t kMaxLeaf = 4;
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kRootSlot = 0x80000000u;  // slot | mesh: the reference goes to the mesh's root entry
 constexpr uint32_t kMaxLevels = 1u << 16;
+constexpr uint32_t kLarge = 8192;  // nodes of more triangles are binned and partitioned by many workgroups
+constexpr uint32_t kChunk = 2048;  // triangles per workgroup of such a node
 
 struct BuildTask {
     uint32_t begin, end;  // range of the triangle permutation
@@ -72,6 +74,10 @@ struct Geo {
     uint32_t* err;
 };
 
+struct Bins;
+struct LargeTask;
+struct Chunk;
+
 struct Level {
     const BuildTask* in;
     BuildTask* out;
@@ -87,6 +93,15 @@ struct Level {
     uint32_t* bcount;
     uint32_t* mesh_root2;
     DTri* tris;
+    // nodes of more than kLarge triangles
+    LargeTask* large;
+    uint32_t* large_count;
+    Chunk* chunks;
+    uint32_t* chunk_count;
+    Bins* gbins;
+    uint32_t* chunk_left;
+    uint32_t* next_large;   // children of more than kLarge triangles, and their chunks
+    uint32_t* next_chunks;
 };
 
 // host Box arithmetic (bvh_build.cpp), replayed on the device
@@ -136,8 +151,7 @@ __global__ void k_geo_clear(Geo g) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *g.err = 0;
 }
 
-// Per triangle: global indices, box, centroid, identity permutation, mesh
-// box union and centroid bounds.
+// Per triangle: global indices, box, centroid, identity permutation.
 __global__ void k_geo_tris(Geo g) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < g.n_tris; t += gridDim.x * blockDim.x) {
         const uint32_t m = find_mesh(g.tofs, g.n_meshes, t);
@@ -157,31 +171,96 @@ __global__ void k_geo_tris(Geo g) {
                 hi[k] = hi[k] < p[k] ? p[k] : hi[k];
             }
         }
-        uint32_t* rb = g.rbox + 12 * (size_t)m;
         for (int k = 0; k < 3; ++k) {
-            const float cen = 0.5f * (lo[k] + hi[k]);
             g.pbox[6 * (size_t)t + k] = lo[k];
             g.pbox[6 * (size_t)t + 3 + k] = hi[k];
-            g.pcen[3 * (size_t)t + k] = cen;
-            atomic_min_f(rb + k, lo[k]);
-            atomic_max_f(rb + 3 + k, hi[k]);
-            atomic_min_f(rb + 6 + k, cen);
-            atomic_max_f(rb + 9 + k, cen);
+            g.pcen[3 * (size_t)t + k] = 0.5f * (lo[k] + hi[k]);
         }
         g.perm[t] = t;
     }
 }
 
-// Per vertex: the mesh's vertex bounds (the host's mesh box for the TLAS).
-__global__ void k_geo_verts(Geo g) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n_verts; v += gridDim.x * blockDim.x) {
-        const uint32_t m = find_mesh(g.vofs, g.n_meshes, v);
-        const float* p = g.pos + 3 * (size_t)v;
-        for (int k = 0; k < 3; ++k) {
-            atomic_min_f(g.mbox + 6 * (size_t)m + k, p[k]);
-            atomic_max_f(g.mbox + 6 * (size_t)m + 3 + k, p[k]);
+// Per-mesh bounds of D values over the items of each mesh (items grouped by
+// mesh through the offsets `ofs`): every lane folds a run of kRun items in
+// order, a wave whose lanes all end in one mesh combines them by shuffles,
+// and the results reach device memory through mapped-float atomics, so the
+// atomics per mesh are about one per 2048 items.
+constexpr uint32_t kRun = 32;
+
+template <int D, class Load, class Flush>
+__device__ void run_bounds(uint32_t n, const uint32_t* ofs, uint32_t n_meshes, Load load, Flush flush) {
+    const uint32_t r0 = (blockIdx.x * blockDim.x + threadIdx.x) * kRun;
+    const bool active = r0 < n;
+    const uint32_t r1 = active ? min(n, r0 + kRun) : r0;
+    float lo[D], hi[D];
+    for (int k = 0; k < D; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+    uint32_t m = active ? find_mesh(ofs, n_meshes, r0) : 0u;
+    uint32_t next = active ? ofs[m + 1] : 0u;
+    bool any = false;
+    for (uint32_t i = r0; i < r1; ++i) {
+        while (i >= next) {  // the run crosses into a later mesh (empty meshes skipped)
+            if (any) flush(m, lo, hi);
+            for (int k = 0; k < D; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+            any = false;
+            ++m;
+            next = ofs[m + 1];
         }
+        float vl[D], vh[D];
+        load(i, vl, vh);
+        for (int k = 0; k < D; ++k) {
+            lo[k] = fminf(lo[k], vl[k]);
+            hi[k] = fmaxf(hi[k], vh[k]);
+        }
+        any = true;
     }
+    const uint32_t m0 = __shfl(m, 0);
+    if (__all(!any || m == m0)) {
+        for (int k = 0; k < D; ++k)
+            for (int o = 32; o > 0; o >>= 1) {
+                lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
+                hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
+            }
+        if ((threadIdx.x & 63u) == 0 && __any(any)) flush(m0, lo, hi);
+    } else if (any) {
+        flush(m, lo, hi);
+    }
+}
+
+// Per mesh: the union of its triangle boxes and the bounds of its centroids.
+__global__ void k_tri_bounds(Geo g) {
+    run_bounds<6>(
+        g.n_tris, g.tofs, g.n_meshes,
+        [&](uint32_t t, float* vl, float* vh) {
+            for (int k = 0; k < 3; ++k) {
+                vl[k] = g.pbox[6 * (size_t)t + k];
+                vh[k] = g.pbox[6 * (size_t)t + 3 + k];
+                vl[3 + k] = vh[3 + k] = g.pcen[3 * (size_t)t + k];
+            }
+        },
+        [&](uint32_t m, const float* lo, const float* hi) {
+            uint32_t* rb = g.rbox + 12 * (size_t)m;
+            for (int k = 0; k < 3; ++k) {
+                atomic_min_f(rb + k, lo[k]);
+                atomic_max_f(rb + 3 + k, hi[k]);
+                atomic_min_f(rb + 6 + k, lo[3 + k]);
+                atomic_max_f(rb + 9 + k, hi[3 + k]);
+            }
+        });
+}
+
+// Per mesh: the bounds of its vertices (the host's mesh box for the TLAS).
+__global__ void k_vert_bounds(Geo g) {
+    run_bounds<3>(
+        g.n_verts, g.vofs, g.n_meshes,
+        [&](uint32_t v, float* vl, float* vh) {
+            for (int k = 0; k < 3; ++k) vl[k] = vh[k] = g.pos[3 * (size_t)v + k];
+        },
+        [&](uint32_t m, const float* lo, const float* hi) {
+            for (int k = 0; k < 3; ++k) {
+                atomic_min_f(g.mbox + 6 * (size_t)m + k, lo[k]);
+                atomic_max_f(g.mbox + 6 * (size_t)m + 3 + k, hi[k]);
+            }
+        });
 }
 
 __device__ void set_child(DNode& nd, int c, const DBox& b, uint32_t ref) {
@@ -197,7 +276,7 @@ __device__ void set_child(DNode& nd, int c, const DBox& b, uint32_t ref) {
 // One root task per mesh with triangles; a mesh without triangles gets a
 // node with two empty children, as on the host.
 __global__ void k_roots(Geo g, BuildTask* out, uint32_t* out_count, DNode* bnodes, uint32_t* bcount, uint32_t* mesh_root2,
-                        float* mesh_box) {
+                        float* mesh_box, uint32_t* n_large, uint32_t* n_chunks) {
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < g.n_meshes; m += gridDim.x * blockDim.x) {
         for (int k = 0; k < 6; ++k) mesh_box[6 * (size_t)m + k] = unmap_f(g.mbox[6 * (size_t)m + k]);
         const uint32_t b = g.tofs[m], e = g.tofs[m + 1];
@@ -222,6 +301,10 @@ __global__ void k_roots(Geo g, BuildTask* out, uint32_t* out_count, DNode* bnode
             t.cbox[k] = unmap_f(g.rbox[12 * (size_t)m + 6 + k]);
         }
         out[atomicAdd(out_count, 1u)] = t;
+        if (e - b > kLarge) {
+            atomicAdd(n_large, 1u);
+            atomicAdd(n_chunks, (e - b + kChunk - 1) / kChunk);
+        }
     }
 }
 
@@ -230,17 +313,36 @@ __device__ __forceinline__ uint32_t bin_of(float c, float lo, float sc) {
     return (uint32_t)(x < (float)(kBins - 1) ? x : (float)(kBins - 1));  // std::min<float>(bins - 1, x)
 }
 
-struct LevelShared {
+// Bins of one node: counts and mapped triangle / centroid boxes per bin and axis.
+struct Bins {
     uint32_t cnt[3][kBins];
-    uint32_t box[3][kBins][6];   // mapped triangle box per bin
-    uint32_t cen[3][kBins][6];   // mapped centroid bounds per bin
-    uint32_t half[2][6];         // mapped boxes of the two halves of a median split
-    uint32_t wsum[kThreads / 64];
-    // decision of lane 0
-    int kind;                    // 0 leaf, 1 binned split, 2 median split
+    uint32_t box[3][kBins][6];
+    uint32_t cen[3][kBins][6];
+};
+constexpr uint32_t kBinCells = 3 * kBins;
+
+// What a node does: 0 leaf, 1 split at a bin plane, 2 split in the middle of
+// its range (all centroids coincide); children's boxes and centroid bounds.
+struct Decision {
+    int kind;
     int axis;
-    uint32_t bin, nleft, node;
+    uint32_t bin, nleft;
     float lbox[6], rbox[6], lcen[6], rcen[6];
+};
+
+// A node of more than kLarge triangles: binned, decided and partitioned by
+// one workgroup per chunk of kChunk triangles.
+struct LargeTask {
+    uint32_t task;                   // index in the level's task list
+    uint32_t first_chunk, n_chunks;
+    uint32_t pad;
+    Decision dec;
+};
+
+struct Chunk {
+    uint32_t large;                  // LargeTask slot
+    uint32_t begin, end;             // range of the permutation
+    uint32_t pad;
 };
 
 __device__ void write_ref(const Level& L, uint32_t slot, uint32_t ref) {
@@ -248,223 +350,388 @@ __device__ void write_ref(const Level& L, uint32_t slot, uint32_t ref) {
     else L.bnodes[slot >> 1].ref[slot & 1] = ref;
 }
 
-__global__ void __launch_bounds__(kThreads) k_build_level(Level L) {
-    __shared__ LevelShared sh;
-    const BuildTask t = L.in[blockIdx.x];
-    const uint32_t n = t.end - t.begin;
-    const uint32_t tid = threadIdx.x;
-    float ext[3], sc[3];
+__device__ void task_scales(const BuildTask& t, float ext[3], float sc[3]) {
     for (int a = 0; a < 3; ++a) {
         ext[a] = t.cbox[3 + a] - t.cbox[a];
         sc[a] = ext[a] > 0 ? (float)kBins / ext[a] : 0.0f;
     }
-    if (n > 1) {
-        for (uint32_t i = tid; i < 3 * kBins; i += kThreads) {
-            (&sh.cnt[0][0])[i] = 0;
-            for (int k = 0; k < 3; ++k) {
-                (&sh.box[0][0][0])[6 * i + k] = kMappedPosInf;
-                (&sh.box[0][0][0])[6 * i + 3 + k] = kMappedNegInf;
-                (&sh.cen[0][0][0])[6 * i + k] = kMappedPosInf;
-                (&sh.cen[0][0][0])[6 * i + 3 + k] = kMappedNegInf;
-            }
+}
+
+__device__ void bins_clear(Bins* b, uint32_t tid, uint32_t nthreads) {
+    for (uint32_t i = tid; i < kBinCells; i += nthreads) {
+        (&b->cnt[0][0])[i] = 0;
+        for (int k = 0; k < 3; ++k) {
+            (&b->box[0][0][0])[6 * i + k] = kMappedPosInf;
+            (&b->box[0][0][0])[6 * i + 3 + k] = kMappedNegInf;
+            (&b->cen[0][0][0])[6 * i + k] = kMappedPosInf;
+            (&b->cen[0][0][0])[6 * i + 3 + k] = kMappedNegInf;
         }
-        __syncthreads();
-        for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
-            const uint32_t p = L.perm_in[i];
-            float c[3], b[6];
-            for (int k = 0; k < 3; ++k) c[k] = L.pcen[3 * (size_t)p + k];
-            for (int k = 0; k < 6; ++k) b[k] = L.pbox[6 * (size_t)p + k];
-            for (int a = 0; a < 3; ++a) {
-                if (!(ext[a] > 0)) continue;
-                const uint32_t k = bin_of(c[a], t.cbox[a], sc[a]);
-                atomicAdd(&sh.cnt[a][k], 1u);
-                for (int j = 0; j < 3; ++j) {
-                    atomicMin(&sh.box[a][k][j], map_f(b[j]));
-                    atomicMax(&sh.box[a][k][3 + j], map_f(b[3 + j]));
-                    atomicMin(&sh.cen[a][k][j], map_f(c[j]));
-                    atomicMax(&sh.cen[a][k][3 + j], map_f(c[j]));
-                }
-            }
-        }
-        __syncthreads();
     }
-    if (tid == 0) {
-        // the host builder's split() on the same bins
-        int kind = 0, best_axis = -1;
-        uint32_t best_bin = 0;
-        if (n > 1) {
-            float best_cost = INFINITY;
-            for (int axis = 0; axis < 3; ++axis) {
-                if (!(ext[axis] > 0)) continue;
-                float rarea[kBins];
-                uint32_t rcnt[kBins];
-                DBox acc;
-                acc.clear();
-                uint32_t c = 0;
-                for (int k = (int)kBins - 1; k > 0; --k) {
-                    float l[3], h[3];
-                    for (int j = 0; j < 3; ++j) { l[j] = unmap_f(sh.box[axis][k][j]); h[j] = unmap_f(sh.box[axis][k][3 + j]); }
-                    acc.grow(l, h);
-                    c += sh.cnt[axis][k];
-                    rarea[k] = acc.empty() ? 0.0f : acc.area();
-                    rcnt[k] = c;
-                }
-                acc.clear();
-                c = 0;
-                for (uint32_t k = 0; k + 1 < kBins; ++k) {
-                    float l[3], h[3];
-                    for (int j = 0; j < 3; ++j) { l[j] = unmap_f(sh.box[axis][k][j]); h[j] = unmap_f(sh.box[axis][k][3 + j]); }
-                    acc.grow(l, h);
-                    c += sh.cnt[axis][k];
-                    if (c == 0 || rcnt[k + 1] == 0) continue;
-                    const float cost = (float)c * acc.area() + (float)rcnt[k + 1] * rarea[k + 1];
-                    if (cost < best_cost) { best_cost = cost; best_axis = axis; best_bin = k; }
-                }
-            }
-            DBox bounds;
-            for (int j = 0; j < 3; ++j) { bounds.lo[j] = t.box[j]; bounds.hi[j] = t.box[3 + j]; }
-            const float parent = bounds.area();
-            const float leaf_cost = (float)n * parent;
-            const float split_cost = parent + best_cost;
-            if (best_axis < 0) kind = n <= kMaxLeaf ? 0 : 2;
-            else kind = (n <= kMaxLeaf && leaf_cost <= split_cost) ? 0 : 1;
-        }
-        sh.kind = kind;
-        sh.axis = best_axis;
-        sh.bin = best_bin;
-        if (kind == 1) {
-            DBox lb, rb, lc, rc;
-            lb.clear(); rb.clear(); lc.clear(); rc.clear();
-            uint32_t nl = 0;
-            for (uint32_t k = 0; k < kBins; ++k) {
-                float l[3], h[3], cl[3], ch[3];
-                for (int j = 0; j < 3; ++j) {
-                    l[j] = unmap_f(sh.box[best_axis][k][j]); h[j] = unmap_f(sh.box[best_axis][k][3 + j]);
-                    cl[j] = unmap_f(sh.cen[best_axis][k][j]); ch[j] = unmap_f(sh.cen[best_axis][k][3 + j]);
-                }
-                if (k <= best_bin) { lb.grow(l, h); lc.grow(cl, ch); nl += sh.cnt[best_axis][k]; }
-                else { rb.grow(l, h); rc.grow(cl, ch); }
-            }
-            sh.nleft = nl;
+}
+
+// Bins the triangles perm_in[begin, end) of task t into b (LDS).
+__device__ void bin_range(Bins* b, const Level& L, const BuildTask& t, uint32_t begin, uint32_t end, const float* ext,
+                          const float* sc, uint32_t tid) {
+    for (uint32_t i = begin + tid; i < end; i += kThreads) {
+        const uint32_t p = L.perm_in[i];
+        float c[3], bx[6];
+        for (int k = 0; k < 3; ++k) c[k] = L.pcen[3 * (size_t)p + k];
+        for (int k = 0; k < 6; ++k) bx[k] = L.pbox[6 * (size_t)p + k];
+        for (int a = 0; a < 3; ++a) {
+            if (!(ext[a] > 0)) continue;
+            const uint32_t k = bin_of(c[a], t.cbox[a], sc[a]);
+            atomicAdd(&b->cnt[a][k], 1u);
             for (int j = 0; j < 3; ++j) {
-                sh.lbox[j] = lb.lo[j]; sh.lbox[3 + j] = lb.hi[j];
-                sh.rbox[j] = rb.lo[j]; sh.rbox[3 + j] = rb.hi[j];
-                sh.lcen[j] = lc.lo[j]; sh.lcen[3 + j] = lc.hi[j];
-                sh.rcen[j] = rc.lo[j]; sh.rcen[3 + j] = rc.hi[j];
+                atomicMin(&b->box[a][k][j], map_f(bx[j]));
+                atomicMax(&b->box[a][k][3 + j], map_f(bx[3 + j]));
+                atomicMin(&b->cen[a][k][j], map_f(c[j]));
+                atomicMax(&b->cen[a][k][3 + j], map_f(c[j]));
             }
-        } else if (kind == 2) {
-            sh.nleft = n / 2;
-            for (int h = 0; h < 2; ++h)
-                for (int j = 0; j < 3; ++j) { sh.half[h][j] = kMappedPosInf; sh.half[h][3 + j] = kMappedNegInf; }
+        }
+    }
+}
+
+// The host builder's split() on the bins (one lane).
+__device__ void decide(const Bins* b, const BuildTask& t, const float* ext, Decision& d) {
+    const uint32_t n = t.end - t.begin;
+    int kind = 0, best_axis = -1;
+    uint32_t best_bin = 0;
+    if (n > 1) {
+        float best_cost = INFINITY;
+        for (int axis = 0; axis < 3; ++axis) {
+            if (!(ext[axis] > 0)) continue;
+            float rarea[kBins];
+            uint32_t rcnt[kBins];
+            DBox acc;
+            acc.clear();
+            uint32_t c = 0;
+            for (int k = (int)kBins - 1; k > 0; --k) {
+                float l[3], h[3];
+                for (int j = 0; j < 3; ++j) { l[j] = unmap_f(b->box[axis][k][j]); h[j] = unmap_f(b->box[axis][k][3 + j]); }
+                acc.grow(l, h);
+                c += b->cnt[axis][k];
+                rarea[k] = acc.empty() ? 0.0f : acc.area();
+                rcnt[k] = c;
+            }
+            acc.clear();
+            c = 0;
+            for (uint32_t k = 0; k + 1 < kBins; ++k) {
+                float l[3], h[3];
+                for (int j = 0; j < 3; ++j) { l[j] = unmap_f(b->box[axis][k][j]); h[j] = unmap_f(b->box[axis][k][3 + j]); }
+                acc.grow(l, h);
+                c += b->cnt[axis][k];
+                if (c == 0 || rcnt[k + 1] == 0) continue;
+                const float cost = (float)c * acc.area() + (float)rcnt[k + 1] * rarea[k + 1];
+                if (cost < best_cost) { best_cost = cost; best_axis = axis; best_bin = k; }
+            }
+        }
+        DBox bounds;
+        for (int j = 0; j < 3; ++j) { bounds.lo[j] = t.box[j]; bounds.hi[j] = t.box[3 + j]; }
+        const float parent = bounds.area();
+        const float leaf_cost = (float)n * parent;
+        const float split_cost = parent + best_cost;
+        if (best_axis < 0) kind = n <= kMaxLeaf ? 0 : 2;
+        else kind = (n <= kMaxLeaf && leaf_cost <= split_cost) ? 0 : 1;
+    }
+    d.kind = kind;
+    d.axis = best_axis;
+    d.bin = best_bin;
+    d.nleft = 0;
+    if (kind == 1) {
+        DBox lb, rb, lc, rc;
+        lb.clear(); rb.clear(); lc.clear(); rc.clear();
+        uint32_t nl = 0;
+        for (uint32_t k = 0; k < kBins; ++k) {
+            float l[3], h[3], cl[3], ch[3];
+            for (int j = 0; j < 3; ++j) {
+                l[j] = unmap_f(b->box[best_axis][k][j]); h[j] = unmap_f(b->box[best_axis][k][3 + j]);
+                cl[j] = unmap_f(b->cen[best_axis][k][j]); ch[j] = unmap_f(b->cen[best_axis][k][3 + j]);
+            }
+            if (k <= best_bin) { lb.grow(l, h); lc.grow(cl, ch); nl += b->cnt[best_axis][k]; }
+            else { rb.grow(l, h); rc.grow(cl, ch); }
+        }
+        d.nleft = nl;
+        for (int j = 0; j < 3; ++j) {
+            d.lbox[j] = lb.lo[j]; d.lbox[3 + j] = lb.hi[j];
+            d.rbox[j] = rb.lo[j]; d.rbox[3 + j] = rb.hi[j];
+            d.lcen[j] = lc.lo[j]; d.lcen[3 + j] = lc.hi[j];
+            d.rcen[j] = rc.lo[j]; d.rcen[3 + j] = rc.hi[j];
+        }
+    } else if (kind == 2) {
+        d.nleft = n / 2;  // boxes of the halves are reduced by the caller
+        for (int j = 0; j < 6; ++j) { d.lcen[j] = t.cbox[j]; d.rcen[j] = t.cbox[j]; }
+    }
+}
+
+// Box reduction of the two halves of a middle split, by the whole workgroup;
+// `half` is LDS of 12 words.  Ends with the boxes in d.
+__device__ void median_boxes(const Level& L, const BuildTask& t, Decision& d, uint32_t* half, uint32_t tid) {
+    if (tid == 0)
+        for (int h = 0; h < 2; ++h)
+            for (int j = 0; j < 3; ++j) { half[6 * h + j] = kMappedPosInf; half[6 * h + 3 + j] = kMappedNegInf; }
+    __syncthreads();
+    for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
+        const uint32_t p = L.perm_in[i];
+        const int h = (i - t.begin) < d.nleft ? 0 : 1;
+        for (int j = 0; j < 3; ++j) {
+            atomicMin(&half[6 * h + j], map_f(L.pbox[6 * (size_t)p + j]));
+            atomicMax(&half[6 * h + 3 + j], map_f(L.pbox[6 * (size_t)p + 3 + j]));
         }
     }
     __syncthreads();
-    const int kind = sh.kind;
-    if (kind == 0) {
-        // leaf: triangle records in place, reference to the parent's slot
-        for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
-            const uint32_t p = L.perm_in[i];
-            const float* p0 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p];
-            const float* p1 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 1];
-            const float* p2 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 2];
-            DTri tr{};
-            for (int c = 0; c < 3; ++c) { tr.v0[c] = p0[c]; tr.e1[c] = p1[c] - p0[c]; tr.e2[c] = p2[c] - p0[c]; }
-            tr.prim = p - L.tofs[t.mesh];
-            L.tris[i] = tr;
+    if (tid == 0)
+        for (int j = 0; j < 6; ++j) {
+            d.lbox[j] = unmap_f(half[j]);
+            d.rbox[j] = unmap_f(half[6 + j]);
         }
-        if (tid == 0) {
-            const uint32_t ref = make_leaf(t.begin, n);
-            if (t.slot & kRootSlot) {
-                // a mesh that is one leaf: a node with the leaf and an empty child, as on the host
-                const uint32_t id = atomicAdd(L.bcount, 1u);
-                DNode nd{};
-                DBox b, none;
-                for (int j = 0; j < 3; ++j) { b.lo[j] = t.box[j]; b.hi[j] = t.box[3 + j]; }
-                none.clear();
-                set_child(nd, 0, b, ref);
-                set_child(nd, 1, none, kEmptyRef);
-                L.bnodes[id] = nd;
-                L.mesh_root2[t.slot & ~kRootSlot] = id;
-            } else {
-                write_ref(L, t.slot, ref);
-            }
-        }
-        return;
+}
+
+__device__ void count_large_child(const Level& L, uint32_t n) {
+    if (n > kLarge) {
+        atomicAdd(L.next_large, 1u);
+        atomicAdd(L.next_chunks, (n + kChunk - 1) / kChunk);
     }
-    const uint32_t nl = sh.nleft;
-    if (kind == 1) {
-        // stable partition of the range by the chosen plane
-        const int axis = sh.axis;
-        const uint32_t bin = sh.bin;
-        const uint32_t lane = tid & 63u, wave = tid >> 6;
-        uint32_t lbase = 0, rbase = 0;
-        for (uint32_t s = t.begin; s < t.end; s += kThreads) {
-            const uint32_t i = s + tid;
-            const bool valid = i < t.end;
-            const uint32_t p = valid ? L.perm_in[i] : 0u;
-            const bool left = valid && bin_of(L.pcen[3 * (size_t)p + axis], t.cbox[axis], sc[axis]) <= bin;
-            const uint64_t mask = __ballot(left);
-            const uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-            if (lane == 0) sh.wsum[wave] = (uint32_t)__popcll(mask);
-            __syncthreads();
-            uint32_t woff = 0, total = 0;
-            for (uint32_t w = 0; w < kThreads / 64; ++w) {
-                if (w < wave) woff += sh.wsum[w];
-                total += sh.wsum[w];
-            }
-            if (valid) {
-                const uint32_t lrank = woff + below;
-                if (left) L.perm_out[t.begin + lbase + lrank] = p;
-                else L.perm_out[t.begin + nl + rbase + (tid - lrank)] = p;
-            }
-            const uint32_t chunk = min(kThreads, t.end - s);
-            lbase += total;
-            rbase += chunk - total;
-            __syncthreads();
-        }
-    } else {
-        // all centroids coincide: split the range in the middle, keep the order
-        for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
-            const uint32_t p = L.perm_in[i];
-            L.perm_out[i] = p;
-            const int h = (i - t.begin) < nl ? 0 : 1;
-            for (int j = 0; j < 3; ++j) {
-                atomicMin(&sh.half[h][j], map_f(L.pbox[6 * (size_t)p + j]));
-                atomicMax(&sh.half[h][3 + j], map_f(L.pbox[6 * (size_t)p + 3 + j]));
-            }
-        }
+}
+
+// One lane: the node of a split, its reference in the parent and the two child tasks.
+__device__ void emit_split(const Level& L, const BuildTask& t, const Decision& d) {
+    const uint32_t id = atomicAdd(L.bcount, 1u);
+    DNode nd{};
+    DBox lb, rb;
+    for (int j = 0; j < 3; ++j) {
+        lb.lo[j] = d.lbox[j]; lb.hi[j] = d.lbox[3 + j];
+        rb.lo[j] = d.rbox[j]; rb.hi[j] = d.rbox[3 + j];
+    }
+    set_child(nd, 0, lb, kEmptyRef);  // references written by the children
+    set_child(nd, 1, rb, kEmptyRef);
+    L.bnodes[id] = nd;
+    write_ref(L, t.slot, id);
+    const uint32_t o = atomicAdd(L.out_count, 2u);
+    BuildTask c0, c1;
+    c0.begin = t.begin; c0.end = t.begin + d.nleft; c0.slot = 2 * id; c0.mesh = t.mesh;
+    c1.begin = t.begin + d.nleft; c1.end = t.end; c1.slot = 2 * id + 1; c1.mesh = t.mesh;
+    for (int j = 0; j < 6; ++j) {
+        c0.box[j] = d.lbox[j]; c0.cbox[j] = d.lcen[j];
+        c1.box[j] = d.rbox[j]; c1.cbox[j] = d.rcen[j];
+    }
+    L.out[o] = c0;
+    L.out[o + 1] = c1;
+    count_large_child(L, c0.end - c0.begin);
+    count_large_child(L, c1.end - c1.begin);
+}
+
+__device__ __forceinline__ bool goes_left(const Level& L, const BuildTask& t, const Decision& d, const float* sc, uint32_t i,
+                                          uint32_t p) {
+    if (d.kind == 2) return i - t.begin < d.nleft;
+    return bin_of(L.pcen[3 * (size_t)p + d.axis], t.cbox[d.axis], sc[d.axis]) <= d.bin;
+}
+
+// Stable partition of perm_in[begin, end): the left ones go to perm_out
+// from `left_at` on, the right ones from `right_at` on, each in range order.
+// Block-uniform (barriers inside); `wsum` is LDS of kThreads / 64 words.
+__device__ void partition_range(const Level& L, const BuildTask& t, const Decision& d, const float* sc, uint32_t begin,
+                                uint32_t end, uint32_t left_at, uint32_t right_at, uint32_t* wsum, uint32_t tid) {
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t s = begin; s < end; s += kThreads) {
+        const uint32_t i = s + tid;
+        const bool valid = i < end;
+        const uint32_t p = valid ? L.perm_in[i] : 0u;
+        const bool left = valid && goes_left(L, t, d, sc, i, p);
+        const uint64_t mask = __ballot(left);
+        const uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = (uint32_t)__popcll(mask);
         __syncthreads();
-        if (tid == 0)
-            for (int j = 0; j < 6; ++j) {
-                sh.lbox[j] = unmap_f(sh.half[0][j]);
-                sh.rbox[j] = unmap_f(sh.half[1][j]);
-                sh.lcen[j] = t.cbox[j];
-                sh.rcen[j] = t.cbox[j];
-            }
+        uint32_t woff = 0, total = 0;
+        for (uint32_t w = 0; w < kThreads / 64; ++w) {
+            if (w < wave) woff += wsum[w];
+            total += wsum[w];
+        }
+        if (valid) {
+            const uint32_t lrank = woff + below;
+            if (left) L.perm_out[left_at + lrank] = p;
+            else L.perm_out[right_at + (tid - lrank)] = p;
+        }
+        const uint32_t chunk = min(kThreads, end - s);
+        left_at += total;
+        right_at += chunk - total;
+        __syncthreads();
+    }
+}
+
+// Leaf: triangle records in place and the reference in the parent's slot.
+__device__ void emit_leaf(const Level& L, const BuildTask& t, uint32_t tid) {
+    const uint32_t n = t.end - t.begin;
+    for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
+        const uint32_t p = L.perm_in[i];
+        const float* p0 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p];
+        const float* p1 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 1];
+        const float* p2 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 2];
+        DTri tr{};
+        for (int c = 0; c < 3; ++c) { tr.v0[c] = p0[c]; tr.e1[c] = p1[c] - p0[c]; tr.e2[c] = p2[c] - p0[c]; }
+        tr.prim = p - L.tofs[t.mesh];
+        L.tris[i] = tr;
     }
     if (tid == 0) {
-        const uint32_t id = atomicAdd(L.bcount, 1u);
-        DNode nd{};
-        DBox lb, rb;
-        for (int j = 0; j < 3; ++j) {
-            lb.lo[j] = sh.lbox[j]; lb.hi[j] = sh.lbox[3 + j];
-            rb.lo[j] = sh.rbox[j]; rb.hi[j] = sh.rbox[3 + j];
+        const uint32_t ref = make_leaf(t.begin, n);
+        if (t.slot & kRootSlot) {
+            // a mesh that is one leaf: a node with the leaf and an empty child, as on the host
+            const uint32_t id = atomicAdd(L.bcount, 1u);
+            DNode nd{};
+            DBox b, none;
+            for (int j = 0; j < 3; ++j) { b.lo[j] = t.box[j]; b.hi[j] = t.box[3 + j]; }
+            none.clear();
+            set_child(nd, 0, b, ref);
+            set_child(nd, 1, none, kEmptyRef);
+            L.bnodes[id] = nd;
+            L.mesh_root2[t.slot & ~kRootSlot] = id;
+        } else {
+            write_ref(L, t.slot, ref);
         }
-        set_child(nd, 0, lb, kEmptyRef);  // references written by the children
-        set_child(nd, 1, rb, kEmptyRef);
-        L.bnodes[id] = nd;
-        write_ref(L, t.slot, id);
-        const uint32_t o = atomicAdd(L.out_count, 2u);
-        BuildTask c0, c1;
-        c0.begin = t.begin; c0.end = t.begin + nl; c0.slot = 2 * id; c0.mesh = t.mesh;
-        c1.begin = t.begin + nl; c1.end = t.end; c1.slot = 2 * id + 1; c1.mesh = t.mesh;
-        for (int j = 0; j < 6; ++j) {
-            c0.box[j] = sh.lbox[j]; c0.cbox[j] = sh.lcen[j];
-            c1.box[j] = sh.rbox[j]; c1.cbox[j] = sh.rcen[j];
-        }
-        L.out[o] = c0;
-        L.out[o + 1] = c1;
     }
+}
+
+// One workgroup per node of the level.  Nodes of more than kLarge
+// triangles are only registered here (their chunks and cleared global bins)
+// and handled by the k_*_large kernels.
+__global__ void __launch_bounds__(kThreads) k_build_level(Level L) {
+    __shared__ Bins bins;
+    __shared__ Decision dec;
+    __shared__ uint32_t half[12];
+    __shared__ uint32_t wsum[kThreads / 64];
+    __shared__ uint32_t reg[2];
+    const BuildTask t = L.in[blockIdx.x];
+    const uint32_t n = t.end - t.begin;
+    const uint32_t tid = threadIdx.x;
+    if (n > kLarge) {
+        const uint32_t nch = (n + kChunk - 1) / kChunk;
+        if (tid == 0) {
+            reg[0] = atomicAdd(L.large_count, 1u);
+            reg[1] = atomicAdd(L.chunk_count, nch);
+            LargeTask lt{};
+            lt.task = blockIdx.x;
+            lt.first_chunk = reg[1];
+            lt.n_chunks = nch;
+            L.large[reg[0]] = lt;
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < nch; k += kThreads)
+            L.chunks[reg[1] + k] = Chunk{reg[0], t.begin + k * kChunk, min(t.end, t.begin + (k + 1) * kChunk), 0u};
+        bins_clear(L.gbins + reg[0], tid, kThreads);
+        return;
+    }
+    float ext[3], sc[3];
+    task_scales(t, ext, sc);
+    if (n > 1) {
+        bins_clear(&bins, tid, kThreads);
+        __syncthreads();
+        bin_range(&bins, L, t, t.begin, t.end, ext, sc, tid);
+        __syncthreads();
+    }
+    if (tid == 0) decide(&bins, t, ext, dec);
+    __syncthreads();
+    if (dec.kind == 0) {
+        emit_leaf(L, t, tid);
+        return;
+    }
+    if (dec.kind == 2) median_boxes(L, t, dec, half, tid);
+    partition_range(L, t, dec, sc, t.begin, t.end, t.begin, t.begin + dec.nleft, wsum, tid);
+    if (tid == 0) emit_split(L, t, dec);
+}
+
+// Large nodes, step 1: one workgroup bins a chunk in LDS and merges the
+// bins into the node's global bins.
+__global__ void __launch_bounds__(kThreads) k_bin_large(Level L) {
+    __shared__ Bins bins;
+    const Chunk ch = L.chunks[blockIdx.x];
+    const BuildTask t = L.in[L.large[ch.large].task];
+    const uint32_t tid = threadIdx.x;
+    float ext[3], sc[3];
+    task_scales(t, ext, sc);
+    bins_clear(&bins, tid, kThreads);
+    __syncthreads();
+    bin_range(&bins, L, t, ch.begin, ch.end, ext, sc, tid);
+    __syncthreads();
+    Bins* g = L.gbins + ch.large;
+    for (uint32_t i = tid; i < kBinCells; i += kThreads) {
+        const uint32_t c = (&bins.cnt[0][0])[i];
+        if (c == 0) continue;
+        atomicAdd(&g->cnt[0][0] + i, c);
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&g->box[0][0][0] + 6 * i + k, (&bins.box[0][0][0])[6 * i + k]);
+            atomicMax(&g->box[0][0][0] + 6 * i + 3 + k, (&bins.box[0][0][0])[6 * i + 3 + k]);
+            atomicMin(&g->cen[0][0][0] + 6 * i + k, (&bins.cen[0][0][0])[6 * i + k]);
+            atomicMax(&g->cen[0][0][0] + 6 * i + 3 + k, (&bins.cen[0][0][0])[6 * i + 3 + k]);
+        }
+    }
+}
+
+// Step 2: one workgroup per large node takes the decision (a large node is
+// never a leaf) and emits the node and its children.
+__global__ void __launch_bounds__(kThreads) k_split_large(Level L) {
+    __shared__ Decision dec;
+    __shared__ uint32_t half[12];
+    LargeTask* lt = L.large + blockIdx.x;
+    const BuildTask t = L.in[lt->task];
+    const uint32_t tid = threadIdx.x;
+    float ext[3], sc[3];
+    task_scales(t, ext, sc);
+    if (tid == 0) decide(L.gbins + blockIdx.x, t, ext, dec);
+    __syncthreads();
+    if (dec.kind == 2) median_boxes(L, t, dec, half, tid);
+    __syncthreads();
+    if (tid == 0) {
+        lt->dec = dec;
+        emit_split(L, t, dec);
+    }
+}
+
+// Step 3: left count of every chunk.
+__global__ void __launch_bounds__(kThreads) k_count_large(Level L) {
+    __shared__ uint32_t wsum[kThreads / 64];
+    const Chunk ch = L.chunks[blockIdx.x];
+    const LargeTask& lt = L.large[ch.large];
+    const BuildTask t = L.in[lt.task];
+    const Decision d = lt.dec;
+    const uint32_t tid = threadIdx.x;
+    float ext[3], sc[3];
+    task_scales(t, ext, sc);
+    uint32_t cnt = 0;
+    for (uint32_t i = ch.begin + tid; i < ch.end; i += kThreads) cnt += goes_left(L, t, d, sc, i, L.perm_in[i]) ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if ((tid & 63u) == 0) wsum[tid >> 6] = cnt;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < kThreads / 64; ++w) s += wsum[w];
+        L.chunk_left[blockIdx.x] = s;
+    }
+}
+
+// Step 4: stable scatter of every chunk after the earlier chunks of its node.
+__global__ void __launch_bounds__(kThreads) k_scatter_large(Level L) {
+    __shared__ uint32_t wsum[kThreads / 64];
+    __shared__ uint32_t before[2];
+    const Chunk ch = L.chunks[blockIdx.x];
+    const LargeTask& lt = L.large[ch.large];
+    const BuildTask t = L.in[lt.task];
+    const Decision d = lt.dec;
+    const uint32_t tid = threadIdx.x;
+    float ext[3], sc[3];
+    task_scales(t, ext, sc);
+    uint32_t s = 0;
+    for (uint32_t k = lt.first_chunk + tid; k < blockIdx.x; k += kThreads) s += L.chunk_left[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((tid & 63u) == 0) wsum[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t a = 0;
+        for (uint32_t w = 0; w < kThreads / 64; ++w) a += wsum[w];
+        before[0] = a;                          // left ones of the earlier chunks
+        before[1] = (ch.begin - t.begin) - a;   // right ones of the earlier chunks
+    }
+    __syncthreads();
+    const uint32_t lb = before[0], rb = before[1];
+    partition_range(L, t, d, sc, ch.begin, ch.end, t.begin + lb, t.begin + d.nleft + rb, wsum, tid);
 }
 
 // ---- collapse to 4-wide nodes ------------------------------------------
@@ -556,6 +823,8 @@ __global__ void k_instance_roots(DInstance* inst, const uint32_t* inst_mesh, con
 }
 
 uint32_t grid_for(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
+// one lane per run of kRun items, every item covered
+uint32_t runs_grid(size_t n) { return (uint32_t)std::max<size_t>(1, (n + 256 * (size_t)kRun - 1) / (256 * (size_t)kRun)); }
 
 // Device allocations of the build itself, freed when it ends.
 struct Temp {
@@ -646,9 +915,16 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     uint32_t* rbox = tmp.get<uint32_t>(12 * (size_t)nm);
     uint32_t* roots = tmp.get<uint32_t>(2 * (size_t)nm);
     WideTask* wtasks[2] = {tmp.get<WideTask>((size_t)nt + nm), tmp.get<WideTask>((size_t)nt + nm)};
-    uint32_t* counters = tmp.get<uint32_t>(16);  // [0] err [1] bnodes [2,3] task counts [4] wide [5] bound [6,7] wide task counts
+    // [0] err [1] binary nodes [2,3] task counts [4] wide nodes [5] stack bound [6,7] wide task counts
+    // [8] large nodes registered [9] their chunks [10] large children [11] their chunks
+    uint32_t* counters = tmp.get<uint32_t>(16);
+    const size_t large_cap = nt / kLarge + 2, chunk_cap = nt / kChunk + large_cap;
+    LargeTask* large = tmp.get<LargeTask>(large_cap);
+    Bins* gbins = tmp.get<Bins>(large_cap);
+    Chunk* chunks = tmp.get<Chunk>(chunk_cap);
+    uint32_t* chunk_left = tmp.get<uint32_t>(chunk_cap);
     if (!lidx || !vofs || !tofs || !xform || !pbox || !pcen || !perm[0] || !perm[1] || !tasks[0] || !tasks[1] || !bnodes ||
-        !mbox || !rbox || !roots || !wtasks[0] || !wtasks[1] || !counters)
+        !mbox || !rbox || !roots || !wtasks[0] || !wtasks[1] || !counters || !large || !gbins || !chunks || !chunk_left)
         return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     PICO_HIP(c, hipMemcpyAsync(lidx, d.indices, 12 * (size_t)nt, hipMemcpyHostToDevice, c->stream));
     PICO_HIP(c, hipMemcpyAsync(vofs, d.mesh_vertex_offset, 4 * ((size_t)nm + 1), hipMemcpyHostToDevice, c->stream));
@@ -663,28 +939,40 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     Geo g{s.pos, lidx, gidx, vofs, tofs, nm, nt, nv, pbox, pcen, perm[0], mbox, rbox, counters};
     PICO_LAUNCH(c, "bvh_geo_clear", k_geo_clear, grid_for(nm), 256, 0, g);
     PICO_LAUNCH(c, "bvh_geo_tris", k_geo_tris, grid_for(nt), 256, 0, g);
-    PICO_LAUNCH(c, "bvh_geo_verts", k_geo_verts, grid_for(nv), 256, 0, g);
+    PICO_LAUNCH(c, "bvh_tri_bounds", k_tri_bounds, runs_grid(nt), 256, 0, g);
+    PICO_LAUNCH(c, "bvh_vert_bounds", k_vert_bounds, runs_grid(nv), 256, 0, g);
     uint32_t* mesh_root2 = roots;
     uint32_t* mesh_root4 = roots + nm;
     PICO_LAUNCH(c, "bvh_roots", k_roots, grid_for(nm), 256, 0, g, tasks[0], counters + 2, bnodes, counters + 1, mesh_root2,
-                mesh_box);
-    uint32_t h[2] = {0, 0};
-    PICO_HIP(c, hipMemcpyAsync(h, counters, 8, hipMemcpyDeviceToHost, c->stream));
-    uint32_t n_tasks = 0;
-    PICO_HIP(c, hipMemcpyAsync(&n_tasks, counters + 2, 4, hipMemcpyDeviceToHost, c->stream));
+                mesh_box, counters + 10, counters + 11);
+    uint32_t h[12] = {};
+    PICO_HIP(c, hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
     if (h[0]) return fail(c, PICO_ERR_ARG, "triangle index out of range");
+    uint32_t n_tasks = h[2], n_large = h[10], n_chunks = h[11];
     // binary levels
     uint32_t levels = 0, cur = 0;
     while (n_tasks > 0) {
         if (++levels > kMaxLevels) return fail(c, PICO_ERR_LIMIT, "BVH build did not converge");
+        if (n_large > large_cap || n_chunks > chunk_cap) return fail(c, PICO_ERR_LIMIT, "BVH build: work lists overflow");
         uint32_t* next_count = counters + 2 + (1 - cur);
         PICO_HIP(c, hipMemsetAsync(next_count, 0, 4, c->stream));
+        PICO_HIP(c, hipMemsetAsync(counters + 8, 0, 16, c->stream));
         Level L{tasks[cur], tasks[1 - cur], next_count, perm[cur], perm[1 - cur], pbox, pcen, s.pos, gidx, tofs,
-                bnodes, counters + 1, mesh_root2, tris};
+                bnodes, counters + 1, mesh_root2, tris, large, counters + 8, chunks, counters + 9, gbins, chunk_left,
+                counters + 10, counters + 11};
         PICO_LAUNCH(c, "bvh_build_level", k_build_level, n_tasks, kThreads, 0, L);
+        if (n_large) {
+            PICO_LAUNCH(c, "bvh_bin_large", k_bin_large, n_chunks, kThreads, 0, L);
+            PICO_LAUNCH(c, "bvh_split_large", k_split_large, n_large, kThreads, 0, L);
+            PICO_LAUNCH(c, "bvh_count_large", k_count_large, n_chunks, kThreads, 0, L);
+            PICO_LAUNCH(c, "bvh_scatter_large", k_scatter_large, n_chunks, kThreads, 0, L);
+        }
         PICO_HIP(c, hipMemcpyAsync(&n_tasks, next_count, 4, hipMemcpyDeviceToHost, c->stream));
+        PICO_HIP(c, hipMemcpyAsync(h + 10, counters + 10, 8, hipMemcpyDeviceToHost, c->stream));
         PICO_HIP(c, hipStreamSynchronize(c->stream));
+        n_large = h[10];
+        n_chunks = h[11];
         cur = 1 - cur;
     }
     uint32_t n_bin = 0;

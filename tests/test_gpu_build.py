@@ -164,16 +164,22 @@ def test_empty_mesh_and_single_leaf_meshes(ctx):
     assert np.array_equal(dev[1], bi) and np.array_equal(dev[2], bp)
 
 
-def test_degenerate_centroids(ctx):
-    """Forty copies of one triangle: every centroid coincides, both builders
-    split by position in the range; the hit is one of the copies."""
-    p, t = [[-1, 0, -1], [1, 0, -1], [0, 0, 1]], [[0, 1, 2]] * 40
+@pytest.mark.parametrize("copies", [40, 10000])
+def test_degenerate_centroids(ctx, copies):
+    """Copies of one triangle: every centroid coincides, both builders split
+    by position in the range (10000 copies take the many-workgroup path for
+    large nodes); the trees have the same shape and the hit is one of the
+    copies."""
+    p, t = [[-1, 0, -1], [1, 0, -1], [0, 0, 1]], [[0, 1, 2]] * copies
     desc = _Desc([(p, t)], [(_translate(0, 0, 0), 0)])
     o = np.array([[0.0, 2.0, -0.2]], np.float32)
     d = np.array([[0.0, -1.0, 0.0]], np.float32)
-    ctx.upload_scene(desc, device_build=True)
+    sh = ctx.upload_scene(desc)
+    sd = ctx.upload_scene(desc, device_build=True)
+    assert (sd.binary_nodes, sd.blas_nodes4, sd.blas_bound, sd.levels) == \
+        (sh.binary_nodes, sh.blas_nodes4, sh.blas_bound, sh.levels)
     tt, inst, prim, _ = ctx.closest_hit(o, d)
-    assert inst[0] == 0 and prim[0] < 40 and abs(tt[0] - 2.0) < 1e-6
+    assert inst[0] == 0 and prim[0] < copies and abs(tt[0] - 2.0) < 1e-6
 
 
 def test_bad_index_rejected(ctx, cornell):
