@@ -743,7 +743,9 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     if (!spill) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     const uint32_t grid_shade = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_s, 16));
     const uint32_t grid_lin = (uint32_t)c->num_cus * 16;
-    const bool per_view_times = n_stats > 1 || hooks;
+    // per-view completion events: several views, a follow-up per view, or a
+    // group of a shortest-first batch (its times count from the batch's epoch)
+    const bool per_view_times = n_stats > 1 || hooks || (epoch && n_stats > 0);
     std::vector<hipEvent_t> view_render(nv, nullptr), view_done(nv, nullptr);
     bool first = true;
     bool cancelled = false;
