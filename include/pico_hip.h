@@ -19,6 +19,13 @@
  *   pico_hip_tonemap        Util::reinhard_tone_mapping (Source/Util/ToneMappers.cpp:12-44)
  *   pico_hip_closest_hit    UpperLevelBVH::get_closest_intersection
  *                           (Source/Core/UpperLevelBVH.cpp:12-20) for a batch of rays
+ *   pico_hip_upload_scene_ex  the same scene set-up with the BVH builds on the GPU
+ *                           (Source/Core/BVH.cpp:73-197, Source/Core/LowerLevelMeshBVH.cpp:48-77)
+ *   pico_hip_set_instance_transforms  UpperLevelBVH::add_lower_level_bvh + build, per frame
+ *                           (Source/Core/UpperLevelBVH.cpp:22-57)
+ *   pico_scene_load_json    Scene::Scene(ThreadPool&, path) and its section handlers
+ *                           (Source/Core/Scene.cpp:36-84, 297-720)
+ *   pico_write_hdr          the file half of Scene::render_scene_to_file (Source/Core/Scene.cpp:236-246)
  *
  * Buffers passed as `float*` RGB/vec3 images use the reference layout: packed
  * 3 x f32 per pixel (glm::vec3, 12 bytes, no padding), pixel (x, y) at y*W+x.
@@ -429,6 +436,23 @@ int  pico_scene_instanced(uint32_t seed, uint32_t n_mesh_types, uint32_t n_insta
  * jittered Cornell box, a small instanced cluster, or a 100^3 instanced field
  * like the benchmark scene seen from a random outside viewpoint. */
 int  pico_scene_training(uint32_t seed, uint32_t microfacet, pico_scene** out);
+/* A scene file in the reference's JSON format (sections GLOBALS, MESH,
+ * MATERIALS, INSTANCE, CAMERA), read as Scene::Scene(ThreadPool&, path) and
+ * its section handlers read it (Source/Core/Scene.cpp:36-84, 297-720):
+ * meshes from Wavefront .obj files (triangulated, flat normals where the file
+ * has none, equal corners joined, the file's first mesh), Constant materials,
+ * instances T * R(quaternion) * S with the Diffuse, Specular, Dielectric,
+ * Light and Delta BSRDFs (Beckmann_All distribution), named cameras.  Entries
+ * of a section are taken in the sorted order of their names.  Image
+ * materials, the Transparent and Fresnel BSRDFs, the built-in Sphere and
+ * Cube, sky boxes and the sun light are refused.  On error PICO_ERR_ARG and
+ * the reason (file, line, entry) in `err` (err_len bytes, may be NULL). */
+int  pico_scene_load_json(const char* path, pico_scene** out, char* err, size_t err_len);
+/* The named cameras of a loaded scene, in name order (none for procedural
+ * scenes); pico_scene_default_camera gives the first one with its own aspect
+ * replaced by the argument. */
+int  pico_scene_camera_count(const pico_scene* scene, uint32_t* count);
+int  pico_scene_camera_get(const pico_scene* scene, uint32_t index, char* name, size_t name_len, pico_camera* out);
 int  pico_scene_desc_get(const pico_scene* scene, pico_scene_desc* out);
 /* A camera that frames the scene as its generator intends. */
 int  pico_scene_default_camera(const pico_scene* scene, float aspect, pico_camera* out);

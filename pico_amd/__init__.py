@@ -174,6 +174,9 @@ def _declare(lib):
         "pico_scene_instanced": (i32, [u32, u32, u32, u32, u32, f32, u32, P(vp)]),
         "pico_scene_training": (i32, [u32, u32, P(vp)]),
         "pico_scene_desc_get": (i32, [vp, P(SceneDesc)]),
+        "pico_scene_load_json": (i32, [C.c_char_p, P(vp), C.c_char_p, sz]),
+        "pico_scene_camera_count": (i32, [vp, P(u32)]),
+        "pico_scene_camera_get": (i32, [vp, u32, C.c_char_p, sz, P(Camera)]),
         "pico_scene_default_camera": (i32, [vp, f32, P(Camera)]),
         "pico_scene_destroy": (None, [vp]),
     }
@@ -250,6 +253,28 @@ class Scene:
         if rc:
             raise PicoError(rc, "pico_scene_training")
         return Scene(h.value)
+
+    @staticmethod
+    def load_json(path):
+        """A scene file in the reference's JSON format (meshes from .obj files)."""
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().pico_scene_load_json(os.fsencode(path), C.byref(h), err, 1024)
+        if rc:
+            raise PicoError(rc, err.value.decode(errors="replace"))
+        return Scene(h.value)
+
+    def cameras(self):
+        """The named cameras of a loaded scene, {name: Camera}, in name order."""
+        n = C.c_uint32()
+        lib().pico_scene_camera_count(self._h, C.byref(n))
+        out = {}
+        for i in range(n.value):
+            name = C.create_string_buffer(512)
+            cam = Camera()
+            lib().pico_scene_camera_get(self._h, i, name, 512, C.byref(cam))
+            out[name.value.decode()] = cam
+        return out
 
     def camera(self, aspect=1.0):
         cam = Camera()

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/pico_hip.h"
+#include "scene_store.h"
 
 namespace {
 
@@ -234,60 +235,41 @@ static pico_material metal_rough(F3 albedo, float metal, float rough, F3 emissiv
 
 }  // namespace
 
-struct pico_scene {
-    std::vector<uint32_t> vofs{0}, tofs{0};
-    std::vector<float> pos, nrm, xf;
-    std::vector<uint32_t> idx, imesh, imat, ibsdf, lights;
-    std::vector<pico_material> mats;
-    float sky[3] = {0, 0, 0};
-    uint32_t microfacet = PICO_MICROFACET_GGX;
-    pico_camera cam{};
-
-    uint32_t add_mesh(const Mesh& m) {
-        for (auto& p : m.pos) pos.insert(pos.end(), {p.x, p.y, p.z});
-        for (auto& n : m.nrm) nrm.insert(nrm.end(), {n.x, n.y, n.z});
-        idx.insert(idx.end(), m.idx.begin(), m.idx.end());
-        vofs.push_back(vofs.back() + (uint32_t)m.pos.size());
-        tofs.push_back(tofs.back() + (uint32_t)m.idx.size() / 3);
-        return (uint32_t)vofs.size() - 2;
-    }
-    uint32_t add_material(const pico_material& m) { mats.push_back(m); return (uint32_t)mats.size() - 1; }
-    uint32_t add_instance(uint32_t mesh, const Mat& M, uint32_t mat, uint32_t bsdf) {
-        xf.insert(xf.end(), M.m, M.m + 16);
-        imesh.push_back(mesh); imat.push_back(mat); ibsdf.push_back(bsdf);
-        const uint32_t id = (uint32_t)imesh.size() - 1;
-        if (bsdf == PICO_BSDF_LIGHT) lights.push_back(id);
-        return id;
-    }
-};
+static uint32_t add_mesh(pico_scene* s, const Mesh& m) {
+    static_assert(sizeof(F3) == 12, "packed float triples");
+    return s->add_mesh(&m.pos[0].x, &m.nrm[0].x, m.pos.size(), m.idx.data(), m.idx.size());
+}
+static uint32_t add_instance(pico_scene* s, uint32_t mesh, const Mat& M, uint32_t mat, uint32_t bsdf) {
+    return s->add_instance(mesh, M.m, mat, bsdf);
+}
 
 static void cornell_into(pico_scene* s, Rng& rng, bool jitter) {
-    const uint32_t wall = s->add_mesh(quad(1.0f));
-    const uint32_t box = s->add_mesh(cube());
-    const uint32_t lightm = s->add_mesh(quad(-1.0f));
+    const uint32_t wall = add_mesh(s, quad(1.0f));
+    const uint32_t box = add_mesh(s, cube());
+    const uint32_t lightm = add_mesh(s, quad(-1.0f));
     auto rand_mat = [&]() {
         F3 a = {rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f)};
         return s->add_material(metal_rough(a, 0.0f, rng.uniform(0.1f, 0.9f), {0, 0, 0}));
     };
     // walls x=+-1, y=+-1, back z=+1 with inward normals; open front
     const Mat S = scale({2.0f, 1.0f, 2.0f});
-    s->add_instance(wall, mul(translate({0, -1, 0}), S), rand_mat(), PICO_BSDF_DIELECTRIC);                          // floor
-    s->add_instance(wall, mul(translate({0, 1, 0}), mul(rot_axis({1, 0, 0}, 180.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);  // ceiling
-    s->add_instance(wall, mul(translate({-1, 0, 0}), mul(rot_axis({0, 0, 1}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC); // left
-    s->add_instance(wall, mul(translate({1, 0, 0}), mul(rot_axis({0, 0, 1}, 90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);   // right
-    s->add_instance(wall, mul(translate({0, 0, 1}), mul(rot_axis({1, 0, 0}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);  // back
+    add_instance(s, wall, mul(translate({0, -1, 0}), S), rand_mat(), PICO_BSDF_DIELECTRIC);                          // floor
+    add_instance(s, wall, mul(translate({0, 1, 0}), mul(rot_axis({1, 0, 0}, 180.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);  // ceiling
+    add_instance(s, wall, mul(translate({-1, 0, 0}), mul(rot_axis({0, 0, 1}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC); // left
+    add_instance(s, wall, mul(translate({1, 0, 0}), mul(rot_axis({0, 0, 1}, 90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);   // right
+    add_instance(s, wall, mul(translate({0, 0, 1}), mul(rot_axis({1, 0, 0}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);  // back
     // two boxes rotated +-18 degrees about y
     const float jx = jitter ? rng.uniform(-0.15f, 0.15f) : 0.0f, jz = jitter ? rng.uniform(-0.15f, 0.15f) : 0.0f;
     const float r1 = jitter ? rng.uniform(-40.0f, 40.0f) : 18.0f, r2 = jitter ? rng.uniform(-40.0f, 40.0f) : -18.0f;
-    s->add_instance(box, mul(translate({-0.35f + jx, -0.698f, 0.3f + jz}), mul(rot_axis({0, 1, 0}, r1), scale({0.6f, 0.6f, 0.6f}))),
+    add_instance(s, box, mul(translate({-0.35f + jx, -0.698f, 0.3f + jz}), mul(rot_axis({0, 1, 0}, r1), scale({0.6f, 0.6f, 0.6f}))),
                     rand_mat(), PICO_BSDF_DIELECTRIC);
-    s->add_instance(box, mul(translate({0.35f - jx, -0.398f, -0.2f - jz}), mul(rot_axis({0, 1, 0}, r2), scale({0.6f, 1.2f, 0.6f}))),
+    add_instance(s, box, mul(translate({0.35f - jx, -0.398f, -0.2f - jz}), mul(rot_axis({0, 1, 0}, r2), scale({0.6f, 1.2f, 0.6f}))),
                     rand_mat(), PICO_BSDF_DIELECTRIC);
     // quad light 0.5 x 0.5 at y = 0.99 facing down
     const float le = jitter ? rng.uniform(5.0f, 20.0f) : 10.0f;
     const uint32_t lm = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {le, le, le}));
     const float lx = jitter ? rng.uniform(-0.4f, 0.4f) : 0.0f, lz = jitter ? rng.uniform(-0.4f, 0.4f) : 0.0f;
-    s->add_instance(lightm, mul(translate({lx, 0.99f, lz}), scale({0.5f, 1.0f, 0.5f})), lm, PICO_BSDF_LIGHT);
+    add_instance(s, lightm, mul(translate({lx, 0.99f, lz}), scale({0.5f, 1.0f, 0.5f})), lm, PICO_BSDF_LIGHT);
     s->cam = pico_camera{{0.0f, 0.0f, -1.0f}, {0.0f, 0.0f, 1.0f}, {0.0f, 1.0f, 0.0f}, 1.0f, 10000.0f};
 }
 
@@ -305,7 +287,7 @@ static void instanced_into(pico_scene* s, Rng& rng, uint32_t n_types, uint32_t n
             uint32_t nu = std::max(3u, (uint32_t)std::lround(target / 2.0 / nv));
             m = torus(nu, nv, 0.25f, rng);
         }
-        meshes.push_back(s->add_mesh(m));
+        meshes.push_back(add_mesh(s, m));
     }
     for (uint32_t i = 0; i < n_inst; ++i) {
         const uint32_t mesh = meshes[rng.next() % n_types];
@@ -317,15 +299,15 @@ static void instanced_into(pico_scene* s, Rng& rng, uint32_t n_types, uint32_t n
         const Mat M = mul(translate(p), mul(quat_mat(qw, qx, qy, qz), scale({sc, sc, sc})));
         F3 a = {rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f)};
         const uint32_t mat = s->add_material(metal_rough(a, 0.0f, rng.uniform(0.1f, 0.9f), {0, 0, 0}));
-        s->add_instance(mesh, M, mat, PICO_BSDF_DIELECTRIC);
+        add_instance(s, mesh, M, mat, PICO_BSDF_DIELECTRIC);
     }
     for (uint32_t l = 0; l < n_lights; ++l) {
-        const uint32_t lmesh = s->add_mesh(quad(-1.0f));
+        const uint32_t lmesh = add_mesh(s, quad(-1.0f));
         const float e = rng.uniform(5.0f, 20.0f);
         const uint32_t mat = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {e, e, e}));
         const float side = 0.15f * extent;
         F3 p = {rng.uniform(0.2f * extent, 0.8f * extent), rng.uniform(0.85f * extent, extent), rng.uniform(0.2f * extent, 0.8f * extent)};
-        s->add_instance(lmesh, mul(translate(p), scale({side, 1.0f, side})), mat, PICO_BSDF_LIGHT);
+        add_instance(s, lmesh, mul(translate(p), scale({side, 1.0f, side})), mat, PICO_BSDF_LIGHT);
     }
     s->sky[0] = s->sky[1] = s->sky[2] = sky;
     const float c = 0.5f * extent;

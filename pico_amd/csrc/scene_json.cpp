@@ -1,0 +1,621 @@
+// Scenes in the reference's JSON format, read into the flattened scene
+// (host code).  The reference's loader is Scene::Scene(ThreadPool&, path)
+// (Source/Core/Scene.cpp:36-84) with one handler per section:
+//   GLOBALS   process_globals    (Scene.cpp:639-720)
+//   MESH      add_mesh           (Scene.cpp:297-318), meshes through assimp
+//   MATERIALS add_material       (Scene.cpp:424-582)
+//   INSTANCE  add_mesh_instance  (Scene.cpp:321-422)
+//   CAMERA    add_camera         (Scene.cpp:584-636)
+// Sections are read in that order and the entries of a section in the
+// sorted order of their names (jsoncpp keeps object members in a std::map);
+// the reference runs the entries of a section as parallel tasks, so its
+// instance order is not fixed, and the order here is the sorted one.
+//
+// What is read:
+//   - meshes from Wavefront OBJ files (the reference hands any format to
+//     assimp with Triangulate | JoinIdenticalVertices | GenNormals and keeps
+//     the file's first mesh): polygons become triangle fans, corners without
+//     a normal get the flat normal of their triangle, and corners with equal
+//     position and normal are joined in order of first use; the first mesh
+//     ends where a later `o`, `g` or `usemtl` starts another one;
+//   - Constant materials in both forms, metalness/roughness and
+//     diffuse/specular/gloss, evaluated as Source/Render/BasicMaterials.cpp
+//     evaluates them (a missing Metalness, uninitialised in the reference,
+//     is 0 here);
+//   - instances with T * R(normalised quaternion x, y, z, w) * S and the
+//     BSRDFs Diffuse, Specular, Dielectric, Light, Delta (Specular and
+//     Dielectric use the reference's Beckmann_All distribution);
+//   - cameras (up (0, 1, 0), aspect 1, far plane 10 unless given; the
+//     reference's rays ignore the near plane and the field of view).
+// Refused with an error naming the entry: image materials (Metalic, Gloss),
+// the Transparent and Fresnel BSRDFs, the built-in Sphere and Cube assets,
+// sky boxes and the sun light, and anything malformed.  A light instance
+// whose mesh is already used by another light gets its own copy of the mesh
+// (the device's light tables are per mesh).  Paths are resolved against the
+// scene file's directory, without regard to case when the exact path is
+// missing (as File_System_Mappings::resolve_path, Source/Core/FileMappings.cpp:37-61).
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/pico_hip.h"
+#include "scene_store.h"
+
+namespace fs = std::filesystem;
+
+namespace {
+
+// ---- JSON (RFC 8259) ----------------------------------------------------
+struct JVal {
+    enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
+    bool b = false;
+    double num = 0.0;
+    std::string str;
+    std::vector<JVal> arr;
+    std::map<std::string, JVal> obj;  // sorted by name; a repeated name keeps the last value
+
+    const JVal* get(const char* k) const {
+        if (kind != Obj) return nullptr;
+        auto it = obj.find(k);
+        return it == obj.end() ? nullptr : &it->second;
+    }
+};
+
+struct LoadError {
+    std::string msg;
+};
+
+class JsonParser {
+public:
+    JsonParser(const std::string& text, const std::string& file) : s_(text), file_(file) {}
+
+    JVal parse() {
+        JVal v = value(0);
+        ws();
+        if (i_ != s_.size()) fail("unexpected text after the top-level value");
+        return v;
+    }
+
+private:
+    const std::string& s_;
+    std::string file_;
+    size_t i_ = 0;
+
+    [[noreturn]] void fail(const std::string& what) const {
+        size_t line = 1, col = 1;
+        for (size_t k = 0; k < i_ && k < s_.size(); ++k) {
+            if (s_[k] == '\n') { ++line; col = 1; } else { ++col; }
+        }
+        throw LoadError{file_ + ":" + std::to_string(line) + ":" + std::to_string(col) + ": " + what};
+    }
+    void ws() {
+        while (i_ < s_.size() && (s_[i_] == ' ' || s_[i_] == '\t' || s_[i_] == '\n' || s_[i_] == '\r')) ++i_;
+    }
+    bool lit(const char* w) {
+        const size_t n = std::strlen(w);
+        if (s_.compare(i_, n, w) == 0) { i_ += n; return true; }
+        return false;
+    }
+    JVal value(int depth) {
+        if (depth > 256) fail("nesting too deep");
+        ws();
+        if (i_ >= s_.size()) fail("unexpected end of file");
+        JVal v;
+        const char c = s_[i_];
+        if (c == '{') {
+            v.kind = JVal::Obj;
+            ++i_;
+            ws();
+            if (i_ < s_.size() && s_[i_] == '}') { ++i_; return v; }
+            for (;;) {
+                ws();
+                if (i_ >= s_.size() || s_[i_] != '"') fail("expected a member name");
+                std::string k = string();
+                ws();
+                if (i_ >= s_.size() || s_[i_] != ':') fail("expected ':'");
+                ++i_;
+                v.obj[k] = value(depth + 1);
+                ws();
+                if (i_ < s_.size() && s_[i_] == ',') { ++i_; continue; }
+                if (i_ < s_.size() && s_[i_] == '}') { ++i_; return v; }
+                fail("expected ',' or '}'");
+            }
+        }
+        if (c == '[') {
+            v.kind = JVal::Arr;
+            ++i_;
+            ws();
+            if (i_ < s_.size() && s_[i_] == ']') { ++i_; return v; }
+            for (;;) {
+                v.arr.push_back(value(depth + 1));
+                ws();
+                if (i_ < s_.size() && s_[i_] == ',') { ++i_; continue; }
+                if (i_ < s_.size() && s_[i_] == ']') { ++i_; return v; }
+                fail("expected ',' or ']'");
+            }
+        }
+        if (c == '"') {
+            v.kind = JVal::Str;
+            v.str = string();
+            return v;
+        }
+        if (lit("true")) { v.kind = JVal::Bool; v.b = true; return v; }
+        if (lit("false")) { v.kind = JVal::Bool; v.b = false; return v; }
+        if (lit("null")) return v;
+        if (c == '-' || (c >= '0' && c <= '9')) {
+            v.kind = JVal::Num;
+            v.num = number();
+            return v;
+        }
+        fail(std::string("unexpected character '") + c + "'");
+    }
+    double number() {
+        const size_t b = i_;
+        if (s_[i_] == '-') ++i_;
+        auto digits = [&] {
+            const size_t d = i_;
+            while (i_ < s_.size() && std::isdigit((unsigned char)s_[i_])) ++i_;
+            if (i_ == d) fail("malformed number");
+        };
+        if (i_ < s_.size() && s_[i_] == '0') ++i_;
+        else digits();
+        if (i_ < s_.size() && s_[i_] == '.') { ++i_; digits(); }
+        if (i_ < s_.size() && (s_[i_] == 'e' || s_[i_] == 'E')) {
+            ++i_;
+            if (i_ < s_.size() && (s_[i_] == '+' || s_[i_] == '-')) ++i_;
+            digits();
+        }
+        // the grammar above admits only what strtod reads the same in the C locale
+        std::istringstream in(s_.substr(b, i_ - b));
+        in.imbue(std::locale::classic());
+        double x = 0.0;
+        in >> x;
+        return x;
+    }
+    static void put_utf8(std::string& o, uint32_t cp) {
+        if (cp < 0x80) o += (char)cp;
+        else if (cp < 0x800) { o += (char)(0xC0 | (cp >> 6)); o += (char)(0x80 | (cp & 0x3F)); }
+        else if (cp < 0x10000) {
+            o += (char)(0xE0 | (cp >> 12)); o += (char)(0x80 | ((cp >> 6) & 0x3F)); o += (char)(0x80 | (cp & 0x3F));
+        } else {
+            o += (char)(0xF0 | (cp >> 18)); o += (char)(0x80 | ((cp >> 12) & 0x3F));
+            o += (char)(0x80 | ((cp >> 6) & 0x3F)); o += (char)(0x80 | (cp & 0x3F));
+        }
+    }
+    uint32_t hex4() {
+        if (i_ + 4 > s_.size()) fail("truncated \\u escape");
+        uint32_t v = 0;
+        for (int k = 0; k < 4; ++k) {
+            const char h = s_[i_++];
+            v <<= 4;
+            if (h >= '0' && h <= '9') v |= (uint32_t)(h - '0');
+            else if (h >= 'a' && h <= 'f') v |= (uint32_t)(h - 'a' + 10);
+            else if (h >= 'A' && h <= 'F') v |= (uint32_t)(h - 'A' + 10);
+            else fail("bad \\u escape");
+        }
+        return v;
+    }
+    std::string string() {
+        ++i_;  // opening quote
+        std::string o;
+        for (;;) {
+            if (i_ >= s_.size()) fail("unterminated string");
+            const char c = s_[i_++];
+            if (c == '"') return o;
+            if ((unsigned char)c < 0x20) fail("control character in a string");
+            if (c != '\\') { o += c; continue; }
+            if (i_ >= s_.size()) fail("unterminated string");
+            const char e = s_[i_++];
+            switch (e) {
+                case '"': o += '"'; break;
+                case '\\': o += '\\'; break;
+                case '/': o += '/'; break;
+                case 'b': o += '\b'; break;
+                case 'f': o += '\f'; break;
+                case 'n': o += '\n'; break;
+                case 'r': o += '\r'; break;
+                case 't': o += '\t'; break;
+                case 'u': {
+                    uint32_t cp = hex4();
+                    if (cp >= 0xD800 && cp < 0xDC00 && i_ + 1 < s_.size() && s_[i_] == '\\' && s_[i_ + 1] == 'u') {
+                        i_ += 2;
+                        const uint32_t lo = hex4();
+                        if (lo >= 0xDC00 && lo < 0xE000) cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+                        else fail("bad surrogate pair");
+                    }
+                    put_utf8(o, cp);
+                    break;
+                }
+                default: fail("bad escape");
+            }
+        }
+    }
+};
+
+std::string read_file(const fs::path& p) {
+    std::ifstream f(p, std::ios::binary);
+    if (!f) throw LoadError{"cannot open " + p.string()};
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+// The reference's path rule: relative to the scene's directory; when that
+// file is missing, the file under the directory whose path matches without
+// regard to case.
+fs::path resolve(const fs::path& root, std::string rel) {
+    std::replace(rel.begin(), rel.end(), '\\', '/');
+    fs::path p = fs::path(rel).is_relative() ? (root / rel).lexically_normal() : fs::path(rel).lexically_normal();
+    std::error_code ec;
+    if (fs::exists(p, ec)) return p;
+    auto lower = [](std::string s) {
+        std::transform(s.begin(), s.end(), s.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+        return s;
+    };
+    const std::string want = lower(p.string());
+    for (auto it = fs::recursive_directory_iterator(root, ec); !ec && it != fs::recursive_directory_iterator();
+         it.increment(ec)) {
+        if (it->is_regular_file(ec) && lower(it->path().lexically_normal().string()) == want) return it->path();
+    }
+    return p;
+}
+
+// ---- Wavefront OBJ --------------------------------------------------------
+struct MeshData {
+    std::vector<float> pos, nrm;
+    std::vector<uint32_t> idx;
+};
+
+MeshData read_obj(const fs::path& path) {
+    const std::string text = read_file(path);
+    std::vector<float> v, vn;
+    struct Corner { int64_t v, n; };
+    std::vector<Corner> corners;  // three per triangle
+    bool started = false, done = false;
+    size_t line_no = 0;
+    auto bad = [&](const std::string& what) {
+        throw LoadError{path.string() + ":" + std::to_string(line_no) + ": " + what};
+    };
+    std::istringstream in(text);
+    in.imbue(std::locale::classic());
+    std::string line;
+    while (!done && std::getline(in, line)) {
+        ++line_no;
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        std::istringstream ls(line);
+        ls.imbue(std::locale::classic());
+        std::string tag;
+        if (!(ls >> tag) || tag[0] == '#') continue;
+        if (tag == "v") {
+            float x, y, z;
+            if (!(ls >> x >> y >> z)) bad("malformed vertex");
+            v.insert(v.end(), {x, y, z});
+        } else if (tag == "vn") {
+            float x, y, z;
+            if (!(ls >> x >> y >> z)) bad("malformed normal");
+            vn.insert(vn.end(), {x, y, z});
+        } else if (tag == "f") {
+            std::vector<Corner> poly;
+            std::string tok;
+            while (ls >> tok) {
+                int64_t ref[3] = {0, 0, 0};
+                int part = 0;
+                size_t k = 0;
+                while (k <= tok.size() && part < 3) {
+                    const size_t e = tok.find('/', k);
+                    const std::string f = tok.substr(k, e == std::string::npos ? std::string::npos : e - k);
+                    if (!f.empty()) {
+                        char* end = nullptr;
+                        ref[part] = std::strtoll(f.c_str(), &end, 10);
+                        if (*end || ref[part] == 0) bad("malformed face index '" + tok + "'");
+                    }
+                    ++part;
+                    if (e == std::string::npos) break;
+                    k = e + 1;
+                }
+                // 1-based, negative counts back from the last vertex read
+                auto fix = [&](int64_t r, size_t count) -> int64_t {
+                    if (r == 0) return -1;
+                    const int64_t i = r > 0 ? r - 1 : (int64_t)count + r;
+                    if (i < 0 || i >= (int64_t)count) bad("face index out of range '" + tok + "'");
+                    return i;
+                };
+                const int64_t vi = fix(ref[0], v.size() / 3);
+                if (vi < 0) bad("face corner without a vertex '" + tok + "'");
+                poly.push_back({vi, fix(ref[2], vn.size() / 3)});
+            }
+            if (poly.size() < 3) bad("face with fewer than 3 corners");
+            for (size_t t = 1; t + 1 < poly.size(); ++t) corners.insert(corners.end(), {poly[0], poly[t], poly[t + 1]});
+            started = true;
+        } else if (tag == "o" || tag == "g" || tag == "usemtl") {
+            if (started) done = true;  // the file's first mesh ends here
+        }
+        // vt, s, mtllib, l, p, ...: not needed
+    }
+    if (corners.empty()) throw LoadError{path.string() + ": no faces"};
+    // flat normals for corners without one, then join equal (position, normal)
+    MeshData m;
+    struct Key {
+        uint32_t b[6];
+        bool operator==(const Key& o) const { return std::memcmp(b, o.b, sizeof(b)) == 0; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const {
+            uint64_t h = 1469598103934665603ull;
+            for (uint32_t x : k.b) { h ^= x; h *= 1099511628211ull; }
+            return (size_t)h;
+        }
+    };
+    std::unordered_map<Key, uint32_t, KeyHash> seen;
+    seen.reserve(corners.size());
+    for (size_t t = 0; t < corners.size(); t += 3) {
+        const float* p0 = &v[3 * corners[t].v];
+        const float* p1 = &v[3 * corners[t + 1].v];
+        const float* p2 = &v[3 * corners[t + 2].v];
+        float fn[3] = {0, 0, 0};
+        {
+            const float a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+            const float b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+            fn[0] = a[1] * b[2] - a[2] * b[1];
+            fn[1] = a[2] * b[0] - a[0] * b[2];
+            fn[2] = a[0] * b[1] - a[1] * b[0];
+            const float l = std::sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+            if (l > 0.0f) for (float& x : fn) x /= l;
+        }
+        for (int c = 0; c < 3; ++c) {
+            const Corner& cr = corners[t + c];
+            const float* p = &v[3 * cr.v];
+            const float* n = cr.n >= 0 ? &vn[3 * cr.n] : fn;
+            Key k;
+            std::memcpy(k.b, p, 12);
+            std::memcpy(k.b + 3, n, 12);
+            auto ins = seen.emplace(k, (uint32_t)(m.pos.size() / 3));
+            if (ins.second) {
+                m.pos.insert(m.pos.end(), p, p + 3);
+                m.nrm.insert(m.nrm.end(), n, n + 3);
+            }
+            m.idx.push_back(ins.first->second);
+        }
+    }
+    return m;
+}
+
+// ---- scene ----------------------------------------------------------------
+float as_float(const JVal& v, const std::string& where) {
+    if (v.kind != JVal::Num) throw LoadError{where + ": expected a number"};
+    return (float)v.num;
+}
+
+std::string as_string(const JVal* v, const std::string& where) {
+    if (!v || v->kind != JVal::Str) throw LoadError{where + ": expected a string"};
+    return v->str;
+}
+
+void vec(const JVal* v, int n, float* out, const std::string& where) {
+    if (!v || v->kind != JVal::Arr || (int)v->arr.size() < n)
+        throw LoadError{where + ": expected an array of " + std::to_string(n) + " numbers"};
+    for (int k = 0; k < n; ++k) out[k] = as_float(v->arr[k], where);
+}
+
+// column-major a * b
+void mat_mul(const float* a, const float* b, float* r) {
+    for (int c = 0; c < 4; ++c)
+        for (int rr = 0; rr < 4; ++rr) {
+            float s = 0.0f;
+            for (int k = 0; k < 4; ++k) s += a[k * 4 + rr] * b[c * 4 + k];
+            r[c * 4 + rr] = s;
+        }
+}
+
+void load_into(pico_scene& sc, const fs::path& file) {
+    const fs::path root = file.parent_path().empty() ? fs::path(".") : file.parent_path();
+    const std::string text = read_file(file);
+    JVal doc = JsonParser(text, file.string()).parse();
+    if (doc.kind != JVal::Obj) throw LoadError{file.string() + ": the top level is not an object"};
+    sc.microfacet = PICO_MICROFACET_BECKMANN_ALL;
+    // GLOBALS
+    if (const JVal* g = doc.get("GLOBALS")) {
+        if (g->kind != JVal::Obj) throw LoadError{"GLOBALS: expected an object"};
+        for (const auto& [name, e] : g->obj) {
+            if (e.get("Skybox")) throw LoadError{"GLOBALS/" + name + ": sky boxes are not supported (the sky is black)"};
+            if (e.get("SunDirection") && e.get("SunColour"))
+                throw LoadError{"GLOBALS/" + name + ": the sun light is not supported"};
+        }
+    }
+    // MESH
+    std::map<std::string, uint32_t> mesh_of;
+    std::vector<MeshData> meshes;
+    if (const JVal* ms = doc.get("MESH")) {
+        if (ms->kind != JVal::Obj) throw LoadError{"MESH: expected an object"};
+        for (const auto& [name, e] : ms->obj) {
+            const std::string where = "MESH/" + name;
+            if (name == "Sphere" || name == "Cube") throw LoadError{where + ": the name of a built-in shape"};
+            const fs::path p = resolve(root, as_string(e.get("Path"), where + "/Path"));
+            std::string ext = p.extension().string();
+            std::transform(ext.begin(), ext.end(), ext.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+            if (ext != ".obj") throw LoadError{where + ": only Wavefront .obj meshes are read (" + p.string() + ")"};
+            mesh_of[name] = (uint32_t)meshes.size();
+            meshes.push_back(read_obj(p));
+        }
+    }
+    // MATERIALS
+    std::map<std::string, uint32_t> mat_of;
+    if (const JVal* mm = doc.get("MATERIALS")) {
+        if (mm->kind != JVal::Obj) throw LoadError{"MATERIALS: expected an object"};
+        for (const auto& [name, e] : mm->obj) {
+            const std::string where = "MATERIALS/" + name;
+            const std::string type = as_string(e.get("Type"), where + "/Type");
+            if (type == "Metalic" || type == "Gloss")
+                throw LoadError{where + ": image materials (" + type + ") are not supported"};
+            if (type != "Constant") throw LoadError{where + ": unknown material type " + type};
+            pico_material m{};
+            float emissive[3] = {0, 0, 0};
+            const bool transparent = e.get("IndexOfRefraction") != nullptr;  // its emission is 0 in the reference
+            if (e.get("Emissive") && !transparent) vec(e.get("Emissive"), 3, emissive, where + "/Emissive");
+            if (e.get("Albedo")) {
+                float albedo[3];
+                vec(e.get("Albedo"), 3, albedo, where + "/Albedo");
+                const float metal = e.get("Metalness") ? as_float(*e.get("Metalness"), where + "/Metalness") : 0.0f;
+                const float rough = e.get("Roughness") ? as_float(*e.get("Roughness"), where + "/Roughness") : 0.0f;
+                for (int k = 0; k < 3; ++k) {
+                    m.diffuse[k] = albedo[k] * (1.0f - 0.04f) * (1.0f - metal);
+                    m.specular[k] = 0.04f + metal * (albedo[k] - 0.04f);  // glm::lerp(0.04, albedo, metalness)
+                }
+                m.roughness = rough;
+            } else if (e.get("Diffuse") || e.get("Specular")) {
+                if (e.get("Diffuse")) vec(e.get("Diffuse"), 3, m.diffuse, where + "/Diffuse");
+                if (e.get("Specular")) vec(e.get("Specular"), 3, m.specular, where + "/Specular");
+                const float gloss = e.get("Gloss") ? as_float(*e.get("Gloss"), where + "/Gloss") : 0.0f;
+                m.roughness = 1.0f - gloss * gloss;
+            } else {
+                throw LoadError{where + ": a Constant material needs Albedo, Diffuse or Specular"};
+            }
+            for (int k = 0; k < 3; ++k) m.emissive[k] = emissive[k];
+            mat_of[name] = sc.add_material(m);
+        }
+    }
+    if (sc.mats.empty()) throw LoadError{"the scene has no materials"};
+    // INSTANCE
+    std::vector<uint32_t> mesh_id(meshes.size(), UINT32_MAX);
+    std::vector<bool> light_uses(meshes.size(), false);
+    auto add_mesh = [&](uint32_t m) {
+        const MeshData& d = meshes[m];
+        return sc.add_mesh(d.pos.data(), d.nrm.data(), d.pos.size() / 3, d.idx.data(), d.idx.size());
+    };
+    if (const JVal* is = doc.get("INSTANCE")) {
+        if (is->kind != JVal::Obj) throw LoadError{"INSTANCE: expected an object"};
+        for (const auto& [name, e] : is->obj) {
+            const std::string where = "INSTANCE/" + name;
+            const std::string asset = as_string(e.get("Asset"), where + "/Asset");
+            if (asset == "Sphere" || asset == "Cube") throw LoadError{where + ": the built-in " + asset + " is not supported"};
+            auto mi = mesh_of.find(asset);
+            if (mi == mesh_of.end()) throw LoadError{where + ": unknown asset " + asset};
+            float position[3] = {0, 0, 0}, scale[3] = {1, 1, 1}, q[4] = {0, 0, 0, 1};  // q: x, y, z, w
+            if (e.get("Position")) vec(e.get("Position"), 3, position, where + "/Position");
+            if (e.get("Scale")) vec(e.get("Scale"), 3, scale, where + "/Scale");
+            if (e.get("Rotation")) {
+                vec(e.get("Rotation"), 4, q, where + "/Rotation");
+                const float l = std::sqrt(q[3] * q[3] + q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);  // glm::normalize(quat)
+                if (!(l > 0.0f)) throw LoadError{where + "/Rotation: zero quaternion"};
+                for (float& x : q) x /= l;
+            }
+            uint32_t material = 0;
+            if (e.get("Material")) {
+                const std::string mn = as_string(e.get("Material"), where + "/Material");
+                auto it = mat_of.find(mn);
+                if (it == mat_of.end()) throw LoadError{where + ": unknown material " + mn};
+                material = it->second;
+            }
+            const std::string kind = as_string(e.get("BSRDF"), where + "/BSRDF");
+            uint32_t bsdf;
+            if (kind == "Diffuse") bsdf = PICO_BSDF_DIFFUSE;
+            else if (kind == "Specular") bsdf = PICO_BSDF_SPECULAR;
+            else if (kind == "Dielectric") bsdf = PICO_BSDF_DIELECTRIC;
+            else if (kind == "Light") bsdf = PICO_BSDF_LIGHT;
+            else if (kind == "Delta") bsdf = PICO_BSDF_DELTA;
+            else if (kind == "Transparent" || kind == "Fresnel") throw LoadError{where + ": the " + kind + " BSRDF is not supported"};
+            else throw LoadError{where + ": unknown BSRDF " + kind};
+            // T * R * S (Scene.cpp:359-361); R = glm::mat4_cast(q)
+            const float x = q[0], y = q[1], z = q[2], w = q[3];
+            float T[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, position[0], position[1], position[2], 1};
+            float R[16] = {1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y), 0,
+                           2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x), 0,
+                           2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y), 0,
+                           0, 0, 0, 1};
+            float S[16] = {scale[0], 0, 0, 0, 0, scale[1], 0, 0, 0, 0, scale[2], 0, 0, 0, 0, 1};
+            float TR[16], M[16];
+            mat_mul(T, R, TR);
+            mat_mul(TR, S, M);
+            const uint32_t m = mi->second;
+            uint32_t id;
+            if (bsdf == PICO_BSDF_LIGHT && light_uses[m]) {
+                id = add_mesh(m);  // a second light on this mesh: its own copy
+            } else {
+                if (mesh_id[m] == UINT32_MAX) mesh_id[m] = add_mesh(m);
+                id = mesh_id[m];
+            }
+            if (bsdf == PICO_BSDF_LIGHT) light_uses[m] = true;
+            sc.add_instance(id, M, material, bsdf);
+        }
+    }
+    if (sc.imesh.empty()) throw LoadError{"the scene has no instances"};
+    // CAMERA
+    if (const JVal* cs = doc.get("CAMERA")) {
+        if (cs->kind != JVal::Obj) throw LoadError{"CAMERA: expected an object"};
+        for (const auto& [name, e] : cs->obj) {
+            const std::string where = "CAMERA/" + name;
+            pico_camera c{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, 1.0f, 10.0f};
+            if (e.get("Position")) vec(e.get("Position"), 3, c.position, where + "/Position");
+            if (e.get("Direction")) {
+                vec(e.get("Direction"), 3, c.direction, where + "/Direction");
+                const float l = std::sqrt(c.direction[0] * c.direction[0] + c.direction[1] * c.direction[1] +
+                                          c.direction[2] * c.direction[2]);
+                if (!(l > 0.0f)) throw LoadError{where + "/Direction: zero vector"};
+                for (float& d : c.direction) d /= l;
+            }
+            if (e.get("Aspect")) c.aspect = as_float(*e.get("Aspect"), where + "/Aspect");
+            if (e.get("FarPlane")) c.far_plane = as_float(*e.get("FarPlane"), where + "/FarPlane");
+            sc.cameras.emplace_back(name, c);
+        }
+    }
+    sc.cam = sc.cameras.empty() ? pico_camera{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, 1.0f, 10.0f} : sc.cameras.front().second;
+}
+
+void set_err(char* err, size_t n, const std::string& msg) {
+    if (!err || n == 0) return;
+    const size_t k = std::min(n - 1, msg.size());
+    std::memcpy(err, msg.data(), k);
+    err[k] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pico_scene_load_json(const char* path, pico_scene** out, char* err, size_t err_len) {
+    set_err(err, err_len, "");
+    if (!path || !out) {
+        set_err(err, err_len, "NULL argument");
+        return PICO_ERR_ARG;
+    }
+    *out = nullptr;
+    auto* s = new pico_scene();
+    try {
+        load_into(*s, fs::path(path));
+    } catch (const LoadError& e) {
+        delete s;
+        set_err(err, err_len, e.msg);
+        return PICO_ERR_ARG;
+    } catch (const std::exception& e) {
+        delete s;
+        set_err(err, err_len, std::string("scene load failed: ") + e.what());
+        return PICO_ERR_ARG;
+    }
+    *out = s;
+    return PICO_OK;
+}
+
+int pico_scene_camera_count(const pico_scene* s, uint32_t* n) {
+    if (!s || !n) return PICO_ERR_ARG;
+    *n = (uint32_t)s->cameras.size();
+    return PICO_OK;
+}
+
+int pico_scene_camera_get(const pico_scene* s, uint32_t i, char* name, size_t name_len, pico_camera* out) {
+    if (!s || i >= s->cameras.size()) return PICO_ERR_ARG;
+    set_err(name, name_len, s->cameras[i].first);
+    if (out) *out = s->cameras[i].second;
+    return PICO_OK;
+}
+
+}  // extern "C"
