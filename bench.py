@@ -402,6 +402,64 @@ def build_leg(ctx, pa, scene):
     return out
 
 
+def ingest_leg(ctx, pa, tris=500000):
+    """Scene ingest: a scene file in the reference's JSON format whose one mesh
+    (about `tris` triangles, positions and normals) is a Wavefront OBJ file;
+    the host load (JSON + OBJ parse, corner joining) and the device BVH build
+    of the loaded scene are timed."""
+    import ctypes as C
+    import tempfile
+    src = pa.Scene.instanced(11, 1, 1, tris, tris, 10.0, pa.MICROFACET_GGX)
+    d = src.desc
+    nv = int(np.ctypeslib.as_array(C.cast(d.mesh_vertex_offset, C.POINTER(C.c_uint32)), (2,))[1])
+    nt = int(np.ctypeslib.as_array(C.cast(d.mesh_triangle_offset, C.POINTER(C.c_uint32)), (2,))[1])
+    pos = np.ctypeslib.as_array(C.cast(d.positions, C.POINTER(C.c_float)), (3 * nv,)).reshape(-1, 3)
+    nrm = np.ctypeslib.as_array(C.cast(d.normals, C.POINTER(C.c_float)), (3 * nv,)).reshape(-1, 3)
+    idx = np.ctypeslib.as_array(C.cast(d.indices, C.POINTER(C.c_uint32)), (3 * nt,)).reshape(-1, 3) + 1
+    with tempfile.TemporaryDirectory() as tmp:
+        obj = os.path.join(tmp, "mesh.obj")
+        with open(obj, "w") as f:
+            f.write("".join(f"v {a:.9g} {b:.9g} {c:.9g}\n" for a, b, c in pos.tolist()))
+            f.write("".join(f"vn {a:.9g} {b:.9g} {c:.9g}\n" for a, b, c in nrm.tolist()))
+            f.write("".join(f"f {a}//{a} {b}//{b} {c}//{c}\n" for a, b, c in idx.tolist()))
+        doc = {"MESH": {"Blob": {"Path": "mesh.obj"}, "Lamp": {"Path": "lamp.obj"}},
+               "MATERIALS": {"Grey": {"Type": "Constant", "Albedo": [0.6, 0.6, 0.6], "Roughness": 0.4},
+                             "Lamp": {"Type": "Constant", "Albedo": [1, 1, 1], "Emissive": [8, 8, 8]}},
+               "INSTANCE": {"blob": {"Asset": "Blob", "BSRDF": "Dielectric", "Material": "Grey"},
+                            "lamp": {"Asset": "Lamp", "BSRDF": "Light", "Material": "Lamp", "Position": [0, 30, 0],
+                                     "Rotation": [1, 0, 0, 0], "Scale": [5, 1, 5]}},
+               "CAMERA": {"main": {"Position": [5, 5, -25], "Direction": [0, 0, 1], "FarPlane": 100.0}}}
+        with open(os.path.join(tmp, "lamp.obj"), "w") as f:
+            f.write("v -1 0 -1\nv -1 0 1\nv 1 0 1\nv 1 0 -1\nf 1 2 3 4\n")
+        path = os.path.join(tmp, "scene.json")
+        with open(path, "w") as f:
+            json.dump(doc, f)
+        mb = os.path.getsize(obj) / 1e6
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            scene = pa.Scene.load_json(path)
+            times.append((time.perf_counter() - t0) * 1e3)
+    load_ms = sorted(times)[1]
+    st = [ctx.upload_scene(scene, device_build=True) for _ in range(2)][-1]
+    cam = scene.cameras()["main"]
+    ld = _desc_counts(scene)
+    out = {"obj_mb": round(mb, 1), "triangles": ld[1], "vertices": ld[0], "load_ms": round(load_ms, 1),
+           "load_mb_per_s": round(mb / (load_ms * 1e-3), 1), "device_bvh_ms": round(st.bvh_ms, 2),
+           "upload_total_ms": round(st.total_ms, 1)}
+    p = pa.params(960, 540, 4, 8, frame_seed=1)
+    out["first_frame_ms_960x540_4spp"] = round(ctx.render(cam, p)[3].render_ms, 2)
+    return out
+
+
+def _desc_counts(scene):
+    import ctypes as C
+    d = scene.desc
+    vofs = np.ctypeslib.as_array(C.cast(d.mesh_vertex_offset, C.POINTER(C.c_uint32)), (d.n_meshes + 1,))
+    tofs = np.ctypeslib.as_array(C.cast(d.mesh_triangle_offset, C.POINTER(C.c_uint32)), (d.n_meshes + 1,))
+    return int(vofs[-1]), int(tofs[-1])
+
+
 def _ptr_array(ctx, ptr):
     """A non-owning device array at a raw pointer (the views' own buffers)."""
     import pico_amd as pa
@@ -528,6 +586,9 @@ def main():
         log(f"[bench] device TLAS rebuild: {tlas}")
         build = build_leg(ctx, pa, scene)
         log(f"[bench] device scene build: {build}")
+        build["json_ingest"] = ingest_leg(ctx, pa)
+        log(f"[bench] JSON ingest: {build['json_ingest']}")
+        ctx.upload_scene(scene)
 
     cpu = None
     if not args.no_cpu:

@@ -36,6 +36,7 @@
 // missing (as File_System_Mappings::resolve_path, Source/Core/FileMappings.cpp:37-61).
 #include <algorithm>
 #include <cctype>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -45,6 +46,7 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <string_view>
 #include <unordered_map>
 #include <vector>
 
@@ -175,11 +177,9 @@ private:
             if (i_ < s_.size() && (s_[i_] == '+' || s_[i_] == '-')) ++i_;
             digits();
         }
-        // the grammar above admits only what strtod reads the same in the C locale
-        std::istringstream in(s_.substr(b, i_ - b));
-        in.imbue(std::locale::classic());
+        // the grammar above is the JSON one; from_chars reads it correctly rounded
         double x = 0.0;
-        in >> x;
+        std::from_chars(s_.data() + b, s_.data() + i_, x);
         return x;
     }
     static void put_utf8(std::string& o, uint32_t cp) {
@@ -243,11 +243,12 @@ private:
 };
 
 std::string read_file(const fs::path& p) {
-    std::ifstream f(p, std::ios::binary);
+    std::ifstream f(p, std::ios::binary | std::ios::ate);
     if (!f) throw LoadError{"cannot open " + p.string()};
-    std::ostringstream ss;
-    ss << f.rdbuf();
-    return ss.str();
+    std::string s((size_t)f.tellg(), '\0');
+    f.seekg(0);
+    if (!f.read(s.data(), (std::streamsize)s.size())) throw LoadError{"cannot read " + p.string()};
+    return s;
 }
 
 // The reference's path rule: relative to the scene's directory; when that
@@ -281,64 +282,81 @@ MeshData read_obj(const fs::path& path) {
     std::vector<float> v, vn;
     struct Corner { int64_t v, n; };
     std::vector<Corner> corners;  // three per triangle
-    bool started = false, done = false;
+    bool started = false;
     size_t line_no = 0;
     auto bad = [&](const std::string& what) {
         throw LoadError{path.string() + ":" + std::to_string(line_no) + ": " + what};
     };
-    std::istringstream in(text);
-    in.imbue(std::locale::classic());
-    std::string line;
-    while (!done && std::getline(in, line)) {
+    const char* p = text.data();
+    const char* const end = p + text.size();
+    auto blank = [](char c) { return c == ' ' || c == '\t' || c == '\r'; };
+    while (p < end) {
+        const char* eol = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
+        if (!eol) eol = end;
         ++line_no;
-        if (!line.empty() && line.back() == '\r') line.pop_back();
-        std::istringstream ls(line);
-        ls.imbue(std::locale::classic());
-        std::string tag;
-        if (!(ls >> tag) || tag[0] == '#') continue;
-        if (tag == "v") {
-            float x, y, z;
-            if (!(ls >> x >> y >> z)) bad("malformed vertex");
-            v.insert(v.end(), {x, y, z});
-        } else if (tag == "vn") {
-            float x, y, z;
-            if (!(ls >> x >> y >> z)) bad("malformed normal");
-            vn.insert(vn.end(), {x, y, z});
+        const char* q = p;
+        p = eol + 1;
+        while (q < eol && blank(*q)) ++q;
+        const char* t0 = q;
+        while (q < eol && !blank(*q)) ++q;
+        const std::string_view tag(t0, (size_t)(q - t0));
+        if (tag.empty() || tag[0] == '#') continue;
+        auto num = [&](float& x) {
+            while (q < eol && blank(*q)) ++q;
+            if (q < eol && *q == '+') ++q;
+            const auto r = std::from_chars(q, eol, x);
+            if (r.ec != std::errc()) return false;
+            q = r.ptr;
+            return true;
+        };
+        if (tag == "v" || tag == "vn") {
+            float x = 0, y = 0, z = 0;
+            if (!num(x) || !num(y) || !num(z)) bad(tag == "v" ? "malformed vertex" : "malformed normal");
+            (tag == "v" ? v : vn).insert((tag == "v" ? v : vn).end(), {x, y, z});
         } else if (tag == "f") {
-            std::vector<Corner> poly;
-            std::string tok;
-            while (ls >> tok) {
+            Corner poly_first{}, prev{};
+            int n_corner = 0;
+            for (;;) {
+                while (q < eol && blank(*q)) ++q;
+                if (q >= eol) break;
+                const char* tk = q;
                 int64_t ref[3] = {0, 0, 0};
-                int part = 0;
-                size_t k = 0;
-                while (k <= tok.size() && part < 3) {
-                    const size_t e = tok.find('/', k);
-                    const std::string f = tok.substr(k, e == std::string::npos ? std::string::npos : e - k);
-                    if (!f.empty()) {
-                        char* end = nullptr;
-                        ref[part] = std::strtoll(f.c_str(), &end, 10);
-                        if (*end || ref[part] == 0) bad("malformed face index '" + tok + "'");
+                for (int part = 0; part < 3; ++part) {
+                    if (q < eol && !blank(*q) && *q != '/') {
+                        if (*q == '+') ++q;
+                        const auto r = std::from_chars(q, eol, ref[part]);
+                        if (r.ec != std::errc() || ref[part] == 0) {
+                            while (q < eol && !blank(*q)) ++q;
+                            bad("malformed face index '" + std::string(tk, q) + "'");
+                        }
+                        q = r.ptr;
                     }
-                    ++part;
-                    if (e == std::string::npos) break;
-                    k = e + 1;
+                    if (q < eol && *q == '/') ++q;
+                    else break;
+                }
+                if (q < eol && !blank(*q)) {
+                    while (q < eol && !blank(*q)) ++q;
+                    bad("malformed face index '" + std::string(tk, q) + "'");
                 }
                 // 1-based, negative counts back from the last vertex read
                 auto fix = [&](int64_t r, size_t count) -> int64_t {
                     if (r == 0) return -1;
                     const int64_t i = r > 0 ? r - 1 : (int64_t)count + r;
-                    if (i < 0 || i >= (int64_t)count) bad("face index out of range '" + tok + "'");
+                    if (i < 0 || i >= (int64_t)count) bad("face index out of range '" + std::string(tk, q) + "'");
                     return i;
                 };
-                const int64_t vi = fix(ref[0], v.size() / 3);
-                if (vi < 0) bad("face corner without a vertex '" + tok + "'");
-                poly.push_back({vi, fix(ref[2], vn.size() / 3)});
+                const Corner c{fix(ref[0], v.size() / 3), fix(ref[2], vn.size() / 3)};
+                if (c.v < 0) bad("face corner without a vertex '" + std::string(tk, q) + "'");
+                // a fan: (first, previous, this) from the third corner on
+                if (n_corner == 0) poly_first = c;
+                else if (n_corner >= 2) corners.insert(corners.end(), {poly_first, prev, c});
+                prev = c;
+                ++n_corner;
             }
-            if (poly.size() < 3) bad("face with fewer than 3 corners");
-            for (size_t t = 1; t + 1 < poly.size(); ++t) corners.insert(corners.end(), {poly[0], poly[t], poly[t + 1]});
+            if (n_corner < 3) bad("face with fewer than 3 corners");
             started = true;
         } else if (tag == "o" || tag == "g" || tag == "usemtl") {
-            if (started) done = true;  // the file's first mesh ends here
+            if (started) break;  // the file's first mesh ends here
         }
         // vt, s, mtllib, l, p, ...: not needed
     }
