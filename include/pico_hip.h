@@ -59,6 +59,11 @@ extern "C" {
 #define PICO_BSDF_DIELECTRIC  2u
 #define PICO_BSDF_LIGHT       3u
 #define PICO_BSDF_DELTA       4u
+/* Transmissive BSDFs (BSRDF_Type::kBTDF): Transparent_BTDF and Fresnel_BTDF
+ * (Source/Render/BSRDF.cpp:240-421) with the ray's index-of-refraction stack
+ * (Source/Core/AABB.cpp:54-79); their material needs index_of_refraction. */
+#define PICO_BSDF_TRANSPARENT 5u
+#define PICO_BSDF_FRESNEL     6u
 
 /* Microfacet distribution used by every specular lobe.  BECKMANN_ALL is the
  * reference's Beckmann_All_Microfacet_Distribution
@@ -69,12 +74,15 @@ extern "C" {
 #define PICO_MICROFACET_GGX          1u
 
 /* Core::EvaluatedMaterial (Source/Core/MaterialManager.hpp:17-46) of a
- * constant material, without the unused normal field. */
+ * constant material, without the unused normal field, plus the index of
+ * refraction of the ConstantTransparent* materials
+ * (Source/Render/BasicMaterials.hpp:41-65); 0 for the others. */
 typedef struct pico_material {
     float diffuse[3];
     float specular[3];
     float roughness;
     float emissive[3];
+    float index_of_refraction;
 } pico_material;
 
 /* Flattened scene.  Meshes are object-space triangle soups with shared
@@ -427,6 +435,10 @@ typedef struct pico_scene pico_scene;
 /* Cornell box: 5 walls x 2 tris, two boxes of 12 tris, one quad light,
  * Dielectric walls/boxes with albedo U[0.2,0.8], roughness U[0.1,0.9]. */
 int  pico_scene_cornell(uint32_t seed, uint32_t microfacet, pico_scene** out);
+/* The same Cornell box with its two boxes transmissive: a Fresnel glass
+ * block (index 1.5, roughness 0.1) and a Transparent one (index 1.33,
+ * roughness 0.2), both with 0.95 transmitted energy. */
+int  pico_scene_cornell_glass(uint32_t seed, uint32_t microfacet, pico_scene** out);
 /* Instanced scene: n_mesh_types meshes (icospheres and noise-displaced tori
  * of min_tris..max_tris triangles, geometric), n_instances random affine
  * instances in a box of side `extent`, 3 quad lights and a constant sky. */
@@ -442,10 +454,10 @@ int  pico_scene_training(uint32_t seed, uint32_t microfacet, pico_scene** out);
  * meshes from Wavefront .obj files (triangulated, flat normals where the file
  * has none, equal corners joined, the file's first mesh), Constant materials,
  * instances T * R(quaternion) * S with the Diffuse, Specular, Dielectric,
- * Light and Delta BSRDFs (Beckmann_All distribution), named cameras.  Entries
- * of a section are taken in the sorted order of their names.  Image
- * materials, the Transparent and Fresnel BSRDFs, the built-in Sphere and
- * Cube, sky boxes and the sun light are refused.  On error PICO_ERR_ARG and
+ * Light, Delta, Transparent and Fresnel BSRDFs (Beckmann_All distribution),
+ * named cameras.  Entries of a section are taken in the sorted order of their
+ * names.  Image materials, the built-in Sphere and Cube, sky boxes and the
+ * sun light are refused.  On error PICO_ERR_ARG and
  * the reason (file, line, entry) in `err` (err_len bytes, may be NULL). */
 int  pico_scene_load_json(const char* path, pico_scene** out, char* err, size_t err_len);
 /* The named cameras of a loaded scene, in name order (none for procedural

@@ -74,6 +74,7 @@ def cpu_lib():
             "cpuref_alias_sample": (u32, [u32, vp, vp, u32, u32, vp]),
             "cpuref_triangle_areas": (None, [u32, vp, vp, vp]),
             "cpuref_bsdf_sample": (None, [u32, u32, P(Material), u32, vp, vp, u32, u64, vp, vp, vp]),
+            "cpuref_btdf_sample": (None, [u32, u32, P(Material), u32, vp, vp, vp, u32, u64, vp, vp, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             if hasattr(_cr, name):
@@ -113,6 +114,7 @@ def ref_lib():
             "ref_hammersley": (None, [u32, vp, vp]),
             "ref_alias_sample": (None, [u32, vp, u32, u32, vp]),
             "ref_bsdf_sample": (None, [u32, P(Material), u32, vp, vp, u32, vp, vp, vp]),
+            "ref_btdf_sample": (None, [u32, P(Material), u32, vp, vp, vp, u32, vp, vp, vp, vp]),
             "ref_write_hdr": (i32, [C.c_char_p, i32, i32, vp]),
             "ref_load_hdr": (i32, [C.c_char_p, i32, i32, vp]),
         }

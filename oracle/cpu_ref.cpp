@@ -685,6 +685,90 @@ static Sample bsdf_sample(uint32_t kind, uint32_t mf, const pico_material& m, V3
     return s;
 }
 
+// The ray's index-of-refraction stack, Source/Core/AABB.cpp:54-79; the
+// camera pushes 1.0 (Source/Core/Camera.cpp:23).  A pop that would remove
+// the camera's entry keeps it (the reference asserts there).
+struct IorStack {
+    std::vector<float> v{1.0f};
+    void push(float x) { v.push_back(x); }
+    void pop() { if (v.size() > 1) v.pop_back(); }
+    float current() const { return v.back(); }
+    bool inside() const { return v.size() > 1; }  // Ray::inside_geometry
+};
+
+static inline bool is_btdf(uint32_t kind) { return kind == PICO_BSDF_TRANSPARENT || kind == PICO_BSDF_FRESNEL; }
+
+// glm::refract (glm/detail/func_geometric.inl): NaN where k < 0, since the
+// square root of a negative is multiplied by 0.
+static V3 refract_glm(V3 I, V3 N, float eta) {
+    const float d = dot(N, I);
+    const float k = 1.0f - eta * eta * (1.0f - d * d);
+    const float s = eta * d + std::sqrt(k);
+    const float sel = (float)(k >= 0.0f);
+    return {(eta * I.x - s * N.x) * sel, (eta * I.y - s * N.y) * sel, (eta * I.z - s * N.z) * sel};
+}
+
+// Transparent_BTDF::sample, Source/Render/BSRDF.cpp:240-309.
+static Sample btdf_transparent(uint32_t mf, const pico_material& m, V3 N, V3 raydir, Rng& rng, IorStack& st) {
+    const V3 V = -raydir;
+    const Frame f = tangent_frame(V, N);
+    const V3 vt = normalize(to_tangent(f, V));
+    const bool enter = vt.z >= 0.0f;  // TangentSpace::cos_theta
+    const V2 xi = rng.next2d();
+    const V3 H = spec_sample(mf, xi, vt, m.roughness);
+    const float eta = 1.0f / m.index_of_refraction;
+    if (enter) st.push(m.index_of_refraction);
+    else st.pop();
+    const V3 L = refract_glm(vt, H, eta);  // Transparent_BTDF::refract always succeeds
+    Sample s{};
+    s.L = normalize(to_world(f, L));
+    const float pdf = spec_pdf(mf, H, m.roughness);
+    const V3 wh = normalize(vt + L * eta);
+    const float den = dot(vt, wh) + eta * dot(L, wh);
+    s.P = pdf * std::abs((eta * eta * dot(L, wh)) / (den * den));
+    s.energy = mdiff(m);
+    return s;
+}
+
+// Fresnel_BTDF::fresnel_factor, Source/Render/BSRDF.cpp:400-421.
+static float fresnel_factor(float cos_i, float eta_i, float eta_t) {
+    cos_i = std::clamp(cos_i, -1.0f, 1.0f);
+    if (!(cos_i > 0.f)) {
+        std::swap(eta_i, eta_t);
+        cos_i = std::abs(cos_i);
+    }
+    const float sin_i = std::sqrt(std::max(0.f, 1.0f - cos_i * cos_i));
+    const float sin_t = eta_i / eta_t * sin_i;
+    const float cos_t = std::sqrt(std::max(0.0f, 1.0f - sin_t * sin_t));
+    const float rpa = ((eta_t * cos_i) - (eta_i * cos_t)) / ((eta_t * cos_i) + (eta_i * cos_t));
+    const float rpe = ((eta_i * cos_i) - (eta_t * cos_t)) / ((eta_i * cos_i) + (eta_t * cos_t));
+    return (rpa * rpa + rpe * rpe) / 2;
+}
+
+// Fresnel_BTDF::sample, Source/Render/BSRDF.cpp:335-362.
+static Sample btdf_fresnel(uint32_t mf, const pico_material& m, V3 N, V3 raydir, Rng& rng, IorStack& st) {
+    const V3 V = -raydir;
+    const Frame f = tangent_frame(V, N);
+    const V3 vt = normalize(to_tangent(f, V));
+    const V2 xi = rng.next2d();
+    const float F = fresnel_factor(vt.z, m.index_of_refraction, st.current());
+    Sample s;
+    if (xi.x < F) {
+        s = bsdf_sample(PICO_BSDF_SPECULAR, mf, m, N, raydir, rng);
+        s.P *= F;
+    } else {
+        s = btdf_transparent(mf, m, N, raydir, rng, st);
+        s.P *= 1.0f - F;
+    }
+    return s;
+}
+
+static Sample any_sample(uint32_t kind, uint32_t mf, const pico_material& m, V3 N, V3 raydir, Rng& rng, IorStack& st) {
+    if (kind == PICO_BSDF_TRANSPARENT) return btdf_transparent(mf, m, N, raydir, rng, st);
+    if (kind == PICO_BSDF_FRESNEL) return btdf_fresnel(mf, m, N, raydir, rng, st);
+    return bsdf_sample(kind, mf, m, N, raydir, rng);
+}
+
 // BSRDF::pdf, Source/Render/BSRDF.cpp:42-45, 87-91, 151-155, 229-232.
 static float bsdf_pdf(uint32_t kind, uint32_t mf, const pico_material& m, V3 wo, V3 H) {
     switch (kind) {
@@ -714,7 +798,7 @@ struct Counters { uint64_t camera = 0, ext = 0, shadow = 0; };
 // (no sun: configurations have none).
 static bool direct_light(const Scene& s, const Hit& frag, V3 wi, V3& rad, float& pdf, Rng& rng, Counters& c) {
     const Instance& fi = s.inst[frag.inst];
-    if (fi.bsdf == PICO_BSDF_LIGHT) return false;
+    if (fi.bsdf == PICO_BSDF_LIGHT || is_btdf(fi.bsdf)) return false;
     const pico_material& mat = s.mats[fi.material];
     uint32_t count = (uint32_t)s.lights.size();
     const float u = rng.uniform();
@@ -774,11 +858,12 @@ static V3 integrate(const Scene& s, const pico_camera& cam, uint32_t W, uint32_t
     c.camera++;
     if (!closest_hit(s, ray, frag)) return s.sky;
     V3 payload = {0, 0, 0}, thr = {1, 1, 1};
+    IorStack ior;
     for (uint32_t depth = 0;; ++depth) {
         if (depth == max_depth) break;
         const Instance& fi = s.inst[frag.inst];
         const pico_material& m = s.mats[fi.material];
-        Sample smp = bsdf_sample(fi.bsdf, s.microfacet, m, frag.nrm, ray.d, rng);
+        Sample smp = any_sample(fi.bsdf, s.microfacet, m, frag.nrm, ray.d, rng, ior);
         if (fi.bsdf == PICO_BSDF_LIGHT) payload = payload + thr * smp.energy;
         V3 rad; float pdf;
         if (direct_light(s, frag, -ray.d, rad, pdf, rng, c)) payload = payload + ((thr * pdf) * rad) * smp.energy;
@@ -787,7 +872,7 @@ static V3 integrate(const Scene& s, const pico_camera& cam, uint32_t W, uint32_t
         if (rng.uniform() > q) break;
         thr = thr * (1.0f / q);
         thr = thr * (smp.P * smp.energy);
-        ray.o = frag.pos + v4(0.01f * frag.nrm, 0.0f);  // IOR stack depth is 1 without BTDFs
+        ray.o = frag.pos + v4(0.01f * (ior.inside() ? -frag.nrm : frag.nrm), 0.0f);
         ray.d = smp.L;
         Hit nh;
         c.ext++;
@@ -1301,6 +1386,31 @@ void cpuref_triangle_areas(uint32_t n_tris, const float* p, const uint32_t* idx,
     for (uint32_t t = 0; t < n_tris; ++t) {
         auto P = [&](uint32_t k) { uint32_t v = idx[3 * t + k]; return V3{p[3 * v], p[3 * v + 1], p[3 * v + 2]}; };
         areas[t] = tri_area(P(0), P(1), P(2));
+    }
+}
+
+/* BTDF samples: each from a ray whose IOR stack is the camera's 1.0, plus
+ * the material's index when inside[i]; depth_after[i] is the stack size after. */
+void cpuref_btdf_sample(uint32_t bsdf, uint32_t mf, const pico_material* m, uint32_t n, const float* v, const float* nrm,
+                        const uint8_t* inside, uint32_t rng_mode, uint64_t seed, float* l, float* p, float* energy,
+                        uint32_t* depth_after) {
+    Rng rng;
+    if (rng_mode == CPUREF_RNG_REFERENCE) {
+        rng.seed_reference((uint32_t)seed);
+    } else {
+        rng.mode = CPUREF_RNG_PCG;
+        rng.hs.seed(seed, 1);
+        rng.ms.seed(seed ^ 0xA0761D6478BD642FULL, 2);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        IorStack st;
+        if (inside[i]) st.push(m->index_of_refraction);
+        const V3 V = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+        Sample s = any_sample(bsdf, mf, *m, {nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]}, -V, rng, st);
+        l[3 * i] = s.L.x; l[3 * i + 1] = s.L.y; l[3 * i + 2] = s.L.z;
+        p[i] = s.P;
+        energy[3 * i] = s.energy.x; energy[3 * i + 1] = s.energy.y; energy[3 * i + 2] = s.energy.z;
+        depth_after[i] = (uint32_t)st.v.size();
     }
 }
 

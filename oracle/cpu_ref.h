@@ -77,6 +77,12 @@ void cpuref_alias_build(uint32_t n, const float* weights, float* bucket_weight, 
 uint32_t cpuref_alias_sample(uint32_t n, const float* bucket_weight, const uint32_t* bucket_alias,
                              uint32_t seed, uint32_t draws, uint32_t* out);
 void cpuref_triangle_areas(uint32_t n_tris, const float* positions, const uint32_t* indices, float* areas);
+/* Transparent / Fresnel samples from rays whose IOR stack is the camera's
+ * 1.0 plus the material's index where inside[i]; depth_after[i] is the
+ * stack size after the sample. */
+void cpuref_btdf_sample(uint32_t bsdf, uint32_t microfacet, const pico_material* m, uint32_t n, const float* v,
+                        const float* nrm, const uint8_t* inside, uint32_t rng_mode, uint64_t seed, float* l, float* p,
+                        float* energy, uint32_t* depth_after);
 /* BSDF sample at a shading point with a fixed random sequence (PCG). */
 void cpuref_bsdf_sample(uint32_t bsdf, uint32_t microfacet, const pico_material* m, uint32_t n,
                         const float* v, const float* nrm, uint32_t rng_mode, uint64_t seed, float* l, float* p,

@@ -35,6 +35,7 @@
 #include "Core/UpperLevelBVH.hpp"
 #include "Core/vectorUtils.hpp"
 #include "Render/BSRDF.hpp"
+#include "Render/BasicMaterials.hpp"
 #include "Render/Distributions.hpp"
 #include "Render/Integrators.hpp"
 #include "Util/AliasTable.hpp"
@@ -61,6 +62,28 @@ public:
 private:
     Core::EvaluatedMaterial m_;
 };
+
+// A constant material with an index of refraction: the transmissive BSDFs
+// read it through Render::ConstantTransparentMaterial (BSRDF.hpp:193-199).
+class FixedTransparentMaterial : public Render::ConstantTransparentMaterial {
+public:
+    FixedTransparentMaterial(const Core::EvaluatedMaterial& m, float ior) : ConstantTransparentMaterial(1.0f, ior), m_(m) {}
+    Core::EvaluatedMaterial evaluate_material(const glm::vec2&) const override { return m_; }
+    bool is_light() const override { return m_.emits_light(); }
+private:
+    Core::EvaluatedMaterial m_;
+};
+
+std::unique_ptr<Core::Material> make_material(const pico_material& pm) {
+    Core::EvaluatedMaterial em{};
+    em.diffuse = glm::vec3(pm.diffuse[0], pm.diffuse[1], pm.diffuse[2]);
+    em.specular = glm::vec3(pm.specular[0], pm.specular[1], pm.specular[2]);
+    em.roughness = pm.roughness;
+    em.normal = glm::vec3(0.0f, 1.0f, 0.0f);
+    em.emissive = glm::vec3(pm.emissive[0], pm.emissive[1], pm.emissive[2]);
+    if (pm.index_of_refraction > 0.0f) return std::make_unique<FixedTransparentMaterial>(em, pm.index_of_refraction);
+    return std::make_unique<FixedMaterial>(em);
+}
 
 uint64_t mix64(uint64_t x) {
     x += 0x9e3779b97f4a7c15ULL;
@@ -131,14 +154,7 @@ ref_scene* ref_scene_create(const pico_scene_desc* d) {
     }
     std::vector<Core::MaterialManager::MaterialID> mids;
     for (uint32_t i = 0; i < d->n_materials; ++i) {
-        const pico_material& pm = d->materials[i];
-        Core::EvaluatedMaterial em{};
-        em.diffuse = glm::vec3(pm.diffuse[0], pm.diffuse[1], pm.diffuse[2]);
-        em.specular = glm::vec3(pm.specular[0], pm.specular[1], pm.specular[2]);
-        em.roughness = pm.roughness;
-        em.normal = glm::vec3(0.0f, 1.0f, 0.0f);
-        em.emissive = glm::vec3(pm.emissive[0], pm.emissive[1], pm.emissive[2]);
-        std::unique_ptr<Core::Material> mat = std::make_unique<FixedMaterial>(em);
+        std::unique_ptr<Core::Material> mat = make_material(d->materials[i]);
         mids.push_back(s->mm.add_material(mat));
     }
     std::vector<char> is_light(d->n_instances, 0);
@@ -172,6 +188,17 @@ ref_scene* ref_scene_create(const pico_scene_desc* d) {
             case PICO_BSDF_LIGHT:
                 b = std::make_unique<Render::Light_BRDF>(s->mm, mat);
                 break;
+            case PICO_BSDF_TRANSPARENT: {
+                std::unique_ptr<Render::Distribution> dist = std::make_unique<Render::Beckmann_All_Microfacet_Distribution>();
+                b = std::make_unique<Render::Transparent_BTDF>(dist, s->mm, mat);
+                break;
+            }
+            case PICO_BSDF_FRESNEL: {
+                std::unique_ptr<Render::Distribution> sd = std::make_unique<Render::Beckmann_All_Microfacet_Distribution>();
+                std::unique_ptr<Render::Distribution> td = std::make_unique<Render::Beckmann_All_Microfacet_Distribution>();
+                b = std::make_unique<Render::Fresnel_BTDF>(sd, td, s->mm, mat);
+                break;
+            }
             default:
                 b = std::make_unique<Render::Specular_Delta_BRDF>(s->mm, mat);
                 break;
@@ -441,6 +468,36 @@ void ref_bsdf_sample(uint32_t bsdf, const pico_material* pm, uint32_t n, const f
     }
 }
 
+
+/* Transparent / Fresnel BTDF samples with the reference generator: each from
+ * a ray whose IOR stack is the camera's 1.0 (Camera.cpp:23), plus the
+ * material's index when inside[i]; inside_after[i] is Ray::inside_geometry
+ * after the sample. */
+void ref_btdf_sample(uint32_t bsdf, const pico_material* pm, uint32_t n, const float* v, const float* nrm,
+                     const uint8_t* inside, uint32_t seed, float* l, float* p, float* energy, uint8_t* inside_after) {
+    Core::MaterialManager mm;
+    std::unique_ptr<Core::Material> mat = make_material(*pm);
+    auto id = mm.add_material(mat);
+    std::unique_ptr<Render::BSRDF> b;
+    std::unique_ptr<Render::Distribution> d1 = std::make_unique<Render::Beckmann_All_Microfacet_Distribution>();
+    std::unique_ptr<Render::Distribution> d2 = std::make_unique<Render::Beckmann_All_Microfacet_Distribution>();
+    if (bsdf == PICO_BSDF_FRESNEL) b = std::make_unique<Render::Fresnel_BTDF>(d1, d2, mm, id);
+    else b = std::make_unique<Render::Transparent_BTDF>(d1, mm, id);
+    Core::Rand::Hammersley_Generator g(seed);
+    for (uint32_t i = 0; i < n; ++i) {
+        InterpolatedVertex frag{};
+        frag.mNormal = glm::vec3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+        Core::Ray ray{};
+        ray.push_index_of_refraction(1.0f);
+        if (inside[i]) ray.push_index_of_refraction(pm->index_of_refraction);
+        ray.mDirection = -glm::vec3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+        Render::Sample smp = b->sample(g, frag, ray);
+        l[3 * i] = smp.L.x; l[3 * i + 1] = smp.L.y; l[3 * i + 2] = smp.L.z;
+        p[i] = smp.P;
+        energy[3 * i] = smp.energy.x; energy[3 * i + 1] = smp.energy.y; energy[3 * i + 2] = smp.energy.z;
+        inside_after[i] = ray.inside_geometry() ? 1 : 0;
+    }
+}
 
 // Radiance .hdr through the reference's own writer and reader (the file half
 // of Scene::render_scene_to_file, Source/Core/Scene.cpp:236-246).

@@ -31,7 +31,7 @@ PICO_ERR_CANCELLED = -4
 PICO_ERR_LIMIT = -5
 PICO_ERR_NOMEM = -6
 
-BSDF_DIFFUSE, BSDF_SPECULAR, BSDF_DIELECTRIC, BSDF_LIGHT, BSDF_DELTA = range(5)
+BSDF_DIFFUSE, BSDF_SPECULAR, BSDF_DIELECTRIC, BSDF_LIGHT, BSDF_DELTA, BSDF_TRANSPARENT, BSDF_FRESNEL = range(7)
 MICROFACET_BECKMANN_ALL, MICROFACET_GGX = 0, 1
 RENDER_RESUME, RENDER_COUNT = 1, 2
 TUNE_SIDE_STREAM = 1
@@ -48,7 +48,7 @@ class PicoError(RuntimeError):
 
 class Material(C.Structure):
     _fields_ = [("diffuse", C.c_float * 3), ("specular", C.c_float * 3), ("roughness", C.c_float),
-                ("emissive", C.c_float * 3)]
+                ("emissive", C.c_float * 3), ("index_of_refraction", C.c_float)]
 
 
 class SceneDesc(C.Structure):
@@ -173,6 +173,7 @@ def _declare(lib):
         "pico_scene_cornell": (i32, [u32, u32, P(vp)]),
         "pico_scene_instanced": (i32, [u32, u32, u32, u32, u32, f32, u32, P(vp)]),
         "pico_scene_training": (i32, [u32, u32, P(vp)]),
+        "pico_scene_cornell_glass": (i32, [u32, u32, P(vp)]),
         "pico_scene_desc_get": (i32, [vp, P(SceneDesc)]),
         "pico_scene_load_json": (i32, [C.c_char_p, P(vp), C.c_char_p, sz]),
         "pico_scene_camera_count": (i32, [vp, P(u32)]),
@@ -235,6 +236,14 @@ class Scene:
         rc = lib().pico_scene_cornell(seed, microfacet, C.byref(h))
         if rc:
             raise PicoError(rc, "pico_scene_cornell")
+        return Scene(h.value)
+
+    @staticmethod
+    def cornell_glass(seed=1, microfacet=MICROFACET_GGX):
+        h = C.c_void_p()
+        rc = lib().pico_scene_cornell_glass(seed, microfacet, C.byref(h))
+        if rc:
+            raise PicoError(rc, "pico_scene_cornell_glass")
         return Scene(h.value)
 
     @staticmethod

@@ -310,6 +310,7 @@ std::string scene_tables(const pico_scene_desc& d, HostScene& out) {
         const float s = std::sqrt(fdot(pm.specular, pm.specular));
         m.reflectance = s / (s + std::sqrt(fdot(pm.diffuse, pm.diffuse)));
         m.roughness = pm.roughness;
+        m.ior = pm.index_of_refraction;
         out.materials.push_back(m);
     }
     // instances (the BLAS roots are filled in by the builder)
@@ -318,7 +319,10 @@ std::string scene_tables(const pico_scene_desc& d, HostScene& out) {
         const uint32_t m = d.instance_mesh[i];
         if (m >= d.n_meshes) return "instance mesh out of range";
         if (d.instance_material[i] >= d.n_materials) return "instance material out of range";
-        if (d.instance_bsdf[i] > PICO_BSDF_DELTA) return "unknown BSDF kind";
+        if (d.instance_bsdf[i] > PICO_BSDF_FRESNEL) return "unknown BSDF kind";
+        if ((d.instance_bsdf[i] == PICO_BSDF_TRANSPARENT || d.instance_bsdf[i] == PICO_BSDF_FRESNEL) &&
+            !(d.materials[d.instance_material[i]].index_of_refraction > 0.0f))
+            return "a transmissive BSDF needs a material with an index of refraction";
         const float* M = &d.transforms[16 * (size_t)i];
         if (M[3] != 0.0f || M[7] != 0.0f || M[11] != 0.0f || M[15] != 1.0f) return "instance transform must be affine";
         DInstance di{};

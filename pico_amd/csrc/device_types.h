@@ -69,7 +69,7 @@ struct alignas(16) DXform { float m[16]; };
 struct alignas(16) DMaterial {
     float diffuse[3];  float reflectance;   // |spec| / (|spec| + |diff|)
     float specular[3]; float roughness;
-    float emissive[3]; float pad;
+    float emissive[3]; float ior;           // index of refraction (transmissive BSDFs)
 };
 
 struct alignas(16) DLight {

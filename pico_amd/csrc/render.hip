@@ -88,6 +88,7 @@ struct Pool {
     uint32_t* q1;
     uint32_t* shq;
     float4* samp;    // finalized radiance + rays
+    float4* ior;     // index-of-refraction stack ring; its count is in thr.w (bits)
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -317,7 +318,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
             const f3 dir = xyz(P.ray_d[i]);
             float4 r4 = P.rad[i];
             f3 rad = xyz(r4);
-            f3 thr = xyz(P.thr[i]);
+            const float4 t4 = P.thr[i];
+            f3 thr = xyz(t4);
+            IorStack ist;
+            ist.count = __float_as_uint(t4.w);
             const f3 sky = mk(s.sky[0], s.sky[1], s.sky[2]);
             if (inst == 0xffffffffu) {
                 if (b == 0) {
@@ -338,10 +342,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
                 rng.set_streams(key & kPixMask);
                 rng.hs = (uint64_t)rs.x | ((uint64_t)rs.y << 32);
                 rng.ms = (uint64_t)rs.z | ((uint64_t)rs.w << 32);
-                const BSample smp = bsdf_sample(kind, s.microfacet, M, f.n, dir, rng);
+                const bool btdf = kind == PICO_BSDF_TRANSPARENT || kind == PICO_BSDF_FRESNEL;
+                BSample smp;
+                if (btdf) {
+                    const float4 iv = P.ior[i];
+                    ist.v[0] = iv.x; ist.v[1] = iv.y; ist.v[2] = iv.z; ist.v[3] = iv.w;
+                    smp = kind == PICO_BSDF_TRANSPARENT ? btdf_transparent(s.microfacet, M, f.n, dir, rng, ist)
+                                                        : btdf_fresnel(s.microfacet, M, f.n, dir, rng, ist);
+                } else {
+                    smp = bsdf_sample(kind, s.microfacet, M, f.n, dir, rng);
+                }
                 if (kind == PICO_BSDF_LIGHT) rad = rad + thr * smp.energy;  // Integrators.cpp:164-168
-                // next-event estimation, Integrators.cpp:46-128
-                if (kind != PICO_BSDF_LIGHT) {
+                // next-event estimation, Integrators.cpp:46-128 (none at lights and BTDFs)
+                if (kind != PICO_BSDF_LIGHT && !btdf) {
                     const float u = rng.uniform();
                     if (s.n_lights) {
                         uint32_t li = (uint32_t)(u * (float)s.n_lights);
@@ -385,9 +398,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
                     if (!(rng.uniform() > qk)) {
                         thr = thr * (1.0f / qk);
                         thr = thr * (smp.P * smp.energy);
-                        P.ray_o[i] = f4(f.p + 0.01f * f.n, 0.0f);
+                        // Integrators.cpp:199: below the surface while inside geometry
+                        P.ray_o[i] = f4(f.p + 0.01f * (ist.inside() ? -f.n : f.n), 0.0f);
                         P.ray_d[i] = f4(smp.L, 0.0f);
-                        P.thr[i] = f4(thr, 0.0f);
+                        P.thr[i] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(ist.count));
+                        if (btdf) P.ior[i] = make_float4(ist.v[0], ist.v[1], ist.v[2], ist.v[3]);
                         cont = true;
                         r4.w += 1.0f;
                     }
@@ -665,7 +680,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         off += S;
     }
     // buffers
-    char* pb = (char*)scratch(c, "pool", paths * (16 * 11 + 4 * 6) + 4096);
+    char* pb = (char*)scratch(c, "pool", paths * (16 * 12 + 4 * 6) + 8192);
     uint32_t* act = (uint32_t*)scratch(c, "active", pix_total * 8 + 64);
     Ctl* ctl = (Ctl*)scratch(c, "ctl", sizeof(Ctl));
     uint32_t* count0 = (uint32_t*)scratch(c, "count0", pix_total * 4 + 64);
@@ -695,6 +710,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         P.sh_o = (float4*)take(paths * 16); P.sh_d = (float4*)take(paths * 16);
         P.sh_tp = (float4*)take(paths * 16); P.sh_en = (float4*)take(paths * 16);
         P.samp = (float4*)take(paths * 16);
+        P.ior = (float4*)take(paths * 16);
         P.key = (uint32_t*)take(paths * 4); P.hprim = (uint32_t*)take(paths * 4);
         P.q0 = (uint32_t*)take(paths * 4); P.q1 = (uint32_t*)take(paths * 4); P.shq = (uint32_t*)take(paths * 4);
     }

@@ -327,6 +327,28 @@ int pico_scene_cornell(uint32_t seed, uint32_t microfacet, pico_scene** out) {
     return PICO_OK;
 }
 
+int pico_scene_cornell_glass(uint32_t seed, uint32_t microfacet, pico_scene** out) {
+    if (!out || microfacet > PICO_MICROFACET_GGX) return PICO_ERR_ARG;
+    auto* s = new pico_scene();
+    Rng rng(seed, 1);
+    cornell_into(s, rng, false);
+    // instances 5 and 6 are the two boxes: a Fresnel glass block and a clear
+    // Transparent one (water-like index)
+    pico_material glass = metal_rough({0.95f, 0.95f, 0.95f}, 0.0f, 0.1f, {0, 0, 0});
+    glass.diffuse[0] = glass.diffuse[1] = glass.diffuse[2] = 0.95f;  // transmitted energy
+    glass.index_of_refraction = 1.5f;
+    pico_material water = glass;
+    water.roughness = 0.2f;
+    water.index_of_refraction = 1.33f;
+    s->imat[5] = s->add_material(glass);
+    s->ibsdf[5] = PICO_BSDF_FRESNEL;
+    s->imat[6] = s->add_material(water);
+    s->ibsdf[6] = PICO_BSDF_TRANSPARENT;
+    s->microfacet = microfacet;
+    *out = s;
+    return PICO_OK;
+}
+
 int pico_scene_instanced(uint32_t seed, uint32_t n_types, uint32_t n_inst, uint32_t tmin, uint32_t tmax, float extent,
                          uint32_t microfacet, pico_scene** out) {
     if (!out || n_types == 0 || n_inst == 0 || tmin < 2 || tmax < tmin || !(extent > 0) || microfacet > PICO_MICROFACET_GGX)

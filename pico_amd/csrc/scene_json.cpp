@@ -23,13 +23,15 @@
 //     evaluates them (a missing Metalness, uninitialised in the reference,
 //     is 0 here);
 //   - instances with T * R(normalised quaternion x, y, z, w) * S and the
-//     BSRDFs Diffuse, Specular, Dielectric, Light, Delta (Specular and
-//     Dielectric use the reference's Beckmann_All distribution);
+//     BSRDFs Diffuse, Specular, Dielectric, Light, Delta, Transparent and
+//     Fresnel (the microfacet lobes use the reference's Beckmann_All
+//     distribution; the transmissive ones need a material with an
+//     IndexOfRefraction);
 //   - cameras (up (0, 1, 0), aspect 1, far plane 10 unless given; the
 //     reference's rays ignore the near plane and the field of view).
 // Refused with an error naming the entry: image materials (Metalic, Gloss),
-// the Transparent and Fresnel BSRDFs, the built-in Sphere and Cube assets,
-// sky boxes and the sun light, and anything malformed.  A light instance
+// the built-in Sphere and Cube assets, sky boxes and the sun light, and
+// anything malformed.  A light instance
 // whose mesh is already used by another light gets its own copy of the mesh
 // (the device's light tables are per mesh).  Paths are resolved against the
 // scene file's directory, without regard to case when the exact path is
@@ -499,6 +501,10 @@ void load_into(pico_scene& sc, const fs::path& file) {
                 throw LoadError{where + ": a Constant material needs Albedo, Diffuse or Specular"};
             }
             for (int k = 0; k < 3; ++k) m.emissive[k] = emissive[k];
+            if (transparent) {
+                m.index_of_refraction = as_float(*e.get("IndexOfRefraction"), where + "/IndexOfRefraction");
+                if (!(m.index_of_refraction > 0.0f)) throw LoadError{where + "/IndexOfRefraction: must be positive"};
+            }
             mat_of[name] = sc.add_material(m);
         }
     }
@@ -541,8 +547,11 @@ void load_into(pico_scene& sc, const fs::path& file) {
             else if (kind == "Dielectric") bsdf = PICO_BSDF_DIELECTRIC;
             else if (kind == "Light") bsdf = PICO_BSDF_LIGHT;
             else if (kind == "Delta") bsdf = PICO_BSDF_DELTA;
-            else if (kind == "Transparent" || kind == "Fresnel") throw LoadError{where + ": the " + kind + " BSRDF is not supported"};
+            else if (kind == "Transparent") bsdf = PICO_BSDF_TRANSPARENT;
+            else if (kind == "Fresnel") bsdf = PICO_BSDF_FRESNEL;
             else throw LoadError{where + ": unknown BSRDF " + kind};
+            if ((bsdf == PICO_BSDF_TRANSPARENT || bsdf == PICO_BSDF_FRESNEL) && !(sc.mats[material].index_of_refraction > 0.0f))
+                throw LoadError{where + ": the " + kind + " BSRDF needs a material with an IndexOfRefraction"};
             // T * R * S (Scene.cpp:359-361); R = glm::mat4_cast(q)
             const float x = q[0], y = q[1], z = q[2], w = q[3];
             float T[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, position[0], position[1], position[2], 1};

@@ -499,6 +499,102 @@ __device__ __forceinline__ BSample bsdf_sample(uint32_t kind, uint32_t mf, const
     return s;
 }
 
+// The ray's index-of-refraction stack, Ray::push / pop /
+// get_current_index_of_refraction / inside_geometry (Source/Core/AABB.cpp:54-79):
+// `count` entries above the camera's 1.0 (the reference's size - 1), kept in
+// a ring of kIorStack, so nesting up to kIorStack transmissive objects deep
+// is exact.  A pop with nothing above the camera's entry leaves it in place
+// (the reference asserts there).
+constexpr uint32_t kIorStack = 4;  // the selects below assume 4
+struct IorStack {
+    float v[kIorStack];
+    uint32_t count;
+    // selects instead of a dynamic index keep the ring in registers
+    __device__ void push(float x) {
+        const uint32_t k = count % kIorStack;
+        v[0] = k == 0 ? x : v[0];
+        v[1] = k == 1 ? x : v[1];
+        v[2] = k == 2 ? x : v[2];
+        v[3] = k == 3 ? x : v[3];
+        ++count;
+    }
+    __device__ void pop() { if (count) --count; }
+    __device__ float current() const {
+        if (!count) return 1.0f;
+        const uint32_t k = (count - 1) % kIorStack;
+        return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+    }
+    __device__ bool inside() const { return count > 0; }
+};
+
+// Transparent_BTDF::sample, Source/Render/BSRDF.cpp:240-309: a microfacet
+// normal from the specular distribution, the stack pushed on entering and
+// popped on leaving, then glm::refract(view, H, 1 / IOR) as the reference
+// calls it (NaN where the square root is of a negative, which the reference
+// carries on to its NaN-pixel rule).
+__device__ __forceinline__ BSample btdf_transparent(uint32_t mf, const DMaterial& m, f3 N, f3 raydir, Rng& rng,
+                                                    IorStack& st) {
+    const f3 V = -raydir;
+    const Frame f = tangent_frame(V, N);
+    const f3 vt = normalize(to_tangent(f, V));
+    const bool enter = vt.z >= 0.0f;
+    const float2 xi = rng.next2d();
+    const f3 H = spec_sample(mf, xi, vt, m.roughness);
+    const float eta = 1.0f / m.ior;
+    if (enter) st.push(m.ior);
+    else st.pop();
+    const float d = dot(H, vt);
+    const float k = 1.0f - eta * eta * (1.0f - d * d);
+    const f3 L = (vt * eta - H * (eta * d + sqrtf(k))) * (k >= 0.0f ? 1.0f : 0.0f);
+    BSample s;
+    s.L = normalize(to_world(f, L));
+    const float pdf = spec_pdf(mf, H, m.roughness);
+    const f3 wh = normalize(vt + L * eta);
+    const float den = dot(vt, wh) + eta * dot(L, wh);
+    s.P = pdf * fabsf((eta * eta * dot(L, wh)) / (den * den));
+    s.energy = ld3(m.diffuse);
+    return s;
+}
+
+// Fresnel_BTDF::fresnel_factor, Source/Render/BSRDF.cpp:400-421 (dielectric
+// Fresnel reflectance; eta_i is the material's index, eta_t the ray's).
+__device__ __forceinline__ float fresnel_factor(float cos_i, float eta_i, float eta_t) {
+    cos_i = cos_i < -1.0f ? -1.0f : (1.0f < cos_i ? 1.0f : cos_i);
+    if (!(cos_i > 0.0f)) {
+        const float t = eta_i;
+        eta_i = eta_t;
+        eta_t = t;
+        cos_i = fabsf(cos_i);
+    }
+    const float sin_i = sqrtf(smax(0.0f, 1.0f - cos_i * cos_i));
+    const float sin_t = eta_i / eta_t * sin_i;
+    const float cos_t = sqrtf(smax(0.0f, 1.0f - sin_t * sin_t));
+    const float rpa = ((eta_t * cos_i) - (eta_i * cos_t)) / ((eta_t * cos_i) + (eta_i * cos_t));
+    const float rpe = ((eta_i * cos_i) - (eta_t * cos_t)) / ((eta_i * cos_i) + (eta_t * cos_t));
+    return (rpa * rpa + rpe * rpe) / 2;
+}
+
+// Fresnel_BTDF::sample, Source/Render/BSRDF.cpp:335-362: one draw chooses
+// reflection (the Specular BRDF) with the Fresnel reflectance, else
+// transmission (the Transparent BTDF); the pdf is scaled by the choice.
+__device__ __forceinline__ BSample btdf_fresnel(uint32_t mf, const DMaterial& m, f3 N, f3 raydir, Rng& rng,
+                                                IorStack& st) {
+    const f3 V = -raydir;
+    const Frame f = tangent_frame(V, N);
+    const f3 vt = normalize(to_tangent(f, V));
+    const float2 xi = rng.next2d();
+    const float F = fresnel_factor(vt.z, m.ior, st.current());
+    BSample s;
+    if (xi.x < F) {
+        s = bsdf_sample(PICO_BSDF_SPECULAR, mf, m, N, raydir, rng);
+        s.P *= F;
+    } else {
+        s = btdf_transparent(mf, m, N, raydir, rng, st);
+        s.P *= 1.0f - F;
+    }
+    return s;
+}
+
 // BSRDF::pdf (Source/Render/BSRDF.cpp:42-45,87-91,151-155,229-232) at the
 // half vector, as direct_lighting_pdf calls it (Source/Render/Integrators.cpp:13-22).
 __device__ __forceinline__ float direct_pdf(uint32_t kind, uint32_t mf, const DMaterial& m, f3 N, f3 wi, f3 wo) {

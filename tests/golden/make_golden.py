@@ -25,6 +25,7 @@ def scenes():
     return {
         "cornell_beckmann": pa.Scene.cornell(1, pa.MICROFACET_BECKMANN_ALL),
         "instanced_small": pa.Scene.instanced(7, 4, 48, 200, 2000, 12.0, pa.MICROFACET_BECKMANN_ALL),
+        "cornell_glass": pa.Scene.cornell_glass(1, pa.MICROFACET_BECKMANN_ALL),
     }
 
 
@@ -99,6 +100,22 @@ def main():
         R.ref_bsdf_sample(bsdf, m, n, P(v), P(nrm), 777, P(l), P(p), P(e))
         g.update({f"bsdf_{bsdf}_v": v, f"bsdf_{bsdf}_n": nrm, f"bsdf_{bsdf}_l": l, f"bsdf_{bsdf}_p": p,
                   f"bsdf_{bsdf}_e": e})
+    # transmissive BSDFs and the IOR stack (entering from either side, leaving, nested)
+    mg = pa.Material((0.9, 0.95, 1.0), (0.5, 0.5, 0.5), 0.3, (0, 0, 0), 1.45)
+    for bsdf in [pa.BSDF_TRANSPARENT, pa.BSDF_FRESNEL]:
+        rng = np.random.default_rng(30 + bsdf)
+        n = 512
+        nrm = rng.normal(size=(n, 3)).astype(np.float32)
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        side = np.where(rng.uniform(size=n) < 0.5, 1.0, -1.0).astype(np.float32)
+        v = side[:, None] * nrm + 0.9 * rng.normal(size=(n, 3)).astype(np.float32)
+        v = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+        inside = (((v * nrm).sum(1) < 0) | (rng.uniform(size=n) < 0.25)).astype(np.uint8)
+        l, p, e = np.empty((n, 3), np.float32), np.empty(n, np.float32), np.empty((n, 3), np.float32)
+        after = np.empty(n, np.uint8)
+        R.ref_btdf_sample(bsdf, mg, n, P(v), P(nrm), P(inside), 91, P(l), P(p), P(e), P(after))
+        g.update({f"btdf_{bsdf}_v": v, f"btdf_{bsdf}_n": nrm, f"btdf_{bsdf}_inside": inside, f"btdf_{bsdf}_l": l,
+                  f"btdf_{bsdf}_p": p, f"btdf_{bsdf}_e": e, f"btdf_{bsdf}_after": after})
     # camera
     cam = pa.Camera((1.0, 2.0, -3.0), (0.2, -0.1, 1.0), (0.0, 1.0, 0.0), 1.5, 1000.0)
     rng = np.random.default_rng(2)

@@ -68,7 +68,8 @@ def test_cornell_layout():
     assert d.n_instances == 8 and d.n_lights == 1
     bsdf = np.ctypeslib.as_array(C.cast(d.instance_bsdf, C.POINTER(C.c_uint32)), (8,))
     assert list(bsdf) == [pa.BSDF_DIELECTRIC] * 7 + [pa.BSDF_LIGHT]
-    mats = np.ctypeslib.as_array(C.cast(d.materials, C.POINTER(C.c_float)), (d.n_materials * 10,)).reshape(-1, 10)
+    mats = np.ctypeslib.as_array(C.cast(d.materials, C.POINTER(C.c_float)), (d.n_materials * 11,)).reshape(-1, 11)
+    assert C.sizeof(pa.Material) == 44 and (mats[:, 10] == 0).all()  # no index of refraction: opaque
     alb = mats[:7, 0:3] / 0.96
     assert (alb >= 0.2 - 1e-6).all() and (alb <= 0.8 + 1e-6).all()
     assert ((mats[:7, 6] >= 0.1) & (mats[:7, 6] <= 0.9)).all()

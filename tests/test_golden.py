@@ -12,12 +12,14 @@ import pico_amd as pa
 
 G = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"), allow_pickle=False)
 P = oracle._p
-SCENES = ["cornell_beckmann", "instanced_small"]
+SCENES = ["cornell_beckmann", "instanced_small", "cornell_glass"]
 
 
 def _scene(name):
     if name == "cornell_beckmann":
         return pa.Scene.cornell(1, pa.MICROFACET_BECKMANN_ALL)
+    if name == "cornell_glass":
+        return pa.Scene.cornell_glass(1, pa.MICROFACET_BECKMANN_ALL)
     return pa.Scene.instanced(7, 4, 48, 200, 2000, 12.0, pa.MICROFACET_BECKMANN_ALL)
 
 
@@ -88,6 +90,24 @@ def test_bsdf_sample(bsdf):
     np.testing.assert_allclose(l, G[f"bsdf_{bsdf}_l"], rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(p, G[f"bsdf_{bsdf}_p"], rtol=1e-4, atol=1e-6)
     np.testing.assert_array_equal(e, G[f"bsdf_{bsdf}_e"])
+
+
+@pytest.mark.parametrize("bsdf", [pa.BSDF_TRANSPARENT, pa.BSDF_FRESNEL])
+def test_btdf_sample(bsdf):
+    v, nrm = np.ascontiguousarray(G[f"btdf_{bsdf}_v"]), np.ascontiguousarray(G[f"btdf_{bsdf}_n"])
+    inside = np.ascontiguousarray(G[f"btdf_{bsdf}_inside"])
+    n = len(v)
+    m = pa.Material((0.9, 0.95, 1.0), (0.5, 0.5, 0.5), 0.3, (0, 0, 0), 1.45)
+    l, p, e = np.empty((n, 3), np.float32), np.empty(n, np.float32), np.empty((n, 3), np.float32)
+    d = np.empty(n, np.uint32)
+    oracle.cpu_lib().cpuref_btdf_sample(bsdf, pa.MICROFACET_BECKMANN_ALL, m, n, P(v), P(nrm), P(inside),
+                                        oracle.RNG_REFERENCE, 91, P(l), P(p), P(e), P(d))
+    assert np.array_equal((d > 1).astype(np.uint8), G[f"btdf_{bsdf}_after"])
+    ok = ~np.isnan(G[f"btdf_{bsdf}_l"]).any(axis=1)
+    assert np.array_equal(np.isnan(l).any(axis=1), ~ok)
+    np.testing.assert_allclose(l[ok], G[f"btdf_{bsdf}_l"][ok], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(p[ok], G[f"btdf_{bsdf}_p"][ok], rtol=1e-4, atol=1e-6)
+    np.testing.assert_array_equal(e, G[f"btdf_{bsdf}_e"])
 
 
 def test_camera_rays():

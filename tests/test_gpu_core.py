@@ -127,3 +127,24 @@ def test_atrous_fixed_matches_oracle(ctx, sigmas):
     g = d[5].download(np.float32, 3 * w * h).reshape(-1, 3)
     c = oracle.atrous_fixed(rgb, n, pos, alb, w, h, 5, sigmas)
     np.testing.assert_allclose(g, c, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("mf", [pa.MICROFACET_BECKMANN_ALL, pa.MICROFACET_GGX])
+def test_transmissive_bsdfs_pathwise(ctx, mf):
+    """Paths through a Fresnel glass block and a Transparent box (the IOR
+    stack, offsets below the surface inside geometry, no NEE at BTDFs)
+    against the oracle with identical random streams."""
+    scene = pa.Scene.cornell_glass(1, mf)
+    ctx.upload_scene(scene)
+    cam = scene.camera(1.0)
+    p = pa.params(64, 64, 4, 8, frame_seed=13)
+    g = ctx.render_samples(cam, p)
+    c, _ = oracle.CpuScene(scene).render_samples(cam, p, rng=oracle.RNG_PCG)
+    nan_g, nan_c = np.isnan(g), np.isnan(c)
+    assert np.array_equal(nan_g, nan_c)
+    g, c = np.where(nan_g, 0, g), np.where(nan_c, 0, c)
+    rel = (np.abs(g - c) / (np.abs(c) + 1e-3)).max(axis=2)
+    assert (rel < 1e-3).mean() >= 0.995, (rel < 1e-3).mean()
+    # the glass boxes are seen and transmit light
+    inst, _ = ctx.primary_ids(cam, 64, 64)
+    assert np.isin(inst, [5, 6]).mean() > 0.05

@@ -19,6 +19,7 @@ def scenes():
     return {
         "cornell_beckmann": pa.Scene.cornell(1, pa.MICROFACET_BECKMANN_ALL),
         "instanced_small": pa.Scene.instanced(7, 4, 48, 200, 2000, 12.0, pa.MICROFACET_BECKMANN_ALL),
+        "cornell_glass": pa.Scene.cornell_glass(1, pa.MICROFACET_BECKMANN_ALL),
     }
 
 
@@ -112,6 +113,52 @@ def test_bsdf_sample_reference_rng(bsdf):
     np.testing.assert_array_equal(out["cpu"][2], out["ref"][2])
 
 
+@pytest.mark.parametrize("bsdf", [pa.BSDF_TRANSPARENT, pa.BSDF_FRESNEL])
+@pytest.mark.parametrize("roughness", [0.05, 0.4])
+def test_btdf_sample_reference_rng(bsdf, roughness):
+    """Transparent_BTDF / Fresnel_BTDF samples and the IOR stack against the
+    reference's own classes: rays entering (stack [1]) and leaving (stack
+    [1, n]) from both sides of the normal, nested entries (stack [1, n])."""
+    rng = np.random.default_rng(30 + bsdf)
+    n = 4000
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    side = np.where(rng.uniform(size=n) < 0.5, 1.0, -1.0).astype(np.float32)
+    v = side[:, None] * nrm + 0.9 * rng.normal(size=(n, 3)).astype(np.float32)
+    v = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+    leaving = (v * nrm).sum(1) < 0
+    # leaving rays are inside; a quarter of the entering ones are nested
+    inside = (leaving | (rng.uniform(size=n) < 0.25)).astype(np.uint8)
+    m = pa.Material((0.9, 0.95, 1.0), (0.5, 0.5, 0.5), roughness, (0, 0, 0), 1.45)
+    out = {}
+    for who in ("cpu", "ref"):
+        l = np.empty((n, 3), np.float32)
+        p = np.empty(n, np.float32)
+        e = np.empty((n, 3), np.float32)
+        if who == "cpu":
+            d = np.empty(n, np.uint32)
+            oracle.cpu_lib().cpuref_btdf_sample(bsdf, pa.MICROFACET_BECKMANN_ALL, m, n, oracle._p(v), oracle._p(nrm),
+                                               oracle._p(inside), oracle.RNG_REFERENCE, 91, oracle._p(l), oracle._p(p),
+                                               oracle._p(e), oracle._p(d))
+            after = (d > 1).astype(np.uint8)
+        else:
+            after = np.empty(n, np.uint8)
+            oracle.ref_lib().ref_btdf_sample(bsdf, m, n, oracle._p(v), oracle._p(nrm), oracle._p(inside), 91,
+                                             oracle._p(l), oracle._p(p), oracle._p(e), oracle._p(after))
+        out[who] = (l, p, e, after)
+    (la, pa_, ea, aa), (lb, pb, eb, ab) = out["cpu"], out["ref"]
+    assert np.array_equal(aa, ab)
+    assert np.array_equal(np.isnan(la), np.isnan(lb))  # total internal reflection gives NaN in both
+    ok = ~np.isnan(lb).any(axis=1)
+    assert ok.mean() > 0.5
+    np.testing.assert_allclose(la[ok], lb[ok], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(pa_[ok], pb[ok], rtol=1e-4, atol=1e-6)
+    np.testing.assert_array_equal(ea, eb)
+    if bsdf == pa.BSDF_FRESNEL:
+        # both lobes are taken: specular (energy = specular) and transmission (energy = diffuse)
+        assert 0.02 < (ea[:, 0] == np.float32(0.5)).mean() < 0.98
+
+
 def test_camera_rays():
     cam = pa.Camera((1.0, 2.0, -3.0), (0.2, -0.1, 1.0), (0.0, 1.0, 0.0), 1.5, 1000.0)
     rng = np.random.default_rng(2)
@@ -150,7 +197,7 @@ def test_closest_hit_and_primary_ids(scenes, name):
     np.testing.assert_allclose(cnrm[m], rnrm[m], rtol=1e-5, atol=5e-5)
 
 
-@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small"])
+@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "cornell_glass"])
 def test_integrator_per_sample_reference_rng(scenes, name):
     s = scenes[name]
     cam = s.camera(1.0)

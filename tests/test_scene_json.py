@@ -22,6 +22,7 @@ def _desc(scene):
     vofs, tofs = u32(d.mesh_vertex_offset, nm + 1), u32(d.mesh_triangle_offset, nm + 1)
     mats = [(tuple(m.diffuse), tuple(m.specular), m.roughness, tuple(m.emissive))
             for m in (pa.Material * d.n_materials).from_address(d.materials)]
+    ior = [m.index_of_refraction for m in (pa.Material * d.n_materials).from_address(d.materials)]
     return {
         "vofs": vofs, "tofs": tofs,
         "pos": f32(d.positions, 3 * int(vofs[-1])).reshape(-1, 3),
@@ -29,7 +30,7 @@ def _desc(scene):
         "idx": u32(d.indices, 3 * int(tofs[-1])).reshape(-1, 3),
         "xf": f32(d.transforms, 16 * ni).reshape(ni, 4, 4).transpose(0, 2, 1),  # row-major matrices
         "mesh": u32(d.instance_mesh, ni), "mat": u32(d.instance_material, ni), "bsdf": u32(d.instance_bsdf, ni),
-        "lights": u32(d.light_instances, d.n_lights), "mats": mats, "sky": tuple(d.sky_rgb), "microfacet": d.microfacet,
+        "lights": u32(d.light_instances, d.n_lights), "mats": mats, "ior": ior, "sky": tuple(d.sky_rgb), "microfacet": d.microfacet,
     }
 
 
@@ -146,6 +147,7 @@ def test_materials(tmp_path):
     assert m["Glossy"][2] == pytest.approx(0.75)
     assert m["Lamp"][3] == pytest.approx((10, 9, 8))
     assert m["Glass"][3] == (0.0, 0.0, 0.0)  # the transparent forms emit nothing
+    assert d["ior"][sorted(mats).index("Glass")] == pytest.approx(1.5) and d["ior"][0] == 0.0
     assert d["mat"][0] == sorted(mats).index("Metal") and d["bsdf"][0] == pa.BSDF_DIELECTRIC
 
 
@@ -195,7 +197,7 @@ def test_json_syntax(tmp_path):
     ('{"MESH": {"Q": {"Path": "quad.obj"}},\n "MATERIALS": [1, 2,]}', "scene.json:2:"),
     ("unknown_asset", "unknown asset"),
     ("sphere", "built-in Sphere"),
-    ("transparent", "Transparent BSRDF"),
+    ("transparent", "needs a material with an IndexOfRefraction"),
     ("image_material", "image materials"),
     ("skybox", "sky boxes"),
     ("sun", "sun light"),
@@ -238,7 +240,8 @@ def test_errors(tmp_path, case, needle):
 
 def cornell_json(tmp_path):
     """A Cornell box in the reference's scene format: walls turned by
-    quaternions, two boxes from an OBJ with normals, one quad light."""
+    quaternions, two boxes from an OBJ with normals (one of Fresnel glass),
+    one quad light."""
     cube = ["v -0.5 -0.5 -0.5", "v 0.5 -0.5 -0.5", "v 0.5 0.5 -0.5", "v -0.5 0.5 -0.5",
             "v -0.5 -0.5 0.5", "v 0.5 -0.5 0.5", "v 0.5 0.5 0.5", "v -0.5 0.5 0.5",
             "vn 0 0 -1", "vn 0 0 1", "vn -1 0 0", "vn 1 0 0", "vn 0 -1 0", "vn 0 1 0",
@@ -258,7 +261,7 @@ def cornell_json(tmp_path):
                  "Rotation": [-s, 0, 0, s]},
         "box1": {"Asset": "Box", "BSRDF": "Dielectric", "Material": "White", "Position": [-0.35, -0.7, 0.3],
                  "Rotation": [0, 0.2588, 0, 0.9659], "Scale": [0.6, 0.6, 0.6]},
-        "box2": {"Asset": "Box", "BSRDF": "Specular", "Material": "Metal", "Position": [0.35, -0.4, -0.2],
+        "box2": {"Asset": "Box", "BSRDF": "Fresnel", "Material": "Glass", "Position": [0.35, -0.4, -0.2],
                  "Rotation": [0, -0.2588, 0, 0.9659], "Scale": [0.6, 1.2, 0.6]},
         "light": {"Asset": "Lamp", "BSRDF": "Light", "Material": "Lamp", "Position": [0, 0.99, 0],
                   "Scale": [0.5, 1, 0.5], "Rotation": [1, 0, 0, 0]},
@@ -267,6 +270,8 @@ def cornell_json(tmp_path):
             "Red": {"Type": "Constant", "Albedo": [0.7, 0.1, 0.1], "Roughness": 0.5},
             "Green": {"Type": "Constant", "Albedo": [0.1, 0.7, 0.1], "Roughness": 0.4},
             "Metal": {"Type": "Constant", "Albedo": [0.9, 0.8, 0.6], "Metalness": 1.0, "Roughness": 0.3},
+            "Glass": {"Type": "Constant", "Diffuse": [0.95, 0.95, 0.95], "Specular": [0.9, 0.9, 0.9], "Gloss": 0.95,
+                      "IndexOfRefraction": 1.5},
             "Lamp": {"Type": "Constant", "Albedo": [0.8, 0.8, 0.8], "Roughness": 0.5, "Emissive": [12, 12, 12]}}
     cams = {"front": {"Position": [0, 0, -1], "Direction": [0, 0, 1], "Aspect": 1.0, "FarPlane": 100.0}}
     doc = {"GLOBALS": {"Settings": {}}, "MESH": {"Wall": {"Path": "wall.obj"}, "Box": {"Path": "box.obj"},
@@ -279,6 +284,7 @@ def test_cornell_json_hits_match_brute_force(tmp_path):
     s = pa.Scene.load_json(cornell_json(tmp_path))
     d = _desc(s)
     assert len(d["bsdf"]) == 8 and list(d["lights"]) == [6]  # "light" is 7th in name order
+    assert d["bsdf"][2] == pa.BSDF_FRESNEL  # box2 (back, box1, box2, ...)
     rng = np.random.default_rng(1)
     o = rng.uniform(-0.9, 0.9, (3000, 3)).astype(np.float32)
     v = rng.normal(size=(3000, 3))
