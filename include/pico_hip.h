@@ -70,6 +70,15 @@ extern "C" {
  * (Source/Render/Distributions.cpp:31-78); GGX is a new Distribution subclass
  * (Walter et al. 2007) plugged into the same interface
  * (Source/Render/Distributions.hpp:10-21). */
+/* Mesh shapes (pico_scene_desc.mesh_shape): a triangle mesh, or one of the
+ * reference's built-in implicit shapes, LowerLevelSphereBVH(0.5) and
+ * LowerLevelCube (Source/Core/LowerLevelImplicitShapesBVH.cpp:16-113,
+ * registered as "Sphere" and "Cube" by Source/Core/Scene.cpp:49-56), which
+ * have no vertices or triangles of their own. */
+#define PICO_SHAPE_TRIANGLES 0u
+#define PICO_SHAPE_SPHERE    1u   /* centre 0, radius 0.5 */
+#define PICO_SHAPE_CUBE      2u   /* [-0.5, 0.5]^3 */
+
 #define PICO_MICROFACET_BECKMANN_ALL 0u
 #define PICO_MICROFACET_GGX          1u
 
@@ -108,6 +117,7 @@ typedef struct pico_scene_desc {
     const uint32_t* light_instances;
     float           sky_rgb[3];            /* constant sky (1x1 cube map) */
     uint32_t        microfacet;            /* PICO_MICROFACET_* */
+    const uint32_t* mesh_shape;            /* n_meshes PICO_SHAPE_*, or NULL: all triangle meshes */
 } pico_scene_desc;
 
 /* Scene::Camera (Source/Core/Camera.hpp:14-35): pinhole with
@@ -439,6 +449,10 @@ int  pico_scene_cornell(uint32_t seed, uint32_t microfacet, pico_scene** out);
  * block (index 1.5, roughness 0.1) and a Transparent one (index 1.33,
  * roughness 0.2), both with 0.95 transmitted energy. */
 int  pico_scene_cornell_glass(uint32_t seed, uint32_t microfacet, pico_scene** out);
+/* A Cornell room of the built-in implicit shapes: a Dielectric sphere, a
+ * metal (Specular) cube turned 30 degrees, a Fresnel glass sphere, and two
+ * lights, the ceiling quad and a small emitting sphere. */
+int  pico_scene_shapes(uint32_t seed, uint32_t microfacet, pico_scene** out);
 /* Instanced scene: n_mesh_types meshes (icospheres and noise-displaced tori
  * of min_tris..max_tris triangles, geometric), n_instances random affine
  * instances in a box of side `extent`, 3 quad lights and a constant sky. */
@@ -455,9 +469,9 @@ int  pico_scene_training(uint32_t seed, uint32_t microfacet, pico_scene** out);
  * has none, equal corners joined, the file's first mesh), Constant materials,
  * instances T * R(quaternion) * S with the Diffuse, Specular, Dielectric,
  * Light, Delta, Transparent and Fresnel BSRDFs (Beckmann_All distribution),
+ * the built-in Sphere and Cube assets (PICO_SHAPE_SPHERE / _CUBE meshes),
  * named cameras.  Entries of a section are taken in the sorted order of their
- * names.  Image materials, the built-in Sphere and Cube, sky boxes and the
- * sun light are refused.  On error PICO_ERR_ARG and
+ * names.  Image materials, sky boxes and the sun light are refused.  On error PICO_ERR_ARG and
  * the reason (file, line, entry) in `err` (err_len bytes, may be NULL). */
 int  pico_scene_load_json(const char* path, pico_scene** out, char* err, size_t err_len);
 /* The named cameras of a loaded scene, in name order (none for procedural

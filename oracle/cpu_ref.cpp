@@ -351,6 +351,7 @@ struct Hit {
 };
 
 struct Mesh {
+    uint32_t shape = PICO_SHAPE_TRIANGLES;  // a built-in shape has no triangles
     std::vector<V3> pos, nrm;
     std::vector<uint32_t> idx;
     std::vector<Node> nodes;
@@ -406,6 +407,59 @@ struct Mesh {
 
 struct Instance { M4 m, minv; uint32_t mesh, material, bsdf; };
 
+// LowerLevelSphereBVH(0.5) / LowerLevelCube::calculate_intersection,
+// Source/Core/LowerLevelImplicitShapesBVH.cpp:21-48, 75-88, on the object-space
+// ray (direction not renormalised): hit point, normal and parameter.
+static bool implicit_intersect(uint32_t shape, const Ray& o, V4& P, V3& N, float& t) {
+    if (shape == PICO_SHAPE_SPHERE) {
+        const V3 m = xyz(o.o);
+        const float b = dot(m, o.d);
+        const float c = dot(m, m) - 0.5f * 0.5f;
+        if (c > 0.0f && b > 0.0f) return false;
+        const float discr = b * b - c;
+        if (discr < 0.0f) return false;
+        t = -b - std::sqrt(discr);
+        if (t < 0.0f) t = 0.0f;  // the ray starts inside
+        P = {o.o.x + t * o.d.x, o.o.y + t * o.d.y, o.o.z + t * o.d.z, 1.0f};
+        N = normalize(xyz(P));
+        return true;
+    }
+    const AABB box{{-0.5f, -0.5f, -0.5f, 1.0f}, {0.5f, 0.5f, 0.5f, 1.0f}};
+    t = box.hit(o);
+    if (t == INFINITY) return false;
+    P = {o.o.x + o.d.x * t, o.o.y + o.d.y * t, o.o.z + o.d.z * t, 1.0f};
+    // maximum_component_index, Source/Core/AABB.cpp:31-41, and glm::sign
+    const float ax = std::fabs(P.x), ay = std::fabs(P.y), az = std::fabs(P.z);
+    const int k = (ax >= ay && ax >= az) ? 0 : ((ay >= ax && ay >= az) ? 1 : 2);
+    const float c = k == 0 ? P.x : (k == 1 ? P.y : P.z);
+    const float sg = (float)((0.0f < c) - (c < 0.0f));
+    N = {k == 0 ? sg : 0.0f, k == 1 ? sg : 0.0f, k == 2 ? sg : 0.0f};
+    return true;
+}
+
+// sample_geometry of the built-in shapes, Source/Core/LowerLevelImplicitShapesBVH.cpp:57-66
+// (0.5 * uniform_sample_sphere, Source/Core/RandUtils.cpp:43-49) and 90-110.
+static V3 implicit_sample(uint32_t shape, Rng& rng) {
+    if (shape == PICO_SHAPE_SPHERE) {
+        const V2 xi = rng.next2d();
+        const float z = 1 - 2 * xi.x;
+        const float r = std::sqrt(std::max(0.0f, 1.0f - z * z));
+        const float phi = (float)(2 * M_PI * (double)xi.y);
+        return {0.5f * (r * std::cos(phi)), 0.5f * (r * std::sin(phi)), 0.5f * z};
+    }
+    const V2 xi = rng.next2d(), xj = rng.next2d(), xk = rng.next2d();
+    auto sgn = [](float x) { return (float)((0.0f < x) - (x < 0.0f)); };
+    const V3 corner = {sgn(xi.x - 0.5f), sgn(xi.y - 0.5f), sgn(xj.x - 0.5f)};
+    uint32_t zero = (uint32_t)(xj.y * 3.0f);
+    if (zero == 3) zero = 2;
+    float off[3];
+    off[zero] = 0.0f;
+    off[(zero + 1) % 3] = xk.x;
+    off[(zero + 2) % 3] = xk.y;
+    return {corner.x * 0.5f + (-corner.x * off[0]), corner.y * 0.5f + (-corner.y * off[1]),
+            corner.z * 0.5f + (-corner.z * off[2])};
+}
+
 struct Scene {
     std::vector<Mesh> meshes;
     std::vector<Instance> inst;
@@ -427,6 +481,16 @@ static bool instance_hit(const Scene& s, const Ray& r, uint32_t ii, float& dist,
     o.inv = v3(1.0f, 1.0f, 1.0f) / o.d;
     o.len = r.len;
     const Mesh& m = s.meshes[e.mesh];
+    if (m.shape != PICO_SHAPE_TRIANGLES) {
+        V4 P; V3 N; float t;
+        if (!implicit_intersect(m.shape, o, P, N, t)) return false;
+        h.pos = mul(e.m, P);
+        h.nrm = normalize(mul3(e.m, N));
+        V4 dd = h.pos - r.o;
+        dist = std::sqrt((dd.x * dd.x + dd.y * dd.y) + (dd.z * dd.z + dd.w * dd.w));
+        h.inst = ii; h.prim = 0; h.u = t; h.v = 0.0f;
+        return true;
+    }
     // LowerLevelMeshBVH::calculate_intersection -> BVH::get_first_intersection (t from +inf)
     float best = INFINITY, bu = 0, bv = 0;
     uint32_t bp = kInvalid;
@@ -807,13 +871,18 @@ static bool direct_light(const Scene& s, const Hit& frag, V3 wi, V3& rad, float&
     if (li == count) li = count - 1;  // reference decrements the count only; index clamped here
     const Instance& L = s.inst[s.lights[li]];
     const Mesh& lm = s.meshes[L.mesh];
-    // LowerLevelMeshBVH::sample_geometry, Source/Core/LowerLevelMeshBVH.cpp:119-132
-    const uint32_t tri = alias_sample(lm.alias_w, lm.alias_i, rng);
-    const V2 xi = rng.next2d();
-    const float su0 = std::sqrt(xi.x);
-    const V2 b = {1 - su0, xi.y * su0};
-    const uint32_t i0 = lm.idx[3 * tri], i1 = lm.idx[3 * tri + 1], i2 = lm.idx[3 * tri + 2];
-    V3 sp = (((1.0f - b.x - b.y) * lm.pos[i0]) + (b.x * lm.pos[i1])) + (b.y * lm.pos[i2]);
+    V3 sp;
+    if (lm.shape != PICO_SHAPE_TRIANGLES) {
+        sp = implicit_sample(lm.shape, rng);
+    } else {
+        // LowerLevelMeshBVH::sample_geometry, Source/Core/LowerLevelMeshBVH.cpp:119-132
+        const uint32_t tri = alias_sample(lm.alias_w, lm.alias_i, rng);
+        const V2 xi = rng.next2d();
+        const float su0 = std::sqrt(xi.x);
+        const V2 b = {1 - su0, xi.y * su0};
+        const uint32_t i0 = lm.idx[3 * tri], i1 = lm.idx[3 * tri + 1], i2 = lm.idx[3 * tri + 2];
+        sp = (((1.0f - b.x - b.y) * lm.pos[i0]) + (b.x * lm.pos[i1])) + (b.y * lm.pos[i2]);
+    }
     sp = xyz(mul(L.m, v4(sp, 1.0f)));
     const V3 to_light = normalize(sp - xyz(frag.pos));
     if (dot(to_light, frag.nrm) < 0.0f) return false;
@@ -919,6 +988,11 @@ cpuref_scene* cpuref_scene_create(const pico_scene_desc* d, uint32_t buckets) {
     s.meshes.resize(d->n_meshes);
     for (uint32_t mi = 0; mi < d->n_meshes; ++mi) {
         Mesh& m = s.meshes[mi];
+        m.shape = d->mesh_shape ? d->mesh_shape[mi] : PICO_SHAPE_TRIANGLES;
+        if (m.shape != PICO_SHAPE_TRIANGLES) {
+            m.bounds = {{-0.5f, -0.5f, -0.5f, 1.0f}, {0.5f, 0.5f, 0.5f, 1.0f}};  // get_bounds
+            continue;
+        }
         const uint32_t v0 = d->mesh_vertex_offset[mi], v1 = d->mesh_vertex_offset[mi + 1];
         const uint32_t t0 = d->mesh_triangle_offset[mi], t1 = d->mesh_triangle_offset[mi + 1];
         for (uint32_t v = v0; v < v1; ++v) {
@@ -964,7 +1038,7 @@ cpuref_scene* cpuref_scene_create(const pico_scene_desc* d, uint32_t buckets) {
         const uint32_t ii = d->light_instances[l];
         s.lights.push_back(ii);
         Mesh& m = s.meshes[s.inst[ii].mesh];
-        if (m.alias_w.empty()) {
+        if (m.shape == PICO_SHAPE_TRIANGLES && m.alias_w.empty()) {
             std::vector<float> areas;
             for (uint32_t t = 0; t < m.idx.size(); t += 3) areas.push_back(tri_area(m.pos[m.idx[t]], m.pos[m.idx[t + 1]], m.pos[m.idx[t + 2]]));
             alias_build(areas, m.alias_w, m.alias_i);

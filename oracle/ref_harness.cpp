@@ -28,6 +28,7 @@
 
 #include "Core/Camera.hpp"
 #include "Core/Image.hpp"
+#include "Core/LowerLevelImplicitShapesBVH.hpp"
 #include "Core/LowerLevelMeshBVH.hpp"
 #include "Core/MaterialManager.hpp"
 #include "Core/RandUtils.hpp"
@@ -98,7 +99,8 @@ uint32_t pixel_seed(uint32_t frame, uint32_t pixel) {
 }  // namespace
 
 struct ref_scene {
-    std::vector<std::unique_ptr<LowerLevelMeshBVH>> blas;
+    std::vector<std::unique_ptr<LowerLevelBVH>> blas;
+    std::vector<uint32_t> inst_shape;  // PICO_SHAPE_* of each instance's mesh
     UpperLevelBVH tlas;
     Core::MaterialManager mm;
     std::vector<Scene::Light> lights;
@@ -120,6 +122,16 @@ extern "C" {
 ref_scene* ref_scene_create(const pico_scene_desc* d) {
     auto* s = new ref_scene();
     for (uint32_t mi = 0; mi < d->n_meshes; ++mi) {
+        // the built-in shapes, as Scene::Scene registers them (Source/Core/Scene.cpp:49-56)
+        const uint32_t shape = d->mesh_shape ? d->mesh_shape[mi] : PICO_SHAPE_TRIANGLES;
+        if (shape == PICO_SHAPE_SPHERE) {
+            s->blas.push_back(std::make_unique<LowerLevelSphereBVH>(0.5f));
+            continue;
+        }
+        if (shape == PICO_SHAPE_CUBE) {
+            s->blas.push_back(std::make_unique<LowerLevelCube>());
+            continue;
+        }
         const uint32_t v0 = d->mesh_vertex_offset[mi];
         const uint32_t t0 = d->mesh_triangle_offset[mi], t1 = d->mesh_triangle_offset[mi + 1];
         const uint32_t nt = t1 - t0;
@@ -204,6 +216,7 @@ ref_scene* ref_scene_create(const pico_scene_desc* d) {
                 break;
         }
         const Render::BSRDF* raw = b.get();
+        s->inst_shape.push_back(d->mesh_shape ? d->mesh_shape[d->instance_mesh[i]] : PICO_SHAPE_TRIANGLES);
         s->tlas.add_lower_level_bvh(s->blas[d->instance_mesh[i]].get(), M, b);
         s->inst_of[raw] = i;
     }
@@ -231,7 +244,7 @@ static bool ref_hit(const ref_scene* s, Core::Ray& r, InterpolatedVertex& v, flo
     glm::vec4 dd = v.mPosition - r.mOrigin;
     t = glm::length(dd);
     inst = s->inst_of.at(v.m_bsrdf);
-    prim = (uint32_t)std::lround(v.mVertexColour.x);
+    prim = s->inst_shape[inst] ? 0u : (uint32_t)std::lround(v.mVertexColour.x);
     return true;
 }
 

@@ -33,6 +33,7 @@ PICO_ERR_NOMEM = -6
 
 BSDF_DIFFUSE, BSDF_SPECULAR, BSDF_DIELECTRIC, BSDF_LIGHT, BSDF_DELTA, BSDF_TRANSPARENT, BSDF_FRESNEL = range(7)
 MICROFACET_BECKMANN_ALL, MICROFACET_GGX = 0, 1
+SHAPE_TRIANGLES, SHAPE_SPHERE, SHAPE_CUBE = 0, 1, 2
 RENDER_RESUME, RENDER_COUNT = 1, 2
 TUNE_SIDE_STREAM = 1
 TUNE_SHORTEST_FIRST = 2
@@ -57,7 +58,7 @@ class SceneDesc(C.Structure):
                 ("n_instances", C.c_uint32), ("transforms", C.c_void_p), ("instance_mesh", C.c_void_p),
                 ("instance_material", C.c_void_p), ("instance_bsdf", C.c_void_p), ("n_materials", C.c_uint32),
                 ("materials", C.c_void_p), ("n_lights", C.c_uint32), ("light_instances", C.c_void_p),
-                ("sky_rgb", C.c_float * 3), ("microfacet", C.c_uint32)]
+                ("sky_rgb", C.c_float * 3), ("microfacet", C.c_uint32), ("mesh_shape", C.c_void_p)]
 
 
 class Camera(C.Structure):
@@ -174,6 +175,7 @@ def _declare(lib):
         "pico_scene_instanced": (i32, [u32, u32, u32, u32, u32, f32, u32, P(vp)]),
         "pico_scene_training": (i32, [u32, u32, P(vp)]),
         "pico_scene_cornell_glass": (i32, [u32, u32, P(vp)]),
+        "pico_scene_shapes": (i32, [u32, u32, P(vp)]),
         "pico_scene_desc_get": (i32, [vp, P(SceneDesc)]),
         "pico_scene_load_json": (i32, [C.c_char_p, P(vp), C.c_char_p, sz]),
         "pico_scene_camera_count": (i32, [vp, P(u32)]),
@@ -236,6 +238,14 @@ class Scene:
         rc = lib().pico_scene_cornell(seed, microfacet, C.byref(h))
         if rc:
             raise PicoError(rc, "pico_scene_cornell")
+        return Scene(h.value)
+
+    @staticmethod
+    def shapes(seed=1, microfacet=MICROFACET_GGX):
+        h = C.c_void_p()
+        rc = lib().pico_scene_shapes(seed, microfacet, C.byref(h))
+        if rc:
+            raise PicoError(rc, "pico_scene_shapes")
         return Scene(h.value)
 
     @staticmethod

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kRayBlock) void k_gbuffer(DevScene s, DCam cam, flo
         Hit h = trace_closest<kLds, false>(s, cam.pos, d, st, nullptr, overflow);
         f3 n, p, a;
         if (h.inst != 0xffffffffu) {
-            const Frag f = make_frag(s, h);
+            const Frag f = hit_frag(s, h, cam.pos, d);
             n = f.n;
             // view matrix (glm::lookAt) times (p, 1): (c0*x + c1*y) + (c2*z + c3)
             p = mk((v0.x * f.p.x + v1.x * f.p.y) + (v2.x * f.p.z + v3.x),

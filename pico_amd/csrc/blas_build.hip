@@ -72,6 +72,7 @@ struct Geo {
     uint32_t* mbox;         // 6 per mesh, mapped: vertex bounds (the TLAS uses them)
     uint32_t* rbox;         // 12 per mesh, mapped: triangle box union, centroid bounds
     uint32_t* err;
+    const uint32_t* shape;  // per mesh PICO_SHAPE_*, or null
 };
 
 struct Bins;
@@ -278,6 +279,12 @@ __device__ void set_child(DNode& nd, int c, const DBox& b, uint32_t ref) {
 __global__ void k_roots(Geo g, BuildTask* out, uint32_t* out_count, DNode* bnodes, uint32_t* bcount, uint32_t* mesh_root2,
                         float* mesh_box, uint32_t* n_large, uint32_t* n_chunks) {
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < g.n_meshes; m += gridDim.x * blockDim.x) {
+        if (g.shape && g.shape[m] != PICO_SHAPE_TRIANGLES) {
+            // a built-in shape: no BLAS, bounds [-0.5, 0.5]^3
+            for (int k = 0; k < 3; ++k) { mesh_box[6 * (size_t)m + k] = -0.5f; mesh_box[6 * (size_t)m + 3 + k] = 0.5f; }
+            mesh_root2[m] = kImplicitRoot | g.shape[m];
+            continue;
+        }
         for (int k = 0; k < 6; ++k) mesh_box[6 * (size_t)m + k] = unmap_f(g.mbox[6 * (size_t)m + k]);
         const uint32_t b = g.tofs[m], e = g.tofs[m + 1];
         if (b == e) {
@@ -773,6 +780,10 @@ __global__ void k_roots4(const uint32_t* mesh_root2, uint32_t n_meshes, WideTask
 __global__ void k_wide_level(Wide W) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < W.n_in; i += gridDim.x * blockDim.x) {
         const WideTask w = W.in[i];
+        if (w.bnode >= kImplicitRoot) {  // a built-in shape's root passes through
+            W.mesh_root4[w.slot & ~kRootSlot] = w.bnode;
+            continue;
+        }
         uint32_t ref[4];
         float lo[4][3], hi[4][3];
         int nc = 0;
@@ -906,6 +917,7 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     uint32_t* vofs = tmp.get<uint32_t>(nm + 1);
     uint32_t* tofs = tmp.get<uint32_t>(nm + 1);
     float* xform = tmp.get<float>(16 * (size_t)ni);
+    uint32_t* shapes = d.mesh_shape ? tmp.get<uint32_t>(nm) : nullptr;
     float* pbox = tmp.get<float>(6 * (size_t)nt);
     float* pcen = tmp.get<float>(3 * (size_t)nt);
     uint32_t* perm[2] = {tmp.get<uint32_t>(nt), tmp.get<uint32_t>(nt)};
@@ -930,13 +942,17 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     PICO_HIP(c, hipMemcpyAsync(vofs, d.mesh_vertex_offset, 4 * ((size_t)nm + 1), hipMemcpyHostToDevice, c->stream));
     PICO_HIP(c, hipMemcpyAsync(tofs, d.mesh_triangle_offset, 4 * ((size_t)nm + 1), hipMemcpyHostToDevice, c->stream));
     PICO_HIP(c, hipMemcpyAsync(xform, d.transforms, 64 * (size_t)ni, hipMemcpyHostToDevice, c->stream));
+    if (d.mesh_shape) {
+        if (!shapes) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+        PICO_HIP(c, hipMemcpyAsync(shapes, d.mesh_shape, 4 * (size_t)nm, hipMemcpyHostToDevice, c->stream));
+    }
     PICO_HIP(c, hipMemsetAsync(counters, 0, 64, c->stream));
     hipEvent_t e0, e1;
     PICO_HIP(c, hipEventCreate(&e0));
     PICO_HIP(c, hipEventCreate(&e1));
     struct Ev { hipEvent_t a, b; ~Ev() { hipEventDestroy(a); hipEventDestroy(b); } } ev{e0, e1};
     PICO_HIP(c, hipEventRecord(e0, c->stream));
-    Geo g{s.pos, lidx, gidx, vofs, tofs, nm, nt, nv, pbox, pcen, perm[0], mbox, rbox, counters};
+    Geo g{s.pos, lidx, gidx, vofs, tofs, nm, nt, nv, pbox, pcen, perm[0], mbox, rbox, counters, shapes};
     PICO_LAUNCH(c, "bvh_geo_clear", k_geo_clear, grid_for(nm), 256, 0, g);
     PICO_LAUNCH(c, "bvh_geo_tris", k_geo_tris, grid_for(nt), 256, 0, g);
     PICO_LAUNCH(c, "bvh_tri_bounds", k_tri_bounds, runs_grid(nt), 256, 0, g);

@@ -298,8 +298,17 @@ std::string check_scene(const pico_scene_desc& d) {
         if (d.mesh_vertex_offset[m + 1] < d.mesh_vertex_offset[m] || d.mesh_triangle_offset[m + 1] < d.mesh_triangle_offset[m])
             return "mesh offsets must be non-decreasing";
     if (d.mesh_triangle_offset[d.n_meshes] > kPrimMask || d.n_instances > kPrimMask) return "too many triangles or instances";
+    if (d.mesh_shape)
+        for (uint32_t m = 0; m < d.n_meshes; ++m) {
+            if (d.mesh_shape[m] > PICO_SHAPE_CUBE) return "unknown mesh shape";
+            if (d.mesh_shape[m] != PICO_SHAPE_TRIANGLES &&
+                (d.mesh_vertex_offset[m + 1] != d.mesh_vertex_offset[m] || d.mesh_triangle_offset[m + 1] != d.mesh_triangle_offset[m]))
+                return "a built-in shape has no vertices or triangles";
+        }
     return "";
 }
+
+uint32_t mesh_shape(const pico_scene_desc& d, uint32_t m) { return d.mesh_shape ? d.mesh_shape[m] : PICO_SHAPE_TRIANGLES; }
 
 std::string scene_tables(const pico_scene_desc& d, HostScene& out) {
     // materials
@@ -347,6 +356,11 @@ std::string scene_tables(const pico_scene_desc& d, HostScene& out) {
         if (!light_meshes.insert(m).second) return "lights must not share a mesh";
         const uint32_t t0 = d.mesh_triangle_offset[m], t1 = d.mesh_triangle_offset[m + 1];
         const uint32_t v0 = d.mesh_vertex_offset[m], nv = d.mesh_vertex_offset[m + 1] - v0;
+        if (mesh_shape(d, m) != PICO_SHAPE_TRIANGLES) {
+            // the shape's own sample_geometry: no table
+            out.lights.push_back(DLight{ii, 0u, 0u, mesh_shape(d, m)});
+            continue;
+        }
         if (t1 == t0) return "light mesh has no triangles";
         std::vector<float> w;
         for (uint32_t t = t0; t < t1; ++t) {
@@ -382,6 +396,14 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
     for (uint32_t m = 0; m < d.n_meshes; ++m) {
         const uint32_t v0 = d.mesh_vertex_offset[m], v1 = d.mesh_vertex_offset[m + 1];
         const uint32_t t0 = d.mesh_triangle_offset[m], t1 = d.mesh_triangle_offset[m + 1];
+        if (mesh_shape(d, m) != PICO_SHAPE_TRIANGLES) {
+            // a built-in shape: bounds [-0.5, 0.5]^3 (its get_bounds), no BLAS
+            for (int k = 0; k < 3; ++k) { mesh_box[m].lo[k] = -0.5f; mesh_box[m].hi[k] = 0.5f; }
+            out.mesh_box.insert(out.mesh_box.end(), mesh_box[m].lo, mesh_box[m].lo + 3);
+            out.mesh_box.insert(out.mesh_box.end(), mesh_box[m].hi, mesh_box[m].hi + 3);
+            mesh_root[m] = kImplicitRoot | mesh_shape(d, m);
+            continue;
+        }
         std::vector<float> lo, hi;
         for (uint32_t t = t0; t < t1; ++t) {
             Box b;
@@ -443,6 +465,10 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
         std::vector<uint32_t> root4(d.n_meshes);
         uint32_t blas_bound = 0;
         for (uint32_t m = 0; m < d.n_meshes; ++m) {
+            if (mesh_root[m] >= kImplicitRoot) {
+                root4[m] = mesh_root[m];
+                continue;
+            }
             uint32_t b = 0;
             root4[m] = collapse_bvh4(out.nodes, mesh_root[m], out.nodes4, b);
             blas_bound = std::max(blas_bound, b);

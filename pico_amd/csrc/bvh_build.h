@@ -57,6 +57,7 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
 // checks of the description, and the materials, instances (inverses, no BLAS
 // roots), transforms and light tables.  The device builder uses these.
 std::string check_scene(const pico_scene_desc& d);
+uint32_t mesh_shape(const pico_scene_desc& d, uint32_t mesh);  // PICO_SHAPE_*
 std::string scene_tables(const pico_scene_desc& d, HostScene& out);
 
 }  // namespace pico

@@ -25,6 +25,11 @@ constexpr uint32_t kPrimMask = (1u << 27) - 1u;
 
 inline constexpr uint32_t make_leaf(uint32_t first, uint32_t count) { return kLeafBit | (count << 27) | first; }
 
+// BLAS root of an instance of a built-in implicit shape: kImplicitRoot | PICO_SHAPE_*
+// (no nodes; the traversal intersects the shape directly).  Lights on such a
+// shape have n_tris == 0 and the shape in tri_offset.
+constexpr uint32_t kImplicitRoot = 0xffffff00u;
+
 struct alignas(16) DNode {
     float c0x[2], c0y[2];   // child 0: lo.x hi.x lo.y hi.y
     float c1x[2], c1y[2];   // child 1: lo.x hi.x lo.y hi.y

@@ -233,6 +233,44 @@ __global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ v
 constexpr int kTraceLds = 16;
 constexpr uint32_t kTraceSpill = 48;
 
+// A point on a mesh light: AliasTable::sample (Source/Util/AliasTable.cpp:89-104)
+// then LowerLevelMeshBVH::sample_geometry (Source/Core/LowerLevelMeshBVH.cpp:119-132).
+__device__ __forceinline__ f3 mesh_light_point(const DevScene& s, const DLight& L, Rng& rng) {
+    const uint32_t bucket = rng.next_u32() % L.n_tris;
+    const float coin = (float)rng.next_u32() / 4294967296.0f;
+    const uint32_t tri = coin <= s.alias_w[L.alias_offset + bucket] ? bucket : s.alias_i[L.alias_offset + bucket];
+    const float2 xi = rng.next2d();
+    const float su0 = sqrtf(xi.x);
+    const float bx = 1 - su0, by = xi.y * su0;
+    const uint32_t t = L.tri_offset + tri;
+    const f3 p0 = ld3(s.pos + 3 * s.idx[3 * t]), p1 = ld3(s.pos + 3 * s.idx[3 * t + 1]), p2 = ld3(s.pos + 3 * s.idx[3 * t + 2]);
+    return (((1.0f - bx - by) * p0) + (bx * p1)) + (by * p2);
+}
+
+// A point on a built-in shape light, sample_geometry of
+// Source/Core/LowerLevelImplicitShapesBVH.cpp:57-66 (sphere: 0.5 *
+// uniform_sample_sphere, Source/Core/RandUtils.cpp:43-49) and 90-110 (cube:
+// a corner, a face axis and two offsets from three draws).
+__device__ __forceinline__ f3 implicit_light_point(uint32_t shape, Rng& rng) {
+    if (shape == PICO_SHAPE_SPHERE) {
+        const float2 xi = rng.next2d();
+        const float z = 1 - 2 * xi.x;
+        const float r = sqrtf(smax(0.0f, 1.0f - z * z));
+        const float phi = (float)(2.0 * PICO_PI_D * (double)xi.y);
+        return mk(0.5f * (r * cosf(phi)), 0.5f * (r * sinf(phi)), 0.5f * z);
+    }
+    const float2 xi = rng.next2d(), xj = rng.next2d(), xk = rng.next2d();
+    auto sgn = [](float x) { return (float)((0.0f < x) - (x < 0.0f)); };
+    const f3 corner = mk(sgn(xi.x - 0.5f), sgn(xi.y - 0.5f), sgn(xj.x - 0.5f));
+    uint32_t zero = (uint32_t)(xj.y * 3.0f);
+    if (zero == 3) zero = 2;
+    f3 off;
+    off.x = zero == 0 ? 0.0f : (zero == 1 ? xk.y : xk.x);   // axis (zero + 1) % 3 gets xk.x, (zero + 2) % 3 gets xk.y
+    off.y = zero == 1 ? 0.0f : (zero == 2 ? xk.y : xk.x);
+    off.z = zero == 2 ? 0.0f : (zero == 0 ? xk.y : xk.x);
+    return (corner * 0.5f) + (-corner * off);
+}
+
 // Lane refill for the persistent trace kernels: a wave keeps a reservoir of
 // queue entries (claimed 64 at a time, one atomic) and hands them to idle
 // lanes between traversal steps, so that lanes whose ray finished early trace
@@ -333,7 +371,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
             } else if (b < V.max_depth) {
                 Hit h;
                 h.t = h4.x; h.u = h4.y; h.v = h4.z; h.inst = inst; h.prim = P.hprim[i];
-                const Frag f = make_frag(s, h);
+                const Frag f = s.inst[inst].blas_root >= kImplicitRoot ? implicit_frag(s, h, xyz(P.ray_o[i]), dir)
+                                                                       : make_frag(s, h);
                 const DInstance& I = s.inst[inst];
                 const uint32_t kind = I.bsdf;
                 const DMaterial& M = s.mats[I.material];
@@ -360,18 +399,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
                         uint32_t li = (uint32_t)(u * (float)s.n_lights);
                         if (li >= s.n_lights) li = s.n_lights - 1;
                         const DLight L = s.lights[li];
-                        // AliasTable::sample, Source/Util/AliasTable.cpp:89-104
-                        const uint32_t bucket = rng.next_u32() % L.n_tris;
-                        const float coin = (float)rng.next_u32() / 4294967296.0f;
-                        const uint32_t tri = coin <= s.alias_w[L.alias_offset + bucket] ? bucket : s.alias_i[L.alias_offset + bucket];
-                        // LowerLevelMeshBVH::sample_geometry, LowerLevelMeshBVH.cpp:119-132
-                        const float2 xi = rng.next2d();
-                        const float su0 = sqrtf(xi.x);
-                        const float bx = 1 - su0, by = xi.y * su0;
-                        const uint32_t t = L.tri_offset + tri;
-                        const f3 p0 = ld3(s.pos + 3 * s.idx[3 * t]), p1 = ld3(s.pos + 3 * s.idx[3 * t + 1]),
-                                 p2 = ld3(s.pos + 3 * s.idx[3 * t + 2]);
-                        const f3 sp = (((1.0f - bx - by) * p0) + (bx * p1)) + (by * p2);
+                        const f3 sp = L.n_tris ? mesh_light_point(s, L, rng) : implicit_light_point(L.tri_offset, rng);
                         const float* LM = s.xf[L.instance].m;
                         const f3 wp = mk((LM[0] * sp.x + LM[4] * sp.y) + (LM[8] * sp.z + LM[12]),
                                          (LM[1] * sp.x + LM[5] * sp.y) + (LM[9] * sp.z + LM[13]),

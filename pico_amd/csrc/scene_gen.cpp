@@ -327,6 +327,47 @@ int pico_scene_cornell(uint32_t seed, uint32_t microfacet, pico_scene** out) {
     return PICO_OK;
 }
 
+int pico_scene_shapes(uint32_t seed, uint32_t microfacet, pico_scene** out) {
+    if (!out || microfacet > PICO_MICROFACET_GGX) return PICO_ERR_ARG;
+    auto* s = new pico_scene();
+    Rng rng(seed, 4);
+    const uint32_t wall = add_mesh(s, quad(1.0f));
+    const uint32_t lightm = add_mesh(s, quad(-1.0f));
+    const uint32_t sphere = s->add_shape(PICO_SHAPE_SPHERE);
+    const uint32_t cubes = s->add_shape(PICO_SHAPE_CUBE);
+    const uint32_t lamp = s->add_shape(PICO_SHAPE_SPHERE);  // the spherical light has its own shape entry
+    auto rand_mat = [&]() {
+        F3 a = {rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f)};
+        return s->add_material(metal_rough(a, 0.0f, rng.uniform(0.1f, 0.9f), {0, 0, 0}));
+    };
+    const Mat S = scale({2.0f, 1.0f, 2.0f});
+    add_instance(s, wall, mul(translate({0, -1, 0}), S), rand_mat(), PICO_BSDF_DIELECTRIC);
+    add_instance(s, wall, mul(translate({0, 1, 0}), mul(rot_axis({1, 0, 0}, 180.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);
+    add_instance(s, wall, mul(translate({-1, 0, 0}), mul(rot_axis({0, 0, 1}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);
+    add_instance(s, wall, mul(translate({1, 0, 0}), mul(rot_axis({0, 0, 1}, 90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);
+    add_instance(s, wall, mul(translate({0, 0, 1}), mul(rot_axis({1, 0, 0}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);
+    // a diffuse-and-specular sphere, a mirror-like cube, a glass sphere
+    add_instance(s, sphere, mul(translate({-0.45f, -0.65f, 0.35f}), scale({0.7f, 0.7f, 0.7f})), rand_mat(),
+                 PICO_BSDF_DIELECTRIC);
+    const uint32_t metal = s->add_material(metal_rough({0.9f, 0.8f, 0.6f}, 1.0f, 0.25f, {0, 0, 0}));
+    add_instance(s, cubes, mul(translate({0.4f, -0.75f, 0.3f}), mul(rot_axis({0, 1, 0}, 30.0f), scale({0.5f, 0.5f, 0.5f}))),
+                 metal, PICO_BSDF_SPECULAR);
+    pico_material glass = metal_rough({0.95f, 0.95f, 0.95f}, 0.0f, 0.1f, {0, 0, 0});
+    glass.diffuse[0] = glass.diffuse[1] = glass.diffuse[2] = 0.95f;
+    glass.index_of_refraction = 1.5f;
+    add_instance(s, sphere, mul(translate({0.15f, -0.25f, -0.35f}), scale({0.5f, 0.5f, 0.5f})), s->add_material(glass),
+                 PICO_BSDF_FRESNEL);
+    // two lights: the ceiling quad and a small sphere
+    const uint32_t lm = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {8, 8, 8}));
+    add_instance(s, lightm, mul(translate({0, 0.99f, 0}), scale({0.5f, 1.0f, 0.5f})), lm, PICO_BSDF_LIGHT);
+    const uint32_t lm2 = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {12, 10, 6}));
+    add_instance(s, lamp, mul(translate({-0.55f, 0.55f, -0.25f}), scale({0.2f, 0.2f, 0.2f})), lm2, PICO_BSDF_LIGHT);
+    s->cam = pico_camera{{0.0f, 0.0f, -1.0f}, {0.0f, 0.0f, 1.0f}, {0.0f, 1.0f, 0.0f}, 1.0f, 10000.0f};
+    s->microfacet = microfacet;
+    *out = s;
+    return PICO_OK;
+}
+
 int pico_scene_cornell_glass(uint32_t seed, uint32_t microfacet, pico_scene** out) {
     if (!out || microfacet > PICO_MICROFACET_GGX) return PICO_ERR_ARG;
     auto* s = new pico_scene();
@@ -414,6 +455,7 @@ int pico_scene_desc_get(const pico_scene* s, pico_scene_desc* d) {
     d->light_instances = s->lights.data();
     std::memcpy(d->sky_rgb, s->sky, sizeof(s->sky));
     d->microfacet = s->microfacet;
+    d->mesh_shape = s->shape.data();
     return PICO_OK;
 }
 

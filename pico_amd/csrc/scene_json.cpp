@@ -29,9 +29,10 @@
 //     IndexOfRefraction);
 //   - cameras (up (0, 1, 0), aspect 1, far plane 10 unless given; the
 //     reference's rays ignore the near plane and the field of view).
-// Refused with an error naming the entry: image materials (Metalic, Gloss),
-// the built-in Sphere and Cube assets, sky boxes and the sun light, and
-// anything malformed.  A light instance
+// The built-in assets Sphere (radius 0.5) and Cube ([-0.5, 0.5]^3) become
+// implicit-shape meshes.  Refused with an error naming the entry: image
+// materials (Metalic, Gloss), sky boxes and the sun light, and anything
+// malformed.  A light instance
 // whose mesh is already used by another light gets its own copy of the mesh
 // (the device's light tables are per mesh).  Paths are resolved against the
 // scene file's directory, without regard to case when the exact path is
@@ -512,6 +513,8 @@ void load_into(pico_scene& sc, const fs::path& file) {
     // INSTANCE
     std::vector<uint32_t> mesh_id(meshes.size(), UINT32_MAX);
     std::vector<bool> light_uses(meshes.size(), false);
+    uint32_t builtin_mesh[3] = {UINT32_MAX, UINT32_MAX, UINT32_MAX};
+    bool builtin_light[3] = {false, false, false};
     auto add_mesh = [&](uint32_t m) {
         const MeshData& d = meshes[m];
         return sc.add_mesh(d.pos.data(), d.nrm.data(), d.pos.size() / 3, d.idx.data(), d.idx.size());
@@ -521,9 +524,9 @@ void load_into(pico_scene& sc, const fs::path& file) {
         for (const auto& [name, e] : is->obj) {
             const std::string where = "INSTANCE/" + name;
             const std::string asset = as_string(e.get("Asset"), where + "/Asset");
-            if (asset == "Sphere" || asset == "Cube") throw LoadError{where + ": the built-in " + asset + " is not supported"};
+            const uint32_t builtin = asset == "Sphere" ? PICO_SHAPE_SPHERE : (asset == "Cube" ? PICO_SHAPE_CUBE : 0u);
             auto mi = mesh_of.find(asset);
-            if (mi == mesh_of.end()) throw LoadError{where + ": unknown asset " + asset};
+            if (!builtin && mi == mesh_of.end()) throw LoadError{where + ": unknown asset " + asset};
             float position[3] = {0, 0, 0}, scale[3] = {1, 1, 1}, q[4] = {0, 0, 0, 1};  // q: x, y, z, w
             if (e.get("Position")) vec(e.get("Position"), 3, position, where + "/Position");
             if (e.get("Scale")) vec(e.get("Scale"), 3, scale, where + "/Scale");
@@ -563,15 +566,27 @@ void load_into(pico_scene& sc, const fs::path& file) {
             float TR[16], M[16];
             mat_mul(T, R, TR);
             mat_mul(TR, S, M);
-            const uint32_t m = mi->second;
             uint32_t id;
-            if (bsdf == PICO_BSDF_LIGHT && light_uses[m]) {
-                id = add_mesh(m);  // a second light on this mesh: its own copy
+            if (builtin) {
+                // the built-in shapes (Scene.cpp:49-56): one mesh entry each,
+                // and its own entry for every further light on the shape
+                if (bsdf == PICO_BSDF_LIGHT && builtin_light[builtin]) {
+                    id = sc.add_shape(builtin);
+                } else {
+                    if (builtin_mesh[builtin] == UINT32_MAX) builtin_mesh[builtin] = sc.add_shape(builtin);
+                    id = builtin_mesh[builtin];
+                }
+                if (bsdf == PICO_BSDF_LIGHT) builtin_light[builtin] = true;
             } else {
-                if (mesh_id[m] == UINT32_MAX) mesh_id[m] = add_mesh(m);
-                id = mesh_id[m];
+                const uint32_t m = mi->second;
+                if (bsdf == PICO_BSDF_LIGHT && light_uses[m]) {
+                    id = add_mesh(m);  // a second light on this mesh: its own copy
+                } else {
+                    if (mesh_id[m] == UINT32_MAX) mesh_id[m] = add_mesh(m);
+                    id = mesh_id[m];
+                }
+                if (bsdf == PICO_BSDF_LIGHT) light_uses[m] = true;
             }
-            if (bsdf == PICO_BSDF_LIGHT) light_uses[m] = true;
             sc.add_instance(id, M, material, bsdf);
         }
     }

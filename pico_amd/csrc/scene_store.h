@@ -13,6 +13,7 @@ struct pico_scene {
     std::vector<uint32_t> vofs{0}, tofs{0};
     std::vector<float> pos, nrm, xf;
     std::vector<uint32_t> idx, imesh, imat, ibsdf, lights;
+    std::vector<uint32_t> shape;  // per mesh: PICO_SHAPE_*
     std::vector<pico_material> mats;
     float sky[3] = {0, 0, 0};
     uint32_t microfacet = PICO_MICROFACET_GGX;
@@ -26,6 +27,14 @@ struct pico_scene {
         idx.insert(idx.end(), ix, ix + ni);
         vofs.push_back(vofs.back() + (uint32_t)nv);
         tofs.push_back(tofs.back() + (uint32_t)(ni / 3));
+        shape.push_back(PICO_SHAPE_TRIANGLES);
+        return (uint32_t)vofs.size() - 2;
+    }
+    // a built-in implicit shape (no vertices or triangles)
+    uint32_t add_shape(uint32_t kind) {
+        vofs.push_back(vofs.back());
+        tofs.push_back(tofs.back());
+        shape.push_back(kind);
         return (uint32_t)vofs.size() - 2;
     }
     uint32_t add_material(const pico_material& m) {

@@ -159,6 +159,61 @@ __device__ __forceinline__ float exp2_scale(uint32_t bits) {
     return __int_as_float(((int)(int8_t)(bits & 0xffu) + 127) << 23);
 }
 
+// The reference's order of std::min / std::max (NaN operands fall through as
+// the reference's comparisons make them).
+
+// Object-space ray (M^-1 o, mat3(M^-1) d) of an instance, transform_ray
+// (Source/Core/AABB.cpp:43-52): M^-1 (o, 1) as (c0*x + c1*y) + (c2*z + c3),
+// mat3(M^-1) d as (c0*x + c1*y) + c2*z.
+__device__ __forceinline__ void object_ray(const DInstance* I, f3 o, f3 d, f3& ro, f3& rd) {
+    const float4 m0 = ldg4(&I->inv[0][0]);
+    const float4 m1 = ldg4(&I->inv[1][0]);
+    const float4 m2 = ldg4(&I->inv[2][0]);
+    ro = mk((m0.x * o.x + m0.y * o.y) + (m0.z * o.z + m0.w), (m1.x * o.x + m1.y * o.y) + (m1.z * o.z + m1.w),
+            (m2.x * o.x + m2.y * o.y) + (m2.z * o.z + m2.w));
+    rd = mk((m0.x * d.x + m0.y * d.y) + m0.z * d.z, (m1.x * d.x + m1.y * d.y) + m1.z * d.z,
+            (m2.x * d.x + m2.y * d.y) + m2.z * d.z);
+}
+
+// Built-in implicit shapes, Source/Core/LowerLevelImplicitShapesBVH.cpp:21-88,
+// in object space with the (not renormalised) object-space direction: the
+// sphere's quadratic as the reference writes it, with t clamped to 0 for a
+// ray starting inside, and the cube's slab test AABB::intersection_distance
+// (Source/Core/AABB.cpp:108-147) with 1/d.  Returns the object-space
+// parameter, or false.
+__device__ __forceinline__ bool implicit_hit(uint32_t shape, f3 ro, f3 rd, float& tt) {
+    if (shape == PICO_SHAPE_SPHERE) {
+        const float b = dot(ro, rd);
+        const float c = dot(ro, ro) - 0.5f * 0.5f;
+        if (c > 0.0f && b > 0.0f) return false;
+        const float discr = b * b - c;
+        if (discr < 0.0f) return false;
+        float t = -b - sqrtf(discr);
+        if (t < 0.0f) t = 0.0f;
+        tt = t;
+        return true;
+    }
+    const f3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    const float t1 = (-0.5f - ro.x) * inv.x, t2 = (0.5f - ro.x) * inv.x;
+    const float t3 = (-0.5f - ro.y) * inv.y, t4 = (0.5f - ro.y) * inv.y;
+    const float t5 = (-0.5f - ro.z) * inv.z, t6 = (0.5f - ro.z) * inv.z;
+    const float a1 = t1 > t2 ? t2 : t1, b1 = t1 > t2 ? t1 : t2;
+    const float a2 = t3 > t4 ? t4 : t3, b2 = t3 > t4 ? t3 : t4;
+    const float a3 = t5 > t6 ? t6 : t5, b3 = t5 > t6 ? t5 : t6;
+    const float tmin = smax(smax(a1, a2), a3);
+    const float tmax = smin(smin(b1, b2), b3);
+    if (tmax < 0.0f || tmin > tmax) return false;
+    if (tmin == INFINITY) return false;
+    tt = tmin;
+    return true;
+}
+
+// World hit point of an object-space point: M (p, 1) as (c0*x + c1*y) + (c2*z + c3).
+__device__ __forceinline__ f3 to_world_point(const float* M, f3 P) {
+    return mk((M[0] * P.x + M[4] * P.y) + (M[8] * P.z + M[12]), (M[1] * P.x + M[5] * P.y) + (M[9] * P.z + M[13]),
+              (M[2] * P.x + M[6] * P.y) + (M[10] * P.z + M[14]));
+}
+
 // Two-level closest-hit traversal over 4-wide nodes.  Each node tests its
 // four quantised child boxes (planes origin + q 2^e, which contain the exact
 // boxes) with the slab test, the plane distances taken as
@@ -277,26 +332,36 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& t, Tra
         }
     } else {
         // TLAS leaf: one instance; move the ray into object space
-        t.cur = t.ref & kPrimMask;
-        const DInstance* I = s.inst + t.cur;
-        const float4 m0 = ldg4(&I->inv[0][0]);
-        const float4 m1 = ldg4(&I->inv[1][0]);
-        const float4 m2 = ldg4(&I->inv[2][0]);
+        const uint32_t cur = t.ref & kPrimMask;
+        const DInstance* I = s.inst + cur;
         const uint32_t root = I->blas_root;
         if (COUNT) cnt->insts++;
-        st.push(kSentinel, overflow);
-        // transform_ray (Source/Core/AABB.cpp:43-52): M^-1 (o, 1) as
-        // (c0*x + c1*y) + (c2*z + c3), mat3(M^-1) d as (c0*x + c1*y) + c2*z
-        const f3 o = t.o, d = t.d;
-        t.ro = mk((m0.x * o.x + m0.y * o.y) + (m0.z * o.z + m0.w), (m1.x * o.x + m1.y * o.y) + (m1.z * o.z + m1.w),
-                  (m2.x * o.x + m2.y * o.y) + (m2.z * o.z + m2.w));
-        t.rd = mk((m0.x * d.x + m0.y * d.y) + m0.z * d.z, (m1.x * d.x + m1.y * d.y) + m1.z * d.z,
-                  (m2.x * d.x + m2.y * d.y) + m2.z * d.z);
-        t.inv = slab_rcp(t.rd);
-        t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
-        t.in_blas = true;
-        t.ref = root;
-        return false;
+        f3 ro, rd;
+        object_ray(I, t.o, t.d, ro, rd);
+        if (root >= kImplicitRoot) {
+            // a built-in shape: its hit, at the world distance |M p - o| the
+            // reference compares (Source/Core/UpperLevelBVH.cpp:66-73); u
+            // keeps the object-space parameter for the shading point
+            float tt;
+            if (implicit_hit(root & 0xffu, ro, rd, tt)) {
+                const f3 w = to_world_point(s.xf[cur].m, ro + rd * tt);
+                const f3 dd = w - t.o;
+                const float dist = sqrtf((dd.x * dd.x + dd.y * dd.y) + (dd.z * dd.z + 0.0f));
+                if (dist < t.h.t) {
+                    t.h.t = dist; t.h.u = tt; t.h.v = 0.0f; t.h.inst = cur; t.h.prim = 0;
+                }
+            }
+        } else {
+            t.cur = cur;
+            st.push(kSentinel, overflow);
+            t.ro = ro;
+            t.rd = rd;
+            t.inv = slab_rcp(t.rd);
+            t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
+            t.in_blas = true;
+            t.ref = root;
+            return false;
+        }
     }
     // pop; an instance sentinel restores the world-space ray
     for (;;) {
@@ -329,6 +394,34 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, f3 o, f3 d, Trac
 // normalize(mat3(M)*n).
 struct Frag { f3 p; f3 n; };
 
+// The shading point of a built-in shape from its object-space parameter:
+// the sphere's normal normalize(p), the cube's the signed axis of p's largest
+// component (Source/Core/LowerLevelImplicitShapesBVH.cpp:41-48, 80-88 and
+// maximum_component_index, Source/Core/AABB.cpp:31-41), then the instance
+// transform.
+__device__ __forceinline__ Frag implicit_frag(const DevScene& s, const Hit& h, f3 o, f3 d) {
+    const DInstance* I = s.inst + h.inst;
+    f3 ro, rd;
+    object_ray(I, o, d, ro, rd);
+    const f3 P = ro + rd * h.u;
+    f3 N;
+    if ((I->blas_root & 0xffu) == PICO_SHAPE_SPHERE) {
+        N = normalize(P);
+    } else {
+        const float ax = fabsf(P.x), ay = fabsf(P.y), az = fabsf(P.z);
+        const int k = (ax >= ay && ax >= az) ? 0 : ((ay >= ax && ay >= az) ? 1 : 2);
+        const float c = k == 0 ? P.x : (k == 1 ? P.y : P.z);
+        const float sg = (float)((0.0f < c) - (c < 0.0f));  // glm::sign
+        N = mk(k == 0 ? sg : 0.0f, k == 1 ? sg : 0.0f, k == 2 ? sg : 0.0f);
+    }
+    const float* M = s.xf[h.inst].m;
+    Frag f;
+    f.p = to_world_point(M, P);
+    f.n = normalize(mk(M[0] * N.x + M[4] * N.y + M[8] * N.z, M[1] * N.x + M[5] * N.y + M[9] * N.z,
+                       M[2] * N.x + M[6] * N.y + M[10] * N.z));
+    return f;
+}
+
 __device__ __forceinline__ Frag make_frag(const DevScene& s, const Hit& h) {
     const DInstance& I = s.inst[h.inst];
     const uint32_t t = I.tri_offset + h.prim;
@@ -344,6 +437,11 @@ __device__ __forceinline__ Frag make_frag(const DevScene& s, const Hit& h) {
     f.n = normalize(mk(M[0] * N.x + M[4] * N.y + M[8] * N.z, M[1] * N.x + M[5] * N.y + M[9] * N.z,
                        M[2] * N.x + M[6] * N.y + M[10] * N.z));
     return f;
+}
+
+// The shading point of any hit; (o, d) is the world ray that found it.
+__device__ __forceinline__ Frag hit_frag(const DevScene& s, const Hit& h, f3 o, f3 d) {
+    return s.inst[h.inst].blas_root >= kImplicitRoot ? implicit_frag(s, h, o, d) : make_frag(s, h);
 }
 
 // construct_world_to_tangent_transform, Source/Core/vectorUtils.cpp:9-35: rows (T, B, N).

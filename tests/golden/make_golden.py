@@ -26,6 +26,7 @@ def scenes():
         "cornell_beckmann": pa.Scene.cornell(1, pa.MICROFACET_BECKMANN_ALL),
         "instanced_small": pa.Scene.instanced(7, 4, 48, 200, 2000, 12.0, pa.MICROFACET_BECKMANN_ALL),
         "cornell_glass": pa.Scene.cornell_glass(1, pa.MICROFACET_BECKMANN_ALL),
+        "shapes": pa.Scene.shapes(1, pa.MICROFACET_BECKMANN_ALL),
     }
 
 

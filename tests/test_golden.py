@@ -12,7 +12,7 @@ import pico_amd as pa
 
 G = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"), allow_pickle=False)
 P = oracle._p
-SCENES = ["cornell_beckmann", "instanced_small", "cornell_glass"]
+SCENES = ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes"]
 
 
 def _scene(name):
@@ -20,6 +20,8 @@ def _scene(name):
         return pa.Scene.cornell(1, pa.MICROFACET_BECKMANN_ALL)
     if name == "cornell_glass":
         return pa.Scene.cornell_glass(1, pa.MICROFACET_BECKMANN_ALL)
+    if name == "shapes":
+        return pa.Scene.shapes(1, pa.MICROFACET_BECKMANN_ALL)
     return pa.Scene.instanced(7, 4, 48, 200, 2000, 12.0, pa.MICROFACET_BECKMANN_ALL)
 
 

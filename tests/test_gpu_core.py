@@ -148,3 +148,38 @@ def test_transmissive_bsdfs_pathwise(ctx, mf):
     # the glass boxes are seen and transmit light
     inst, _ = ctx.primary_ids(cam, 64, 64)
     assert np.isin(inst, [5, 6]).mean() > 0.05
+
+
+@pytest.mark.parametrize("mf", [pa.MICROFACET_BECKMANN_ALL, pa.MICROFACET_GGX])
+def test_implicit_shapes_pathwise(ctx, mf):
+    """The built-in sphere and cube (a Dielectric sphere, a metal cube, a
+    Fresnel glass sphere and an emitting sphere sampled by next-event
+    estimation) against the oracle with identical random streams."""
+    scene = pa.Scene.shapes(1, mf)
+    ctx.upload_scene(scene)
+    cam = scene.camera(1.0)
+    p = pa.params(64, 64, 4, 8, frame_seed=17)
+    g = ctx.render_samples(cam, p)
+    c, _ = oracle.CpuScene(scene).render_samples(cam, p, rng=oracle.RNG_PCG)
+    nan_g, nan_c = np.isnan(g), np.isnan(c)
+    assert np.array_equal(nan_g, nan_c)
+    g, c = np.where(nan_g, 0, g), np.where(nan_c, 0, c)
+    rel = (np.abs(g - c) / (np.abs(c) + 1e-3)).max(axis=2)
+    assert (rel < 1e-3).mean() >= 0.995, (rel < 1e-3).mean()
+    inst, _ = ctx.primary_ids(cam, 64, 64)
+    for k in (5, 6, 7):  # each shape is in view
+        assert (inst == k).mean() > 0.01, k
+
+
+def test_implicit_shapes_closest_hit(ctx):
+    scene = pa.Scene.shapes(1, pa.MICROFACET_GGX)
+    o, d = _random_rays(30000, 21, -0.9, 0.9)
+    ct, ci, cp, *_ = oracle.CpuScene(scene).closest_hit(o, d)
+    for device_build in (False, True):
+        ctx.upload_scene(scene, device_build=device_build)
+        t, inst, prim, uv = ctx.closest_hit(o, d)
+        agree = (inst == ci) & (prim == cp)
+        assert agree.mean() >= 0.9999, agree.mean()
+        m = agree & np.isfinite(ct)
+        np.testing.assert_allclose(t[m], ct[m], rtol=1e-4, atol=1e-5)
+        assert np.isin(inst, [5, 6, 7, 9]).mean() > 0.1

@@ -20,6 +20,7 @@ def scenes():
         "cornell_beckmann": pa.Scene.cornell(1, pa.MICROFACET_BECKMANN_ALL),
         "instanced_small": pa.Scene.instanced(7, 4, 48, 200, 2000, 12.0, pa.MICROFACET_BECKMANN_ALL),
         "cornell_glass": pa.Scene.cornell_glass(1, pa.MICROFACET_BECKMANN_ALL),
+        "shapes": pa.Scene.shapes(1, pa.MICROFACET_BECKMANN_ALL),
     }
 
 
@@ -172,7 +173,7 @@ def test_camera_rays():
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small"])
+@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "shapes"])
 def test_closest_hit_and_primary_ids(scenes, name):
     s = scenes[name]
     cs, rs = oracle.CpuScene(s), oracle.RefScene(s)
@@ -197,7 +198,7 @@ def test_closest_hit_and_primary_ids(scenes, name):
     np.testing.assert_allclose(cnrm[m], rnrm[m], rtol=1e-5, atol=5e-5)
 
 
-@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "cornell_glass"])
+@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes"])
 def test_integrator_per_sample_reference_rng(scenes, name):
     s = scenes[name]
     cam = s.camera(1.0)
@@ -220,7 +221,7 @@ def test_render_loop_accumulation(scenes):
     np.testing.assert_allclose(cvar, rvar, rtol=1e-3, atol=1e-5)
 
 
-@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small"])
+@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "shapes"])
 def test_gbuffer(scenes, name):
     s = scenes[name]
     cam = s.camera(1.0)

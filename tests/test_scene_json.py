@@ -177,6 +177,31 @@ def test_order_lights_and_cameras(tmp_path):
     assert tuple(s.camera(2.0).direction) == pytest.approx((0.6, 0.0, 0.8))  # the first camera by name
 
 
+def test_builtin_shapes(tmp_path):
+    inst = {"a_ball": {"Asset": "Sphere", "BSRDF": "Diffuse", "Scale": [2, 2, 2]},
+            "b_box": {"Asset": "Cube", "BSRDF": "Specular"},
+            "c_ball": {"Asset": "Sphere", "BSRDF": "Dielectric"},
+            "d_lamp": {"Asset": "Sphere", "BSRDF": "Light", "Material": "Lamp"},
+            "e_lamp": {"Asset": "Sphere", "BSRDF": "Light", "Material": "Lamp"},
+            "f_quad": {"Asset": "Quad", "BSRDF": "Diffuse"}}
+    mats = dict(MAT, Lamp={"Type": "Constant", "Albedo": [1, 1, 1], "Emissive": [4, 4, 4]})
+    path = _write(tmp_path, _scene({"Quad": "quad.obj"}, inst, materials=mats), {"quad.obj": QUAD})
+    s = pa.Scene.load_json(path)
+    d = _desc(s)
+    shapes = np.ctypeslib.as_array(C.cast(s.desc.mesh_shape, C.POINTER(C.c_uint32)), (s.desc.n_meshes,))
+    kinds = shapes[d["mesh"]]
+    assert list(kinds) == [pa.SHAPE_SPHERE, pa.SHAPE_CUBE, pa.SHAPE_SPHERE, pa.SHAPE_SPHERE, pa.SHAPE_SPHERE,
+                           pa.SHAPE_TRIANGLES]
+    # one entry for the sphere; a second light on it gets its own (lights must not share one)
+    assert d["mesh"][0] == d["mesh"][2] == d["mesh"][3] != d["mesh"][4]
+    assert (np.diff(d["vofs"])[shapes != 0] == 0).all() and (np.diff(d["tofs"])[shapes != 0] == 0).all()
+    # all at the origin: the ray along +z meets the unit cube's face at 4.5;
+    # the sphere scaled by 2 is missed, as in the reference, whose quadratic
+    # takes the object-space direction (here of length 0.5) as unit length
+    t, i, *_ = oracle.CpuScene(s).closest_hit(np.array([[0, 0.1, -5]], np.float32), np.array([[0, 0, 1]], np.float32))
+    assert i[0] == 1 and abs(t[0] - 4.5) < 1e-5
+
+
 def test_case_insensitive_paths(tmp_path):
     path = _write(tmp_path, _scene({"Quad": "Meshes\\QUAD.OBJ"}, {"A": {"Asset": "Quad", "BSRDF": "Diffuse"}}),
                   {"meshes/quad.obj": QUAD})
@@ -196,7 +221,7 @@ def test_json_syntax(tmp_path):
 @pytest.mark.parametrize("case,needle", [
     ('{"MESH": {"Q": {"Path": "quad.obj"}},\n "MATERIALS": [1, 2,]}', "scene.json:2:"),
     ("unknown_asset", "unknown asset"),
-    ("sphere", "built-in Sphere"),
+    ("sphere", "the name of a built-in shape"),
     ("transparent", "needs a material with an IndexOfRefraction"),
     ("image_material", "image materials"),
     ("skybox", "sky boxes"),
@@ -214,7 +239,7 @@ def test_errors(tmp_path, case, needle):
     if case == "unknown_asset":
         inst = {"A": {"Asset": "Nope", "BSRDF": "Diffuse"}}
     elif case == "sphere":
-        inst = {"A": {"Asset": "Sphere", "BSRDF": "Diffuse"}}
+        meshes = {"Sphere": "quad.obj"}
     elif case == "transparent":
         inst = {"A": {"Asset": "Quad", "BSRDF": "Transparent"}}
     elif case == "image_material":
