@@ -450,8 +450,8 @@ int  pico_scene_cornell(uint32_t seed, uint32_t microfacet, pico_scene** out);
  * roughness 0.2), both with 0.95 transmitted energy. */
 int  pico_scene_cornell_glass(uint32_t seed, uint32_t microfacet, pico_scene** out);
 /* A Cornell room of the built-in implicit shapes: a Dielectric sphere, a
- * metal (Specular) cube turned 30 degrees, a Fresnel glass sphere, and two
- * lights, the ceiling quad and a small emitting sphere. */
+ * metal (Specular) cube turned 30 degrees, a Fresnel glass sphere, and three
+ * lights, the ceiling quad, an emitting sphere and a small emitting cube. */
 int  pico_scene_shapes(uint32_t seed, uint32_t microfacet, pico_scene** out);
 /* Instanced scene: n_mesh_types meshes (icospheres and noise-displaced tori
  * of min_tris..max_tris triangles, geometric), n_instances random affine

@@ -335,7 +335,8 @@ int pico_scene_shapes(uint32_t seed, uint32_t microfacet, pico_scene** out) {
     const uint32_t lightm = add_mesh(s, quad(-1.0f));
     const uint32_t sphere = s->add_shape(PICO_SHAPE_SPHERE);
     const uint32_t cubes = s->add_shape(PICO_SHAPE_CUBE);
-    const uint32_t lamp = s->add_shape(PICO_SHAPE_SPHERE);  // the spherical light has its own shape entry
+    const uint32_t lamp_ball = s->add_shape(PICO_SHAPE_SPHERE);  // each light has its own shape entry
+    const uint32_t lamp_box = s->add_shape(PICO_SHAPE_CUBE);
     auto rand_mat = [&]() {
         F3 a = {rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f)};
         return s->add_material(metal_rough(a, 0.0f, rng.uniform(0.1f, 0.9f), {0, 0, 0}));
@@ -346,23 +347,26 @@ int pico_scene_shapes(uint32_t seed, uint32_t microfacet, pico_scene** out) {
     add_instance(s, wall, mul(translate({-1, 0, 0}), mul(rot_axis({0, 0, 1}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);
     add_instance(s, wall, mul(translate({1, 0, 0}), mul(rot_axis({0, 0, 1}, 90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);
     add_instance(s, wall, mul(translate({0, 0, 1}), mul(rot_axis({1, 0, 0}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);
-    // a diffuse-and-specular sphere, a mirror-like cube, a glass sphere
-    add_instance(s, sphere, mul(translate({-0.45f, -0.65f, 0.35f}), scale({0.7f, 0.7f, 0.7f})), rand_mat(),
-                 PICO_BSDF_DIELECTRIC);
+    // Spheres keep unit scale: the reference solves the sphere's quadratic as
+    // if the object-space direction were unit length, which holds only then.
+    // A diffuse-and-specular sphere, a metal cube turned 30 degrees, a glass sphere.
+    add_instance(s, sphere, mul(translate({-0.45f, -0.5f, 0.4f}), rot_axis({0, 1, 0}, 25.0f)), rand_mat(), PICO_BSDF_DIELECTRIC);
     const uint32_t metal = s->add_material(metal_rough({0.9f, 0.8f, 0.6f}, 1.0f, 0.25f, {0, 0, 0}));
-    add_instance(s, cubes, mul(translate({0.4f, -0.75f, 0.3f}), mul(rot_axis({0, 1, 0}, 30.0f), scale({0.5f, 0.5f, 0.5f}))),
+    add_instance(s, cubes, mul(translate({0.45f, -0.75f, 0.35f}), mul(rot_axis({0, 1, 0}, 30.0f), scale({0.5f, 0.5f, 0.5f}))),
                  metal, PICO_BSDF_SPECULAR);
     pico_material glass = metal_rough({0.95f, 0.95f, 0.95f}, 0.0f, 0.1f, {0, 0, 0});
     glass.diffuse[0] = glass.diffuse[1] = glass.diffuse[2] = 0.95f;
     glass.index_of_refraction = 1.5f;
-    add_instance(s, sphere, mul(translate({0.15f, -0.25f, -0.35f}), scale({0.5f, 0.5f, 0.5f})), s->add_material(glass),
-                 PICO_BSDF_FRESNEL);
-    // two lights: the ceiling quad and a small sphere
-    const uint32_t lm = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {8, 8, 8}));
+    add_instance(s, sphere, translate({0.45f, 0.1f, -0.3f}), s->add_material(glass), PICO_BSDF_FRESNEL);
+    // three lights: the ceiling quad, a sphere in a top corner, a small cube
+    const uint32_t lm = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {6, 6, 6}));
     add_instance(s, lightm, mul(translate({0, 0.99f, 0}), scale({0.5f, 1.0f, 0.5f})), lm, PICO_BSDF_LIGHT);
-    const uint32_t lm2 = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {12, 10, 6}));
-    add_instance(s, lamp, mul(translate({-0.55f, 0.55f, -0.25f}), scale({0.2f, 0.2f, 0.2f})), lm2, PICO_BSDF_LIGHT);
-    s->cam = pico_camera{{0.0f, 0.0f, -1.0f}, {0.0f, 0.0f, 1.0f}, {0.0f, 1.0f, 0.0f}, 1.0f, 10000.0f};
+    const uint32_t lm2 = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {3, 2.5f, 1.5f}));
+    add_instance(s, lamp_ball, translate({-0.5f, 0.5f, 0.5f}), lm2, PICO_BSDF_LIGHT);
+    const uint32_t lm3 = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {4, 6, 8}));
+    add_instance(s, lamp_box, mul(translate({-0.6f, 0.6f, -0.3f}), mul(rot_axis({1, 1, 0}, 40.0f), scale({0.2f, 0.2f, 0.2f}))),
+                 lm3, PICO_BSDF_LIGHT);
+    s->cam = pico_camera{{0.0f, 0.0f, -2.6f}, {0.0f, 0.0f, 1.0f}, {0.0f, 1.0f, 0.0f}, 1.0f, 10000.0f};
     s->microfacet = microfacet;
     *out = s;
     return PICO_OK;
