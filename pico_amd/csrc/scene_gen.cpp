@@ -349,10 +349,11 @@ int pico_scene_shapes(uint32_t seed, uint32_t microfacet, pico_scene** out) {
     add_instance(s, wall, mul(translate({0, 0, 1}), mul(rot_axis({1, 0, 0}, -90.0f), S)), rand_mat(), PICO_BSDF_DIELECTRIC);
     // Spheres keep unit scale: the reference solves the sphere's quadratic as
     // if the object-space direction were unit length, which holds only then.
-    // A diffuse-and-specular sphere, a metal cube turned 30 degrees, a glass sphere.
+    // A diffuse-and-specular sphere, a metal cube turned 30 degrees (just
+    // above the floor: coplanar faces would tie), a glass sphere.
     add_instance(s, sphere, mul(translate({-0.45f, -0.5f, 0.4f}), rot_axis({0, 1, 0}, 25.0f)), rand_mat(), PICO_BSDF_DIELECTRIC);
     const uint32_t metal = s->add_material(metal_rough({0.9f, 0.8f, 0.6f}, 1.0f, 0.25f, {0, 0, 0}));
-    add_instance(s, cubes, mul(translate({0.45f, -0.75f, 0.35f}), mul(rot_axis({0, 1, 0}, 30.0f), scale({0.5f, 0.5f, 0.5f}))),
+    add_instance(s, cubes, mul(translate({0.45f, -0.74f, 0.35f}), mul(rot_axis({0, 1, 0}, 30.0f), scale({0.5f, 0.5f, 0.5f}))),
                  metal, PICO_BSDF_SPECULAR);
     pico_material glass = metal_rough({0.95f, 0.95f, 0.95f}, 0.0f, 0.1f, {0, 0, 0});
     glass.diffuse[0] = glass.diffuse[1] = glass.diffuse[2] = 0.95f;
@@ -364,7 +365,7 @@ int pico_scene_shapes(uint32_t seed, uint32_t microfacet, pico_scene** out) {
     const uint32_t lm2 = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {3, 2.5f, 1.5f}));
     add_instance(s, lamp_ball, translate({-0.5f, 0.5f, 0.5f}), lm2, PICO_BSDF_LIGHT);
     const uint32_t lm3 = s->add_material(metal_rough({0.8f, 0.8f, 0.8f}, 0.0f, 0.5f, {4, 6, 8}));
-    add_instance(s, lamp_box, mul(translate({-0.6f, 0.6f, -0.3f}), mul(rot_axis({1, 1, 0}, 40.0f), scale({0.2f, 0.2f, 0.2f}))),
+    add_instance(s, lamp_box, mul(translate({-0.6f, 0.6f, -0.3f}), mul(rot_axis({1.0f, 0.7f, 0.2f}, 37.0f), scale({0.2f, 0.2f, 0.2f}))),
                  lm3, PICO_BSDF_LIGHT);
     s->cam = pico_camera{{0.0f, 0.0f, -2.6f}, {0.0f, 0.0f, 1.0f}, {0.0f, 1.0f, 0.0f}, 1.0f, 10000.0f};
     s->microfacet = microfacet;
