@@ -65,11 +65,6 @@ extern "C" {
 #define PICO_BSDF_TRANSPARENT 5u
 #define PICO_BSDF_FRESNEL     6u
 
-/* Microfacet distribution used by every specular lobe.  BECKMANN_ALL is the
- * reference's Beckmann_All_Microfacet_Distribution
- * (Source/Render/Distributions.cpp:31-78); GGX is a new Distribution subclass
- * (Walter et al. 2007) plugged into the same interface
- * (Source/Render/Distributions.hpp:10-21). */
 /* Mesh shapes (pico_scene_desc.mesh_shape): a triangle mesh, or one of the
  * reference's built-in implicit shapes, LowerLevelSphereBVH(0.5) and
  * LowerLevelCube (Source/Core/LowerLevelImplicitShapesBVH.cpp:16-113,
@@ -79,6 +74,11 @@ extern "C" {
 #define PICO_SHAPE_SPHERE    1u   /* centre 0, radius 0.5 */
 #define PICO_SHAPE_CUBE      2u   /* [-0.5, 0.5]^3 */
 
+/* Microfacet distribution used by every specular lobe.  BECKMANN_ALL is the
+ * reference's Beckmann_All_Microfacet_Distribution
+ * (Source/Render/Distributions.cpp:31-78); GGX is a new Distribution subclass
+ * (Walter et al. 2007) plugged into the same interface
+ * (Source/Render/Distributions.hpp:10-21). */
 #define PICO_MICROFACET_BECKMANN_ALL 0u
 #define PICO_MICROFACET_GGX          1u
 
@@ -93,6 +93,22 @@ typedef struct pico_material {
     float emissive[3];
     float index_of_refraction;
 } pico_material;
+
+/* Sky image, Core::ImageCube (Source/Core/Image.cpp:361-466): looked up
+ * with the nearest texel in the direction of every ray that leaves the scene
+ * and for the G-buffer's sky pixels.  faces = 6: a cube map, faces +x, -x,
+ * +y, -y, +z, -z one after the other (a JSON "Skybox" array); faces = 1: an
+ * equirectangular image (a JSON "Skybox" HDRI file).  Texels are row-major,
+ * the first row first; RGBA8 texels are divided by 255, alpha is unused. */
+#define PICO_TEXEL_RGBA8   0u   /* 4 x u8  */
+#define PICO_TEXEL_RGBA32F 1u   /* 4 x f32 */
+typedef struct pico_sky_image {
+    uint32_t    width;
+    uint32_t    height;
+    uint32_t    faces;      /* 6 or 1 */
+    uint32_t    format;     /* PICO_TEXEL_* */
+    const void* texels;     /* faces x height x width texels */
+} pico_sky_image;
 
 /* Flattened scene.  Meshes are object-space triangle soups with shared
  * vertices; instances reference a mesh, a material and a BSDF kind with a 4x4
@@ -118,6 +134,16 @@ typedef struct pico_scene_desc {
     float           sky_rgb[3];            /* constant sky (1x1 cube map) */
     uint32_t        microfacet;            /* PICO_MICROFACET_* */
     const uint32_t* mesh_shape;            /* n_meshes PICO_SHAPE_*, or NULL: all triangle meshes */
+    /* The sun (Scene::Sun, Source/Core/Scene.hpp:52-58): with use_sun set,
+     * next-event estimation picks among n_lights + 1 lights, the sun last,
+     * and the sun lights a point when a ray from it along -sun_direction
+     * meets nothing (Source/Render/Integrators.cpp:55-77).  sun_direction is
+     * the direction the light travels, used as given. */
+    uint32_t        use_sun;
+    float           sun_direction[3];
+    float           sun_colour[3];
+    /* Sky image replacing the constant sky_rgb, or NULL. */
+    const struct pico_sky_image* sky_image;
 } pico_scene_desc;
 
 /* Scene::Camera (Source/Core/Camera.hpp:14-35): pinhole with
@@ -470,8 +496,10 @@ int  pico_scene_training(uint32_t seed, uint32_t microfacet, pico_scene** out);
  * instances T * R(quaternion) * S with the Diffuse, Specular, Dielectric,
  * Light, Delta, Transparent and Fresnel BSRDFs (Beckmann_All distribution),
  * the built-in Sphere and Cube assets (PICO_SHAPE_SPHERE / _CUBE meshes),
- * named cameras.  Entries of a section are taken in the sorted order of their
- * names.  Image materials, sky boxes and the sun light are refused.  On error PICO_ERR_ARG and
+ * named cameras, and from GLOBALS the sun (SunDirection, SunColour) and the
+ * sky (Skybox: six PNG cube faces, or one PNG or Radiance .hdr
+ * equirectangular image).  Entries of a section are taken in the sorted
+ * order of their names.  Image materials are refused.  On error PICO_ERR_ARG and
  * the reason (file, line, entry) in `err` (err_len bytes, may be NULL). */
 int  pico_scene_load_json(const char* path, pico_scene** out, char* err, size_t err_len);
 /* The named cameras of a loaded scene, in name order (none for procedural
@@ -480,6 +508,12 @@ int  pico_scene_load_json(const char* path, pico_scene** out, char* err, size_t 
 int  pico_scene_camera_count(const pico_scene* scene, uint32_t* count);
 int  pico_scene_camera_get(const pico_scene* scene, uint32_t index, char* name, size_t name_len, pico_camera* out);
 int  pico_scene_desc_get(const pico_scene* scene, pico_scene_desc* out);
+/* The sun and the sky image of a scene, as the command-line options
+ * -SunDirection/-SunColour and -Skybox set them for a scene the reference
+ * imports (Source/Core/Scene.cpp:86-123): direction = colour = NULL turns the
+ * sun off; img = NULL restores the constant sky.  The texels are copied. */
+int  pico_scene_set_sun(pico_scene* scene, const float* direction, const float* colour);
+int  pico_scene_set_sky_image(pico_scene* scene, const pico_sky_image* img);
 /* A camera that frames the scene as its generator intends. */
 int  pico_scene_default_camera(const pico_scene* scene, float aspect, pico_camera* out);
 void pico_scene_destroy(pico_scene* scene);

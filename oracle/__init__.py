@@ -75,6 +75,7 @@ def cpu_lib():
             "cpuref_triangle_areas": (None, [u32, vp, vp, vp]),
             "cpuref_bsdf_sample": (None, [u32, u32, P(Material), u32, vp, vp, u32, u64, vp, vp, vp]),
             "cpuref_btdf_sample": (None, [u32, u32, P(Material), u32, vp, vp, vp, u32, u64, vp, vp, vp, vp]),
+            "cpuref_sky_sample": (None, [vp, vp, u32, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             if hasattr(_cr, name):
@@ -115,6 +116,9 @@ def ref_lib():
             "ref_alias_sample": (None, [u32, vp, u32, u32, vp]),
             "ref_bsdf_sample": (None, [u32, P(Material), u32, vp, vp, u32, vp, vp, vp]),
             "ref_btdf_sample": (None, [u32, P(Material), u32, vp, vp, vp, u32, vp, vp, vp, vp]),
+            "ref_sky_sample": (None, [vp, vp, u32, vp, vp]),
+            "ref_stbi_load": (vp, [C.c_char_p, C.c_int, P(C.c_int), P(C.c_int)]),
+            "ref_stbi_free": (None, [vp]),
             "ref_write_hdr": (i32, [C.c_char_p, i32, i32, vp]),
             "ref_load_hdr": (i32, [C.c_char_p, i32, i32, vp]),
         }
@@ -274,6 +278,22 @@ def ref_load_hdr(path, w, h):
     out = np.empty((w * h, 3), np.float32)
     assert ref_lib().ref_load_hdr(os.fsencode(path), w, h, _p(out)) == 0
     return out
+
+
+def ref_image(path, want_float=False):
+    """The reference's stb_image reading `path` with 4 channels (stbi_load, or
+    stbi_loadf with want_float): [h, w, 4] uint8 / float32, or None."""
+    w, h = C.c_int(), C.c_int()
+    ptr = ref_lib().ref_stbi_load(str(path).encode(), 1 if want_float else 0, C.byref(w), C.byref(h))
+    if not ptr:
+        return None
+    try:
+        n = w.value * h.value * 4
+        ct = C.c_float if want_float else C.c_uint8
+        arr = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), (n,)).copy()
+    finally:
+        ref_lib().ref_stbi_free(ptr)
+    return arr.reshape(h.value, w.value, 4)
 
 
 def ref_tonemap(rgb, w, h):

@@ -469,7 +469,66 @@ struct Scene {
     std::vector<uint32_t> lights;  // instance ids
     V3 sky;
     uint32_t microfacet;
+    bool use_sun = false;
+    V3 sun_dir{0, -1, 0}, sun_col{1, 1, 1};
+    // sky image (width 0: the constant sky)
+    uint32_t sky_w = 0, sky_h = 0, sky_faces = 0, sky_fmt = 0;
+    std::vector<uint8_t> sky_tex;
 };
+
+// ImageCube::sample4 with resolve_cubemap_UV and get_data_ptr,
+// Source/Core/Image.cpp:361-466: the nearest texel in direction d.  The
+// equirectangular form's asin is the double overload (the reference calls
+// asin on a float, which resolves to ::asin(double) there); its atan2 is
+// glm::atan, the float overload.
+static V3 sky_lookup(const Scene& s, V3 d) {
+    if (!s.sky_w) return s.sky;
+    uint32_t face = 0;
+    float u, v;
+    if (s.sky_faces == 6) {
+        const V3 a = {std::fabs(d.x), std::fabs(d.y), std::fabs(d.z)};
+        float ma;
+        V2 uv;
+        if (a.z >= a.x && a.z >= a.y) {
+            face = d.z < 0.0f ? 5 : 4;
+            ma = 0.5f / a.z;
+            uv = {d.z < 0.0f ? -d.x : d.x, -d.y};
+        } else if (a.y >= a.x) {
+            face = d.y < 0.0f ? 3 : 2;
+            ma = 0.5f / a.y;
+            uv = {d.x, d.y < 0.0f ? -d.z : d.z};
+        } else {
+            face = d.x < 0.0f ? 1 : 0;
+            ma = 0.5f / a.x;
+            uv = {d.x < 0.0f ? d.z : -d.z, -d.y};
+        }
+        u = uv.x * ma + 0.5f;
+        v = uv.y * ma + 0.5f;
+    } else {
+        u = std::atan2(d.z, d.x);
+        v = (float)std::asin((double)d.y);
+        u *= (float)(1.0 / (3.14159265358979323846 * 2.0f));
+        v *= (float)(1.0 / 3.14159265358979323846);
+        u += 0.5f;
+        v += 0.5f;
+        v = 1.0f - v;
+    }
+    // uint32_t(x) of a NaN is 0 on the reference's x86-64 build (the 64-bit
+    // truncating conversion gives 2^63, whose low word is 0)
+    auto texel = [](float c, uint32_t n) {
+        const float x = c * (float)n;
+        const uint32_t i = std::isnan(x) ? 0u : (uint32_t)(int64_t)x;
+        return i % n;
+    };
+    const size_t t = ((size_t)face * s.sky_h + texel(v, s.sky_h)) * s.sky_w + texel(u, s.sky_w);
+    if (s.sky_fmt == PICO_TEXEL_RGBA32F) {
+        float f[3];
+        std::memcpy(f, s.sky_tex.data() + 16 * t, 12);
+        return {f[0], f[1], f[2]};
+    }
+    const uint8_t* c = s.sky_tex.data() + 4 * t;
+    return {c[0] / 255.0f, c[1] / 255.0f, c[2] / 255.0f};
+}
 
 // UpperLevelBVH::lower_level_intersector::intersects, Source/Core/UpperLevelBVH.cpp:59-80,
 // and transform_ray, Source/Core/AABB.cpp:43-52.
@@ -858,17 +917,34 @@ static float direct_pdf(uint32_t kind, uint32_t mf, const pico_material& m, V3 N
 
 struct Counters { uint64_t camera = 0, ext = 0, shadow = 0; };
 
-// Monte_Carlo_Integrator::sample_direct_lighting, Source/Render/Integrators.cpp:46-128
-// (no sun: configurations have none).
+// Monte_Carlo_Integrator::sample_direct_lighting, Source/Render/Integrators.cpp:46-128.
 static bool direct_light(const Scene& s, const Hit& frag, V3 wi, V3& rad, float& pdf, Rng& rng, Counters& c) {
     const Instance& fi = s.inst[frag.inst];
     if (fi.bsdf == PICO_BSDF_LIGHT || is_btdf(fi.bsdf)) return false;
     const pico_material& mat = s.mats[fi.material];
-    uint32_t count = (uint32_t)s.lights.size();
+    const uint32_t n_lights = (uint32_t)s.lights.size();
+    uint32_t count = n_lights + (s.use_sun ? 1u : 0u);
     const float u = rng.uniform();
-    if (count == 0) return false;  // the reference indexes an empty light list here
-    uint32_t li = (uint32_t)(u * (float)count);
-    if (li == count) li = count - 1;  // reference decrements the count only; index clamped here
+    const uint32_t pick = (uint32_t)(u * (float)count);
+    if (pick == count) count--;
+    // the sun, Integrators.cpp:61-77: light index >= count - 1 after the
+    // decrement, i.e. pick >= n_lights; lit when nothing is hit at all
+    if (s.use_sun && pick >= count - 1) {
+        const V3 to_sun = -s.sun_dir;
+        Ray r;
+        r.d = to_sun;
+        r.o = frag.pos + v4(0.01f * to_sun, 0.0f);
+        r.len = 10000.0f;
+        Hit h;
+        c.shadow++;
+        if (closest_hit(s, r, h)) return false;
+        pdf = direct_pdf(fi.bsdf, s.microfacet, mat, frag.nrm, wi, to_sun);
+        rad = s.sun_col;
+        return true;
+    }
+    if (n_lights == 0) return false;  // the reference indexes an empty light list here
+    // reference decrements the count only when u * count rounds up; index clamped here
+    const uint32_t li = pick >= n_lights ? n_lights - 1 : pick;
     const Instance& L = s.inst[s.lights[li]];
     const Mesh& lm = s.meshes[L.mesh];
     V3 sp;
@@ -925,7 +1001,7 @@ static V3 integrate(const Scene& s, const pico_camera& cam, uint32_t W, uint32_t
     Ray ray = camera_ray(cam, W, H, rng.next2d(), px, py);
     Hit frag;
     c.camera++;
-    if (!closest_hit(s, ray, frag)) return s.sky;
+    if (!closest_hit(s, ray, frag)) return sky_lookup(s, ray.d);
     V3 payload = {0, 0, 0}, thr = {1, 1, 1};
     IorStack ior;
     for (uint32_t depth = 0;; ++depth) {
@@ -946,7 +1022,7 @@ static V3 integrate(const Scene& s, const pico_camera& cam, uint32_t W, uint32_t
         Hit nh;
         c.ext++;
         if (closest_hit(s, ray, nh)) { frag = nh; continue; }
-        payload = payload + thr * s.sky;
+        payload = payload + thr * sky_lookup(s, smp.L);
         break;
     }
     if (bad(payload)) payload = {1.0f, 0.4f, 0.7f};
@@ -1053,6 +1129,16 @@ cpuref_scene* cpuref_scene_create(const pico_scene_desc* d, uint32_t buckets) {
     s.tlas_depth = b.max_depth_seen;
     s.sky = {d->sky_rgb[0], d->sky_rgb[1], d->sky_rgb[2]};
     s.microfacet = d->microfacet;
+    s.use_sun = d->use_sun != 0;
+    s.sun_dir = {d->sun_direction[0], d->sun_direction[1], d->sun_direction[2]};
+    s.sun_col = {d->sun_colour[0], d->sun_colour[1], d->sun_colour[2]};
+    if (d->sky_image) {
+        const pico_sky_image& k = *d->sky_image;
+        s.sky_w = k.width; s.sky_h = k.height; s.sky_faces = k.faces; s.sky_fmt = k.format;
+        const size_t bytes = (size_t)k.width * k.height * k.faces * (k.format == PICO_TEXEL_RGBA32F ? 16 : 4);
+        const uint8_t* p = static_cast<const uint8_t*>(k.texels);
+        s.sky_tex.assign(p, p + bytes);
+    }
     return out;
 }
 
@@ -1250,7 +1336,7 @@ void cpuref_gbuffer(const cpuref_scene* sc, const pico_camera* cam, uint32_t w, 
             } else {
                 n = -r.d;
                 pp = r.d * r.len;
-                a = s.sky;
+                a = sky_lookup(s, r.d);
             }
             normal[3 * flat] = n.x; normal[3 * flat + 1] = n.y; normal[3 * flat + 2] = n.z;
             position[3 * flat] = pp.x; position[3 * flat + 1] = pp.y; position[3 * flat + 2] = pp.z;
@@ -1465,6 +1551,21 @@ void cpuref_triangle_areas(uint32_t n_tris, const float* p, const uint32_t* idx,
 
 /* BTDF samples: each from a ray whose IOR stack is the camera's 1.0, plus
  * the material's index when inside[i]; depth_after[i] is the stack size after. */
+void cpuref_sky_sample(const pico_sky_image* img, const float* sky_rgb, uint32_t n, const float* dirs, float* out) {
+    Scene s;
+    s.sky = {sky_rgb[0], sky_rgb[1], sky_rgb[2]};
+    if (img) {
+        s.sky_w = img->width; s.sky_h = img->height; s.sky_faces = img->faces; s.sky_fmt = img->format;
+        const size_t bytes = (size_t)img->width * img->height * img->faces * (img->format == PICO_TEXEL_RGBA32F ? 16 : 4);
+        const uint8_t* p = static_cast<const uint8_t*>(img->texels);
+        s.sky_tex.assign(p, p + bytes);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3 c = sky_lookup(s, {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]});
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+}
+
 void cpuref_btdf_sample(uint32_t bsdf, uint32_t mf, const pico_material* m, uint32_t n, const float* v, const float* nrm,
                         const uint8_t* inside, uint32_t rng_mode, uint64_t seed, float* l, float* p, float* energy,
                         uint32_t* depth_after) {

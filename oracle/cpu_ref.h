@@ -80,6 +80,9 @@ void cpuref_triangle_areas(uint32_t n_tris, const float* positions, const uint32
 /* Transparent / Fresnel samples from rays whose IOR stack is the camera's
  * 1.0 plus the material's index where inside[i]; depth_after[i] is the
  * stack size after the sample. */
+/* The sky seen in n directions (ImageCube::sample4, Source/Core/Image.cpp:361-466);
+ * img NULL: the constant sky_rgb. */
+void cpuref_sky_sample(const pico_sky_image* img, const float* sky_rgb, uint32_t n, const float* dirs, float* out);
 void cpuref_btdf_sample(uint32_t bsdf, uint32_t microfacet, const pico_material* m, uint32_t n, const float* v,
                         const float* nrm, const uint8_t* inside, uint32_t rng_mode, uint64_t seed, float* l, float* p,
                         float* energy, uint32_t* depth_after);

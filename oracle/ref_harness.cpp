@@ -21,6 +21,7 @@
  *    instead of std::random_device, so runs are repeatable.
  */
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -44,6 +45,7 @@
 #include "Util/Tiler.hpp"
 #include "Util/ToneMappers.hpp"
 #include "assimp/mesh.h"
+#include "stb_image.h"
 
 #include "../include/pico_hip.h"
 
@@ -108,6 +110,24 @@ struct ref_scene {
     Scene::Sun sun{};
     std::map<const Render::BSRDF*, uint32_t> inst_of;
 };
+
+// The sky as Scene::process_globals makes it (Source/Core/Scene.cpp:639-690):
+// a cube of 6 RGBA8 faces, a single RGBA float image, or here also the
+// constant sky as a 1x1 float image.  Image frees its data with free().
+static std::unique_ptr<Core::ImageCube> make_sky(const pico_sky_image* img, const float* rgb) {
+    if (!img) {
+        float* px = static_cast<float*>(std::malloc(16));
+        px[0] = rgb[0]; px[1] = rgb[1]; px[2] = rgb[2]; px[3] = 1.0f;
+        return std::make_unique<Core::ImageCube>(reinterpret_cast<unsigned char*>(px), Core::ImageExtent{1, 1, 1},
+                                                 Core::Format::kRGBA_F);
+    }
+    const size_t bytes = (size_t)img->width * img->height * img->faces * (img->format == PICO_TEXEL_RGBA32F ? 16 : 4);
+    unsigned char* px = static_cast<unsigned char*>(std::malloc(bytes));
+    std::memcpy(px, img->texels, bytes);
+    return std::make_unique<Core::ImageCube>(px, Core::ImageExtent{img->width, img->height, img->faces},
+                                             img->format == PICO_TEXEL_RGBA32F ? Core::Format::kRGBA_F
+                                                                               : Core::Format::kRBGA_8UNorm);
+}
 
 static Scene::Camera make_camera(const pico_camera* c, uint32_t w, uint32_t h) {
     Scene::Camera cam(glm::vec3(c->position[0], c->position[1], c->position[2]),
@@ -227,17 +247,36 @@ ref_scene* ref_scene_create(const pico_scene_desc* d) {
         s->lights.push_back({xf[ii], glm::inverse(xf[ii]), g});
     }
     s->tlas.build();
-    float* sky = new float[4]{d->sky_rgb[0], d->sky_rgb[1], d->sky_rgb[2], 1.0f};
-    s->sky = std::make_unique<Core::ImageCube>(reinterpret_cast<unsigned char*>(sky), Core::ImageExtent{1, 1, 1},
-                                               Core::Format::kRGBA_F);
+    s->sky = make_sky(d->sky_image, d->sky_rgb);
     s->sun.m_sky_box = s->sky.get();
-    s->sun.m_use_sun = false;
-    s->sun.m_sun_direction = glm::vec3(0.0f, -1.0f, 0.0f);
-    s->sun.m_sun_colour = glm::vec3(1.0f);
+    s->sun.m_use_sun = d->use_sun != 0;
+    s->sun.m_sun_direction = glm::vec3(d->sun_direction[0], d->sun_direction[1], d->sun_direction[2]);
+    s->sun.m_sun_colour = glm::vec3(d->sun_colour[0], d->sun_colour[1], d->sun_colour[2]);
     return s;
 }
 
+/* ImageCube::sample4 of a sky image (or the 1x1 constant sky) for n
+ * directions; rgb out (alpha dropped as the integrator's vec3 does). */
+void ref_sky_sample(const pico_sky_image* img, const float* sky_rgb, uint32_t n, const float* dirs, float* out) {
+    auto sky = make_sky(img, sky_rgb);
+    for (uint32_t i = 0; i < n; ++i) {
+        const glm::vec4 c = sky->sample4(glm::vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+}
+
 void ref_scene_destroy(ref_scene* s) { delete s; }
+
+/* The reference's image reader (its stb_image, compiled from the reference
+ * tree) as Scene::process_globals calls it: stbi_load (want_float 0) or
+ * stbi_loadf (want_float 1) with 4 channels.  Returns stb's buffer (free with
+ * ref_stbi_free) or NULL. */
+void* ref_stbi_load(const char* path, int want_float, int* w, int* h) {
+    int comp = 0;
+    if (want_float) return stbi_loadf(path, w, h, &comp, 4);
+    return stbi_load(path, w, h, &comp, 4);
+}
+void ref_stbi_free(void* p) { stbi_image_free(p); }
 
 static bool ref_hit(const ref_scene* s, Core::Ray& r, InterpolatedVertex& v, float& t, uint32_t& inst, uint32_t& prim) {
     if (!s->tlas.get_closest_intersection(r, &v)) return false;

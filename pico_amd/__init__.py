@@ -33,6 +33,7 @@ PICO_ERR_NOMEM = -6
 
 BSDF_DIFFUSE, BSDF_SPECULAR, BSDF_DIELECTRIC, BSDF_LIGHT, BSDF_DELTA, BSDF_TRANSPARENT, BSDF_FRESNEL = range(7)
 MICROFACET_BECKMANN_ALL, MICROFACET_GGX = 0, 1
+TEXEL_RGBA8, TEXEL_RGBA32F = 0, 1
 SHAPE_TRIANGLES, SHAPE_SPHERE, SHAPE_CUBE = 0, 1, 2
 RENDER_RESUME, RENDER_COUNT = 1, 2
 TUNE_SIDE_STREAM = 1
@@ -58,7 +59,14 @@ class SceneDesc(C.Structure):
                 ("n_instances", C.c_uint32), ("transforms", C.c_void_p), ("instance_mesh", C.c_void_p),
                 ("instance_material", C.c_void_p), ("instance_bsdf", C.c_void_p), ("n_materials", C.c_uint32),
                 ("materials", C.c_void_p), ("n_lights", C.c_uint32), ("light_instances", C.c_void_p),
-                ("sky_rgb", C.c_float * 3), ("microfacet", C.c_uint32), ("mesh_shape", C.c_void_p)]
+                ("sky_rgb", C.c_float * 3), ("microfacet", C.c_uint32), ("mesh_shape", C.c_void_p),
+                ("use_sun", C.c_uint32), ("sun_direction", C.c_float * 3), ("sun_colour", C.c_float * 3),
+                ("sky_image", C.c_void_p)]
+
+
+class SkyImage(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("faces", C.c_uint32), ("format", C.c_uint32),
+                ("texels", C.c_void_p)]
 
 
 class Camera(C.Structure):
@@ -182,6 +190,8 @@ def _declare(lib):
         "pico_scene_camera_get": (i32, [vp, u32, C.c_char_p, sz, P(Camera)]),
         "pico_scene_default_camera": (i32, [vp, f32, P(Camera)]),
         "pico_scene_destroy": (None, [vp]),
+        "pico_scene_set_sun": (i32, [vp, vp, vp]),
+        "pico_scene_set_sky_image": (i32, [vp, P(SkyImage)]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(lib, name):
@@ -282,6 +292,44 @@ class Scene:
         if rc:
             raise PicoError(rc, err.value.decode(errors="replace"))
         return Scene(h.value)
+
+    def _refresh(self):
+        rc = lib().pico_scene_desc_get(self._h, C.byref(self.desc))
+        if rc:
+            raise PicoError(rc, "pico_scene_desc_get")
+
+    def set_sun(self, direction=None, colour=None):
+        """The sun light (direction the light travels, colour), or None: off."""
+        if direction is None:
+            rc = lib().pico_scene_set_sun(self._h, None, None)
+        else:
+            d = (C.c_float * 3)(*direction)
+            c = (C.c_float * 3)(*colour)
+            rc = lib().pico_scene_set_sun(self._h, C.cast(d, C.c_void_p), C.cast(c, C.c_void_p))
+        if rc:
+            raise PicoError(rc, "pico_scene_set_sun")
+        self._refresh()
+        return self
+
+    def set_sky_image(self, texels=None):
+        """The sky image: uint8 [faces, h, w, 4] (RGBA8) or float32 (RGBA32F)
+        with faces 6 (cube: +x, -x, +y, -y, +z, -z) or 1 (equirectangular); a
+        3-D array is one face.  None restores the constant sky."""
+        if texels is None:
+            rc = lib().pico_scene_set_sky_image(self._h, None)
+        else:
+            t = np.ascontiguousarray(texels)
+            if t.ndim == 3:
+                t = t[None]
+            if t.ndim != 4 or t.shape[-1] != 4 or t.dtype not in (np.uint8, np.float32):
+                raise ValueError("sky texels must be [faces, h, w, 4] uint8 or float32")
+            img = SkyImage(t.shape[2], t.shape[1], t.shape[0], TEXEL_RGBA32F if t.dtype == np.float32 else TEXEL_RGBA8,
+                           t.ctypes.data)
+            rc = lib().pico_scene_set_sky_image(self._h, C.byref(img))
+        if rc:
+            raise PicoError(rc, "pico_scene_set_sky_image")
+        self._refresh()
+        return self
 
     def cameras(self):
         """The named cameras of a loaded scene, {name: Camera}, in name order."""

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kRayBlock) void k_gbuffer(DevScene s, DCam cam, flo
         } else {
             n = -d;
             p = d * cam.far_plane;
-            a = mk(s.sky[0], s.sky[1], s.sky[2]);
+            a = sky_radiance(s, d);
         }
         normal[3 * i] = n.x; normal[3 * i + 1] = n.y; normal[3 * i + 2] = n.z;
         position[3 * i] = p.x; position[3 * i + 1] = p.y; position[3 * i + 2] = p.z;
@@ -115,6 +115,38 @@ int upload_vec(pico_hip_ctx* c, const std::vector<T>& v, const T** out, size_t c
     return PICO_OK;
 }
 
+}  // namespace
+
+namespace pico {
+// The sky image's texels (when there is one) into a scene allocation, and the
+// sky and sun parameters into the device scene; the host copy keeps no
+// pointer to the description's texels.
+int upload_sky_sun(pico_hip_ctx* c, HostScene& hs, DevScene& s) {
+    std::memcpy(s.sky, hs.sky, sizeof(s.sky));
+    s.use_sun = hs.use_sun;
+    std::memcpy(s.sun_to, hs.sun_to, sizeof(s.sun_to));
+    std::memcpy(s.sun_col, hs.sun_col, sizeof(s.sun_col));
+    s.sky_tex = nullptr;
+    const pico_sky_image& k = hs.sky_image;
+    if (k.width && k.texels) {
+        const size_t bytes = sky_bytes(k);
+        void* p = nullptr;
+        PICO_HIP(c, hipMalloc(&p, bytes));
+        c->scene_allocs.push_back(p);
+        PICO_HIP(c, hipMemcpyAsync(p, k.texels, bytes, hipMemcpyHostToDevice, c->stream));
+        PICO_HIP(c, hipStreamSynchronize(c->stream));
+        s.sky_tex = static_cast<const uint8_t*>(p);
+        s.sky_w = k.width;
+        s.sky_h = k.height;
+        s.sky_faces = k.faces;
+        s.sky_fmt = k.format;
+    }
+    hs.sky_image.texels = nullptr;
+    return PICO_OK;
+}
+}  // namespace pico
+
+namespace {
 void free_scene(pico_hip_ctx* c) {
     if (!c->scene_allocs.empty()) hipStreamSynchronize(c->stream);
     for (void* p : c->scene_allocs) hipFree(p);
@@ -322,8 +354,8 @@ int pico_hip_upload_scene_ex(pico_hip_ctx* c, const pico_scene_desc* d, uint32_t
     PICO_HIP(c, hipStreamSynchronize(c->stream));
     s.tlas_root = hs.tlas_root;
     s.n_lights = (uint32_t)hs.lights.size();
-    std::memcpy(s.sky, hs.sky, sizeof(s.sky));
     s.microfacet = hs.microfacet;
+    if ((rc = upload_sky_sun(c, hs, s))) return rc;
     c->dev = s;
     c->stack_need = need;
     c->d_mesh_box = mesh_box;

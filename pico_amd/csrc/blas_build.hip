@@ -1019,8 +1019,8 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     if (blas_bound + 1 > 64) return fail(c, PICO_ERR_LIMIT, "BVH too deep for the 64-entry traversal stack");
     // the TLAS, by the rebuild path, into the node array after the BLAS
     s.n_lights = (uint32_t)hs.lights.size();
-    std::memcpy(s.sky, hs.sky, sizeof(s.sky));
     s.microfacet = hs.microfacet;
+    if (int rc = upload_sky_sun(c, hs, s)) return rc;
     c->dev = s;
     c->d_mesh_box = mesh_box;
     c->d_instance_mesh = inst_mesh;

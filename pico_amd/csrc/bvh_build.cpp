@@ -305,7 +305,21 @@ std::string check_scene(const pico_scene_desc& d) {
                 (d.mesh_vertex_offset[m + 1] != d.mesh_vertex_offset[m] || d.mesh_triangle_offset[m + 1] != d.mesh_triangle_offset[m]))
                 return "a built-in shape has no vertices or triangles";
         }
+    if (d.use_sun)
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(d.sun_direction[c]) || !std::isfinite(d.sun_colour[c])) return "sun direction and colour must be finite";
+    if (d.sky_image) {
+        const pico_sky_image& k = *d.sky_image;
+        if (k.faces != 1 && k.faces != 6) return "sky image needs 1 (equirectangular) or 6 (cube) faces";
+        if (k.format > PICO_TEXEL_RGBA32F) return "unknown sky texel format";
+        if (k.width == 0 || k.height == 0 || !k.texels) return "sky image is empty";
+        if ((uint64_t)k.width * k.height * k.faces > (1ull << 28)) return "sky image too large";
+    }
     return "";
+}
+
+size_t sky_bytes(const pico_sky_image& k) {
+    return (size_t)k.width * k.height * k.faces * (k.format == PICO_TEXEL_RGBA32F ? 16 : 4);
 }
 
 uint32_t mesh_shape(const pico_scene_desc& d, uint32_t m) { return d.mesh_shape ? d.mesh_shape[m] : PICO_SHAPE_TRIANGLES; }
@@ -379,6 +393,12 @@ std::string scene_tables(const pico_scene_desc& d, HostScene& out) {
     }
     std::memcpy(out.sky, d.sky_rgb, sizeof(out.sky));
     out.microfacet = d.microfacet;
+    out.use_sun = d.use_sun ? 1u : 0u;
+    for (int c = 0; c < 3; ++c) {
+        out.sun_to[c] = -d.sun_direction[c];
+        out.sun_col[c] = d.sun_colour[c];
+    }
+    if (d.sky_image) out.sky_image = *d.sky_image;
     return "";
 }
 

@@ -35,7 +35,14 @@ struct HostScene {
     uint32_t n_instances = 0;
     float sky[3] = {0, 0, 0};
     uint32_t microfacet = 0;
+    uint32_t use_sun = 0;
+    float sun_to[3] = {0, 0, 0};       // -sun_direction
+    float sun_col[3] = {0, 0, 0};
+    pico_sky_image sky_image{};        // texels still the description's (width 0: none)
 };
+
+// Bytes of a sky image's texels.
+size_t sky_bytes(const pico_sky_image& k);
 
 // Binned SAH (bins per axis, all three axes) BVH2 over boxes; returns the
 // child-pair root node index.  Leaves hold at most max_leaf primitives.

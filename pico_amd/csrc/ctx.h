@@ -96,6 +96,9 @@ void* scratch(pico_hip_ctx* c, const std::string& name, size_t bytes);
 // previous scene is already freed.
 int upload_scene_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_stats* stats);
 
+// Sky image and sun of a scene being uploaded (api.hip).
+int upload_sky_sun(pico_hip_ctx* c, HostScene& hs, DevScene& s);
+
 // Device TLAS rebuild over the context's n instances (tlas.hip).
 int tlas_rebuild(pico_hip_ctx* c, uint32_t n, const float* d_transforms, pico_rebuild_stats* stats);
 

@@ -360,8 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
             f3 thr = xyz(t4);
             IorStack ist;
             ist.count = __float_as_uint(t4.w);
-            const f3 sky = mk(s.sky[0], s.sky[1], s.sky[2]);
             if (inst == 0xffffffffu) {
+                const f3 sky = sky_radiance(s, dir);
                 if (b == 0) {
                     rad = sky;  // Integrators.cpp:152 (no NaN rule on this path)
                     P.ray_o[i].w = 1.0f;
@@ -395,9 +395,24 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
                 // next-event estimation, Integrators.cpp:46-128 (none at lights and BTDFs)
                 if (kind != PICO_BSDF_LIGHT && !btdf) {
                     const float u = rng.uniform();
-                    if (s.n_lights) {
-                        uint32_t li = (uint32_t)(u * (float)s.n_lights);
-                        if (li >= s.n_lights) li = s.n_lights - 1;
+                    // the sun, when there is one, is light n_lights (Integrators.cpp:55-62)
+                    const uint32_t count = s.n_lights + s.use_sun;
+                    const uint32_t pick = (uint32_t)(u * (float)count);
+                    if (s.use_sun && pick >= s.n_lights) {
+                        // Integrators.cpp:61-77: lit when the ray towards the sun meets
+                        // nothing; no facing test and no length bound
+                        const f3 to_sun = mk(s.sun_to[0], s.sun_to[1], s.sun_to[2]);
+                        const float pdf = direct_pdf(kind, s.microfacet, M, f.n, -dir, to_sun);
+                        P.sh_o[i] = f4(f.p + 0.01f * to_sun, 0.0f);
+                        P.sh_d[i] = f4(to_sun, __int_as_float(0x7f800000));
+                        P.sh_tp[i] = f4(thr * pdf, 1.0f);
+                        P.sh_en[i] = f4(smp.energy, 0.0f);
+                        shadow = true;
+                        r4.w += 1.0f;
+                    } else if (s.n_lights) {
+                        // the reference only decrements its count when u * count
+                        // rounds up to count; the index is clamped here
+                        const uint32_t li = pick >= s.n_lights ? s.n_lights - 1 : pick;
                         const DLight L = s.lights[li];
                         const f3 sp = L.n_tris ? mesh_light_point(s, L, rng) : implicit_light_point(L.tri_offset, rng);
                         const float* LM = s.xf[L.instance].m;
@@ -413,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
                             const f3 dl = wp - f.p;
                             P.sh_o[i] = f4(f.p + 0.01f * to_light, 0.0f);
                             P.sh_d[i] = f4(to_light, 1.01f * sqrtf(dot(dl, dl)) + 1e-3f);
-                            P.sh_tp[i] = f4(thr * pdf, 0.0f);
+                            P.sh_tp[i] = f4(thr * pdf, 0.0f);  // w = 0: a mesh or shape light
                             P.sh_en[i] = f4(smp.energy, 0.0f);
                             shadow = true;
                             r4.w += 1.0f;
@@ -468,14 +483,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         }
         if (!__ballot(item != kDeadPath)) break;
         if (item != kDeadPath && trace_step<kTraceLds, COUNT>(s, t, st, &tc, &ctl->overflow)) {
-            if (t.h.inst != 0xffffffffu) {
+            const float4 tp = P.sh_tp[item];
+            bool lit = false;
+            f3 em;
+            if (tp.w != 0.0f) {
+                // towards the sun: lit when the ray met nothing
+                lit = t.h.inst == 0xffffffffu;
+                em = mk(s.sun_col[0], s.sun_col[1], s.sun_col[2]);
+            } else if (t.h.inst != 0xffffffffu) {
+                // towards a light: lit when the closest hit is any light
                 const DInstance& I = s.inst[t.h.inst];
-                if (I.bsdf == PICO_BSDF_LIGHT) {
-                    const f3 em = ld3(s.mats[I.material].emissive);
-                    const float4 r = P.rad[item];
-                    const f3 add = (xyz(P.sh_tp[item]) * em) * xyz(P.sh_en[item]);
-                    P.rad[item] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
-                }
+                lit = I.bsdf == PICO_BSDF_LIGHT;
+                if (lit) em = ld3(s.mats[I.material].emissive);
+            }
+            if (lit) {
+                const float4 r = P.rad[item];
+                const f3 add = (xyz(tp) * em) * xyz(P.sh_en[item]);
+                P.rad[item] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
             }
             item = kDeadPath;
         }

@@ -461,6 +461,38 @@ int pico_scene_desc_get(const pico_scene* s, pico_scene_desc* d) {
     std::memcpy(d->sky_rgb, s->sky, sizeof(s->sky));
     d->microfacet = s->microfacet;
     d->mesh_shape = s->shape.data();
+    d->use_sun = s->use_sun;
+    std::memcpy(d->sun_direction, s->sun_dir, sizeof(s->sun_dir));
+    std::memcpy(d->sun_colour, s->sun_col, sizeof(s->sun_col));
+    d->sky_image = s->sky_img.width ? &s->sky_img : nullptr;
+    return PICO_OK;
+}
+
+int pico_scene_set_sun(pico_scene* s, const float* direction, const float* colour) {
+    if (!s || (!direction) != (!colour)) return PICO_ERR_ARG;
+    s->use_sun = direction ? 1u : 0u;
+    if (direction) {
+        std::memcpy(s->sun_dir, direction, sizeof(s->sun_dir));
+        std::memcpy(s->sun_col, colour, sizeof(s->sun_col));
+    }
+    return PICO_OK;
+}
+
+int pico_scene_set_sky_image(pico_scene* s, const pico_sky_image* img) {
+    if (!s) return PICO_ERR_ARG;
+    if (!img) {
+        s->sky_img = pico_sky_image{};
+        s->sky_texels.clear();
+        return PICO_OK;
+    }
+    if ((img->faces != 1 && img->faces != 6) || img->format > PICO_TEXEL_RGBA32F || !img->width || !img->height ||
+        !img->texels || (uint64_t)img->width * img->height * img->faces > (1ull << 28))
+        return PICO_ERR_ARG;
+    const size_t bytes = (size_t)img->width * img->height * img->faces * (img->format == PICO_TEXEL_RGBA32F ? 16 : 4);
+    const uint8_t* p = static_cast<const uint8_t*>(img->texels);
+    s->sky_texels.assign(p, p + bytes);
+    s->sky_img = *img;
+    s->sky_img.texels = s->sky_texels.data();
     return PICO_OK;
 }
 

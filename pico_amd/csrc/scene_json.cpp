@@ -30,9 +30,11 @@
 //   - cameras (up (0, 1, 0), aspect 1, far plane 10 unless given; the
 //     reference's rays ignore the near plane and the field of view).
 // The built-in assets Sphere (radius 0.5) and Cube ([-0.5, 0.5]^3) become
-// implicit-shape meshes.  Refused with an error naming the entry: image
-// materials (Metalic, Gloss), sky boxes and the sun light, and anything
-// malformed.  A light instance
+// implicit-shape meshes.  GLOBALS give the sky (a cube map of six PNG faces
+// or one equirectangular PNG / Radiance image, decoded as stb_image does) and
+// the sun; without them the sky is black and there is no sun.  Refused with
+// an error naming the entry: image materials (Metalic, Gloss), other image
+// formats, and anything malformed.  A light instance
 // whose mesh is already used by another light gets its own copy of the mesh
 // (the device's light tables are per mesh).  Paths are resolved against the
 // scene file's directory, without regard to case when the exact path is
@@ -54,6 +56,7 @@
 #include <vector>
 
 #include "../../include/pico_hip.h"
+#include "image_io.h"
 #include "scene_store.h"
 
 namespace fs = std::filesystem;
@@ -428,6 +431,44 @@ void vec(const JVal* v, int n, float* out, const std::string& where) {
     for (int k = 0; k < n; ++k) out[k] = as_float(v->arr[k], where);
 }
 
+// The sky of a GLOBALS entry (Scene.cpp:641-680): an array of six images,
+// the faces +x, -x, +y, -y, +z, -z of a cube map read as 8-bit RGBA
+// (stbi_load), or one equirectangular image read as float RGBA (stbi_loadf).
+// The reference uses the last face's size for all six; faces of different
+// sizes are refused here.
+void load_sky(pico_scene& sc, const fs::path& root, const JVal& v, const std::string& where) {
+    pico_sky_image img{};
+    if (v.kind == JVal::Arr) {
+        if (v.arr.size() < 6) throw LoadError{where + ": a cube map needs six faces"};
+        for (int f = 0; f < 6; ++f) {
+            const fs::path p = resolve(root, as_string(&v.arr[f], where + "[" + std::to_string(f) + "]"));
+            uint32_t w = 0, h = 0;
+            std::vector<uint8_t> px;
+            const std::string e = pico_img::load_rgba8(p.string(), w, h, px);
+            if (!e.empty()) throw LoadError{where + ": " + e};
+            if (f && (w != img.width || h != img.height)) throw LoadError{where + ": the cube faces differ in size"};
+            img.width = w;
+            img.height = h;
+            sc.sky_texels.insert(sc.sky_texels.end(), px.begin(), px.end());
+        }
+        img.faces = 6;
+        img.format = PICO_TEXEL_RGBA8;
+    } else if (v.kind == JVal::Str) {
+        const fs::path p = resolve(root, v.str);
+        std::vector<float> px;
+        const std::string e = pico_img::load_rgba32f(p.string(), img.width, img.height, px);
+        if (!e.empty()) throw LoadError{where + ": " + e};
+        sc.sky_texels.resize(px.size() * sizeof(float));
+        std::memcpy(sc.sky_texels.data(), px.data(), sc.sky_texels.size());
+        img.faces = 1;
+        img.format = PICO_TEXEL_RGBA32F;
+    } else {
+        throw LoadError{where + ": expected six image paths or one image path"};
+    }
+    img.texels = sc.sky_texels.data();
+    sc.sky_img = img;
+}
+
 // column-major a * b
 void mat_mul(const float* a, const float* b, float* r) {
     for (int c = 0; c < 4; ++c)
@@ -444,13 +485,27 @@ void load_into(pico_scene& sc, const fs::path& file) {
     JVal doc = JsonParser(text, file.string()).parse();
     if (doc.kind != JVal::Obj) throw LoadError{file.string() + ": the top level is not an object"};
     sc.microfacet = PICO_MICROFACET_BECKMANN_ALL;
-    // GLOBALS
+    // GLOBALS: process_globals per entry (Scene.cpp:639-720); every entry
+    // sets the sky and the sun again, so the last one in name order decides
     if (const JVal* g = doc.get("GLOBALS")) {
         if (g->kind != JVal::Obj) throw LoadError{"GLOBALS: expected an object"};
         for (const auto& [name, e] : g->obj) {
-            if (e.get("Skybox")) throw LoadError{"GLOBALS/" + name + ": sky boxes are not supported (the sky is black)"};
-            if (e.get("SunDirection") && e.get("SunColour"))
-                throw LoadError{"GLOBALS/" + name + ": the sun light is not supported"};
+            const std::string where = "GLOBALS/" + name;
+            if (e.kind != JVal::Obj) throw LoadError{where + ": expected an object"};
+            sc.sky_img = pico_sky_image{};
+            sc.sky_texels.clear();
+            if (const JVal* sb = e.get("Skybox")) load_sky(sc, root, *sb, where + "/Skybox");
+            // SunDirection is normalised as glm::normalize does, v * (1 / sqrt(dot(v, v)))
+            float dir[3] = {0.0f, -1.0f, 0.0f}, col[3] = {1.0f, 1.0f, 1.0f};
+            if (e.get("SunDirection")) {
+                vec(e.get("SunDirection"), 3, dir, where + "/SunDirection");
+                const float inv = 1.0f / std::sqrt((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
+                for (float& x : dir) x = x * inv;
+            }
+            if (e.get("SunColour")) vec(e.get("SunColour"), 3, col, where + "/SunColour");
+            sc.use_sun = e.get("SunDirection") && e.get("SunColour") ? 1u : 0u;
+            std::memcpy(sc.sun_dir, dir, sizeof(dir));
+            std::memcpy(sc.sun_col, col, sizeof(col));
         }
     }
     // MESH

@@ -17,6 +17,10 @@ struct pico_scene {
     std::vector<pico_material> mats;
     float sky[3] = {0, 0, 0};
     uint32_t microfacet = PICO_MICROFACET_GGX;
+    uint32_t use_sun = 0;
+    float sun_dir[3] = {0, -1, 0}, sun_col[3] = {1, 1, 1};  // Scene.cpp:692-712 defaults
+    pico_sky_image sky_img{};       // texels point into sky_texels (width 0: constant sky)
+    std::vector<uint8_t> sky_texels;
     pico_camera cam{};
     std::vector<std::pair<std::string, pico_camera>> cameras;  // named cameras of a loaded scene, by name
 

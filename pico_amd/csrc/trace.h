@@ -111,6 +111,11 @@ struct DevScene {
     uint32_t n_lights;
     float sky[3];
     uint32_t microfacet;
+    const uint8_t* sky_tex;  // sky image texels, or nullptr for the constant sky
+    uint32_t sky_w, sky_h, sky_faces, sky_fmt;
+    uint32_t use_sun;
+    float sun_to[3];         // towards the sun: -sun_direction
+    float sun_col[3];
 };
 
 struct Hit { float t, u, v; uint32_t inst, prim; };
@@ -442,6 +447,52 @@ __device__ __forceinline__ Frag make_frag(const DevScene& s, const Hit& h) {
 // The shading point of any hit; (o, d) is the world ray that found it.
 __device__ __forceinline__ Frag hit_frag(const DevScene& s, const Hit& h, f3 o, f3 d) {
     return s.inst[h.inst].blas_root >= kImplicitRoot ? implicit_frag(s, h, o, d) : make_frag(s, h);
+}
+
+// ImageCube::sample4 (Source/Core/Image.cpp:361-466) of the sky in direction
+// d: the constant sky, or the nearest texel of the sky image.  A cube map
+// takes the face of the dominant axis (ties go to z, then y) with
+// resolve_cubemap_UV's face coordinates; a single image is equirectangular,
+// u = atan2(z, x) / 2pi + 1/2, v = 1/2 - asin(y) / pi, with asin evaluated in
+// double as the reference's call resolves.  Texel = uint(coordinate * size)
+// mod size, so a coordinate of exactly 1 wraps to texel 0.
+__device__ __forceinline__ f3 sky_radiance(const DevScene& s, f3 d) {
+    if (!s.sky_tex) return mk(s.sky[0], s.sky[1], s.sky[2]);
+    float u, v;
+    uint32_t face = 0;
+    if (s.sky_faces == 6) {
+        const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+        float ma, a, b;
+        if (az >= ax && az >= ay) {
+            face = d.z < 0.0f ? 5u : 4u;
+            ma = 0.5f / az;
+            a = d.z < 0.0f ? -d.x : d.x;
+            b = -d.y;
+        } else if (ay >= ax) {
+            face = d.y < 0.0f ? 3u : 2u;
+            ma = 0.5f / ay;
+            a = d.x;
+            b = d.y < 0.0f ? -d.z : d.z;
+        } else {
+            face = d.x < 0.0f ? 1u : 0u;
+            ma = 0.5f / ax;
+            a = d.x < 0.0f ? d.z : -d.z;
+            b = -d.y;
+        }
+        u = a * ma + 0.5f;
+        v = b * ma + 0.5f;
+    } else {
+        const float inv2pi = (float)(1.0 / (3.14159265358979323846 * 2.0));
+        const float invpi = (float)(1.0 / 3.14159265358979323846);
+        u = atan2f(d.z, d.x) * inv2pi + 0.5f;
+        v = 1.0f - ((float)asin((double)d.y) * invpi + 0.5f);
+    }
+    const uint32_t x = (uint32_t)(u * (float)s.sky_w) % s.sky_w;
+    const uint32_t y = (uint32_t)(v * (float)s.sky_h) % s.sky_h;
+    const size_t texel = ((size_t)face * s.sky_h + y) * s.sky_w + x;
+    if (s.sky_fmt == PICO_TEXEL_RGBA32F) return ld3(reinterpret_cast<const float*>(s.sky_tex) + 4 * texel);
+    const uchar4 c = reinterpret_cast<const uchar4*>(s.sky_tex)[texel];
+    return mk(c.x / 255.0f, c.y / 255.0f, c.z / 255.0f);
 }
 
 // construct_world_to_tangent_transform, Source/Core/vectorUtils.cpp:9-35: rows (T, B, N).

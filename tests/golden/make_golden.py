@@ -15,8 +15,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
 import oracle  # noqa: E402
 import pico_amd as pa  # noqa: E402
+from sky_scenes import sun_cube_sky, sun_hdri_sky  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
 
@@ -27,6 +30,8 @@ def scenes():
         "instanced_small": pa.Scene.instanced(7, 4, 48, 200, 2000, 12.0, pa.MICROFACET_BECKMANN_ALL),
         "cornell_glass": pa.Scene.cornell_glass(1, pa.MICROFACET_BECKMANN_ALL),
         "shapes": pa.Scene.shapes(1, pa.MICROFACET_BECKMANN_ALL),
+        "sun_cube_sky": sun_cube_sky(),
+        "sun_hdri_sky": sun_hdri_sky(),
     }
 
 
@@ -36,7 +41,14 @@ def scene_hash(s):
                                (d.n_meshes + 1,))[-1]
     pos = np.ctypeslib.as_array(oracle.C.cast(d.positions, oracle.C.POINTER(oracle.C.c_float)), (3 * int(nv),))
     xf = np.ctypeslib.as_array(oracle.C.cast(d.transforms, oracle.C.POINTER(oracle.C.c_float)), (16 * d.n_instances,))
-    return hashlib.sha256(pos.tobytes() + xf.tobytes()).hexdigest()[:16]
+    extra = b""
+    if d.use_sun:
+        extra += bytes(d.sun_direction) + bytes(d.sun_colour)
+    if d.sky_image:
+        k = pa.SkyImage.from_address(d.sky_image)
+        n = k.width * k.height * k.faces * (16 if k.format == pa.TEXEL_RGBA32F else 4)
+        extra += bytes(k)[:16] + oracle.C.string_at(k.texels, n)
+    return hashlib.sha256(pos.tobytes() + xf.tobytes() + extra).hexdigest()[:16]
 
 
 def main():
@@ -136,7 +148,7 @@ def main():
         g[f"{name}_ids_inst"], g[f"{name}_ids_prim"] = inst, prim
         rng = np.random.default_rng(1)
         n = 512
-        o = rng.uniform(-1, 13, (n, 3)).astype(np.float32) if name == "instanced_small" else \
+        o = rng.uniform(-1, 13, (n, 3)).astype(np.float32) if name in ("instanced_small", "sun_cube_sky") else \
             rng.uniform(-0.9, 0.9, (n, 3)).astype(np.float32)
         dd = rng.normal(size=(n, 3)).astype(np.float32)
         dd /= np.linalg.norm(dd, axis=1, keepdims=True)
@@ -152,6 +164,18 @@ def main():
             pix, cnt, var = rs.render(cam, p, threads=1)
             g.update({"accum_pix": pix, "accum_cnt": cnt, "accum_var": var})
         del rs
+    # sky lookup (ImageCube::sample4): a cube of RGBA8 faces, an equirectangular float image
+    from sky_scenes import sky_dirs
+    d = sky_dirs(2048, 7)
+    rng = np.random.default_rng(12)
+    for name, tex in [("cube", rng.integers(0, 256, (6, 11, 9, 4), dtype=np.uint8)),
+                      ("hdri", rng.random((1, 13, 27, 4), dtype=np.float32))]:
+        img = pa.SkyImage(tex.shape[2], tex.shape[1], tex.shape[0],
+                          pa.TEXEL_RGBA32F if tex.dtype == np.float32 else pa.TEXEL_RGBA8, tex.ctypes.data)
+        sky = (oracle.C.c_float * 3)(0.0, 0.0, 0.0)
+        out = np.empty((len(d), 3), np.float32)
+        R.ref_sky_sample(oracle.C.addressof(img), oracle.C.addressof(sky), len(d), P(d), P(out))
+        g.update({f"sky_{name}_tex": tex, f"sky_{name}_dir": d, f"sky_{name}_out": out})
     # a-trous and tone mapping
     rng = np.random.default_rng(8)
     w, h = 41, 31

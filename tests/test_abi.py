@@ -85,3 +85,31 @@ def test_gpu_entry_points_fail_cleanly_without_device():
         pa.lib().pico_hip_destroy(h)
     else:
         assert rc in (pa.PICO_ERR_HIP, pa.PICO_ERR_ARG)
+
+
+def test_ctypes_layouts_match_the_header(tmp_path):
+    """Every struct the binding declares has the C header's size and field
+    offsets (compiled with the host C compiler)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"pico_scene_desc": pa.SceneDesc, "pico_sky_image": pa.SkyImage, "pico_material": pa.Material,
+               "pico_camera": pa.Camera}
+    inc = os.path.join(os.path.dirname(pa.__file__), "..", "include")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pico_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f in py._fields_:
+            assert int(got[f"{cname}.{f[0]}"]) == getattr(py, f[0]).offset, (cname, f[0])

@@ -9,13 +9,18 @@ import pytest
 
 import oracle
 import pico_amd as pa
+from sky_scenes import sun_cube_sky, sun_hdri_sky
 
 G = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"), allow_pickle=False)
 P = oracle._p
-SCENES = ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes"]
+SCENES = ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes", "sun_cube_sky", "sun_hdri_sky"]
 
 
 def _scene(name):
+    if name == "sun_cube_sky":
+        return sun_cube_sky()
+    if name == "sun_hdri_sky":
+        return sun_hdri_sky()
     if name == "cornell_beckmann":
         return pa.Scene.cornell(1, pa.MICROFACET_BECKMANN_ALL)
     if name == "cornell_glass":
@@ -156,6 +161,18 @@ def test_gbuffer(scenes, name):
     out = oracle.CpuScene(scenes[name]).gbuffer(scenes[name].camera(1.0), 32, 24)
     for x, k in zip(out, ("n", "p", "a")):
         np.testing.assert_allclose(x, G[f"{name}_gb_{k}"], rtol=1e-5, atol=5e-4)
+
+
+@pytest.mark.parametrize("name", ["cube", "hdri"])
+def test_sky_lookup(name):
+    import ctypes as C
+    tex, d = np.ascontiguousarray(G[f"sky_{name}_tex"]), np.ascontiguousarray(G[f"sky_{name}_dir"])
+    img = pa.SkyImage(tex.shape[2], tex.shape[1], tex.shape[0],
+                      pa.TEXEL_RGBA32F if tex.dtype == np.float32 else pa.TEXEL_RGBA8, tex.ctypes.data)
+    sky = (C.c_float * 3)(0.0, 0.0, 0.0)
+    out = np.empty((len(d), 3), np.float32)
+    oracle.cpu_lib().cpuref_sky_sample(C.addressof(img), C.addressof(sky), len(d), P(d), P(out))
+    assert np.array_equal(out, G[f"sky_{name}_out"])
 
 
 @pytest.mark.parametrize("tile", [(16, 16), (5, 7)])

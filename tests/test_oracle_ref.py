@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 import pico_amd as pa
+from sky_scenes import sky_dirs, sun_cube_sky, sun_hdri_sky
 
 pytestmark = pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference subset not built")
 
@@ -21,7 +22,32 @@ def scenes():
         "instanced_small": pa.Scene.instanced(7, 4, 48, 200, 2000, 12.0, pa.MICROFACET_BECKMANN_ALL),
         "cornell_glass": pa.Scene.cornell_glass(1, pa.MICROFACET_BECKMANN_ALL),
         "shapes": pa.Scene.shapes(1, pa.MICROFACET_BECKMANN_ALL),
+        "sun_cube_sky": sun_cube_sky(),
+        "sun_hdri_sky": sun_hdri_sky(),
     }
+
+
+@pytest.mark.parametrize("shape,dtype", [((6, 33, 17, 4), np.uint8), ((1, 64, 128, 4), np.float32),
+                                         ((1, 7, 5, 4), np.float32), ((6, 1, 1, 4), np.float32),
+                                         ((1, 9, 16, 4), np.uint8)])
+def test_sky_lookup_bitwise(shape, dtype):
+    """ImageCube::sample4: face choice (ties on the diagonals included), face
+    coordinates, the equirectangular mapping and the texel conversion."""
+    import ctypes as C
+    rng = np.random.default_rng(shape[1])
+    t = rng.integers(0, 256, shape, dtype=np.uint8) if dtype == np.uint8 else rng.random(shape, dtype=np.float32)
+    img = pa.SkyImage(shape[2], shape[1], shape[0], pa.TEXEL_RGBA32F if dtype == np.float32 else pa.TEXEL_RGBA8,
+                      t.ctypes.data)
+    sky = (C.c_float * 3)(0.1, 0.2, 0.3)
+    d = sky_dirs(100000, 5)
+    a, b = np.zeros((len(d), 3), np.float32), np.zeros((len(d), 3), np.float32)
+    oracle.cpu_lib().cpuref_sky_sample(C.addressof(img), C.addressof(sky), len(d), oracle._p(d), oracle._p(a))
+    oracle.ref_lib().ref_sky_sample(C.addressof(img), C.addressof(sky), len(d), oracle._p(d), oracle._p(b))
+    assert np.array_equal(a, b)
+    # every texel of a small image is reached
+    if shape[1] * shape[2] <= 64:
+        texels = t.reshape(-1, 4)[:, :3].astype(np.float32) / (255.0 if dtype == np.uint8 else 1.0)
+        assert len(np.unique(a, axis=0)) == len(np.unique(texels, axis=0))
 
 
 def test_xorshift_and_hammersley_bitwise():
@@ -198,7 +224,8 @@ def test_closest_hit_and_primary_ids(scenes, name):
     np.testing.assert_allclose(cnrm[m], rnrm[m], rtol=1e-5, atol=5e-5)
 
 
-@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes"])
+@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes", "sun_cube_sky",
+                                  "sun_hdri_sky"])
 def test_integrator_per_sample_reference_rng(scenes, name):
     s = scenes[name]
     cam = s.camera(1.0)
@@ -221,7 +248,7 @@ def test_render_loop_accumulation(scenes):
     np.testing.assert_allclose(cvar, rvar, rtol=1e-3, atol=1e-5)
 
 
-@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "shapes"])
+@pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "shapes", "sun_cube_sky", "sun_hdri_sky"])
 def test_gbuffer(scenes, name):
     s = scenes[name]
     cam = s.camera(1.0)

@@ -224,8 +224,8 @@ def test_json_syntax(tmp_path):
     ("sphere", "the name of a built-in shape"),
     ("transparent", "needs a material with an IndexOfRefraction"),
     ("image_material", "image materials"),
-    ("skybox", "sky boxes"),
-    ("sun", "sun light"),
+    ("skybox", "cannot read"),
+    ("sun", "SunColour: expected an array of 3 numbers"),
     ("missing_file", "cannot open"),
     ("bad_face", "face index out of range"),
     ("no_bsdf", "BSRDF: expected a string"),
@@ -247,7 +247,7 @@ def test_errors(tmp_path, case, needle):
     elif case == "skybox":
         glob = {"Sky": {"Skybox": "sky.hdr"}}
     elif case == "sun":
-        glob = {"Sky": {"SunDirection": [0, -1, 0], "SunColour": [1, 1, 1]}}
+        glob = {"Sky": {"SunDirection": [0, -1, 0], "SunColour": [1, 1]}}
     elif case == "missing_file":
         meshes = {"Quad": "nothere.obj"}
     elif case == "bad_face":
