@@ -25,6 +25,8 @@
  *                           (Source/Core/UpperLevelBVH.cpp:22-57)
  *   pico_scene_load_json    Scene::Scene(ThreadPool&, path) and its section handlers
  *                           (Source/Core/Scene.cpp:36-84, 297-720)
+ *   pico_scene_set_sun / _set_sky_image  the -SunDirection/-SunColour/-Skybox options of
+ *                           Scene::Scene(ThreadPool&, path, aiScene, options) (Source/Core/Scene.cpp:86-123)
  *   pico_write_hdr          the file half of Scene::render_scene_to_file (Source/Core/Scene.cpp:236-246)
  *
  * Buffers passed as `float*` RGB/vec3 images use the reference layout: packed
