@@ -5,11 +5,13 @@ ctypes wrappers of
     (``oracle/cpu_ref.cpp``, ``oracle/cnn_ref.cpp``), and
   * ``_ref/libpico_ref.so``: Pico's own hot-path sources compiled from the
     reference tree plus ``oracle/ref_harness.cpp`` (present only where it was
-    built; the tests that need it skip otherwise).
+    built; the tests that need it skip otherwise), and
+  * ``_ref/libjson_check.so``: jsoncpp from the reference tree beside the
+    library's scene-file JSON reader (``oracle/json_check.cpp``).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
-``bench.py`` may import this package.  Parity status of every oracle function
-(what pins it) is listed in DESIGN.md and exercised in tests/test_oracle_*.py.
+``bench.py`` may import this package.  What pins each oracle function is
+exercised in tests/test_oracle_*.py, tests/test_golden.py and the device tests.
 """
 from __future__ import annotations
 
@@ -87,6 +89,35 @@ def cpu_lib():
 
 def ref_available():
     return os.path.exists(REF_PATH)
+
+
+JSON_CHECK_PATH = os.path.join(_HERE, "_ref", "libjson_check.so")
+_jc = None
+
+
+def json_check_available():
+    return os.path.exists(JSON_CHECK_PATH)
+
+
+def json_flatten(text, which):
+    """Canonical value tree of `text` (bytes) read by jsoncpp (which="ref") or
+    by the library's reader (which="pico"), or None when that reader refuses it."""
+    global _jc
+    if _jc is None:
+        _jc = C.CDLL(JSON_CHECK_PATH)
+        for f in (_jc.jc_ref_flatten, _jc.jc_pico_flatten):
+            f.restype = C.c_int
+            f.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]
+    fn = _jc.jc_ref_flatten if which == "ref" else _jc.jc_pico_flatten
+    cap = 1 << 16
+    while True:
+        buf = C.create_string_buffer(cap)
+        rc = fn(text, len(text), buf, cap)
+        if rc == 1:
+            return buf.value.decode()
+        if rc == 0:
+            return None
+        cap *= 4
 
 
 def ref_lib():

@@ -57,195 +57,17 @@
 
 #include "../../include/pico_hip.h"
 #include "image_io.h"
+#include "json_reader.h"
 #include "scene_store.h"
 
 namespace fs = std::filesystem;
 
 namespace {
 
-// ---- JSON (RFC 8259) ----------------------------------------------------
-struct JVal {
-    enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
-    bool b = false;
-    double num = 0.0;
-    std::string str;
-    std::vector<JVal> arr;
-    std::map<std::string, JVal> obj;  // sorted by name; a repeated name keeps the last value
-
-    const JVal* get(const char* k) const {
-        if (kind != Obj) return nullptr;
-        auto it = obj.find(k);
-        return it == obj.end() ? nullptr : &it->second;
-    }
-};
+using JVal = pico_json::Value;
 
 struct LoadError {
     std::string msg;
-};
-
-class JsonParser {
-public:
-    JsonParser(const std::string& text, const std::string& file) : s_(text), file_(file) {}
-
-    JVal parse() {
-        JVal v = value(0);
-        ws();
-        if (i_ != s_.size()) fail("unexpected text after the top-level value");
-        return v;
-    }
-
-private:
-    const std::string& s_;
-    std::string file_;
-    size_t i_ = 0;
-
-    [[noreturn]] void fail(const std::string& what) const {
-        size_t line = 1, col = 1;
-        for (size_t k = 0; k < i_ && k < s_.size(); ++k) {
-            if (s_[k] == '\n') { ++line; col = 1; } else { ++col; }
-        }
-        throw LoadError{file_ + ":" + std::to_string(line) + ":" + std::to_string(col) + ": " + what};
-    }
-    void ws() {
-        while (i_ < s_.size() && (s_[i_] == ' ' || s_[i_] == '\t' || s_[i_] == '\n' || s_[i_] == '\r')) ++i_;
-    }
-    bool lit(const char* w) {
-        const size_t n = std::strlen(w);
-        if (s_.compare(i_, n, w) == 0) { i_ += n; return true; }
-        return false;
-    }
-    JVal value(int depth) {
-        if (depth > 256) fail("nesting too deep");
-        ws();
-        if (i_ >= s_.size()) fail("unexpected end of file");
-        JVal v;
-        const char c = s_[i_];
-        if (c == '{') {
-            v.kind = JVal::Obj;
-            ++i_;
-            ws();
-            if (i_ < s_.size() && s_[i_] == '}') { ++i_; return v; }
-            for (;;) {
-                ws();
-                if (i_ >= s_.size() || s_[i_] != '"') fail("expected a member name");
-                std::string k = string();
-                ws();
-                if (i_ >= s_.size() || s_[i_] != ':') fail("expected ':'");
-                ++i_;
-                v.obj[k] = value(depth + 1);
-                ws();
-                if (i_ < s_.size() && s_[i_] == ',') { ++i_; continue; }
-                if (i_ < s_.size() && s_[i_] == '}') { ++i_; return v; }
-                fail("expected ',' or '}'");
-            }
-        }
-        if (c == '[') {
-            v.kind = JVal::Arr;
-            ++i_;
-            ws();
-            if (i_ < s_.size() && s_[i_] == ']') { ++i_; return v; }
-            for (;;) {
-                v.arr.push_back(value(depth + 1));
-                ws();
-                if (i_ < s_.size() && s_[i_] == ',') { ++i_; continue; }
-                if (i_ < s_.size() && s_[i_] == ']') { ++i_; return v; }
-                fail("expected ',' or ']'");
-            }
-        }
-        if (c == '"') {
-            v.kind = JVal::Str;
-            v.str = string();
-            return v;
-        }
-        if (lit("true")) { v.kind = JVal::Bool; v.b = true; return v; }
-        if (lit("false")) { v.kind = JVal::Bool; v.b = false; return v; }
-        if (lit("null")) return v;
-        if (c == '-' || (c >= '0' && c <= '9')) {
-            v.kind = JVal::Num;
-            v.num = number();
-            return v;
-        }
-        fail(std::string("unexpected character '") + c + "'");
-    }
-    double number() {
-        const size_t b = i_;
-        if (s_[i_] == '-') ++i_;
-        auto digits = [&] {
-            const size_t d = i_;
-            while (i_ < s_.size() && std::isdigit((unsigned char)s_[i_])) ++i_;
-            if (i_ == d) fail("malformed number");
-        };
-        if (i_ < s_.size() && s_[i_] == '0') ++i_;
-        else digits();
-        if (i_ < s_.size() && s_[i_] == '.') { ++i_; digits(); }
-        if (i_ < s_.size() && (s_[i_] == 'e' || s_[i_] == 'E')) {
-            ++i_;
-            if (i_ < s_.size() && (s_[i_] == '+' || s_[i_] == '-')) ++i_;
-            digits();
-        }
-        // the grammar above is the JSON one; from_chars reads it correctly rounded
-        double x = 0.0;
-        std::from_chars(s_.data() + b, s_.data() + i_, x);
-        return x;
-    }
-    static void put_utf8(std::string& o, uint32_t cp) {
-        if (cp < 0x80) o += (char)cp;
-        else if (cp < 0x800) { o += (char)(0xC0 | (cp >> 6)); o += (char)(0x80 | (cp & 0x3F)); }
-        else if (cp < 0x10000) {
-            o += (char)(0xE0 | (cp >> 12)); o += (char)(0x80 | ((cp >> 6) & 0x3F)); o += (char)(0x80 | (cp & 0x3F));
-        } else {
-            o += (char)(0xF0 | (cp >> 18)); o += (char)(0x80 | ((cp >> 12) & 0x3F));
-            o += (char)(0x80 | ((cp >> 6) & 0x3F)); o += (char)(0x80 | (cp & 0x3F));
-        }
-    }
-    uint32_t hex4() {
-        if (i_ + 4 > s_.size()) fail("truncated \\u escape");
-        uint32_t v = 0;
-        for (int k = 0; k < 4; ++k) {
-            const char h = s_[i_++];
-            v <<= 4;
-            if (h >= '0' && h <= '9') v |= (uint32_t)(h - '0');
-            else if (h >= 'a' && h <= 'f') v |= (uint32_t)(h - 'a' + 10);
-            else if (h >= 'A' && h <= 'F') v |= (uint32_t)(h - 'A' + 10);
-            else fail("bad \\u escape");
-        }
-        return v;
-    }
-    std::string string() {
-        ++i_;  // opening quote
-        std::string o;
-        for (;;) {
-            if (i_ >= s_.size()) fail("unterminated string");
-            const char c = s_[i_++];
-            if (c == '"') return o;
-            if ((unsigned char)c < 0x20) fail("control character in a string");
-            if (c != '\\') { o += c; continue; }
-            if (i_ >= s_.size()) fail("unterminated string");
-            const char e = s_[i_++];
-            switch (e) {
-                case '"': o += '"'; break;
-                case '\\': o += '\\'; break;
-                case '/': o += '/'; break;
-                case 'b': o += '\b'; break;
-                case 'f': o += '\f'; break;
-                case 'n': o += '\n'; break;
-                case 'r': o += '\r'; break;
-                case 't': o += '\t'; break;
-                case 'u': {
-                    uint32_t cp = hex4();
-                    if (cp >= 0xD800 && cp < 0xDC00 && i_ + 1 < s_.size() && s_[i_] == '\\' && s_[i_ + 1] == 'u') {
-                        i_ += 2;
-                        const uint32_t lo = hex4();
-                        if (lo >= 0xDC00 && lo < 0xE000) cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
-                        else fail("bad surrogate pair");
-                    }
-                    put_utf8(o, cp);
-                    break;
-                }
-                default: fail("bad escape");
-            }
-        }
-    }
 };
 
 std::string read_file(const fs::path& p) {
@@ -417,7 +239,7 @@ MeshData read_obj(const fs::path& path) {
 // ---- scene ----------------------------------------------------------------
 float as_float(const JVal& v, const std::string& where) {
     if (v.kind != JVal::Num) throw LoadError{where + ": expected a number"};
-    return (float)v.num;
+    return v.as_float();
 }
 
 std::string as_string(const JVal* v, const std::string& where) {
@@ -482,7 +304,12 @@ void mat_mul(const float* a, const float* b, float* r) {
 void load_into(pico_scene& sc, const fs::path& file) {
     const fs::path root = file.parent_path().empty() ? fs::path(".") : file.parent_path();
     const std::string text = read_file(file);
-    JVal doc = JsonParser(text, file.string()).parse();
+    JVal doc;
+    try {
+        doc = pico_json::Reader(text, file.string()).parse();
+    } catch (const pico_json::Error& e) {
+        throw LoadError{e.msg};
+    }
     if (doc.kind != JVal::Obj) throw LoadError{file.string() + ": the top level is not an object"};
     sc.microfacet = PICO_MICROFACET_BECKMANN_ALL;
     // GLOBALS: process_globals per entry (Scene.cpp:639-720); every entry

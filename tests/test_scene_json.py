@@ -216,10 +216,16 @@ def test_json_syntax(tmp_path):
     path = _write(tmp_path, text, {"quad.obj": QUAD})
     d = _desc(pa.Scene.load_json(path))
     assert d["bsdf"][0] == pa.BSDF_DIFFUSE and d["mats"][0][2] == 0.25 and d["xf"][0][0, 3] == 1.0
+    # jsoncpp's extensions the reference's loader accepts: comments, trailing
+    # commas, a byte-order mark, text after the document
+    text2 = ("\ufeff// scene\n" + text[:-1] + ", /* more */ }  trailing").encode()
+    (tmp_path / "scene.json").write_bytes(text2)
+    d2 = _desc(pa.Scene.load_json(tmp_path / "scene.json"))
+    assert d2["bsdf"][0] == pa.BSDF_DIFFUSE and d2["mats"] == d["mats"]
 
 
 @pytest.mark.parametrize("case,needle", [
-    ('{"MESH": {"Q": {"Path": "quad.obj"}},\n "MATERIALS": [1, 2,]}', "scene.json:2:"),
+    ('{"MESH": {"Q": {"Path": "quad.obj"}},\n "MATERIALS": [1, 2 3]}', "scene.json:2:"),
     ("unknown_asset", "unknown asset"),
     ("sphere", "the name of a built-in shape"),
     ("transparent", "needs a material with an IndexOfRefraction"),
