@@ -42,6 +42,7 @@ constexpr uint32_t kRootSlot = 0x80000000u;  // slot | mesh: the reference goes 
 constexpr uint32_t kMaxLevels = 1u << 16;
 constexpr uint32_t kLarge = 8192;  // nodes of more triangles are binned and partitioned by many workgroups
 constexpr uint32_t kChunk = 2048;  // triangles per workgroup of such a node
+constexpr uint32_t kSmall = 64;    // nodes of at most this many triangles: the whole subtree by one wave
 
 struct BuildTask {
     uint32_t begin, end;  // range of the triangle permutation
@@ -79,6 +80,14 @@ struct Bins;
 struct LargeTask;
 struct Chunk;
 
+// A subtree of at most kSmall triangles, built by one wave: its root task,
+// the permutation buffer that holds its range, and the level its root would
+// have in the level-by-level build.
+struct SmallTask {
+    BuildTask t;
+    uint32_t parity, level, pad[2];
+};
+
 struct Level {
     const BuildTask* in;
     BuildTask* out;
@@ -103,6 +112,11 @@ struct Level {
     uint32_t* chunk_left;
     uint32_t* next_large;   // children of more than kLarge triangles, and their chunks
     uint32_t* next_chunks;
+    // children of at most kSmall triangles: built later, one subtree per wave
+    SmallTask* small;
+    uint32_t* small_count;
+    uint32_t out_parity;    // which permutation buffer perm_out is (0 or 1)
+    uint32_t level;         // this level's number (1 for the roots)
 };
 
 // host Box arithmetic (bvh_build.cpp), replayed on the device
@@ -376,6 +390,22 @@ __device__ void bins_clear(Bins* b, uint32_t tid, uint32_t nthreads) {
     }
 }
 
+// One triangle (centroid c, box bx) of task t into the bins b (LDS).
+__device__ __forceinline__ void bin_tri(Bins* b, const BuildTask& t, const float* c, const float* bx, const float* ext,
+                                        const float* sc) {
+    for (int a = 0; a < 3; ++a) {
+        if (!(ext[a] > 0)) continue;
+        const uint32_t k = bin_of(c[a], t.cbox[a], sc[a]);
+        atomicAdd(&b->cnt[a][k], 1u);
+        for (int j = 0; j < 3; ++j) {
+            atomicMin(&b->box[a][k][j], map_f(bx[j]));
+            atomicMax(&b->box[a][k][3 + j], map_f(bx[3 + j]));
+            atomicMin(&b->cen[a][k][j], map_f(c[j]));
+            atomicMax(&b->cen[a][k][3 + j], map_f(c[j]));
+        }
+    }
+}
+
 // Bins the triangles perm_in[begin, end) of task t into b (LDS).
 __device__ void bin_range(Bins* b, const Level& L, const BuildTask& t, uint32_t begin, uint32_t end, const float* ext,
                           const float* sc, uint32_t tid) {
@@ -384,54 +414,69 @@ __device__ void bin_range(Bins* b, const Level& L, const BuildTask& t, uint32_t 
         float c[3], bx[6];
         for (int k = 0; k < 3; ++k) c[k] = L.pcen[3 * (size_t)p + k];
         for (int k = 0; k < 6; ++k) bx[k] = L.pbox[6 * (size_t)p + k];
-        for (int a = 0; a < 3; ++a) {
-            if (!(ext[a] > 0)) continue;
-            const uint32_t k = bin_of(c[a], t.cbox[a], sc[a]);
-            atomicAdd(&b->cnt[a][k], 1u);
-            for (int j = 0; j < 3; ++j) {
-                atomicMin(&b->box[a][k][j], map_f(bx[j]));
-                atomicMax(&b->box[a][k][3 + j], map_f(bx[3 + j]));
-                atomicMin(&b->cen[a][k][j], map_f(c[j]));
-                atomicMax(&b->cen[a][k][3 + j], map_f(c[j]));
-            }
-        }
+        bin_tri(b, t, c, bx, ext, sc);
     }
 }
 
-// The host builder's split() on the bins (one lane).
-__device__ void decide(const Bins* b, const BuildTask& t, const float* ext, Decision& d) {
+// The rest of the host builder's split() once the best plane is known
+// (best_axis -1: none): leaf, plane or middle split, and the children's
+// boxes (one lane).
+__device__ void finish_decision(const Bins* b, const BuildTask& t, float best_cost, int best_axis, uint32_t best_bin,
+                                Decision& d);
+
+// The host builder's split() on the bins, by a whole wave (every lane of
+// it calls this): lane 15 * axis + k prices the plane
+// after bin k of `axis`, sweeping the bins in the host's order (right part
+// from the last bin down, left part from the first bin up), and the wave
+// takes the host's choice, the first of the cheapest planes in axis-major
+// order (a cost that is not below infinity is never chosen).  Lane 0 then
+// finishes the decision.
+__device__ void decide_wave(const Bins* b, const BuildTask& t, const float* ext, Decision& d, uint32_t lane) {
     const uint32_t n = t.end - t.begin;
-    int kind = 0, best_axis = -1;
-    uint32_t best_bin = 0;
-    if (n > 1) {
-        float best_cost = INFINITY;
-        for (int axis = 0; axis < 3; ++axis) {
-            if (!(ext[axis] > 0)) continue;
-            float rarea[kBins];
-            uint32_t rcnt[kBins];
-            DBox acc;
-            acc.clear();
-            uint32_t c = 0;
-            for (int k = (int)kBins - 1; k > 0; --k) {
-                float l[3], h[3];
-                for (int j = 0; j < 3; ++j) { l[j] = unmap_f(b->box[axis][k][j]); h[j] = unmap_f(b->box[axis][k][3 + j]); }
-                acc.grow(l, h);
-                c += b->cnt[axis][k];
-                rarea[k] = acc.empty() ? 0.0f : acc.area();
-                rcnt[k] = c;
-            }
-            acc.clear();
-            c = 0;
-            for (uint32_t k = 0; k + 1 < kBins; ++k) {
-                float l[3], h[3];
-                for (int j = 0; j < 3; ++j) { l[j] = unmap_f(b->box[axis][k][j]); h[j] = unmap_f(b->box[axis][k][3 + j]); }
-                acc.grow(l, h);
-                c += b->cnt[axis][k];
-                if (c == 0 || rcnt[k + 1] == 0) continue;
-                const float cost = (float)c * acc.area() + (float)rcnt[k + 1] * rarea[k + 1];
-                if (cost < best_cost) { best_cost = cost; best_axis = axis; best_bin = k; }
-            }
+    float cost = INFINITY;
+    uint32_t idx = 0xffffffffu;
+    const int axis = (int)(lane / (kBins - 1));
+    const uint32_t k = lane % (kBins - 1);
+    if (n > 1 && axis < 3 && ext[axis] > 0) {
+        DBox acc;
+        acc.clear();
+        uint32_t rc = 0;
+        for (int j = (int)kBins - 1; j > (int)k; --j) {
+            float l[3], h[3];
+            for (int q = 0; q < 3; ++q) { l[q] = unmap_f(b->box[axis][j][q]); h[q] = unmap_f(b->box[axis][j][3 + q]); }
+            acc.grow(l, h);
+            rc += b->cnt[axis][j];
         }
+        const float rarea = acc.empty() ? 0.0f : acc.area();
+        acc.clear();
+        uint32_t lc = 0;
+        for (uint32_t j = 0; j <= k; ++j) {
+            float l[3], h[3];
+            for (int q = 0; q < 3; ++q) { l[q] = unmap_f(b->box[axis][j][q]); h[q] = unmap_f(b->box[axis][j][3 + q]); }
+            acc.grow(l, h);
+            lc += b->cnt[axis][j];
+        }
+        if (lc != 0 && rc != 0) {
+            const float c = (float)lc * acc.area() + (float)rc * rarea;
+            if (c < INFINITY) { cost = c; idx = lane; }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float oc = __shfl_xor(cost, o);
+        const uint32_t oi = __shfl_xor(idx, o);
+        if (oc < cost || (oc == cost && oi < idx)) { cost = oc; idx = oi; }
+    }
+    if (lane == 0) {
+        const bool found = idx != 0xffffffffu;
+        finish_decision(b, t, cost, found ? (int)(idx / (kBins - 1)) : -1, found ? idx % (kBins - 1) : 0u, d);
+    }
+}
+
+__device__ void finish_decision(const Bins* b, const BuildTask& t, float best_cost, int best_axis, uint32_t best_bin,
+                                Decision& d) {
+    const uint32_t n = t.end - t.begin;
+    int kind = 0;
+    if (n > 1) {
         DBox bounds;
         for (int j = 0; j < 3; ++j) { bounds.lo[j] = t.box[j]; bounds.hi[j] = t.box[3 + j]; }
         const float parent = bounds.area();
@@ -500,8 +545,9 @@ __device__ void count_large_child(const Level& L, uint32_t n) {
     }
 }
 
-// One lane: the node of a split, its reference in the parent and the two child tasks.
-__device__ void emit_split(const Level& L, const BuildTask& t, const Decision& d) {
+// One lane: the binary node of a split and its reference in the parent;
+// returns the node's index.
+__device__ uint32_t emit_node(const Level& L, const BuildTask& t, const Decision& d) {
     const uint32_t id = atomicAdd(L.bcount, 1u);
     DNode nd{};
     DBox lb, rb;
@@ -513,18 +559,34 @@ __device__ void emit_split(const Level& L, const BuildTask& t, const Decision& d
     set_child(nd, 1, rb, kEmptyRef);
     L.bnodes[id] = nd;
     write_ref(L, t.slot, id);
-    const uint32_t o = atomicAdd(L.out_count, 2u);
-    BuildTask c0, c1;
+    return id;
+}
+
+// The two child tasks of node `id`, split as d says.
+__device__ void child_tasks(const BuildTask& t, const Decision& d, uint32_t id, BuildTask& c0, BuildTask& c1) {
     c0.begin = t.begin; c0.end = t.begin + d.nleft; c0.slot = 2 * id; c0.mesh = t.mesh;
     c1.begin = t.begin + d.nleft; c1.end = t.end; c1.slot = 2 * id + 1; c1.mesh = t.mesh;
     for (int j = 0; j < 6; ++j) {
         c0.box[j] = d.lbox[j]; c0.cbox[j] = d.lcen[j];
         c1.box[j] = d.rbox[j]; c1.cbox[j] = d.rcen[j];
     }
-    L.out[o] = c0;
-    L.out[o + 1] = c1;
-    count_large_child(L, c0.end - c0.begin);
-    count_large_child(L, c1.end - c1.begin);
+}
+
+// One lane: the node of a split, its reference in the parent and the two
+// child tasks, for the next level or, when small, for the subtree kernel.
+__device__ void emit_split(const Level& L, const BuildTask& t, const Decision& d) {
+    const uint32_t id = emit_node(L, t, d);
+    BuildTask ch[2];
+    child_tasks(t, d, id, ch[0], ch[1]);
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t n = ch[k].end - ch[k].begin;
+        if (n <= kSmall) {
+            L.small[atomicAdd(L.small_count, 1u)] = SmallTask{ch[k], L.out_parity, L.level + 1, {0u, 0u}};
+        } else {
+            L.out[atomicAdd(L.out_count, 1u)] = ch[k];
+            count_large_child(L, n);
+        }
+    }
 }
 
 __device__ __forceinline__ bool goes_left(const Level& L, const BuildTask& t, const Decision& d, const float* sc, uint32_t i,
@@ -565,20 +627,29 @@ __device__ void partition_range(const Level& L, const BuildTask& t, const Decisi
     }
 }
 
+// The record of triangle p (of mesh `mesh`) at leaf position i.
+__device__ __forceinline__ void write_tri(const Level& L, uint32_t i, uint32_t p, uint32_t mesh) {
+    const float* p0 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p];
+    const float* p1 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 1];
+    const float* p2 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 2];
+    DTri tr{};
+    for (int c = 0; c < 3; ++c) { tr.v0[c] = p0[c]; tr.e1[c] = p1[c] - p0[c]; tr.e2[c] = p2[c] - p0[c]; }
+    tr.prim = p - L.tofs[mesh];
+    L.tris[i] = tr;
+}
+
+// One lane: the reference of leaf t in the parent's slot.
+__device__ void leaf_ref(const Level& L, const BuildTask& t);
+
 // Leaf: triangle records in place and the reference in the parent's slot.
 __device__ void emit_leaf(const Level& L, const BuildTask& t, uint32_t tid) {
+    for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) write_tri(L, i, L.perm_in[i], t.mesh);
+    if (tid == 0) leaf_ref(L, t);
+}
+
+__device__ void leaf_ref(const Level& L, const BuildTask& t) {
     const uint32_t n = t.end - t.begin;
-    for (uint32_t i = t.begin + tid; i < t.end; i += kThreads) {
-        const uint32_t p = L.perm_in[i];
-        const float* p0 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p];
-        const float* p1 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 1];
-        const float* p2 = L.pos + 3 * (size_t)L.gidx[3 * (size_t)p + 2];
-        DTri tr{};
-        for (int c = 0; c < 3; ++c) { tr.v0[c] = p0[c]; tr.e1[c] = p1[c] - p0[c]; tr.e2[c] = p2[c] - p0[c]; }
-        tr.prim = p - L.tofs[t.mesh];
-        L.tris[i] = tr;
-    }
-    if (tid == 0) {
+    {
         const uint32_t ref = make_leaf(t.begin, n);
         if (t.slot & kRootSlot) {
             // a mesh that is one leaf: a node with the leaf and an empty child, as on the host
@@ -634,7 +705,7 @@ __global__ void __launch_bounds__(kThreads) k_build_level(Level L) {
         bin_range(&bins, L, t, t.begin, t.end, ext, sc, tid);
         __syncthreads();
     }
-    if (tid == 0) decide(&bins, t, ext, dec);
+    if (tid < 64) decide_wave(&bins, t, ext, dec, tid);
     __syncthreads();
     if (dec.kind == 0) {
         emit_leaf(L, t, tid);
@@ -643,6 +714,118 @@ __global__ void __launch_bounds__(kThreads) k_build_level(Level L) {
     if (dec.kind == 2) median_boxes(L, t, dec, half, tid);
     partition_range(L, t, dec, sc, t.begin, t.end, t.begin, t.begin + dec.nleft, wsum, tid);
     if (tid == 0) emit_split(L, t, dec);
+}
+
+// Subtrees of at most kSmall triangles, one wave each, depth first: lane j
+// holds the triangle at position root.begin + j (its index, centroid and
+// box in registers); every node is binned in the wave's LDS, decided by
+// the wave with decide_wave(), and partitioned stably by moving the
+// lanes' triangles (ballot ranks, then a gather through LDS), so the
+// subtree is the one the level-by-level kernels would build.  Leaves write
+// their triangle records at their final positions.  `max_level` receives
+// the deepest level reached (in the level-by-level numbering).
+struct SubtreeLds {
+    Bins bins;
+    Decision dec;
+    uint32_t half[12];
+    uint32_t src[64];
+    SmallTask stack[kSmall];
+};
+constexpr uint32_t kWaves = kThreads / 64;
+
+__global__ void __launch_bounds__(kThreads) k_build_small(Level L, const SmallTask* tasks, uint32_t n_tasks,
+                                                         const uint32_t* perm0, const uint32_t* perm1,
+                                                         uint32_t* max_level) {
+    __shared__ SubtreeLds lds[kWaves];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t ti = blockIdx.x * kWaves + wave;
+    if (ti >= n_tasks) return;  // the whole wave; no workgroup barrier below
+    SubtreeLds& W = lds[wave];
+    SmallTask cur = tasks[ti];
+    const uint32_t base = cur.t.begin, n0 = cur.t.end - cur.t.begin;
+    const uint32_t* pin = cur.parity ? perm1 : perm0;
+    const bool has = lane < n0;
+    uint32_t p = has ? pin[base + lane] : 0u;
+    float c[3] = {0.0f, 0.0f, 0.0f}, bx[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (has) {
+        for (int k = 0; k < 3; ++k) c[k] = L.pcen[3 * (size_t)p + k];
+        for (int k = 0; k < 6; ++k) bx[k] = L.pbox[6 * (size_t)p + k];
+    }
+    uint32_t sp = 0, deepest = 0;
+    for (;;) {
+        const BuildTask& t = cur.t;
+        deepest = max(deepest, cur.level);
+        const uint32_t lo = t.begin - base, hi = t.end - base, n = hi - lo;
+        const bool mine = lane >= lo && lane < hi;
+        float ext[3], sc[3];
+        task_scales(t, ext, sc);
+        if (n > 1) {
+            bins_clear(&W.bins, lane, 64u);
+            __builtin_amdgcn_wave_barrier();
+            if (mine) bin_tri(&W.bins, t, c, bx, ext, sc);
+            __builtin_amdgcn_wave_barrier();
+        }
+        decide_wave(&W.bins, t, ext, W.dec, lane);
+        __builtin_amdgcn_wave_barrier();
+        const int kind = W.dec.kind;
+        if (kind == 0) {
+            if (mine) write_tri(L, base + lane, p, t.mesh);
+            if (lane == 0) leaf_ref(L, t);
+            if (sp == 0) break;
+            cur = W.stack[--sp];
+            continue;
+        }
+        const uint32_t nleft = W.dec.nleft;
+        if (kind == 2) {
+            // boxes of the two halves of a middle split
+            if (lane < 12) W.half[lane] = (lane % 6) < 3 ? kMappedPosInf : kMappedNegInf;
+            __builtin_amdgcn_wave_barrier();
+            if (mine) {
+                const int h = (lane - lo) < nleft ? 0 : 1;
+                for (int j = 0; j < 3; ++j) {
+                    atomicMin(&W.half[6 * h + j], map_f(bx[j]));
+                    atomicMax(&W.half[6 * h + 3 + j], map_f(bx[3 + j]));
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0)
+                for (int j = 0; j < 6; ++j) {
+                    W.dec.lbox[j] = unmap_f(W.half[j]);
+                    W.dec.rbox[j] = unmap_f(W.half[6 + j]);
+                }
+            __builtin_amdgcn_wave_barrier();
+        }
+        // stable partition of the lanes [lo, hi): left ones first
+        // (selects rather than a run-time index keep c, cbox and sc in registers)
+        const int axis = W.dec.axis;
+        const uint32_t bin = W.dec.bin;
+        const float ca = axis == 0 ? c[0] : (axis == 1 ? c[1] : c[2]);
+        const float la = axis == 0 ? t.cbox[0] : (axis == 1 ? t.cbox[1] : t.cbox[2]);
+        const float sa = axis == 0 ? sc[0] : (axis == 1 ? sc[1] : sc[2]);
+        const bool left = mine && (kind == 2 ? (lane - lo) < nleft : bin_of(ca, la, sa) <= bin);
+        const uint64_t lmask = __ballot(left), rmask = __ballot(mine && !left);
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t dest = lane;
+        if (mine) dest = left ? lo + (uint32_t)__popcll(lmask & below) : lo + nleft + (uint32_t)__popcll(rmask & below);
+        W.src[dest] = lane;
+        __builtin_amdgcn_wave_barrier();
+        const int from = (int)W.src[lane];
+        p = __shfl(p, from);
+        for (int k = 0; k < 3; ++k) c[k] = __shfl(c[k], from);
+        for (int k = 0; k < 6; ++k) bx[k] = __shfl(bx[k], from);
+        // the node, then its left child now and its right child later
+        uint32_t id = 0;
+        if (lane == 0) id = emit_node(L, t, W.dec);
+        id = __shfl(id, 0);
+        BuildTask c0, c1;
+        child_tasks(t, W.dec, id, c0, c1);
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) W.stack[sp] = SmallTask{c1, cur.parity, cur.level + 1, {0u, 0u}};
+        ++sp;
+        cur = SmallTask{c0, cur.parity, cur.level + 1, {0u, 0u}};
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) atomicMax(max_level, deepest);
 }
 
 // Large nodes, step 1: one workgroup bins a chunk in LDS and merges the
@@ -682,7 +865,7 @@ __global__ void __launch_bounds__(kThreads) k_split_large(Level L) {
     const uint32_t tid = threadIdx.x;
     float ext[3], sc[3];
     task_scales(t, ext, sc);
-    if (tid == 0) decide(L.gbins + blockIdx.x, t, ext, dec);
+    if (tid < 64) decide_wave(L.gbins + blockIdx.x, t, ext, dec, tid);
     __syncthreads();
     if (dec.kind == 2) median_boxes(L, t, dec, half, tid);
     __syncthreads();
@@ -929,14 +1112,17 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     WideTask* wtasks[2] = {tmp.get<WideTask>((size_t)nt + nm), tmp.get<WideTask>((size_t)nt + nm)};
     // [0] err [1] binary nodes [2,3] task counts [4] wide nodes [5] stack bound [6,7] wide task counts
     // [8] large nodes registered [9] their chunks [10] large children [11] their chunks
+    // [12] small subtrees [13] deepest level they reach
     uint32_t* counters = tmp.get<uint32_t>(16);
+    SmallTask* small = tmp.get<SmallTask>(std::max<uint32_t>(nt, 1));
     const size_t large_cap = nt / kLarge + 2, chunk_cap = nt / kChunk + large_cap;
     LargeTask* large = tmp.get<LargeTask>(large_cap);
     Bins* gbins = tmp.get<Bins>(large_cap);
     Chunk* chunks = tmp.get<Chunk>(chunk_cap);
     uint32_t* chunk_left = tmp.get<uint32_t>(chunk_cap);
     if (!lidx || !vofs || !tofs || !xform || !pbox || !pcen || !perm[0] || !perm[1] || !tasks[0] || !tasks[1] || !bnodes ||
-        !mbox || !rbox || !roots || !wtasks[0] || !wtasks[1] || !counters || !large || !gbins || !chunks || !chunk_left)
+        !mbox || !rbox || !roots || !wtasks[0] || !wtasks[1] || !counters || !large || !gbins || !chunks || !chunk_left ||
+        !small)
         return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     PICO_HIP(c, hipMemcpyAsync(lidx, d.indices, 12 * (size_t)nt, hipMemcpyHostToDevice, c->stream));
     PICO_HIP(c, hipMemcpyAsync(vofs, d.mesh_vertex_offset, 4 * ((size_t)nm + 1), hipMemcpyHostToDevice, c->stream));
@@ -976,7 +1162,7 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
         PICO_HIP(c, hipMemsetAsync(counters + 8, 0, 16, c->stream));
         Level L{tasks[cur], tasks[1 - cur], next_count, perm[cur], perm[1 - cur], pbox, pcen, s.pos, gidx, tofs,
                 bnodes, counters + 1, mesh_root2, tris, large, counters + 8, chunks, counters + 9, gbins, chunk_left,
-                counters + 10, counters + 11};
+                counters + 10, counters + 11, small, counters + 12, 1u - cur, levels};
         PICO_LAUNCH(c, "bvh_build_level", k_build_level, n_tasks, kThreads, 0, L);
         if (n_large) {
             PICO_LAUNCH(c, "bvh_bin_large", k_bin_large, n_chunks, kThreads, 0, L);
@@ -990,6 +1176,21 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
         n_large = h[10];
         n_chunks = h[11];
         cur = 1 - cur;
+    }
+    // the small subtrees, one wave each
+    uint32_t hsm[2] = {0, 0};
+    PICO_HIP(c, hipMemcpyAsync(hsm, counters + 12, 8, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    if (hsm[0] > std::max<uint32_t>(nt, 1)) return fail(c, PICO_ERR_LIMIT, "BVH build: work lists overflow");
+    if (hsm[0]) {
+        Level L{nullptr, nullptr, nullptr, nullptr, nullptr, pbox, pcen, s.pos, gidx, tofs, bnodes, counters + 1,
+                mesh_root2, tris, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                nullptr, 0u, 0u};
+        PICO_LAUNCH(c, "bvh_build_small", k_build_small, (hsm[0] + kWaves - 1) / kWaves, kThreads, 0, L, small, hsm[0],
+                    perm[0], perm[1], counters + 13);
+        PICO_HIP(c, hipMemcpyAsync(hsm + 1, counters + 13, 4, hipMemcpyDeviceToHost, c->stream));
+        PICO_HIP(c, hipStreamSynchronize(c->stream));
+        levels = std::max(levels, hsm[1]);
     }
     uint32_t n_bin = 0;
     PICO_HIP(c, hipMemcpyAsync(&n_bin, counters + 1, 4, hipMemcpyDeviceToHost, c->stream));
