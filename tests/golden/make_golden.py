@@ -19,7 +19,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle  # noqa: E402
 import pico_amd as pa  # noqa: E402
-from sky_scenes import sun_cube_sky, sun_hdri_sky  # noqa: E402
+from sky_scenes import shapes_sun_sky, sun_cube_sky, sun_hdri_sky  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
 
@@ -32,6 +32,7 @@ def scenes():
         "shapes": pa.Scene.shapes(1, pa.MICROFACET_BECKMANN_ALL),
         "sun_cube_sky": sun_cube_sky(),
         "sun_hdri_sky": sun_hdri_sky(),
+        "shapes_sun_sky": shapes_sun_sky(),
     }
 
 

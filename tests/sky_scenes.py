@@ -30,3 +30,13 @@ def sky_dirs(n, seed):
                      [0, 1, 1], [1, 1, 1], [-1, -1, -1], [1, -1, 0], [-1, 0, 1], [0, -1, -1]], float)
     d = np.concatenate([d, axes])
     return np.ascontiguousarray((d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32))
+
+
+def shapes_sun_sky(microfacet=pa.MICROFACET_BECKMANN_ALL):
+    """Every feature at once: the built-in sphere and cube (also as lights),
+    Fresnel and Transparent glass with the index-of-refraction stack, the sun
+    shining in through the open front and a cube-map sky."""
+    s = pa.Scene.shapes(1, microfacet)
+    d = np.array([-0.3, -0.4, 1.0])
+    faces = np.random.default_rng(5).integers(0, 256, (6, 8, 8, 4), dtype=np.uint8)
+    return s.set_sun(tuple(d / np.linalg.norm(d)), (2.0, 2.0, 1.5)).set_sky_image(faces)

@@ -9,11 +9,12 @@ import pytest
 
 import oracle
 import pico_amd as pa
-from sky_scenes import sun_cube_sky, sun_hdri_sky
+from sky_scenes import shapes_sun_sky, sun_cube_sky, sun_hdri_sky
 
 G = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"), allow_pickle=False)
 P = oracle._p
-SCENES = ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes", "sun_cube_sky", "sun_hdri_sky"]
+SCENES = ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes", "sun_cube_sky", "sun_hdri_sky",
+          "shapes_sun_sky"]
 
 
 def _scene(name):
@@ -21,6 +22,8 @@ def _scene(name):
         return sun_cube_sky()
     if name == "sun_hdri_sky":
         return sun_hdri_sky()
+    if name == "shapes_sun_sky":
+        return shapes_sun_sky()
     if name == "cornell_beckmann":
         return pa.Scene.cornell(1, pa.MICROFACET_BECKMANN_ALL)
     if name == "cornell_glass":

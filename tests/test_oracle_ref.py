@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 import pico_amd as pa
-from sky_scenes import sky_dirs, sun_cube_sky, sun_hdri_sky
+from sky_scenes import shapes_sun_sky, sky_dirs, sun_cube_sky, sun_hdri_sky
 
 pytestmark = pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference subset not built")
 
@@ -24,6 +24,7 @@ def scenes():
         "shapes": pa.Scene.shapes(1, pa.MICROFACET_BECKMANN_ALL),
         "sun_cube_sky": sun_cube_sky(),
         "sun_hdri_sky": sun_hdri_sky(),
+        "shapes_sun_sky": shapes_sun_sky(),
     }
 
 
@@ -225,7 +226,7 @@ def test_closest_hit_and_primary_ids(scenes, name):
 
 
 @pytest.mark.parametrize("name", ["cornell_beckmann", "instanced_small", "cornell_glass", "shapes", "sun_cube_sky",
-                                  "sun_hdri_sky"])
+                                  "sun_hdri_sky", "shapes_sun_sky"])
 def test_integrator_per_sample_reference_rng(scenes, name):
     s = scenes[name]
     cam = s.camera(1.0)
