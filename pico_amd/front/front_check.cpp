@@ -1,7 +1,10 @@
 // Drives the C++ front the way Pico's main() drives Scene::render_scene_to_file
-// (Source/main.cpp:97): one Cornell-box frame with denoising and tone mapping,
-// then the a-trous entry point on its own.  Exit status 0 on success; without
-// a visible GPU it reports the device error and exits 0 as well ("nodevice").
+// (Source/main.cpp:44-97): a JSON scene file loaded with its named camera (no
+// device needed), one Cornell-box frame with denoising and tone mapping, the
+// a-trous entry point on its own, a cancelled render, a Radiance file, and the
+// scene file rendered with its camera.  Exit status 0 on success; without a
+// visible GPU it reports the device error after the loading part and exits 0
+// as well ("nodevice").
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -16,7 +19,35 @@ extern "C" int pico_scene_desc_get(const struct pico_scene* s, pico_scene_desc* 
 extern "C" int pico_scene_default_camera(const struct pico_scene* s, float aspect, pico_camera* out);
 extern "C" void pico_scene_destroy(struct pico_scene* s);
 
+// A scene file with a named camera, as Pico's main() loads one (written to
+// the working directory).
+static void write_scene_file() {
+    FILE* o = std::fopen("front_check_quad.obj", "w");
+    FILE* j = std::fopen("front_check_scene.json", "w");
+    if (!o || !j) return;
+    std::fputs("v -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\nf 1 2 3 4\n", o);
+    std::fputs("{ // comments and trailing commas, as jsoncpp reads them\n"
+               " \"GLOBALS\": {\"S\": {\"SunDirection\": [0.3, -1, 0.2], \"SunColour\": [2, 2, 2]}},\n"
+               " \"MESH\": {\"Quad\": {\"Path\": \"front_check_quad.obj\"}},\n"
+               " \"MATERIALS\": {\"Grey\": {\"Type\": \"Constant\", \"Albedo\": [0.6, 0.6, 0.6], \"Roughness\": 0.5},},\n"
+               " \"INSTANCE\": {\"floor\": {\"Asset\": \"Quad\", \"BSRDF\": \"Diffuse\", \"Material\": \"Grey\"},\n"
+               "              \"ball\": {\"Asset\": \"Sphere\", \"BSRDF\": \"Dielectric\", \"Material\": \"Grey\",\n"
+               "                       \"Position\": [0, 0.5, 0]}},\n"
+               " \"CAMERA\": {\"main\": {\"Position\": [0, 1, -3], \"Direction\": [0, -0.3, 1], \"FarPlane\": 100}},\n"
+               "}\n", j);
+    std::fclose(o);
+    std::fclose(j);
+}
+
 int main() {
+    // the scene file loads without a device
+    write_scene_file();
+    pico_front::SceneFile sf("front_check_scene.json");
+    std::remove("front_check_quad.obj");
+    std::remove("front_check_scene.json");
+    if (sf.status() != PICO_OK) { std::printf("scene file %d %s\n", sf.status(), sf.error()); return 11; }
+    const pico_camera* c = sf.get_camera("main");
+    if (!c || sf.get_camera("none") || sf.desc().n_instances != 2 || !sf.desc().use_sun) return 12;
     pico_front::Renderer r(0);
     if (r.status() != PICO_OK) {
         std::printf("nodevice status=%d\n", r.status());
@@ -59,6 +90,17 @@ int main() {
     std::fclose(f);
     std::remove(path);
     pico_scene_destroy(s);
+    // the scene file rendered with its camera
+    {
+        if (int rc = r.upload(sf)) { std::printf("upload file %d %s\n", rc, r.last_error().c_str()); return 13; }
+        std::vector<uint32_t> cnt4(W * H, 0);
+        pico_front::RenderParams<vec3> p4{5, 4, 0.0f, H, W, false, false, pix.data(), cnt4.data(), var.data()};
+        bool go = false;
+        if (int rc = r.render_scene_to_memory(*c, p4, &go)) { std::printf("render file %d %s\n", rc, r.last_error().c_str()); return 15; }
+        double lit = 0.0;
+        for (uint32_t i = 0; i < W * H; ++i) lit += pix[i].x;
+        if (!(lit > 0.0)) return 16;  // the sun lights the floor
+    }
     std::printf("ok\n");
     return 0;
 }

@@ -6,6 +6,8 @@
 //       (Source/Core/Scene.cpp:236-246)
 //   glm::vec3* Util::atrous_denoise(pixels, normals, position, diffuse, Tiler&)
 //       (Source/Util/Denoisers.hpp:9; returns a new[] buffer the caller delete[]s)
+//   Scene::Scene(ThreadPool&, const std::filesystem::path& sceneFile) and
+//   Scene::get_camera(name) (Source/Core/Scene.hpp:64-79): SceneFile
 //
 // Header-only; it calls the C ABI of libpico_hip.so (include/pico_hip.h) and
 // throws nothing: failures are reported through the int status of each call,
@@ -18,6 +20,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <string>
 #include <thread>
 
@@ -40,6 +43,44 @@ struct RenderParams {
     Vec3* m_variance;
 };
 
+// A scene file in the reference's JSON format, loaded as Scene::Scene(ThreadPool&,
+// path) loads it, with its named cameras as Scene::get_camera gives them.
+class SceneFile {
+public:
+    explicit SceneFile(const std::string& path) {
+        status_ = pico_scene_load_json(path.c_str(), &scene_, error_, sizeof error_);
+        if (status_ != PICO_OK) return;
+        status_ = pico_scene_desc_get(scene_, &desc_);
+        uint32_t n = 0;
+        if (status_ == PICO_OK) status_ = pico_scene_camera_count(scene_, &n);
+        for (uint32_t i = 0; status_ == PICO_OK && i < n; ++i) {
+            char name[256];
+            pico_camera cam{};
+            status_ = pico_scene_camera_get(scene_, i, name, sizeof name, &cam);
+            if (status_ == PICO_OK) cameras_[name] = cam;
+        }
+    }
+    ~SceneFile() { pico_scene_destroy(scene_); }
+    SceneFile(const SceneFile&) = delete;
+    SceneFile& operator=(const SceneFile&) = delete;
+
+    int status() const { return status_; }
+    const char* error() const { return error_; }
+    const pico_scene_desc& desc() const { return desc_; }
+    // nullptr when the file has no camera of that name, as the reference
+    const pico_camera* get_camera(const std::string& name) const {
+        auto it = cameras_.find(name);
+        return it == cameras_.end() ? nullptr : &it->second;
+    }
+
+private:
+    pico_scene* scene_ = nullptr;
+    pico_scene_desc desc_{};
+    std::map<std::string, pico_camera> cameras_;
+    int status_ = PICO_ERR_STATE;
+    char error_[512] = {0};
+};
+
 class Renderer {
 public:
     explicit Renderer(int device = 0) { status_ = pico_hip_create(device, &ctx_); }
@@ -53,6 +94,7 @@ public:
 
     // Scene::Scene's BVH build (Source/Core/Scene.cpp:36-141): a flattened scene.
     int upload(const pico_scene_desc& scene) { return ctx_ ? pico_hip_upload_scene(ctx_, &scene) : status_; }
+    int upload(const SceneFile& file) { return file.status() != PICO_OK ? file.status() : upload(file.desc()); }
 
     // Scene::render_scene_to_memory: render (resuming from the pixels'
     // sample counts like the reference does), then, as the reference orders

@@ -1,6 +1,8 @@
 """The C++ front (pico_amd/front/pico_front.hpp) compiles against the C ABI,
-links against libpico_hip.so and, with a GPU, renders a denoised, tone-mapped
-Cornell frame with the call shape of Scene::render_scene_to_memory."""
+links against libpico_hip.so, loads a JSON scene file with its named camera
+(SceneFile, as Scene::Scene(ThreadPool&, path) and Scene::get_camera) and,
+with a GPU, renders a denoised, tone-mapped Cornell frame and the scene file
+with the call shape of Scene::render_scene_to_memory."""
 import os
 import subprocess
 
