@@ -171,6 +171,115 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     }
 }
 
+// The same convolution for 3x3 taps with every operand from LDS.  Per K
+// chunk of KS input channels the workgroup stages the three input rows its
+// output row reads (its run of pixels plus one on each side, zero outside
+// the image), then the chunk's weights one row of three taps at a time; the
+// taps read their A fragments (16 consecutive pixels x KS channels: one
+// contiguous block per wave) and B fragments from LDS.  Input rows are read
+// from memory once per chunk instead of once per tap.
+template <int MT, int NT, int KS>
+__global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
+    extern __shared__ bf16 lds[];
+    constexpr int kRun = 4 * MT * 16;
+    constexpr int kBand = kRun + 2;     // pixels of one staged row
+    constexpr int ldw = KS + 8;         // weight rows padded against bank conflicts
+    constexpr int KL = KS / 4;          // channels per lane and fragment
+    bf16* band = lds;                   // [3][kBand][KS]
+    bf16* wl = lds + 3 * kBand * KS;    // [3][NT * 16][ldw]: one row of taps
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int runs_x = (a.W + kRun - 1) / kRun;
+    const int row = blockIdx.x / runs_x;  // b*H + y
+    const int x0 = (blockIdx.x % runs_x) * kRun;
+    const int xw = x0 + wave * MT * 16;
+    const int b = row / a.H, y = row % a.H;
+    const int n0 = blockIdx.y * NT * 16;
+    const int mi = lane & 15, kq = lane >> 4;
+    f4v acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    const bool live = xw < a.W;  // every wave takes part in the barriers
+    constexpr int kVec = KS / 8;      // 16-byte vectors per pixel and chunk
+    for (int k0 = 0; k0 < a.kpad; k0 += KS) {
+        __syncthreads();
+        // input band of this chunk
+        for (int e = threadIdx.x; e < 3 * kBand * kVec; e += 256) {
+            const int r = e / (kBand * kVec), rem = e - r * kBand * kVec;
+            const int px = rem / kVec, c8 = rem - px * kVec;
+            const int yy = y + r - 1, xx = x0 + px - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                v = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + yy) * a.W + xx) * a.in_stride + a.in_off +
+                                                    k0 + 8 * c8);
+            *reinterpret_cast<uint4*>(band + (r * kBand + px) * KS + 8 * c8) = v;
+        }
+        // the taps in three groups of three (one row of the 3x3 kernel each)
+        for (int g = 0; g < 3; ++g) {
+            if (g) __syncthreads();
+            for (int e = threadIdx.x; e < 3 * NT * 16 * kVec; e += 256) {
+                const int tg = e / (NT * 16 * kVec), rem = e - tg * NT * 16 * kVec;
+                const int nn = rem / kVec, c8 = rem - nn * kVec;
+                *reinterpret_cast<uint4*>(wl + (tg * NT * 16 + nn) * ldw + 8 * c8) = *reinterpret_cast<const uint4*>(
+                    a.w + ((size_t)(3 * g + tg) * a.npad + n0 + nn) * a.kpad + k0 + 8 * c8);
+            }
+            __syncthreads();
+            if (!live) continue;
+            for (int tg = 0; tg < 3; ++tg) {
+                const int tap = 3 * g + tg;
+                int dy = tap / 3 - 1, dx = tap % 3 - 1;
+                if (a.flip) { dy = -dy; dx = -dx; }
+                const bf16* ab = band + ((dy + 1) * kBand + (xw - x0) + mi + dx + 1) * KS + KL * kq;
+                const bf16* bb = wl + (tg * NT * 16 + mi) * ldw + KL * kq;
+                if constexpr (KS == 32) {
+                    uint4 av[MT], bv[NT];
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const uint4*>(ab + m * 16 * KS);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const uint4*>(bb + t * 16 * ldw);
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[m]),
+                                                                                __builtin_bit_cast(bf16x8, bv[t]), acc[m][t], 0, 0, 0);
+                } else {
+                    s4 av[MT], bv[NT];
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const s4*>(ab + m * 16 * KS);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const s4*>(bb + t * 16 * ldw);
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[t], acc[m][t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (xw >= a.W) return;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int n = n0 + t * 16 + mi;
+        const float bias = a.bias ? a.bias[n] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = xw + m * 16 + 4 * kq + r;
+                if (x >= a.W) continue;
+                float v = acc[m][t][r] + bias;
+                if (a.relu) v = v > 0.0f ? v : 0.0f;
+                const size_t p = (size_t)(b * a.H + y) * a.W + x;
+                if (a.mask && !(bf2f(a.mask[p * a.mask_stride + a.mask_off + n]) > 0.0f)) v = 0.0f;
+                if (a.out_f32) a.out_f32[p * a.npad + n] = v;
+                else a.out[p * a.out_stride + a.out_off + n] = f2bf(v);
+            }
+    }
+}
+
 struct WgradArgs {
     const bf16* x; int x_stride, x_off;   // layer input (kpad channels)
     const bf16* g; int g_stride, g_off;   // gradient of the pre-activation (npad channels)
@@ -504,6 +613,14 @@ int launch_conv_k(pico_cnn* n, const char* name, const ConvArgs& a) {
     const int nt = (a.npad % 64 == 0) ? 4 : (a.npad % 32 == 0) ? 2 : 1;
     const int run = 4 * (8 / nt) * 16;
     const dim3 grid((unsigned)(a.B * a.H * ((a.W + run - 1) / run)), (unsigned)(a.npad / (nt * 16)));
+    if (a.taps == 9 && nt >= 2) {
+        // 3x3 with all operands in LDS: three rows of run + 2 pixels and
+        // one row of three taps' weights, per chunk of KS channels
+        const size_t lds3 = (3 * (size_t)(run + 2) * KS + 3 * (size_t)nt * 16 * (KS + 8)) * sizeof(bf16);
+        if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv3<2, 4, KS>), grid, 256, lds3, a);
+        else PICO_LAUNCH(n->c, name, (k_conv3<4, 2, KS>), grid, 256, lds3, a);
+        return PICO_OK;
+    }
     const size_t lds = (size_t)nt * 16 * (a.kpad + 8) * sizeof(bf16);
     if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv<2, 4, KS>), grid, 256, lds, a);
     else if (nt == 2) PICO_LAUNCH(n->c, name, (k_conv<4, 2, KS>), grid, 256, lds, a);
