@@ -101,7 +101,8 @@ typedef struct pico_material {
  * and for the G-buffer's sky pixels.  faces = 6: a cube map, faces +x, -x,
  * +y, -y, +z, -z one after the other (a JSON "Skybox" array); faces = 1: an
  * equirectangular image (a JSON "Skybox" HDRI file).  Texels are row-major,
- * the first row first; RGBA8 texels are divided by 255, alpha is unused. */
+ * the first row first; RGBA8 texels are divided by 255, alpha is unused.
+ * At most 2^26 texels (all faces). */
 #define PICO_TEXEL_RGBA8   0u   /* 4 x u8  */
 #define PICO_TEXEL_RGBA32F 1u   /* 4 x f32 */
 typedef struct pico_sky_image {

@@ -313,7 +313,7 @@ std::string check_scene(const pico_scene_desc& d) {
         if (k.faces != 1 && k.faces != 6) return "sky image needs 1 (equirectangular) or 6 (cube) faces";
         if (k.format > PICO_TEXEL_RGBA32F) return "unknown sky texel format";
         if (k.width == 0 || k.height == 0 || !k.texels) return "sky image is empty";
-        if ((uint64_t)k.width * k.height * k.faces > (1ull << 28)) return "sky image too large";
+        if ((uint64_t)k.width * k.height * k.faces > (1ull << 26)) return "sky image larger than 2^26 texels";
     }
     return "";
 }

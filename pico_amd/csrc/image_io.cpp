@@ -263,7 +263,7 @@ std::string decode_png(const std::vector<uint8_t>& f, uint32_t& w, uint32_t& h, 
             if (depth != 1 && depth != 2 && depth != 4 && depth != 8 && depth != 16) return "bad bit depth";
             if (color > 6 || (color != 3 && (color & 1)) || (color == 3 && depth == 16)) return "bad colour type";
             if ((color == 2 || color == 4 || color == 6) && depth < 8) return "bad bit depth for the colour type";
-            if (!w || !h || w > (1u << 24) || h > (1u << 24) || (uint64_t)w * h > (1ull << 28)) return "bad image size";
+            if (!w || !h || w > (1u << 24) || h > (1u << 24) || (uint64_t)w * h > (1ull << 26)) return "image larger than 2^26 pixels";
         } else if (!have_ihdr) {
             return "the first chunk is not IHDR";
         } else if (is("PLTE")) {
@@ -444,7 +444,7 @@ std::string decode_rgbe(const std::vector<uint8_t>& f, uint32_t& w, uint32_t& h,
     while (*end == ' ') ++end;
     if (std::strncmp(end, "+X ", 3) != 0) return "RGBE layout other than -Y h +X w";
     ww = std::strtol(end + 3, nullptr, 10);
-    if (ww <= 0 || hh <= 0 || ww > (1 << 24) || hh > (1 << 24) || (uint64_t)ww * hh > (1ull << 28)) return "bad RGBE image size";
+    if (ww <= 0 || hh <= 0 || ww > (1 << 24) || hh > (1 << 24) || (uint64_t)ww * hh > (1ull << 26)) return "RGBE image empty or larger than 2^26 pixels";
     w = (uint32_t)ww;
     h = (uint32_t)hh;
     rgba.assign((size_t)w * h * 4, 0.0f);

@@ -486,7 +486,7 @@ int pico_scene_set_sky_image(pico_scene* s, const pico_sky_image* img) {
         return PICO_OK;
     }
     if ((img->faces != 1 && img->faces != 6) || img->format > PICO_TEXEL_RGBA32F || !img->width || !img->height ||
-        !img->texels || (uint64_t)img->width * img->height * img->faces > (1ull << 28))
+        !img->texels || (uint64_t)img->width * img->height * img->faces > (1ull << 26))
         return PICO_ERR_ARG;
     const size_t bytes = (size_t)img->width * img->height * img->faces * (img->format == PICO_TEXEL_RGBA32F ? 16 : 4);
     const uint8_t* p = static_cast<const uint8_t*>(img->texels);
