@@ -2,7 +2,8 @@
 pinned by closed forms: a constant image is a fixed point; with every
 edge-stopping term off it is the clamped, dilated B3-spline blur computed
 here by brute force; a single bright pixel under the colour term alone
-keeps the weight the definition gives it; edges in normals stop the blur."""
+keeps the weight the definition gives it; edges in normals stop the blur;
+the colour width halves from one level to the next."""
 import numpy as np
 
 import oracle
@@ -82,3 +83,41 @@ def test_normal_edge_stops_the_blur():
                               (0.1, 0.0, 0.0, 0.0)).reshape(h, w, 3)
     # |dn|^2 = 2 across the edge: weight exp(-200), nothing crosses
     np.testing.assert_allclose(out, rgb, atol=1e-30)
+
+
+def _colour_pass(img, w, h, step, sigma):
+    """One pass of the colour-only edge-stopping B3 filter (Dammertz et al.
+    2010): weight h(u) h(v) exp(-|c_q - c_p|^2 / sigma^2), clamped taps at
+    offsets step * (-2..2), normalised by the weight sum."""
+    hb = np.array([1, 4, 6, 4, 1], np.float64) / 16
+    a = img.reshape(h, w, 3).astype(np.float64)
+    out = np.zeros_like(a)
+    for y in range(h):
+        for x in range(w):
+            acc, ws = np.zeros(3), 0.0
+            for v in range(5):
+                for u in range(5):
+                    yy = min(max(y + (v - 2) * step, 0), h - 1)
+                    xx = min(max(x + (u - 2) * step, 0), w - 1)
+                    wt = hb[u] * hb[v] * np.exp(-np.sum((a[yy, xx] - a[y, x]) ** 2) / sigma ** 2)
+                    acc += wt * a[yy, xx]
+                    ws += wt
+            out[y, x] = acc / ws
+    return out.reshape(-1, 3)
+
+
+def test_colour_width_halves_per_level():
+    """Level l uses the colour width sigma / 2^l: with the other terms off,
+    two levels equal one level followed by a step-2 pass at half the width
+    (and clearly not a step-2 pass at the full width)."""
+    rng = np.random.default_rng(11)
+    w, h = 13, 9
+    rgb = rng.uniform(0, 1, (w * h, 3)).astype(np.float32)
+    nrm, pos, alb = _gbuf(rng, w * h)
+    sigma = 0.6
+    one = oracle.atrous_fixed(rgb, nrm, pos, alb, w, h, 1, (0.0, 0.0, 0.0, sigma))
+    two = oracle.atrous_fixed(rgb, nrm, pos, alb, w, h, 2, (0.0, 0.0, 0.0, sigma))
+    np.testing.assert_allclose(one, _colour_pass(rgb, w, h, 1, sigma), rtol=1e-5, atol=1e-6)
+    halved = _colour_pass(one, w, h, 2, sigma / 2)
+    np.testing.assert_allclose(two, halved, rtol=1e-5, atol=1e-6)
+    assert np.abs(_colour_pass(one, w, h, 2, sigma) - halved).max() > 1e-2
