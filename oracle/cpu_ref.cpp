@@ -4,9 +4,11 @@
  * A clean-room CPU restatement of Pico's rendering hot path: reference-style
  * binary BVH with the reference's binned SAH builder, recursive closest-hit
  * traversal, Moller-Trumbore intersection with the reference's epsilons, the
- * Monte-Carlo integrator with its next-event estimation, the BSDFs and
- * distributions, the per-pixel accumulation loop, the G-buffer pass, the
- * a-trous denoiser and the Reinhard tone mapper.  Each function cites the
+ * Monte-Carlo integrator with its next-event estimation (mesh, built-in
+ * shape and sun lights), the BSDFs and distributions (the Transparent and
+ * Fresnel BTDFs with the ray's index-of-refraction stack included), the
+ * built-in sphere and cube, the sky image lookup, the per-pixel accumulation
+ * loop, the G-buffer pass, the a-trous denoiser and the Reinhard tone mapper.  Each function cites the
  * reference file:line it follows.  Arithmetic is written in the reference's
  * evaluation order (glm evaluates dot products as (x*x + y*y) + z*z and
  * matrix-vector products column by column) so that, with the reference's own
