@@ -545,10 +545,8 @@ __device__ void count_large_child(const Level& L, uint32_t n) {
     }
 }
 
-// One lane: the binary node of a split and its reference in the parent;
-// returns the node's index.
-__device__ uint32_t emit_node(const Level& L, const BuildTask& t, const Decision& d) {
-    const uint32_t id = atomicAdd(L.bcount, 1u);
+// One lane: the binary node `id` of a split and its reference in the parent.
+__device__ void write_node(const Level& L, const BuildTask& t, const Decision& d, uint32_t id) {
     DNode nd{};
     DBox lb, rb;
     for (int j = 0; j < 3; ++j) {
@@ -559,6 +557,12 @@ __device__ uint32_t emit_node(const Level& L, const BuildTask& t, const Decision
     set_child(nd, 1, rb, kEmptyRef);
     L.bnodes[id] = nd;
     write_ref(L, t.slot, id);
+}
+
+// One lane: write_node() at a new index; returns it.
+__device__ uint32_t emit_node(const Level& L, const BuildTask& t, const Decision& d) {
+    const uint32_t id = atomicAdd(L.bcount, 1u);
+    write_node(L, t, d, id);
     return id;
 }
 
@@ -724,6 +728,12 @@ __global__ void __launch_bounds__(kThreads) k_build_level(Level L) {
 // subtree is the one the level-by-level kernels would build.  Leaves write
 // their triangle records at their final positions.  `max_level` receives
 // the deepest level reached (in the level-by-level numbering).
+// Node indices: every split divides at least two triangles into two
+// non-empty parts, so a subtree of n triangles has at most n - 1 binary
+// nodes; the wave reserves that many with one atomic (one per node on the
+// shared counter serialises the whole launch) and adds the ones it leaves
+// unused to `unused`, so the node count stays exact.  Unused indices are
+// never referenced.
 struct SubtreeLds {
     Bins bins;
     Decision dec;
@@ -735,7 +745,7 @@ constexpr uint32_t kWaves = kThreads / 64;
 
 __global__ void __launch_bounds__(kThreads) k_build_small(Level L, const SmallTask* tasks, uint32_t n_tasks,
                                                          const uint32_t* perm0, const uint32_t* perm1,
-                                                         uint32_t* max_level) {
+                                                         uint32_t* max_level, uint32_t* unused) {
     __shared__ SubtreeLds lds[kWaves];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t ti = blockIdx.x * kWaves + wave;
@@ -751,6 +761,9 @@ __global__ void __launch_bounds__(kThreads) k_build_small(Level L, const SmallTa
         for (int k = 0; k < 3; ++k) c[k] = L.pcen[3 * (size_t)p + k];
         for (int k = 0; k < 6; ++k) bx[k] = L.pbox[6 * (size_t)p + k];
     }
+    uint32_t first_id = 0, used = 0;
+    if (lane == 0 && n0 > 1) first_id = atomicAdd(L.bcount, n0 - 1);
+    first_id = __shfl(first_id, 0);
     uint32_t sp = 0, deepest = 0;
     for (;;) {
         const BuildTask& t = cur.t;
@@ -814,9 +827,8 @@ __global__ void __launch_bounds__(kThreads) k_build_small(Level L, const SmallTa
         for (int k = 0; k < 3; ++k) c[k] = __shfl(c[k], from);
         for (int k = 0; k < 6; ++k) bx[k] = __shfl(bx[k], from);
         // the node, then its left child now and its right child later
-        uint32_t id = 0;
-        if (lane == 0) id = emit_node(L, t, W.dec);
-        id = __shfl(id, 0);
+        const uint32_t id = first_id + used++;
+        if (lane == 0) write_node(L, t, W.dec, id);
         BuildTask c0, c1;
         child_tasks(t, W.dec, id, c0, c1);
         __builtin_amdgcn_wave_barrier();
@@ -825,7 +837,10 @@ __global__ void __launch_bounds__(kThreads) k_build_small(Level L, const SmallTa
         cur = SmallTask{c0, cur.parity, cur.level + 1, {0u, 0u}};
         __builtin_amdgcn_wave_barrier();
     }
-    if (lane == 0) atomicMax(max_level, deepest);
+    if (lane == 0) {
+        atomicMax(max_level, deepest);
+        if (n0 > 1 + used) atomicAdd(unused, n0 - 1 - used);
+    }
 }
 
 // Large nodes, step 1: one workgroup bins a chunk in LDS and merges the
@@ -1112,7 +1127,7 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     WideTask* wtasks[2] = {tmp.get<WideTask>((size_t)nt + nm), tmp.get<WideTask>((size_t)nt + nm)};
     // [0] err [1] binary nodes [2,3] task counts [4] wide nodes [5] stack bound [6,7] wide task counts
     // [8] large nodes registered [9] their chunks [10] large children [11] their chunks
-    // [12] small subtrees [13] deepest level they reach
+    // [12] small subtrees [13] deepest level they reach [14] binary node indices they reserved and left unused
     uint32_t* counters = tmp.get<uint32_t>(16);
     SmallTask* small = tmp.get<SmallTask>(std::max<uint32_t>(nt, 1));
     const size_t large_cap = nt / kLarge + 2, chunk_cap = nt / kChunk + large_cap;
@@ -1187,13 +1202,13 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
                 mesh_root2, tris, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                 nullptr, 0u, 0u};
         PICO_LAUNCH(c, "bvh_build_small", k_build_small, (hsm[0] + kWaves - 1) / kWaves, kThreads, 0, L, small, hsm[0],
-                    perm[0], perm[1], counters + 13);
+                    perm[0], perm[1], counters + 13, counters + 14);
         PICO_HIP(c, hipMemcpyAsync(hsm + 1, counters + 13, 4, hipMemcpyDeviceToHost, c->stream));
         PICO_HIP(c, hipStreamSynchronize(c->stream));
         levels = std::max(levels, hsm[1]);
     }
-    uint32_t n_bin = 0;
-    PICO_HIP(c, hipMemcpyAsync(&n_bin, counters + 1, 4, hipMemcpyDeviceToHost, c->stream));
+    uint32_t nb[14] = {};
+    PICO_HIP(c, hipMemcpyAsync(nb, counters + 1, sizeof(nb), hipMemcpyDeviceToHost, c->stream));
     // 4-wide collapse
     PICO_LAUNCH(c, "bvh_roots4", k_roots4, grid_for(nm), 256, 0, mesh_root2, nm, wtasks[0], counters + 6);
     uint32_t n_wide_tasks = nm;
@@ -1238,7 +1253,7 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
         *stats = pico_build_stats{};
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         stats->bvh_ms = blas_ms + ts.device_ms;
-        stats->binary_nodes = n_bin;
+        stats->binary_nodes = nb[0] - nb[13];  // allocated less reserved-but-unused
         stats->blas_nodes4 = n_blas4;
         stats->blas_bound = blas_bound;
         stats->stack_bound = ts.stack_bound;
