@@ -743,7 +743,16 @@ struct SubtreeLds {
 };
 constexpr uint32_t kWaves = kThreads / 64;
 
-__global__ void __launch_bounds__(kThreads) k_build_small(Level L, const SmallTask* tasks, uint32_t n_tasks,
+// A task that every lane of the wave holds alike, moved to scalar registers.
+__device__ __forceinline__ void to_scalar(SmallTask& s) {
+    static_assert(sizeof(SmallTask) % 4 == 0, "SmallTask is whole words");
+    uint32_t w[sizeof(SmallTask) / 4];
+    __builtin_memcpy(w, &s, sizeof(SmallTask));
+    for (uint32_t k = 0; k < sizeof(SmallTask) / 4; ++k) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
+    __builtin_memcpy(&s, w, sizeof(SmallTask));
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_build_small(Level L, const SmallTask* tasks, uint32_t n_tasks,
                                                          const uint32_t* perm0, const uint32_t* perm1,
                                                          uint32_t* max_level, uint32_t* unused) {
     __shared__ SubtreeLds lds[kWaves];
@@ -766,6 +775,7 @@ __global__ void __launch_bounds__(kThreads) k_build_small(Level L, const SmallTa
     first_id = __shfl(first_id, 0);
     uint32_t sp = 0, deepest = 0;
     for (;;) {
+        to_scalar(cur);
         const BuildTask& t = cur.t;
         deepest = max(deepest, cur.level);
         const uint32_t lo = t.begin - base, hi = t.end - base, n = hi - lo;
