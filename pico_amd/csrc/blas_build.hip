@@ -725,8 +725,8 @@ __global__ void __launch_bounds__(kThreads) k_build_level(Level L) {
 // box in registers); every node is binned in the wave's LDS, decided by
 // the wave with decide_wave(), and partitioned stably by moving the
 // lanes' triangles (ballot ranks, then a gather through LDS), so the
-// subtree is the one the level-by-level kernels would build.  Leaves write
-// their triangle records at their final positions.  `max_level` receives
+// subtree is the one the level-by-level kernels would build.  The triangle
+// records are written at their final positions once the subtree is done.  `max_level` receives
 // the deepest level reached (in the level-by-level numbering).
 // Node indices: every split divides at least two triangles into two
 // non-empty parts, so a subtree of n triangles has at most n - 1 binary
@@ -761,7 +761,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
     if (ti >= n_tasks) return;  // the whole wave; no workgroup barrier below
     SubtreeLds& W = lds[wave];
     SmallTask cur = tasks[ti];
-    const uint32_t base = cur.t.begin, n0 = cur.t.end - cur.t.begin;
+    const uint32_t base = cur.t.begin, n0 = cur.t.end - cur.t.begin, mesh = cur.t.mesh;
     const uint32_t* pin = cur.parity ? perm1 : perm0;
     const bool has = lane < n0;
     uint32_t p = has ? pin[base + lane] : 0u;
@@ -792,7 +792,6 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
         __builtin_amdgcn_wave_barrier();
         const int kind = W.dec.kind;
         if (kind == 0) {
-            if (mine) write_tri(L, base + lane, p, t.mesh);
             if (lane == 0) leaf_ref(L, t);
             if (sp == 0) break;
             cur = W.stack[--sp];
@@ -847,6 +846,9 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
         cur = SmallTask{c0, cur.parity, cur.level + 1, {0u, 0u}};
         __builtin_amdgcn_wave_barrier();
     }
+    // no partition moves a lane's triangle once its leaf is emitted, so lane j
+    // now holds the triangle of position root.begin + j: all records at once
+    if (has) write_tri(L, base + lane, p, mesh);
     if (lane == 0) {
         atomicMax(max_level, deepest);
         if (n0 > 1 + used) atomicAdd(unused, n0 - 1 - used);
