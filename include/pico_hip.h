@@ -362,7 +362,9 @@ int  pico_hip_atrous_fixed_d(pico_hip_ctx* ctx, const float* d_rgb, const float*
 int  pico_hip_tonemap(pico_hip_ctx* ctx, float* rgb, uint32_t width, uint32_t height);
 
 /* relMSE = mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2) with the
- * reference r = d_ref_sum / ref_count (FP64), reduced in FP64. */
+ * reference r = d_ref_sum / ref_count (FP64), reduced in FP64 in a fixed
+ * order (the same inputs give the same bits).  At most 2^28 pixels;
+ * n_pixels = 0 gives 0.  Synchronises the context's stream. */
 int  pico_hip_relmse_d(pico_hip_ctx* ctx, const float* d_img, const double* d_ref_sum, double ref_count,
                        uint32_t n_pixels, double* out);
 

@@ -38,6 +38,10 @@ constexpr int kRayBlock = 128;
 // kLds entries per lane in LDS, the rest (up to kSpill) in device memory.
 constexpr int kLds = 32;
 constexpr uint32_t kSpill = 32;
+// Uploads refuse scenes whose traversal needs more than 64 entries, so these
+// stacks never overflow; the kernels' overflow flags are a second guard
+// (read back by the entry points that synchronise).
+static_assert(kLds + kSpill >= 64, "ray-batch stacks hold the 64 entries an upload admits");
 
 struct StackArgs { uint32_t* spill; uint32_t cap; };
 
@@ -439,7 +443,10 @@ int pico_hip_primary_ids(pico_hip_ctx* c, const pico_camera* cam, uint32_t w, ui
     PICO_RAY_LAUNCH(c, "primary_ids", k_primary_ids, (uint32_t)n, c->dev, dc, make_float2(xx, xy), b, b + n, ovf);
     PICO_HIP(c, hipMemcpyAsync(inst, b, 4 * n, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipMemcpyAsync(prim, b + n, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    uint32_t o_flag = 0;
+    PICO_HIP(c, hipMemcpyAsync(&o_flag, ovf, 4, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
+    if (o_flag) return fail(c, PICO_ERR_LIMIT, "traversal stack overflow");
     return PICO_OK;
 }
 
@@ -483,7 +490,10 @@ int pico_hip_gbuffer(pico_hip_ctx* c, const pico_camera* cam, uint32_t w, uint32
     PICO_HIP(c, hipMemcpyAsync(n, b, 12 * px, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipMemcpyAsync(p, b + 3 * px, 12 * px, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipMemcpyAsync(a, b + 6 * px, 12 * px, hipMemcpyDeviceToHost, c->stream));
+    uint32_t o_flag = 0;
+    PICO_HIP(c, hipMemcpyAsync(&o_flag, scratch(c, "gb_ovf", 64), 4, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
+    if (o_flag) return fail(c, PICO_ERR_LIMIT, "traversal stack overflow");
     return PICO_OK;
 }
 

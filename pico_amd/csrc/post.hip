@@ -176,19 +176,51 @@ __global__ void k_reinhard(float* rgb, uint32_t n3, const uint32_t* white_bits) 
     }
 }
 
-__global__ void k_relmse(const float* __restrict__ img, const double* __restrict__ ref, double inv_count, uint32_t n3,
-                         double* out) {
+// relMSE in two deterministic stages: block b sums its grid-stride share of
+// the terms into part[b] (every block writes its slot, so nothing needs
+// clearing first); one block then adds the parts in block order and writes
+// the mean.  The same grid for the same size gives the same bits.
+constexpr uint32_t kRelmseBlocks = 1024;
+
+__global__ void k_zero_f64(double* p) { *p = 0.0; }
+
+__global__ __launch_bounds__(256) void k_relmse_part(const float* __restrict__ img, const double* __restrict__ ref,
+                                                     double inv_count, size_t n3, double* __restrict__ part) {
+    __shared__ double red[4];
     double acc = 0.0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n3; i += (size_t)gridDim.x * 256) {
         const double r = ref[i] * inv_count;
         const double d = (double)img[i] - r;
         acc += d * d / (r * r + 1e-2);
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-__global__ void k_zero_f64(double* p) { *p = 0.0; }
+__global__ __launch_bounds__(256) void k_relmse_final(const double* __restrict__ part, uint32_t n_part, double inv_n3,
+                                                      double* __restrict__ out) {
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n_part; i += 256) acc += part[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = ((red[0] + red[1]) + (red[2] + red[3])) * inv_n3;
+}
+
+inline uint32_t relmse_grid(size_t n3) { return (uint32_t)std::min<size_t>((n3 + 255) / 256, kRelmseBlocks); }
+
+// Both relMSE entry points: the parts in `part` (relmse_grid(n3) doubles), the
+// mean in *d_out (device).
+int relmse_launch(pico_hip_ctx* c, const float* img, const double* ref, double count, size_t n3, double* part,
+                  double* d_out) {
+    const uint32_t g = relmse_grid(n3);
+    PICO_LAUNCH(c, "relmse", k_relmse_part, g, 256, 0, img, ref, 1.0 / count, n3, part);
+    PICO_LAUNCH(c, "relmse_final", k_relmse_final, 1, 256, 0, part, g, 1.0 / (double)n3, d_out);
+    return PICO_OK;
+}
 
 // 4 independent 16-byte loads in flight per lane, non-temporal stores.
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -230,7 +262,7 @@ int pico_hip_atrous_d(pico_hip_ctx* c, const float* rgb, const float* nrm, const
         PICO_HIP(c, hipMemcpyAsync(out, rgb, 12 * n, hipMemcpyDeviceToDevice, c->stream));
         return PICO_OK;
     }
-    float* plen = (float*)scratch(c, "atrous_plen", 4 * n + 64);
+    float* plen = (float*)pico::scratch(c, "atrous_plen", 4 * n + 64);
     if (!plen) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     PICO_LAUNCH(c, "atrous_plen", k_plen, std::min<size_t>((n + 255) / 256, 8192), 256, 0, pos, plen, (uint32_t)n);
     const int tiles_x = (int)((w + tw - 1) / tw), tiles_y = (int)((h + th - 1) / th);
@@ -259,7 +291,7 @@ int pico_hip_atrous_fixed_d(pico_hip_ctx* c, const float* rgb, const float* nrm,
         PICO_HIP(c, hipMemcpyAsync(out, rgb, 12 * n, hipMemcpyDeviceToDevice, c->stream));
         return PICO_OK;
     }
-    float4* guide = (float4*)scratch(c, "atrous_guide", 16 * n + 64);
+    float4* guide = (float4*)pico::scratch(c, "atrous_guide", 16 * n + 64);
     if (!guide) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)c->num_cus * 16);
     PICO_LAUNCH(c, "atrous_guide", k_guide, grid, 256, 0, nrm, pos, guide, (uint32_t)n);
@@ -280,8 +312,9 @@ int pico_hip_atrous(pico_hip_ctx* c, const float* rgb, const float* nrm, const f
                     uint32_t h, uint32_t tw, uint32_t th, uint32_t levels, float* out) {
     if (!c) return PICO_ERR_ARG;
     if (!rgb || !nrm || !pos || !alb || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if ((uint64_t)w * h > (1ull << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
     const size_t n = (size_t)w * h;
-    float* b = (float*)scratch(c, "atrous_io", 72 * n + 64);
+    float* b = (float*)pico::scratch(c, "atrous_io", 72 * n + 64);
     if (!b) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     float* d_rgb = b;
     float* d_n = b + 3 * n;
@@ -303,8 +336,10 @@ int pico_hip_atrous(pico_hip_ctx* c, const float* rgb, const float* nrm, const f
 int pico_hip_tonemap(pico_hip_ctx* c, float* rgb, uint32_t w, uint32_t h) {
     if (!c) return PICO_ERR_ARG;
     if (!rgb) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if ((uint64_t)w * h > (1ull << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
     const size_t n = (size_t)w * h;
-    float* d = (float*)scratch(c, "tonemap", 12 * n + 64);
+    if (n == 0) return PICO_OK;
+    float* d = (float*)pico::scratch(c, "tonemap", 12 * n + 64);
     if (!d) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     uint32_t* wb = (uint32_t*)(d + 3 * n);
     PICO_HIP(c, hipMemcpyAsync(d, rgb, 12 * n, hipMemcpyHostToDevice, c->stream));
@@ -320,15 +355,20 @@ int pico_hip_relmse_d(pico_hip_ctx* c, const float* img, const double* ref, doub
     if (!c) return PICO_ERR_ARG;
     if (!img || !ref || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
     if (!(count > 0)) return fail(c, PICO_ERR_ARG, "reference count must be positive");
-    double* acc = (double*)scratch(c, "relmse", 64);
-    if (!acc) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
-    const uint32_t n3 = 3 * n_pixels;
-    PICO_LAUNCH(c, "zero", k_zero_f64, 1, 1, 0, acc);
-    PICO_LAUNCH(c, "relmse", k_relmse, std::min<uint32_t>((n3 + 255) / 256, 4096), 256, 0, img, ref, 1.0 / count, n3, acc);
+    if (n_pixels > (1u << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
+    if (n_pixels == 0) {
+        *out = 0.0;
+        return PICO_OK;
+    }
+    const size_t n3 = 3 * (size_t)n_pixels;
+    double* part = (double*)pico::scratch(c, "relmse", 8 * (kRelmseBlocks + 1));
+    if (!part) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    int rc = relmse_launch(c, img, ref, count, n3, part + 1, part);
+    if (rc) return rc;
     double h = 0.0;
-    PICO_HIP(c, hipMemcpyAsync(&h, acc, 8, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(&h, part, 8, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
-    *out = n_pixels ? h / (double)n3 : 0.0;
+    *out = h;
     return PICO_OK;
 }
 

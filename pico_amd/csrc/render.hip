@@ -232,6 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ v
 // the kernels ask for 6 waves per SIMD.
 constexpr int kTraceLds = 16;
 constexpr uint32_t kTraceSpill = 48;
+static_assert(kTraceLds + kTraceSpill >= 64, "trace stacks hold the 64 entries an upload admits");
 
 // A point on a mesh light: AliasTable::sample (Source/Util/AliasTable.cpp:89-104)
 // then LowerLevelMeshBVH::sample_geometry (Source/Core/LowerLevelMeshBVH.cpp:119-132).

@@ -106,3 +106,17 @@ def test_learned_filter_on_ragged_sizes(ctx, w, h):
     c = oracle.cnn_forward(p.astype(np.float64), x, h, w)
     err = np.abs(g - c) / (np.abs(c) + 1e-2)
     assert np.median(err) < 5e-3 and err.max() < 0.1, (np.median(err), err.max())
+
+
+def test_zero_size_relmse_and_tonemap(ctx):
+    d = ctx.alloc(64)
+    assert ctx.relmse_d(d, d, 4.0, 0) == 0.0
+    out = ctx.tonemap(np.zeros((0, 3), np.float32), 0, 5)
+    assert out.size == 0
+
+
+def test_relmse_size_limit_is_refused(ctx):
+    d = ctx.alloc(64)
+    with pytest.raises(pa.PicoError) as e:
+        ctx.relmse_d(d, d, 4.0, (1 << 28) + 1)
+    assert e.value.code == pa.PICO_ERR_LIMIT
