@@ -368,6 +368,19 @@ int  pico_hip_tonemap(pico_hip_ctx* ctx, float* rgb, uint32_t width, uint32_t he
 int  pico_hip_relmse_d(pico_hip_ctx* ctx, const float* d_img, const double* d_ref_sum, double ref_count,
                        uint32_t n_pixels, double* out);
 
+/* Allocation-free variant of pico_hip_relmse_d.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  It
+ * calls no allocator and does not synchronise: the mean is written to
+ * d_out (one double in device memory) in stream order, and the per-block
+ * parts of the reduction live in a fixed 8 KiB static device array, one copy
+ * per device, held until the process ends.  Calls that overlap on one device
+ * race on that array.  Same kernels and grid as pico_hip_relmse_d, so the
+ * same bits; at most 2^28 pixels (PICO_ERR_LIMIT beyond); n_pixels = 0
+ * writes 0. */
+int  pico_hip_relmse_noalloc_d(pico_hip_ctx* ctx, const float* d_img, const double* d_ref_sum, double ref_count,
+                               uint32_t n_pixels, double* d_out);
+
 /* ---- learned denoiser (kernel-predicting U-Net) ------------------------ */
 /* Input: 10 channels per pixel (radiance 3, albedo 3, normal 3, depth 1),
  * planar layout [10][H][W] f32.  Network: 3-level U-Net 32/64/128 with 3x3

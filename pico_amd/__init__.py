@@ -158,6 +158,7 @@ def _declare(lib):
         "pico_hip_tonemap": (i32, [vp, vp, u32, u32]),
         "pico_hip_atrous_fixed_d": (i32, [vp, vp, vp, vp, vp, u32, u32, u32, f32, f32, f32, f32, vp, vp]),
         "pico_hip_relmse_d": (i32, [vp, vp, vp, f64, u32, P(f64)]),
+        "pico_hip_relmse_noalloc_d": (i32, [vp, vp, vp, f64, u32, vp]),
         "pico_hip_render_views_d": (i32, [vp, P(View), u32, u32, P(Stats)]),
         "pico_hip_set_tuning": (i32, [vp, P(Tuning)]),
         "pico_hip_get_tuning": (i32, [vp, P(Tuning)]),
@@ -554,6 +555,12 @@ class Context:
         out = C.c_double()
         self._check(lib().pico_hip_relmse_d(self._h, d_img.ptr, d_ref_sum.ptr, float(ref_count), n_pixels, C.byref(out)))
         return out.value
+
+    def relmse_noalloc_d(self, d_img, d_ref_sum, ref_count, n_pixels, d_out):
+        """The allocation-free relMSE: the mean goes to d_out (one device
+        double) without a synchronisation."""
+        self._check(lib().pico_hip_relmse_noalloc_d(self._h, d_img.ptr, d_ref_sum.ptr, float(ref_count), n_pixels,
+                                                    d_out.ptr))
 
     def render_views_d(self, views, queue_paths=0):
         arr = (View * len(views))(*views)

@@ -219,6 +219,11 @@ int pico_hip_create(int device, pico_hip_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (resolve_static_scratch(c) != PICO_OK) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return PICO_ERR_HIP;
+    }
     *out = c;
     return PICO_OK;
 }

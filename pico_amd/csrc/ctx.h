@@ -58,6 +58,9 @@ struct pico_hip_ctx {
     // timing
     bool timing = false;
     std::map<std::string, KernelTimes> timers;
+    // this device's copy of the static scratch of the allocation-free entry
+    // point (resolved at create with the device current)
+    void* static_scratch = nullptr;
     // learned-denoiser instances created on this context (destroyed with it)
     std::set<pico_cnn*> cnns;
 };
@@ -88,6 +91,10 @@ struct StreamSwap {
 // Learned-denoiser queries used by the batched render (defined with the network).
 bool cnn_belongs_to(const pico_cnn* n, const pico_hip_ctx* c);
 int cnn_check_size(pico_cnn* n, uint32_t w, uint32_t h);
+
+// Resolves the context's device address of the allocation-free entry point's
+// static scratch (post.hip).
+int resolve_static_scratch(pico_hip_ctx* c);
 
 // Returns a device buffer of at least `bytes` under `name` (contents undefined).
 void* scratch(pico_hip_ctx* c, const std::string& name, size_t bytes);

@@ -18,6 +18,14 @@
 
 using namespace pico;
 
+// Scratch of pico_hip_relmse_noalloc_d: the per-block parts of the relMSE
+// reduction (at most kRelmseBlocks doubles, 8 KiB).  One copy per device,
+// held until the process ends; every call on a device shares it, so calls
+// that overlap on one device race.  Host code must use the per-device
+// address from hipGetSymbolAddress (pico_hip_ctx::static_scratch), never the
+// name: in host code the name is only a stand-in for the symbol.
+static __device__ __align__(256) double scratch[1024];
+
 namespace {
 
 constexpr int kTileThreads = 256;
@@ -210,6 +218,8 @@ __global__ __launch_bounds__(256) void k_relmse_final(const double* __restrict__
     if (threadIdx.x == 0) *out = ((red[0] + red[1]) + (red[2] + red[3])) * inv_n3;
 }
 
+static_assert(sizeof(::scratch) / sizeof(double) >= kRelmseBlocks, "static scratch holds every relMSE part");
+
 inline uint32_t relmse_grid(size_t n3) { return (uint32_t)std::min<size_t>((n3 + 255) / 256, kRelmseBlocks); }
 
 // Both relMSE entry points: the parts in `part` (relmse_grid(n3) doubles), the
@@ -238,6 +248,21 @@ __global__ __launch_bounds__(256) void k_copy(v4f* __restrict__ dst, const v4f* 
 }
 
 }  // namespace
+
+namespace pico {
+
+// Called by pico_hip_create with the context's device current: resolves
+// this device's copy of the static scratch once.
+int resolve_static_scratch(pico_hip_ctx* c) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(c, PICO_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    c->static_scratch = ptr;
+    return PICO_OK;
+}
+
+}  // namespace pico
 
 extern "C" {
 
@@ -370,6 +395,21 @@ int pico_hip_relmse_d(pico_hip_ctx* c, const float* img, const double* ref, doub
     PICO_HIP(c, hipStreamSynchronize(c->stream));
     *out = h;
     return PICO_OK;
+}
+
+int pico_hip_relmse_noalloc_d(pico_hip_ctx* c, const float* img, const double* ref, double count, uint32_t n_pixels,
+                              double* d_out) {
+    if (!c) return PICO_ERR_ARG;
+    if (!img || !ref || !d_out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if (!(count > 0)) return fail(c, PICO_ERR_ARG, "reference count must be positive");
+    if (n_pixels > (1u << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
+    if (!c->static_scratch) return fail(c, PICO_ERR_STATE, "static scratch of this device not resolved");
+    if (n_pixels == 0) {
+        PICO_LAUNCH(c, "relmse_zero", k_zero_f64, 1, 1, 0, d_out);
+        return PICO_OK;
+    }
+    // the parts live in this device's copy of `scratch` (relmse_grid <= kRelmseBlocks)
+    return relmse_launch(c, img, ref, count, 3 * (size_t)n_pixels, (double*)c->static_scratch, d_out);
 }
 
 }  // extern "C"
