@@ -196,7 +196,8 @@ typedef struct pico_stats {
                                 of a batched launch: launch start -> end of the chunk
                                 holding the view's last sample */
     int32_t  cancelled;
-    int32_t  pad;
+    int32_t  denoise_fallback; /* pico_hip_render_views_denoise_d: 1 when the view's output is
+                                  the a-trous fallback (see pico_hip_denoise_gbuffer_d) */
     double   done_ms;        /* batched launches: launch start -> the view's output is
                                 final (after its denoiser when one is requested) */
 } pico_stats;
@@ -408,6 +409,23 @@ int  pico_hip_cnn_denoise_gbuffer(pico_cnn* cnn, const float* rgb, const float* 
                                   const float* albedo, uint32_t width, uint32_t height, float* out_rgb);
 int  pico_hip_cnn_denoise_gbuffer_d(pico_cnn* cnn, const float* d_rgb, const float* d_normal, const float* d_position,
                                     const float* d_albedo, uint32_t width, uint32_t height, float* d_out_rgb);
+/* The denoise step of Scene::render_scene_to_memory (Source/Core/Scene.cpp:210-222)
+ * with the learned filter in front and the reference's filter as its
+ * fallback: the network's output, unless the network is NULL or refuses the
+ * call (e.g. the image exceeds its size), or any value of its output is not
+ * finite; then the whole image is Util::atrous_denoise's (reference mode, 5
+ * levels, tiles of tile_w x tile_h, as pico_hip_atrous_d), the call site the
+ * learned filter replaces.  The check and the fallback run on the device in
+ * stream order (no synchronisation); d_fallback (one device u32) receives 1
+ * when the output is the a-trous filter's, else 0.  d_scratch: W*H*3 f32.
+ * The host variant stages the buffers and returns the flag in
+ * *used_fallback (may be NULL). */
+int  pico_hip_denoise_gbuffer_d(pico_hip_ctx* ctx, pico_cnn* cnn, const float* d_rgb, const float* d_normal,
+                                const float* d_position, const float* d_albedo, uint32_t width, uint32_t height,
+                                uint32_t tile_w, uint32_t tile_h, float* d_scratch, float* d_out, uint32_t* d_fallback);
+int  pico_hip_denoise_gbuffer(pico_hip_ctx* ctx, pico_cnn* cnn, const float* rgb, const float* normal,
+                              const float* position, const float* albedo, uint32_t width, uint32_t height,
+                              uint32_t tile_w, uint32_t tile_h, float* out_rgb, int32_t* used_fallback);
 /* Loss (mean relMSE of the filtered radiance vs target, FP64 reduction) and
  * FP32 gradients of every parameter for one batch (host buffers). */
 int  pico_hip_cnn_loss_grad(pico_cnn* cnn, const float* inputs, const float* targets, uint32_t batch,
@@ -461,9 +479,13 @@ int  pico_hip_render_views_d(pico_hip_ctx* ctx, const pico_view* views, uint32_t
                              pico_stats* stats);
 
 /* The same launch, each view followed by the learned denoiser as soon as its
- * last chunk of samples is done: G-buffer + network on a second stream,
- * overlapping the remaining passes.  d_denoised[i] receives view i's
- * filtered image (W*H*3 f32, device); views must fit the network's size. */
+ * last chunk of samples is done: G-buffer + network between chunks or on a
+ * second stream (PICO_TUNE_SIDE_STREAM), overlapping the remaining passes.
+ * d_denoised[i] receives view i's filtered image (W*H*3 f32, device); views
+ * must fit the network's size.  A view whose network output holds a value
+ * that is not finite gets the reference's a-trous filter instead (tiles of
+ * its params' tile size, 64 x 64 when 0), as pico_hip_denoise_gbuffer_d;
+ * stats[i].denoise_fallback says which. */
 int  pico_hip_render_views_denoise_d(pico_hip_ctx* ctx, const pico_view* views, uint32_t n, pico_cnn* cnn,
                                      float* const* d_denoised, pico_stats* stats);
 

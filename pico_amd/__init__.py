@@ -86,7 +86,8 @@ class Stats(C.Structure):
                 ("instance_visits", C.c_uint64), ("shadow_node_visits", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("shadow_instance_visits", C.c_uint64),
                 ("samples", C.c_uint64), ("accumulated_rays", C.c_uint64),
-                ("render_ms", C.c_double), ("cancelled", C.c_int32), ("pad", C.c_int32), ("done_ms", C.c_double)]
+                ("render_ms", C.c_double), ("cancelled", C.c_int32), ("denoise_fallback", C.c_int32),
+                ("done_ms", C.c_double)]
 
     @property
     def rays(self):
@@ -176,6 +177,8 @@ def _declare(lib):
         "pico_hip_cnn_denoise_gbuffer": (i32, [vp, vp, vp, vp, vp, u32, u32, vp]),
         "pico_hip_cnn_denoise_gbuffer_d": (i32, [vp, vp, vp, vp, vp, u32, u32, vp]),
         "pico_hip_cnn_set_clip": (i32, [vp, f32]),
+        "pico_hip_denoise_gbuffer_d": (i32, [vp, vp, vp, vp, vp, vp, u32, u32, u32, u32, vp, vp, vp]),
+        "pico_hip_denoise_gbuffer": (i32, [vp, vp, vp, vp, vp, vp, u32, u32, u32, u32, vp, P(i32)]),
         "pico_hip_cnn_last_grad_norm": (i32, [vp, P(f64)]),
         "pico_hip_cnn_loss_grad": (i32, [vp, vp, vp, u32, u32, u32, P(f64), vp]),
         "pico_hip_cnn_train_step_d": (i32, [vp, vp, vp, u32, u32, u32, f32, P(f64)]),
@@ -592,6 +595,22 @@ class Context:
         self._check(lib().pico_hip_autotune_views(self._h, arr, len(views), calib_spp, C.byref(t), C.byref(cm),
                                                   C.byref(dm)))
         return t.as_tuple(), cm.value, dm.value
+
+    def denoise_gbuffer_d(self, cnn, d_rgb, d_n, d_p, d_a, w, h, d_scratch, d_out, d_flag, tile=(16, 16)):
+        """Learned filter with the reference a-trous as fallback (cnn may be
+        None); d_flag (one device u32) receives 1 when the fallback ran."""
+        self._check(lib().pico_hip_denoise_gbuffer_d(self._h, None if cnn is None else cnn._h, d_rgb.ptr, d_n.ptr,
+                                                     d_p.ptr, d_a.ptr, w, h, tile[0], tile[1], d_scratch.ptr, d_out.ptr,
+                                                     d_flag.ptr))
+
+    def denoise_gbuffer(self, cnn, rgb, normal, position, albedo, w, h, tile=(16, 16)):
+        """Host variant: (output, used_fallback)."""
+        args = [_f32(x, (-1, 3)) for x in (rgb, normal, position, albedo)]
+        out = np.empty((w * h, 3), np.float32)
+        fb = C.c_int32()
+        self._check(lib().pico_hip_denoise_gbuffer(self._h, None if cnn is None else cnn._h, *[_ptr(a) for a in args], w,
+                                                   h, tile[0], tile[1], _ptr(out), C.byref(fb)))
+        return out, bool(fb.value)
 
     def enable_kernel_timing(self, on=True):
         self._check(lib().pico_hip_enable_kernel_timing(self._h, 1 if on else 0))

@@ -571,6 +571,18 @@ __global__ void k_zero_f32(float* p, size_t n) {
 }
 __global__ void k_zero1(double* p) { *p = 0.0; }
 
+// Fallback check of the denoiser's output: flag = 1 when any value is not
+// finite (NaN or inf), else 0.  The flag is written by the first block
+// (cleared) and raised by any lane that sees a bad value; the one-block clear
+// runs as its own launch first.
+__global__ void k_flag_set(uint32_t* flag, uint32_t v) { *flag = v; }
+__global__ void k_flag_nonfinite(const float* __restrict__ x, size_t n, uint32_t* flag) {
+    bool bad = false;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        bad |= !isfinite(x[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
 uint32_t grid_for(size_t n, int block = 256) { return (uint32_t)std::min<size_t>((n + block - 1) / block, 16384); }
 
 }  // namespace
@@ -918,6 +930,58 @@ int pico_hip_cnn_denoise_gbuffer(pico_cnn* n, const float* rgb, const float* nor
     if ((rc = pico_hip_cnn_denoise_gbuffer_d(n, d, d + 3 * px, d + 6 * px, d + 9 * px, w, h, d + 12 * px))) return rc;
     PICO_HIP(n->c, hipMemcpyAsync(out, d + 12 * px, px * 12, hipMemcpyDeviceToHost, n->c->stream));
     PICO_HIP(n->c, hipStreamSynchronize(n->c->stream));
+    return PICO_OK;
+}
+
+int pico_hip_denoise_gbuffer_d(pico_hip_ctx* c, pico_cnn* n, const float* d_rgb, const float* d_normal,
+                               const float* d_position, const float* d_albedo, uint32_t w, uint32_t h, uint32_t tile_w,
+                               uint32_t tile_h, float* d_scratch, float* d_out, uint32_t* d_fallback) {
+    if (!c) return PICO_ERR_ARG;
+    if (n && !cnn_belongs_to(n, c)) return fail(c, PICO_ERR_ARG, "network created on another context");
+    if (!d_rgb || !d_normal || !d_position || !d_albedo || !d_scratch || !d_out || !d_fallback)
+        return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if (w == 0 || h == 0 || tile_w == 0 || tile_h == 0) return fail(c, PICO_ERR_ARG, "sizes must be positive");
+    const size_t n3 = (size_t)w * h * 3;
+    // the network, unless there is none or it refuses the call (its size
+    // limit); then the a-trous filter unconditionally
+    const int rc = n ? pico_hip_cnn_denoise_gbuffer_d(n, d_rgb, d_normal, d_position, d_albedo, w, h, d_out) : PICO_ERR_STATE;
+    if (rc) {
+        const std::string why = n ? c->err : std::string("no network");
+        PICO_LAUNCH(c, "denoise_flag", k_flag_set, 1, 1, 0, d_fallback, 1u);
+        const int rc2 = atrous_gated(c, d_rgb, d_normal, d_position, d_albedo, w, h, tile_w, tile_h, 5, d_scratch, d_out,
+                                     nullptr);
+        if (rc2) return rc2;
+        c->err = "learned denoiser not used (" + why + "): a-trous output";
+        return PICO_OK;
+    }
+    // a non-finite value anywhere in the network's output: the a-trous filter
+    // replaces the whole image (kernels gated on the device flag)
+    PICO_LAUNCH(c, "denoise_flag", k_flag_set, 1, 1, 0, d_fallback, 0u);
+    PICO_LAUNCH(c, "denoise_check", k_flag_nonfinite, grid_for(n3), 256, 0, d_out, n3, d_fallback);
+    return atrous_gated(c, d_rgb, d_normal, d_position, d_albedo, w, h, tile_w, tile_h, 5, d_scratch, d_out, d_fallback);
+}
+
+int pico_hip_denoise_gbuffer(pico_hip_ctx* c, pico_cnn* n, const float* rgb, const float* normal, const float* position,
+                             const float* albedo, uint32_t w, uint32_t h, uint32_t tile_w, uint32_t tile_h, float* out,
+                             int32_t* used_fallback) {
+    if (!c) return PICO_ERR_ARG;
+    if (!rgb || !normal || !position || !albedo || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if ((uint64_t)w * h > (1ull << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
+    const size_t px = (size_t)w * h;
+    float* d = (float*)scratch(c, "denoise_io", px * 72 + 64);
+    if (!d) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    uint32_t* flag = (uint32_t*)(d + 18 * px);
+    const float* src[4] = {rgb, normal, position, albedo};
+    for (int k = 0; k < 4; ++k)
+        PICO_HIP(c, hipMemcpyAsync(d + 3 * k * px, src[k], px * 12, hipMemcpyHostToDevice, c->stream));
+    int rc = pico_hip_denoise_gbuffer_d(c, n, d, d + 3 * px, d + 6 * px, d + 9 * px, w, h, tile_w, tile_h, d + 12 * px,
+                                        d + 15 * px, flag);
+    if (rc) return rc;
+    uint32_t hf = 0;
+    PICO_HIP(c, hipMemcpyAsync(out, d + 15 * px, px * 12, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    if (used_fallback) *used_fallback = hf ? 1 : 0;
     return PICO_OK;
 }
 

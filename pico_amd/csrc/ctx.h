@@ -96,6 +96,12 @@ int cnn_check_size(pico_cnn* n, uint32_t w, uint32_t h);
 // static scratch (post.hip).
 int resolve_static_scratch(pico_hip_ctx* c);
 
+// Reference-mode a-trous (post.hip); every kernel is skipped unless
+// gate == NULL or *gate != 0 on the device.
+int atrous_gated(pico_hip_ctx* c, const float* rgb, const float* nrm, const float* pos, const float* alb, uint32_t w,
+                 uint32_t h, uint32_t tw, uint32_t th, uint32_t levels, float* scratch_buf, float* out,
+                 const uint32_t* gate);
+
 // Returns a device buffer of at least `bytes` under `name` (contents undefined).
 void* scratch(pico_hip_ctx* c, const std::string& name, size_t bytes);
 

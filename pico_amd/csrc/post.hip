@@ -2,13 +2,20 @@
 // FP64 relMSE reduction.
 //
 // A-trous (Source/Util/Denoisers.cpp:18-126): one workgroup per tile and level.
-// The workgroup first reduces the tile-local constants over exactly the set of
-// pixels the tile's 25-tap kernels read (white point = max luminance of the
-// level input, range = max - min of |position|), then filters its pixels.
-// The tapped set is reduced tap by tap, not as a dense dilated rectangle: for
-// tiles narrower than the dilation the two differ.  Per-pixel |position| is
-// computed once per call.  HBM-bound: 4 x 12 B read + 12 B written per pixel
-// and level, the tile reduce re-reads through L2.
+// The workgroup first reduces the tile-local constants (white point = max
+// luminance of the level input, range = max - min of |position|) over the
+// pixels the tile's 25-tap kernels read, then filters its pixels.  The tapped
+// set is a product of a column set and a row set: along x it is
+// {clamp(x0 + k + o s)} for k < tile width, o in -2..2, i.e. one run of
+// width + 4s columns when the dilation s is at most the tile width, else five
+// runs of the tile width; the same along y.  The reduce walks those runs once
+// (max and min do not mind the repeats clamping makes) instead of 25 taps per
+// pixel.  Luminance and |position| come from per-pixel arrays: |position| is
+// computed once per call, the luminance of each level's output is written by
+// the level that produces it.  The filter takes one exponential per tap:
+// exp(-a) exp(-b) exp(-c) = exp2(-(a + b + c) log2 e), with the same NaN and
+// zero cases as the reference's three (a width of 0 gives 0 / 0 = NaN on the
+// tap at distance 0 and 0 elsewhere).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,80 +42,205 @@ __constant__ float kAtrousW[25] = {0.00292f, 0.01306f, 0.02154f, 0.01306f, 0.002
                                    0.02154f, 0.01306f, 0.05855f, 0.09563f, 0.05855f, 0.01306f, 0.00292f,
                                    0.01306f, 0.02154f, 0.01306f, 0.00292f};
 
-__global__ void k_plen(const float* __restrict__ pos, float* __restrict__ plen, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+__device__ __forceinline__ float lum_of(f3 c) { return c.x * 0.2126f + c.y * 0.587f + c.z * 0.114f; }
+
+// |position| and the luminance of the first level's input, per pixel.
+__global__ void k_atrous_prep(const float* __restrict__ rgb, const float* __restrict__ pos, float* __restrict__ plen,
+                              float* __restrict__ lum, uint32_t n, const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0u) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         plen[i] = length(ld3(pos + 3 * (size_t)i));
-}
-
-__device__ __forceinline__ uint32_t tap_index(int x, int y, int u, int scale, int w, int h) {
-    const int ox = u % 5 - 2, oy = u / 5 - 2;
-    const int ix = min(max(x + ox * scale, 0), w - 1);
-    const int iy = min(max(y + oy * scale, 0), h - 1);
-    return (uint32_t)iy * (uint32_t)w + (uint32_t)ix;
-}
-
-__global__ __launch_bounds__(kTileThreads) void k_atrous_level(const float* __restrict__ src, const float* __restrict__ nrm,
-                                                               const float* __restrict__ pos, const float* __restrict__ alb,
-                                                               const float* __restrict__ plen, float* __restrict__ dst,
-                                                               int w, int h, int tw, int th, int tiles_y, int scale) {
-    __shared__ float red[3][kTileThreads / 64];
-    const int tx0 = (blockIdx.x / tiles_y) * tw, ty0 = (blockIdx.x % tiles_y) * th;
-    const int cw = min(tw, w - tx0), ch = min(th, h - ty0);
-    const int npx = cw * ch;
-    float white = 0.0f, dmin = INFINITY, dmax = -INFINITY;
-    for (int q = threadIdx.x; q < npx * 25; q += kTileThreads) {
-        const int pidx = q / 25, u = q - pidx * 25;
-        const int x = tx0 + pidx % cw, y = ty0 + pidx / cw;
-        const uint32_t fi = tap_index(x, y, u, scale, w, h);
-        const f3 c = ld3(src + 3 * (size_t)fi);
-        const float lum = c.x * 0.2126f + c.y * 0.587f + c.z * 0.114f;
-        white = fmaxf(white, lum);
-        const float l = plen[fi];
-        dmin = fminf(dmin, l);
-        dmax = fmaxf(dmax, l);
+        lum[i] = lum_of(ld3(rgb + 3 * (size_t)i));
     }
-    // wave reduce, then across the workgroup's waves
+}
+
+// Runs of the tapped set along one axis for a tile [t0, t0 + c) at dilation s.
+struct TapRuns {
+    int n, len, start, step;   // n runs of len, run r starts at start + r * step
+};
+__device__ __forceinline__ TapRuns tap_runs(int t0, int c, int s) {
+    if (s <= c) return {1, c + 4 * s, t0 - 2 * s, 0};
+    return {5, c, t0 - 2 * s, s};
+}
+
+// Block index spread so that each of the 8 XCDs (each with its own L2) gets a
+// contiguous run of blocks: hardware hands consecutive workgroups to the XCDs
+// in turn, and neighbouring blocks share the halo of their taps.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t g) {
+    const uint32_t xcd = b & 7u, k = b >> 3, per = g >> 3, rem = g & 7u;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+// Tile-local constants of one level, one workgroup per tile: the exp2 factors
+// log2(e) / range^2 and log2(e) / white^2 of the position and albedo terms.
+__global__ __launch_bounds__(kTileThreads) void k_atrous_consts(const float* __restrict__ lum,
+                                                                const float* __restrict__ plen, float2* __restrict__ consts,
+                                                                int w, int h, int tw, int th, int tiles_y, int scale,
+                                                                const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0u) return;
+    __shared__ float red[3][kTileThreads / 64];
+    const int t = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int tx0 = (t / tiles_y) * tw, ty0 = (t % tiles_y) * th;
+    const int cw = min(tw, w - tx0), ch = min(th, h - ty0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float white = 0.0f, dmin = INFINITY, dmax = -INFINITY;
+    const TapRuns rx = tap_runs(tx0, cw, scale), ry = tap_runs(ty0, ch, scale);
+    for (int r = 0; r < ry.n; ++r)
+        for (int j = wave; j < ry.len; j += kTileThreads / 64) {
+            const uint32_t row = (uint32_t)min(max(ry.start + r * ry.step + j, 0), h - 1) * (uint32_t)w;
+            for (int q = 0; q < rx.n; ++q)
+                for (int k = lane; k < rx.len; k += 64) {
+                    const uint32_t fi = row + (uint32_t)min(max(rx.start + q * rx.step + k, 0), w - 1);
+                    white = fmaxf(white, lum[fi]);
+                    const float l = plen[fi];
+                    dmin = fminf(dmin, l);
+                    dmax = fmaxf(dmax, l);
+                }
+        }
     for (int o = 32; o > 0; o >>= 1) {
         white = fmaxf(white, __shfl_xor(white, o));
         dmin = fminf(dmin, __shfl_xor(dmin, o));
         dmax = fmaxf(dmax, __shfl_xor(dmax, o));
     }
-    const int wave = threadIdx.x / 64;
-    if ((threadIdx.x & 63) == 0) { red[0][wave] = white; red[1][wave] = dmin; red[2][wave] = dmax; }
+    if (lane == 0) { red[0][wave] = white; red[1][wave] = dmin; red[2][wave] = dmax; }
     __syncthreads();
-    white = red[0][0]; dmin = red[1][0]; dmax = red[2][0];
-    for (int k = 1; k < kTileThreads / 64; ++k) {
-        white = fmaxf(white, red[0][k]);
-        dmin = fminf(dmin, red[1][k]);
-        dmax = fmaxf(dmax, red[2][k]);
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kTileThreads / 64; ++k) {
+            white = fmaxf(white, red[0][k]);
+            dmin = fminf(dmin, red[1][k]);
+            dmax = fmaxf(dmax, red[2][k]);
+        }
+        // exp(-d / r^2) exp(-d' / w^2) as exp2 factors; the reciprocals keep
+        // 0 / 0 = NaN and d / 0 = inf for d > 0
+        const float range = dmax - dmin;
+        const float kLog2e = 1.44269504088896341f;
+        consts[t] = make_float2(kLog2e / (range * range), kLog2e / (white * white));
     }
-    const float range = dmax - dmin;
-    const float r2 = range * range, w2 = white * white;
-    for (int pidx = threadIdx.x; pidx < npx; pidx += kTileThreads) {
-        const int x = tx0 + pidx % cw, y = ty0 + pidx / cw;
-        const size_t ci = (size_t)y * w + x;
-        const f3 cn = ld3(nrm + 3 * ci), cp = ld3(pos + 3 * ci), ca = ld3(alb + 3 * ci);
-        f3 res = mk(0.0f, 0.0f, 0.0f);
-        float wsum = 0.0f;
+}
+
+// One level of the filter.  A workgroup filters a block of BW x BH pixels
+// (256 threads, one pixel each); wider blocks for wider dilations keep the
+// union of the 25 taps small (it is (BW + 4s) x (BH + 4s) pixels, or 5 BW
+// columns once s > BW).  Each pixel looks up its tile's constants.
+template <int BW, int BH>
+__global__ __launch_bounds__(256) void k_atrous_filter(
+    const float* __restrict__ src, const float* __restrict__ nrm, const float* __restrict__ pos,
+    const float* __restrict__ alb, const float2* __restrict__ consts, float* __restrict__ dst,
+    float* __restrict__ lum_out, int w, int h, int tw, int th, int tiles_y, int blocks_x, int scale,
+    const uint32_t* __restrict__ gate) {
+    static_assert(BW * BH == 256, "one pixel per thread");
+    if (gate && *gate == 0u) return;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int x = (int)(b % (uint32_t)blocks_x) * BW + (int)threadIdx.x % BW;
+    const int y = (int)(b / (uint32_t)blocks_x) * BH + (int)threadIdx.x / BW;
+    if (x >= w || y >= h) return;
+    const float2 kc = consts[(x / tw) * tiles_y + y / th];
+    const float kn = 4.0f * 1.44269504088896341f;
+    const uint32_t ci = (uint32_t)y * (uint32_t)w + (uint32_t)x;
+    const f3 cn = ld3(nrm + 3 * ci), cp = ld3(pos + 3 * ci), ca = ld3(alb + 3 * ci);
+    f3 res = mk(0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
 #pragma unroll
-        for (int u = 0; u < 25; ++u) {
-            const size_t fi = tap_index(x, y, u, scale, w, h);
-            f3 t = cn - ld3(nrm + 3 * fi);
-            float dist = dot(t, t);
-            const float nw = smin(expf(-dist / 0.25f), 1.0f);
-            t = cp - ld3(pos + 3 * fi);
-            dist = dot(t, t);
-            const float pw = smin(expf(-dist / r2), 1.0f);
-            t = ca - ld3(alb + 3 * fi);
-            dist = dot(t, t);
-            const float aw = smin(expf(-dist / w2), 1.0f);
-            const float tw_ = kAtrousW[u] * nw * pw * aw;
-            res = res + ld3(src + 3 * fi) * tw_;
+    for (int v = 0; v < 5; ++v) {
+        const uint32_t row = (uint32_t)min(max(y + (v - 2) * scale, 0), h - 1) * (uint32_t)w;
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            const uint32_t f3i = 3u * (row + (uint32_t)min(max(x + (u - 2) * scale, 0), w - 1));
+            f3 t = cn - ld3(nrm + f3i);
+            const float dn = fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x));
+            t = cp - ld3(pos + f3i);
+            const float dp = fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x));
+            t = ca - ld3(alb + f3i);
+            const float da = fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x));
+            const float e = fmaf(dn, kn, fmaf(dp, kc.x, da * kc.y));
+            const float tw_ = kAtrousW[5 * v + u] * __builtin_amdgcn_exp2f(-e);
+            const f3 c = ld3(src + f3i);
+            res = mk(fmaf(c.x, tw_, res.x), fmaf(c.y, tw_, res.y), fmaf(c.z, tw_, res.z));
             wsum += tw_;
         }
-        const f3 o = res / wsum;
-        dst[3 * ci] = o.x; dst[3 * ci + 1] = o.y; dst[3 * ci + 2] = o.z;
     }
+    const f3 o = res / wsum;
+    dst[3 * ci] = o.x; dst[3 * ci + 1] = o.y; dst[3 * ci + 2] = o.z;
+    if (lum_out) lum_out[ci] = lum_of(o);
+}
+
+// One level at dilation S <= 16 with the taps staged in LDS.  A workgroup
+// filters 64 columns x 4 rows of one row phase: rows ph + (4g + j) S, j < 4,
+// so the 5 tap rows of its 4 rows are 8 rows of the same phase, and its tap
+// columns are the 64 + 4S columns around its own.  Those 8 x (64 + 4S)
+// pixels (normal, position, albedo, colour: 48 B each) are staged once; each
+// tap is then four LDS reads at offsets fixed at compile time.
+constexpr int kLdsCols = 128;  // 64 + 4 * 16
+
+template <int S>
+__global__ __launch_bounds__(256) void k_atrous_lds(
+    const float* __restrict__ src, const float* __restrict__ nrm, const float* __restrict__ pos,
+    const float* __restrict__ alb, const float2* __restrict__ consts, float* __restrict__ dst,
+    float* __restrict__ lum_out, int w, int h, int tw, int th, int tiles_y, int blocks_x, int groups,
+    const uint32_t* __restrict__ gate) {
+    static_assert(64 + 4 * S <= kLdsCols, "staged columns");
+    constexpr int FW = 64 + 4 * S;
+    constexpr int kArr = 8 * kLdsCols * 3;  // floats per staged field
+    if (gate && *gate == 0u) return;
+    __shared__ float st[4 * kArr];
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int bx = (int)(b % (uint32_t)blocks_x);
+    const int rest = (int)(b / (uint32_t)blocks_x);
+    const int ph = rest % S, g = rest / S;
+    const int x0 = bx * 64;
+    // stage rows ph + (4g + r - 2) S, r < 8, columns x0 - 2S + k, k < FW (clamped)
+    for (int i = threadIdx.x; i < 8 * FW; i += 256) {
+        const int r = i / FW, k = i - r * FW;
+        const int yy = min(max(ph + (4 * g + r - 2) * S, 0), h - 1);
+        const int xx = min(max(x0 - 2 * S + k, 0), w - 1);
+        const uint32_t gi = 3u * ((uint32_t)yy * (uint32_t)w + (uint32_t)xx);
+        const int li = 3 * (r * kLdsCols + k);
+        const f3 a = ld3(nrm + gi), p = ld3(pos + gi), d = ld3(alb + gi), c = ld3(src + gi);
+        st[li] = a.x; st[li + 1] = a.y; st[li + 2] = a.z;
+        st[kArr + li] = p.x; st[kArr + li + 1] = p.y; st[kArr + li + 2] = p.z;
+        st[2 * kArr + li] = d.x; st[2 * kArr + li + 1] = d.y; st[2 * kArr + li + 2] = d.z;
+        st[3 * kArr + li] = c.x; st[3 * kArr + li + 1] = c.y; st[3 * kArr + li + 2] = c.z;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 63, j = threadIdx.x >> 6;
+    const int x = x0 + lx, y = ph + (4 * g + j) * S;
+    if (x >= w || y >= h || g >= groups) return;
+    const float2 kc = consts[(x / tw) * tiles_y + y / th];
+    const float kn = 4.0f * 1.44269504088896341f;
+    // this pixel's entry: row j + 2, column lx + 2S
+    const float* base = st + 3 * (j * kLdsCols + lx);
+    const int cidx = 3 * (2 * kLdsCols + 2 * S);
+    const f3 cn = ld3(base + cidx), cp = ld3(base + kArr + cidx), ca = ld3(base + 2 * kArr + cidx);
+    f3 res = mk(0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            const int o = 3 * (v * kLdsCols + u * S);
+            f3 t = cn - ld3(base + o);
+            const float dn = fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x));
+            t = cp - ld3(base + kArr + o);
+            const float dp = fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x));
+            t = ca - ld3(base + 2 * kArr + o);
+            const float da = fmaf(t.z, t.z, fmaf(t.y, t.y, t.x * t.x));
+            const float e = fmaf(dn, kn, fmaf(dp, kc.x, da * kc.y));
+            const float tw_ = kAtrousW[5 * v + u] * __builtin_amdgcn_exp2f(-e);
+            const f3 c = ld3(base + 3 * kArr + o);
+            res = mk(fmaf(c.x, tw_, res.x), fmaf(c.y, tw_, res.y), fmaf(c.z, tw_, res.z));
+            wsum += tw_;
+        }
+    }
+    const f3 out = res / wsum;
+    const uint32_t ci = (uint32_t)y * (uint32_t)w + (uint32_t)x;
+    dst[3 * ci] = out.x; dst[3 * ci + 1] = out.y; dst[3 * ci + 2] = out.z;
+    if (lum_out) lum_out[ci] = lum_of(out);
+}
+
+__global__ void k_copy_gated(float* __restrict__ dst, const float* __restrict__ src, uint32_t n,
+                             const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0u) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
 // Labelled mode with no counterpart in the reference: the edge-avoiding
@@ -262,6 +394,70 @@ int resolve_static_scratch(pico_hip_ctx* c) {
     return PICO_OK;
 }
 
+// Reference-mode a-trous; with a gate (device u32) every kernel does nothing
+// unless *gate != 0, so the filter can follow a check on the device without
+// a host synchronisation.
+int atrous_gated(pico_hip_ctx* c, const float* rgb, const float* nrm, const float* pos, const float* alb, uint32_t w,
+                 uint32_t h, uint32_t tw, uint32_t th, uint32_t levels, float* scratch_buf, float* out,
+                 const uint32_t* gate) {
+    if (!rgb || !nrm || !pos || !alb || !scratch_buf || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
+    if (w == 0 || h == 0 || tw == 0 || th == 0) return fail(c, PICO_ERR_ARG, "sizes must be positive");
+    if ((uint64_t)w * h > (1ull << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
+    if (levels > 16) return fail(c, PICO_ERR_LIMIT, "at most 16 levels");
+    const size_t n = (size_t)w * h;
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)c->num_cus * 16);
+    if (levels == 0) {
+        PICO_LAUNCH(c, "atrous_copy", k_copy_gated, std::min<size_t>((3 * n + 255) / 256, 8192), 256, 0, out, rgb,
+                    (uint32_t)(3 * n), gate);
+        return PICO_OK;
+    }
+    const int tiles_x = (int)((w + tw - 1) / tw), tiles_y = (int)((h + th - 1) / th);
+    const size_t tiles = (size_t)tiles_x * tiles_y;
+    // |position|, two luminance buffers, then the tile constants (16-B aligned);
+    // at most 20 B per pixel + 80 B, whatever the tile size
+    const size_t consts_off = (12 * n + 15) & ~(size_t)15;
+    char* sb = (char*)pico::scratch(c, "atrous_plen", consts_off + 8 * tiles + 64);
+    if (!sb) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    float* plen = (float*)sb;
+    float* lum[2] = {plen + n, plen + 2 * n};
+    float2* consts = (float2*)(sb + consts_off);
+    PICO_LAUNCH(c, "atrous_prep", k_atrous_prep, grid, 256, 0, rgb, pos, plen, lum[0], (uint32_t)n, gate);
+    static const char* const kLevelName[5] = {"atrous_l0", "atrous_l1", "atrous_l2", "atrous_l3", "atrous_l4"};
+    const float* src = rgb;
+    for (uint32_t l = 0; l < levels; ++l) {
+        float* dst = ((levels - 1 - l) % 2 == 0) ? out : scratch_buf;
+        float* lum_next = l + 1 < levels ? lum[(l + 1) & 1] : nullptr;
+        const int s = 1 << l;
+        PICO_LAUNCH(c, "atrous_consts", k_atrous_consts, (uint32_t)tiles, kTileThreads, 0, lum[l & 1], plen, consts,
+                    (int)w, (int)h, (int)tw, (int)th, tiles_y, s, gate);
+        const char* name = l < 5 ? kLevelName[l] : "atrous_l5+";
+        if (s <= 16) {
+            // taps staged in LDS: blocks of 64 columns x 4 rows of one row phase
+            const int bx = (int)((w + 63) / 64), groups = (int)(((h + s - 1) / s + 3) / 4);
+            const uint32_t nb = (uint32_t)(bx * s * groups);
+#define PICO_ATROUS_LDS(S_)                                                                                        \
+    PICO_LAUNCH(c, name, k_atrous_lds<S_>, nb, 256, 0, src, nrm, pos, alb, consts, dst, lum_next, (int)w, (int)h, \
+                (int)tw, (int)th, tiles_y, bx, groups, gate)
+            switch (s) {
+                case 1: PICO_ATROUS_LDS(1); break;
+                case 2: PICO_ATROUS_LDS(2); break;
+                case 4: PICO_ATROUS_LDS(4); break;
+                case 8: PICO_ATROUS_LDS(8); break;
+                default: PICO_ATROUS_LDS(16); break;
+            }
+#undef PICO_ATROUS_LDS
+        } else {
+            // dilations of 32 and more (levels beyond the reference's 5): the
+            // taps straight from memory, one row of 256 pixels per workgroup
+            const int bx = (int)((w + 255) / 256), by = (int)h;
+            PICO_LAUNCH(c, name, (k_atrous_filter<256, 1>), bx * by, 256, 0, src, nrm, pos, alb, consts, dst, lum_next,
+                        (int)w, (int)h, (int)tw, (int)th, tiles_y, bx, s, gate);
+        }
+        src = dst;
+    }
+    return PICO_OK;
+}
+
 }  // namespace pico
 
 extern "C" {
@@ -278,27 +474,7 @@ int pico_hip_copy_d(pico_hip_ctx* c, void* dst, const void* src, size_t bytes) {
 int pico_hip_atrous_d(pico_hip_ctx* c, const float* rgb, const float* nrm, const float* pos, const float* alb, uint32_t w,
                       uint32_t h, uint32_t tw, uint32_t th, uint32_t levels, float* scratch_buf, float* out) {
     if (!c) return PICO_ERR_ARG;
-    if (!rgb || !nrm || !pos || !alb || !scratch_buf || !out) return fail(c, PICO_ERR_ARG, "NULL buffer");
-    if (w == 0 || h == 0 || tw == 0 || th == 0) return fail(c, PICO_ERR_ARG, "sizes must be positive");
-    if ((uint64_t)w * h > (1ull << 28)) return fail(c, PICO_ERR_LIMIT, "image larger than 2^28 pixels");
-    if (levels > 16) return fail(c, PICO_ERR_LIMIT, "at most 16 levels");
-    const size_t n = (size_t)w * h;
-    if (levels == 0) {
-        PICO_HIP(c, hipMemcpyAsync(out, rgb, 12 * n, hipMemcpyDeviceToDevice, c->stream));
-        return PICO_OK;
-    }
-    float* plen = (float*)pico::scratch(c, "atrous_plen", 4 * n + 64);
-    if (!plen) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
-    PICO_LAUNCH(c, "atrous_plen", k_plen, std::min<size_t>((n + 255) / 256, 8192), 256, 0, pos, plen, (uint32_t)n);
-    const int tiles_x = (int)((w + tw - 1) / tw), tiles_y = (int)((h + th - 1) / th);
-    const float* src = rgb;
-    for (uint32_t l = 0; l < levels; ++l) {
-        float* dst = ((levels - 1 - l) % 2 == 0) ? out : scratch_buf;
-        PICO_LAUNCH(c, "atrous", k_atrous_level, tiles_x * tiles_y, kTileThreads, 0, src, nrm, pos, alb, plen, dst, (int)w,
-                    (int)h, (int)tw, (int)th, tiles_y, 1 << l);
-        src = dst;
-    }
-    return PICO_OK;
+    return pico::atrous_gated(c, rgb, nrm, pos, alb, w, h, tw, th, levels, scratch_buf, out, nullptr);
 }
 
 int pico_hip_atrous_fixed_d(pico_hip_ctx* c, const float* rgb, const float* nrm, const float* pos, const float* alb,

@@ -743,14 +743,18 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     if (!pb || !act || !ctl || !count0 || !done || !vstats || !dv) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     // the follow-up's buffers, sized before the first launch so that no
     // allocation happens while the two streams run
-    float *gn = nullptr, *gp = nullptr, *ga = nullptr;
+    float *gn = nullptr, *gp = nullptr, *ga = nullptr, *gs = nullptr;
+    uint32_t* fallback = nullptr;
     if (hooks && hooks->cnn) {
         size_t maxpx = 0;
         for (auto& v : hv) maxpx = std::max(maxpx, (size_t)v.cam.w * v.cam.h);
-        float* g = (float*)scratch(c, "views_gbuffer", maxpx * 36 + 64);
-        if (!g || !scratch(c, "cnn_frame_in", maxpx * 40 + 64) || !scratch(c, "gb_ovf", 64))
+        float* g = (float*)scratch(c, "views_gbuffer", maxpx * 48 + 64);
+        fallback = (uint32_t*)scratch(c, "views_fallback", 4 * nv + 64);
+        // the network's input, and the a-trous fallback's per-pixel arrays
+        if (!g || !fallback || !scratch(c, "cnn_frame_in", maxpx * 40 + 64) || !scratch(c, "gb_ovf", 64) ||
+            !scratch(c, "atrous_plen", maxpx * 20 + 128))
             return fail(c, PICO_ERR_NOMEM, "device allocation failed");
-        gn = g; gp = g + 3 * maxpx; ga = g + 6 * maxpx;
+        gn = g; gp = g + 3 * maxpx; ga = g + 6 * maxpx; gs = g + 9 * maxpx;
         if (!c->side) PICO_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     }
     Pool P;
@@ -856,8 +860,12 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
                     if (side) PICO_HIP(c, hipStreamWaitEvent(c->side, ce, 0));
                     StreamSwap sw(c, side ? c->side : c->stream);
                     const uint32_t w = hv[v].cam.w, h = hv[v].cam.h;
+                    const pico_render_params& vp = hooks->views[v].params;
+                    const uint32_t ftw = vp.tile_w ? vp.tile_w : 64, fth = vp.tile_h ? vp.tile_h : 64;
                     int rc = pico_hip_gbuffer_d(c, &hooks->views[v].camera, w, h, gn, gp, ga);
-                    if (!rc) rc = pico_hip_cnn_denoise_gbuffer_d(hooks->cnn, hv[v].pixels, gn, gp, ga, w, h, hooks->d_out[v]);
+                    if (!rc)
+                        rc = pico_hip_denoise_gbuffer_d(c, hooks->cnn, hv[v].pixels, gn, gp, ga, w, h, ftw, fth, gs,
+                                                        hooks->d_out[v], fallback + v);
                     if (rc) return rc;
                     if (!new_event(&view_done[v])) return fail(c, PICO_ERR_HIP, "hipEventCreate failed");
                     PICO_HIP(c, hipEventRecord(view_done[v], c->stream));
@@ -879,6 +887,8 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     std::vector<unsigned long long> hs(4 * nv);
     PICO_HIP(c, hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipMemcpyAsync(hs.data(), vstats, nv * 32, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> hfb(nv, 0u);
+    if (fallback) PICO_HIP(c, hipMemcpyAsync(hfb.data(), fallback, 4 * nv, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0.0f;
     hipEventElapsedTime(&ms, e0, e1);
@@ -890,6 +900,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
             st.render_ms = ms;
             st.done_ms = ms;
             st.cancelled = cancelled ? 1 : 0;
+            st.denoise_fallback = hfb[v] ? 1 : 0;
             if (n_stats == 1) {
                 // the whole launch: rays of every pass (including samples the
                 // early-out discarded), and rays of the accumulated samples

@@ -75,10 +75,42 @@ int main() {
     vec3* den = pico_front::atrous_denoise(r.context(), pix.data(), var.data(), var.data(), pix.data(), W, H);
     if (!den) return 6;
     delete[] den;
+    // the learned filter with its fallback: no network, and a network whose
+    // parameters are NaN, both give the a-trous filter's image
+    {
+        std::vector<vec3> nrm(W * H), pos(W * H), alb(W * H);
+        if (pico_hip_gbuffer(r.context(), &cam, W, H, &nrm[0].x, &pos[0].x, &alb[0].x)) return 17;
+        vec3* ref = pico_front::atrous_denoise(r.context(), pix.data(), nrm.data(), pos.data(), alb.data(), W, H, 16);
+        int32_t fb = 0;
+        vec3* a = pico_front::cnn_denoise(r.context(), (pico_cnn*)nullptr, pix.data(), nrm.data(), pos.data(), alb.data(),
+                                          W, H, 16, &fb);
+        pico_cnn* net = nullptr;
+        if (!ref || !a || fb != 1 || pico_hip_cnn_create(r.context(), W, H, 1, &net)) return 18;
+        std::vector<float> nan(pico_hip_cnn_num_params(), NAN);
+        int32_t fb2 = 0;
+        vec3* b = pico_hip_cnn_set_params(net, nan.data()) ? nullptr
+                  : pico_front::cnn_denoise(r.context(), net, pix.data(), nrm.data(), pos.data(), alb.data(), W, H, 16, &fb2);
+        if (!b || fb2 != 1) return 19;
+        for (uint32_t i = 0; i < W * H; ++i) {
+            const bool same_a = (a[i].x == ref[i].x || (std::isnan(a[i].x) && std::isnan(ref[i].x)));
+            const bool same_b = (b[i].x == ref[i].x || (std::isnan(b[i].x) && std::isnan(ref[i].x)));
+            if (!same_a || !same_b) { std::printf("fallback pixel %u: %g %g vs %g\n", i, a[i].x, b[i].x, ref[i].x); return 20; }
+        }
+        delete[] ref;
+        delete[] a;
+        delete[] b;
+        pico_hip_cnn_destroy(net);
+    }
     quit = true;  // a set flag stops the next render before it starts
     std::vector<uint32_t> cnt2(W * H, 0);
     pico_front::RenderParams<vec3> p2{5, 8, 0.0f, H, W, false, false, pix.data(), cnt2.data(), var.data()};
     if (r.render_scene_to_memory(cam, p2, &quit) != PICO_ERR_CANCELLED) return 7;
+    // a cancelled render still tone-maps the (unchanged) frame, as the reference does
+    for (uint32_t i = 0; i < W * H; ++i) pix[i] = {4.0f, 4.0f, 4.0f};
+    pico_front::RenderParams<vec3> p2b{5, 8, 0.0f, H, W, true, false, pix.data(), cnt2.data(), var.data()};
+    if (r.render_scene_to_memory(cam, p2b, &quit) != PICO_ERR_CANCELLED) return 21;
+    for (uint32_t i = 0; i < W * H; ++i)
+        if (!(pix[i].x > 0.0f && pix[i].x < 4.0f) || cnt2[i] != 0) { std::printf("cancelled frame pixel %u: %g\n", i, pix[i].x); return 22; }
     // render_scene_to_file: a Radiance file whose first line is the magic
     std::vector<uint32_t> cnt3(W * H, 0);
     pico_front::RenderParams<vec3> p3{5, 4, 0.0f, H, W, false, false, pix.data(), cnt3.data(), var.data()};

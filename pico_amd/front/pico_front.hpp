@@ -129,8 +129,11 @@ public:
                                  reinterpret_cast<float*>(params.m_variance), &cancel, nullptr);
         done.store(true, std::memory_order_release);
         if (watch.joinable()) watch.join();
-        if (rc == PICO_ERR_CANCELLED) return rc;  // the reference returns early too
-        if (rc) return rc;
+        // a cancelled render still runs the denoiser and the tone mapping on
+        // the partial frame, as the reference does: its tiles return, then
+        // Scene.cpp:205-234 runs unconditionally
+        const int render_rc = rc;
+        if (rc && rc != PICO_ERR_CANCELLED) return rc;
         const size_t n = (size_t)params.m_Width * params.m_Height;
         if (params.m_denoise) {
             Vec3* nrm = new Vec3[n];
@@ -149,7 +152,7 @@ public:
         }
         if (params.m_tonemap)
             rc = pico_hip_tonemap(ctx_, reinterpret_cast<float*>(params.m_Pixels), params.m_Width, params.m_Height);
-        return rc;
+        return rc ? rc : render_rc;
     }
 
     // Scene::render_scene_to_file (Source/Core/Scene.cpp:236-246): render,
@@ -187,17 +190,22 @@ Vec3* atrous_denoise(pico_hip_ctx* ctx, const Vec3* pixels, const Vec3* normals,
     return out;
 }
 
-// The learned filter in the same shape (no counterpart in the reference).
+// The learned filter in the same shape (no counterpart in the reference),
+// with Util::atrous_denoise as its fallback: when cnn is NULL, refuses the
+// image or gives any value that is not finite, the returned buffer is the
+// a-trous filter's (tiles of `tile`), and *used_fallback (may be NULL) says
+// so.  Returns a new[] buffer, or nullptr on failure.
 template <class Vec3>
-Vec3* cnn_denoise(pico_cnn* cnn, const Vec3* pixels, const Vec3* normals, const Vec3* position, const Vec3* diffuse,
-                  uint32_t width, uint32_t height) {
+Vec3* cnn_denoise(pico_hip_ctx* ctx, pico_cnn* cnn, const Vec3* pixels, const Vec3* normals, const Vec3* position,
+                  const Vec3* diffuse, uint32_t width, uint32_t height, uint32_t tile = 64,
+                  int32_t* used_fallback = nullptr) {
     static_assert(sizeof(Vec3) == 12, "packed 3 x f32 per pixel");
     Vec3* out = new Vec3[(size_t)width * height];
-    const int rc = pico_hip_cnn_denoise_gbuffer(cnn, reinterpret_cast<const float*>(pixels),
-                                                reinterpret_cast<const float*>(normals),
-                                                reinterpret_cast<const float*>(position),
-                                                reinterpret_cast<const float*>(diffuse), width, height,
-                                                reinterpret_cast<float*>(out));
+    const int rc = pico_hip_denoise_gbuffer(ctx, cnn, reinterpret_cast<const float*>(pixels),
+                                            reinterpret_cast<const float*>(normals),
+                                            reinterpret_cast<const float*>(position),
+                                            reinterpret_cast<const float*>(diffuse), width, height, tile, tile,
+                                            reinterpret_cast<float*>(out), used_fallback);
     if (rc) {
         delete[] out;
         return nullptr;
