@@ -23,9 +23,11 @@
  *     e0a e0b e1a e1b ba bb d1a d1b d0a d0b head.
  */
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -45,10 +47,26 @@ struct T {  // H x W x C, channel fastest
     double at(int y, int x, int k) const { return v[((size_t)y * w + x) * c + k]; }
 };
 
+// Rows of an output on the host's threads (each row written by one thread,
+// so the result does not depend on the thread count).
+template <class F>
+void for_rows(int n, F f) {
+    const int threads = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));
+    if (threads <= 1 || n < 16) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t)
+        ts.emplace_back([&]() { for (int i; (i = next.fetch_add(1)) < n;) f(i); });
+    for (auto& t : ts) t.join();
+}
+
 void conv(const T& in, const double* W, const double* b, const Layer& L, bool relu, T& out) {
     out = T(in.h, in.w, L.cout);
     const int r = L.taps == 9 ? 1 : 0;
-    for (int y = 0; y < in.h; ++y)
+    for_rows(in.h, [&](int y) {
         for (int x = 0; x < in.w; ++x)
             for (int co = 0; co < L.cout; ++co) {
                 double s = b[co];
@@ -62,6 +80,7 @@ void conv(const T& in, const double* W, const double* b, const Layer& L, bool re
                     }
                 out.at(y, x, co) = relu ? (s > 0 ? s : 0) : s;
             }
+    });
 }
 
 // gradients of conv: given dPre (gradient of the pre-activation), accumulate dW, db, and dIn
