@@ -9,11 +9,15 @@ reference's running mean / variance with its early-out).  The scene is built
 and uploaded before timing.
 
 Printed: one JSON line with Mrays/s (rays of the accumulated samples / wall
-time of the timed steps), the roofline of the traversal kernel (`extend`),
-the reference CPU renderer's Mrays/s on the host cores, and the quality leg:
-relMSE against a 4096-spp FP64-accumulated render of the same view for the
-64-spp frame, 4 and 16 spp with the a-trous filter and 4 spp with the learned
-denoiser, each with its ms/frame (render + G-buffer + filter).
+time of the timed steps), the roofline of the traversal kernel (`extend`:
+algorithmic bytes from a counting run over its HIP-event launch time, and its
+HBM traffic from two rocprofv3 counter passes in this run), the reference CPU
+renderer's Mrays/s on every logical CPU of the host, and the quality leg:
+relMSE against a 4096-spp FP64-accumulated render of the same view (two
+independent halves, whose mutual relMSE is the oracle's noise floor) for the
+64-spp frame, 4 and 16 spp with the a-trous filters and 4 spp with the learned
+denoiser, each with its ms/frame (render + G-buffer + filter), for GGX and for
+the reference's Beckmann_All distribution.
 
 Ray = one closest-hit scene query (camera, extension or shadow ray).
 """
@@ -68,31 +72,70 @@ def cpu_baseline(scene, cam, w, h, spp, depth, seed):
         os.close(saved)
 
 
+def host_cpu():
+    """What the host offers this process: CPU model, sockets, physical cores,
+    threads per core and logical CPUs from /proc/cpuinfo, the CPUs this
+    process may run on, and the cgroup CPU quota."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            blocks = [b for b in f.read().split("\n\n") if b.strip()]
+        procs = []
+        for b in blocks:
+            d = {}
+            for line in b.splitlines():
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    d[k.strip()] = v.strip()
+            procs.append(d)
+        info["model"] = procs[0].get("model name", "?") if procs else "?"
+        cores = {(d.get("physical id", "0"), d.get("core id", str(i))) for i, d in enumerate(procs)}
+        info["sockets"] = len({d.get("physical id", "0") for d in procs})
+        info["physical_cores"] = len(cores)
+        info["threads_per_core"] = round(len(procs) / max(1, len(cores)), 2)
+    except OSError:
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+        info["cgroup_cpu_quota"] = None if q == "max" else round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    return info
+
+
 def _cpu_baseline(oracle, pa, scene, cam, w, h, spp, depth, seed):
-    threads = min(16, os.cpu_count() or 1)
+    # the reference's pool runs std::thread::hardware_concurrency() workers
+    # (Source/Util/ThreadPool.hpp:31): every logical CPU of the host
+    threads = os.cpu_count() or 1
     p = pa.params(w, h, spp, depth, tile=64, frame_seed=seed)
     cs = oracle.CpuScene(scene)
     _, _, _, st = cs.render(cam, p, rng=oracle.RNG_REFERENCE, threads=threads)
     rays = st.camera_rays + st.extension_rays + st.shadow_rays
     if oracle.ref_available():
         rs = oracle.RefScene(scene)
-        times = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            rs.render(cam, p, threads=threads)
-            times.append(time.perf_counter() - t0)
+        run = lambda: rs.render(cam, p, threads=threads)  # noqa: E731
         kind = "reference"
     else:
-        times = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            cs.render(cam, p, rng=oracle.RNG_REFERENCE, threads=threads)
-            times.append(time.perf_counter() - t0)
+        run = lambda: cs.render(cam, p, rng=oracle.RNG_REFERENCE, threads=threads)  # noqa: E731
         kind = "port"
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
     t = sorted(times)[1]
+    host = host_cpu()
     return {"value": round(rays / t / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": kind,
-            "sample": f"same scene and camera at {w}x{h}, {spp} spp, depth {depth}, tiles 64x64; "
-                      f"{rays} rays, median of 3 runs {t:.2f} s"}
+            "sample": f"same scene and camera at {w}x{h}, {spp} spp, depth {depth}, tiles 64x64, {threads} threads "
+                      f"(every logical CPU); {rays} rays, median of 3 runs {t:.2f} s "
+                      f"(runs {min(times):.2f}-{max(times):.2f} s)",
+            "host": host, "runs_s": [round(x, 3) for x in times],
+            "mrays_per_s_per_thread": round(rays / t / 1e6 / threads, 3)}
 
 
 def _median_ms(fn, reps, sync):
@@ -106,22 +149,53 @@ def _median_ms(fn, reps, sync):
     return sorted(times)[len(times) // 2]
 
 
-def quality_leg(ctx, pa, scene, cam, W, H, depth, oracle_spp, weights, d_frame):
-    """relMSE vs an FP64-accumulated oracle (frame seed 1000) and ms/frame of
-    the a-trous and learned filters at 4 and 16 spp (frame seed 7)."""
+def fp64_oracle(ctx, pa, cam, W, H, depth, spp, seed=1000, chunk=256):
+    """Two independent FP64 accumulations of spp/2 samples each (samples
+    [0, spp/2) and [spp/2, spp) of frame seed `seed`), and their sum: the
+    spp-sample oracle.  Returns (d_sum, half_a, half_b) with the halves on the
+    host as float64 [n, 3] means."""
     n = W * H
+    half = spp // 2
     d_sum = ctx.alloc(24 * n)
     d_sum.zero()
+    halves = []
     t0 = time.time()
-    chunk = 256
-    for base in range(0, oracle_spp, chunk):
-        p = pa.params(W, H, min(chunk, oracle_spp - base), depth, tile=16, frame_seed=1000)
-        ctx.render_accum64_d(cam, p, base, d_sum)
+    for h in range(2):
+        for base in range(h * half, (h + 1) * half, chunk):
+            p = pa.params(W, H, min(chunk, (h + 1) * half - base), depth, tile=16, frame_seed=seed)
+            ctx.render_accum64_d(cam, p, base, d_sum)
         ctx.sync()
-        log(f"[bench] oracle {base + chunk}/{oracle_spp} spp, {time.time() - t0:.1f} s")
+        halves.append(d_sum.download(np.float64, 3 * n).reshape(n, 3).copy())
+        log(f"[bench] oracle half {h}: {half} spp, {time.time() - t0:.1f} s")
+    a = halves[0] / half
+    b = (halves[1] - halves[0]) / half
+    return d_sum, a, b
+
+
+def relmse_np(x, ref):
+    return float(((x - ref) ** 2 / (ref ** 2 + 1e-2)).mean())
+
+
+def quality_leg(ctx, pa, scene, cam, W, H, depth, oracle_spp, weights, d_frame=None):
+    """relMSE vs an FP64-accumulated oracle (frame seed 1000) and ms/frame of
+    the a-trous and learned filters at 4 and 16 spp (frame seed 7).  The
+    oracle is two independent halves; their mutual relMSE is its noise floor
+    (the full oracle's own error is about a quarter of it)."""
+    n = W * H
+    t0 = time.time()
+    d_sum, half_a, half_b = fp64_oracle(ctx, pa, cam, W, H, depth, oracle_spp)
     t_oracle = time.time() - t0
+    split = relmse_np(half_a, half_b)
     out = {"oracle_spp": oracle_spp, "oracle_s": round(t_oracle, 1), "oracle_frame_seed": 1000,
-           "frame_relmse": round(ctx.relmse_d(d_frame, d_sum, oracle_spp, n), 6)}
+           "oracle_split_half_relmse": round(split, 6),
+           "oracle_noise_floor_relmse": round(split / 4, 6),
+           "oracle_split_half_worst_0.01pct_share": None}
+    e = ((half_a - half_b) ** 2 / (half_b ** 2 + 1e-2)).sum(axis=1)
+    k = max(1, n // 10000)
+    out["oracle_split_half_worst_0.01pct_share"] = round(float(np.partition(e, n - k)[n - k:].sum() / max(e.sum(), 1e-300)), 4)
+    del half_a, half_b
+    if d_frame is not None:
+        out["frame_relmse"] = round(ctx.relmse_d(d_frame, d_sum, oracle_spp, n), 6)
     d_pix, d_cnt, d_var = ctx.alloc(12 * n), ctx.alloc(4 * n), ctx.alloc(12 * n)
     d_n, d_p, d_a = ctx.alloc(12 * n), ctx.alloc(12 * n), ctx.alloc(12 * n)
     d_out, d_scr = ctx.alloc(12 * n), ctx.alloc(12 * n)
@@ -155,25 +229,20 @@ def quality_leg(ctx, pa, scene, cam, W, H, depth, oracle_spp, weights, d_frame):
         frame(spp, filt)  # warm
         ms = _median_ms(lambda: frame(spp, filt), 5, ctx.sync)
         img = frame(spp, filt)
+        if ref is None:
+            ref = d_sum.download(np.float64, 3 * n).reshape(n, 3) / oracle_spp
+        x = img.download(np.float32, 3 * n).reshape(n, 3).astype(np.float64)
         if filt == "atrous_ref":
             # the reference's tile-local widths divide by zero on tiles with
             # no luminance or no depth range (NaN, as in the reference): the
             # error is taken over the finite pixels and the rest are counted
-            if ref is None:
-                ref = d_sum.download(np.float64, 3 * n).reshape(n, 3) / oracle_spp
-            x = img.download(np.float32, 3 * n).reshape(n, 3).astype(np.float64)
             ok = np.isfinite(x).all(axis=1)
             e = (x[ok] - ref[ok]) ** 2 / (ref[ok] ** 2 + 1e-2)
             rows[name] = {"relmse_finite": round(float(e.mean()), 6), "nonfinite_pixels": int((~ok).sum()),
                           "ms_per_frame": round(ms, 2)}
         else:
             rows[name] = {"relmse": round(ctx.relmse_d(img, d_sum, oracle_spp, n), 6), "ms_per_frame": round(ms, 2)}
-            # share of the error carried by the worst 0.01 % of pixels: the
-            # reference's next-event weight (BSDF pdf times emission) makes the
-            # per-sample radiance heavy-tailed, so rare fireflies dominate
-            if ref is None:
-                ref = d_sum.download(np.float64, 3 * n).reshape(n, 3) / oracle_spp
-            x = img.download(np.float32, 3 * n).reshape(n, 3).astype(np.float64)
+            # share of the error carried by the worst 0.01 % of pixels (fireflies)
             e = ((x - ref) ** 2 / (ref ** 2 + 1e-2)).sum(axis=1)
             k = max(1, n // 10000)
             rows[name]["worst_0.01pct_share"] = round(float(np.partition(e, n - k)[n - k:].sum() / max(e.sum(), 1e-300)), 4)
@@ -209,6 +278,7 @@ def quality_leg(ctx, pa, scene, cam, W, H, depth, oracle_spp, weights, d_frame):
     if out.get("cnn_4spp") and out.get("atrous_16spp"):
         c, a = out["cnn_4spp"], out["atrous_16spp"]
         out["cnn4_beats_atrous16"] = bool(c["relmse"] < a["relmse"] and c["ms_per_frame"] <= a["ms_per_frame"])
+        out["cnn4_above_noise_floor"] = round(c["relmse"] / max(out["oracle_noise_floor_relmse"], 1e-300), 2)
     for d in (d_sum, d_pix, d_cnt, d_var, d_n, d_p, d_a, d_out, d_scr):
         d.free()
     if net is not None:
@@ -227,7 +297,7 @@ def look_at_box(rng, centre=50.0, r=(60.0, 110.0), jitter=15.0):
     return pa.Camera(tuple(pos), tuple(f), (0.0, 1.0, 0.0), 1.0, 10000.0)
 
 
-def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3):
+def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3, oracle_spp=4096):
     """64 views of the scene with resolutions from {128^2, 256^2, 512^2,
     1024^2} and 4-64 spp in one wavefront launch (autotuned launch shape),
     each followed by the learned denoiser on a second stream."""
@@ -285,6 +355,20 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3):
         net.denoise_gbuffer_d(_ptr_array(ctx, v.d_pixels), d_n, d_p, d_a, w, h, o)
     ctx.sync()
     seq = time.perf_counter() - t0
+    # error of the smallest and the largest view against an FP64 oracle of
+    # that view (oracle_spp samples, frame seed 1000): raw and denoised
+    quality = {}
+    for label, i in (("smallest", int(np.argmin(sizes))), ("largest", int(np.argmax(sizes)))):
+        v = views[i]
+        w, h = v.params.width, v.params.height
+        d_sum, a, b = fp64_oracle(ctx, pa, v.camera, w, h, depth, oracle_spp, seed=1000)
+        ref = (a + b) / 2
+        raw = keep[i][0].download(np.float32, 3 * w * h).reshape(-1, 3).astype(np.float64)
+        den = outs[i].download(np.float32, 3 * w * h).reshape(-1, 3).astype(np.float64)
+        quality[label] = {"view": i, "size": w, "spp": v.params.spp, "oracle_spp": oracle_spp,
+                          "relmse_raw": round(relmse_np(raw, ref), 6), "relmse_denoised": round(relmse_np(den, ref), 6),
+                          "oracle_noise_floor_relmse": round(relmse_np(a, b) / 4, 6)}
+        d_sum.free()
     net.close()
     for b in keep:
         for x in b:
@@ -306,7 +390,7 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3):
             "second_stream": {"ms_per_batch": round(wall2 * 1e3, 2),
                               "latency_ms_p50": round(float(np.percentile(lat2, 50)), 2),
                               "latency_ms_p99": round(float(np.percentile(lat2, 99)), 2)},
-            "sequential_ms": round(seq * 1e3, 2),
+            "sequential_ms": round(seq * 1e3, 2), "view_relmse": quality,
             "tuning": {"queue_paths": tuning[0], "tile_w": tuning[1], "tile_h": tuning[2],
                        "calib_ms": round(cal_ms, 2), "default_calib_ms": round(def_ms, 2)}}
 
@@ -460,6 +544,95 @@ def _desc_counts(scene):
     return int(vofs[-1]), int(tofs[-1])
 
 
+def traffic_child(args):
+    """Child of the traffic leg, run under rocprofv3 --pmc: one streaming copy
+    of --copy-mb MiB (the counters' calibration), then one frame of the bench
+    workload.  Prints nothing on stdout."""
+    import pico_amd as pa
+    ctx = pa.Context(0)
+    nbytes = args.copy_mb << 20
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    src.zero()
+    ctx.copy_d(dst, src, nbytes)
+    ctx.sync()
+    scene = pa.Scene.instanced(2, 16, 1024, 2000, 20000, 100.0, pa.MICROFACET_GGX)
+    ctx.upload_scene(scene)
+    W, H = args.width, args.height
+    n = W * H
+    d_pix, d_cnt, d_var = ctx.alloc(12 * n), ctx.alloc(4 * n), ctx.alloc(12 * n)
+    ctx.render_d(scene.camera(W / H), pa.params(W, H, args.spp, args.depth, tile=16, frame_seed=1), d_pix, d_cnt, d_var)
+    ctx.sync()
+    ctx.close()
+
+
+def _pmc_pass(counter, args):
+    """One rocprofv3 counter pass over the traffic child: the counter's value
+    per dispatch, averaged per kernel name."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        raise RuntimeError("rocprofv3 not found")
+    tmp = tempfile.mkdtemp(prefix="pico_pmc_", dir="/tmp")
+    try:
+        cmd = [prof, "--pmc", counter, "--kernel-include-regex", "k_extend|k_shadow|k_copy", "--output-format", "csv",
+               "-d", tmp, "-o", "run", "--", sys.executable, os.path.join(ROOT, "bench.py"), "--traffic-child",
+               "--width", str(args.width), "--height", str(args.height), "--spp", str(args.spp), "--depth",
+               str(args.depth), "--copy-mb", str(args.copy_mb)]
+        r = subprocess.run(cmd, cwd=tmp, env=dict(os.environ, TMPDIR="/tmp"), capture_output=True, text=True,
+                           timeout=240)
+        files = glob.glob(os.path.join(tmp, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise RuntimeError(f"rocprofv3 exit {r.returncode}, no counter file: {r.stderr[-400:]}")
+        per = {}
+        seen = set()
+        for fn in files:
+            with open(fn, newline="") as f:
+                for row in csv.DictReader(f):
+                    seen.add((row.get("Kernel_Name", "?")[:60], row.get("Counter_Name")))
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    key = (row["Kernel_Name"], row.get("Dispatch_Id", ""))
+                    per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+        out = {}
+        for (name, _), v in per.items():
+            for short in ("k_extend", "k_shadow", "k_copy"):
+                if "::" + short + "<" in name or "::" + short + "(" in name:
+                    out.setdefault(short, []).append(v)
+        if "k_copy" not in out or "k_extend" not in out:
+            raise RuntimeError(f"{counter}: dispatches of k_copy and k_extend expected, found {sorted(seen)[:8]}")
+        return {k: (len(v), sum(v) / len(v)) for k, v in out.items()}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def traffic_leg(args):
+    """HBM bytes per dispatch of the traversal kernels, measured in this run:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) from two rocprofv3 counter
+    passes over one frame, each scaled by what the same counter reads for the
+    streaming copy of known size."""
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        log(f"[bench] traffic: rocprofv3 --pmc {counter} over one frame ...")
+        res[counter] = _pmc_pass(counter, args)
+    copy_kib = float(args.copy_mb << 10)
+    f_scale = copy_kib / res["FETCH_SIZE"]["k_copy"][1]
+    w_scale = copy_kib / res["WRITE_SIZE"]["k_copy"][1]
+    out = {"fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+           "calibration": f"k_copy of {args.copy_mb} MiB read + written: FETCH_SIZE "
+                          f"{res['FETCH_SIZE']['k_copy'][1]:.0f} KiB, WRITE_SIZE {res['WRITE_SIZE']['k_copy'][1]:.0f} KiB"}
+    for k in ("k_extend", "k_shadow"):
+        if k in res["FETCH_SIZE"] and k in res["WRITE_SIZE"]:
+            fk, wk = res["FETCH_SIZE"][k][1], res["WRITE_SIZE"][k][1]
+            out[k] = {"dispatches": res["FETCH_SIZE"][k][0], "fetch_kib_per_dispatch": round(fk, 1),
+                      "write_kib_per_dispatch": round(wk, 1),
+                      "hbm_bytes_per_dispatch": round(fk * 1024 * f_scale + wk * 1024 * w_scale)}
+    return out
+
+
 def _ptr_array(ctx, ptr):
     """A non-owning device array at a raw pointer (the views' own buffers)."""
     import pico_amd as pa
@@ -487,7 +660,12 @@ def main():
     ap.add_argument("--no-tlas", action="store_true", help="skip the device TLAS rebuild and scene build legs")
     ap.add_argument("--weights", default=None, help="trained denoiser parameters (.npy; default: the shipped ones)")
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the streaming-copy calibration")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes (roofline traffic)")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_child:
+        traffic_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard by "
               "image bands or by views, one scene replica per GPU)", file=sys.stderr)
@@ -555,13 +733,17 @@ def main():
         launches_per_frame = ext_n / args.steps
         bytes_per_launch = traversal_bytes(count) / launches_per_frame
         achieved = bytes_per_launch / (ext_ms * 1e-3) / 1e9
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", "traffic_extend_instanced_1080p.json")
-        if os.path.exists(tpath):
-            with open(tpath) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+        traffic, tdetail = None, None
+        if not args.no_traffic:
+            try:
+                tdetail = traffic_leg(args)
+                traffic = tdetail["k_extend"]["hbm_bytes_per_dispatch"]
+                log(f"[bench] traffic {tdetail}")
+            except Exception as e:  # no profiler on this host: no traffic figure
+                tdetail = {"error": str(e)[:300]}
+                log(f"[bench] traffic leg failed: {e}")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "traffic_detail": tdetail,
                 "measured_copy_GBps": round(copy_gbps, 1),
                 "kernel": "extend", "launches": ext_n, "mean_launch_ms": round(ext_ms, 4),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch),
@@ -574,10 +756,17 @@ def main():
     if not args.no_quality:
         quality = quality_leg(ctx, pa, scene, cam, W, H, args.depth, args.oracle_spp, args.weights or pa.weights_path(),
                               d_pix)
+        # the same rows with the reference's own distribution, Beckmann_All,
+        # whose clamped D bounds the next-event weight
+        beck = pa.Scene.instanced(2, 16, 1024, 2000, 20000, 100.0, pa.MICROFACET_BECKMANN_ALL)
+        ctx.upload_scene(beck)
+        quality["beckmann_all"] = quality_leg(ctx, pa, beck, cam, W, H, args.depth, args.oracle_spp,
+                                              args.weights or pa.weights_path())
+        ctx.upload_scene(scene)
 
     batched = None
     if not args.no_batched:
-        batched = batched_leg(ctx, pa, args.depth, args.weights or pa.weights_path())
+        batched = batched_leg(ctx, pa, args.depth, args.weights or pa.weights_path(), oracle_spp=args.oracle_spp)
         log(f"[bench] batched views: {batched}")
 
     tlas = build = None
