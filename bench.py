@@ -748,6 +748,7 @@ def main():
                 "kernel": "extend", "launches": ext_n, "mean_launch_ms": round(ext_ms, 4),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch),
                 "bytes_per_ray": round(traversal_bytes(count) / max(1, count.camera_rays + count.extension_rays), 1),
+                "stack_spill_pushes_per_frame": count.spill_pushes, "stack_spill_pops_per_frame": count.spill_pops,
                 "shadow_kernel_mean_launch_ms": round(sh_ms, 4),
                 "shadow_achieved_GBps": round(traversal_bytes(count, True) / (sh_n / args.steps) / (sh_ms * 1e-3) / 1e9, 1)
                 if sh_n else None}

@@ -200,6 +200,8 @@ typedef struct pico_stats {
                                   the a-trous fallback (see pico_hip_denoise_gbuffer_d) */
     double   done_ms;        /* batched launches: launch start -> the view's output is
                                 final (after its denoiser when one is requested) */
+    uint64_t spill_pushes;   /* PICO_RENDER_COUNT only: traversal-stack entries written past */
+    uint64_t spill_pops;     /* the LDS part (device memory), and read back, all rays */
 } pico_stats;
 
 /* ---- context ----------------------------------------------------------- */

@@ -69,6 +69,7 @@ struct Ctl {
     unsigned long long rays_cam, rays_ext, rays_sh, rays_acc, samples_acc;
     unsigned long long nodes, tris, insts;           // extend (camera + extension rays)
     unsigned long long nodes_sh, tris_sh, insts_sh;  // shadow rays
+    unsigned long long spill_push, spill_pop;        // traversal-stack entries past LDS (both kernels)
 };
 
 struct Pool {
@@ -78,7 +79,7 @@ struct Pool {
     float4* rad;     // radiance, w: rays traced by the path
     uint4* rng;      // hs state, ms state
     uint32_t* key;   // (view << 26) | pixel
-    float4* hit;     // t, u, v, instance bits
+    float4* hit;     // t, u, v, instance bits, by extension-queue position
     uint32_t* hprim;
     float4* sh_o;
     float4* sh_d;
@@ -307,9 +308,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     const uint32_t n = ctl->ext_count[b];
-    TraceCounters tc{0, 0, 0};
+    TraceCounters tc{0, 0, 0, 0, 0};
     TraceState t;
-    uint32_t item = kDeadPath, res_base = 0, res_left = 0;
+    uint32_t item = kDeadPath, slot = 0, res_base = 0, res_left = 0;
     bool drained = false;
     for (;;) {
         // idle lanes take queue entries until none is idle or the queue is empty
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 const uint32_t i = q[e];
                 if (i != kDeadPath) {
                     item = i;
+                    slot = e;
                     trace_begin(s, t, st, xyz(P.ray_o[i]), xyz(P.ray_d[i]));
                 }
             }
@@ -327,8 +329,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         }
         if (!__ballot(item != kDeadPath)) break;
         if (item != kDeadPath && trace_step<kTraceLds, COUNT>(s, t, st, &tc, &ctl->overflow)) {
-            P.hit[item] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.inst));
-            P.hprim[item] = t.h.prim;
+            // hit records in queue order: the lanes of a wave hold neighbouring
+            // queue entries, so their records share cache lines (the shading
+            // pass reads them in the same order)
+            P.hit[slot] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.inst));
+            P.hprim[slot] = t.h.prim;
             item = kDeadPath;
         }
     }
@@ -336,6 +341,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         atomicAdd(&ctl->nodes, (unsigned long long)tc.nodes);
         atomicAdd(&ctl->tris, (unsigned long long)tc.tris);
         atomicAdd(&ctl->insts, (unsigned long long)tc.insts);
+        atomicAdd(&ctl->spill_push, (unsigned long long)tc.spill_push);
+        atomicAdd(&ctl->spill_pop, (unsigned long long)tc.spill_pop);
     }
 }
 
@@ -350,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
         bool cont = false, shadow = false;
         const uint32_t i = e < n ? q[e] : kDeadPath;
         if (i != kDeadPath) {
-            const float4 h4 = P.hit[i];
+            const float4 h4 = P.hit[e];
             const uint32_t inst = __float_as_uint(h4.w);
             const uint32_t key = P.key[i];
             const DView& V = views[key >> kViewShift];
@@ -371,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
                 }
             } else if (b < V.max_depth) {
                 Hit h;
-                h.t = h4.x; h.u = h4.y; h.v = h4.z; h.inst = inst; h.prim = P.hprim[i];
+                h.t = h4.x; h.u = h4.y; h.v = h4.z; h.inst = inst; h.prim = P.hprim[e];
                 const Frag f = s.inst[inst].blas_root >= kImplicitRoot ? implicit_frag(s, h, xyz(P.ray_o[i]), dir)
                                                                        : make_frag(s, h);
                 const DInstance& I = s.inst[inst];
@@ -468,7 +475,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
     const uint32_t n = ctl->sh_count[b];
-    TraceCounters tc{0, 0, 0};
+    TraceCounters tc{0, 0, 0, 0, 0};
     TraceState t;
     uint32_t item = kDeadPath, res_base = 0, res_left = 0;
     bool drained = false;
@@ -509,6 +516,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         atomicAdd(&ctl->nodes_sh, (unsigned long long)tc.nodes);
         atomicAdd(&ctl->tris_sh, (unsigned long long)tc.tris);
         atomicAdd(&ctl->insts_sh, (unsigned long long)tc.insts);
+        atomicAdd(&ctl->spill_push, (unsigned long long)tc.spill_push);
+        atomicAdd(&ctl->spill_pop, (unsigned long long)tc.spill_pop);
     }
 }
 
@@ -913,6 +922,8 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
                 st.shadow_node_visits = hc.nodes_sh;
                 st.shadow_tri_tests = hc.tris_sh;
                 st.shadow_instance_visits = hc.insts_sh;
+                st.spill_pushes = hc.spill_push;
+                st.spill_pops = hc.spill_pop;
                 for (uint32_t k = 0; k < nv; ++k) {
                     st.accumulated_rays += hs[4 * k];
                     st.samples += hs[4 * k + 1];

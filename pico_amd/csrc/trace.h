@@ -120,7 +120,7 @@ struct DevScene {
 
 struct Hit { float t, u, v; uint32_t inst, prim; };
 
-struct TraceCounters { uint32_t nodes, tris, insts; };
+struct TraceCounters { uint32_t nodes, tris, insts, spill_push, spill_pop; };
 
 constexpr uint32_t kSentinel = 0xfffffffeu;
 
@@ -139,23 +139,40 @@ __device__ __forceinline__ f3 slab_rcp(f3 d) {
 // Traversal stack of one lane: the first STACK entries in LDS (`lds`, stride
 // `lds_stride`), the next `spill_cap` in device memory (`spill`, stride
 // `spill_stride`, private to the lane); pushing beyond both sets `overflow`.
+// The LDS part is addressed through an address-space-3 pointer, so that its
+// accesses are LDS instructions: through a generic pointer (which a select
+// between the LDS and the spill pointer makes) they become flat accesses,
+// which also wait for every outstanding global load.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(1))) uint32_t global_u32;
+
 template <int STACK>
 struct TraceStack {
-    uint32_t* lds;
+    lds_u32* lds;
     uint32_t lds_stride;
-    uint32_t* spill;
+    global_u32* spill;
     uint32_t spill_stride;
     uint32_t spill_cap;
     uint32_t sp = 0;
+    __device__ __forceinline__ TraceStack(uint32_t* lds_, uint32_t lds_stride_, uint32_t* spill_, uint32_t spill_stride_,
+                                          uint32_t spill_cap_)
+        : lds((lds_u32*)lds_), lds_stride(lds_stride_), spill((global_u32*)spill_), spill_stride(spill_stride_),
+          spill_cap(spill_cap_) {}
     __device__ __forceinline__ void push(uint32_t v, uint32_t* overflow) {
-        if (sp < STACK) lds[sp * lds_stride] = v;
-        else if (sp < STACK + spill_cap) spill[(sp - STACK) * spill_stride] = v;
-        else { *overflow = 1u; return; }
+        if (sp < STACK) {
+            lds[sp * lds_stride] = v;
+        } else if (sp < STACK + spill_cap) {
+            spill[(sp - STACK) * spill_stride] = v;
+        } else {
+            *overflow = 1u;
+            return;
+        }
         ++sp;
     }
     __device__ __forceinline__ uint32_t pop() {
         --sp;
-        return sp < STACK ? lds[sp * lds_stride] : spill[(sp - STACK) * spill_stride];
+        if (sp < STACK) return lds[sp * lds_stride];
+        return spill[(sp - STACK) * spill_stride];
     }
 };
 
@@ -256,6 +273,135 @@ __device__ __forceinline__ void trace_begin(const DevScene& s, TraceState& t, Tr
     st.sp = 0;
 }
 
+// Inner node t.ref: its four quantised child boxes against the ray; on a hit
+// t.ref becomes the nearest child and the others are pushed farthest first
+// (returns true), else nothing is left of it (returns false).
+template <int STACK, bool COUNT>
+__device__ __forceinline__ bool inner_node(const DevScene& s, TraceState& t, TraceStack<STACK>& st, TraceCounters* cnt,
+                                           uint32_t* overflow) {
+    // No FP contraction here (the library is built without it): the
+    // object-space transform and Moller-Trumbore round exactly as the CPU
+    // restatement does; only the box tests use explicit fmaf (conservative,
+    // they only cull).
+    const float kWide = 1.0000005f;
+    const DNode4* n = s.nodes + t.ref;
+    const float4 a = ldg4(&n->origin[0]);
+    const uint4 qxy = *reinterpret_cast<const uint4*>(&n->qx[0]);
+    const uint2 qz = *reinterpret_cast<const uint2*>(&n->qz[0]);
+    const uint4 r = *reinterpret_cast<const uint4*>(&n->ref[0]);
+    if (COUNT) cnt->nodes++;
+    const uint32_t eb = __float_as_uint(a.w);
+    const float sx = exp2_scale(eb), sy = exp2_scale(eb >> 8), sz = exp2_scale(eb >> 16);
+    float key[4];
+    uint32_t cref[4] = {r.x, r.y, r.z, r.w};
+    const float Ax = sx * t.inv.x, Ay = sy * t.inv.y, Az = sz * t.inv.z;
+    const float Bx = fmaf(a.x, t.inv.x, t.oi.x), By = fmaf(a.y, t.inv.y, t.oi.y), Bz = fmaf(a.z, t.inv.z, t.oi.z);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int sh = 8 * i;
+        const float x0 = fmaf((float)((qxy.x >> sh) & 0xffu), Ax, Bx), x1 = fmaf((float)((qxy.y >> sh) & 0xffu), Ax, Bx);
+        const float y0 = fmaf((float)((qxy.z >> sh) & 0xffu), Ay, By), y1 = fmaf((float)((qxy.w >> sh) & 0xffu), Ay, By);
+        const float z0 = fmaf((float)((qz.x >> sh) & 0xffu), Az, Bz), z1 = fmaf((float)((qz.y >> sh) & 0xffu), Az, Bz);
+        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kWide;
+        const bool hit = cref[i] != kEmptyRef && tf >= 0.0f && tn <= tf && tn < t.h.t;
+        key[i] = hit ? tn : INFINITY;
+    }
+    // sort the four (distance, child) pairs, nearest first
+#define PICO_CSWAP(i, j)                                                  \
+    if (key[j] < key[i]) {                                                \
+        const float tk = key[i]; key[i] = key[j]; key[j] = tk;            \
+        const uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr;     \
+    }
+    PICO_CSWAP(0, 1) PICO_CSWAP(2, 3) PICO_CSWAP(0, 2) PICO_CSWAP(1, 3) PICO_CSWAP(1, 2)
+#undef PICO_CSWAP
+    const int nh = (key[0] != INFINITY) + (key[1] != INFINITY) + (key[2] != INFINITY) + (key[3] != INFINITY);
+    if (nh == 0) return false;
+    if (COUNT && nh > 1) cnt->spill_push += (uint32_t)max(0, (int)st.sp + nh - 1 - max((int)st.sp, STACK));
+    if (nh > 3) st.push(cref[3], overflow);
+    if (nh > 2) st.push(cref[2], overflow);
+    if (nh > 1) st.push(cref[1], overflow);
+    t.ref = cref[0];
+    return true;
+}
+
+// The triangles of BLAS leaf `leaf`, Moller-Trumbore with the reference's
+// epsilons (Source/Core/LowerLevelMeshBVH.cpp:165-207), in the current
+// (object) space.
+template <bool COUNT>
+__device__ __forceinline__ void leaf_tris(const DevScene& s, TraceState& t, uint32_t leaf, TraceCounters* cnt) {
+    const uint32_t first = leaf & kPrimMask, count = (leaf >> 27) & 15u;
+    for (uint32_t k = 0; k < count; ++k) {
+        const DTri* tr = s.tris + first + k;
+        const float4 p0 = ldg4(&tr->v0[0]);
+        const float4 p1 = ldg4(&tr->e1[0]);
+        const float4 p2 = ldg4(&tr->e2[0]);
+        if (COUNT) cnt->tris++;
+        const f3 e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z);
+        const f3 hh = cross(t.rd, e2);
+        const float det = dot(e1, hh);
+        if (det > -0.0000001f && det < 0.0000001f) continue;
+        const float f = 1.0f / det;
+        const f3 sv = t.ro - mk(p0.x, p0.y, p0.z);
+        const float u = f * dot(sv, hh);
+        if (u < 0.0f || u > 1.0f) continue;
+        const f3 q = cross(sv, e1);
+        const float v = f * dot(t.rd, q);
+        if (v < 0.0f || u + v > 1.0f) continue;
+        const float tt = f * dot(e2, q);
+        if (tt > 0.0000001f && tt < t.h.t) {
+            t.h.t = tt; t.h.u = u; t.h.v = v; t.h.inst = t.cur; t.h.prim = __float_as_uint(p0.w);
+        }
+    }
+}
+
+// TLAS leaf t.ref (one instance): a built-in shape is intersected at once
+// (returns false: nothing to descend into); a mesh instance pushes the
+// instance sentinel, moves the ray into object space and makes the BLAS root
+// the next item (returns true).
+template <int STACK, bool COUNT>
+__device__ __forceinline__ bool enter_instance(const DevScene& s, TraceState& t, TraceStack<STACK>& st,
+                                               TraceCounters* cnt, uint32_t* overflow) {
+    const uint32_t cur = t.ref & kPrimMask;
+    const DInstance* I = s.inst + cur;
+    const uint32_t root = I->blas_root;
+    if (COUNT) cnt->insts++;
+    f3 ro, rd;
+    object_ray(I, t.o, t.d, ro, rd);
+    if (root >= kImplicitRoot) {
+        // a built-in shape: its hit, at the world distance |M p - o| the
+        // reference compares (Source/Core/UpperLevelBVH.cpp:66-73); u keeps
+        // the object-space parameter for the shading point
+        float tt;
+        if (implicit_hit(root & 0xffu, ro, rd, tt)) {
+            const f3 w = to_world_point(s.xf[cur].m, ro + rd * tt);
+            const f3 dd = w - t.o;
+            const float dist = sqrtf((dd.x * dd.x + dd.y * dd.y) + (dd.z * dd.z + 0.0f));
+            if (dist < t.h.t) {
+                t.h.t = dist; t.h.u = tt; t.h.v = 0.0f; t.h.inst = cur; t.h.prim = 0;
+            }
+        }
+        return false;
+    }
+    t.cur = cur;
+    if (COUNT && st.sp >= STACK) cnt->spill_push++;
+    st.push(kSentinel, overflow);
+    t.ro = ro;
+    t.rd = rd;
+    t.inv = slab_rcp(t.rd);
+    t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
+    t.in_blas = true;
+    t.ref = root;
+    return true;
+}
+
+__device__ __forceinline__ void leave_instance(TraceState& t) {
+    t.ro = t.o; t.rd = t.d;
+    t.inv = slab_rcp(t.d);
+    t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
+    t.in_blas = false;
+}
+
 // One traversal step: an inner node (continue with its nearest hit child, the
 // others pushed farthest first), a BLAS leaf (its triangles) or a TLAS leaf
 // (move into the instance's object space); when that leads nowhere further the
@@ -264,119 +410,20 @@ __device__ __forceinline__ void trace_begin(const DevScene& s, TraceState& t, Tr
 template <int STACK, bool COUNT>
 __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& t, TraceStack<STACK>& st, TraceCounters* cnt,
                                            uint32_t* overflow) {
-    // No FP contraction here (the library is built without it): the
-    // object-space transform and Moller-Trumbore round exactly as the CPU
-    // restatement does; only the box tests use explicit fmaf (conservative,
-    // they only cull).
-    const float kWide = 1.0000005f;
     if (!(t.ref & kLeafBit)) {
-        const DNode4* n = s.nodes + t.ref;
-        const float4 a = ldg4(&n->origin[0]);
-        const uint4 qxy = *reinterpret_cast<const uint4*>(&n->qx[0]);
-        const uint2 qz = *reinterpret_cast<const uint2*>(&n->qz[0]);
-        const uint4 r = *reinterpret_cast<const uint4*>(&n->ref[0]);
-        if (COUNT) cnt->nodes++;
-        const uint32_t eb = __float_as_uint(a.w);
-        const float sx = exp2_scale(eb), sy = exp2_scale(eb >> 8), sz = exp2_scale(eb >> 16);
-        float key[4];
-        uint32_t cref[4] = {r.x, r.y, r.z, r.w};
-        const float Ax = sx * t.inv.x, Ay = sy * t.inv.y, Az = sz * t.inv.z;
-        const float Bx = fmaf(a.x, t.inv.x, t.oi.x), By = fmaf(a.y, t.inv.y, t.oi.y), Bz = fmaf(a.z, t.inv.z, t.oi.z);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int sh = 8 * i;
-            const float x0 = fmaf((float)((qxy.x >> sh) & 0xffu), Ax, Bx), x1 = fmaf((float)((qxy.y >> sh) & 0xffu), Ax, Bx);
-            const float y0 = fmaf((float)((qxy.z >> sh) & 0xffu), Ay, By), y1 = fmaf((float)((qxy.w >> sh) & 0xffu), Ay, By);
-            const float z0 = fmaf((float)((qz.x >> sh) & 0xffu), Az, Bz), z1 = fmaf((float)((qz.y >> sh) & 0xffu), Az, Bz);
-            const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
-            const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kWide;
-            const bool hit = cref[i] != kEmptyRef && tf >= 0.0f && tn <= tf && tn < t.h.t;
-            key[i] = hit ? tn : INFINITY;
-        }
-        // sort the four (distance, child) pairs, nearest first
-#define PICO_CSWAP(i, j)                                                  \
-        if (key[j] < key[i]) {                                            \
-            const float tk = key[i]; key[i] = key[j]; key[j] = tk;        \
-            const uint32_t tr = cref[i]; cref[i] = cref[j]; cref[j] = tr; \
-        }
-        PICO_CSWAP(0, 1) PICO_CSWAP(2, 3) PICO_CSWAP(0, 2) PICO_CSWAP(1, 3) PICO_CSWAP(1, 2)
-#undef PICO_CSWAP
-        const int nh = (key[0] != INFINITY) + (key[1] != INFINITY) + (key[2] != INFINITY) + (key[3] != INFINITY);
-        if (nh > 0) {
-            if (nh > 3) st.push(cref[3], overflow);
-            if (nh > 2) st.push(cref[2], overflow);
-            if (nh > 1) st.push(cref[1], overflow);
-            t.ref = cref[0];
-            return false;
-        }
+        if (inner_node<STACK, COUNT>(s, t, st, cnt, overflow)) return false;
     } else if (t.in_blas) {
-        const uint32_t first = t.ref & kPrimMask, count = (t.ref >> 27) & 15u;
-        for (uint32_t k = 0; k < count; ++k) {
-            const DTri* tr = s.tris + first + k;
-            const float4 p0 = ldg4(&tr->v0[0]);
-            const float4 p1 = ldg4(&tr->e1[0]);
-            const float4 p2 = ldg4(&tr->e2[0]);
-            if (COUNT) cnt->tris++;
-            // Moller-Trumbore with the reference's epsilons,
-            // Source/Core/LowerLevelMeshBVH.cpp:165-207
-            const f3 e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z);
-            const f3 hh = cross(t.rd, e2);
-            const float det = dot(e1, hh);
-            if (det > -0.0000001f && det < 0.0000001f) continue;
-            const float f = 1.0f / det;
-            const f3 sv = t.ro - mk(p0.x, p0.y, p0.z);
-            const float u = f * dot(sv, hh);
-            if (u < 0.0f || u > 1.0f) continue;
-            const f3 q = cross(sv, e1);
-            const float v = f * dot(t.rd, q);
-            if (v < 0.0f || u + v > 1.0f) continue;
-            const float tt = f * dot(e2, q);
-            if (tt > 0.0000001f && tt < t.h.t) {
-                t.h.t = tt; t.h.u = u; t.h.v = v; t.h.inst = t.cur; t.h.prim = __float_as_uint(p0.w);
-            }
-        }
+        leaf_tris<COUNT>(s, t, t.ref, cnt);
     } else {
-        // TLAS leaf: one instance; move the ray into object space
-        const uint32_t cur = t.ref & kPrimMask;
-        const DInstance* I = s.inst + cur;
-        const uint32_t root = I->blas_root;
-        if (COUNT) cnt->insts++;
-        f3 ro, rd;
-        object_ray(I, t.o, t.d, ro, rd);
-        if (root >= kImplicitRoot) {
-            // a built-in shape: its hit, at the world distance |M p - o| the
-            // reference compares (Source/Core/UpperLevelBVH.cpp:66-73); u
-            // keeps the object-space parameter for the shading point
-            float tt;
-            if (implicit_hit(root & 0xffu, ro, rd, tt)) {
-                const f3 w = to_world_point(s.xf[cur].m, ro + rd * tt);
-                const f3 dd = w - t.o;
-                const float dist = sqrtf((dd.x * dd.x + dd.y * dd.y) + (dd.z * dd.z + 0.0f));
-                if (dist < t.h.t) {
-                    t.h.t = dist; t.h.u = tt; t.h.v = 0.0f; t.h.inst = cur; t.h.prim = 0;
-                }
-            }
-        } else {
-            t.cur = cur;
-            st.push(kSentinel, overflow);
-            t.ro = ro;
-            t.rd = rd;
-            t.inv = slab_rcp(t.rd);
-            t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
-            t.in_blas = true;
-            t.ref = root;
-            return false;
-        }
+        if (enter_instance<STACK, COUNT>(s, t, st, cnt, overflow)) return false;
     }
     // pop; an instance sentinel restores the world-space ray
     for (;;) {
         if (st.sp == 0) return true;
+        if (COUNT && st.sp > STACK) cnt->spill_pop++;
         t.ref = st.pop();
         if (t.ref != kSentinel) return false;
-        t.ro = t.o; t.rd = t.d;
-        t.inv = slab_rcp(t.d);
-        t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
-        t.in_blas = false;
+        leave_instance(t);
     }
 }
 
