@@ -58,6 +58,11 @@ struct ConvArgs {
     float* out_f32;         // [P][npad] (head logits) or null
     const bf16* mask; int mask_stride, mask_off;  // multiply by (mask > 0)
     int B, H, W, kpad, npad, taps, flip, relu;
+    // optional: input channels [0, up_ch) read from `up`, a tensor of half the
+    // resolution (stride up_stride), with the nearest-neighbour rule of the
+    // upsampling (y / 2, x / 2): the upsample + concatenation fused into the
+    // convolution's input staging
+    const bf16* up; int up_stride, up_ch;
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -202,19 +207,37 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
         for (int t = 0; t < NT; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
     const bool live = xw < a.W;  // every wave takes part in the barriers
     constexpr int kVec = KS / 8;      // 16-byte vectors per pixel and chunk
+    constexpr int kBandVec = 3 * kBand * kVec;
+    constexpr int kPer = (kBandVec + 255) / 256;  // band vectors per thread
+    // A chunk's band is loaded into registers with every load in flight at
+    // once, and the next chunk's loads are issued before the current chunk's
+    // MFMAs, so the global-load latency overlaps the arithmetic.
+    uint4 pre[kPer];
+    auto load_band = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            pre[j] = make_uint4(0, 0, 0, 0);
+            if (e < kBandVec) {
+                const int r = e / (kBand * kVec), rem = e - r * kBand * kVec;
+                const int px = rem / kVec, c8 = rem - px * kVec;
+                const int yy = y + r - 1, xx = x0 + px - 1;
+                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                    pre[j] = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + yy) * a.W + xx) * a.in_stride +
+                                                             a.in_off + k0 + 8 * c8);
+            }
+        }
+    };
+    load_band(0);
     for (int k0 = 0; k0 < a.kpad; k0 += KS) {
         __syncthreads();
-        // input band of this chunk
-        for (int e = threadIdx.x; e < 3 * kBand * kVec; e += 256) {
-            const int r = e / (kBand * kVec), rem = e - r * kBand * kVec;
-            const int px = rem / kVec, c8 = rem - px * kVec;
-            const int yy = y + r - 1, xx = x0 + px - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                v = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + yy) * a.W + xx) * a.in_stride + a.in_off +
-                                                    k0 + 8 * c8);
-            *reinterpret_cast<uint4*>(band + (r * kBand + px) * KS + 8 * c8) = v;
+        // input band of this chunk (rows y-1..y+1, pixels x0-1..x0+kRun)
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            if (e < kBandVec) *reinterpret_cast<uint4*>(band + e * 8) = pre[j];
         }
+        if (k0 + KS < a.kpad) load_band(k0 + KS);
         // the taps in three groups of three (one row of the 3x3 kernel each)
         for (int g = 0; g < 3; ++g) {
             if (g) __syncthreads();
@@ -269,6 +292,183 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int x = xw + m * 16 + 4 * kq + r;
+                if (x >= a.W) continue;
+                float v = acc[m][t][r] + bias;
+                if (a.relu) v = v > 0.0f ? v : 0.0f;
+                const size_t p = (size_t)(b * a.H + y) * a.W + x;
+                if (a.mask && !(bf2f(a.mask[p * a.mask_stride + a.mask_off + n]) > 0.0f)) v = 0.0f;
+                if (a.out_f32) a.out_f32[p * a.npad + n] = v;
+                else a.out[p * a.out_stride + a.out_off + n] = f2bf(v);
+            }
+    }
+}
+
+// 3x3 convolution with four output rows per workgroup, one per wave, every
+// operand from LDS.  Per K chunk of KS input channels the workgroup stages
+// the six input rows its four output rows read (its MT * 16 pixels plus one
+// on each side, zero outside the image) and the chunk's weights for all nine
+// taps.  Input rows are read 1.5 times per layer (6 rows for 4) instead of 3
+// times with one output row per workgroup; every global load of a chunk is
+// issued at once into registers, the next chunk's while the current one runs
+// on MFMA, so a chunk costs two barriers and the loads' latency is hidden
+// behind the arithmetic of the other workgroups and chunks.  Plain bf16
+// outputs go through LDS and leave as 16-byte stores of 8 channels.
+template <int MT, int NT, int KS>
+__global__ __launch_bounds__(256) void k_conv3r(ConvArgs a) {
+    extern __shared__ bf16 lds[];
+    constexpr int kRun = MT * 16;       // pixels of one output row
+    constexpr int kBand = kRun + 2;     // pixels of one staged row
+    constexpr int kRows = 6;            // staged input rows: y0 - 1 .. y0 + 4
+    constexpr int ldw = KS + 8;         // weight rows padded against bank conflicts
+    constexpr int KL = KS / 4;          // channels per lane and fragment
+    constexpr int kVec = KS / 8;        // 16-byte vectors per pixel (or weight row) and chunk
+    constexpr int kBandVec = kRows * kBand * kVec;
+    constexpr int kPer = (kBandVec + 255) / 256;
+    bf16* band = lds;                   // [6][kBand][KS]
+    bf16* wl = lds + kRows * kBand * KS;  // [3][NT * 16][ldw]: one row of taps
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int runs_x = (a.W + kRun - 1) / kRun, groups = (a.H + 3) / 4;
+    // consecutive workgroups go to the 8 XCDs in turn: give each XCD a
+    // contiguous run of blocks, so row groups that share halo rows share an L2
+    const uint32_t xb = blockIdx.x & 7u, kb = blockIdx.x >> 3, per = gridDim.x >> 3, remb = gridDim.x & 7u;
+    const int blk = (int)(xb < remb ? xb * (per + 1) + kb : remb * (per + 1) + (xb - remb) * per + kb);
+    const int rest = blk / runs_x;
+    const int x0 = (blk - rest * runs_x) * kRun;
+    const int b = rest / groups, y0 = (rest - b * groups) * 4;
+    const int y = y0 + wave;
+    const int n0 = blockIdx.y * NT * 16;
+    const int mi = lane & 15, kq = lane >> 4;
+    const int Hh = a.H >> 1, Wh = a.W >> 1;
+    f4v acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    const bool live = y < a.H;  // every wave takes part in the barriers
+    uint4 pre[kPer];
+    auto load_band = [&](int k0) {
+        const bool from_up = a.up && k0 < a.up_ch;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            pre[j] = make_uint4(0, 0, 0, 0);
+            if (e < kBandVec) {
+                const int r = e / (kBand * kVec), rem = e - r * kBand * kVec;
+                const int px = rem / kVec, c8 = rem - px * kVec;
+                const int yy = y0 + r - 1, xx = x0 + px - 1;
+                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                    const bf16* src = from_up
+                        ? a.up + ((size_t)(b * Hh + (yy >> 1)) * Wh + (xx >> 1)) * a.up_stride + k0 + 8 * c8
+                        : a.in + ((size_t)(b * a.H + yy) * a.W + xx) * a.in_stride + a.in_off + k0 + 8 * c8;
+                    pre[j] = *reinterpret_cast<const uint4*>(src);
+                }
+            }
+        }
+    };
+    load_band(0);
+    for (int k0 = 0; k0 < a.kpad; k0 += KS) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            if (e < kBandVec) *reinterpret_cast<uint4*>(band + e * 8) = pre[j];
+        }
+        if (k0 + KS < a.kpad) load_band(k0 + KS);
+        // the taps in three groups of three (one row of the 3x3 kernel each)
+        for (int g = 0; g < 3; ++g) {
+            if (g) __syncthreads();
+            for (int e = threadIdx.x; e < 3 * NT * 16 * kVec; e += 256) {
+                const int tg = e / (NT * 16 * kVec), rem = e - tg * NT * 16 * kVec;
+                const int nn = rem / kVec, c8 = rem - nn * kVec;
+                *reinterpret_cast<uint4*>(wl + (tg * NT * 16 + nn) * ldw + 8 * c8) = *reinterpret_cast<const uint4*>(
+                    a.w + ((size_t)(3 * g + tg) * a.npad + n0 + nn) * a.kpad + k0 + 8 * c8);
+            }
+            __syncthreads();
+            if (!live) continue;
+#pragma unroll
+            for (int tg = 0; tg < 3; ++tg) {
+                const int tap = 3 * g + tg;
+                int dy = tap / 3 - 1, dx = tap % 3 - 1;
+                if (a.flip) { dy = -dy; dx = -dx; }
+                const bf16* ab = band + ((wave + dy + 1) * kBand + mi + dx + 1) * KS + KL * kq;
+                const bf16* bb = wl + (tg * NT * 16 + mi) * ldw + KL * kq;
+                if constexpr (KS == 32) {
+                    uint4 av[MT], bv[NT];
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const uint4*>(ab + m * 16 * KS);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const uint4*>(bb + t * 16 * ldw);
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[m]),
+                                                                                __builtin_bit_cast(bf16x8, bv[t]), acc[m][t], 0, 0, 0);
+                } else {
+                    s4 av[MT], bv[NT];
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const s4*>(ab + m * 16 * KS);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const s4*>(bb + t * 16 * ldw);
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[t], acc[m][t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (!a.mask && !a.out_f32) {
+        // bf16 outputs: the wave's kRun x NT*16 tile through LDS (the band,
+        // free once every wave is past its last chunk), then 16-byte stores
+        constexpr int kCh = NT * 16, ldo = kCh + 8;
+        // (the launch sizes the LDS for the larger of staging and output tiles)
+        __syncthreads();
+        bf16* ot = lds + wave * kRun * ldo;
+        if (live) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int c = t * 16 + mi;
+                const float bias = a.bias ? a.bias[n0 + c] : 0.0f;
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        float v = acc[m][t][r] + bias;
+                        if (a.relu) v = v > 0.0f ? v : 0.0f;
+                        ot[(m * 16 + 4 * kq + r) * ldo + c] = f2bf(v);
+                    }
+            }
+        }
+        __syncthreads();
+        if (!live) return;
+        constexpr int kOutVec = kRun * kCh / 8;
+#pragma unroll
+        for (int j = 0; j < (kOutVec + 63) / 64; ++j) {
+            const int e = lane + 64 * j;
+            if (e < kOutVec) {
+                const int px = e / (kCh / 8), c8 = e - px * (kCh / 8);
+                const int x = x0 + px;
+                if (x < a.W) {
+                    const size_t p = (size_t)(b * a.H + y) * a.W + x;
+                    *reinterpret_cast<uint4*>(a.out + p * a.out_stride + a.out_off + n0 + 8 * c8) =
+                        *reinterpret_cast<const uint4*>(ot + px * ldo + 8 * c8);
+                }
+            }
+        }
+        return;
+    }
+    if (!live) return;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int n = n0 + t * 16 + mi;
+        const float bias = a.bias ? a.bias[n] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = x0 + m * 16 + 4 * kq + r;
                 if (x >= a.W) continue;
                 float v = acc[m][t][r] + bias;
                 if (a.relu) v = v > 0.0f ? v : 0.0f;
@@ -626,13 +826,18 @@ int launch_conv_k(pico_cnn* n, const char* name, const ConvArgs& a) {
     const int run = 4 * (8 / nt) * 16;
     const dim3 grid((unsigned)(a.B * a.H * ((a.W + run - 1) / run)), (unsigned)(a.npad / (nt * 16)));
     if (a.taps == 9 && nt >= 2) {
-        // 3x3 with all operands in LDS: three rows of run + 2 pixels and
-        // one row of three taps' weights, per chunk of KS channels
-        const size_t lds3 = (3 * (size_t)(run + 2) * KS + 3 * (size_t)nt * 16 * (KS + 8)) * sizeof(bf16);
-        if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv3<2, 4, KS>), grid, 256, lds3, a);
-        else PICO_LAUNCH(n->c, name, (k_conv3<4, 2, KS>), grid, 256, lds3, a);
+        // 3x3 with all operands in LDS, four output rows per workgroup: six
+        // rows of MT * 16 + 2 pixels and one row of three taps' weights, per
+        // chunk of KS channels
+        const int mt = 8 / nt, run4 = mt * 16;
+        const dim3 grid4((unsigned)(a.B * ((a.H + 3) / 4) * ((a.W + run4 - 1) / run4)), (unsigned)(a.npad / (nt * 16)));
+        const size_t lds3 = std::max(6 * (size_t)(run4 + 2) * KS + 3 * (size_t)nt * 16 * (KS + 8),
+                                     4 * (size_t)run4 * (nt * 16 + 8)) * sizeof(bf16);
+        if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv3r<2, 4, KS>), grid4, 256, lds3, a);
+        else PICO_LAUNCH(n->c, name, (k_conv3r<4, 2, KS>), grid4, 256, lds3, a);
         return PICO_OK;
     }
+    if (a.up) return fail(n->c, PICO_ERR_STATE, "fused upsampling needs the 3x3 kernel");
     const size_t lds = (size_t)nt * 16 * (a.kpad + 8) * sizeof(bf16);
     if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv<2, 4, KS>), grid, 256, lds, a);
     else if (nt == 2) PICO_LAUNCH(n->c, name, (k_conv<4, 2, KS>), grid, 256, lds, a);
@@ -645,10 +850,10 @@ int launch_conv(pico_cnn* n, const char* name, const ConvArgs& a) {
 }
 
 int conv(pico_cnn* n, int l, const bf16* in, int in_stride, int in_off, bf16* out, int out_stride, int out_off, int B,
-         int H, int W, bool relu, float* out_f32 = nullptr) {
+         int H, int W, bool relu, float* out_f32 = nullptr, const bf16* up = nullptr, int up_stride = 0, int up_ch = 0) {
     const LayerDef& L = kL[l];
     ConvArgs a{in, in_stride, in_off, n->wf[l], n->bpad[l], out, out_stride, out_off, out_f32, nullptr, 0, 0,
-               B, H, W, L.kpad, L.npad, L.taps, 0, relu ? 1 : 0};
+               B, H, W, L.kpad, L.npad, L.taps, 0, relu ? 1 : 0, up, up_stride, up_ch};
     return launch_conv(n, "cnn_conv_fwd", a);
 }
 
@@ -706,11 +911,23 @@ int forward(pico_cnn* n, const float* d_in, int B, int H, int W, bool keep_wk) {
     PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 16 * 8), 256, 0, n->c1, 192, 128, n->p1, 64, B, H1, W1);
     if ((rc = conv(n, 4, n->p1, 64, 0, n->ba, 128, 0, B, H2, W2, true))) return rc;
     if ((rc = conv(n, 5, n->ba, 128, 0, n->bb, 128, 0, B, H2, W2, true))) return rc;
-    PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 / 4 * 16), 256, 0, n->bb, n->c1, 192, 0, 128, B, H1, W1);
-    if ((rc = conv(n, 6, n->c1, 192, 0, n->d1a, 64, 0, B, H1, W1, true))) return rc;
+    // The decoder's inputs are [nearest upsample of the level below, skip]:
+    // a training pass materialises them (the backward pass reads them), an
+    // inference pass reads the upsampled channels straight from the
+    // half-resolution tensor while staging the convolution's input.
+    if (keep_wk) {
+        PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 / 4 * 16), 256, 0, n->bb, n->c1, 192, 0, 128, B, H1, W1);
+        if ((rc = conv(n, 6, n->c1, 192, 0, n->d1a, 64, 0, B, H1, W1, true))) return rc;
+    } else if ((rc = conv(n, 6, n->c1, 192, 0, n->d1a, 64, 0, B, H1, W1, true, nullptr, n->bb, 128, 128))) {
+        return rc;
+    }
     if ((rc = conv(n, 7, n->d1a, 64, 0, n->d1b, 64, 0, B, H1, W1, true))) return rc;
-    PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 * 8), 256, 0, n->d1b, n->c0, 96, 0, 64, B, Hp, Wp);
-    if ((rc = conv(n, 8, n->c0, 96, 0, n->d0a, 32, 0, B, Hp, Wp, true))) return rc;
+    if (keep_wk) {
+        PICO_LAUNCH(n->c, "cnn_up", k_up, grid_for(P0 * 8), 256, 0, n->d1b, n->c0, 96, 0, 64, B, Hp, Wp);
+        if ((rc = conv(n, 8, n->c0, 96, 0, n->d0a, 32, 0, B, Hp, Wp, true))) return rc;
+    } else if ((rc = conv(n, 8, n->c0, 96, 0, n->d0a, 32, 0, B, Hp, Wp, true, nullptr, n->d1b, 64, 64))) {
+        return rc;
+    }
     if ((rc = conv(n, 9, n->d0a, 32, 0, n->d0b, 32, 0, B, Hp, Wp, true))) return rc;
     if ((rc = conv(n, 10, n->d0b, 32, 0, nullptr, 32, 0, B, Hp, Wp, false, n->head))) return rc;
     PICO_LAUNCH(n->c, "cnn_kpn_apply", k_kpn_apply, grid_for((size_t)B * H * W), 256, 0, n->head, d_in,
