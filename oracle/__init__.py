@@ -339,26 +339,32 @@ def cnn_num_params():
     return f()
 
 
-def cnn_forward(params, in10, h, w):
+def cnn_forward(params, in10, h, w, bf16=False):
+    """FP64 forward of the learned denoiser; with bf16 the weights, the input
+    and every stored activation are rounded as the device stores them."""
     lib = cpu_lib()
-    lib.cnnref_forward.restype = None
-    lib.cnnref_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    fn = lib.cnnref_forward_bf16 if bf16 else lib.cnnref_forward
+    fn.restype = None
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     p = np.ascontiguousarray(params, np.float64)
     x = np.ascontiguousarray(in10, np.float32)
     out = np.empty((h * w, 3), np.float64)
-    lib.cnnref_forward(_p(p), _p(x), h, w, _p(out))
+    fn(_p(p), _p(x), h, w, _p(out))
     return out
 
 
-def cnn_loss_grad(params, in10, targets, batch, h, w, grads=True):
+def cnn_loss_grad(params, in10, targets, batch, h, w, grads=True, bf16=False):
+    """Loss and FP64 parameter gradients; with bf16 at the device's storage
+    precision (weights, activations and stored gradients in bfloat16)."""
     lib = cpu_lib()
-    lib.cnnref_loss_grad.restype = C.c_double
-    lib.cnnref_loss_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    fn = lib.cnnref_loss_grad_bf16 if bf16 else lib.cnnref_loss_grad
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
     p = np.ascontiguousarray(params, np.float64)
     x = np.ascontiguousarray(in10, np.float32)
     t = np.ascontiguousarray(targets, np.float32)
     g = np.zeros(p.size, np.float64) if grads else None
-    loss = lib.cnnref_loss_grad(_p(p), _p(x), _p(t), batch, h, w, _p(g))
+    loss = fn(_p(p), _p(x), _p(t), batch, h, w, _p(g))
     return loss, g
 
 

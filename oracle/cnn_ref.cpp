@@ -155,7 +155,25 @@ struct Acts {
     std::vector<double> wk;  // softmax weights, 25 per pixel
 };
 
-void forward(const Net& n, const float* in10, int h, int w, Acts& A, std::vector<double>& out_rgb) {
+// Round to bfloat16 (round to nearest even, through float), as the device
+// stores its weights and activations.
+double round_bf16(double v) {
+    float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+void round_all(T& t) {
+    for (double& v : t.v) v = round_bf16(v);
+}
+
+// `bf16`: round the input, every stored activation (conv outputs after the
+// ReLU, pools) to bfloat16 and the head's logits to float, as the device
+// network stores them; the caller passes bf16 weights and float biases.
+void forward(const Net& n, const float* in10, int h, int w, Acts& A, std::vector<double>& out_rgb, bool bf16 = false) {
     // the network runs on the image padded to a multiple of 4 by repeating its
     // last row and column; the kernel is applied on the image itself
     const int hp = (h + 3) / 4 * 4, wp = (w + 3) / 4 * 4;
@@ -166,23 +184,26 @@ void forward(const Net& n, const float* in10, int h, int w, Acts& A, std::vector
                 double v = in10[(size_t)k * h * w + (size_t)std::min(y, h - 1) * w + std::min(x, w - 1)];
                 if (k < 3) v = std::log1p(v > 0 ? v : 0.0);
                 if (k == 9) v = std::log1p(v);
-                A.x.at(y, x, k) = v;
+                A.x.at(y, x, k) = bf16 ? round_bf16((double)(float)v) : v;
             }
-    conv(A.x, n.p[0][0], n.p[0][1], kLayers[0], true, A.e0a);
-    conv(A.e0a, n.p[1][0], n.p[1][1], kLayers[1], true, A.e0b);
-    A.p0 = pool(A.e0b);
-    conv(A.p0, n.p[2][0], n.p[2][1], kLayers[2], true, A.e1a);
-    conv(A.e1a, n.p[3][0], n.p[3][1], kLayers[3], true, A.e1b);
-    A.p1 = pool(A.e1b);
-    conv(A.p1, n.p[4][0], n.p[4][1], kLayers[4], true, A.ba);
-    conv(A.ba, n.p[5][0], n.p[5][1], kLayers[5], true, A.bb);
+    auto R = [&](T& t) { if (bf16) round_all(t); };
+    conv(A.x, n.p[0][0], n.p[0][1], kLayers[0], true, A.e0a); R(A.e0a);
+    conv(A.e0a, n.p[1][0], n.p[1][1], kLayers[1], true, A.e0b); R(A.e0b);
+    A.p0 = pool(A.e0b); R(A.p0);
+    conv(A.p0, n.p[2][0], n.p[2][1], kLayers[2], true, A.e1a); R(A.e1a);
+    conv(A.e1a, n.p[3][0], n.p[3][1], kLayers[3], true, A.e1b); R(A.e1b);
+    A.p1 = pool(A.e1b); R(A.p1);
+    conv(A.p1, n.p[4][0], n.p[4][1], kLayers[4], true, A.ba); R(A.ba);
+    conv(A.ba, n.p[5][0], n.p[5][1], kLayers[5], true, A.bb); R(A.bb);
     A.c1 = up_concat(A.bb, A.e1b);
-    conv(A.c1, n.p[6][0], n.p[6][1], kLayers[6], true, A.d1a);
-    conv(A.d1a, n.p[7][0], n.p[7][1], kLayers[7], true, A.d1b);
+    conv(A.c1, n.p[6][0], n.p[6][1], kLayers[6], true, A.d1a); R(A.d1a);
+    conv(A.d1a, n.p[7][0], n.p[7][1], kLayers[7], true, A.d1b); R(A.d1b);
     A.c0 = up_concat(A.d1b, A.e0b);
-    conv(A.c0, n.p[8][0], n.p[8][1], kLayers[8], true, A.d0a);
-    conv(A.d0a, n.p[9][0], n.p[9][1], kLayers[9], true, A.d0b);
+    conv(A.c0, n.p[8][0], n.p[8][1], kLayers[8], true, A.d0a); R(A.d0a);
+    conv(A.d0a, n.p[9][0], n.p[9][1], kLayers[9], true, A.d0b); R(A.d0b);
     conv(A.d0b, n.p[10][0], n.p[10][1], kLayers[10], false, A.head);
+    if (bf16)
+        for (double& v : A.head.v) v = (double)(float)v;
     A.wk.assign((size_t)h * w * 25, 0.0);
     out_rgb.assign((size_t)h * w * 3, 0.0);
     for (int y = 0; y < h; ++y)
@@ -219,13 +240,53 @@ void cnnref_forward(const double* params, const float* in10, int h, int w, doubl
     std::memcpy(out_rgb, o.data(), o.size() * sizeof(double));
 }
 
+/* The same forward at the device's storage precision: weights rounded to
+ * bfloat16, biases to float, the input and every stored activation to
+ * bfloat16, the logits to float; sums in FP64.  What remains between this
+ * and the device is the order and precision of the device's FP32 sums. */
+void cnnref_forward_bf16(const double* params, const float* in10, int h, int w, double* out_rgb) {
+    std::vector<double> p(cnnref_num_params());
+    size_t o = 0;
+    for (auto& l : kLayers) {
+        const size_t nw = (size_t)l.taps * l.cin * l.cout;
+        for (size_t i = 0; i < nw; ++i, ++o) p[o] = round_bf16(params[o]);
+        for (int i = 0; i < l.cout; ++i, ++o) p[o] = (double)(float)params[o];
+    }
+    Net n(p.data());
+    Acts A;
+    std::vector<double> out;
+    forward(n, in10, h, w, A, out, true);
+    std::memcpy(out_rgb, out.data(), out.size() * sizeof(double));
+}
+
 /* Loss (mean relMSE over the batch's pixels and channels) and its gradient
  * with respect to every parameter (added into grads, FP64). targets packed
  * h*w*3 per image. */
-double cnnref_loss_grad(const double* params, const float* in10, const float* targets, int batch, int h, int w,
-                        double* grads) {
+}  // extern "C"
+
+namespace {
+
+// Weights rounded to bfloat16 and biases to float, as the device keeps them.
+std::vector<double> device_params(const double* params) {
+    std::vector<double> p(cnnref_num_params());
+    size_t o = 0;
+    for (auto& l : kLayers) {
+        const size_t nw = (size_t)l.taps * l.cin * l.cout;
+        for (size_t i = 0; i < nw; ++i, ++o) p[o] = round_bf16(params[o]);
+        for (int i = 0; i < l.cout; ++i, ++o) p[o] = (double)(float)params[o];
+    }
+    return p;
+}
+
+// `bf16`: the forward at the device's storage precision, and every gradient
+// tensor the device stores in bfloat16 rounded where it stores it (logit
+// gradients, each layer's input gradient after its ReLU mask, the
+// upsampling's sums, the pooling gradients and the skip sums).
+double loss_grad(const double* params, const float* in10, const float* targets, int batch, int h, int w,
+                 double* grads, bool bf16) {
     if (h % 4 || w % 4) return NAN;  // the backward is written for sizes the pooling divides
     Net n(params);
+    auto R = [&](T& t) { if (bf16) round_all(t); };
     const double count = (double)batch * h * w * 3;
     double loss = 0.0;
     std::vector<double> zero(cnnref_num_params(), 0.0);
@@ -235,7 +296,7 @@ double cnnref_loss_grad(const double* params, const float* in10, const float* ta
         const float* tg = targets + (size_t)b * h * w * 3;
         Acts A;
         std::vector<double> out;
-        forward(n, in, h, w, A, out);
+        forward(n, in, h, w, A, out, bf16);
         T dhead(h, w, 25);
         for (int y = 0; y < h; ++y)
             for (int x = 0; x < w; ++x) {
@@ -257,6 +318,7 @@ double cnnref_loss_grad(const double* params, const float* in10, const float* ta
                 }
                 for (int k = 0; k < 25; ++k) dhead.at(y, x, k) = wk[k] * (dw[k] - dot);
             }
+        R(dhead);
         if (!grads) continue;
         auto G = [&](int l, int k) {
             size_t o = 0;
@@ -266,12 +328,15 @@ double cnnref_loss_grad(const double* params, const float* in10, const float* ta
         T d;
         conv_bwd(A.d0b, n.p[10][0], kLayers[10], dhead, G(10, 0), G(10, 1), &d);
         relu_mask(d, A.d0b);
+        R(d);
         T d2;
         conv_bwd(A.d0a, n.p[9][0], kLayers[9], d, G(9, 0), G(9, 1), &d2);
         relu_mask(d2, A.d0a);
+        R(d2);
         T dc0;
         conv_bwd(A.c0, n.p[8][0], kLayers[8], d2, G(8, 0), G(8, 1), &dc0);
         relu_mask(dc0, A.c0);
+        R(dc0);
         // split concat0 = [up(d1b) 64, e0b 32]
         T dd1b(h / 2, w / 2, 64), de0b(h, w, 32);
         for (int y = 0; y < h; ++y)
@@ -279,40 +344,69 @@ double cnnref_loss_grad(const double* params, const float* in10, const float* ta
                 for (int k = 0; k < 64; ++k) dd1b.at(y / 2, x / 2, k) += dc0.at(y, x, k);
                 for (int k = 0; k < 32; ++k) de0b.at(y, x, k) = dc0.at(y, x, 64 + k);
             }
+        R(dd1b);
         T d3;
         conv_bwd(A.d1a, n.p[7][0], kLayers[7], dd1b, G(7, 0), G(7, 1), &d3);
         relu_mask(d3, A.d1a);
+        R(d3);
         T dc1;
         conv_bwd(A.c1, n.p[6][0], kLayers[6], d3, G(6, 0), G(6, 1), &dc1);
         relu_mask(dc1, A.c1);
+        R(dc1);
         T dbb(h / 4, w / 4, 128), de1b(h / 2, w / 2, 64);
         for (int y = 0; y < h / 2; ++y)
             for (int x = 0; x < w / 2; ++x) {
                 for (int k = 0; k < 128; ++k) dbb.at(y / 2, x / 2, k) += dc1.at(y, x, k);
                 for (int k = 0; k < 64; ++k) de1b.at(y, x, k) = dc1.at(y, x, 128 + k);
             }
+        R(dbb);
         T d4;
         conv_bwd(A.ba, n.p[5][0], kLayers[5], dbb, G(5, 0), G(5, 1), &d4);
         relu_mask(d4, A.ba);
+        R(d4);
         T dp1;
         conv_bwd(A.p1, n.p[4][0], kLayers[4], d4, G(4, 0), G(4, 1), &dp1);
+        R(dp1);
         T pb1 = pool_bwd(dp1, h / 2, w / 2);
         for (size_t i = 0; i < de1b.v.size(); ++i) de1b.v[i] += pb1.v[i];
         relu_mask(de1b, A.e1b);
+        R(de1b);
         T d5;
         conv_bwd(A.e1a, n.p[3][0], kLayers[3], de1b, G(3, 0), G(3, 1), &d5);
         relu_mask(d5, A.e1a);
+        R(d5);
         T dp0;
         conv_bwd(A.p0, n.p[2][0], kLayers[2], d5, G(2, 0), G(2, 1), &dp0);
+        R(dp0);
         T pb0 = pool_bwd(dp0, h, w);
         for (size_t i = 0; i < de0b.v.size(); ++i) de0b.v[i] += pb0.v[i];
         relu_mask(de0b, A.e0b);
+        R(de0b);
         T d6;
         conv_bwd(A.e0a, n.p[1][0], kLayers[1], de0b, G(1, 0), G(1, 1), &d6);
         relu_mask(d6, A.e0a);
+        R(d6);
         conv_bwd(A.x, n.p[0][0], kLayers[0], d6, G(0, 0), G(0, 1), nullptr);
     }
     return loss / count;
+}
+
+}  // namespace
+
+extern "C" {
+
+double cnnref_loss_grad(const double* params, const float* in10, const float* targets, int batch, int h, int w,
+                        double* grads) {
+    return loss_grad(params, in10, targets, batch, h, w, grads, false);
+}
+
+/* The same at the device's storage precision (see cnnref_forward_bf16), with
+ * the backward's stored gradients rounded to bfloat16 where the device
+ * stores them. */
+double cnnref_loss_grad_bf16(const double* params, const float* in10, const float* targets, int batch, int h, int w,
+                             double* grads) {
+    const std::vector<double> p = device_params(params);
+    return loss_grad(p.data(), in10, targets, batch, h, w, grads, true);
 }
 
 }  // extern "C"

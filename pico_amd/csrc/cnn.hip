@@ -176,133 +176,6 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     }
 }
 
-// The same convolution for 3x3 taps with every operand from LDS.  Per K
-// chunk of KS input channels the workgroup stages the three input rows its
-// output row reads (its run of pixels plus one on each side, zero outside
-// the image), then the chunk's weights one row of three taps at a time; the
-// taps read their A fragments (16 consecutive pixels x KS channels: one
-// contiguous block per wave) and B fragments from LDS.  Input rows are read
-// from memory once per chunk instead of once per tap.
-template <int MT, int NT, int KS>
-__global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
-    extern __shared__ bf16 lds[];
-    constexpr int kRun = 4 * MT * 16;
-    constexpr int kBand = kRun + 2;     // pixels of one staged row
-    constexpr int ldw = KS + 8;         // weight rows padded against bank conflicts
-    constexpr int KL = KS / 4;          // channels per lane and fragment
-    bf16* band = lds;                   // [3][kBand][KS]
-    bf16* wl = lds + 3 * kBand * KS;    // [3][NT * 16][ldw]: one row of taps
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int runs_x = (a.W + kRun - 1) / kRun;
-    const int row = blockIdx.x / runs_x;  // b*H + y
-    const int x0 = (blockIdx.x % runs_x) * kRun;
-    const int xw = x0 + wave * MT * 16;
-    const int b = row / a.H, y = row % a.H;
-    const int n0 = blockIdx.y * NT * 16;
-    const int mi = lane & 15, kq = lane >> 4;
-    f4v acc[MT][NT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-    const bool live = xw < a.W;  // every wave takes part in the barriers
-    constexpr int kVec = KS / 8;      // 16-byte vectors per pixel and chunk
-    constexpr int kBandVec = 3 * kBand * kVec;
-    constexpr int kPer = (kBandVec + 255) / 256;  // band vectors per thread
-    // A chunk's band is loaded into registers with every load in flight at
-    // once, and the next chunk's loads are issued before the current chunk's
-    // MFMAs, so the global-load latency overlaps the arithmetic.
-    uint4 pre[kPer];
-    auto load_band = [&](int k0) {
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int e = threadIdx.x + 256 * j;
-            pre[j] = make_uint4(0, 0, 0, 0);
-            if (e < kBandVec) {
-                const int r = e / (kBand * kVec), rem = e - r * kBand * kVec;
-                const int px = rem / kVec, c8 = rem - px * kVec;
-                const int yy = y + r - 1, xx = x0 + px - 1;
-                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                    pre[j] = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + yy) * a.W + xx) * a.in_stride +
-                                                             a.in_off + k0 + 8 * c8);
-            }
-        }
-    };
-    load_band(0);
-    for (int k0 = 0; k0 < a.kpad; k0 += KS) {
-        __syncthreads();
-        // input band of this chunk (rows y-1..y+1, pixels x0-1..x0+kRun)
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int e = threadIdx.x + 256 * j;
-            if (e < kBandVec) *reinterpret_cast<uint4*>(band + e * 8) = pre[j];
-        }
-        if (k0 + KS < a.kpad) load_band(k0 + KS);
-        // the taps in three groups of three (one row of the 3x3 kernel each)
-        for (int g = 0; g < 3; ++g) {
-            if (g) __syncthreads();
-            for (int e = threadIdx.x; e < 3 * NT * 16 * kVec; e += 256) {
-                const int tg = e / (NT * 16 * kVec), rem = e - tg * NT * 16 * kVec;
-                const int nn = rem / kVec, c8 = rem - nn * kVec;
-                *reinterpret_cast<uint4*>(wl + (tg * NT * 16 + nn) * ldw + 8 * c8) = *reinterpret_cast<const uint4*>(
-                    a.w + ((size_t)(3 * g + tg) * a.npad + n0 + nn) * a.kpad + k0 + 8 * c8);
-            }
-            __syncthreads();
-            if (!live) continue;
-            for (int tg = 0; tg < 3; ++tg) {
-                const int tap = 3 * g + tg;
-                int dy = tap / 3 - 1, dx = tap % 3 - 1;
-                if (a.flip) { dy = -dy; dx = -dx; }
-                const bf16* ab = band + ((dy + 1) * kBand + (xw - x0) + mi + dx + 1) * KS + KL * kq;
-                const bf16* bb = wl + (tg * NT * 16 + mi) * ldw + KL * kq;
-                if constexpr (KS == 32) {
-                    uint4 av[MT], bv[NT];
-#pragma unroll
-                    for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const uint4*>(ab + m * 16 * KS);
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const uint4*>(bb + t * 16 * ldw);
-#pragma unroll
-                    for (int m = 0; m < MT; ++m)
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[m]),
-                                                                                __builtin_bit_cast(bf16x8, bv[t]), acc[m][t], 0, 0, 0);
-                } else {
-                    s4 av[MT], bv[NT];
-#pragma unroll
-                    for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const s4*>(ab + m * 16 * KS);
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const s4*>(bb + t * 16 * ldw);
-#pragma unroll
-                    for (int m = 0; m < MT; ++m)
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[t], acc[m][t], 0, 0, 0);
-                }
-            }
-        }
-    }
-    if (xw >= a.W) return;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int n = n0 + t * 16 + mi;
-        const float bias = a.bias ? a.bias[n] : 0.0f;
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = xw + m * 16 + 4 * kq + r;
-                if (x >= a.W) continue;
-                float v = acc[m][t][r] + bias;
-                if (a.relu) v = v > 0.0f ? v : 0.0f;
-                const size_t p = (size_t)(b * a.H + y) * a.W + x;
-                if (a.mask && !(bf2f(a.mask[p * a.mask_stride + a.mask_off + n]) > 0.0f)) v = 0.0f;
-                if (a.out_f32) a.out_f32[p * a.npad + n] = v;
-                else a.out[p * a.out_stride + a.out_off + n] = f2bf(v);
-            }
-    }
-}
-
 // 3x3 convolution with four output rows per workgroup, one per wave, every
 // operand from LDS.  Per K chunk of KS input channels the workgroup stages
 // the six input rows its four output rows read (its MT * 16 pixels plus one
@@ -488,7 +361,7 @@ struct WgradArgs {
     int B, H, W, kpad, npad, cin, cout, taps, chunk;
 };
 
-constexpr int kWgTile = 64;
+constexpr int kWgTile = 64;  // pixels staged per step (a multiple of 32)
 constexpr int kWgSlots = 8;  // 16x16 output tiles per wave (8 waves cover up to 64 tiles)
 
 // dW[t][ci][co] = sum_p x[p + off_t][ci] g[p][co]; grid.x pixel chunks, grid.y taps.
@@ -531,21 +404,24 @@ __global__ __launch_bounds__(512) void k_conv_wgrad(WgradArgs a) {
         __syncthreads();
         if (t == 0 && threadIdx.x < a.npad)
             for (int q = 0; q < kWgTile; ++q) bsum += bf2f(gs[q * a.npad + threadIdx.x]);
+        // K = pixels, 32 per instruction (v_mfma_f32_16x16x32_bf16): lane
+        // (mi, kq) holds pixels kk + 8 kq .. + 7 of input channel ci (A) and
+        // output channel co (B)
 #pragma unroll 1
-        for (int kk = 0; kk < kWgTile; kk += 16) {
+        for (int kk = 0; kk < kWgTile; kk += 32) {
 #pragma unroll
             for (int slot = 0; slot < kWgSlots; ++slot) {
                 const int ti = wave + 8 * slot;
                 if (ti >= tiles) continue;
                 const int ci = (ti / tiles_n) * 16 + mi, co = (ti % tiles_n) * 16 + mi;
-                s4 av, bv;
+                bf16x8 av, bv;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int q = kk + 4 * kq + e;
-                    av[e] = (short)xs[q * a.kpad + ci];
-                    bv[e] = (short)gs[q * a.npad + co];
+                for (int e = 0; e < 8; ++e) {
+                    const int q = kk + 8 * kq + e;
+                    av[e] = __builtin_bit_cast(__bf16, xs[q * a.kpad + ci]);
+                    bv[e] = __builtin_bit_cast(__bf16, gs[q * a.npad + co]);
                 }
-                acc[slot] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[slot], 0, 0, 0);
+                acc[slot] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[slot], 0, 0, 0);
             }
         }
     }

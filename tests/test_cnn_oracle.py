@@ -55,3 +55,24 @@ def test_gradient_matches_finite_differences():
         fd = (oracle.cnn_loss_grad(pp, x, t, b, h, w, grads=False)[0] -
               oracle.cnn_loss_grad(pm, x, t, b, h, w, grads=False)[0]) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-5 * max(abs(fd), 1e-4), (i, fd, g[i])
+
+
+def test_device_precision_oracle_is_the_fp64_one_rounded():
+    """The device-precision forward and gradients differ from the FP64 ones
+    by bfloat16 rounding only: small but not zero, and equal where rounding
+    changes nothing (weights and activations already bfloat16: an all-zero
+    input gives the same output either way)."""
+    rng = np.random.default_rng(2)
+    h = w = 8
+    p = rng.normal(0, 0.2, oracle.cnn_num_params())
+    x = rng.uniform(0.1, 1.0, (10, h, w)).astype(np.float32)
+    a = oracle.cnn_forward(p, x, h, w)
+    b = oracle.cnn_forward(p, x, h, w, bf16=True)
+    rel = np.abs(a - b) / (np.abs(a) + 1e-2)
+    assert 0 < rel.max() < 0.05
+    t = rng.uniform(0.1, 1.0, (1, h * w, 3)).astype(np.float32)
+    la, ga = oracle.cnn_loss_grad(p, x[None], t, 1, h, w)
+    lb, gb = oracle.cnn_loss_grad(p, x[None], t, 1, h, w, bf16=True)
+    assert abs(la - lb) < 0.05 * la
+    cos = ga @ gb / (np.linalg.norm(ga) * np.linalg.norm(gb))
+    assert 0.99 < cos < 1.0

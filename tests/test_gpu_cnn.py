@@ -48,6 +48,32 @@ def test_forward_matches_oracle(ctx, h, w):
     assert np.median(err) < 5e-3 and err.max() < 0.1, (np.median(err), err.max())
 
 
+def _round_bf16(a):
+    """float32 -> nearest bfloat16 (ties to even), back in float32."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("h,w,shipped", [(16, 16, False), (24, 40, False), (48, 64, True)])
+def test_forward_matches_the_oracle_at_device_precision(ctx, h, w, shipped):
+    """The oracle with the device's storage precision (bf16 weights, input and
+    activations, float logits; FP64 sums): what is left is the FP32
+    accumulation order, so the bound is an order of magnitude tighter than
+    against the full-FP64 oracle."""
+    rng = np.random.default_rng(10 * h + w)
+    x, _ = _data(rng, 1, h, w)
+    p = np.load(pa.weights_path(), allow_pickle=False) if shipped else _params(rng)
+    net = pa.CNN(ctx, w, h, 1)
+    net.set_params(p)
+    g = net.denoise(x[0], w, h)
+    c = oracle.cnn_forward(p.astype(np.float64), x[0], h, w, bf16=True)
+    err = np.abs(g - c) / (np.abs(c) + 1e-2)
+    print(f"[bf16 oracle] {h}x{w} median {np.median(err):.2e} p99 {np.percentile(err, 99):.2e} max {err.max():.2e}")
+    assert np.median(err) < 1e-5 and np.percentile(err, 99) < 1e-2 and err.max() < 3e-2, \
+        (np.median(err), np.percentile(err, 99), err.max())
+
+
 def test_gradients_match_oracle(ctx):
     rng = np.random.default_rng(3)
     b, h, w = 2, 16, 16
@@ -64,6 +90,17 @@ def test_gradients_match_oracle(ctx):
             cos = a @ r / (np.linalg.norm(a) * np.linalg.norm(r) + 1e-30)
             rel = np.linalg.norm(a - r) / (np.linalg.norm(r) + 1e-30)
             assert cos > 0.99 and rel < 0.15, (i, s, cos, rel)
+    # against the oracle at the device's storage precision (bf16 weights,
+    # activations and stored gradients): only the FP32 sums differ
+    bl, bg = oracle.cnn_loss_grad(p.astype(np.float64), x, t, b, h, w, bf16=True)
+    worst = 0.0
+    for i, (ws, bs) in enumerate(oracle.cnn_layer_slices()):
+        for s in (ws, bs):
+            a, r = gg[s].astype(np.float64), bg[s]
+            rel = np.linalg.norm(a - r) / (np.linalg.norm(r) + 1e-30)
+            worst = max(worst, rel)
+    print(f"[bf16 oracle] loss rel {abs(gl - bl) / bl:.2e}, worst per-tensor gradient rel {worst:.2e}")
+    assert abs(gl - bl) <= 1e-6 * bl and worst < 1e-2, (abs(gl - bl) / bl, worst)
 
 
 def test_training_lowers_the_loss(ctx):
