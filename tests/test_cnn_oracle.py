@@ -64,7 +64,12 @@ def test_device_precision_oracle_is_the_fp64_one_rounded():
     input gives the same output either way)."""
     rng = np.random.default_rng(2)
     h = w = 8
-    p = rng.normal(0, 0.2, oracle.cnn_num_params())
+    # He-scaled weights, so that the softmax does not saturate (a saturated
+    # one leaves only rounding noise in the gradients)
+    p = np.zeros(oracle.cnn_num_params())
+    for (cin, cout, taps), (ws, bs) in zip(oracle.CNN_LAYERS, oracle.cnn_layer_slices()):
+        p[ws] = rng.normal(0, np.sqrt(2.0 / (taps * cin)), ws.stop - ws.start)
+        p[bs] = rng.normal(0, 0.05, cout)
     x = rng.uniform(0.1, 1.0, (10, h, w)).astype(np.float32)
     a = oracle.cnn_forward(p, x, h, w)
     b = oracle.cnn_forward(p, x, h, w, bf16=True)
