@@ -25,6 +25,8 @@
  *                           (Source/Core/UpperLevelBVH.cpp:22-57)
  *   pico_scene_load_json    Scene::Scene(ThreadPool&, path) and its section handlers
  *                           (Source/Core/Scene.cpp:36-84, 297-720)
+ *   pico_scene_load_obj     Scene::Scene(ThreadPool&, dir, aiScene*, options) with parse_node
+ *                           (Source/Core/Scene.cpp:86-141, 722-971) for an OBJ model file
  *   pico_scene_set_sun / _set_sky_image  the -SunDirection/-SunColour/-Skybox options of
  *                           Scene::Scene(ThreadPool&, path, aiScene, options) (Source/Core/Scene.cpp:86-123)
  *   pico_write_hdr          the file half of Scene::render_scene_to_file (Source/Core/Scene.cpp:236-246)
@@ -320,6 +322,20 @@ int  pico_hip_render_d(pico_hip_ctx* ctx, const pico_camera* cam, const pico_ren
                        float* d_pixels, uint32_t* d_sample_count, float* d_variance,
                        const volatile int32_t* cancel, pico_stats* stats);
 
+/* Progressive display: pico_hip_render (host buffers) that, after every
+ * chunk of samples, copies the accumulated pixels and sample counts to a
+ * snapshot and calls `cb` on the calling thread with it (valid during the
+ * call only); samples_done is the sample index the chunk reached.  A
+ * non-zero return stops the render with PICO_ERR_CANCELLED, the samples so
+ * far kept.  Replaces the reference's viewer reading frame_memory while the
+ * render thread writes it (Source/main.cpp:119-132) with a consistent copy
+ * between two chunks. */
+typedef int (*pico_pass_callback)(void* user, const float* pixels, const uint32_t* sample_count, uint32_t width,
+                                  uint32_t height, uint32_t samples_done);
+int  pico_hip_render_progressive(pico_hip_ctx* ctx, const pico_camera* cam, const pico_render_params* params,
+                                 float* pixels, uint32_t* sample_count, float* variance, pico_pass_callback cb,
+                                 void* user, pico_stats* stats);
+
 /* Radiance of every (pixel, sample) with sample index in
  * [sample_base, sample_base + params->spp), without accumulation: out is
  * spp x H x W x 3 f32 (sample-major).  Used for per-path parity checks. */
@@ -480,6 +496,20 @@ int  pico_hip_get_tuning(pico_hip_ctx* ctx, pico_tuning* tuning);
 int  pico_hip_render_views_d(pico_hip_ctx* ctx, const pico_view* views, uint32_t n, uint32_t queue_paths,
                              pico_stats* stats);
 
+/* pico_hip_render_views_d for views in host memory (the reference's
+ * layout per view): staged to the device, rendered in one launch, copied
+ * back.  With PICO_RENDER_RESUME in a view's params its buffers are uploaded
+ * first. */
+typedef struct pico_view_host {
+    pico_camera        camera;
+    pico_render_params params;
+    float*    pixels;           /* host, W*H*3 */
+    uint32_t* sample_count;     /* host, W*H   */
+    float*    variance;         /* host, W*H*3 */
+} pico_view_host;
+int  pico_hip_render_views(pico_hip_ctx* ctx, const pico_view_host* views, uint32_t n, uint32_t queue_paths,
+                           pico_stats* stats);
+
 /* The same launch, each view followed by the learned denoiser as soon as its
  * last chunk of samples is done: G-buffer + network between chunks or on a
  * second stream (PICO_TUNE_SIDE_STREAM), overlapping the remaining passes.
@@ -544,6 +574,19 @@ int  pico_scene_training(uint32_t seed, uint32_t microfacet, pico_scene** out);
  * order of their names.  Image materials are refused.  On error PICO_ERR_ARG and
  * the reason (file, line, entry) in `err` (err_len bytes, may be NULL). */
 int  pico_scene_load_json(const char* path, pico_scene** out, char* err, size_t err_len);
+/* A model file through the reference's import path, Scene::Scene(ThreadPool&,
+ * dir, aiScene*, options) with parse_node and add_material
+ * (Source/Core/Scene.cpp:86-141, 722-971), for Wavefront .obj files as assimp
+ * imports them with the flags of Source/main.cpp:53-59: the file's meshes
+ * (split at objects, groups and material changes), each an instance with
+ * the identity transform, a Light when its material emits (Ke) and otherwise
+ * a Dielectric with the Beckmann_All lobe; materials from the file's mtllib
+ * libraries (Kd, Ks, Ke; roughness 0 as the reference derives it; assimp's
+ * default diffuse 0.6 where none applies).  Black sky and no sun: set them
+ * with pico_scene_set_sky_image / pico_scene_set_sun as the -Skybox, -Sun*
+ * options do.  No cameras (the reference takes the camera from its options).
+ * Errors as pico_scene_load_json. */
+int  pico_scene_load_obj(const char* path, pico_scene** out, char* err, size_t err_len);
 /* The named cameras of a loaded scene, in name order (none for procedural
  * scenes); pico_scene_default_camera gives the first one with its own aspect
  * replaced by the argument. */

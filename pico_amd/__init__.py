@@ -117,6 +117,16 @@ class View(C.Structure):
                 ("d_sample_count", C.c_void_p), ("d_variance", C.c_void_p)]
 
 
+class ViewHost(C.Structure):
+    _fields_ = [("camera", Camera), ("params", RenderParams), ("pixels", C.c_void_p), ("sample_count", C.c_void_p),
+                ("variance", C.c_void_p)]
+
+
+# int (*)(void* user, const float* pixels, const uint32_t* sample_count, uint32_t w, uint32_t h, uint32_t samples_done)
+PassCallback = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32,
+                           C.c_uint32)
+
+
 def params(width, height, spp, max_depth, tile=16, frame_seed=0, view_id=0, max_variance=0.0, flags=0):
     return RenderParams(width, height, spp, max_depth, tile, tile, frame_seed, view_id, max_variance, flags)
 
@@ -161,6 +171,8 @@ def _declare(lib):
         "pico_hip_relmse_d": (i32, [vp, vp, vp, f64, u32, P(f64)]),
         "pico_hip_relmse_noalloc_d": (i32, [vp, vp, vp, f64, u32, vp]),
         "pico_hip_render_views_d": (i32, [vp, P(View), u32, u32, P(Stats)]),
+        "pico_hip_render_views": (i32, [vp, P(ViewHost), u32, u32, P(Stats)]),
+        "pico_hip_render_progressive": (i32, [vp, P(Camera), P(RenderParams), vp, vp, vp, PassCallback, vp, P(Stats)]),
         "pico_hip_set_tuning": (i32, [vp, P(Tuning)]),
         "pico_hip_get_tuning": (i32, [vp, P(Tuning)]),
         "pico_hip_render_views_denoise_d": (i32, [vp, P(View), u32, vp, P(vp), P(Stats)]),
@@ -190,6 +202,7 @@ def _declare(lib):
         "pico_scene_shapes": (i32, [u32, u32, P(vp)]),
         "pico_scene_desc_get": (i32, [vp, P(SceneDesc)]),
         "pico_scene_load_json": (i32, [C.c_char_p, P(vp), C.c_char_p, sz]),
+        "pico_scene_load_obj": (i32, [C.c_char_p, P(vp), C.c_char_p, sz]),
         "pico_scene_camera_count": (i32, [vp, P(u32)]),
         "pico_scene_camera_get": (i32, [vp, u32, C.c_char_p, sz, P(Camera)]),
         "pico_scene_default_camera": (i32, [vp, f32, P(Camera)]),
@@ -293,6 +306,17 @@ class Scene:
         h = C.c_void_p()
         err = C.create_string_buffer(1024)
         rc = lib().pico_scene_load_json(os.fsencode(path), C.byref(h), err, 1024)
+        if rc:
+            raise PicoError(rc, err.value.decode(errors="replace"))
+        return Scene(h.value)
+
+    @staticmethod
+    def load_obj(path):
+        """A model file (.obj) through the reference's import path: every mesh
+        an instance, Light or Dielectric by its material's emission."""
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().pico_scene_load_obj(os.fsencode(path), C.byref(h), err, 1024)
         if rc:
             raise PicoError(rc, err.value.decode(errors="replace"))
         return Scene(h.value)
@@ -504,6 +528,39 @@ class Context:
         if rc and rc != PICO_ERR_CANCELLED:
             self._check(rc)
         return pixels, counts, variance, st
+
+    def render_progressive(self, cam, p, on_pass, pixels=None, counts=None, variance=None):
+        """pico_hip_render with a snapshot after every chunk of samples:
+        on_pass(pixels [H*W, 3], counts [H*W], samples_done) -> truthy to stop."""
+        n = p.width * p.height
+        pixels = np.zeros((n, 3), np.float32) if pixels is None else pixels
+        counts = np.zeros(n, np.uint32) if counts is None else counts
+        variance = np.zeros((n, 3), np.float32) if variance is None else variance
+
+        def cb(_user, px, cnt, w, h, done):
+            a = np.ctypeslib.as_array(px, (w * h * 3,)).reshape(w * h, 3).copy()
+            b = np.ctypeslib.as_array(cnt, (w * h,)).copy()
+            return 1 if on_pass(a, b, done) else 0
+        fn = PassCallback(cb)
+        st = Stats()
+        rc = lib().pico_hip_render_progressive(self._h, C.byref(cam), C.byref(p), _ptr(pixels), _ptr(counts),
+                                               _ptr(variance), fn, None, C.byref(st))
+        if rc and rc != PICO_ERR_CANCELLED:
+            self._check(rc)
+        return pixels, counts, variance, st
+
+    def render_views(self, cams, ps):
+        """Host-buffer batched views: [(pixels, counts, variance)] per view."""
+        arr = (ViewHost * len(ps))()
+        outs = []
+        for i, (cam, p) in enumerate(zip(cams, ps)):
+            n = p.width * p.height
+            o = (np.zeros((n, 3), np.float32), np.zeros(n, np.uint32), np.zeros((n, 3), np.float32))
+            arr[i] = ViewHost(cam, p, o[0].ctypes.data, o[1].ctypes.data, o[2].ctypes.data)
+            outs.append(o)
+        st = (Stats * len(ps))()
+        self._check(lib().pico_hip_render_views(self._h, arr, len(ps), 0, st))
+        return outs, list(st)
 
     def render_d(self, cam, p, d_pixels, d_counts, d_variance):
         st = Stats()

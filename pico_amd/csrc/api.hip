@@ -237,6 +237,7 @@ void pico_hip_destroy(pico_hip_ctx* c) {
     free_scene(c);
     for (auto& kv : c->bufs)
         if (kv.second.p) hipFree(kv.second.p);
+    if (c->pinned) hipHostFree(c->pinned);
     for (auto& kv : c->timers)
         for (auto& e : kv.second.pending) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
     if (c->side) {

@@ -58,6 +58,9 @@ struct pico_hip_ctx {
     // timing
     bool timing = false;
     std::map<std::string, KernelTimes> timers;
+    // pinned host staging of the progressive render's per-pass snapshots
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
     // this device's copy of the static scratch of the allocation-free entry
     // point (resolved at create with the device current)
     void* static_scratch = nullptr;

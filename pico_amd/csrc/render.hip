@@ -701,6 +701,13 @@ struct ViewHooks {
     pico_cnn* cnn = nullptr;
     float* const* d_out = nullptr;
     const pico_view* views = nullptr;
+    // progressive display of a single view: after every chunk of samples the
+    // view's pixels and counts are copied to the host staging buffers and the
+    // callback is run (a non-zero return stops the render as a cancel does)
+    pico_pass_callback pass_cb = nullptr;
+    void* pass_user = nullptr;
+    float* h_pixels = nullptr;
+    uint32_t* h_counts = nullptr;
 };
 
 // Runs the chunked wavefront over `nv` views already described on the host.
@@ -852,6 +859,19 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         PICO_LAUNCH(c, "accumulate", k_accumulate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         PICO_LAUNCH(c, "compact", k_compact, grid_lin, kBlock, 0, dv, act_a, act_b, ctl);
         std::swap(act_a, act_b);
+        if (hooks && hooks->pass_cb) {
+            // a consistent snapshot for the caller between two chunks (the
+            // reference's viewer reads the live buffers while tiles write them)
+            const size_t npx = (size_t)hv[0].cam.w * hv[0].cam.h;
+            PICO_HIP(c, hipMemcpyAsync(hooks->h_pixels, hv[0].pixels, 12 * npx, hipMemcpyDeviceToHost, c->stream));
+            PICO_HIP(c, hipMemcpyAsync(hooks->h_counts, hv[0].counts, 4 * npx, hipMemcpyDeviceToHost, c->stream));
+            PICO_HIP(c, hipStreamSynchronize(c->stream));
+            if (hooks->pass_cb(hooks->pass_user, hooks->h_pixels, hooks->h_counts, hv[0].cam.w, hv[0].cam.h,
+                               std::min(off + S, hv[0].spp))) {
+                cancelled = true;
+                break;
+            }
+        }
         if (per_view_times) {
             hipEvent_t ce = nullptr;
             for (uint32_t v = 0; v < nv; ++v) {
@@ -1114,6 +1134,91 @@ int pico_hip_render_accum64_d(pico_hip_ctx* c, const pico_camera* cam, const pic
     v[0].sample_base = sample_base;
     v[0].sum64 = d_sum;
     return run_views(c, v, c->tuning, nullptr, false, stats, stats ? 1 : 0);
+}
+
+int pico_hip_render_progressive(pico_hip_ctx* c, const pico_camera* cam, const pico_render_params* p, float* pix,
+                                uint32_t* cnt, float* var, pico_pass_callback cb, void* user, pico_stats* stats) {
+    if (!c) return PICO_ERR_ARG;
+    int rc = check_params(c, cam, p);
+    if (rc) return rc;
+    if (!pix || !cnt || !var || !cb) return fail(c, PICO_ERR_ARG, "NULL buffer or callback");
+    if (p->flags & PICO_RENDER_COUNT) return fail(c, PICO_ERR_ARG, "no counting run with a pass callback");
+    const size_t n = (size_t)p->width * p->height;
+    char* b = (char*)scratch(c, "frame", n * 28 + 64);
+    if (!b) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    float* dp = (float*)b;
+    float* dv = dp + 3 * n;
+    uint32_t* dc = (uint32_t*)(dv + 3 * n);
+    if (c->pinned_bytes < n * 16) {
+        if (c->pinned) hipHostFree(c->pinned);
+        c->pinned = nullptr;
+        c->pinned_bytes = 0;
+        PICO_HIP(c, hipHostMalloc(&c->pinned, n * 16));
+        c->pinned_bytes = n * 16;
+    }
+    if (p->flags & PICO_RENDER_RESUME) {
+        PICO_HIP(c, hipMemcpyAsync(dp, pix, 12 * n, hipMemcpyHostToDevice, c->stream));
+        PICO_HIP(c, hipMemcpyAsync(dv, var, 12 * n, hipMemcpyHostToDevice, c->stream));
+        PICO_HIP(c, hipMemcpyAsync(dc, cnt, 4 * n, hipMemcpyHostToDevice, c->stream));
+    }
+    std::vector<DView> v(1, base_view(cam, p));
+    v[0].mode = MODE_ACCUM;
+    v[0].pixels = dp;
+    v[0].counts = dc;
+    v[0].variance = dv;
+    ViewHooks hk;
+    hk.pass_cb = cb;
+    hk.pass_user = user;
+    hk.h_pixels = (float*)c->pinned;
+    hk.h_counts = (uint32_t*)((char*)c->pinned + 12 * n);
+    rc = run_views(c, v, c->tuning, nullptr, false, stats, stats ? 1 : 0, &hk);
+    if (rc && rc != PICO_ERR_CANCELLED) return rc;
+    PICO_HIP(c, hipMemcpyAsync(pix, dp, 12 * n, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(var, dv, 12 * n, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(cnt, dc, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+int pico_hip_render_views(pico_hip_ctx* c, const pico_view_host* views, uint32_t n, uint32_t queue_paths,
+                          pico_stats* stats) {
+    if (!c) return PICO_ERR_ARG;
+    if (!views || n == 0 || n > kMaxViews) return fail(c, PICO_ERR_LIMIT, "between 1 and 64 views per launch");
+    std::vector<size_t> off(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const pico_render_params& p = views[i].params;
+        if (!views[i].pixels || !views[i].sample_count || !views[i].variance) return fail(c, PICO_ERR_ARG, "NULL buffer");
+        if ((uint64_t)p.width * p.height > (1ull << 28)) return fail(c, PICO_ERR_LIMIT, "view larger than 2^28 pixels");
+        off[i + 1] = off[i] + (size_t)p.width * p.height;
+    }
+    char* b = (char*)scratch(c, "views_host", off[n] * 28 + 256 * (size_t)n + 64);
+    if (!b) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    std::vector<pico_view> dv(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const size_t px = off[i + 1] - off[i];
+        char* vb = b + 28 * off[i] + 256 * (size_t)i;  // each view's block 4-byte aligned, apart
+        dv[i].camera = views[i].camera;
+        dv[i].params = views[i].params;
+        dv[i].d_pixels = (float*)vb;
+        dv[i].d_variance = (float*)(vb + 12 * px);
+        dv[i].d_sample_count = (uint32_t*)(vb + 24 * px);
+        if (views[i].params.flags & PICO_RENDER_RESUME) {
+            PICO_HIP(c, hipMemcpyAsync(dv[i].d_pixels, views[i].pixels, 12 * px, hipMemcpyHostToDevice, c->stream));
+            PICO_HIP(c, hipMemcpyAsync(dv[i].d_variance, views[i].variance, 12 * px, hipMemcpyHostToDevice, c->stream));
+            PICO_HIP(c, hipMemcpyAsync(dv[i].d_sample_count, views[i].sample_count, 4 * px, hipMemcpyHostToDevice,
+                                       c->stream));
+        }
+    }
+    int rc = pico_hip_render_views_d(c, dv.data(), n, queue_paths, stats);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        const size_t px = off[i + 1] - off[i];
+        PICO_HIP(c, hipMemcpyAsync(views[i].pixels, dv[i].d_pixels, 12 * px, hipMemcpyDeviceToHost, c->stream));
+        PICO_HIP(c, hipMemcpyAsync(views[i].variance, dv[i].d_variance, 12 * px, hipMemcpyDeviceToHost, c->stream));
+        PICO_HIP(c, hipMemcpyAsync(views[i].sample_count, dv[i].d_sample_count, 4 * px, hipMemcpyDeviceToHost, c->stream));
+    }
+    PICO_HIP(c, hipStreamSynchronize(c->stream));
+    return PICO_OK;
 }
 
 int pico_hip_render_views_d(pico_hip_ctx* c, const pico_view* views, uint32_t n, uint32_t queue_paths, pico_stats* stats) {

@@ -103,17 +103,51 @@ fs::path resolve(const fs::path& root, std::string rel) {
 struct MeshData {
     std::vector<float> pos, nrm;
     std::vector<uint32_t> idx;
+    std::string material;  // usemtl name ("" for the default material)
 };
 
-MeshData read_obj(const fs::path& path) {
-    const std::string text = read_file(path);
+struct Corner { int64_t v, n; };
+
+// Faces of one mesh, as corners (three per triangle) into the file's vertex
+// and normal lists, and the material the mesh uses.
+struct RawMesh {
+    std::vector<Corner> corners;
+    std::string material;
+};
+
+// An object of the file (`o`, or `g` with a new name): its meshes, one per
+// run of faces under one material.
+struct RawObject {
+    std::string name;
+    std::vector<RawMesh> meshes;
+};
+
+struct RawObj {
     std::vector<float> v, vn;
-    struct Corner { int64_t v, n; };
-    std::vector<Corner> corners;  // three per triangle
+    std::vector<RawObject> objects;  // in the order the file creates them
+    std::vector<std::string> mtllibs;
+};
+
+// Reads the file's vertices and faces.  Meshes are split as assimp's OBJ
+// importer splits them (the reference's loader): `o name` continues the
+// object of that name or starts one, `g name` with a name other than the
+// active group starts an object, `usemtl` starts a new mesh in the current
+// object once the current mesh has faces and the material changes, and faces
+// before any `o` or `g` go to a default object.  With first_only the reading
+// stops where the file's first mesh ends (a later `o`, `g` or `usemtl`).
+RawObj parse_obj(const fs::path& path, bool first_only) {
+    const std::string text = read_file(path);
+    RawObj r;
+    std::string material, active_group;
+    int cur = -1;  // current object
     bool started = false;
     size_t line_no = 0;
     auto bad = [&](const std::string& what) {
         throw LoadError{path.string() + ":" + std::to_string(line_no) + ": " + what};
+    };
+    auto new_object = [&](const std::string& name) {
+        r.objects.push_back({name, {RawMesh{{}, material}}});
+        cur = (int)r.objects.size() - 1;
     };
     const char* p = text.data();
     const char* const end = p + text.size();
@@ -132,16 +166,24 @@ MeshData read_obj(const fs::path& path) {
         auto num = [&](float& x) {
             while (q < eol && blank(*q)) ++q;
             if (q < eol && *q == '+') ++q;
-            const auto r = std::from_chars(q, eol, x);
-            if (r.ec != std::errc()) return false;
-            q = r.ptr;
+            const auto res = std::from_chars(q, eol, x);
+            if (res.ec != std::errc()) return false;
+            q = res.ptr;
             return true;
+        };
+        auto rest = [&]() {
+            while (q < eol && blank(*q)) ++q;
+            const char* e = eol;
+            while (e > q && blank(e[-1])) --e;
+            return std::string(q, (size_t)(e - q));
         };
         if (tag == "v" || tag == "vn") {
             float x = 0, y = 0, z = 0;
             if (!num(x) || !num(y) || !num(z)) bad(tag == "v" ? "malformed vertex" : "malformed normal");
-            (tag == "v" ? v : vn).insert((tag == "v" ? v : vn).end(), {x, y, z});
+            (tag == "v" ? r.v : r.vn).insert((tag == "v" ? r.v : r.vn).end(), {x, y, z});
         } else if (tag == "f") {
+            if (cur < 0) new_object("defaultobject");
+            std::vector<Corner>& corners = r.objects[cur].meshes.back().corners;
             Corner poly_first{}, prev{};
             int n_corner = 0;
             for (;;) {
@@ -152,12 +194,12 @@ MeshData read_obj(const fs::path& path) {
                 for (int part = 0; part < 3; ++part) {
                     if (q < eol && !blank(*q) && *q != '/') {
                         if (*q == '+') ++q;
-                        const auto r = std::from_chars(q, eol, ref[part]);
-                        if (r.ec != std::errc() || ref[part] == 0) {
+                        const auto res = std::from_chars(q, eol, ref[part]);
+                        if (res.ec != std::errc() || ref[part] == 0) {
                             while (q < eol && !blank(*q)) ++q;
                             bad("malformed face index '" + std::string(tk, q) + "'");
                         }
-                        q = r.ptr;
+                        q = res.ptr;
                     }
                     if (q < eol && *q == '/') ++q;
                     else break;
@@ -167,13 +209,13 @@ MeshData read_obj(const fs::path& path) {
                     bad("malformed face index '" + std::string(tk, q) + "'");
                 }
                 // 1-based, negative counts back from the last vertex read
-                auto fix = [&](int64_t r, size_t count) -> int64_t {
-                    if (r == 0) return -1;
-                    const int64_t i = r > 0 ? r - 1 : (int64_t)count + r;
+                auto fix = [&](int64_t x, size_t count) -> int64_t {
+                    if (x == 0) return -1;
+                    const int64_t i = x > 0 ? x - 1 : (int64_t)count + x;
                     if (i < 0 || i >= (int64_t)count) bad("face index out of range '" + std::string(tk, q) + "'");
                     return i;
                 };
-                const Corner c{fix(ref[0], v.size() / 3), fix(ref[2], vn.size() / 3)};
+                const Corner c{fix(ref[0], r.v.size() / 3), fix(ref[2], r.vn.size() / 3)};
                 if (c.v < 0) bad("face corner without a vertex '" + std::string(tk, q) + "'");
                 // a fan: (first, previous, this) from the third corner on
                 if (n_corner == 0) poly_first = c;
@@ -184,13 +226,41 @@ MeshData read_obj(const fs::path& path) {
             if (n_corner < 3) bad("face with fewer than 3 corners");
             started = true;
         } else if (tag == "o" || tag == "g" || tag == "usemtl") {
-            if (started) break;  // the file's first mesh ends here
+            if (first_only && started) break;  // the file's first mesh ends here
+            const std::string name = rest();
+            if (tag == "usemtl") {
+                if (cur >= 0 && !r.objects[cur].meshes.back().corners.empty() && name != material)
+                    r.objects[cur].meshes.push_back(RawMesh{{}, name});
+                else if (cur >= 0)
+                    r.objects[cur].meshes.back().material = name;
+                material = name;
+            } else if (tag == "o") {
+                int found = -1;
+                for (size_t i = 0; i < r.objects.size(); ++i)
+                    if (r.objects[i].name == name) found = (int)i;
+                if (found >= 0) cur = found;
+                else new_object(name);
+            } else if (name != active_group) {
+                active_group = name;
+                new_object(name);
+            }
+        } else if (tag == "mtllib") {
+            r.mtllibs.push_back(rest());
         }
-        // vt, s, mtllib, l, p, ...: not needed
+        // vt, s, l, p, ...: not needed
     }
-    if (corners.empty()) throw LoadError{path.string() + ": no faces"};
-    // flat normals for corners without one, then join equal (position, normal)
+    return r;
+}
+
+// One mesh of the file: flat normals for corners without one, then corners
+// with equal position and normal joined in order of first use
+// (Triangulate | JoinIdenticalVertices | GenNormals).
+MeshData build_mesh(const RawObj& r, const RawMesh& raw) {
+    const std::vector<float>& v = r.v;
+    const std::vector<float>& vn = r.vn;
+    const std::vector<Corner>& corners = raw.corners;
     MeshData m;
+    m.material = raw.material;
     struct Key {
         uint32_t b[6];
         bool operator==(const Key& o) const { return std::memcmp(b, o.b, sizeof(b)) == 0; }
@@ -234,6 +304,15 @@ MeshData read_obj(const fs::path& path) {
         }
     }
     return m;
+}
+
+// The file's first mesh (Scene::add_mesh keeps mMeshes[0], Scene.cpp:297-318).
+MeshData read_obj(const fs::path& path) {
+    const RawObj r = parse_obj(path, true);
+    for (const RawObject& o : r.objects)
+        for (const RawMesh& m : o.meshes)
+            if (!m.corners.empty()) return build_mesh(r, m);
+    throw LoadError{path.string() + ": no faces"};
 }
 
 // ---- scene ----------------------------------------------------------------
@@ -495,6 +574,86 @@ void load_into(pico_scene& sc, const fs::path& file) {
     sc.cam = sc.cameras.empty() ? pico_camera{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, 1.0f, 10.0f} : sc.cameras.front().second;
 }
 
+// ---- a model file through the reference's import path ----------------------
+// Scene::Scene(ThreadPool&, dir, aiScene*, options) (Source/Core/Scene.cpp:86-141)
+// with parse_node (Scene.cpp:722-783) and add_material (Scene.cpp:785-971) on
+// an OBJ file as assimp imports it (Source/main.cpp:53-59).  Every mesh of
+// the file becomes an instance with the identity transform (OBJ nodes have
+// none): a Light when its material emits, else a Dielectric with the
+// Beckmann_All lobe; lights are listed in mesh order (the reference lists them
+// in the order its loading tasks finish).  Materials come from the file's
+// mtllib libraries as the constant branch of add_material reads them: Kd, Ks
+// and Ke into ConstantDiffuseSpecularMaterial with "gloss" = the glTF
+// roughness factor, which OBJ materials do not carry, so 1, and the material
+// roughness 1 - 1^2 = 0 (Source/Render/BasicMaterials.cpp:40-46); OBJ sets no
+// transparency factor, so no material is transparent.  A mesh without a
+// material, or naming one no library defines, gets assimp's default material
+// (diffuse 0.6).  The sky is black and there is no sun unless the options
+// set them (pico_scene_set_sky_image, pico_scene_set_sun).
+struct ObjMaterial {
+    float kd[3] = {0.6f, 0.6f, 0.6f}, ks[3] = {0, 0, 0}, ke[3] = {0, 0, 0};
+};
+
+void read_mtl(const fs::path& path, std::map<std::string, ObjMaterial>& out) {
+    const std::string text = read_file(path);
+    std::istringstream in(text);
+    std::string line, cur;
+    size_t line_no = 0;
+    while (std::getline(in, line)) {
+        ++line_no;
+        std::istringstream ls(line);
+        std::string tag;
+        if (!(ls >> tag) || tag[0] == '#') continue;
+        if (tag == "newmtl") {
+            std::getline(ls >> std::ws, cur);
+            while (!cur.empty() && (cur.back() == '\r' || cur.back() == ' ' || cur.back() == '\t')) cur.pop_back();
+            out[cur] = ObjMaterial{};
+        } else if (tag == "Kd" || tag == "Ks" || tag == "Ke") {
+            if (cur.empty() && !out.count(cur)) out[cur] = ObjMaterial{};
+            ObjMaterial& m = out[cur];
+            float* dst = tag == "Kd" ? m.kd : tag == "Ks" ? m.ks : m.ke;
+            float c[3];
+            if (!(ls >> c[0])) throw LoadError{path.string() + ":" + std::to_string(line_no) + ": malformed " + tag};
+            // one value sets all three channels
+            if (!(ls >> c[1] >> c[2])) c[1] = c[2] = c[0];
+            std::copy(c, c + 3, dst);
+        }
+        // Ns, Ni, d, Tr, illum, map_*: no effect on the reference's constant materials
+    }
+}
+
+void load_obj_scene(pico_scene& sc, const fs::path& file) {
+    const RawObj r = parse_obj(file, false);
+    const fs::path root = file.parent_path();
+    std::map<std::string, ObjMaterial> lib;
+    for (const std::string& m : r.mtllibs) read_mtl(resolve(root, m), lib);
+    std::map<std::string, uint32_t> mat_id;
+    for (const RawObject& o : r.objects)
+        for (const RawMesh& raw : o.meshes) {
+            if (raw.corners.empty()) continue;
+            const MeshData m = build_mesh(r, raw);
+            auto it = mat_id.find(raw.material);
+            if (it == mat_id.end()) {
+                const auto f = lib.find(raw.material);
+                const ObjMaterial om = f == lib.end() ? ObjMaterial{} : f->second;
+                pico_material pm{};
+                std::copy(om.kd, om.kd + 3, pm.diffuse);
+                std::copy(om.ks, om.ks + 3, pm.specular);
+                pm.roughness = 1.0f - 1.0f * 1.0f;
+                std::copy(om.ke, om.ke + 3, pm.emissive);
+                it = mat_id.emplace(raw.material, sc.add_material(pm)).first;
+            }
+            const pico_material& pm = sc.mats[it->second];
+            const bool light = pm.emissive[0] > 0.0f || pm.emissive[1] > 0.0f || pm.emissive[2] > 0.0f;
+            const uint32_t id = sc.add_mesh(m.pos.data(), m.nrm.data(), m.pos.size() / 3, m.idx.data(), m.idx.size());
+            const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+            sc.add_instance(id, I, it->second, light ? PICO_BSDF_LIGHT : PICO_BSDF_DIELECTRIC);
+        }
+    if (sc.imesh.empty()) throw LoadError{file.string() + ": no faces"};
+    sc.microfacet = PICO_MICROFACET_BECKMANN_ALL;
+    sc.cam = pico_camera{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, 1.0f, 10.0f};
+}
+
 void set_err(char* err, size_t n, const std::string& msg) {
     if (!err || n == 0) return;
     const size_t k = std::min(n - 1, msg.size());
@@ -516,6 +675,29 @@ int pico_scene_load_json(const char* path, pico_scene** out, char* err, size_t e
     auto* s = new pico_scene();
     try {
         load_into(*s, fs::path(path));
+    } catch (const LoadError& e) {
+        delete s;
+        set_err(err, err_len, e.msg);
+        return PICO_ERR_ARG;
+    } catch (const std::exception& e) {
+        delete s;
+        set_err(err, err_len, std::string("scene load failed: ") + e.what());
+        return PICO_ERR_ARG;
+    }
+    *out = s;
+    return PICO_OK;
+}
+
+int pico_scene_load_obj(const char* path, pico_scene** out, char* err, size_t err_len) {
+    set_err(err, err_len, "");
+    if (!path || !out) {
+        set_err(err, err_len, "NULL argument");
+        return PICO_ERR_ARG;
+    }
+    *out = nullptr;
+    auto* s = new pico_scene();
+    try {
+        load_obj_scene(*s, fs::path(path));
     } catch (const LoadError& e) {
         delete s;
         set_err(err, err_len, e.msg);

@@ -1,5 +1,6 @@
 // Drives the C++ front the way Pico's main() drives Scene::render_scene_to_file
-// (Source/main.cpp:44-97): a JSON scene file loaded with its named camera (no
+// (Source/main.cpp:44-97): an OBJ model file through the import path and a
+// JSON scene file loaded with its named camera (no
 // device needed), one Cornell-box frame with denoising and tone mapping, the
 // a-trous entry point on its own, a cancelled render, a Radiance file, and the
 // scene file rendered with its camera.  Exit status 0 on success; without a
@@ -40,6 +41,23 @@ static void write_scene_file() {
 }
 
 int main() {
+    // a model file through the import path (no device needed): two meshes,
+    // the emitting one a light
+    {
+        FILE* m = std::fopen("front_check_room.mtl", "w");
+        FILE* o = std::fopen("front_check_room.obj", "w");
+        if (!m || !o) return 23;
+        std::fputs("newmtl Grey\nKd 0.5 0.5 0.5\nnewmtl Lamp\nKe 4 4 4\n", m);
+        std::fputs("mtllib front_check_room.mtl\nv -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\n"
+                   "o floor\nusemtl Grey\nf 1 2 3 4\no lamp\nusemtl Lamp\nf 4 3 2 1\n", o);
+        std::fclose(m);
+        std::fclose(o);
+        pico_front::SceneFile model("front_check_room.obj");
+        std::remove("front_check_room.mtl");
+        std::remove("front_check_room.obj");
+        if (model.status() != PICO_OK) { std::printf("model file %d %s\n", model.status(), model.error()); return 24; }
+        if (model.desc().n_instances != 2 || model.desc().n_lights != 1 || model.get_camera("main")) return 25;
+    }
     // the scene file loads without a device
     write_scene_file();
     pico_front::SceneFile sf("front_check_scene.json");

@@ -47,8 +47,12 @@ struct RenderParams {
 // path) loads it, with its named cameras as Scene::get_camera gives them.
 class SceneFile {
 public:
+    // as Pico's main() chooses (Source/main.cpp:45-60): a .json scene file,
+    // or any other file as a model through the import path
     explicit SceneFile(const std::string& path) {
-        status_ = pico_scene_load_json(path.c_str(), &scene_, error_, sizeof error_);
+        const bool json = path.size() >= 5 && path.compare(path.size() - 5, 5, ".json") == 0;
+        status_ = json ? pico_scene_load_json(path.c_str(), &scene_, error_, sizeof error_)
+                       : pico_scene_load_obj(path.c_str(), &scene_, error_, sizeof error_);
         if (status_ != PICO_OK) return;
         status_ = pico_scene_desc_get(scene_, &desc_);
         uint32_t n = 0;

@@ -116,3 +116,43 @@ def test_shortest_first_finishes_cheap_views_first(ctx, scene, restore_tuning):
     work = [w * h * spp for (w, h, spp, _) in SPECS]
     cheap, dear = int(np.argmin(work)), int(np.argmax(work))
     assert stats[cheap].render_ms < stats[dear].render_ms
+
+
+def test_host_buffer_views_equal_device_views(ctx, scene):
+    """pico_hip_render_views (host buffers) gives what the device-buffer
+    launch gives, view by view."""
+    ctx.upload_scene(scene)
+    specs = SPECS[:3]
+    views, bufs = _views(ctx, scene, specs, seed=8)
+    ctx.render_views_d(views)
+    dev = _read(bufs, specs)
+    outs, st = ctx.render_views([v.camera for v in views], [v.params for v in views])
+    for (dp, dc, dvar), (hp, hc, hv) in zip(dev, outs):
+        assert np.array_equal(dc, hc)
+        assert np.array_equal(dp, hp.reshape(-1))
+        assert np.array_equal(dvar, hv.reshape(-1))
+    assert all(s.samples > 0 for s in st)
+
+
+def test_progressive_snapshots(ctx, scene):
+    """The pass callback sees growing sample counts after every chunk, the
+    last snapshot is the final image, and a non-zero return stops the render
+    with the samples so far kept (as a cancel)."""
+    ctx.upload_scene(scene)
+    w, h = 48, 40
+    cam = scene.camera(w / h)
+    p = pa.params(w, h, 24, 5, frame_seed=6)
+    old = ctx.get_tuning()
+    ctx.set_tuning(w * h * 4, old[1:3], old[3])  # four samples per chunk after the first
+    try:
+        seen = []
+        pix, cnt, _, st = ctx.render_progressive(cam, p, lambda a, b, done: seen.append((a, b, done)) and False)
+        ref, rcnt, _, _ = ctx.render(cam, p)
+        assert len(seen) >= 3 and [s[2] for s in seen] == sorted(s[2] for s in seen) and seen[-1][2] == 24
+        assert all((b1 >= b0).all() for (_, b0, _), (_, b1, _) in zip(seen, seen[1:]))
+        assert np.array_equal(seen[-1][0], pix) and np.array_equal(pix, ref) and np.array_equal(cnt, rcnt)
+        calls = []
+        pix2, cnt2, _, st2 = ctx.render_progressive(cam, p, lambda a, b, done: calls.append(done) or len(calls) >= 2)
+        assert st2.cancelled == 1 and len(calls) == 2 and cnt2.max() == calls[-1] < 24
+    finally:
+        ctx.set_tuning(old[0], old[1:3], old[3])
