@@ -567,6 +567,89 @@ __global__ void k_kpn_apply(const float* __restrict__ logits, const float* __res
 }
 
 // relMSE loss (FP64 reduction) and its gradient with respect to the output.
+// Inference tail: the 1x1 head (32 -> 25 logits, v_mfma_f32_16x16x32_bf16)
+// with the softmax and the 5x5 kernel apply after it, so the logits never
+// reach memory.  A wave takes 64 pixels of a row (4 MFMA tiles of 16); it
+// puts their logits in LDS and stages the radiance its 5x5 windows read
+// (rows y - 2 .. y + 2, columns x0 - 2 .. x0 + 65, clamped to the image),
+// then each lane finishes one pixel as k_kpn_apply does, from LDS.
+constexpr int kHeadLd = 33;  // logits per pixel in LDS (32 + 1 against bank conflicts)
+__global__ __launch_bounds__(256) void k_head_kpn(const bf16* __restrict__ feat, const bf16* __restrict__ w,
+                                                  const float* __restrict__ bias, const float* __restrict__ in,
+                                                  float* __restrict__ out, int B, int H, int W, int Hp, int Wp) {
+    __shared__ float lg[4][64 * kHeadLd];
+    __shared__ float rs[4][3][5][68];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mi = lane & 15, kq = lane >> 4;
+    const int runs = (Wp + 255) / 256;
+    const int row = blockIdx.x / runs;  // b * Hp + y
+    const int x0 = (blockIdx.x - row * runs) * 256 + wave * 64;
+    const int b = row / Hp, y = row - b * Hp;
+    if (x0 >= Wp) return;  // no barriers below: each wave works alone
+    const float* rad = in + (size_t)b * 10 * H * W;
+    const size_t plane = (size_t)H * W;
+    // radiance window of the wave (row y of the padded grid repeats the
+    // image's last row, as the network's padding does)
+    float* rw = &rs[wave][0][0][0];
+    for (int e = lane; e < 3 * 5 * 68; e += 64) {
+        const int c = e / (5 * 68), rem = e - c * 5 * 68, r = rem / 68, k = rem - r * 68;
+        const int yy = min(max(min(y, H - 1) + r - 2, 0), H - 1), xx = min(max(x0 + k - 2, 0), W - 1);
+        rw[e] = rad[c * plane + (size_t)yy * W + xx];
+    }
+    uint4 bv[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) bv[t] = *reinterpret_cast<const uint4*>(w + (t * 16 + mi) * 32 + 8 * kq);
+    float* lw = lg[wave];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int xa = x0 + m * 16 + mi;  // this lane's A row
+        uint4 av = make_uint4(0, 0, 0, 0);
+        if (xa < Wp) av = *reinterpret_cast<const uint4*>(feat + ((size_t)row * Wp + xa) * 32 + 8 * kq);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            f4v l = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+            l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv[t]), l,
+                                                        0, 0, 0);
+            const float bs = bias[t * 16 + mi];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) lw[(m * 16 + 4 * kq + r) * kHeadLd + t * 16 + mi] = l[r] + bs;
+        }
+    }
+    // the wave's lanes exchange the staged values through LDS
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // one pixel per lane: softmax over its 25 logits, then the clamped 5x5
+    // window of the radiance (k_kpn_apply's arithmetic)
+    const int x = x0 + lane;
+    if (x >= W || y >= H) return;
+    const float* g = lw + lane * kHeadLd;
+    float gl[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) gl[k] = g[k];
+    float mx = gl[0];
+#pragma unroll
+    for (int k = 1; k < 25; ++k) mx = fmaxf(mx, gl[k]);
+    float e[25], sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 25; ++k) { e[k] = expf(gl[k] - mx); sum += e[k]; }
+    const float is = 1.0f / sum;
+    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+        const float wgt = e[k] * is;
+        const int ry = k / 5;  // window row (y + ry - 2, clamped when staged)
+        const int xx = min(max(x + k % 5 - 2, 0), W - 1) - x0 + 2;
+        r0 += wgt * rw[(0 * 5 + ry) * 68 + xx];
+        r1 += wgt * rw[(1 * 5 + ry) * 68 + xx];
+        r2 += wgt * rw[(2 * 5 + ry) * 68 + xx];
+    }
+    const size_t p = (size_t)(b * H + y) * W + x;
+    out[3 * p] = r0;
+    out[3 * p + 1] = r1;
+    out[3 * p + 2] = r2;
+}
+
 __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ tgt, float* __restrict__ dout, size_t n3,
                        double inv_count, double* loss) {
     double acc = 0.0;
@@ -805,9 +888,16 @@ int forward(pico_cnn* n, const float* d_in, int B, int H, int W, bool keep_wk) {
         return rc;
     }
     if ((rc = conv(n, 9, n->d0a, 32, 0, n->d0b, 32, 0, B, Hp, Wp, true))) return rc;
+    if (!keep_wk) {
+        // inference: head, softmax and kernel apply in one pass
+        const unsigned grid = (unsigned)(B * Hp * ((Wp + 255) / 256));
+        PICO_LAUNCH(n->c, "cnn_conv_fwd", k_head_kpn, grid, 256, 0, n->d0b, n->wf[10], n->bpad[10], d_in, n->out, B, H, W,
+                    Hp, Wp);
+        return PICO_OK;
+    }
     if ((rc = conv(n, 10, n->d0b, 32, 0, nullptr, 32, 0, B, Hp, Wp, false, n->head))) return rc;
     PICO_LAUNCH(n->c, "cnn_kpn_apply", k_kpn_apply, grid_for((size_t)B * H * W), 256, 0, n->head, d_in,
-                keep_wk ? n->wk : nullptr, n->out, B, H, W, Hp, Wp);
+                n->wk, n->out, B, H, W, Hp, Wp);
     return PICO_OK;
 }
 
