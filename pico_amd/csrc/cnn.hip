@@ -63,6 +63,11 @@ struct ConvArgs {
     // upsampling (y / 2, x / 2): the upsample + concatenation fused into the
     // convolution's input staging
     const bf16* up; int up_stride, up_ch;
+    // optional (the last 3x3 layer at inference): the 1x1 head, softmax and
+    // 5x5 kernel apply on this layer's outputs, writing the filtered radiance
+    // (rgb, [B][img_h][img_w][3]) from the planar network input (rad) instead
+    // of the layer's activations
+    const bf16* head_w; const float* head_b; const float* rad; float* rgb; int img_h, img_w;
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -186,7 +191,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 // on MFMA, so a chunk costs two barriers and the loads' latency is hidden
 // behind the arithmetic of the other workgroups and chunks.  Plain bf16
 // outputs go through LDS and leave as 16-byte stores of 8 channels.
-template <int MT, int NT, int KS>
+template <int MT, int NT, int KS, bool HEAD = false>
 __global__ __launch_bounds__(256) void k_conv3r(ConvArgs a) {
     extern __shared__ bf16 lds[];
     constexpr int kRun = MT * 16;       // pixels of one output row
@@ -315,6 +320,69 @@ __global__ __launch_bounds__(256) void k_conv3r(ConvArgs a) {
             }
         }
         __syncthreads();
+        if constexpr (HEAD) {
+            // the head on this wave's tile: A fragments of 16 pixels x 32
+            // channels straight from the tile, logits to LDS (the staging
+            // area is free once every wave holds its fragments), then one
+            // pixel per lane: softmax of its 25 logits and the clamped 5x5
+            // window of the input radiance (k_kpn_apply's arithmetic)
+            static_assert(NT == 2 && KS == 32 && MT == 4, "head after a 32-channel layer, 64 pixels per wave");
+            uint4 hv[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                hv[m] = live ? *reinterpret_cast<const uint4*>(ot + (m * 16 + mi) * ldo + 8 * kq) : make_uint4(0, 0, 0, 0);
+            __syncthreads();
+            if (!live) return;
+            float* lgt = reinterpret_cast<float*>(lds) + wave * kRun * 25;
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const uint4 bw = *reinterpret_cast<const uint4*>(a.head_w + (t * 16 + mi) * 32 + 8 * kq);
+                const int c = t * 16 + mi;
+                const float hb = a.head_b[c];
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    f4v l = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+                    l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, hv[m]),
+                                                                __builtin_bit_cast(bf16x8, bw), l, 0, 0, 0);
+                    if (c < 25)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) lgt[(m * 16 + 4 * kq + r) * 25 + c] = l[r] + hb;
+                }
+            }
+            // the wave's lanes exchange the logits through LDS
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int x = x0 + lane;
+            if (x >= a.img_w || y >= a.img_h) return;
+            float gl[25];
+#pragma unroll
+            for (int k = 0; k < 25; ++k) gl[k] = lgt[lane * 25 + k];
+            float mx = gl[0];
+#pragma unroll
+            for (int k = 1; k < 25; ++k) mx = fmaxf(mx, gl[k]);
+            float sum = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 25; ++k) { gl[k] = expf(gl[k] - mx); sum += gl[k]; }
+            const float is = 1.0f / sum;
+            const float* rad = a.rad + (size_t)b * 10 * a.img_h * a.img_w;
+            const size_t plane = (size_t)a.img_h * a.img_w;
+            float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                const float wgt = gl[k] * is;
+                const int yy = min(max(y + k / 5 - 2, 0), a.img_h - 1), xx = min(max(x + k % 5 - 2, 0), a.img_w - 1);
+                const size_t q = (size_t)yy * a.img_w + xx;
+                r0 += wgt * rad[q];
+                r1 += wgt * rad[plane + q];
+                r2 += wgt * rad[2 * plane + q];
+            }
+            const size_t p = (size_t)(b * a.img_h + y) * a.img_w + x;
+            a.rgb[3 * p] = r0;
+            a.rgb[3 * p + 1] = r1;
+            a.rgb[3 * p + 2] = r2;
+            return;
+        }
         if (!live) return;
         constexpr int kOutVec = kRun * kCh / 8;
 #pragma unroll
@@ -567,89 +635,6 @@ __global__ void k_kpn_apply(const float* __restrict__ logits, const float* __res
 }
 
 // relMSE loss (FP64 reduction) and its gradient with respect to the output.
-// Inference tail: the 1x1 head (32 -> 25 logits, v_mfma_f32_16x16x32_bf16)
-// with the softmax and the 5x5 kernel apply after it, so the logits never
-// reach memory.  A wave takes 64 pixels of a row (4 MFMA tiles of 16); it
-// puts their logits in LDS and stages the radiance its 5x5 windows read
-// (rows y - 2 .. y + 2, columns x0 - 2 .. x0 + 65, clamped to the image),
-// then each lane finishes one pixel as k_kpn_apply does, from LDS.
-constexpr int kHeadLd = 33;  // logits per pixel in LDS (32 + 1 against bank conflicts)
-__global__ __launch_bounds__(256) void k_head_kpn(const bf16* __restrict__ feat, const bf16* __restrict__ w,
-                                                  const float* __restrict__ bias, const float* __restrict__ in,
-                                                  float* __restrict__ out, int B, int H, int W, int Hp, int Wp) {
-    __shared__ float lg[4][64 * kHeadLd];
-    __shared__ float rs[4][3][5][68];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int mi = lane & 15, kq = lane >> 4;
-    const int runs = (Wp + 255) / 256;
-    const int row = blockIdx.x / runs;  // b * Hp + y
-    const int x0 = (blockIdx.x - row * runs) * 256 + wave * 64;
-    const int b = row / Hp, y = row - b * Hp;
-    if (x0 >= Wp) return;  // no barriers below: each wave works alone
-    const float* rad = in + (size_t)b * 10 * H * W;
-    const size_t plane = (size_t)H * W;
-    // radiance window of the wave (row y of the padded grid repeats the
-    // image's last row, as the network's padding does)
-    float* rw = &rs[wave][0][0][0];
-    for (int e = lane; e < 3 * 5 * 68; e += 64) {
-        const int c = e / (5 * 68), rem = e - c * 5 * 68, r = rem / 68, k = rem - r * 68;
-        const int yy = min(max(min(y, H - 1) + r - 2, 0), H - 1), xx = min(max(x0 + k - 2, 0), W - 1);
-        rw[e] = rad[c * plane + (size_t)yy * W + xx];
-    }
-    uint4 bv[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) bv[t] = *reinterpret_cast<const uint4*>(w + (t * 16 + mi) * 32 + 8 * kq);
-    float* lw = lg[wave];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const int xa = x0 + m * 16 + mi;  // this lane's A row
-        uint4 av = make_uint4(0, 0, 0, 0);
-        if (xa < Wp) av = *reinterpret_cast<const uint4*>(feat + ((size_t)row * Wp + xa) * 32 + 8 * kq);
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            f4v l = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-            l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv[t]), l,
-                                                        0, 0, 0);
-            const float bs = bias[t * 16 + mi];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) lw[(m * 16 + 4 * kq + r) * kHeadLd + t * 16 + mi] = l[r] + bs;
-        }
-    }
-    // the wave's lanes exchange the staged values through LDS
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // one pixel per lane: softmax over its 25 logits, then the clamped 5x5
-    // window of the radiance (k_kpn_apply's arithmetic)
-    const int x = x0 + lane;
-    if (x >= W || y >= H) return;
-    const float* g = lw + lane * kHeadLd;
-    float gl[25];
-#pragma unroll
-    for (int k = 0; k < 25; ++k) gl[k] = g[k];
-    float mx = gl[0];
-#pragma unroll
-    for (int k = 1; k < 25; ++k) mx = fmaxf(mx, gl[k]);
-    float e[25], sum = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 25; ++k) { e[k] = expf(gl[k] - mx); sum += e[k]; }
-    const float is = 1.0f / sum;
-    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 25; ++k) {
-        const float wgt = e[k] * is;
-        const int ry = k / 5;  // window row (y + ry - 2, clamped when staged)
-        const int xx = min(max(x + k % 5 - 2, 0), W - 1) - x0 + 2;
-        r0 += wgt * rw[(0 * 5 + ry) * 68 + xx];
-        r1 += wgt * rw[(1 * 5 + ry) * 68 + xx];
-        r2 += wgt * rw[(2 * 5 + ry) * 68 + xx];
-    }
-    const size_t p = (size_t)(b * H + y) * W + x;
-    out[3 * p] = r0;
-    out[3 * p + 1] = r1;
-    out[3 * p + 2] = r2;
-}
-
 __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ tgt, float* __restrict__ dout, size_t n3,
                        double inv_count, double* loss) {
     double acc = 0.0;
@@ -792,6 +777,16 @@ int launch_conv_k(pico_cnn* n, const char* name, const ConvArgs& a) {
         const dim3 grid4((unsigned)(a.B * ((a.H + 3) / 4) * ((a.W + run4 - 1) / run4)), (unsigned)(a.npad / (nt * 16)));
         const size_t lds3 = std::max(6 * (size_t)(run4 + 2) * KS + 3 * (size_t)nt * 16 * (KS + 8),
                                      4 * (size_t)run4 * (nt * 16 + 8)) * sizeof(bf16);
+        if (a.head_w) {
+            if constexpr (KS == 32) {
+                if (nt != 2 || a.mask || a.out_f32) return fail(n->c, PICO_ERR_STATE, "head fusion needs 32 channels");
+                // logits of the four waves (64 x 25 floats each) share the area
+                const size_t ldsh = std::max(lds3, 4 * (size_t)run4 * 25 * sizeof(float));
+                PICO_LAUNCH(n->c, name, (k_conv3r<4, 2, 32, true>), grid4, 256, ldsh, a);
+                return PICO_OK;
+            }
+            return fail(n->c, PICO_ERR_STATE, "head fusion needs 32 input channels");
+        }
         if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv3r<2, 4, KS>), grid4, 256, lds3, a);
         else PICO_LAUNCH(n->c, name, (k_conv3r<4, 2, KS>), grid4, 256, lds3, a);
         return PICO_OK;
@@ -887,14 +882,17 @@ int forward(pico_cnn* n, const float* d_in, int B, int H, int W, bool keep_wk) {
     } else if ((rc = conv(n, 8, n->c0, 96, 0, n->d0a, 32, 0, B, Hp, Wp, true, nullptr, n->d1b, 64, 64))) {
         return rc;
     }
-    if ((rc = conv(n, 9, n->d0a, 32, 0, n->d0b, 32, 0, B, Hp, Wp, true))) return rc;
     if (!keep_wk) {
-        // inference: head, softmax and kernel apply in one pass
-        const unsigned grid = (unsigned)(B * Hp * ((Wp + 255) / 256));
-        PICO_LAUNCH(n->c, "cnn_conv_fwd", k_head_kpn, grid, 256, 0, n->d0b, n->wf[10], n->bpad[10], d_in, n->out, B, H, W,
-                    Hp, Wp);
-        return PICO_OK;
+        // inference: the last 3x3 layer with the head, softmax and kernel
+        // apply on its output tile, so neither its activations nor the
+        // logits reach memory
+        const LayerDef& L = kL[9];
+        ConvArgs a{n->d0a, 32, 0, n->wf[9], n->bpad[9], n->d0b, 32, 0, nullptr, nullptr, 0, 0,
+                   B, Hp, Wp, L.kpad, L.npad, L.taps, 0, 1, nullptr, 0, 0,
+                   n->wf[10], n->bpad[10], d_in, n->out, H, W};
+        return launch_conv(n, "cnn_conv_fwd", a);
     }
+    if ((rc = conv(n, 9, n->d0a, 32, 0, n->d0b, 32, 0, B, Hp, Wp, true))) return rc;
     if ((rc = conv(n, 10, n->d0b, 32, 0, nullptr, 32, 0, B, Hp, Wp, false, n->head))) return rc;
     PICO_LAUNCH(n->c, "cnn_kpn_apply", k_kpn_apply, grid_for((size_t)B * H * W), 256, 0, n->head, d_in,
                 n->wk, n->out, B, H, W, Hp, Wp);
