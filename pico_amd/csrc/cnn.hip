@@ -48,6 +48,14 @@ __host__ __device__ inline bf16 f2bf(float f) {
 }
 __device__ inline float bf2f(bf16 b) { return __uint_as_float((uint32_t)b << 16); }
 
+// input channel k of the network from its planar value: log1p of the
+// (non-negative) radiance and of the distance, the guides as given
+__device__ inline float in_feature(float f, int k) {
+    if (k < 3) f = log1pf(f > 0.0f ? f : 0.0f);
+    if (k == 9) f = log1pf(f);
+    return f;
+}
+
 struct Tensor { bf16* p; int stride, off; };
 
 struct ConvArgs {
@@ -68,6 +76,9 @@ struct ConvArgs {
     // (rgb, [B][img_h][img_w][3]) from the planar network input (rad) instead
     // of the layer's activations
     const bf16* head_w; const float* head_b; const float* rad; float* rgb; int img_h, img_w;
+    // optional: the 2x2 average pool of this layer's bf16 outputs, written to
+    // `pool` (half resolution, stride pool_stride) from the output tile
+    bf16* pool; int pool_stride;
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -320,6 +331,33 @@ __global__ __launch_bounds__(256) void k_conv3r(ConvArgs a) {
             }
         }
         __syncthreads();
+        if (a.pool) {
+            // 2x2 average pool of the four rows' tiles (rows 2pr, 2pr + 1 of
+            // the workgroup make pooled row pr): the four bf16 outputs summed
+            // in FP32 in (y, x) order, times 1/4, rounded once
+            constexpr int kC8 = kCh / 8, kHalf = kRun / 2;
+            const int Hq = a.H >> 1, Wq = a.W >> 1;
+            for (int e = threadIdx.x; e < 2 * kHalf * kC8; e += 256) {
+                const int c8 = e % kC8, rest = e / kC8;
+                const int px = rest % kHalf, pr = rest / kHalf;
+                const int yp = (y0 >> 1) + pr, xp = (x0 >> 1) + px;
+                if (yp >= Hq || xp >= Wq) continue;
+                const bf16* t0 = lds + (2 * pr * kRun + 2 * px) * ldo + 8 * c8;
+                const bf16* t1 = t0 + kRun * ldo;
+                uint4 v[4] = {*reinterpret_cast<const uint4*>(t0), *reinterpret_cast<const uint4*>(t0 + ldo),
+                              *reinterpret_cast<const uint4*>(t1), *reinterpret_cast<const uint4*>(t1 + ldo)};
+                bf16 o[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc += bf2f(reinterpret_cast<const bf16*>(&v[j])[k]);
+                    o[k] = f2bf(0.25f * acc);
+                }
+                *reinterpret_cast<uint4*>(a.pool + ((size_t)(b * Hq + yp) * Wq + xp) * a.pool_stride + n0 + 8 * c8) =
+                    *reinterpret_cast<const uint4*>(o);
+            }
+        }
         if constexpr (HEAD) {
             // the head on this wave's tile: A fragments of 16 pixels x 32
             // channels straight from the tile, logits to LDS (the staging
@@ -515,16 +553,41 @@ __global__ void k_pack(const float* __restrict__ in, bf16* __restrict__ x, int B
         const int y = min(rem / Wp, H - 1), xx = min(rem % Wp, W - 1);
         const float* src = in + (size_t)b * 10 * H * W + (size_t)y * W + xx;
         bf16 v[16];
-        for (int k = 0; k < 10; ++k) {
-            float f = src[(size_t)k * H * W];
-            if (k < 3) f = log1pf(f > 0.0f ? f : 0.0f);
-            if (k == 9) f = log1pf(f);
-            v[k] = f2bf(f);
-        }
+        for (int k = 0; k < 10; ++k) v[k] = f2bf(in_feature(src[(size_t)k * H * W], k));
         for (int k = 10; k < 16; ++k) v[k] = 0;
         uint4* dst = reinterpret_cast<uint4*>(x + (size_t)p * 16);
         dst[0] = *reinterpret_cast<uint4*>(&v[0]);
         dst[1] = *reinterpret_cast<uint4*>(&v[8]);
+    }
+}
+
+// G-buffer -> the first layer's packed input at the padded size (k_pack of
+// what k_inputs writes: the same values, transform and rounding) plus the
+// three radiance planes the kernel apply reads, in one pass
+__global__ void k_inputs_pack(const float* __restrict__ rgb, const float* __restrict__ nrm, const float* __restrict__ pos,
+                              const float* __restrict__ alb, float* __restrict__ rad, bf16* __restrict__ x, int H, int W,
+                              int Hp, int Wp) {
+    const int P = Hp * Wp;
+    const size_t plane = (size_t)H * W;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const int yp = p / Wp, xp = p - yp * Wp;
+        const size_t i = (size_t)min(yp, H - 1) * W + min(xp, W - 1);
+        float f[10];
+        for (int c = 0; c < 3; ++c) {
+            f[c] = rgb[3 * i + c];
+            f[3 + c] = alb[3 * i + c];
+            f[6 + c] = nrm[3 * i + c];
+        }
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        f[9] = sqrtf(px * px + py * py + pz * pz);
+        bf16 v[16];
+        for (int k = 0; k < 10; ++k) v[k] = f2bf(in_feature(f[k], k));
+        for (int k = 10; k < 16; ++k) v[k] = 0;
+        uint4* dst = reinterpret_cast<uint4*>(x + (size_t)p * 16);
+        dst[0] = *reinterpret_cast<uint4*>(&v[0]);
+        dst[1] = *reinterpret_cast<uint4*>(&v[8]);
+        if (yp < H && xp < W)
+            for (int c = 0; c < 3; ++c) rad[c * plane + i] = f[c];
     }
 }
 
@@ -539,28 +602,6 @@ __global__ void k_inputs(const float* __restrict__ rgb, const float* __restrict_
             for (int c = 0; c < 3; ++c) out[(3 * k + c) * px + i] = src[k][3 * i + c];
         const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
         out[9 * px + i] = sqrtf(x * x + y * y + z * z);
-    }
-}
-
-// 2x2 average pool: src level (H, W) slice -> dst (H/2, W/2) with C channels
-__global__ void k_pool(const bf16* __restrict__ s, int s_stride, int s_off, bf16* __restrict__ d, int C, int B, int H, int W) {
-    // one thread per (output pixel, 8 channels): 16-byte loads and stores
-    const uint32_t h = H / 2, w = W / 2, c8 = C / 8;
-    const uint32_t n = (uint32_t)B * h * w * c8;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t g = i % c8, p = i / c8;
-        const uint32_t x = p % w, q = p / w, y = q % h, b = q / h;
-        const bf16* s0 = s + ((size_t)(b * H + 2 * y) * W + 2 * x) * s_stride + s_off + 8 * g;
-        const uint4 v[4] = {*reinterpret_cast<const uint4*>(s0), *reinterpret_cast<const uint4*>(s0 + s_stride),
-                            *reinterpret_cast<const uint4*>(s0 + (size_t)W * s_stride),
-                            *reinterpret_cast<const uint4*>(s0 + (size_t)W * s_stride + s_stride)};
-        bf16 o[8];
-        for (int k = 0; k < 8; ++k) {
-            float acc = 0.0f;
-            for (int j = 0; j < 4; ++j) acc += bf2f(reinterpret_cast<const bf16*>(&v[j])[k]);
-            o[k] = f2bf(0.25f * acc);
-        }
-        *reinterpret_cast<uint4*>(d + (size_t)p * C + 8 * g) = *reinterpret_cast<const uint4*>(o);
     }
 }
 
@@ -769,6 +810,8 @@ int launch_conv_k(pico_cnn* n, const char* name, const ConvArgs& a) {
     const int nt = (a.npad % 64 == 0) ? 4 : (a.npad % 32 == 0) ? 2 : 1;
     const int run = 4 * (8 / nt) * 16;
     const dim3 grid((unsigned)(a.B * a.H * ((a.W + run - 1) / run)), (unsigned)(a.npad / (nt * 16)));
+    if (a.pool && !(a.taps == 9 && nt >= 2)) return fail(n->c, PICO_ERR_STATE, "fused pool needs the 3x3 kernel");
+    if (a.pool && (a.mask || a.out_f32 || a.head_w || ((a.H | a.W) & 1))) return fail(n->c, PICO_ERR_STATE, "fused pool needs bf16 outputs of even size");
     if (a.taps == 9 && nt >= 2) {
         // 3x3 with all operands in LDS, four output rows per workgroup: six
         // rows of MT * 16 + 2 pixels and one row of three taps' weights, per
@@ -804,10 +847,13 @@ int launch_conv(pico_cnn* n, const char* name, const ConvArgs& a) {
 }
 
 int conv(pico_cnn* n, int l, const bf16* in, int in_stride, int in_off, bf16* out, int out_stride, int out_off, int B,
-         int H, int W, bool relu, float* out_f32 = nullptr, const bf16* up = nullptr, int up_stride = 0, int up_ch = 0) {
+         int H, int W, bool relu, float* out_f32 = nullptr, const bf16* up = nullptr, int up_stride = 0, int up_ch = 0,
+         bf16* pool = nullptr, int pool_stride = 0) {
     const LayerDef& L = kL[l];
     ConvArgs a{in, in_stride, in_off, n->wf[l], n->bpad[l], out, out_stride, out_off, out_f32, nullptr, 0, 0,
                B, H, W, L.kpad, L.npad, L.taps, 0, relu ? 1 : 0, up, up_stride, up_ch};
+    a.pool = pool;
+    a.pool_stride = pool_stride;
     return launch_conv(n, "cnn_conv_fwd", a);
 }
 
@@ -850,19 +896,23 @@ int check_dims(pico_cnn* n, uint32_t B, uint32_t W, uint32_t H) {
     return PICO_OK;
 }
 
-// forward of a device-resident batch; the filtered radiance lands in n->out
-int forward(pico_cnn* n, const float* d_in, int B, int H, int W, bool keep_wk) {
+// Forward of a device-resident batch of B planar inputs; the filtered
+// radiance lands in n->out, or at inference in d_out when given.  prepacked:
+// the first layer's input (x0) is already written and d_in need only hold
+// the radiance planes.
+int forward(pico_cnn* n, const float* d_in, int B, int H, int W, bool keep_wk, bool prepacked = false,
+            float* d_out = nullptr) {
     int rc;
     const int Hp = (H + 3) / 4 * 4, Wp = (W + 3) / 4 * 4;
     const int H1 = Hp / 2, W1 = Wp / 2, H2 = Hp / 4, W2 = Wp / 4;
     const size_t P0 = (size_t)B * Hp * Wp;
-    PICO_LAUNCH(n->c, "cnn_pack", k_pack, grid_for(P0), 256, 0, d_in, n->x0, B, H, W, Hp, Wp);
+    if (!prepacked) PICO_LAUNCH(n->c, "cnn_pack", k_pack, grid_for(P0), 256, 0, d_in, n->x0, B, H, W, Hp, Wp);
     if ((rc = conv(n, 0, n->x0, 16, 0, n->e0a, 32, 0, B, Hp, Wp, true))) return rc;
-    if ((rc = conv(n, 1, n->e0a, 32, 0, n->c0, 96, 64, B, Hp, Wp, true))) return rc;  // e0b -> skip slice of c0
-    PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 4 * 4), 256, 0, n->c0, 96, 64, n->p0, 32, B, Hp, Wp);
+    // e0b -> skip slice of c0, and its 2x2 pool -> p0
+    if ((rc = conv(n, 1, n->e0a, 32, 0, n->c0, 96, 64, B, Hp, Wp, true, nullptr, nullptr, 0, 0, n->p0, 32))) return rc;
     if ((rc = conv(n, 2, n->p0, 32, 0, n->e1a, 64, 0, B, H1, W1, true))) return rc;
-    if ((rc = conv(n, 3, n->e1a, 64, 0, n->c1, 192, 128, B, H1, W1, true))) return rc;  // e1b -> skip slice of c1
-    PICO_LAUNCH(n->c, "cnn_pool", k_pool, grid_for(P0 / 16 * 8), 256, 0, n->c1, 192, 128, n->p1, 64, B, H1, W1);
+    // e1b -> skip slice of c1, and its 2x2 pool -> p1
+    if ((rc = conv(n, 3, n->e1a, 64, 0, n->c1, 192, 128, B, H1, W1, true, nullptr, nullptr, 0, 0, n->p1, 64))) return rc;
     if ((rc = conv(n, 4, n->p1, 64, 0, n->ba, 128, 0, B, H2, W2, true))) return rc;
     if ((rc = conv(n, 5, n->ba, 128, 0, n->bb, 128, 0, B, H2, W2, true))) return rc;
     // The decoder's inputs are [nearest upsample of the level below, skip]:
@@ -889,7 +939,7 @@ int forward(pico_cnn* n, const float* d_in, int B, int H, int W, bool keep_wk) {
         const LayerDef& L = kL[9];
         ConvArgs a{n->d0a, 32, 0, n->wf[9], n->bpad[9], n->d0b, 32, 0, nullptr, nullptr, 0, 0,
                    B, Hp, Wp, L.kpad, L.npad, L.taps, 0, 1, nullptr, 0, 0,
-                   n->wf[10], n->bpad[10], d_in, n->out, H, W};
+                   n->wf[10], n->bpad[10], d_in, d_out ? d_out : n->out, H, W};
         return launch_conv(n, "cnn_conv_fwd", a);
     }
     if ((rc = conv(n, 9, n->d0a, 32, 0, n->d0b, 32, 0, B, Hp, Wp, true))) return rc;
@@ -1056,8 +1106,12 @@ int pico_hip_cnn_denoise_d(pico_cnn* n, const float* d_in, uint32_t w, uint32_t 
     if (!d_in || !d_out) return fail(n->c, PICO_ERR_ARG, "NULL buffer");
     int rc = check_dims(n, 1, w, h);
     if (rc) return rc;
-    if ((rc = forward(n, d_in, 1, (int)h, (int)w, false))) return rc;
-    PICO_HIP(n->c, hipMemcpyAsync(d_out, n->out, (size_t)w * h * 12, hipMemcpyDeviceToDevice, n->c->stream));
+    // the output is written in place unless it overlaps the radiance planes
+    // the kernel apply reads
+    const size_t px = (size_t)w * h;
+    const bool overlap = (const char*)d_out < (const char*)(d_in + 3 * px) && (const char*)d_in < (const char*)(d_out + 3 * px);
+    if ((rc = forward(n, d_in, 1, (int)h, (int)w, false, false, overlap ? nullptr : d_out))) return rc;
+    if (overlap) PICO_HIP(n->c, hipMemcpyAsync(d_out, n->out, px * 12, hipMemcpyDeviceToDevice, n->c->stream));
     return PICO_OK;
 }
 
@@ -1090,10 +1144,19 @@ int pico_hip_cnn_denoise_gbuffer_d(pico_cnn* n, const float* d_rgb, const float*
     if (!n) return PICO_ERR_ARG;
     int rc = check_dims(n, 1, w, h);
     if (rc) return rc;
-    float* d = (float*)scratch(n->c, "cnn_frame_in", (size_t)w * h * 40 + 64);
+    if (!d_rgb || !d_normal || !d_position || !d_albedo || !d_out) return fail(n->c, PICO_ERR_ARG, "NULL buffer");
+    const size_t px = (size_t)w * h;
+    // the packed input straight from the G-buffer (the planar input's
+    // radiance planes for the kernel apply), then the network
+    float* d = (float*)scratch(n->c, "cnn_frame_in", px * 12 + 64);
     if (!d) return fail(n->c, PICO_ERR_NOMEM, "device allocation failed");
-    if ((rc = pico_hip_cnn_inputs_d(n->c, d_rgb, d_normal, d_position, d_albedo, w, h, d))) return rc;
-    return pico_hip_cnn_denoise_d(n, d, w, h, d_out);
+    const int Hp = ((int)h + 3) / 4 * 4, Wp = ((int)w + 3) / 4 * 4;
+    PICO_LAUNCH(n->c, "cnn_inputs_pack", k_inputs_pack, grid_for((size_t)Hp * Wp), 256, 0, d_rgb, d_normal, d_position,
+                d_albedo, d, n->x0, (int)h, (int)w, Hp, Wp);
+    const bool overlap = (const char*)d_out < (const char*)(d + 3 * px) && (const char*)d < (const char*)(d_out + 3 * px);
+    if ((rc = forward(n, d, 1, (int)h, (int)w, false, true, overlap ? nullptr : d_out))) return rc;
+    if (overlap) PICO_HIP(n->c, hipMemcpyAsync(d_out, n->out, px * 12, hipMemcpyDeviceToDevice, n->c->stream));
+    return PICO_OK;
 }
 
 int pico_hip_cnn_denoise_gbuffer(pico_cnn* n, const float* rgb, const float* normal, const float* position,
