@@ -97,7 +97,10 @@ def test_render_samples_match_oracle_pathwise(ctx, cornell):
     assert (rel < 1e-3).mean() >= 0.99, (rel < 1e-3).mean()
 
 
-def test_atrous_matches_oracle(ctx):
+@pytest.mark.parametrize("levels", [5, 7])
+def test_atrous_matches_oracle(ctx, levels):
+    # 5 levels: dilations 1..16 (the LDS-staged kernel); 7 levels adds
+    # dilations 32 and 64 (the direct kernel)
     rng = np.random.default_rng(5)
     w, h = 100, 75
     rgb = rng.gamma(2.0, 0.5, size=(w * h, 3)).astype(np.float32)
@@ -106,8 +109,8 @@ def test_atrous_matches_oracle(ctx):
     pos = rng.normal(size=(w * h, 3)).astype(np.float32) * 3
     alb = rng.uniform(0, 1, size=(w * h, 3)).astype(np.float32)
     for tile in [(16, 16), (64, 64), (7, 5)]:
-        g = ctx.atrous(rgb, n, pos, alb, w, h, tile, 5)
-        c = oracle.atrous(rgb, n, pos, alb, w, h, tile, 5)
+        g = ctx.atrous(rgb, n, pos, alb, w, h, tile, levels)
+        c = oracle.atrous(rgb, n, pos, alb, w, h, tile, levels)
         np.testing.assert_allclose(g, c, rtol=1e-4, atol=1e-6)
 
 
