@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -192,19 +193,21 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     }
 }
 
-// 3x3 convolution with four output rows per workgroup, one per wave, every
+// 3x3 convolution with four output rows per tile, one per wave, every
 // operand from LDS.  Per K chunk of KS input channels the workgroup stages
 // the six input rows its four output rows read (its MT * 16 pixels plus one
-// on each side, zero outside the image) and the chunk's weights for all nine
-// taps.  Input rows are read 1.5 times per layer (6 rows for 4) instead of 3
-// times with one output row per workgroup; every global load of a chunk is
-// issued at once into registers, the next chunk's while the current one runs
-// on MFMA, so a chunk costs two barriers and the loads' latency is hidden
-// behind the arithmetic of the other workgroups and chunks.  Plain bf16
-// outputs go through LDS and leave as 16-byte stores of 8 channels.
-template <int MT, int NT, int KS, bool HEAD = false>
-__global__ __launch_bounds__(256) void k_conv3r(ConvArgs a) {
-    extern __shared__ bf16 lds[];
+// on each side, zero outside the image) and the chunk's weights.  Input rows
+// are read 1.5 times per layer (6 rows for 4) instead of 3 times with one
+// output row per workgroup.  A workgroup takes the tiles of its XCD in turn
+// (one tile unless PERSIST), and every global load of the next (tile, chunk)
+// is issued into registers while the current one runs on MFMA and through
+// its epilogue, so only a workgroup's first load is exposed.  A layer of one
+// chunk with 32 outputs stages its weights once for the whole kernel.  Plain
+// bf16 outputs go through LDS and leave as 16-byte stores of 8 channels.
+// LDS: the weights, then one area for the band, the output tiles and (HEAD)
+// the logits in turn.
+template <int MT, int NT, int KS, bool HEAD, bool PERSIST>
+__device__ __forceinline__ void conv3r_body(const ConvArgs& a, bf16* lds) {
     constexpr int kRun = MT * 16;       // pixels of one output row
     constexpr int kBand = kRun + 2;     // pixels of one staged row
     constexpr int kRows = 6;            // staged input rows: y0 - 1 .. y0 + 4
@@ -213,250 +216,371 @@ __global__ __launch_bounds__(256) void k_conv3r(ConvArgs a) {
     constexpr int kVec = KS / 8;        // 16-byte vectors per pixel (or weight row) and chunk
     constexpr int kBandVec = kRows * kBand * kVec;
     constexpr int kPer = (kBandVec + 255) / 256;
-    bf16* band = lds;                   // [6][kBand][KS]
-    bf16* wl = lds + kRows * kBand * KS;  // [3][NT * 16][ldw]: one row of taps
+    constexpr int kCh = NT * 16, ldo = kCh + 8;
+    // persistent (32 outputs, one chunk): the weights of all nine taps, staged
+    // once; otherwise one row of three taps at a time (all nine per chunk
+    // measured slower there: more LDS traffic per barrier, no fewer stalls)
+    constexpr bool kAllW = PERSIST;
+    static_assert(!PERSIST || NT == 2, "persistent layers have 32 outputs");
+    constexpr int kWTaps = kAllW ? 9 : 3;
+    bf16* wl = lds;                                // [kWTaps][NT * 16][ldw]
+    bf16* area = lds + kWTaps * NT * 16 * ldw;     // band [6][kBand][KS] / output tiles / logits
+    bf16* band = area;
+    // 32-channel band: the four 16-byte chunks of pixel p are stored in the
+    // order c ^ ((p >> 1) & 3).  An A fragment reads one chunk of 16
+    // consecutive pixels (64 B apart); unswizzled, every 128-byte bank window
+    // holds only two of the eight pixels a read cycle serves (4-way bank
+    // conflicts), swizzled it holds eight distinct chunks.
+    constexpr bool kSwz = KS == 32;
+    auto swz = [](int px) { return (px >> 1) & 3; };
+    auto band_slot = [&](int e) {  // bf16 offset of 16-byte vector e = (pixel, chunk) of the band
+        if constexpr (kSwz) {
+            const int px = e >> 2;
+            return (px * 4 + ((e & 3) ^ swz(px))) * 8;
+        } else {
+            return e * 8;
+        }
+    };
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int runs_x = (a.W + kRun - 1) / kRun, groups = (a.H + 3) / 4;
-    // consecutive workgroups go to the 8 XCDs in turn: give each XCD a
-    // contiguous run of blocks, so row groups that share halo rows share an L2
-    const uint32_t xb = blockIdx.x & 7u, kb = blockIdx.x >> 3, per = gridDim.x >> 3, remb = gridDim.x & 7u;
-    const int blk = (int)(xb < remb ? xb * (per + 1) + kb : remb * (per + 1) + (xb - remb) * per + kb);
-    const int rest = blk / runs_x;
-    const int x0 = (blk - rest * runs_x) * kRun;
-    const int b = rest / groups, y0 = (rest - b * groups) * 4;
-    const int y = y0 + wave;
+    const int n_tiles = a.B * groups * runs_x;
+    // consecutive workgroups go to the 8 XCDs in turn: each XCD gets a
+    // contiguous range of tiles, which its workgroups take in turn, so the
+    // tiles in flight on one XCD are neighbours sharing halo rows in its L2
+    const uint32_t G = gridDim.x, xb = blockIdx.x & 7u, kb = blockIdx.x >> 3, per = G >> 3, remb = G & 7u;
+    const uint32_t nwg = per + (xb < remb ? 1u : 0u);
+    const uint32_t first = xb < remb ? xb * (per + 1) : remb * (per + 1) + (xb - remb) * per;
+    const int hi = (int)(((uint64_t)n_tiles * (first + nwg)) / G);
+    int tile = (int)(((uint64_t)n_tiles * first) / G) + (int)kb;
+    if (tile >= hi) return;  // the whole workgroup, before any barrier
     const int n0 = blockIdx.y * NT * 16;
     const int mi = lane & 15, kq = lane >> 4;
     const int Hh = a.H >> 1, Wh = a.W >> 1;
-    f4v acc[MT][NT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-    const bool live = y < a.H;  // every wave takes part in the barriers
+    const int nchunk = a.kpad / KS;
+    constexpr bool wresident = kAllW;  // the launch gives PERSIST only layers of one chunk
+    auto coords = [&](int t, int& tb, int& ty0, int& tx0) {
+        const int rest = t / runs_x;
+        tx0 = (t - rest * runs_x) * kRun;
+        tb = rest / groups;
+        ty0 = (rest - tb * groups) * 4;
+    };
     uint4 pre[kPer];
-    auto load_band = [&](int k0) {
+    auto load_band = [&](int tb, int ty0, int tx0, int k0) {
         const bool from_up = a.up && k0 < a.up_ch;
+        // the thread's element indices are recomputed per call rather than
+        // kept in registers across the tile loop (an empty asm hides the
+        // thread index from loop-invariant hoisting)
+        int tid = (int)threadIdx.x;
+        if constexpr (PERSIST) asm volatile("" : "+v"(tid));
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            const int e = threadIdx.x + 256 * j;
+            const int e = tid + 256 * j;
             pre[j] = make_uint4(0, 0, 0, 0);
             if (e < kBandVec) {
                 const int r = e / (kBand * kVec), rem = e - r * kBand * kVec;
                 const int px = rem / kVec, c8 = rem - px * kVec;
-                const int yy = y0 + r - 1, xx = x0 + px - 1;
+                const int yy = ty0 + r - 1, xx = tx0 + px - 1;
                 if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
                     const bf16* src = from_up
-                        ? a.up + ((size_t)(b * Hh + (yy >> 1)) * Wh + (xx >> 1)) * a.up_stride + k0 + 8 * c8
-                        : a.in + ((size_t)(b * a.H + yy) * a.W + xx) * a.in_stride + a.in_off + k0 + 8 * c8;
+                        ? a.up + ((size_t)(tb * Hh + (yy >> 1)) * Wh + (xx >> 1)) * a.up_stride + k0 + 8 * c8
+                        : a.in + ((size_t)(tb * a.H + yy) * a.W + xx) * a.in_stride + a.in_off + k0 + 8 * c8;
                     pre[j] = *reinterpret_cast<const uint4*>(src);
                 }
             }
         }
     };
-    load_band(0);
-    for (int k0 = 0; k0 < a.kpad; k0 += KS) {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int e = threadIdx.x + 256 * j;
-            if (e < kBandVec) *reinterpret_cast<uint4*>(band + e * 8) = pre[j];
+    auto stage_weights = [&](int t0, int k0) {
+        for (int e = threadIdx.x; e < kWTaps * NT * 16 * kVec; e += 256) {
+            const int tg = e / (NT * 16 * kVec), rem = e - tg * NT * 16 * kVec;
+            const int nn = rem / kVec, c8 = rem - nn * kVec;
+            *reinterpret_cast<uint4*>(wl + (tg * NT * 16 + nn) * ldw + 8 * c8) = *reinterpret_cast<const uint4*>(
+                a.w + ((size_t)(t0 + tg) * a.npad + n0 + nn) * a.kpad + k0 + 8 * c8);
         }
-        if (k0 + KS < a.kpad) load_band(k0 + KS);
-        // the taps in three groups of three (one row of the 3x3 kernel each)
-        for (int g = 0; g < 3; ++g) {
-            if (g) __syncthreads();
-            for (int e = threadIdx.x; e < 3 * NT * 16 * kVec; e += 256) {
-                const int tg = e / (NT * 16 * kVec), rem = e - tg * NT * 16 * kVec;
-                const int nn = rem / kVec, c8 = rem - nn * kVec;
-                *reinterpret_cast<uint4*>(wl + (tg * NT * 16 + nn) * ldw + 8 * c8) = *reinterpret_cast<const uint4*>(
-                    a.w + ((size_t)(3 * g + tg) * a.npad + n0 + nn) * a.kpad + k0 + 8 * c8);
+    };
+    // streamed weights (not resident): item 3c + g is the row g of taps of
+    // chunk c; the next item is loaded into registers while the current one
+    // runs on MFMA, and stored to LDS after the barrier that ends it
+    constexpr int kWVec = 3 * NT * 16 * kVec;
+    constexpr int kWPer = (kWVec + 255) / 256;
+    static_assert(kWPer <= 3, "three weight vectors per thread at most");
+    uint4 wpre[3] = {};
+    // (each vector by a literal index, so the array stays in registers)
+    auto load_w1 = [&](int j, int cc, int t0) {
+        const int e = (int)threadIdx.x + 256 * j;
+        uint4 v = make_uint4(0, 0, 0, 0);  // assigned on every path, or the array is not promoted
+        if (e < kWVec) {
+            const int tg = e / (NT * 16 * kVec), rem = e - tg * NT * 16 * kVec;
+            const int nn = rem / kVec, c8 = rem - nn * kVec;
+            v = *reinterpret_cast<const uint4*>(a.w + ((size_t)(t0 + tg) * a.npad + n0 + nn) * a.kpad + cc * KS + 8 * c8);
+        }
+        wpre[j] = v;
+    };
+    auto load_w = [&](int item) {
+        const int cc = item / 3, t0 = 3 * (item - 3 * cc);
+        load_w1(0, cc, t0);
+        if constexpr (kWPer > 1) load_w1(1, cc, t0);
+        if constexpr (kWPer > 2) load_w1(2, cc, t0);
+    };
+    auto store_w1 = [&](int j) {
+        const int e = (int)threadIdx.x + 256 * j;
+        if (e < kWVec) {
+            const int tg = e / (NT * 16 * kVec), rem = e - tg * NT * 16 * kVec;
+            const int nn = rem / kVec, c8 = rem - nn * kVec;
+            *reinterpret_cast<uint4*>(wl + (tg * NT * 16 + nn) * ldw + 8 * c8) = wpre[j];
+        }
+    };
+    auto store_w = [&]() {
+        store_w1(0);
+        if constexpr (kWPer > 1) store_w1(1);
+        if constexpr (kWPer > 2) store_w1(2);
+    };
+    int b, y0, x0;
+    coords(tile, b, y0, x0);
+    load_band(b, y0, x0, 0);
+    if constexpr (wresident) stage_weights(0, 0);  // visible after the first chunk's barriers
+    else load_w(0);
+    for (;;) {
+        const int y = y0 + wave;
+        const bool live = y < a.H;  // every wave takes part in the barriers
+        const int next = PERSIST ? tile + (int)nwg : hi;
+        int nb = 0, ny0 = 0, nx0 = 0;
+        if (next < hi) coords(next, nb, ny0, nx0);
+        f4v acc[MT][NT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int c = 0; c < nchunk; ++c) {
+            const int k0 = c * KS;
+            __syncthreads();  // the area's previous readers are done
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int e = threadIdx.x + 256 * j;
+                if (e < kBandVec) *reinterpret_cast<uint4*>(band + band_slot(e)) = pre[j];
             }
-            __syncthreads();
-            if (!live) continue;
+            {
+                // one call site for the prefetch of the next (tile, chunk)
+                const bool same = c + 1 < nchunk;
+                if (same || next < hi) load_band(same ? b : nb, same ? y0 : ny0, same ? x0 : nx0, same ? k0 + KS : 0);
+            }
+            // the taps in three groups of three (one row of the 3x3 kernel each)
 #pragma unroll
-            for (int tg = 0; tg < 3; ++tg) {
-                const int tap = 3 * g + tg;
-                int dy = tap / 3 - 1, dx = tap % 3 - 1;
-                if (a.flip) { dy = -dy; dx = -dx; }
-                const bf16* ab = band + ((wave + dy + 1) * kBand + mi + dx + 1) * KS + KL * kq;
-                const bf16* bb = wl + (tg * NT * 16 + mi) * ldw + KL * kq;
-                if constexpr (KS == 32) {
-                    uint4 av[MT], bv[NT];
-#pragma unroll
-                    for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const uint4*>(ab + m * 16 * KS);
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const uint4*>(bb + t * 16 * ldw);
-#pragma unroll
-                    for (int m = 0; m < MT; ++m)
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[m]),
-                                                                                __builtin_bit_cast(bf16x8, bv[t]), acc[m][t], 0, 0, 0);
+            for (int g = 0; g < 3; ++g) {
+                if constexpr (wresident) {
+                    if (g == 0) __syncthreads();
                 } else {
-                    s4 av[MT], bv[NT];
+                    if (g) __syncthreads();  // the previous row of taps is done
+                    store_w();
+                    const int item = 3 * c + g + 1;
+                    if (item < 3 * nchunk) load_w(item);
+                    else if (next < hi) load_w(0);
+                    __syncthreads();
+                }
+                if (!live) continue;
 #pragma unroll
-                    for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const s4*>(ab + m * 16 * KS);
+                for (int tg = 0; tg < 3; ++tg) {
+                    const int tap = 3 * g + tg;
+                    int dy = tap / 3 - 1, dx = tap % 3 - 1;
+                    if (a.flip) { dy = -dy; dx = -dx; }
+                    const int px0 = (wave + dy + 1) * kBand + mi + dx + 1;  // + 16 m: the same swizzle
+                    const bf16* ab = band + px0 * KS + KL * (kSwz ? kq ^ swz(px0) : kq);
+                    const bf16* bb = wl + ((kAllW ? tap : tg) * NT * 16 + mi) * ldw + KL * kq;
+                    if constexpr (KS == 32) {
+                        uint4 av[MT], bv[NT];
 #pragma unroll
-                    for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const s4*>(bb + t * 16 * ldw);
+                        for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const uint4*>(ab + m * 16 * KS);
 #pragma unroll
-                    for (int m = 0; m < MT; ++m)
+                        for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const uint4*>(bb + t * 16 * ldw);
 #pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[t], acc[m][t], 0, 0, 0);
+                        for (int m = 0; m < MT; ++m)
+#pragma unroll
+                            for (int t = 0; t < NT; ++t)
+                                acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    __builtin_bit_cast(bf16x8, av[m]), __builtin_bit_cast(bf16x8, bv[t]), acc[m][t], 0, 0, 0);
+                    } else {
+                        s4 av[MT], bv[NT];
+#pragma unroll
+                        for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const s4*>(ab + m * 16 * KS);
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const s4*>(bb + t * 16 * ldw);
+#pragma unroll
+                        for (int m = 0; m < MT; ++m)
+#pragma unroll
+                            for (int t = 0; t < NT; ++t)
+                                acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv[t], acc[m][t], 0, 0, 0);
+                    }
                 }
             }
         }
-    }
-    if (!a.mask && !a.out_f32) {
-        // bf16 outputs: the wave's kRun x NT*16 tile through LDS (the band,
-        // free once every wave is past its last chunk), then 16-byte stores
-        constexpr int kCh = NT * 16, ldo = kCh + 8;
-        // (the launch sizes the LDS for the larger of staging and output tiles)
-        __syncthreads();
-        bf16* ot = lds + wave * kRun * ldo;
-        if (live) {
+        if (!a.mask && !a.out_f32) {
+            // bf16 outputs: the wave's kRun x NT*16 tile through LDS (the
+            // area, free once every wave is past its last chunk), then
+            // 16-byte stores
+            __syncthreads();
+            bf16* ot = area + wave * kRun * ldo;
+            if (live) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int ch = t * 16 + mi;
+                    const float bias = a.bias ? a.bias[n0 + ch] : 0.0f;
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            float v = acc[m][t][r] + bias;
+                            if (a.relu) v = v > 0.0f ? v : 0.0f;
+                            ot[(m * 16 + 4 * kq + r) * ldo + ch] = f2bf(v);
+                        }
+                }
+            }
+            __syncthreads();
+            if (a.pool) {
+                // 2x2 average pool of the four rows' tiles (rows 2pr, 2pr + 1
+                // of the tile make pooled row pr): the four bf16 outputs
+                // summed in FP32 in (y, x) order, times 1/4, rounded once
+                constexpr int kC8 = kCh / 8, kHalf = kRun / 2;
+                const int Hq = a.H >> 1, Wq = a.W >> 1;
+                for (int e = threadIdx.x; e < 2 * kHalf * kC8; e += 256) {
+                    const int c8 = e % kC8, rest = e / kC8;
+                    const int px = rest % kHalf, pr = rest / kHalf;
+                    const int yp = (y0 >> 1) + pr, xp = (x0 >> 1) + px;
+                    if (yp >= Hq || xp >= Wq) continue;
+                    const bf16* t0 = area + (2 * pr * kRun + 2 * px) * ldo + 8 * c8;
+                    const bf16* t1 = t0 + kRun * ldo;
+                    uint4 v[4] = {*reinterpret_cast<const uint4*>(t0), *reinterpret_cast<const uint4*>(t0 + ldo),
+                                  *reinterpret_cast<const uint4*>(t1), *reinterpret_cast<const uint4*>(t1 + ldo)};
+                    bf16 o[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        float sum = 0.0f;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) sum += bf2f(reinterpret_cast<const bf16*>(&v[j])[k]);
+                        o[k] = f2bf(0.25f * sum);
+                    }
+                    *reinterpret_cast<uint4*>(a.pool + ((size_t)(b * Hq + yp) * Wq + xp) * a.pool_stride + n0 + 8 * c8) =
+                        *reinterpret_cast<const uint4*>(o);
+                }
+            }
+            if constexpr (HEAD) {
+                // the head on this wave's tile: A fragments of 16 pixels x 32
+                // channels straight from the tile, logits to LDS (the area is
+                // free once every wave holds its fragments), then one pixel
+                // per lane: softmax of its 25 logits and the clamped 5x5
+                // window of the input radiance (k_kpn_apply's arithmetic)
+                static_assert(NT == 2 && KS == 32 && MT == 4, "head after a 32-channel layer, 64 pixels per wave");
+                uint4 hv[MT];
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+                    hv[m] = live ? *reinterpret_cast<const uint4*>(ot + (m * 16 + mi) * ldo + 8 * kq) : make_uint4(0, 0, 0, 0);
+                __syncthreads();
+                if (live) {
+                    float* lgt = reinterpret_cast<float*>(area) + wave * kRun * 25;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const uint4 bw = *reinterpret_cast<const uint4*>(a.head_w + (t * 16 + mi) * 32 + 8 * kq);
+                        const int ch = t * 16 + mi;
+                        const float hb = a.head_b[ch];
+#pragma unroll
+                        for (int m = 0; m < MT; ++m) {
+                            f4v l = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+                            l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, hv[m]),
+                                                                        __builtin_bit_cast(bf16x8, bw), l, 0, 0, 0);
+                            if (ch < 25)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) lgt[(m * 16 + 4 * kq + r) * 25 + ch] = l[r] + hb;
+                        }
+                    }
+                    // the wave's lanes exchange the logits through LDS
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const int x = x0 + lane;
+                    if (x < a.img_w && y < a.img_h) {
+                        float gl[25];
+#pragma unroll
+                        for (int k = 0; k < 25; ++k) gl[k] = lgt[lane * 25 + k];
+                        float mx = gl[0];
+#pragma unroll
+                        for (int k = 1; k < 25; ++k) mx = fmaxf(mx, gl[k]);
+                        float sum = 0.0f;
+#pragma unroll
+                        for (int k = 0; k < 25; ++k) { gl[k] = expf(gl[k] - mx); sum += gl[k]; }
+                        const float is = 1.0f / sum;
+                        const float* rad = a.rad + (size_t)b * 10 * a.img_h * a.img_w;
+                        const size_t plane = (size_t)a.img_h * a.img_w;
+                        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+#pragma unroll
+                        for (int k = 0; k < 25; ++k) {
+                            const float wgt = gl[k] * is;
+                            const int yy = min(max(y + k / 5 - 2, 0), a.img_h - 1), xx = min(max(x + k % 5 - 2, 0), a.img_w - 1);
+                            const size_t q = (size_t)yy * a.img_w + xx;
+                            r0 += wgt * rad[q];
+                            r1 += wgt * rad[plane + q];
+                            r2 += wgt * rad[2 * plane + q];
+                        }
+                        const size_t p = (size_t)(b * a.img_h + y) * a.img_w + x;
+                        a.rgb[3 * p] = r0;
+                        a.rgb[3 * p + 1] = r1;
+                        a.rgb[3 * p + 2] = r2;
+                    }
+                }
+            } else if (live) {
+                constexpr int kOutVec = kRun * kCh / 8;
+#pragma unroll
+                for (int j = 0; j < (kOutVec + 63) / 64; ++j) {
+                    const int e = lane + 64 * j;
+                    if (e < kOutVec) {
+                        const int px = e / (kCh / 8), c8 = e - px * (kCh / 8);
+                        const int x = x0 + px;
+                        if (x < a.W) {
+                            const size_t p = (size_t)(b * a.H + y) * a.W + x;
+                            *reinterpret_cast<uint4*>(a.out + p * a.out_stride + a.out_off + n0 + 8 * c8) =
+                                *reinterpret_cast<const uint4*>(ot + px * ldo + 8 * c8);
+                        }
+                    }
+                }
+            }
+        } else if (live) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const int c = t * 16 + mi;
-                const float bias = a.bias ? a.bias[n0 + c] : 0.0f;
+                const int n = n0 + t * 16 + mi;
+                const float bias = a.bias ? a.bias[n] : 0.0f;
 #pragma unroll
                 for (int m = 0; m < MT; ++m)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
+                        const int x = x0 + m * 16 + 4 * kq + r;
+                        if (x >= a.W) continue;
                         float v = acc[m][t][r] + bias;
                         if (a.relu) v = v > 0.0f ? v : 0.0f;
-                        ot[(m * 16 + 4 * kq + r) * ldo + c] = f2bf(v);
+                        const size_t p = (size_t)(b * a.H + y) * a.W + x;
+                        if (a.mask && !(bf2f(a.mask[p * a.mask_stride + a.mask_off + n]) > 0.0f)) v = 0.0f;
+                        if (a.out_f32) a.out_f32[p * a.npad + n] = v;
+                        else a.out[p * a.out_stride + a.out_off + n] = f2bf(v);
                     }
             }
         }
-        __syncthreads();
-        if (a.pool) {
-            // 2x2 average pool of the four rows' tiles (rows 2pr, 2pr + 1 of
-            // the workgroup make pooled row pr): the four bf16 outputs summed
-            // in FP32 in (y, x) order, times 1/4, rounded once
-            constexpr int kC8 = kCh / 8, kHalf = kRun / 2;
-            const int Hq = a.H >> 1, Wq = a.W >> 1;
-            for (int e = threadIdx.x; e < 2 * kHalf * kC8; e += 256) {
-                const int c8 = e % kC8, rest = e / kC8;
-                const int px = rest % kHalf, pr = rest / kHalf;
-                const int yp = (y0 >> 1) + pr, xp = (x0 >> 1) + px;
-                if (yp >= Hq || xp >= Wq) continue;
-                const bf16* t0 = lds + (2 * pr * kRun + 2 * px) * ldo + 8 * c8;
-                const bf16* t1 = t0 + kRun * ldo;
-                uint4 v[4] = {*reinterpret_cast<const uint4*>(t0), *reinterpret_cast<const uint4*>(t0 + ldo),
-                              *reinterpret_cast<const uint4*>(t1), *reinterpret_cast<const uint4*>(t1 + ldo)};
-                bf16 o[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    float acc = 0.0f;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) acc += bf2f(reinterpret_cast<const bf16*>(&v[j])[k]);
-                    o[k] = f2bf(0.25f * acc);
-                }
-                *reinterpret_cast<uint4*>(a.pool + ((size_t)(b * Hq + yp) * Wq + xp) * a.pool_stride + n0 + 8 * c8) =
-                    *reinterpret_cast<const uint4*>(o);
-            }
-        }
-        if constexpr (HEAD) {
-            // the head on this wave's tile: A fragments of 16 pixels x 32
-            // channels straight from the tile, logits to LDS (the staging
-            // area is free once every wave holds its fragments), then one
-            // pixel per lane: softmax of its 25 logits and the clamped 5x5
-            // window of the input radiance (k_kpn_apply's arithmetic)
-            static_assert(NT == 2 && KS == 32 && MT == 4, "head after a 32-channel layer, 64 pixels per wave");
-            uint4 hv[MT];
-#pragma unroll
-            for (int m = 0; m < MT; ++m)
-                hv[m] = live ? *reinterpret_cast<const uint4*>(ot + (m * 16 + mi) * ldo + 8 * kq) : make_uint4(0, 0, 0, 0);
-            __syncthreads();
-            if (!live) return;
-            float* lgt = reinterpret_cast<float*>(lds) + wave * kRun * 25;
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const uint4 bw = *reinterpret_cast<const uint4*>(a.head_w + (t * 16 + mi) * 32 + 8 * kq);
-                const int c = t * 16 + mi;
-                const float hb = a.head_b[c];
-#pragma unroll
-                for (int m = 0; m < MT; ++m) {
-                    f4v l = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-                    l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, hv[m]),
-                                                                __builtin_bit_cast(bf16x8, bw), l, 0, 0, 0);
-                    if (c < 25)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) lgt[(m * 16 + 4 * kq + r) * 25 + c] = l[r] + hb;
-                }
-            }
-            // the wave's lanes exchange the logits through LDS
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int x = x0 + lane;
-            if (x >= a.img_w || y >= a.img_h) return;
-            float gl[25];
-#pragma unroll
-            for (int k = 0; k < 25; ++k) gl[k] = lgt[lane * 25 + k];
-            float mx = gl[0];
-#pragma unroll
-            for (int k = 1; k < 25; ++k) mx = fmaxf(mx, gl[k]);
-            float sum = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 25; ++k) { gl[k] = expf(gl[k] - mx); sum += gl[k]; }
-            const float is = 1.0f / sum;
-            const float* rad = a.rad + (size_t)b * 10 * a.img_h * a.img_w;
-            const size_t plane = (size_t)a.img_h * a.img_w;
-            float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 25; ++k) {
-                const float wgt = gl[k] * is;
-                const int yy = min(max(y + k / 5 - 2, 0), a.img_h - 1), xx = min(max(x + k % 5 - 2, 0), a.img_w - 1);
-                const size_t q = (size_t)yy * a.img_w + xx;
-                r0 += wgt * rad[q];
-                r1 += wgt * rad[plane + q];
-                r2 += wgt * rad[2 * plane + q];
-            }
-            const size_t p = (size_t)(b * a.img_h + y) * a.img_w + x;
-            a.rgb[3 * p] = r0;
-            a.rgb[3 * p + 1] = r1;
-            a.rgb[3 * p + 2] = r2;
-            return;
-        }
-        if (!live) return;
-        constexpr int kOutVec = kRun * kCh / 8;
-#pragma unroll
-        for (int j = 0; j < (kOutVec + 63) / 64; ++j) {
-            const int e = lane + 64 * j;
-            if (e < kOutVec) {
-                const int px = e / (kCh / 8), c8 = e - px * (kCh / 8);
-                const int x = x0 + px;
-                if (x < a.W) {
-                    const size_t p = (size_t)(b * a.H + y) * a.W + x;
-                    *reinterpret_cast<uint4*>(a.out + p * a.out_stride + a.out_off + n0 + 8 * c8) =
-                        *reinterpret_cast<const uint4*>(ot + px * ldo + 8 * c8);
-                }
-            }
-        }
-        return;
+        if (next >= hi) break;
+        tile = next;
+        b = nb;
+        y0 = ny0;
+        x0 = nx0;
     }
-    if (!live) return;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int n = n0 + t * 16 + mi;
-        const float bias = a.bias ? a.bias[n] : 0.0f;
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = x0 + m * 16 + 4 * kq + r;
-                if (x >= a.W) continue;
-                float v = acc[m][t][r] + bias;
-                if (a.relu) v = v > 0.0f ? v : 0.0f;
-                const size_t p = (size_t)(b * a.H + y) * a.W + x;
-                if (a.mask && !(bf2f(a.mask[p * a.mask_stride + a.mask_off + n]) > 0.0f)) v = 0.0f;
-                if (a.out_f32) a.out_f32[p * a.npad + n] = v;
-                else a.out[p * a.out_stride + a.out_off + n] = f2bf(v);
-            }
-    }
+}
+
+// The kernels: one tile per workgroup, or (PERSIST, layers of one K chunk
+// whose weights stay in LDS) as many workgroups as are resident, each
+// taking tiles in turn with the next tile's band loaded during the current
+// one; three waves per SIMD keep three such workgroups on a CU.
+template <int MT, int NT, int KS, bool HEAD = false>
+__global__ __launch_bounds__(256) void k_conv3r(ConvArgs a) {
+    extern __shared__ bf16 lds[];
+    conv3r_body<MT, NT, KS, HEAD, false>(a, lds);
+}
+template <int MT, int NT, int KS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_conv3p(ConvArgs a) {
+    extern __shared__ bf16 lds[];
+    conv3r_body<MT, NT, KS, false, true>(a, lds);
 }
 
 struct WgradArgs {
@@ -802,6 +926,33 @@ T* dalloc(pico_cnn* n, size_t count) {
     return (T*)p;
 }
 
+// The persistent 3x3 kernel: as many workgroups as are resident at once
+// (per CU, at this LDS size; one size per instantiation), at most one per
+// tile, per block of output channels.
+template <int MT, int NT, int KS, bool HEAD>
+int launch_conv3r(pico_cnn* n, const char* name, const ConvArgs& a, size_t lds) {
+    constexpr int run4 = MT * 16;
+    const long tiles = (long)a.B * ((a.H + 3) / 4) * ((a.W + run4 - 1) / run4);
+    const unsigned ny = (unsigned)(a.npad / (NT * 16));
+    if constexpr (NT == 2 && !HEAD) {
+        if (a.kpad == KS) {
+            static std::atomic<int> per_cu{0};
+            int r = per_cu.load(std::memory_order_relaxed);
+            if (r == 0) {
+                int nb = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_conv3p<MT, NT, KS>, 256, lds) != hipSuccess || nb < 1)
+                    nb = 1;
+                per_cu.store(r = nb, std::memory_order_relaxed);
+            }
+            const long cap = std::max(1L, (long)r * std::max(1, n->c->num_cus) / (long)ny);
+            PICO_LAUNCH(n->c, name, (k_conv3p<MT, NT, KS>), dim3((unsigned)std::min(tiles, cap), ny), 256, lds, a);
+            return PICO_OK;
+        }
+    }
+    PICO_LAUNCH(n->c, name, (k_conv3r<MT, NT, KS, HEAD>), dim3((unsigned)tiles, ny), 256, lds, a);
+    return PICO_OK;
+}
+
 // Launches k_conv with the widest channel block (16, 32 or 64) that divides
 // the output channels, 8 accumulators per wave, and the 32-deep instruction
 // where the input channels allow it.
@@ -813,26 +964,24 @@ int launch_conv_k(pico_cnn* n, const char* name, const ConvArgs& a) {
     if (a.pool && !(a.taps == 9 && nt >= 2)) return fail(n->c, PICO_ERR_STATE, "fused pool needs the 3x3 kernel");
     if (a.pool && (a.mask || a.out_f32 || a.head_w || ((a.H | a.W) & 1))) return fail(n->c, PICO_ERR_STATE, "fused pool needs bf16 outputs of even size");
     if (a.taps == 9 && nt >= 2) {
-        // 3x3 with all operands in LDS, four output rows per workgroup: six
-        // rows of MT * 16 + 2 pixels and one row of three taps' weights, per
-        // chunk of KS channels
+        // 3x3 with all operands in LDS, four output rows per tile: the
+        // chunk's weights (k_conv3r's kWTaps taps), then one area for the six
+        // staged rows of MT * 16 + 2 pixels, the output tiles and the logits
         const int mt = 8 / nt, run4 = mt * 16;
-        const dim3 grid4((unsigned)(a.B * ((a.H + 3) / 4) * ((a.W + run4 - 1) / run4)), (unsigned)(a.npad / (nt * 16)));
-        const size_t lds3 = std::max(6 * (size_t)(run4 + 2) * KS + 3 * (size_t)nt * 16 * (KS + 8),
-                                     4 * (size_t)run4 * (nt * 16 + 8)) * sizeof(bf16);
+        // (nine taps for the persistent kernel: 32 outputs, one chunk, no head)
+        const size_t wtaps = nt == 2 && a.kpad == KS && !a.head_w ? 9 : 3;
+        const size_t wbytes = wtaps * nt * 16 * (KS + 8) * sizeof(bf16);
+        const size_t area = std::max(6 * (size_t)(run4 + 2) * KS, 4 * (size_t)run4 * (nt * 16 + 8)) * sizeof(bf16);
         if (a.head_w) {
             if constexpr (KS == 32) {
                 if (nt != 2 || a.mask || a.out_f32) return fail(n->c, PICO_ERR_STATE, "head fusion needs 32 channels");
                 // logits of the four waves (64 x 25 floats each) share the area
-                const size_t ldsh = std::max(lds3, 4 * (size_t)run4 * 25 * sizeof(float));
-                PICO_LAUNCH(n->c, name, (k_conv3r<4, 2, 32, true>), grid4, 256, ldsh, a);
-                return PICO_OK;
+                return launch_conv3r<4, 2, 32, true>(n, name, a, wbytes + std::max(area, 4 * (size_t)run4 * 25 * sizeof(float)));
             }
             return fail(n->c, PICO_ERR_STATE, "head fusion needs 32 input channels");
         }
-        if (nt == 4) PICO_LAUNCH(n->c, name, (k_conv3r<2, 4, KS>), grid4, 256, lds3, a);
-        else PICO_LAUNCH(n->c, name, (k_conv3r<4, 2, KS>), grid4, 256, lds3, a);
-        return PICO_OK;
+        if (nt == 4) return launch_conv3r<2, 4, KS, false>(n, name, a, wbytes + area);
+        return launch_conv3r<4, 2, KS, false>(n, name, a, wbytes + area);
     }
     if (a.up) return fail(n->c, PICO_ERR_STATE, "fused upsampling needs the 3x3 kernel");
     const size_t lds = (size_t)nt * 16 * (a.kpad + 8) * sizeof(bf16);
