@@ -176,9 +176,9 @@ struct TraceStack {
     }
 };
 
-__device__ __forceinline__ float exp2_scale(uint32_t bits) {
-    // int8 exponent -> 2^e as a float (e in [-126, 126])
-    return __int_as_float(((int)(int8_t)(bits & 0xffu) + 127) << 23);
+__device__ __forceinline__ int exp_of(uint32_t bits) {
+    // the node's int8 scale exponent e (in [-126, 126]) of one axis
+    return (int)(bits << 24) >> 24;
 }
 
 // The reference's order of std::min / std::max (NaN operands fall through as
@@ -236,6 +236,10 @@ __device__ __forceinline__ f3 to_world_point(const float* M, f3 P) {
               (M[2] * P.x + M[6] * P.y) + (M[10] * P.z + M[14]));
 }
 
+// Inner nodes a lane may take in one traversal step (1: 1024 Mrays/s on the
+// instanced 1080p frame, 2: 1057, 3: 1025).
+constexpr int kInnerSteps = 2;
+
 // Two-level closest-hit traversal over 4-wide nodes.  Each node tests its
 // four quantised child boxes (planes origin + q 2^e, which contain the exact
 // boxes) with the slab test, the plane distances taken as
@@ -255,6 +259,7 @@ struct TraceState {
     f3 o, d;      // world ray
     f3 ro, rd;    // ray in the current space (world or the instance's object space)
     f3 inv, oi;   // slab-test reciprocals and -ro * inv
+    f3 winv, woi; // the same of the world ray, restored on leaving an instance
     Hit h;
     uint32_t ref, cur;
     bool in_blas;
@@ -266,6 +271,8 @@ __device__ __forceinline__ void trace_begin(const DevScene& s, TraceState& t, Tr
     t.o = o; t.d = d; t.ro = o; t.rd = d;
     t.inv = slab_rcp(d);
     t.oi = mk(-o.x * t.inv.x, -o.y * t.inv.y, -o.z * t.inv.z);
+    t.winv = t.inv;
+    t.woi = t.oi;
     t.h.t = tmax; t.h.u = 0.0f; t.h.v = 0.0f; t.h.inst = 0xffffffffu; t.h.prim = 0xffffffffu;
     t.ref = s.tlas_root;
     t.cur = 0xffffffffu;
@@ -291,10 +298,10 @@ __device__ __forceinline__ bool inner_node(const DevScene& s, TraceState& t, Tra
     const uint4 r = *reinterpret_cast<const uint4*>(&n->ref[0]);
     if (COUNT) cnt->nodes++;
     const uint32_t eb = __float_as_uint(a.w);
-    const float sx = exp2_scale(eb), sy = exp2_scale(eb >> 8), sz = exp2_scale(eb >> 16);
     float key[4];
     uint32_t cref[4] = {r.x, r.y, r.z, r.w};
-    const float Ax = sx * t.inv.x, Ay = sy * t.inv.y, Az = sz * t.inv.z;
+    // 2^e inv per axis: one v_ldexp_f32 each, the same bits as the product
+    const float Ax = ldexpf(t.inv.x, exp_of(eb)), Ay = ldexpf(t.inv.y, exp_of(eb >> 8)), Az = ldexpf(t.inv.z, exp_of(eb >> 16));
     const float Bx = fmaf(a.x, t.inv.x, t.oi.x), By = fmaf(a.y, t.inv.y, t.oi.y), Bz = fmaf(a.z, t.inv.z, t.oi.z);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -397,8 +404,8 @@ __device__ __forceinline__ bool enter_instance(const DevScene& s, TraceState& t,
 
 __device__ __forceinline__ void leave_instance(TraceState& t) {
     t.ro = t.o; t.rd = t.d;
-    t.inv = slab_rcp(t.d);
-    t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
+    t.inv = t.winv;
+    t.oi = t.woi;
     t.in_blas = false;
 }
 
@@ -411,7 +418,14 @@ template <int STACK, bool COUNT>
 __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& t, TraceStack<STACK>& st, TraceCounters* cnt,
                                            uint32_t* overflow) {
     if (!(t.ref & kLeafBit)) {
-        if (inner_node<STACK, COUNT>(s, t, st, cnt, overflow)) return false;
+        // up to kInnerSteps inner nodes in a row: most steps are inner nodes,
+        // and the wave pays for the leaf and instance paths once per step
+        // whichever lanes take them
+#pragma unroll
+        for (int k = 0; k < kInnerSteps; ++k) {
+            if (!inner_node<STACK, COUNT>(s, t, st, cnt, overflow)) break;
+            if (k + 1 == kInnerSteps || (t.ref & kLeafBit)) return false;
+        }
     } else if (t.in_blas) {
         leaf_tris<COUNT>(s, t, t.ref, cnt);
     } else {
