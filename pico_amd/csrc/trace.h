@@ -236,9 +236,14 @@ __device__ __forceinline__ f3 to_world_point(const float* M, f3 P) {
               (M[2] * P.x + M[6] * P.y) + (M[10] * P.z + M[14]));
 }
 
-// Inner nodes a lane may take in one traversal step (1: 1024 Mrays/s on the
-// instanced 1080p frame, 2: 1057, 3: 1025).
+// Inner nodes a lane may take in one traversal step, and triangles of a
+// BLAS leaf it may test in one step.  The wave runs every kind of step that
+// any of its lanes takes, so the common one (inner nodes) is batched and the
+// leaf loop is short.  Instanced 1080p frame, Mrays/s: inner 1 / 2 / 3 with
+// whole leaves: 1024 / 1057 / 1025; inner 2 with 4 / 2 / 1 triangles per
+// step: 1066 / 1072 / 1079; inner 3 with 1 triangle: 1029.
 constexpr int kInnerSteps = 2;
+constexpr int kLeafTris = 1;
 
 // Two-level closest-hit traversal over 4-wide nodes.  Each node tests its
 // four quantised child boxes (planes origin + q 2^e, which contain the exact
@@ -335,10 +340,13 @@ __device__ __forceinline__ bool inner_node(const DevScene& s, TraceState& t, Tra
 // The triangles of BLAS leaf `leaf`, Moller-Trumbore with the reference's
 // epsilons (Source/Core/LowerLevelMeshBVH.cpp:165-207), in the current
 // (object) space.
+// At most kLeafTris triangles per call: a leaf with more is left in t.ref
+// with its remaining triangles (returns true), to go on in the next step.
 template <bool COUNT>
-__device__ __forceinline__ void leaf_tris(const DevScene& s, TraceState& t, uint32_t leaf, TraceCounters* cnt) {
+__device__ __forceinline__ bool leaf_tris(const DevScene& s, TraceState& t, uint32_t leaf, TraceCounters* cnt) {
     const uint32_t first = leaf & kPrimMask, count = (leaf >> 27) & 15u;
-    for (uint32_t k = 0; k < count; ++k) {
+    const uint32_t now = count < (uint32_t)kLeafTris ? count : (uint32_t)kLeafTris;
+    for (uint32_t k = 0; k < now; ++k) {
         const DTri* tr = s.tris + first + k;
         const float4 p0 = ldg4(&tr->v0[0]);
         const float4 p1 = ldg4(&tr->e1[0]);
@@ -360,6 +368,9 @@ __device__ __forceinline__ void leaf_tris(const DevScene& s, TraceState& t, uint
             t.h.t = tt; t.h.u = u; t.h.v = v; t.h.inst = t.cur; t.h.prim = __float_as_uint(p0.w);
         }
     }
+    if (now == count) return false;
+    t.ref = kLeafBit | ((count - now) << 27) | (first + now);
+    return true;
 }
 
 // TLAS leaf t.ref (one instance): a built-in shape is intersected at once
@@ -427,7 +438,7 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& t, Tra
             if (k + 1 == kInnerSteps || (t.ref & kLeafBit)) return false;
         }
     } else if (t.in_blas) {
-        leaf_tris<COUNT>(s, t, t.ref, cnt);
+        if (leaf_tris<COUNT>(s, t, t.ref, cnt)) return false;
     } else {
         if (enter_instance<STACK, COUNT>(s, t, st, cnt, overflow)) return false;
     }
