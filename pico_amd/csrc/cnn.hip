@@ -218,8 +218,9 @@ __device__ __forceinline__ void conv3r_body(const ConvArgs& a, bf16* lds) {
     constexpr int kPer = (kBandVec + 255) / 256;
     constexpr int kCh = NT * 16, ldo = kCh + 8;
     // persistent (32 outputs, one chunk): the weights of all nine taps, staged
-    // once; otherwise one row of three taps at a time (all nine per chunk
-    // measured slower there: more LDS traffic per barrier, no fewer stalls)
+    // once; otherwise one row of three taps at a time, streamed through
+    // registers (all nine per chunk measured slower there: their registers
+    // cost a wave per SIMD)
     constexpr bool kAllW = PERSIST;
     static_assert(!PERSIST || NT == 2, "persistent layers have 32 outputs");
     constexpr int kWTaps = kAllW ? 9 : 3;
