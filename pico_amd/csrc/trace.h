@@ -308,14 +308,25 @@ __device__ __forceinline__ bool inner_node(const DevScene& s, TraceState& t, Tra
     // 2^e inv per axis: one v_ldexp_f32 each, the same bits as the product
     const float Ax = ldexpf(t.inv.x, exp_of(eb)), Ay = ldexpf(t.inv.y, exp_of(eb >> 8)), Az = ldexpf(t.inv.z, exp_of(eb >> 16));
     const float Bx = fmaf(a.x, t.inv.x, t.oi.x), By = fmaf(a.y, t.inv.y, t.oi.y), Bz = fmaf(a.z, t.inv.z, t.oi.z);
+    // The plane distance fma(q, A, B) is monotonic in q (one rounding, A of
+    // the reciprocal's sign and never 0), so on each axis the near plane is
+    // the low one when the reciprocal is positive and the high one otherwise:
+    // one select per axis and node of the four children's low / high bytes
+    // replaces a min and a max per axis and child (same values; a NaN plane,
+    // only possible with an infinite A, is skipped by max3 / min3, which can
+    // only keep a box).
+    const bool nx = t.inv.x < 0.0f, ny = t.inv.y < 0.0f, nz = t.inv.z < 0.0f;
+    const uint32_t xn = nx ? qxy.y : qxy.x, xf = nx ? qxy.x : qxy.y;
+    const uint32_t yn = ny ? qxy.w : qxy.z, yf = ny ? qxy.z : qxy.w;
+    const uint32_t zn = nz ? qz.y : qz.x, zf = nz ? qz.x : qz.y;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int sh = 8 * i;
-        const float x0 = fmaf((float)((qxy.x >> sh) & 0xffu), Ax, Bx), x1 = fmaf((float)((qxy.y >> sh) & 0xffu), Ax, Bx);
-        const float y0 = fmaf((float)((qxy.z >> sh) & 0xffu), Ay, By), y1 = fmaf((float)((qxy.w >> sh) & 0xffu), Ay, By);
-        const float z0 = fmaf((float)((qz.x >> sh) & 0xffu), Az, Bz), z1 = fmaf((float)((qz.y >> sh) & 0xffu), Az, Bz);
-        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
-        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kWide;
+        const float x0 = fmaf((float)((xn >> sh) & 0xffu), Ax, Bx), x1 = fmaf((float)((xf >> sh) & 0xffu), Ax, Bx);
+        const float y0 = fmaf((float)((yn >> sh) & 0xffu), Ay, By), y1 = fmaf((float)((yf >> sh) & 0xffu), Ay, By);
+        const float z0 = fmaf((float)((zn >> sh) & 0xffu), Az, Bz), z1 = fmaf((float)((zf >> sh) & 0xffu), Az, Bz);
+        const float tn = fmaxf(fmaxf(x0, y0), z0);
+        const float tf = fminf(fminf(x1, y1), z1) * kWide;
         const bool hit = cref[i] != kEmptyRef && tf >= 0.0f && tn <= tf && tn < t.h.t;
         key[i] = hit ? tn : INFINITY;
     }
