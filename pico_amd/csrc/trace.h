@@ -316,6 +316,8 @@ __device__ __forceinline__ bool inner_node(const DevScene& s, TraceState& t, Tra
     // only possible with an infinite A, is skipped by max3 / min3, which can
     // only keep a box).
     const bool nx = t.inv.x < 0.0f, ny = t.inv.y < 0.0f, nz = t.inv.z < 0.0f;
+    // the largest float below the hit distance found so far (always > 0)
+    const float hp = __uint_as_float(__float_as_uint(t.h.t) - 1u);
     const uint32_t xn = nx ? qxy.y : qxy.x, xf = nx ? qxy.x : qxy.y;
     const uint32_t yn = ny ? qxy.w : qxy.z, yf = ny ? qxy.z : qxy.w;
     const uint32_t zn = nz ? qz.y : qz.x, zf = nz ? qz.x : qz.y;
@@ -327,7 +329,10 @@ __device__ __forceinline__ bool inner_node(const DevScene& s, TraceState& t, Tra
         const float z0 = fmaf((float)((zn >> sh) & 0xffu), Az, Bz), z1 = fmaf((float)((zf >> sh) & 0xffu), Az, Bz);
         const float tn = fmaxf(fmaxf(x0, y0), z0);
         const float tf = fminf(fminf(x1, y1), z1) * kWide;
-        const bool hit = cref[i] != kEmptyRef && tf >= 0.0f && tn <= tf && tn < t.h.t;
+        // tf >= 0, tn <= tf and tn < h as one comparison (h > 0, so tn < h
+        // is tn <= hp); a NaN bound, only from a NaN plane, is skipped by
+        // fmaxf / fminf, which can only keep a box
+        const bool hit = cref[i] != kEmptyRef && fmaxf(tn, 0.0f) <= fminf(tf, hp);
         key[i] = hit ? tn : INFINITY;
     }
     // sort the four (distance, child) pairs, nearest first
