@@ -466,7 +466,8 @@ typedef struct pico_view {
 } pico_view;
 
 /* Launch shape of the wavefront, chosen by the autotuner:
- *   queue_paths      paths in flight per pass (samples per chunk = queue_paths / pixels); 0 = 16M
+ *   queue_paths      paths in flight per pass (samples per chunk = queue_paths / pixels); 0 = 128M
+ *                    (216 B of path state each: 27 GB of device memory at 128M)
  *   tile_w, tile_h   order of the pixels in the active list, i.e. which pixels
  *                    neighbouring lanes trace: tiles of tile_w x tile_h in
  *                    row-major tile order, row-major inside a tile; 0 x 0 =

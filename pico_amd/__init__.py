@@ -628,7 +628,7 @@ class Context:
         self._check(lib().pico_hip_render_views_d(self._h, arr, len(views), queue_paths, st))
         return list(st)
 
-    def set_tuning(self, queue_paths=16 << 20, tile=(0, 0), flags=0):
+    def set_tuning(self, queue_paths=128 << 20, tile=(0, 0), flags=0):
         t = Tuning(queue_paths, tile[0], tile[1], flags)
         self._check(lib().pico_hip_set_tuning(self._h, C.byref(t)))
 

@@ -23,6 +23,13 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Default path budget of a render pass (paths in flight; 216 B of path state
+// each, 27 GB at 128M).  Samples per chunk = budget / live pixels, so a 1080p
+// 64-spp frame runs as 2 chunks (6 + 58 samples) instead of 9 at 16M: the
+// last bounces, whose few long rays set each launch's duration, are paid
+// twice instead of nine times (1680 against 1150 Mrays/s).
+constexpr uint32_t kDefaultQueuePaths = 128u << 20;
+
 struct TuneEntry {
     pico_tuning tuning;
     float chosen_ms, default_ms;
@@ -53,7 +60,7 @@ struct pico_hip_ctx {
     // named scratch buffers (grown on demand, freed at destroy)
     std::map<std::string, DevBuf> bufs;
     // launch shape of the wavefront and the autotuner's cache
-    pico_tuning tuning{16u << 20, 0, 0, 0};  // queue depth, tile, flags
+    pico_tuning tuning{kDefaultQueuePaths, 0, 0, 0};  // queue depth, tile, flags
     std::map<uint64_t, TuneEntry> tune_cache;
     // timing
     bool timing = false;

@@ -740,7 +740,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     if (pix_total > (1ull << 31)) return fail(c, PICO_ERR_LIMIT, "too many pixels in one launch");
     if (pl.max_depth > kMaxBounce - 1) return fail(c, PICO_ERR_LIMIT, "max_depth above 63");
     if (pl.max_spp == 0) return PICO_OK;
-    uint32_t cap = tune.queue_paths ? tune.queue_paths : (16u << 20);
+    uint32_t cap = tune.queue_paths ? tune.queue_paths : kDefaultQueuePaths;
     cap = std::max<uint32_t>(cap, 64);
     const bool accum_mode = hv[0].mode == MODE_ACCUM;
     // Chunk schedule, fixed on the host: a chunk gives every pixel of the
@@ -1304,9 +1304,9 @@ int pico_hip_autotune_views(pico_hip_ctx* c, const pico_view* views, uint32_t n,
     if (it == c->tune_cache.end()) {
         for (uint32_t i = 0; i < n; ++i)
             hv[i].spp = std::max<uint32_t>(1, (uint32_t)(((uint64_t)views[i].params.spp * calib_spp + max_spp - 1) / max_spp));
-        const pico_tuning defaults{16u << 20, 0, 0, 0};
+        const pico_tuning defaults{kDefaultQueuePaths, 0, 0, 0};
         std::vector<pico_tuning> cand{defaults};
-        const uint32_t queues[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20};
+        const uint32_t queues[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20, 128u << 20};
         const uint32_t tiles[][2] = {{0, 0}, {8, 8}, {16, 16}, {32, 4}};
         for (uint32_t q : queues)
             for (auto& t : tiles) {
