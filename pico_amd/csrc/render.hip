@@ -14,7 +14,7 @@
 //   compact   surviving items -> next chunk's active list
 // Every count lives in device memory; the host never synchronises inside a
 // render.  Queue appends are wave-aggregated (ballot + popcount, one atomic per
-// wave); persistent waves fetch 64 queue entries per atomic.
+// wave and queue); persistent waves fetch 64 or more queue entries per atomic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -94,24 +94,10 @@ struct Pool {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Wave-aggregated append: every active lane calls it; lanes with pred get
-// consecutive slots.  Returns the slot (valid only where pred).
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-    const uint64_t m = __ballot(pred);
-    const uint32_t n = (uint32_t)__popcll(m);
-    const uint64_t active = __ballot(1);
-    const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)active) - 1);
+// Persistent work loop helper: each wave claims `n` consecutive entries.
+__device__ __forceinline__ uint32_t wave_claim(uint32_t* head, uint32_t n) {
     uint32_t base = 0;
-    if (lane_id() == leader && n) base = atomicAdd(counter, n);
-    base = __shfl(base, leader);
-    const uint64_t below = m & ((1ull << lane_id()) - 1ull);
-    return base + (uint32_t)__popcll(below);
-}
-
-// Persistent work loop helper: each wave claims 64 consecutive entries.
-__device__ __forceinline__ uint32_t wave_claim(uint32_t* head) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(head, 64u);
+    if (lane_id() == 0) base = atomicAdd(head, n);
     return __shfl(base, 0);
 }
 
@@ -273,6 +259,8 @@ __device__ __forceinline__ f3 implicit_light_point(uint32_t shape, Rng& rng) {
     return (corner * 0.5f) + (-corner * off);
 }
 
+constexpr uint32_t kTraceClaim = 64;
+
 // Lane refill for the persistent trace kernels: a wave keeps a reservoir of
 // queue entries (claimed 64 at a time, one atomic) and hands them to idle
 // lanes between traversal steps, so that lanes whose ray finished early trace
@@ -285,13 +273,13 @@ __device__ __forceinline__ uint32_t refill_slot(bool idle, uint32_t* head, uint3
     const uint64_t im = __ballot(idle);
     if (!im) return kDeadPath;
     if (res_left == 0) {
-        const uint32_t base = wave_claim(head);
+        const uint32_t base = wave_claim(head, kTraceClaim);
         if (base >= n) {
             drained = true;
             return kDeadPath;
         }
         res_base = base;
-        res_left = min(64u, n - base);
+        res_left = min(kTraceClaim, n - base);
     }
     const uint32_t take = min((uint32_t)__popcll(im), res_left);
     const uint32_t rank = (uint32_t)__popcll(im & ((1ull << lane_id()) - 1ull));
@@ -358,126 +346,171 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     }
 }
 
+// Shading of path i, found at extension-queue entry e of bounce b: emission,
+// BSDF sample, next-event estimation's shadow ray, Russian roulette and the
+// next extension ray (Integrators.cpp:130-209).  Returns bit 0 when the path
+// continues (next extension queue) and bit 1 when it has a shadow ray.
+__device__ __forceinline__ uint32_t shade_path(const DevScene& s, const DView* __restrict__ views, uint32_t b,
+                                               const Pool& P, uint32_t e, uint32_t i) {
+    bool cont = false, shadow = false;
+    {
+        const float4 h4 = P.hit[e];
+        const uint32_t inst = __float_as_uint(h4.w);
+        const uint32_t key = P.key[i];
+        const DView& V = views[key >> kViewShift];
+        const f3 dir = xyz(P.ray_d[i]);
+        float4 r4 = P.rad[i];
+        f3 rad = xyz(r4);
+        const float4 t4 = P.thr[i];
+        f3 thr = xyz(t4);
+        IorStack ist;
+        ist.count = __float_as_uint(t4.w);
+        if (inst == 0xffffffffu) {
+            const f3 sky = sky_radiance(s, dir);
+            if (b == 0) {
+                rad = sky;  // Integrators.cpp:152 (no NaN rule on this path)
+                P.ray_o[i].w = 1.0f;
+            } else {
+                rad = rad + thr * sky;  // Integrators.cpp:208
+            }
+        } else if (b < V.max_depth) {
+            Hit h;
+            h.t = h4.x; h.u = h4.y; h.v = h4.z; h.inst = inst; h.prim = P.hprim[e];
+            const Frag f = s.inst[inst].blas_root >= kImplicitRoot ? implicit_frag(s, h, xyz(P.ray_o[i]), dir)
+                                                                   : make_frag(s, h);
+            const DInstance& I = s.inst[inst];
+            const uint32_t kind = I.bsdf;
+            const DMaterial& M = s.mats[I.material];
+            const uint4 rs = P.rng[i];
+            Rng rng;
+            rng.set_streams(key & kPixMask);
+            rng.hs = (uint64_t)rs.x | ((uint64_t)rs.y << 32);
+            rng.ms = (uint64_t)rs.z | ((uint64_t)rs.w << 32);
+            const bool btdf = kind == PICO_BSDF_TRANSPARENT || kind == PICO_BSDF_FRESNEL;
+            BSample smp;
+            if (btdf) {
+                const float4 iv = P.ior[i];
+                ist.v[0] = iv.x; ist.v[1] = iv.y; ist.v[2] = iv.z; ist.v[3] = iv.w;
+                smp = kind == PICO_BSDF_TRANSPARENT ? btdf_transparent(s.microfacet, M, f.n, dir, rng, ist)
+                                                    : btdf_fresnel(s.microfacet, M, f.n, dir, rng, ist);
+            } else {
+                smp = bsdf_sample(kind, s.microfacet, M, f.n, dir, rng);
+            }
+            if (kind == PICO_BSDF_LIGHT) rad = rad + thr * smp.energy;  // Integrators.cpp:164-168
+            // next-event estimation, Integrators.cpp:46-128 (none at lights and BTDFs)
+            if (kind != PICO_BSDF_LIGHT && !btdf) {
+                const float u = rng.uniform();
+                // the sun, when there is one, is light n_lights (Integrators.cpp:55-62)
+                const uint32_t count = s.n_lights + s.use_sun;
+                const uint32_t pick = (uint32_t)(u * (float)count);
+                if (s.use_sun && pick >= s.n_lights) {
+                    // Integrators.cpp:61-77: lit when the ray towards the sun meets
+                    // nothing; no facing test and no length bound
+                    const f3 to_sun = mk(s.sun_to[0], s.sun_to[1], s.sun_to[2]);
+                    const float pdf = direct_pdf(kind, s.microfacet, M, f.n, -dir, to_sun);
+                    P.sh_o[i] = f4(f.p + 0.01f * to_sun, 0.0f);
+                    P.sh_d[i] = f4(to_sun, __int_as_float(0x7f800000));
+                    P.sh_tp[i] = f4(thr * pdf, 1.0f);
+                    P.sh_en[i] = f4(smp.energy, 0.0f);
+                    shadow = true;
+                    r4.w += 1.0f;
+                } else if (s.n_lights) {
+                    // the reference only decrements its count when u * count
+                    // rounds up to count; the index is clamped here
+                    const uint32_t li = pick >= s.n_lights ? s.n_lights - 1 : pick;
+                    const DLight L = s.lights[li];
+                    const f3 sp = L.n_tris ? mesh_light_point(s, L, rng) : implicit_light_point(L.tri_offset, rng);
+                    const float* LM = s.xf[L.instance].m;
+                    const f3 wp = mk((LM[0] * sp.x + LM[4] * sp.y) + (LM[8] * sp.z + LM[12]),
+                                     (LM[1] * sp.x + LM[5] * sp.y) + (LM[9] * sp.z + LM[13]),
+                                     (LM[2] * sp.x + LM[6] * sp.y) + (LM[10] * sp.z + LM[14]));
+                    const f3 to_light = normalize(wp - f.p);
+                    if (!(dot(to_light, f.n) < 0.0f)) {
+                        const float pdf = direct_pdf(kind, s.microfacet, M, f.n, -dir, to_light);
+                        // the shadow ray's closest hit lies before the sampled
+                        // light point (the light itself, or a blocker), so the
+                        // query stops a margin past it
+                        const f3 dl = wp - f.p;
+                        P.sh_o[i] = f4(f.p + 0.01f * to_light, 0.0f);
+                        P.sh_d[i] = f4(to_light, 1.01f * sqrtf(dot(dl, dl)) + 1e-3f);
+                        P.sh_tp[i] = f4(thr * pdf, 0.0f);  // w = 0: a mesh or shape light
+                        P.sh_en[i] = f4(smp.energy, 0.0f);
+                        shadow = true;
+                        r4.w += 1.0f;
+                    }
+                }
+            }
+            if (smp.P != 0.0f) {
+                // Russian roulette then throughput update, Integrators.cpp:185-197
+                const float qk = smax(smax(thr.x, thr.y), thr.z);
+                if (!(rng.uniform() > qk)) {
+                    thr = thr * (1.0f / qk);
+                    thr = thr * (smp.P * smp.energy);
+                    // Integrators.cpp:199: below the surface while inside geometry
+                    P.ray_o[i] = f4(f.p + 0.01f * (ist.inside() ? -f.n : f.n), 0.0f);
+                    P.ray_d[i] = f4(smp.L, 0.0f);
+                    P.thr[i] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(ist.count));
+                    if (btdf) P.ior[i] = make_float4(ist.v[0], ist.v[1], ist.v[2], ist.v[3]);
+                    cont = true;
+                    r4.w += 1.0f;
+                }
+            }
+            P.rng[i] = make_uint4((uint32_t)rng.hs, (uint32_t)(rng.hs >> 32), (uint32_t)rng.ms, (uint32_t)(rng.ms >> 32));
+        }
+        P.rad[i] = make_float4(rad.x, rad.y, rad.z, r4.w);
+    }
+    return (cont ? 1u : 0u) | (shadow ? 2u : 0u);
+}
+
+// Appends, with one atomic on `counter`, the queue entries q[base + 64 k +
+// lane] whose bit k is set in the lane's `bits` (k < nb), in batch then lane
+// order.  Every lane of the wave calls it.
+__device__ __forceinline__ void append_batches(uint32_t* counter, uint32_t bits, uint32_t nb, uint32_t* __restrict__ out,
+                                               const uint32_t* __restrict__ q, uint32_t base) {
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < nb; ++k) tot += (uint32_t)__popcll(__ballot((bits >> k) & 1u));
+    if (tot == 0) return;
+    uint32_t o = 0;
+    if (lane_id() == 0) o = atomicAdd(counter, tot);
+    o = __shfl(o, 0);
+    const uint64_t below = (1ull << lane_id()) - 1ull;
+    for (uint32_t k = 0; k < nb; ++k) {
+        const uint64_t m = __ballot((bits >> k) & 1u);
+        if ((bits >> k) & 1u) out[o + (uint32_t)__popcll(m & below)] = q[base + 64u * k + lane_id()];
+        o += (uint32_t)__popcll(m);
+    }
+}
+
+// Device-scope atomics on one address serialise (about 11.5 ns each on
+// MI355X, whatever the number of waves), so with one claim and two appends per
+// 64 paths the shading pass could not exceed one batch of 64 paths per 11.5 ns
+// (5.4 G paths/s).  A wave claims up to kShadeBatches batches with one atomic
+// and appends the survivors of all of them with one atomic per queue.  Short
+// queues (the last bounces) take one batch per claim, so that their few
+// entries spread over as many waves as possible.
+constexpr uint32_t kShadeBatches = 4;
+
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __restrict__ views, Ctl* ctl, uint32_t b, Pool P) {
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     uint32_t* qn = (b & 1) ? P.q0 : P.q1;
     const uint32_t n = ctl->ext_count[b];
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    const uint32_t nb = n / (64u * kShadeBatches) >= 2u * waves ? kShadeBatches : 1u;
     for (;;) {
-        const uint32_t base = wave_claim(&ctl->shade_head[b]);
+        const uint32_t base = wave_claim(&ctl->shade_head[b], 64u * nb);
         if (base >= n) break;
-        const uint32_t e = base + lane_id();
-        bool cont = false, shadow = false;
-        const uint32_t i = e < n ? q[e] : kDeadPath;
-        if (i != kDeadPath) {
-            const float4 h4 = P.hit[e];
-            const uint32_t inst = __float_as_uint(h4.w);
-            const uint32_t key = P.key[i];
-            const DView& V = views[key >> kViewShift];
-            const f3 dir = xyz(P.ray_d[i]);
-            float4 r4 = P.rad[i];
-            f3 rad = xyz(r4);
-            const float4 t4 = P.thr[i];
-            f3 thr = xyz(t4);
-            IorStack ist;
-            ist.count = __float_as_uint(t4.w);
-            if (inst == 0xffffffffu) {
-                const f3 sky = sky_radiance(s, dir);
-                if (b == 0) {
-                    rad = sky;  // Integrators.cpp:152 (no NaN rule on this path)
-                    P.ray_o[i].w = 1.0f;
-                } else {
-                    rad = rad + thr * sky;  // Integrators.cpp:208
-                }
-            } else if (b < V.max_depth) {
-                Hit h;
-                h.t = h4.x; h.u = h4.y; h.v = h4.z; h.inst = inst; h.prim = P.hprim[e];
-                const Frag f = s.inst[inst].blas_root >= kImplicitRoot ? implicit_frag(s, h, xyz(P.ray_o[i]), dir)
-                                                                       : make_frag(s, h);
-                const DInstance& I = s.inst[inst];
-                const uint32_t kind = I.bsdf;
-                const DMaterial& M = s.mats[I.material];
-                const uint4 rs = P.rng[i];
-                Rng rng;
-                rng.set_streams(key & kPixMask);
-                rng.hs = (uint64_t)rs.x | ((uint64_t)rs.y << 32);
-                rng.ms = (uint64_t)rs.z | ((uint64_t)rs.w << 32);
-                const bool btdf = kind == PICO_BSDF_TRANSPARENT || kind == PICO_BSDF_FRESNEL;
-                BSample smp;
-                if (btdf) {
-                    const float4 iv = P.ior[i];
-                    ist.v[0] = iv.x; ist.v[1] = iv.y; ist.v[2] = iv.z; ist.v[3] = iv.w;
-                    smp = kind == PICO_BSDF_TRANSPARENT ? btdf_transparent(s.microfacet, M, f.n, dir, rng, ist)
-                                                        : btdf_fresnel(s.microfacet, M, f.n, dir, rng, ist);
-                } else {
-                    smp = bsdf_sample(kind, s.microfacet, M, f.n, dir, rng);
-                }
-                if (kind == PICO_BSDF_LIGHT) rad = rad + thr * smp.energy;  // Integrators.cpp:164-168
-                // next-event estimation, Integrators.cpp:46-128 (none at lights and BTDFs)
-                if (kind != PICO_BSDF_LIGHT && !btdf) {
-                    const float u = rng.uniform();
-                    // the sun, when there is one, is light n_lights (Integrators.cpp:55-62)
-                    const uint32_t count = s.n_lights + s.use_sun;
-                    const uint32_t pick = (uint32_t)(u * (float)count);
-                    if (s.use_sun && pick >= s.n_lights) {
-                        // Integrators.cpp:61-77: lit when the ray towards the sun meets
-                        // nothing; no facing test and no length bound
-                        const f3 to_sun = mk(s.sun_to[0], s.sun_to[1], s.sun_to[2]);
-                        const float pdf = direct_pdf(kind, s.microfacet, M, f.n, -dir, to_sun);
-                        P.sh_o[i] = f4(f.p + 0.01f * to_sun, 0.0f);
-                        P.sh_d[i] = f4(to_sun, __int_as_float(0x7f800000));
-                        P.sh_tp[i] = f4(thr * pdf, 1.0f);
-                        P.sh_en[i] = f4(smp.energy, 0.0f);
-                        shadow = true;
-                        r4.w += 1.0f;
-                    } else if (s.n_lights) {
-                        // the reference only decrements its count when u * count
-                        // rounds up to count; the index is clamped here
-                        const uint32_t li = pick >= s.n_lights ? s.n_lights - 1 : pick;
-                        const DLight L = s.lights[li];
-                        const f3 sp = L.n_tris ? mesh_light_point(s, L, rng) : implicit_light_point(L.tri_offset, rng);
-                        const float* LM = s.xf[L.instance].m;
-                        const f3 wp = mk((LM[0] * sp.x + LM[4] * sp.y) + (LM[8] * sp.z + LM[12]),
-                                         (LM[1] * sp.x + LM[5] * sp.y) + (LM[9] * sp.z + LM[13]),
-                                         (LM[2] * sp.x + LM[6] * sp.y) + (LM[10] * sp.z + LM[14]));
-                        const f3 to_light = normalize(wp - f.p);
-                        if (!(dot(to_light, f.n) < 0.0f)) {
-                            const float pdf = direct_pdf(kind, s.microfacet, M, f.n, -dir, to_light);
-                            // the shadow ray's closest hit lies before the sampled
-                            // light point (the light itself, or a blocker), so the
-                            // query stops a margin past it
-                            const f3 dl = wp - f.p;
-                            P.sh_o[i] = f4(f.p + 0.01f * to_light, 0.0f);
-                            P.sh_d[i] = f4(to_light, 1.01f * sqrtf(dot(dl, dl)) + 1e-3f);
-                            P.sh_tp[i] = f4(thr * pdf, 0.0f);  // w = 0: a mesh or shape light
-                            P.sh_en[i] = f4(smp.energy, 0.0f);
-                            shadow = true;
-                            r4.w += 1.0f;
-                        }
-                    }
-                }
-                if (smp.P != 0.0f) {
-                    // Russian roulette then throughput update, Integrators.cpp:185-197
-                    const float qk = smax(smax(thr.x, thr.y), thr.z);
-                    if (!(rng.uniform() > qk)) {
-                        thr = thr * (1.0f / qk);
-                        thr = thr * (smp.P * smp.energy);
-                        // Integrators.cpp:199: below the surface while inside geometry
-                        P.ray_o[i] = f4(f.p + 0.01f * (ist.inside() ? -f.n : f.n), 0.0f);
-                        P.ray_d[i] = f4(smp.L, 0.0f);
-                        P.thr[i] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(ist.count));
-                        if (btdf) P.ior[i] = make_float4(ist.v[0], ist.v[1], ist.v[2], ist.v[3]);
-                        cont = true;
-                        r4.w += 1.0f;
-                    }
-                }
-                P.rng[i] = make_uint4((uint32_t)rng.hs, (uint32_t)(rng.hs >> 32), (uint32_t)rng.ms, (uint32_t)(rng.ms >> 32));
+        uint32_t cbits = 0, sbits = 0;
+        for (uint32_t k = 0; k < nb; ++k) {
+            const uint32_t e = base + 64u * k + lane_id();
+            const uint32_t i = e < n ? q[e] : kDeadPath;
+            if (i != kDeadPath) {
+                const uint32_t r = shade_path(s, views, b, P, e, i);
+                cbits |= (r & 1u) << k;
+                sbits |= (r >> 1) << k;
             }
-            P.rad[i] = make_float4(rad.x, rad.y, rad.z, r4.w);
         }
-        const uint32_t s1 = wave_append(&ctl->ext_count[b + 1], cont);
-        if (cont) qn[s1] = i;
-        const uint32_t s2 = wave_append(&ctl->sh_count[b], shadow);
-        if (shadow) P.shq[s2] = i;
+        append_batches(&ctl->ext_count[b + 1], cbits, nb, qn, q, base);
+        append_batches(&ctl->sh_count[b], sbits, nb, P.shq, q, base);
     }
 }
 
