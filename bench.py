@@ -9,7 +9,8 @@ reference's running mean / variance with its early-out).  The scene is built
 and uploaded before timing.
 
 Printed: one JSON line with Mrays/s (rays of the accumulated samples / wall
-time of the timed steps), the roofline of the traversal kernel (`extend`:
+time of the timed steps), the roofline of the traversal kernel (`trace`, the
+extension rays of a bounce with the shadow rays of the bounce before it:
 algorithmic bytes from a counting run over its HIP-event launch time, and its
 HBM traffic from two rocprofv3 counter passes in this run), the reference CPU
 renderer's Mrays/s on every logical CPU of the host, and the quality leg:
@@ -44,13 +45,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traversal_bytes(st, shadow=False):
-    if shadow:
-        rays = st.shadow_rays
-        return rays * B_RAY + st.shadow_node_visits * B_NODE + st.shadow_tri_tests * B_TRI + \
-            st.shadow_instance_visits * B_INST
-    rays = st.camera_rays + st.extension_rays
-    return rays * B_RAY + st.node_visits * B_NODE + st.tri_tests * B_TRI + st.instance_visits * B_INST
+def traversal_bytes(st):
+    """Algorithmic bytes of every closest-hit query of a counting run: camera,
+    extension and shadow rays."""
+    rays = st.camera_rays + st.extension_rays + st.shadow_rays
+    return rays * B_RAY + (st.node_visits + st.shadow_node_visits) * B_NODE + \
+        (st.tri_tests + st.shadow_tri_tests) * B_TRI + (st.instance_visits + st.shadow_instance_visits) * B_INST
 
 
 def cpu_baseline(scene, cam, w, h, spp, depth, seed):
@@ -592,7 +592,7 @@ def _pmc_pass(counter, args):
         raise RuntimeError("rocprofv3 not found")
     tmp = tempfile.mkdtemp(prefix="pico_pmc_", dir="/tmp")
     try:
-        cmd = [prof, "--pmc", counter, "--kernel-include-regex", "k_extend|k_shadow|k_copy", "--output-format", "csv",
+        cmd = [prof, "--pmc", counter, "--kernel-include-regex", "k_trace|k_copy", "--output-format", "csv",
                "-d", tmp, "-o", "run", "--", sys.executable, os.path.join(ROOT, "bench.py"), "--traffic-child",
                "--width", str(args.width), "--height", str(args.height), "--spp", str(args.spp), "--depth",
                str(args.depth), "--copy-mb", str(args.copy_mb)]
@@ -613,18 +613,18 @@ def _pmc_pass(counter, args):
                     per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
         out = {}
         for (name, _), v in per.items():
-            for short in ("k_extend", "k_shadow", "k_copy"):
+            for short in ("k_trace", "k_copy"):
                 if "::" + short + "<" in name or "::" + short + "(" in name:
                     out.setdefault(short, []).append(v)
-        if "k_copy" not in out or "k_extend" not in out:
-            raise RuntimeError(f"{counter}: dispatches of k_copy and k_extend expected, found {sorted(seen)[:8]}")
+        if "k_copy" not in out or "k_trace" not in out:
+            raise RuntimeError(f"{counter}: dispatches of k_copy and k_trace expected, found {sorted(seen)[:8]}")
         return {k: (len(v), sum(v) / len(v)) for k, v in out.items()}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
 
 def traffic_leg(args):
-    """HBM bytes per dispatch of the traversal kernels, measured in this run:
+    """HBM bytes per dispatch of the traversal kernel, measured in this run:
     FETCH_SIZE and WRITE_SIZE (KiB per dispatch) from two rocprofv3 counter
     passes over one frame, each scaled by what the same counter reads for the
     streaming copy of known size."""
@@ -638,7 +638,7 @@ def traffic_leg(args):
     out = {"fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
            "calibration": f"k_copy of {args.copy_mb} MiB read + written: FETCH_SIZE "
                           f"{res['FETCH_SIZE']['k_copy'][1]:.0f} KiB, WRITE_SIZE {res['WRITE_SIZE']['k_copy'][1]:.0f} KiB"}
-    for k in ("k_extend", "k_shadow"):
+    for k in ("k_trace",):
         if k in res["FETCH_SIZE"] and k in res["WRITE_SIZE"]:
             fk, wk = res["FETCH_SIZE"][k][1], res["WRITE_SIZE"][k][1]
             out[k] = {"dispatches": res["FETCH_SIZE"][k][0], "fetch_kib_per_dispatch": round(fk, 1),
@@ -736,22 +736,21 @@ def main():
         log(f"[bench] step {i}: {st.render_ms:.1f} ms device")
     ctx.sync()
     dt = time.perf_counter() - t0
-    ext_n, ext_ms = ctx.kernel_time("extend")
-    sh_n, sh_ms = ctx.kernel_time("shadow")
+    tr_n, tr_ms = ctx.kernel_time("trace")
     ctx.enable_kernel_timing(False)
     ms_step = dt * 1e3 / args.steps
     mrays = rays / dt / 1e6
 
     roof = None
-    if count is not None and ext_n:
-        launches_per_frame = ext_n / args.steps
+    if count is not None and tr_n:
+        launches_per_frame = tr_n / args.steps
         bytes_per_launch = traversal_bytes(count) / launches_per_frame
-        achieved = bytes_per_launch / (ext_ms * 1e-3) / 1e9
+        achieved = bytes_per_launch / (tr_ms * 1e-3) / 1e9
         traffic, tdetail = None, None
         if not args.no_traffic:
             try:
                 tdetail = traffic_leg(args)
-                traffic = tdetail["k_extend"]["hbm_bytes_per_dispatch"]
+                traffic = tdetail["k_trace"]["hbm_bytes_per_dispatch"]
                 log(f"[bench] traffic {tdetail}")
             except Exception as e:  # no profiler on this host: no traffic figure
                 tdetail = {"error": str(e)[:300]}
@@ -759,13 +758,11 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "traffic_detail": tdetail,
                 "measured_copy_GBps": round(copy_gbps, 1),
-                "kernel": "extend", "launches": ext_n, "mean_launch_ms": round(ext_ms, 4),
+                "kernel": "trace", "launches": tr_n, "mean_launch_ms": round(tr_ms, 4),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch),
-                "bytes_per_ray": round(traversal_bytes(count) / max(1, count.camera_rays + count.extension_rays), 1),
-                "stack_spill_pushes_per_frame": count.spill_pushes, "stack_spill_pops_per_frame": count.spill_pops,
-                "shadow_kernel_mean_launch_ms": round(sh_ms, 4),
-                "shadow_achieved_GBps": round(traversal_bytes(count, True) / (sh_n / args.steps) / (sh_ms * 1e-3) / 1e9, 1)
-                if sh_n else None}
+                "bytes_per_ray": round(traversal_bytes(count) /
+                                       max(1, count.camera_rays + count.extension_rays + count.shadow_rays), 1),
+                "stack_spill_pushes_per_frame": count.spill_pushes, "stack_spill_pops_per_frame": count.spill_pops}
 
     quality = None
     if not args.no_quality:

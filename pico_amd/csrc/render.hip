@@ -6,9 +6,10 @@
 // item and S samples of it, i.e. P = n_active * S paths, and runs them
 // through persistent kernels that pull work from device-resident queues:
 //   generate  camera rays, per-(pixel, sample) PCG32 streams   -> extend queue 0
-//   extend    closest hit (TLAS -> BLAS)                        -> hit records
-//   shade     emission, BSDF sample, NEE light sample, RR       -> shadow queue, extend queue b+1
-//   shadow    closest hit of the NEE ray; light -> add radiance
+//   trace     closest hits (TLAS -> BLAS) of the extension rays of bounce b
+//             -> hit records, and of the shadow rays of bounce b - 1 -> a light
+//             hit adds radiance
+//   shade     emission, BSDF sample, NEE light sample, RR       -> shadow queue b, extend queue b+1
 //   finalize  per-path radiance (NaN -> pink), ray count
 //   accumulate  the reference's running mean / variance and early-out, in sample order
 //   compact   surviving items -> next chunk's active list
@@ -65,7 +66,6 @@ struct Ctl {
     uint32_t ext_head[kMaxBounce + 2];
     uint32_t shade_head[kMaxBounce + 2];
     uint32_t sh_count[kMaxBounce + 2];
-    uint32_t sh_head[kMaxBounce + 2];
     unsigned long long rays_cam, rays_ext, rays_sh, rays_acc, samples_acc;
     unsigned long long nodes, tris, insts;           // extend (camera + extension rays)
     unsigned long long nodes_sh, tris_sh, insts_sh;  // shadow rays
@@ -131,7 +131,7 @@ __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
     __syncthreads();
     for (uint32_t b = t; b < kMaxBounce + 2; b += blockDim.x) {
         ctl->ext_count[b] = 0; ctl->ext_head[b] = 0; ctl->shade_head[b] = 0;
-        ctl->sh_count[b] = 0; ctl->sh_head[b] = 0;
+        ctl->sh_count[b] = 0;
     }
     __syncthreads();
     if (t == 0) {
@@ -299,15 +299,25 @@ __device__ __forceinline__ bool refill_due(bool idle) {
     return (uint32_t)__popcll(im) >= kRefillMin || im == __ballot(true);
 }
 
+// The closest-hit queries of bounce b in one launch: the extension rays of
+// bounce b (entries [0, n_ext) of the combined queue) and the shadow rays that
+// the shading of bounce b - 1 made (entries [n_ext, n_ext + n_sh)).  Neither
+// kind reads what the other writes (an extension ray reads its path's ray and
+// writes a hit record, a shadow ray reads its shadow ray and adds to its
+// path's radiance), so a bounce pays one tail of slow rays instead of two.
+// Extension rays come first: unbounded, they are the longer ones.
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
     __shared__ uint32_t stk[kTraceLds * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
-    const uint32_t n = ctl->ext_count[b];
-    TraceCounters tc{0, 0, 0, 0, 0};
+    const uint32_t n_ext = ctl->ext_count[b];
+    const uint32_t n = n_ext + (b ? ctl->sh_count[b - 1] : 0u);
+    TraceCounters tc{0, 0, 0, 0, 0}, tce{0, 0, 0, 0, 0}, tcs{0, 0, 0, 0, 0};
     TraceState t;
+    // slot: the extension ray's queue entry (its hit record goes there), or
+    // kDeadPath for a shadow ray
     uint32_t item = kDeadPath, slot = 0, res_base = 0, res_left = 0;
     bool drained = false;
     for (;;) {
@@ -318,31 +328,64 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             if (!refill_due(item == kDeadPath)) break;
             const uint32_t e = refill_slot(item == kDeadPath, &ctl->ext_head[b], n, res_base, res_left, drained);
             if (e != kDeadPath) {
-                const uint32_t i = q[e];
+                const bool sh = e >= n_ext;
+                const uint32_t i = sh ? P.shq[e - n_ext] : q[e];
                 if (i != kDeadPath) {
+                    const float4 o4 = (sh ? P.sh_o : P.ray_o)[i];
+                    const float4 d4 = (sh ? P.sh_d : P.ray_d)[i];
                     item = i;
-                    slot = e;
-                    trace_begin(s, t, st, xyz(P.ray_o[i]), xyz(P.ray_d[i]));
+                    slot = sh ? kDeadPath : e;
+                    trace_begin(s, t, st, xyz(o4), xyz(d4), sh ? d4.w : INFINITY);
                 }
             }
             if (drained || !__ballot(item == kDeadPath)) break;
         }
         if (!__ballot(item != kDeadPath)) break;
         if (item != kDeadPath && trace_step<kTraceLds, COUNT>(s, t, st, &tc, &ctl->overflow)) {
-            // hit records in queue order: the lanes of a wave hold neighbouring
-            // queue entries, so their records share cache lines (the shading
-            // pass reads them in the same order)
-            P.hit[slot] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.inst));
-            P.hprim[slot] = t.h.prim;
+            if (slot != kDeadPath) {
+                // hit records in queue order: the lanes of a wave hold neighbouring
+                // queue entries, so their records share cache lines (the shading
+                // pass reads them in the same order)
+                P.hit[slot] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.inst));
+                P.hprim[slot] = t.h.prim;
+            } else {
+                const float4 tp = P.sh_tp[item];
+                bool lit = false;
+                f3 em;
+                if (tp.w != 0.0f) {
+                    // towards the sun: lit when the ray met nothing
+                    lit = t.h.inst == 0xffffffffu;
+                    em = mk(s.sun_col[0], s.sun_col[1], s.sun_col[2]);
+                } else if (t.h.inst != 0xffffffffu) {
+                    // towards a light: lit when the closest hit is any light
+                    const DInstance& I = s.inst[t.h.inst];
+                    lit = I.bsdf == PICO_BSDF_LIGHT;
+                    if (lit) em = ld3(s.mats[I.material].emissive);
+                }
+                if (lit) {
+                    const float4 r = P.rad[item];
+                    const f3 add = (xyz(tp) * em) * xyz(P.sh_en[item]);
+                    P.rad[item] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
+                }
+            }
+            if (COUNT) {
+                TraceCounters& d = slot != kDeadPath ? tce : tcs;
+                d.nodes += tc.nodes; d.tris += tc.tris; d.insts += tc.insts;
+                d.spill_push += tc.spill_push; d.spill_pop += tc.spill_pop;
+                tc = TraceCounters{0, 0, 0, 0, 0};
+            }
             item = kDeadPath;
         }
     }
     if (COUNT) {
-        atomicAdd(&ctl->nodes, (unsigned long long)tc.nodes);
-        atomicAdd(&ctl->tris, (unsigned long long)tc.tris);
-        atomicAdd(&ctl->insts, (unsigned long long)tc.insts);
-        atomicAdd(&ctl->spill_push, (unsigned long long)tc.spill_push);
-        atomicAdd(&ctl->spill_pop, (unsigned long long)tc.spill_pop);
+        atomicAdd(&ctl->nodes, (unsigned long long)tce.nodes);
+        atomicAdd(&ctl->tris, (unsigned long long)tce.tris);
+        atomicAdd(&ctl->insts, (unsigned long long)tce.insts);
+        atomicAdd(&ctl->nodes_sh, (unsigned long long)tcs.nodes);
+        atomicAdd(&ctl->tris_sh, (unsigned long long)tcs.tris);
+        atomicAdd(&ctl->insts_sh, (unsigned long long)tcs.insts);
+        atomicAdd(&ctl->spill_push, (unsigned long long)(tce.spill_push + tcs.spill_push));
+        atomicAdd(&ctl->spill_pop, (unsigned long long)(tce.spill_pop + tcs.spill_pop));
     }
 }
 
@@ -514,59 +557,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
-    __shared__ uint32_t stk[kTraceLds * kBlock];
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
-    const uint32_t n = ctl->sh_count[b];
-    TraceCounters tc{0, 0, 0, 0, 0};
-    TraceState t;
-    uint32_t item = kDeadPath, res_base = 0, res_left = 0;
-    bool drained = false;
-    for (;;) {
-        for (;;) {
-            if (!refill_due(item == kDeadPath)) break;
-            const uint32_t e = refill_slot(item == kDeadPath, &ctl->sh_head[b], n, res_base, res_left, drained);
-            if (e != kDeadPath) {
-                item = P.shq[e];
-                const float4 sd = P.sh_d[item];
-                trace_begin(s, t, st, xyz(P.sh_o[item]), xyz(sd), sd.w);
-            }
-            if (drained || !__ballot(item == kDeadPath)) break;
-        }
-        if (!__ballot(item != kDeadPath)) break;
-        if (item != kDeadPath && trace_step<kTraceLds, COUNT>(s, t, st, &tc, &ctl->overflow)) {
-            const float4 tp = P.sh_tp[item];
-            bool lit = false;
-            f3 em;
-            if (tp.w != 0.0f) {
-                // towards the sun: lit when the ray met nothing
-                lit = t.h.inst == 0xffffffffu;
-                em = mk(s.sun_col[0], s.sun_col[1], s.sun_col[2]);
-            } else if (t.h.inst != 0xffffffffu) {
-                // towards a light: lit when the closest hit is any light
-                const DInstance& I = s.inst[t.h.inst];
-                lit = I.bsdf == PICO_BSDF_LIGHT;
-                if (lit) em = ld3(s.mats[I.material].emissive);
-            }
-            if (lit) {
-                const float4 r = P.rad[item];
-                const f3 add = (xyz(tp) * em) * xyz(P.sh_en[item]);
-                P.rad[item] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
-            }
-            item = kDeadPath;
-        }
-    }
-    if (COUNT) {
-        atomicAdd(&ctl->nodes_sh, (unsigned long long)tc.nodes);
-        atomicAdd(&ctl->tris_sh, (unsigned long long)tc.tris);
-        atomicAdd(&ctl->insts_sh, (unsigned long long)tc.insts);
-        atomicAdd(&ctl->spill_push, (unsigned long long)tc.spill_push);
-        atomicAdd(&ctl->spill_pop, (unsigned long long)tc.spill_pop);
-    }
-}
-
 // Per-workgroup (rays, samples) counters of every view in LDS: items add to
 // them (one LDS atomic pair per wave when the wave is on one view), and the
 // workgroup adds its non-zero totals to the views' counters once at the end.
@@ -729,15 +719,15 @@ struct Plan {
     uint32_t max_spp = 0, max_depth = 0, total_items = 0;
 };
 
+// Bounce b: the extension rays of bounce b with the shadow rays of bounce
+// b - 1, then the shading of bounce b (which makes the extension rays of
+// bounce b + 1 and the shadow rays of bounce b).  The shading of the last
+// bounce makes no shadow rays.
 int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, const Pool& P, const DView* dviews, bool count,
-                          uint32_t grid_trace, uint32_t grid_shade, bool with_shadow, uint32_t* spill, uint32_t spill_cap) {
-    if (count) PICO_LAUNCH(c, "extend", (k_extend<true>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
-    else PICO_LAUNCH(c, "extend", (k_extend<false>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
+                          uint32_t grid_trace, uint32_t grid_shade, uint32_t* spill, uint32_t spill_cap) {
+    if (count) PICO_LAUNCH(c, "trace", (k_trace<true>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
+    else PICO_LAUNCH(c, "trace", (k_trace<false>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
     PICO_LAUNCH(c, "shade", k_shade, grid_shade, kBlock, 0, c->dev, dviews, ctl, b, P);
-    if (with_shadow) {
-        if (count) PICO_LAUNCH(c, "shadow", (k_shadow<true>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
-        else PICO_LAUNCH(c, "shadow", (k_shadow<false>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
-    }
     return PICO_OK;
 }
 
@@ -869,7 +859,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     }
     // persistent grids: as many workgroups as stay resident at once
     int occ_t = 0, occ_s = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, k_extend<false>, kBlock, 0);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, k_trace<false>, kBlock, 0);
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_shade, kBlock, 0);
     const uint32_t grid_trace = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_t, 16));
     // stack entries beyond the LDS part, per trace lane, in device memory
@@ -897,7 +887,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         PICO_LAUNCH(c, "generate", k_generate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         for (uint32_t b = 0; b <= pl.max_depth; ++b) {
             int rc;
-            rc = launch_bounce_kernels(c, ctl, b, P, dv, count, grid_trace, grid_shade, b < pl.max_depth, spill, spill_cap);
+            rc = launch_bounce_kernels(c, ctl, b, P, dv, count, grid_trace, grid_shade, spill, spill_cap);
             if (rc) return rc;
         }
         PICO_LAUNCH(c, "sum_rays", k_sum_rays, 1, 64, 0, ctl, pl.max_depth);
