@@ -54,7 +54,7 @@ struct DView {
     float* samples_out;
     uint32_t* count0;   // per-pixel sample count at render start
     uint8_t* done;      // per-pixel
-    unsigned long long* stats;  // 4 counters: rays, samples, camera, (pad)
+    unsigned long long* stats;  // kStatSlots slots of (rays, samples), kStatStride apart
 };
 
 struct Ctl {
@@ -94,35 +94,70 @@ struct Pool {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Persistent work loop helper: each wave claims `n` consecutive entries.
-__device__ __forceinline__ uint32_t wave_claim(uint32_t* head, uint32_t n) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(head, n);
-    return __shfl(base, 0);
-}
+constexpr uint32_t kDeadPath = 0xffffffffu;
 
-// Workgroup-aggregated append: one atomic per workgroup and call, slots in
-// wave then lane order.  Every thread of the workgroup must call it.
-template <int NW>
-__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred) {
-    __shared__ uint32_t wcount[NW];
+// Work distribution of the persistent kernels over the n entries of a queue,
+// `chunk` entries per claim.  Device-scope atomics on one address serialise
+// (about 11.5 ns each on MI355X), so a claim per wave at the start of a launch
+// kept a grid of 6144 waves some 70 us on its counter before the last wave
+// began, and another 70 us at the end, however short the queue.  Here a wave's
+// first batch is fixed by its index (entries [w chunk, (w + 1) chunk)) and
+// later batches are claimed with an atomic past the first batches of all
+// waves; a wave whose fixed batch lies past the end of the queue stops without
+// touching the counter, so a short queue costs atomics only from the waves
+// that had work.  (Checking the counter with an atomic load before claiming,
+// to spare the final claims, made the launches three times slower: such a
+// load bypasses L2 and waits behind the atomics on the same address.)
+// claim() returns the batch's first entry, n or more when none is left;
+// wave-uniform.  Every lane of the wave calls it.
+struct WaveClaim {
+    uint32_t* head;
+    uint32_t n, chunk, first;  // first: the wave's fixed batch, kDeadPath once taken
+    __device__ __forceinline__ WaveClaim(uint32_t* head_, uint32_t n_, uint32_t chunk_)
+        : head(head_), n(n_), chunk(chunk_), first((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * chunk_) {}
+    __device__ __forceinline__ uint32_t claim() {
+        if (first != kDeadPath) {
+            const uint32_t b = first;
+            first = kDeadPath;
+            return b;
+        }
+        const uint32_t past = gridDim.x * (blockDim.x >> 6) * chunk;
+        uint32_t base = n;
+        if (lane_id() == 0 && past < n) base = past + atomicAdd(head, chunk);
+        return __shfl(base, 0);
+    }
+};
+
+// Workgroup-aggregated, order-preserving append of R rounds of items with one
+// atomic per workgroup and call: bit r of `bits` says whether this thread's
+// item of round r is kept; pos[r] receives its slot (valid where kept).  Slots
+// follow round, then wave, then lane order.  Every thread of the workgroup
+// must call it.  (One atomic per round of a 4096-workgroup grid left a
+// compaction waiting on its counter: same-address atomics serialise.)
+template <int NW, int R>
+__device__ __forceinline__ void block_append(uint32_t* counter, uint32_t bits, uint32_t (&pos)[R]) {
+    __shared__ uint32_t cnt[R][NW];
     __shared__ uint32_t wbase;
-    const uint64_t m = __ballot(pred);
     const uint32_t wave = threadIdx.x >> 6;
-    if (lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
+    uint64_t m[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        m[r] = __ballot((bits >> r) & 1u);
+        if (lane_id() == 0) cnt[r][wave] = (uint32_t)__popcll(m[r]);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (int w = 0; w < NW; ++w) { const uint32_t c = wcount[w]; wcount[w] = tot; tot += c; }
+        for (int r = 0; r < R; ++r)
+            for (int w = 0; w < NW; ++w) { const uint32_t c = cnt[r][w]; cnt[r][w] = tot; tot += c; }
         wbase = tot ? atomicAdd(counter, tot) : 0u;
     }
     __syncthreads();
-    const uint32_t r = wbase + wcount[wave] + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
-    __syncthreads();  // the next call reuses wcount / wbase
-    return r;
+    const uint64_t below = (1ull << lane_id()) - 1ull;
+#pragma unroll
+    for (int r = 0; r < R; ++r) pos[r] = wbase + cnt[r][wave] + (uint32_t)__popcll(m[r] & below);
+    __syncthreads();  // the next call reuses cnt / wbase
 }
-
-constexpr uint32_t kDeadPath = 0xffffffffu;
 
 __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
     const uint32_t t = threadIdx.x;
@@ -149,36 +184,47 @@ __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
 // pixels in tiles of tw x th (row-major tile order, row-major inside a tile),
 // which decides which pixels neighbouring lanes trace; tw = width, th = 1 is
 // plain image order.
-__global__ void k_init_view(DView* views, uint32_t vi, uint32_t* active, Ctl* ctl, uint32_t tw, uint32_t th) {
+constexpr int kInitRounds = 8;
+
+__global__ __launch_bounds__(256) void k_init_view(DView* views, uint32_t vi, uint32_t* active, Ctl* ctl, uint32_t tw, uint32_t th) {
     const DView& V = views[vi];
     const uint32_t w = V.cam.w, h = V.cam.h;
     const uint32_t tiles_x = (w + tw - 1) / tw, tiles_y = (h + th - 1) / th, tsz = tw * th;
     const uint64_t total = (uint64_t)tiles_x * tiles_y * tsz;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t t = base + threadIdx.x;
-        bool keep = false;
-        uint32_t p = 0;
-        if (t < total) {
-            const uint32_t tile = (uint32_t)(t / tsz), r = (uint32_t)(t - (uint64_t)tile * tsz);
-            const uint32_t x = (tile % tiles_x) * tw + r % tw, y = (tile / tiles_x) * th + r / tw;
-            if (x < w && y < h) {
-                p = y * w + x;
-                uint32_t c0 = 0;
-                if (V.mode == MODE_ACCUM) {
-                    if (V.resume) {
-                        c0 = V.counts[p];
-                    } else {
-                        V.counts[p] = 0;
-                        for (int k = 0; k < 3; ++k) { V.pixels[3 * p + k] = 0.0f; V.variance[3 * p + k] = 0.0f; }
+    const uint64_t span = (uint64_t)kInitRounds * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * span; base < total; base += (uint64_t)gridDim.x * span) {
+        uint32_t bits = 0, item[kInitRounds];
+#pragma unroll
+        for (int r = 0; r < kInitRounds; ++r) {
+            const uint64_t t = base + (uint64_t)r * blockDim.x + threadIdx.x;
+            item[r] = 0;
+            if (t < total) {
+                const uint32_t tile = (uint32_t)(t / tsz), rr = (uint32_t)(t - (uint64_t)tile * tsz);
+                const uint32_t x = (tile % tiles_x) * tw + rr % tw, y = (tile / tiles_x) * th + rr / tw;
+                if (x < w && y < h) {
+                    const uint32_t p = y * w + x;
+                    uint32_t c0 = 0;
+                    if (V.mode == MODE_ACCUM) {
+                        if (V.resume) {
+                            c0 = V.counts[p];
+                        } else {
+                            V.counts[p] = 0;
+                            for (int k = 0; k < 3; ++k) { V.pixels[3 * p + k] = 0.0f; V.variance[3 * p + k] = 0.0f; }
+                        }
                     }
+                    V.count0[p] = c0;
+                    const bool keep = V.mode != MODE_ACCUM || c0 < V.spp;
+                    V.done[p] = keep ? 0 : 1;
+                    if (keep) bits |= 1u << r;
+                    item[r] = (vi << kViewShift) | p;
                 }
-                V.count0[p] = c0;
-                keep = V.mode != MODE_ACCUM || c0 < V.spp;
-                V.done[p] = keep ? 0 : 1;
             }
         }
-        const uint32_t slot = block_append<4>(&ctl->n_active, keep);
-        if (keep) active[slot] = (vi << kViewShift) | p;
+        uint32_t pos[kInitRounds];
+        block_append<4, kInitRounds>(&ctl->n_active, bits, pos);
+#pragma unroll
+        for (int r = 0; r < kInitRounds; ++r)
+            if ((bits >> r) & 1u) active[pos[r]] = item[r];
     }
 }
 
@@ -262,24 +308,24 @@ __device__ __forceinline__ f3 implicit_light_point(uint32_t shape, Rng& rng) {
 constexpr uint32_t kTraceClaim = 64;
 
 // Lane refill for the persistent trace kernels: a wave keeps a reservoir of
-// queue entries (claimed 64 at a time, one atomic) and hands them to idle
-// lanes between traversal steps, so that lanes whose ray finished early trace
-// the next one instead of waiting for the wave's slowest ray.  Returns the
-// queue position this lane takes, or kDeadPath; wave-uniform state in
-// res_base / res_left / drained.
-__device__ __forceinline__ uint32_t refill_slot(bool idle, uint32_t* head, uint32_t n, uint32_t& res_base,
-                                                uint32_t& res_left, bool& drained) {
+// queue entries (claimed kTraceClaim at a time) and hands them to idle lanes
+// between traversal steps, so that lanes whose ray finished early trace the
+// next one instead of waiting for the wave's slowest ray.  Returns the queue
+// position this lane takes, or kDeadPath; wave-uniform state in res_base /
+// res_left / drained.
+__device__ __forceinline__ uint32_t refill_slot(bool idle, WaveClaim& wc, uint32_t& res_base, uint32_t& res_left,
+                                                bool& drained) {
     if (drained) return kDeadPath;
     const uint64_t im = __ballot(idle);
     if (!im) return kDeadPath;
     if (res_left == 0) {
-        const uint32_t base = wave_claim(head, kTraceClaim);
-        if (base >= n) {
+        const uint32_t base = wc.claim();
+        if (base >= wc.n) {
             drained = true;
             return kDeadPath;
         }
         res_base = base;
-        res_left = min(kTraceClaim, n - base);
+        res_left = min(kTraceClaim, wc.n - base);
     }
     const uint32_t take = min((uint32_t)__popcll(im), res_left);
     const uint32_t rank = (uint32_t)__popcll(im & ((1ull << lane_id()) - 1ull));
@@ -320,13 +366,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     // kDeadPath for a shadow ray
     uint32_t item = kDeadPath, slot = 0, res_base = 0, res_left = 0;
     bool drained = false;
+    WaveClaim wc(&ctl->ext_head[b], n, kTraceClaim);
     for (;;) {
         // idle lanes take queue entries until none is idle or the queue is empty
         // (dead entries, samples a view does not need, leave the lane idle),
         // once at least kRefillMin lanes are idle
         for (;;) {
             if (!refill_due(item == kDeadPath)) break;
-            const uint32_t e = refill_slot(item == kDeadPath, &ctl->ext_head[b], n, res_base, res_left, drained);
+            const uint32_t e = refill_slot(item == kDeadPath, wc, res_base, res_left, drained);
             if (e != kDeadPath) {
                 const bool sh = e >= n_ext;
                 const uint32_t i = sh ? P.shq[e - n_ext] : q[e];
@@ -539,8 +586,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
     const uint32_t n = ctl->ext_count[b];
     const uint32_t waves = gridDim.x * (kBlock / 64);
     const uint32_t nb = n / (64u * kShadeBatches) >= 2u * waves ? kShadeBatches : 1u;
+    WaveClaim wc(&ctl->shade_head[b], n, 64u * nb);
     for (;;) {
-        const uint32_t base = wave_claim(&ctl->shade_head[b], 64u * nb);
+        const uint32_t base = wc.claim();
         if (base >= n) break;
         uint32_t cbits = 0, sbits = 0;
         for (uint32_t k = 0; k < nb; ++k) {
@@ -560,6 +608,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
 // Per-workgroup (rays, samples) counters of every view in LDS: items add to
 // them (one LDS atomic pair per wave when the wave is on one view), and the
 // workgroup adds its non-zero totals to the views' counters once at the end.
+// Per-view ray and sample counters: kStatSlots slots of (rays, samples),
+// kStatStride counters (256 B) apart, summed by the host.
+constexpr uint32_t kStatSlots = 16;
+constexpr uint32_t kStatStride = 32;
+
 struct BlockViewStats {
     unsigned long long v[kMaxViews][2];
     __device__ __forceinline__ void init() {
@@ -578,12 +631,17 @@ struct BlockViewStats {
             atomicAdd(&v[view][1], ns);
         }
     }
+    // Workgroup b adds to counter slot b mod kStatSlots of each view: every
+    // workgroup of a grid flushes at about the same time, and atomics on one
+    // address serialise, so one slot per view would hold the grid's last
+    // workgroups for 4096 atomics.
     __device__ __forceinline__ void flush(const DView* views) {
         __syncthreads();
+        const uint32_t slot = kStatStride * (blockIdx.x % kStatSlots);
         for (uint32_t i = threadIdx.x; i < kMaxViews; i += blockDim.x)
             if (v[i][0] | v[i][1]) {
-                atomicAdd(&views[i].stats[0], v[i][0]);
-                atomicAdd(&views[i].stats[1], v[i][1]);
+                atomicAdd(&views[i].stats[slot], v[i][0]);
+                atomicAdd(&views[i].stats[slot + 1], v[i][1]);
             }
     }
 };
@@ -688,19 +746,28 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const DView* __restrict__
     bvs.flush(views);
 }
 
+constexpr int kCompactRounds = 16;
+
 __global__ __launch_bounds__(kBlock) void k_compact(const DView* __restrict__ views, const uint32_t* __restrict__ active,
                                                     uint32_t* __restrict__ next, Ctl* ctl) {
     const uint32_t na = ctl->n_active;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < na; base += gridDim.x * blockDim.x) {
-        const uint32_t k = base + threadIdx.x;
-        bool keep = false;
-        uint32_t key = 0;
-        if (k < na) {
-            key = active[k];
-            keep = views[key >> kViewShift].done[key & kPixMask] == 0;
+    const uint32_t span = (uint32_t)kCompactRounds * kBlock;
+    for (uint32_t base = blockIdx.x * span; base < na; base += gridDim.x * span) {
+        uint32_t bits = 0, key[kCompactRounds];
+#pragma unroll
+        for (int r = 0; r < kCompactRounds; ++r) {
+            const uint32_t k = base + (uint32_t)r * kBlock + threadIdx.x;
+            key[r] = 0;
+            if (k < na) {
+                key[r] = active[k];
+                if (views[key[r] >> kViewShift].done[key[r] & kPixMask] == 0) bits |= 1u << r;
+            }
         }
-        const uint32_t slot = block_append<kBlock / 64>(&ctl->n_next, keep);
-        if (keep) next[slot] = key;
+        uint32_t pos[kCompactRounds];
+        block_append<kBlock / 64, kCompactRounds>(&ctl->n_next, bits, pos);
+#pragma unroll
+        for (int r = 0; r < kCompactRounds; ++r)
+            if ((bits >> r) & 1u) next[pos[r]] = key[r];
     }
 }
 
@@ -790,7 +857,8 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     Ctl* ctl = (Ctl*)scratch(c, "ctl", sizeof(Ctl));
     uint32_t* count0 = (uint32_t*)scratch(c, "count0", pix_total * 4 + 64);
     uint8_t* done = (uint8_t*)scratch(c, "done", pix_total + 64);
-    unsigned long long* vstats = (unsigned long long*)scratch(c, "vstats", nv * 32 + 64);
+    const size_t vstat_words = (size_t)kStatSlots * kStatStride;  // per view
+    unsigned long long* vstats = (unsigned long long*)scratch(c, "vstats", nv * vstat_words * 8 + 64);
     DView* dv = (DView*)scratch(c, "views", sizeof(DView) * nv);
     if (!pb || !act || !ctl || !count0 || !done || !vstats || !dv) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     // the follow-up's buffers, sized before the first launch so that no
@@ -829,12 +897,12 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     for (uint32_t v = 0; v < nv; ++v) {
         hv[v].count0 = count0 + off_px;
         hv[v].done = done + off_px;
-        hv[v].stats = vstats + 4 * v;
+        hv[v].stats = vstats + vstat_words * v;
         off_px += (size_t)hv[v].cam.w * hv[v].cam.h;
     }
     PICO_HIP(c, hipMemcpyAsync(dv, hv.data(), sizeof(DView) * nv, hipMemcpyHostToDevice, c->stream));
     PICO_HIP(c, hipMemsetAsync(ctl, 0, sizeof(Ctl), c->stream));
-    PICO_HIP(c, hipMemsetAsync(vstats, 0, nv * 32, c->stream));
+    PICO_HIP(c, hipMemsetAsync(vstats, 0, nv * vstat_words * 8, c->stream));
     // events: launch start/end, the end of every chunk (batched launches) and
     // each view's final output (follow-up on the second stream)
     std::vector<hipEvent_t> evs;
@@ -854,7 +922,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         const uint32_t n = hv[v].cam.w * hv[v].cam.h;
         uint32_t tw = tune.tile_w ? std::min(tune.tile_w, hv[v].cam.w) : hv[v].cam.w;
         uint32_t th = tune.tile_w ? std::max<uint32_t>(1, std::min(tune.tile_h, hv[v].cam.h)) : 1;
-        const uint32_t g = std::min<uint32_t>((n + 255) / 256, 4096);
+        const uint32_t g = std::min<uint32_t>((n + 256 * kInitRounds - 1) / (256 * kInitRounds), 4096);
         PICO_LAUNCH(c, "init_view", k_init_view, g, 256, 0, dv, v, act_a, ctl, tw, th);
     }
     // persistent grids: as many workgroups as stay resident at once
@@ -949,12 +1017,17 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     }
     PICO_HIP(c, hipEventRecord(e1, c->stream));
     Ctl hc;
-    std::vector<unsigned long long> hs(4 * nv);
+    std::vector<unsigned long long> hsl(nv * vstat_words), hs(2 * nv, 0ull);
     PICO_HIP(c, hipMemcpyAsync(&hc, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
-    PICO_HIP(c, hipMemcpyAsync(hs.data(), vstats, nv * 32, hipMemcpyDeviceToHost, c->stream));
+    PICO_HIP(c, hipMemcpyAsync(hsl.data(), vstats, nv * vstat_words * 8, hipMemcpyDeviceToHost, c->stream));
     std::vector<uint32_t> hfb(nv, 0u);
     if (fallback) PICO_HIP(c, hipMemcpyAsync(hfb.data(), fallback, 4 * nv, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
+    for (uint32_t v = 0; v < nv; ++v)
+        for (uint32_t k = 0; k < kStatSlots; ++k) {
+            hs[2 * v] += hsl[v * vstat_words + k * kStatStride];
+            hs[2 * v + 1] += hsl[v * vstat_words + k * kStatStride + 1];
+        }
     float ms = 0.0f;
     hipEventElapsedTime(&ms, e0, e1);
     if (hc.overflow) return fail(c, PICO_ERR_LIMIT, "traversal stack overflow");
@@ -981,12 +1054,12 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
                 st.spill_pushes = hc.spill_push;
                 st.spill_pops = hc.spill_pop;
                 for (uint32_t k = 0; k < nv; ++k) {
-                    st.accumulated_rays += hs[4 * k];
-                    st.samples += hs[4 * k + 1];
+                    st.accumulated_rays += hs[2 * k];
+                    st.samples += hs[2 * k + 1];
                 }
             } else {
-                st.accumulated_rays = hs[4 * v];
-                st.samples = hs[4 * v + 1];
+                st.accumulated_rays = hs[2 * v];
+                st.samples = hs[2 * v + 1];
             }
             if (per_view_times && view_render[v]) {
                 float a = 0.0f, b = 0.0f;
