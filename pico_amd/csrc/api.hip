@@ -349,7 +349,9 @@ int pico_hip_upload_scene_ex(pico_hip_ctx* c, const pico_scene_desc* d, uint32_t
     if ((rc = upload_vec(c, hs.nodes4, &s.nodes, node_cap))) return rc;
     const float* mesh_box = nullptr;
     const uint32_t* inst_mesh = nullptr;
+    const uint32_t* mesh_vrange = nullptr;
     if ((rc = upload_vec(c, hs.mesh_box, &mesh_box))) return rc;
+    if ((rc = upload_vec(c, hs.mesh_vrange, &mesh_vrange))) return rc;
     if ((rc = upload_vec(c, hs.instance_mesh, &inst_mesh))) return rc;
     if ((rc = upload_vec(c, hs.tris, &s.tris))) return rc;
     if ((rc = upload_vec(c, hs.instances, &s.inst))) return rc;
@@ -369,6 +371,7 @@ int pico_hip_upload_scene_ex(pico_hip_ctx* c, const pico_scene_desc* d, uint32_t
     c->dev = s;
     c->stack_need = need;
     c->d_mesh_box = mesh_box;
+    c->d_mesh_vrange = mesh_vrange;
     c->d_instance_mesh = inst_mesh;
     c->tlas_base = hs.tlas_base;
     c->blas_bound = hs.blas_bound;

@@ -1120,6 +1120,8 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     if (he == hipSuccess) s.alias_w = scene_upload(c, hs.alias_weight.data(), hs.alias_weight.size(), he);
     if (he == hipSuccess) s.alias_i = scene_upload(c, hs.alias_index.data(), hs.alias_index.size(), he);
     const uint32_t* inst_mesh = he == hipSuccess ? scene_upload(c, hs.instance_mesh.data(), hs.instance_mesh.size(), he) : nullptr;
+    const std::vector<uint32_t> vrange = mesh_vertex_ranges(d);
+    const uint32_t* mesh_vrange = he == hipSuccess ? scene_upload(c, vrange.data(), vrange.size(), he) : nullptr;
     if (he != hipSuccess) return fail(c, PICO_ERR_HIP, std::string("scene upload: ") + hipGetErrorString(he));
     // build inputs and work arrays
     Temp tmp;
@@ -1251,6 +1253,7 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     if (int rc = upload_sky_sun(c, hs, s)) return rc;
     c->dev = s;
     c->d_mesh_box = mesh_box;
+    c->d_mesh_vrange = mesh_vrange;
     c->d_instance_mesh = inst_mesh;
     c->tlas_base = n_blas4;
     c->blas_bound = blas_bound;

@@ -402,6 +402,16 @@ std::string scene_tables(const pico_scene_desc& d, HostScene& out) {
     return "";
 }
 
+std::vector<uint32_t> mesh_vertex_ranges(const pico_scene_desc& d) {
+    std::vector<uint32_t> r(2 * (size_t)d.n_meshes);
+    for (uint32_t m = 0; m < d.n_meshes; ++m) {
+        const bool tri = mesh_shape(d, m) == PICO_SHAPE_TRIANGLES;
+        r[2 * m] = d.mesh_vertex_offset[m];
+        r[2 * m + 1] = tri ? d.mesh_vertex_offset[m + 1] : d.mesh_vertex_offset[m];
+    }
+    return r;
+}
+
 std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out) {
     out = HostScene();
     std::string e = check_scene(d);
@@ -413,6 +423,7 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
     out.mesh_tri_offset.assign(d.mesh_triangle_offset, d.mesh_triangle_offset + d.n_meshes + 1);
     std::vector<uint32_t> mesh_root(d.n_meshes);
     std::vector<Box> mesh_box(d.n_meshes);
+    out.mesh_vrange = mesh_vertex_ranges(d);
     for (uint32_t m = 0; m < d.n_meshes; ++m) {
         const uint32_t v0 = d.mesh_vertex_offset[m], v1 = d.mesh_vertex_offset[m + 1];
         const uint32_t t0 = d.mesh_triangle_offset[m], t1 = d.mesh_triangle_offset[m + 1];
@@ -458,12 +469,20 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
     out.blas_nodes2 = (uint32_t)out.nodes.size();
     e = scene_tables(d, out);
     if (!e.empty()) return e;
-    // TLAS over the instance world boxes
+    // TLAS over the instance world boxes: a mesh instance's vertices through
+    // its transform, a built-in shape's box corners
     std::vector<float> lo, hi;
     for (uint32_t i = 0; i < d.n_instances; ++i) {
         const uint32_t m = d.instance_mesh[i];
+        const float* M = &d.transforms[16 * (size_t)i];
         Box wb;
-        instance_world_box(&d.transforms[16 * (size_t)i], mesh_box[m].lo, mesh_box[m].hi, wb.lo, wb.hi);
+        const uint32_t v0 = out.mesh_vrange[2 * m], v1 = out.mesh_vrange[2 * m + 1];
+        if (v0 < v1) {
+            for (uint32_t v = v0; v < v1; ++v) grow_world_box(M, &d.positions[3 * (size_t)v], wb.lo, wb.hi);
+            widen_world_box(wb.lo, wb.hi);
+        } else {
+            instance_world_box(M, mesh_box[m].lo, mesh_box[m].hi, wb.lo, wb.hi);
+        }
         lo.insert(lo.end(), wb.lo, wb.lo + 3);
         hi.insert(hi.end(), wb.hi, wb.hi + 3);
     }

@@ -24,6 +24,7 @@ struct HostScene {
     std::vector<uint32_t> indices;     // 3 x u32 per triangle, global vertex ids
     std::vector<uint32_t> mesh_tri_offset;
     std::vector<float> mesh_box;       // 6 x f32 per mesh: object-space vertex bounds lo, hi
+    std::vector<uint32_t> mesh_vrange; // 2 x u32 per mesh: first and end vertex (empty for the built-in shapes)
     std::vector<uint32_t> instance_mesh;
     uint32_t tlas_root = 0;            // DNode4 index
     uint32_t tlas_base = 0;            // first TLAS DNode4 (every BLAS node comes before it)
@@ -59,6 +60,8 @@ uint32_t collapse_bvh4(const std::vector<DNode>& n2, uint32_t root2, std::vector
 
 // Validates the description and builds everything; returns an error text or "".
 std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out);
+// per mesh (first, end) vertex of its triangles; an empty range for a built-in shape
+std::vector<uint32_t> mesh_vertex_ranges(const pico_scene_desc& d);
 
 // The parts of build_scene that do not touch the triangles: the structural
 // checks of the description, and the materials, instances (inverses, no BLAS

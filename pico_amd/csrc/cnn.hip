@@ -800,9 +800,11 @@ __global__ void k_kpn_apply(const float* __restrict__ logits, const float* __res
     }
 }
 
-// relMSE loss (FP64 reduction) and its gradient with respect to the output.
-__global__ void k_loss(const float* __restrict__ out, const float* __restrict__ tgt, float* __restrict__ dout, size_t n3,
-                       double inv_count, double* loss) {
+// relMSE loss (FP64 reduction, one atomic per workgroup: atomics on one
+// address serialise) and its gradient with respect to the output.
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ out, const float* __restrict__ tgt,
+                                              float* __restrict__ dout, size_t n3, double inv_count, double* loss) {
+    __shared__ double wsum[4];
     double acc = 0.0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
         const double r = tgt[i], d = (double)out[i] - r, den = r * r + 1e-2;
@@ -810,7 +812,9 @@ __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ 
         dout[i] = (float)(2.0 * d / den * inv_count);
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, acc);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]));
 }
 
 // backward of the kernel application and softmax -> bf16 logit gradient [Pp][32]
@@ -854,12 +858,16 @@ __global__ void k_relayout(const float* __restrict__ w, const float* __restrict_
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npad; i += gridDim.x * blockDim.x) bpad[i] = i < cout ? bias[i] : 0.0f;
 }
 
-// squared L2 norm of the gradient (FP64 reduction)
-__global__ void k_sumsq(const float* __restrict__ g, int n, double* out) {
+// squared L2 norm of the gradient (FP64 reduction, one atomic per workgroup:
+// atomics on one address serialise)
+__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ g, int n, double* out) {
+    __shared__ double wsum[4];
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += (double)g[i] * g[i];
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]));
 }
 
 // Adam on the gradient scaled to a global L2 norm of at most `clip` (0: no clipping)
@@ -1108,7 +1116,8 @@ int backward(pico_cnn* n, const float* d_in, const float* d_tgt, int B, int H, i
     const size_t n3 = (size_t)B * H * W * 3;
     PICO_LAUNCH(n->c, "cnn_zero", k_zero_f32, grid_for(n->nparams), 256, 0, n->grads, n->nparams);
     PICO_LAUNCH(n->c, "cnn_zero", k_zero1, 1, 1, 0, n->loss);
-    PICO_LAUNCH(n->c, "cnn_loss", k_loss, grid_for(n3), 256, 0, n->out, d_tgt, n->dout, n3, 1.0 / (double)n3, n->loss);
+    PICO_LAUNCH(n->c, "cnn_loss", k_loss, std::min<uint32_t>(grid_for(n3), 1024), 256, 0, n->out, d_tgt, n->dout, n3,
+                1.0 / (double)n3, n->loss);
     PICO_LAUNCH(n->c, "cnn_kpn_bwd", k_kpn_bwd, grid_for(P0), 256, 0, n->dout, n->wk, d_in, n->g_head, B, H, W, Hp, Wp);
     // head
     if ((rc = conv_dw(n, 10, n->d0b, 32, 0, n->g_head, 32, B, Hp, Wp))) return rc;
@@ -1428,7 +1437,7 @@ int pico_hip_cnn_train_step_d(pico_cnn* n, const float* d_in, const float* d_tgt
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
     PICO_LAUNCH(n->c, "cnn_zero", k_zero1, 1, 1, 0, n->gnorm2);
-    PICO_LAUNCH(n->c, "cnn_sumsq", k_sumsq, grid_for(n->nparams), 256, 0, n->grads, (int)n->nparams, n->gnorm2);
+    PICO_LAUNCH(n->c, "cnn_sumsq", k_sumsq, std::min<uint32_t>(grid_for(n->nparams), 512), 256, 0, n->grads, (int)n->nparams, n->gnorm2);
     PICO_LAUNCH(n->c, "cnn_adam", k_adam, grid_for(n->nparams), 256, 0, n->params, n->grads, n->m, n->v, (int)n->nparams, lr,
                 b1, b2, c1, c2, 1e-8f, n->gnorm2, n->clip);
     if ((rc = relayout(n))) return rc;

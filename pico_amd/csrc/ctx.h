@@ -52,6 +52,7 @@ struct pico_hip_ctx {
     // room for the largest TLAS a rebuild can emit after tlas_base
     const uint32_t* d_instance_mesh = nullptr;
     const float* d_mesh_box = nullptr;
+    const uint32_t* d_mesh_vrange = nullptr;  // per mesh (first, end) vertex in dev.pos; empty for built-in shapes
     uint32_t tlas_base = 0;
     uint32_t blas_bound = 0;
     // reported by pico_hip_scene_info (either builder)

@@ -292,14 +292,23 @@ __global__ __launch_bounds__(256) void k_atrous_fixed(const float* __restrict__ 
     }
 }
 
-__global__ void k_lum_max(const float* __restrict__ rgb, uint32_t n, uint32_t* white_bits) {
+// Largest luminance, one atomic per workgroup: atomics on one address
+// serialise, so a grid-sized number of them (one per wave) cost more than
+// reading the frame.
+__global__ __launch_bounds__(256) void k_lum_max(const float* __restrict__ rgb, uint32_t n, uint32_t* white_bits) {
+    __shared__ float wm[4];
     float m = 0.0f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float l = rgb[3 * i] * 0.2126f + rgb[3 * i + 1] * 0.587f + rgb[3 * i + 2] * 0.114f;
         m = smax(m, l);
     }
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(white_bits, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float b = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+        if (b > 0.0f) atomicMax(white_bits, __float_as_uint(b));
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
@@ -545,7 +554,7 @@ int pico_hip_tonemap(pico_hip_ctx* c, float* rgb, uint32_t w, uint32_t h) {
     uint32_t* wb = (uint32_t*)(d + 3 * n);
     PICO_HIP(c, hipMemcpyAsync(d, rgb, 12 * n, hipMemcpyHostToDevice, c->stream));
     PICO_LAUNCH(c, "fill", k_fill_u32, 1, 64, 0, wb, 0u, 1u);
-    PICO_LAUNCH(c, "lum_max", k_lum_max, std::min<size_t>((n + 255) / 256, 4096), 256, 0, d, (uint32_t)n, wb);
+    PICO_LAUNCH(c, "lum_max", k_lum_max, std::min<size_t>((n + 255) / 256, 1024), 256, 0, d, (uint32_t)n, wb);
     PICO_LAUNCH(c, "reinhard", k_reinhard, std::min<size_t>((3 * n + 255) / 256, 8192), 256, 0, d, (uint32_t)(3 * n), wb);
     PICO_HIP(c, hipMemcpyAsync(rgb, d, 12 * n, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));

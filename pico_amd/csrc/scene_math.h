@@ -42,24 +42,41 @@ PICO_HD bool affine_inverse_rows(const float* m, float inv[3][4]) {
     return true;
 }
 
-// World box of an instance: the 8 corners of the mesh box through the
-// column-major transform, widened by a relative epsilon so that float
-// rounding of the transform never culls a hit.
-PICO_HD void instance_world_box(const float* M, const float mlo[3], const float mhi[3], float lo[3], float hi[3]) {
-    for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
-    for (int c = 0; c < 8; ++c) {
-        const float p[3] = {(c & 1) ? mhi[0] : mlo[0], (c & 2) ? mhi[1] : mlo[1], (c & 4) ? mhi[2] : mlo[2]};
-        for (int r = 0; r < 3; ++r) {
-            const float w = M[r] * p[0] + M[4 + r] * p[1] + M[8 + r] * p[2] + M[12 + r];
-            lo[r] = fminf(lo[r], w);
-            hi[r] = fmaxf(hi[r], w);
-        }
+// World boxes of instances.  A point's world position through the
+// column-major transform is ((M[r] x + M[4+r] y) + M[8+r] z) + M[12+r] per
+// row r (no contraction), on the host and on the device alike; the finished
+// box is widened by a relative epsilon so that float rounding of the
+// transform never culls a hit.
+PICO_HD void grow_world_box(const float* M, const float p[3], float lo[3], float hi[3]) {
+    for (int r = 0; r < 3; ++r) {
+        const float w = M[r] * p[0] + M[4 + r] * p[1] + M[8 + r] * p[2] + M[12 + r];
+        lo[r] = fminf(lo[r], w);
+        hi[r] = fmaxf(hi[r], w);
     }
+}
+
+PICO_HD void widen_world_box(float lo[3], float hi[3]) {
     for (int k = 0; k < 3; ++k) {
         const float e = 1e-5f * (fabsf(lo[k]) + fabsf(hi[k]) + 1.0f);
         lo[k] -= e;
         hi[k] += e;
     }
+}
+
+// World box of an instance from the 8 corners of its object-space box (the
+// built-in shapes).  An instance of a triangle mesh is bounded by its
+// vertices instead (grow_world_box over them, then widen_world_box): under a
+// rotation the corners' box is much larger than the mesh's (on the benchmark
+// scene the instance boxes' total area is 2.06 times the exact one), and a
+// ray that enters an instance box it would have missed pays for the
+// instance's transform and BLAS root.
+PICO_HD void instance_world_box(const float* M, const float mlo[3], const float mhi[3], float lo[3], float hi[3]) {
+    for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+    for (int c = 0; c < 8; ++c) {
+        const float p[3] = {(c & 1) ? mhi[0] : mlo[0], (c & 2) ? mhi[1] : mlo[1], (c & 4) ? mhi[2] : mlo[2]};
+        grow_world_box(M, p, lo, hi);
+    }
+    widen_world_box(lo, hi);
 }
 
 // Quantises `nc` child boxes into a DNode4 (references left empty): per axis

@@ -8,7 +8,9 @@
 //
 // Steps (one stream, one host synchronisation at the end):
 //   1  per instance: affine check, float cofactor inverse (bit-identical to the
-//      host build), world box of the mesh box (staged, not yet visible)
+//      host build) (staged, not yet visible); world box of its mesh's vertices
+//      (one workgroup per instance; a built-in shape: its box corners), as the
+//      host build bounds an instance
 //   2  centroid bounds, 30-bit Morton code | instance id as a unique 64-bit key
 //   3  rocPRIM radix sort of the keys
 //   4  Karras hierarchy: n - 1 binary nodes, children and parents
@@ -39,6 +41,8 @@ struct TlasBuf {
     const float* xform;       // n x 16, column-major
     const uint32_t* mesh;     // instance -> mesh
     const float* mesh_box;    // mesh -> lo[3], hi[3]
+    const uint32_t* vrange;   // mesh -> first, end vertex (empty for a built-in shape)
+    const float* pos;         // vertex positions
     float* inv;               // staged inverse rows, n x 12
     float* box;               // instance world boxes, n x 6
     unsigned long long* key;  // n (unsorted), then key_s sorted
@@ -73,7 +77,7 @@ __global__ void k_tlas_init(TlasBuf t) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < t.n; i += gridDim.x * blockDim.x) t.visits[i] = 0;
 }
 
-// step 1 and the centroid bounds of step 2
+// step 1: affine check and inverse
 __global__ void k_tlas_instances(TlasBuf t) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += gridDim.x * blockDim.x) {
         const float* M = t.xform + 16 * (size_t)i;
@@ -81,20 +85,86 @@ __global__ void k_tlas_instances(TlasBuf t) {
         const bool affine = M[3] == 0.0f && M[7] == 0.0f && M[11] == 0.0f && M[15] == 1.0f;
         if (!affine || !affine_inverse_rows(M, inv)) {
             atomicOr(&t.info[0], 1u);
-            for (int k = 0; k < 6; ++k) t.box[6 * (size_t)i + k] = 0.0f;
             continue;
         }
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 4; ++c) t.inv[12 * (size_t)i + 4 * r + c] = inv[r][c];
-        const float* mb = t.mesh_box + 6 * (size_t)t.mesh[i];
-        float lo[3], hi[3];
-        instance_world_box(M, mb, mb + 3, lo, hi);
+    }
+}
+
+// step 1: world boxes, one workgroup of 256 per instance: the mesh's vertices
+// through the transform (min / max are exact, so the box is the host
+// build's), or a built-in shape's box corners; an invalid transform (the
+// rebuild is then refused) gets an empty box at the origin
+__global__ __launch_bounds__(256) void k_tlas_world_boxes(TlasBuf t) {
+    __shared__ float red[6][4];
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t i = blockIdx.x; i < t.n; i += gridDim.x) {
+        const float* M = t.xform + 16 * (size_t)i;
+        const uint32_t m = t.mesh[i];
+        const uint32_t v0 = t.vrange[2 * m], v1 = t.vrange[2 * m + 1];
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) grow_world_box(M, t.pos + 3 * (size_t)v, lo, hi);
+        for (int o = 32; o > 0; o >>= 1)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
+                hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
+            }
+        if ((threadIdx.x & 63) == 0)
+            for (int k = 0; k < 3; ++k) { red[k][wave] = lo[k]; red[3 + k][wave] = hi[k]; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float inv[3][4];
+            const bool ok = M[3] == 0.0f && M[7] == 0.0f && M[11] == 0.0f && M[15] == 1.0f && affine_inverse_rows(M, inv);
+            float blo[3], bhi[3];
+            if (!ok) {
+                for (int k = 0; k < 3; ++k) { blo[k] = 0.0f; bhi[k] = 0.0f; }
+            } else if (v0 < v1) {
+                for (int k = 0; k < 3; ++k) {
+                    blo[k] = fminf(fminf(red[k][0], red[k][1]), fminf(red[k][2], red[k][3]));
+                    bhi[k] = fmaxf(fmaxf(red[3 + k][0], red[3 + k][1]), fmaxf(red[3 + k][2], red[3 + k][3]));
+                }
+                widen_world_box(blo, bhi);
+            } else {
+                const float* mb = t.mesh_box + 6 * (size_t)m;
+                instance_world_box(M, mb, mb + 3, blo, bhi);
+            }
+            for (int k = 0; k < 3; ++k) {
+                t.box[6 * (size_t)i + k] = blo[k];
+                t.box[6 * (size_t)i + 3 + k] = bhi[k];
+            }
+        }
+        __syncthreads();  // red is reused by the next instance
+    }
+}
+
+// step 2: centroid bounds of the world boxes, one atomic per workgroup and
+// bound (atomics on one address serialise)
+__global__ __launch_bounds__(256) void k_tlas_centroid_bounds(TlasBuf t) {
+    __shared__ float red[6][4];
+    const uint32_t wave = threadIdx.x >> 6;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += gridDim.x * blockDim.x)
         for (int k = 0; k < 3; ++k) {
-            t.box[6 * (size_t)i + k] = lo[k];
-            t.box[6 * (size_t)i + 3 + k] = hi[k];
-            const float c = 0.5f * (lo[k] + hi[k]);
-            atomic_min_f(&t.info[2 + k], c);
-            atomic_max_f(&t.info[5 + k], c);
+            const float c = 0.5f * (t.box[6 * (size_t)i + k] + t.box[6 * (size_t)i + 3 + k]);
+            lo[k] = fminf(lo[k], c);
+            hi[k] = fmaxf(hi[k], c);
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) { red[k][wave] = lo[k]; red[3 + k][wave] = hi[k]; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        const float l = fminf(fminf(red[k][0], red[k][1]), fminf(red[k][2], red[k][3]));
+        const float h = fmaxf(fmaxf(red[3 + k][0], red[3 + k][1]), fmaxf(red[3 + k][2], red[3 + k][3]));
+        if (l <= h) {
+            atomic_min_f(&t.info[2 + k], l);
+            atomic_max_f(&t.info[5 + k], h);
         }
     }
 }
@@ -293,6 +363,8 @@ int tlas_rebuild(pico_hip_ctx* c, uint32_t n, const float* d_transforms, pico_re
     t.xform = d_transforms;
     t.mesh = c->d_instance_mesh;
     t.mesh_box = c->d_mesh_box;
+    t.vrange = c->d_mesh_vrange;
+    t.pos = c->dev.pos;
     const size_t m = std::max<uint32_t>(n, 2);
     t.inv = (float*)scratch(c, "tlas_inv", m * 48);
     t.box = (float*)scratch(c, "tlas_box", m * 24);
@@ -318,6 +390,8 @@ int tlas_rebuild(pico_hip_ctx* c, uint32_t n, const float* d_transforms, pico_re
     PICO_HIP(c, hipEventRecord(e0, c->stream));
     PICO_LAUNCH(c, "tlas_init", k_tlas_init, grid_of(n), 256, 0, t);
     PICO_LAUNCH(c, "tlas_instances", k_tlas_instances, grid_of(n), 256, 0, t);
+    PICO_LAUNCH(c, "tlas_world_boxes", k_tlas_world_boxes, std::min<uint32_t>(std::max<uint32_t>(n, 1), 65535), 256, 0, t);
+    PICO_LAUNCH(c, "tlas_centroid_bounds", k_tlas_centroid_bounds, std::min<uint32_t>(grid_of(n), 64), 256, 0, t);
     PICO_LAUNCH(c, "tlas_keys", k_tlas_keys, grid_of(n), 256, 0, t);
     size_t tmp = 0;
     PICO_HIP(c, rocprim::radix_sort_keys(nullptr, tmp, t.key, t.key_s, n, 0, 64, c->stream));
