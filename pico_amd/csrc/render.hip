@@ -83,8 +83,7 @@ struct Pool {
     uint32_t* hprim;
     float4* sh_o;
     float4* sh_d;
-    float4* sh_tp;   // throughput * direct pdf
-    float4* sh_en;   // BSDF sample energy
+    float4* sh_c;    // (throughput * direct pdf) * BSDF sample energy; w: 1 towards the sun
     uint32_t* q0;
     uint32_t* q1;
     uint32_t* shq;
@@ -249,8 +248,6 @@ __global__ __launch_bounds__(kBlock) void k_generate(const DView* __restrict__ v
                 const f3 d = camera_dir(V.cam, xi, p % V.cam.w, p / V.cam.w);
                 P.ray_o[i] = f4(V.cam.pos, 0.0f);
                 P.ray_d[i] = f4(d, 0.0f);
-                P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-                P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
                 P.rng[i] = make_uint4((uint32_t)rng.hs, (uint32_t)(rng.hs >> 32), (uint32_t)rng.ms, (uint32_t)(rng.ms >> 32));
                 P.key[i] = key;
             }
@@ -396,10 +393,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 P.hit[slot] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.inst));
                 P.hprim[slot] = t.h.prim;
             } else {
-                const float4 tp = P.sh_tp[item];
+                const float4 tc = P.sh_c[item];
                 bool lit = false;
                 f3 em;
-                if (tp.w != 0.0f) {
+                if (tc.w != 0.0f) {
                     // towards the sun: lit when the ray met nothing
                     lit = t.h.inst == 0xffffffffu;
                     em = mk(s.sun_col[0], s.sun_col[1], s.sun_col[2]);
@@ -411,7 +408,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 }
                 if (lit) {
                     const float4 r = P.rad[item];
-                    const f3 add = (xyz(tp) * em) * xyz(P.sh_en[item]);
+                    const f3 add = xyz(tc) * em;
                     P.rad[item] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
                 }
             }
@@ -449,9 +446,11 @@ __device__ __forceinline__ uint32_t shade_path(const DevScene& s, const DView* _
         const uint32_t key = P.key[i];
         const DView& V = views[key >> kViewShift];
         const f3 dir = xyz(P.ray_d[i]);
-        float4 r4 = P.rad[i];
+        // a camera ray's path starts with no radiance (one ray traced) and
+        // unit throughput outside any transmissive object: not stored
+        float4 r4 = b ? P.rad[i] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         f3 rad = xyz(r4);
-        const float4 t4 = P.thr[i];
+        const float4 t4 = b ? P.thr[i] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         f3 thr = xyz(t4);
         IorStack ist;
         ist.count = __float_as_uint(t4.w);
@@ -500,8 +499,7 @@ __device__ __forceinline__ uint32_t shade_path(const DevScene& s, const DView* _
                     const float pdf = direct_pdf(kind, s.microfacet, M, f.n, -dir, to_sun);
                     P.sh_o[i] = f4(f.p + 0.01f * to_sun, 0.0f);
                     P.sh_d[i] = f4(to_sun, __int_as_float(0x7f800000));
-                    P.sh_tp[i] = f4(thr * pdf, 1.0f);
-                    P.sh_en[i] = f4(smp.energy, 0.0f);
+                    P.sh_c[i] = f4((thr * pdf) * smp.energy, 1.0f);
                     shadow = true;
                     r4.w += 1.0f;
                 } else if (s.n_lights) {
@@ -523,8 +521,11 @@ __device__ __forceinline__ uint32_t shade_path(const DevScene& s, const DView* _
                         const f3 dl = wp - f.p;
                         P.sh_o[i] = f4(f.p + 0.01f * to_light, 0.0f);
                         P.sh_d[i] = f4(to_light, 1.01f * sqrtf(dot(dl, dl)) + 1e-3f);
-                        P.sh_tp[i] = f4(thr * pdf, 0.0f);  // w = 0: a mesh or shape light
-                        P.sh_en[i] = f4(smp.energy, 0.0f);
+                        // the contribution without the light's emission, which the
+                        // shadow ray's hit supplies: Integrators.cpp:173-176 forms
+                        // ((throughput pdf) L_e) energy, here ((throughput pdf)
+                        // energy) L_e (the same products, rounded in another order)
+                        P.sh_c[i] = f4((thr * pdf) * smp.energy, 0.0f);  // w = 0: a mesh or shape light
                         shadow = true;
                         r4.w += 1.0f;
                     }
@@ -852,7 +853,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         off += S;
     }
     // buffers
-    char* pb = (char*)scratch(c, "pool", paths * (16 * 12 + 4 * 6) + 8192);
+    char* pb = (char*)scratch(c, "pool", paths * (16 * 11 + 4 * 6) + 8192);
     uint32_t* act = (uint32_t*)scratch(c, "active", pix_total * 8 + 64);
     Ctl* ctl = (Ctl*)scratch(c, "ctl", sizeof(Ctl));
     uint32_t* count0 = (uint32_t*)scratch(c, "count0", pix_total * 4 + 64);
@@ -885,7 +886,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         P.thr = (float4*)take(paths * 16); P.rad = (float4*)take(paths * 16);
         P.rng = (uint4*)take(paths * 16); P.hit = (float4*)take(paths * 16);
         P.sh_o = (float4*)take(paths * 16); P.sh_d = (float4*)take(paths * 16);
-        P.sh_tp = (float4*)take(paths * 16); P.sh_en = (float4*)take(paths * 16);
+        P.sh_c = (float4*)take(paths * 16);
         P.samp = (float4*)take(paths * 16);
         P.ior = (float4*)take(paths * 16);
         P.key = (uint32_t*)take(paths * 4); P.hprim = (uint32_t*)take(paths * 4);
