@@ -54,8 +54,15 @@ struct DView {
     float* samples_out;
     uint32_t* count0;   // per-pixel sample count at render start
     uint8_t* done;      // per-pixel
-    unsigned long long* stats;  // kStatSlots slots of (rays, samples), kStatStride apart
+    unsigned long long* stats;  // kCtrSlots counter slots of (rays, samples)
 };
+
+// Counter slots: atomics on one address serialise, so counters that every
+// workgroup or wave of a grid adds to at the end are spread over kCtrSlots
+// slots kCtrStride 64-bit words (256 B) apart and summed by their reader.
+constexpr uint32_t kCtrSlots = 16;
+constexpr uint32_t kCtrStride = 32;
+constexpr uint32_t kNoFinish = 0xffffffffu;
 
 struct Ctl {
     uint32_t n_active;
@@ -70,6 +77,8 @@ struct Ctl {
     unsigned long long nodes, tris, insts;           // extend (camera + extension rays)
     unsigned long long nodes_sh, tris_sh, insts_sh;  // shadow rays
     unsigned long long spill_push, spill_pop;        // traversal-stack entries past LDS (both kernels)
+    uint32_t finish_from;                            // first bounce left to k_finish, or kNoFinish
+    unsigned long long fin_rays[kCtrSlots * kCtrStride];  // k_finish's extension / shadow rays (words 0, 1 of a slot)
 };
 
 struct Pool {
@@ -172,6 +181,7 @@ __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
         ctl->n_active = na;
         ctl->n_next = 0;
         ctl->gen_head = 0;
+        ctl->finish_from = kNoFinish;
         // bounce 0 uses identity slots: path i sits at queue entry i (dead
         // paths of views with fewer samples are marked kDeadPath)
         ctl->ext_count[0] = na * S;
@@ -342,6 +352,39 @@ __device__ __forceinline__ bool refill_due(bool idle) {
     return (uint32_t)__popcll(im) >= kRefillMin || im == __ballot(true);
 }
 
+// The closest hit h of path i's shadow ray: towards the sun the point is lit
+// when the ray met nothing, towards a light when the closest hit is any light
+// (Integrators.cpp:61-77, 101-117); a lit point adds the ray's contribution
+// times the emission to the path's radiance.
+__device__ __forceinline__ void shadow_hit(const DevScene& s, const Pool& P, const Hit& h, uint32_t i) {
+    const float4 tc = P.sh_c[i];
+    bool lit = false;
+    f3 em;
+    if (tc.w != 0.0f) {
+        lit = h.inst == 0xffffffffu;
+        em = mk(s.sun_col[0], s.sun_col[1], s.sun_col[2]);
+    } else if (h.inst != 0xffffffffu) {
+        const DInstance& I = s.inst[h.inst];
+        lit = I.bsdf == PICO_BSDF_LIGHT;
+        if (lit) em = ld3(s.mats[I.material].emissive);
+    }
+    if (lit) {
+        const float4 r = P.rad[i];
+        const f3 add = xyz(tc) * em;
+        P.rad[i] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
+    }
+}
+
+// Starts the traversal of path i's extension ray, or of its shadow ray (which
+// stops short of the sampled light point).
+template <int STACK>
+__device__ __forceinline__ void begin_path_ray(const DevScene& s, const Pool& P, TraceState& t, TraceStack<STACK>& st,
+                                               uint32_t i, bool shadow) {
+    const float4 o4 = (shadow ? P.sh_o : P.ray_o)[i];
+    const float4 d4 = (shadow ? P.sh_d : P.ray_d)[i];
+    trace_begin(s, t, st, xyz(o4), xyz(d4), shadow ? d4.w : INFINITY);
+}
+
 // The closest-hit queries of bounce b in one launch: the extension rays of
 // bounce b (entries [0, n_ext) of the combined queue) and the shadow rays that
 // the shading of bounce b - 1 made (entries [n_ext, n_ext + n_sh)).  Neither
@@ -351,6 +394,7 @@ __device__ __forceinline__ bool refill_due(bool idle) {
 // Extension rays come first: unbounded, they are the longer ones.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_trace(DevScene s, Ctl* ctl, uint32_t b, Pool P, uint32_t* spill, uint32_t spill_cap) {
+    if (ctl->finish_from <= b) return;
     __shared__ uint32_t stk[kTraceLds * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
@@ -375,11 +419,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 const bool sh = e >= n_ext;
                 const uint32_t i = sh ? P.shq[e - n_ext] : q[e];
                 if (i != kDeadPath) {
-                    const float4 o4 = (sh ? P.sh_o : P.ray_o)[i];
-                    const float4 d4 = (sh ? P.sh_d : P.ray_d)[i];
                     item = i;
                     slot = sh ? kDeadPath : e;
-                    trace_begin(s, t, st, xyz(o4), xyz(d4), sh ? d4.w : INFINITY);
+                    begin_path_ray(s, P, t, st, i, sh);
                 }
             }
             if (drained || !__ballot(item == kDeadPath)) break;
@@ -393,24 +435,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 P.hit[slot] = make_float4(t.h.t, t.h.u, t.h.v, __uint_as_float(t.h.inst));
                 P.hprim[slot] = t.h.prim;
             } else {
-                const float4 tc = P.sh_c[item];
-                bool lit = false;
-                f3 em;
-                if (tc.w != 0.0f) {
-                    // towards the sun: lit when the ray met nothing
-                    lit = t.h.inst == 0xffffffffu;
-                    em = mk(s.sun_col[0], s.sun_col[1], s.sun_col[2]);
-                } else if (t.h.inst != 0xffffffffu) {
-                    // towards a light: lit when the closest hit is any light
-                    const DInstance& I = s.inst[t.h.inst];
-                    lit = I.bsdf == PICO_BSDF_LIGHT;
-                    if (lit) em = ld3(s.mats[I.material].emissive);
-                }
-                if (lit) {
-                    const float4 r = P.rad[item];
-                    const f3 add = xyz(tc) * em;
-                    P.rad[item] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
-                }
+                shadow_hit(s, P, t.h, item);
             }
             if (COUNT) {
                 TraceCounters& d = slot != kDeadPath ? tce : tcs;
@@ -433,16 +458,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     }
 }
 
-// Shading of path i, found at extension-queue entry e of bounce b: emission,
+// Shading of path i, whose extension ray of bounce b found h: emission,
 // BSDF sample, next-event estimation's shadow ray, Russian roulette and the
 // next extension ray (Integrators.cpp:130-209).  Returns bit 0 when the path
 // continues (next extension queue) and bit 1 when it has a shadow ray.
 __device__ __forceinline__ uint32_t shade_path(const DevScene& s, const DView* __restrict__ views, uint32_t b,
-                                               const Pool& P, uint32_t e, uint32_t i) {
+                                               const Pool& P, const Hit& h, uint32_t i) {
     bool cont = false, shadow = false;
     {
-        const float4 h4 = P.hit[e];
-        const uint32_t inst = __float_as_uint(h4.w);
+        const uint32_t inst = h.inst;
         const uint32_t key = P.key[i];
         const DView& V = views[key >> kViewShift];
         const f3 dir = xyz(P.ray_d[i]);
@@ -463,8 +487,6 @@ __device__ __forceinline__ uint32_t shade_path(const DevScene& s, const DView* _
                 rad = rad + thr * sky;  // Integrators.cpp:208
             }
         } else if (b < V.max_depth) {
-            Hit h;
-            h.t = h4.x; h.u = h4.y; h.v = h4.z; h.inst = inst; h.prim = P.hprim[e];
             const Frag f = s.inst[inst].blas_root >= kImplicitRoot ? implicit_frag(s, h, xyz(P.ray_o[i]), dir)
                                                                    : make_frag(s, h);
             const DInstance& I = s.inst[inst];
@@ -555,9 +577,11 @@ __device__ __forceinline__ uint32_t shade_path(const DevScene& s, const DView* _
 
 // Appends, with one atomic on `counter`, the queue entries q[base + 64 k +
 // lane] whose bit k is set in the lane's `bits` (k < nb), in batch then lane
-// order.  Every lane of the wave calls it.
-__device__ __forceinline__ void append_batches(uint32_t* counter, uint32_t bits, uint32_t nb, uint32_t* __restrict__ out,
-                                               const uint32_t* __restrict__ q, uint32_t base) {
+// order, with bit 31 set where bit k of `flags` is.  Every lane of the wave
+// calls it.
+__device__ __forceinline__ void append_batches(uint32_t* counter, uint32_t bits, uint32_t flags, uint32_t nb,
+                                               uint32_t* __restrict__ out, const uint32_t* __restrict__ q,
+                                               uint32_t base) {
     uint32_t tot = 0;
     for (uint32_t k = 0; k < nb; ++k) tot += (uint32_t)__popcll(__ballot((bits >> k) & 1u));
     if (tot == 0) return;
@@ -567,7 +591,8 @@ __device__ __forceinline__ void append_batches(uint32_t* counter, uint32_t bits,
     const uint64_t below = (1ull << lane_id()) - 1ull;
     for (uint32_t k = 0; k < nb; ++k) {
         const uint64_t m = __ballot((bits >> k) & 1u);
-        if ((bits >> k) & 1u) out[o + (uint32_t)__popcll(m & below)] = q[base + 64u * k + lane_id()];
+        if ((bits >> k) & 1u)
+            out[o + (uint32_t)__popcll(m & below)] = q[base + 64u * k + lane_id()] | (((flags >> k) & 1u) << 31);
         o += (uint32_t)__popcll(m);
     }
 }
@@ -581,10 +606,23 @@ __device__ __forceinline__ void append_batches(uint32_t* counter, uint32_t bits,
 // entries spread over as many waves as possible.
 constexpr uint32_t kShadeBatches = 4;
 
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __restrict__ views, Ctl* ctl, uint32_t b, Pool P) {
+// Paths entering a shading pass below which the remaining bounces of the
+// chunk go to k_finish.
+constexpr uint32_t kFinishPaths = 1u << 20;
+
+// `last`: the chunk's last bounce.  With fewer than kFinishPaths paths
+// entering bounce b < last, the pass hands the rest of the chunk over to
+// k_finish: a continuing path's queue entry carries bit 31 when the path also
+// has a shadow ray (k_finish traces it first), and only the shadow rays of
+// paths that end here go to the shadow queue.
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __restrict__ views, Ctl* ctl, uint32_t b,
+                                                  uint32_t last, Pool P) {
+    if (ctl->finish_from <= b) return;
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     uint32_t* qn = (b & 1) ? P.q0 : P.q1;
     const uint32_t n = ctl->ext_count[b];
+    const bool handoff = b < last && n < kFinishPaths;
+    if (handoff && blockIdx.x == 0 && threadIdx.x == 0) ctl->finish_from = b + 1;
     const uint32_t waves = gridDim.x * (kBlock / 64);
     const uint32_t nb = n / (64u * kShadeBatches) >= 2u * waves ? kShadeBatches : 1u;
     WaveClaim wc(&ctl->shade_head[b], n, 64u * nb);
@@ -596,23 +634,124 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __res
             const uint32_t e = base + 64u * k + lane_id();
             const uint32_t i = e < n ? q[e] : kDeadPath;
             if (i != kDeadPath) {
-                const uint32_t r = shade_path(s, views, b, P, e, i);
+                const float4 h4 = P.hit[e];
+                const Hit h{h4.x, h4.y, h4.z, __float_as_uint(h4.w), P.hprim[e]};
+                const uint32_t r = shade_path(s, views, b, P, h, i);
                 cbits |= (r & 1u) << k;
                 sbits |= (r >> 1) << k;
             }
         }
-        append_batches(&ctl->ext_count[b + 1], cbits, nb, qn, q, base);
-        append_batches(&ctl->sh_count[b], sbits, nb, P.shq, q, base);
+        append_batches(&ctl->ext_count[b + 1], cbits, handoff ? sbits : 0u, nb, qn, q, base);
+        append_batches(&ctl->sh_count[b], handoff ? sbits & ~cbits : sbits, 0u, nb, P.shq, q, base);
+    }
+}
+
+// The bounces of a chunk from ctl->finish_from on, path by path.  Once few
+// paths are left, a launch per bounce lasts as long as its slowest ray, and
+// those times add up over the remaining bounces; here a lane takes a path and
+// runs its remaining work in order (a pending shadow ray, then the extension
+// ray, its shading point, that point's shadow ray, the next extension ray
+// ...), so the chunk ends with its slowest path instead.  Work items: the
+// entries of the extension queue of bounce b0 (bit 31: the path has a shadow
+// ray of bounce b0 - 1 to trace first), then the shadow queue of bounce
+// b0 - 1 (paths that end there).  Per path the arithmetic is that of k_trace
+// and k_shade, so the output does not depend on where the hand-over happens.
+enum : uint32_t { kPhExt = 1, kPhShadowThenExt = 2, kPhShadowOnly = 3 };
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_finish(DevScene s, const DView* __restrict__ views, Ctl* ctl, Pool P,
+                                                   uint32_t* spill, uint32_t spill_cap) {
+    const uint32_t b0 = ctl->finish_from;
+    if (b0 == kNoFinish) return;
+    __shared__ uint32_t stk[kTraceLds * kBlock];
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    TraceStack<kTraceLds> st{stk + threadIdx.x, kBlock, spill + gtid, gridDim.x * kBlock, spill_cap};
+    const uint32_t* q = (b0 & 1) ? P.q1 : P.q0;
+    const uint32_t n_ext = ctl->ext_count[b0];
+    const uint32_t n = n_ext + ctl->sh_count[b0 - 1];
+    TraceCounters tc{0, 0, 0, 0, 0}, tce{0, 0, 0, 0, 0}, tcs{0, 0, 0, 0, 0};
+    TraceState t;
+    // bb: the bounce of the path's next extension ray
+    uint32_t item = kDeadPath, bb = 0, ph = 0, res_base = 0, res_left = 0;
+    uint32_t rays_ext = 0, rays_sh = 0;  // rays no queue count holds
+    bool drained = false;
+    WaveClaim wc(&ctl->ext_head[b0], n, kTraceClaim);
+    for (;;) {
+        for (;;) {
+            if (!refill_due(item == kDeadPath)) break;
+            const uint32_t e = refill_slot(item == kDeadPath, wc, res_base, res_left, drained);
+            if (e != kDeadPath) {
+                if (e < n_ext) {
+                    const uint32_t v = q[e];
+                    item = v & 0x7fffffffu;
+                    bb = b0;
+                    ph = (v >> 31) ? kPhShadowThenExt : kPhExt;
+                    rays_sh += v >> 31;
+                } else {
+                    item = P.shq[e - n_ext];
+                    ph = kPhShadowOnly;
+                }
+                begin_path_ray(s, P, t, st, item, ph != kPhExt);
+            }
+            if (drained || !__ballot(item == kDeadPath)) break;
+        }
+        if (!__ballot(item != kDeadPath)) break;
+        if (item != kDeadPath && trace_step<kTraceLds, COUNT>(s, t, st, &tc, &ctl->overflow)) {
+            if (COUNT) {
+                TraceCounters& d = ph == kPhExt ? tce : tcs;
+                d.nodes += tc.nodes; d.tris += tc.tris; d.insts += tc.insts;
+                d.spill_push += tc.spill_push; d.spill_pop += tc.spill_pop;
+                tc = TraceCounters{0, 0, 0, 0, 0};
+            }
+            if (ph != kPhExt) {
+                shadow_hit(s, P, t.h, item);
+                if (ph == kPhShadowThenExt) {
+                    ph = kPhExt;
+                    begin_path_ray(s, P, t, st, item, false);
+                } else {
+                    item = kDeadPath;
+                }
+            } else {
+                const uint32_t r = shade_path(s, views, bb, P, t.h, item);
+                ++bb;
+                rays_ext += r & 1u;
+                rays_sh += r >> 1;
+                if (r & 2u) {
+                    ph = (r & 1u) ? kPhShadowThenExt : kPhShadowOnly;
+                    begin_path_ray(s, P, t, st, item, true);
+                } else if (r & 1u) {
+                    begin_path_ray(s, P, t, st, item, false);
+                } else {
+                    item = kDeadPath;
+                }
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        rays_ext += __shfl_xor(rays_ext, o);
+        rays_sh += __shfl_xor(rays_sh, o);
+    }
+    if (lane_id() == 0 && (rays_ext | rays_sh)) {
+        unsigned long long* slot = ctl->fin_rays + kCtrStride * ((gtid >> 6) % kCtrSlots);
+        atomicAdd(slot, (unsigned long long)rays_ext);
+        atomicAdd(slot + 1, (unsigned long long)rays_sh);
+    }
+    if (COUNT) {
+        atomicAdd(&ctl->nodes, (unsigned long long)tce.nodes);
+        atomicAdd(&ctl->tris, (unsigned long long)tce.tris);
+        atomicAdd(&ctl->insts, (unsigned long long)tce.insts);
+        atomicAdd(&ctl->nodes_sh, (unsigned long long)tcs.nodes);
+        atomicAdd(&ctl->tris_sh, (unsigned long long)tcs.tris);
+        atomicAdd(&ctl->insts_sh, (unsigned long long)tcs.insts);
+        atomicAdd(&ctl->spill_push, (unsigned long long)(tce.spill_push + tcs.spill_push));
+        atomicAdd(&ctl->spill_pop, (unsigned long long)(tce.spill_pop + tcs.spill_pop));
     }
 }
 
 // Per-workgroup (rays, samples) counters of every view in LDS: items add to
 // them (one LDS atomic pair per wave when the wave is on one view), and the
-// workgroup adds its non-zero totals to the views' counters once at the end.
-// Per-view ray and sample counters: kStatSlots slots of (rays, samples),
-// kStatStride counters (256 B) apart, summed by the host.
-constexpr uint32_t kStatSlots = 16;
-constexpr uint32_t kStatStride = 32;
+// workgroup adds its non-zero totals to the views' counters once at the end,
+// into counter slot (rays, samples) workgroup mod kCtrSlots of each view.
 
 struct BlockViewStats {
     unsigned long long v[kMaxViews][2];
@@ -632,13 +771,11 @@ struct BlockViewStats {
             atomicAdd(&v[view][1], ns);
         }
     }
-    // Workgroup b adds to counter slot b mod kStatSlots of each view: every
-    // workgroup of a grid flushes at about the same time, and atomics on one
-    // address serialise, so one slot per view would hold the grid's last
-    // workgroups for 4096 atomics.
+    // every workgroup of a grid flushes at about the same time: one slot per
+    // view would hold the grid's last workgroups for 4096 atomics
     __device__ __forceinline__ void flush(const DView* views) {
         __syncthreads();
-        const uint32_t slot = kStatStride * (blockIdx.x % kStatSlots);
+        const uint32_t slot = kCtrStride * (blockIdx.x % kCtrSlots);
         for (uint32_t i = threadIdx.x; i < kMaxViews; i += blockDim.x)
             if (v[i][0] | v[i][1]) {
                 atomicAdd(&views[i].stats[slot], v[i][0]);
@@ -775,8 +912,16 @@ __global__ __launch_bounds__(kBlock) void k_compact(const DView* __restrict__ vi
 __global__ void k_sum_rays(Ctl* ctl, uint32_t bounces) {
     if (threadIdx.x == 0) {
         unsigned long long e = 0, sh = 0;
-        for (uint32_t b = 1; b <= bounces; ++b) e += ctl->ext_count[b];
-        for (uint32_t b = 0; b <= bounces; ++b) sh += ctl->sh_count[b];
+        // queues of the bounces that ran as launches, and k_finish's rays
+        const uint32_t last = ctl->finish_from == kNoFinish ? bounces : ctl->finish_from;
+        for (uint32_t b = 1; b <= last; ++b) e += ctl->ext_count[b];
+        for (uint32_t b = 0; b < last; ++b) sh += ctl->sh_count[b];
+        for (uint32_t k = 0; k < kCtrSlots; ++k) {
+            e += ctl->fin_rays[kCtrStride * k];
+            sh += ctl->fin_rays[kCtrStride * k + 1];
+            ctl->fin_rays[kCtrStride * k] = 0;
+            ctl->fin_rays[kCtrStride * k + 1] = 0;
+        }
         ctl->rays_cam += ctl->ext_count[0];
         ctl->rays_ext += e;
         ctl->rays_sh += sh;
@@ -791,11 +936,11 @@ struct Plan {
 // b - 1, then the shading of bounce b (which makes the extension rays of
 // bounce b + 1 and the shadow rays of bounce b).  The shading of the last
 // bounce makes no shadow rays.
-int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, const Pool& P, const DView* dviews, bool count,
-                          uint32_t grid_trace, uint32_t grid_shade, uint32_t* spill, uint32_t spill_cap) {
+int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, uint32_t last, const Pool& P, const DView* dviews,
+                          bool count, uint32_t grid_trace, uint32_t grid_shade, uint32_t* spill, uint32_t spill_cap) {
     if (count) PICO_LAUNCH(c, "trace", (k_trace<true>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
     else PICO_LAUNCH(c, "trace", (k_trace<false>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
-    PICO_LAUNCH(c, "shade", k_shade, grid_shade, kBlock, 0, c->dev, dviews, ctl, b, P);
+    PICO_LAUNCH(c, "shade", k_shade, grid_shade, kBlock, 0, c->dev, dviews, ctl, b, last, P);
     return PICO_OK;
 }
 
@@ -858,7 +1003,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     Ctl* ctl = (Ctl*)scratch(c, "ctl", sizeof(Ctl));
     uint32_t* count0 = (uint32_t*)scratch(c, "count0", pix_total * 4 + 64);
     uint8_t* done = (uint8_t*)scratch(c, "done", pix_total + 64);
-    const size_t vstat_words = (size_t)kStatSlots * kStatStride;  // per view
+    const size_t vstat_words = (size_t)kCtrSlots * kCtrStride;  // per view
     unsigned long long* vstats = (unsigned long long*)scratch(c, "vstats", nv * vstat_words * 8 + 64);
     DView* dv = (DView*)scratch(c, "views", sizeof(DView) * nv);
     if (!pb || !act || !ctl || !count0 || !done || !vstats || !dv) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
@@ -936,6 +1081,9 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     uint32_t* spill = (uint32_t*)scratch(c, "trace_spill", (size_t)grid_trace * kBlock * kTraceSpill * 4 + 64);
     if (!spill) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     const uint32_t grid_shade = (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_s, 16));
+    int occ_f = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, k_finish<false>, kBlock, 0);
+    const uint32_t grid_finish = std::min(grid_trace, (uint32_t)c->num_cus * (uint32_t)std::max(1, std::min(occ_f, 16)));
     const uint32_t grid_lin = (uint32_t)c->num_cus * 16;
     // per-view completion events: several views, a follow-up per view, or a
     // group of a shortest-first batch (its times count from the batch's epoch)
@@ -956,9 +1104,11 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         PICO_LAUNCH(c, "generate", k_generate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         for (uint32_t b = 0; b <= pl.max_depth; ++b) {
             int rc;
-            rc = launch_bounce_kernels(c, ctl, b, P, dv, count, grid_trace, grid_shade, spill, spill_cap);
+            rc = launch_bounce_kernels(c, ctl, b, pl.max_depth, P, dv, count, grid_trace, grid_shade, spill, spill_cap);
             if (rc) return rc;
         }
+        if (count) PICO_LAUNCH(c, "finish", (k_finish<true>), grid_finish, kBlock, 0, c->dev, dv, ctl, P, spill, spill_cap);
+        else PICO_LAUNCH(c, "finish", (k_finish<false>), grid_finish, kBlock, 0, c->dev, dv, ctl, P, spill, spill_cap);
         PICO_LAUNCH(c, "sum_rays", k_sum_rays, 1, 64, 0, ctl, pl.max_depth);
         PICO_LAUNCH(c, "finalize", k_finalize, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         PICO_LAUNCH(c, "accumulate", k_accumulate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
@@ -1025,9 +1175,9 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     if (fallback) PICO_HIP(c, hipMemcpyAsync(hfb.data(), fallback, 4 * nv, hipMemcpyDeviceToHost, c->stream));
     PICO_HIP(c, hipStreamSynchronize(c->stream));
     for (uint32_t v = 0; v < nv; ++v)
-        for (uint32_t k = 0; k < kStatSlots; ++k) {
-            hs[2 * v] += hsl[v * vstat_words + k * kStatStride];
-            hs[2 * v + 1] += hsl[v * vstat_words + k * kStatStride + 1];
+        for (uint32_t k = 0; k < kCtrSlots; ++k) {
+            hs[2 * v] += hsl[v * vstat_words + k * kCtrStride];
+            hs[2 * v + 1] += hsl[v * vstat_words + k * kCtrStride + 1];
         }
     float ms = 0.0f;
     hipEventElapsedTime(&ms, e0, e1);
