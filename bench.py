@@ -53,6 +53,13 @@ def traversal_bytes(st):
         (st.tri_tests + st.shadow_tri_tests) * B_TRI + (st.instance_visits + st.shadow_instance_visits) * B_INST
 
 
+def finish_bytes(st):
+    """The part of traversal_bytes traced by the kernel that ends a chunk's
+    last bounces path by path (k_finish)."""
+    return st.finish_rays * B_RAY + st.finish_node_visits * B_NODE + st.finish_tri_tests * B_TRI + \
+        st.finish_instance_visits * B_INST
+
+
 def cpu_baseline(scene, cam, w, h, spp, depth, seed):
     """Pico's own CPU renderer (compiled reference subset) on the host cores,
     on a bounded sample of the same workload; rays counted by the oracle on the
@@ -737,6 +744,7 @@ def main():
     ctx.sync()
     dt = time.perf_counter() - t0
     tr_n, tr_ms = ctx.kernel_time("trace")
+    fin_n, fin_ms = ctx.kernel_time("finish")
     ctx.enable_kernel_timing(False)
     ms_step = dt * 1e3 / args.steps
     mrays = rays / dt / 1e6
@@ -744,7 +752,10 @@ def main():
     roof = None
     if count is not None and tr_n:
         launches_per_frame = tr_n / args.steps
-        bytes_per_launch = traversal_bytes(count) / launches_per_frame
+        # k_trace's rays: all but those of the last bounces' hand-over kernel
+        trace_bytes = traversal_bytes(count) - finish_bytes(count)
+        trace_rays = count.camera_rays + count.extension_rays + count.shadow_rays - count.finish_rays
+        bytes_per_launch = trace_bytes / launches_per_frame
         achieved = bytes_per_launch / (tr_ms * 1e-3) / 1e9
         traffic, tdetail = None, None
         if not args.no_traffic:
@@ -760,9 +771,16 @@ def main():
                 "measured_copy_GBps": round(copy_gbps, 1),
                 "kernel": "trace", "launches": tr_n, "mean_launch_ms": round(tr_ms, 4),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch),
-                "bytes_per_ray": round(traversal_bytes(count) /
-                                       max(1, count.camera_rays + count.extension_rays + count.shadow_rays), 1),
-                "stack_spill_pushes_per_frame": count.spill_pushes, "stack_spill_pops_per_frame": count.spill_pops}
+                "bytes_per_ray": round(trace_bytes / max(1, trace_rays), 1),
+                "rays_per_frame": trace_rays,
+                "stack_spill_pushes_per_frame": count.spill_pushes, "stack_spill_pops_per_frame": count.spill_pops,
+                # the last bounces of each chunk, path by path in one launch
+                # (traversal and shading): rays, algorithmic traversal bytes
+                # and launch time per frame
+                "finish_kernel": {"rays_per_frame": count.finish_rays,
+                                  "traversal_bytes_per_frame": round(finish_bytes(count)),
+                                  "launches_per_frame": round(fin_n / args.steps, 2),
+                                  "ms_per_frame": round(fin_ms * fin_n / args.steps, 3)}}
 
     quality = None
     if not args.no_quality:

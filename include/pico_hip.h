@@ -204,6 +204,11 @@ typedef struct pico_stats {
                                 final (after its denoiser when one is requested) */
     uint64_t spill_pushes;   /* PICO_RENDER_COUNT only: traversal-stack entries written past */
     uint64_t spill_pops;     /* the LDS part (device memory), and read back, all rays */
+    uint64_t finish_rays;    /* rays of a chunk's last bounces, which one persistent launch traces path
+                                by path once few paths are left (included in the ray counts above) */
+    uint64_t finish_node_visits;     /* PICO_RENDER_COUNT only: of those rays (included above) */
+    uint64_t finish_tri_tests;       /* PICO_RENDER_COUNT only */
+    uint64_t finish_instance_visits; /* PICO_RENDER_COUNT only */
 } pico_stats;
 
 /* ---- context ----------------------------------------------------------- */

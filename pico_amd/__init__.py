@@ -87,7 +87,9 @@ class Stats(C.Structure):
                 ("shadow_tri_tests", C.c_uint64), ("shadow_instance_visits", C.c_uint64),
                 ("samples", C.c_uint64), ("accumulated_rays", C.c_uint64),
                 ("render_ms", C.c_double), ("cancelled", C.c_int32), ("denoise_fallback", C.c_int32),
-                ("done_ms", C.c_double), ("spill_pushes", C.c_uint64), ("spill_pops", C.c_uint64)]
+                ("done_ms", C.c_double), ("spill_pushes", C.c_uint64), ("spill_pops", C.c_uint64),
+                ("finish_rays", C.c_uint64), ("finish_node_visits", C.c_uint64), ("finish_tri_tests", C.c_uint64),
+                ("finish_instance_visits", C.c_uint64)]
 
     @property
     def rays(self):

@@ -372,6 +372,7 @@ int pico_hip_upload_scene_ex(pico_hip_ctx* c, const pico_scene_desc* d, uint32_t
     c->stack_need = need;
     c->d_mesh_box = mesh_box;
     c->d_mesh_vrange = mesh_vrange;
+    c->max_mesh_vertices = max_range(hs.mesh_vrange);
     c->d_instance_mesh = inst_mesh;
     c->tlas_base = hs.tlas_base;
     c->blas_bound = hs.blas_bound;

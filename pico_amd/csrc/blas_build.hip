@@ -1254,6 +1254,7 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     c->dev = s;
     c->d_mesh_box = mesh_box;
     c->d_mesh_vrange = mesh_vrange;
+    c->max_mesh_vertices = max_range(vrange);
     c->d_instance_mesh = inst_mesh;
     c->tlas_base = n_blas4;
     c->blas_bound = blas_bound;

@@ -62,6 +62,12 @@ uint32_t collapse_bvh4(const std::vector<DNode>& n2, uint32_t root2, std::vector
 std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out);
 // per mesh (first, end) vertex of its triangles; an empty range for a built-in shape
 std::vector<uint32_t> mesh_vertex_ranges(const pico_scene_desc& d);
+// the largest range of such (first, end) pairs
+inline uint32_t max_range(const std::vector<uint32_t>& r) {
+    uint32_t m = 0;
+    for (size_t k = 0; k + 1 < r.size(); k += 2) m = r[k + 1] - r[k] > m ? r[k + 1] - r[k] : m;
+    return m;
+}
 
 // The parts of build_scene that do not touch the triangles: the structural
 // checks of the description, and the materials, instances (inverses, no BLAS

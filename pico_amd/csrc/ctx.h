@@ -53,6 +53,7 @@ struct pico_hip_ctx {
     const uint32_t* d_instance_mesh = nullptr;
     const float* d_mesh_box = nullptr;
     const uint32_t* d_mesh_vrange = nullptr;  // per mesh (first, end) vertex in dev.pos; empty for built-in shapes
+    uint32_t max_mesh_vertices = 0;           // the largest of those ranges
     uint32_t tlas_base = 0;
     uint32_t blas_bound = 0;
     // reported by pico_hip_scene_info (either builder)

@@ -77,8 +77,12 @@ struct Ctl {
     unsigned long long nodes, tris, insts;           // extend (camera + extension rays)
     unsigned long long nodes_sh, tris_sh, insts_sh;  // shadow rays
     unsigned long long spill_push, spill_pop;        // traversal-stack entries past LDS (both kernels)
+    unsigned long long nodes_fin, tris_fin, insts_fin;  // k_finish's share of the node / triangle / instance counts
+    unsigned long long rays_fin;                        // rays traced by k_finish, every chunk
     uint32_t finish_from;                            // first bounce left to k_finish, or kNoFinish
-    unsigned long long fin_rays[kCtrSlots * kCtrStride];  // k_finish's extension / shadow rays (words 0, 1 of a slot)
+    // k_finish's extension rays, shadow rays (both: those no queue count
+    // holds) and every ray it traced: words 0, 1, 2 of a counter slot
+    unsigned long long fin_rays[kCtrSlots * kCtrStride];
 };
 
 struct Pool {
@@ -674,6 +678,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(DevScene s, const DView* __re
     // bb: the bounce of the path's next extension ray
     uint32_t item = kDeadPath, bb = 0, ph = 0, res_base = 0, res_left = 0;
     uint32_t rays_ext = 0, rays_sh = 0;  // rays no queue count holds
+    uint32_t traced = 0;
     bool drained = false;
     WaveClaim wc(&ctl->ext_head[b0], n, kTraceClaim);
     for (;;) {
@@ -692,6 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(DevScene s, const DView* __re
                     ph = kPhShadowOnly;
                 }
                 begin_path_ray(s, P, t, st, item, ph != kPhExt);
+                ++traced;
             }
             if (drained || !__ballot(item == kDeadPath)) break;
         }
@@ -708,6 +714,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(DevScene s, const DView* __re
                 if (ph == kPhShadowThenExt) {
                     ph = kPhExt;
                     begin_path_ray(s, P, t, st, item, false);
+                    ++traced;
                 } else {
                     item = kDeadPath;
                 }
@@ -719,8 +726,10 @@ __global__ __launch_bounds__(kBlock) void k_finish(DevScene s, const DView* __re
                 if (r & 2u) {
                     ph = (r & 1u) ? kPhShadowThenExt : kPhShadowOnly;
                     begin_path_ray(s, P, t, st, item, true);
+                    ++traced;
                 } else if (r & 1u) {
                     begin_path_ray(s, P, t, st, item, false);
+                    ++traced;
                 } else {
                     item = kDeadPath;
                 }
@@ -730,13 +739,18 @@ __global__ __launch_bounds__(kBlock) void k_finish(DevScene s, const DView* __re
     for (int o = 32; o > 0; o >>= 1) {
         rays_ext += __shfl_xor(rays_ext, o);
         rays_sh += __shfl_xor(rays_sh, o);
+        traced += __shfl_xor(traced, o);
     }
-    if (lane_id() == 0 && (rays_ext | rays_sh)) {
+    if (lane_id() == 0 && traced) {
         unsigned long long* slot = ctl->fin_rays + kCtrStride * ((gtid >> 6) % kCtrSlots);
         atomicAdd(slot, (unsigned long long)rays_ext);
         atomicAdd(slot + 1, (unsigned long long)rays_sh);
+        atomicAdd(slot + 2, (unsigned long long)traced);
     }
     if (COUNT) {
+        atomicAdd(&ctl->nodes_fin, (unsigned long long)(tce.nodes + tcs.nodes));
+        atomicAdd(&ctl->tris_fin, (unsigned long long)(tce.tris + tcs.tris));
+        atomicAdd(&ctl->insts_fin, (unsigned long long)(tce.insts + tcs.insts));
         atomicAdd(&ctl->nodes, (unsigned long long)tce.nodes);
         atomicAdd(&ctl->tris, (unsigned long long)tce.tris);
         atomicAdd(&ctl->insts, (unsigned long long)tce.insts);
@@ -919,8 +933,10 @@ __global__ void k_sum_rays(Ctl* ctl, uint32_t bounces) {
         for (uint32_t k = 0; k < kCtrSlots; ++k) {
             e += ctl->fin_rays[kCtrStride * k];
             sh += ctl->fin_rays[kCtrStride * k + 1];
+            ctl->rays_fin += ctl->fin_rays[kCtrStride * k + 2];
             ctl->fin_rays[kCtrStride * k] = 0;
             ctl->fin_rays[kCtrStride * k + 1] = 0;
+            ctl->fin_rays[kCtrStride * k + 2] = 0;
         }
         ctl->rays_cam += ctl->ext_count[0];
         ctl->rays_ext += e;
@@ -1204,6 +1220,10 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
                 st.shadow_instance_visits = hc.insts_sh;
                 st.spill_pushes = hc.spill_push;
                 st.spill_pops = hc.spill_pop;
+                st.finish_rays = hc.rays_fin;
+                st.finish_node_visits = hc.nodes_fin;
+                st.finish_tri_tests = hc.tris_fin;
+                st.finish_instance_visits = hc.insts_fin;
                 for (uint32_t k = 0; k < nv; ++k) {
                     st.accumulated_rays += hs[2 * k];
                     st.samples += hs[2 * k + 1];

@@ -9,8 +9,8 @@
 // Steps (one stream, one host synchronisation at the end):
 //   1  per instance: affine check, float cofactor inverse (bit-identical to the
 //      host build) (staged, not yet visible); world box of its mesh's vertices
-//      (one workgroup per instance; a built-in shape: its box corners), as the
-//      host build bounds an instance
+//      (chunks of vertices, one workgroup each; a built-in shape: its box
+//      corners), as the host build bounds an instance
 //   2  centroid bounds, 30-bit Morton code | instance id as a unique 64-bit key
 //   3  rocPRIM radix sort of the keys
 //   4  Karras hierarchy: n - 1 binary nodes, children and parents
@@ -45,6 +45,7 @@ struct TlasBuf {
     const float* pos;         // vertex positions
     float* inv;               // staged inverse rows, n x 12
     float* box;               // instance world boxes, n x 6
+    uint32_t* wbox;           // the same as order-preserving integers while chunks combine
     unsigned long long* key;  // n (unsorted), then key_s sorted
     unsigned long long* key_s;
     uint32_t* child;          // (n - 1) x 2: internal index or kLeafBit | sorted leaf position
@@ -92,19 +93,32 @@ __global__ void k_tlas_instances(TlasBuf t) {
     }
 }
 
-// step 1: world boxes, one workgroup of 256 per instance: the mesh's vertices
-// through the transform (min / max are exact, so the box is the host
-// build's), or a built-in shape's box corners; an invalid transform (the
-// rebuild is then refused) gets an empty box at the origin
-__global__ __launch_bounds__(256) void k_tlas_world_boxes(TlasBuf t) {
+// step 1: world boxes.  An instance's vertices are split into chunks of
+// kBoxChunk, one workgroup of 256 each (a single instance of a 2M-triangle
+// mesh is 245 chunks); each chunk's box goes into the instance's box through
+// order-preserving integer atomics, so the result is the exact minimum and
+// maximum, the host build's box, whatever the order.
+constexpr uint32_t kBoxChunk = 4096;
+
+__global__ void k_tlas_wbox_init(TlasBuf t) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 3; ++k) { t.wbox[6 * (size_t)i + k] = kMappedPosInf; t.wbox[6 * (size_t)i + 3 + k] = kMappedNegInf; }
+}
+
+// workgroup w: instance w / chunks, chunk w % chunks of its mesh's vertices
+__global__ __launch_bounds__(256) void k_tlas_world_boxes(TlasBuf t, uint32_t chunks) {
     __shared__ float red[6][4];
     const uint32_t wave = threadIdx.x >> 6;
-    for (uint32_t i = blockIdx.x; i < t.n; i += gridDim.x) {
+    for (size_t w = blockIdx.x; w < (size_t)t.n * chunks; w += gridDim.x) {
+        const uint32_t i = (uint32_t)(w / chunks), c = (uint32_t)(w % chunks);
         const float* M = t.xform + 16 * (size_t)i;
         const uint32_t m = t.mesh[i];
         const uint32_t v0 = t.vrange[2 * m], v1 = t.vrange[2 * m + 1];
+        const uint32_t c0 = v0 + c * kBoxChunk;
+        if (c0 >= v1) continue;  // uniform over the workgroup
+        const uint32_t c1 = min(v1, c0 + kBoxChunk);
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) grow_world_box(M, t.pos + 3 * (size_t)v, lo, hi);
+        for (uint32_t v = c0 + threadIdx.x; v < c1; v += blockDim.x) grow_world_box(M, t.pos + 3 * (size_t)v, lo, hi);
         for (int o = 32; o > 0; o >>= 1)
             for (int k = 0; k < 3; ++k) {
                 lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
@@ -113,28 +127,42 @@ __global__ __launch_bounds__(256) void k_tlas_world_boxes(TlasBuf t) {
         if ((threadIdx.x & 63) == 0)
             for (int k = 0; k < 3; ++k) { red[k][wave] = lo[k]; red[3 + k][wave] = hi[k]; }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            float inv[3][4];
-            const bool ok = M[3] == 0.0f && M[7] == 0.0f && M[11] == 0.0f && M[15] == 1.0f && affine_inverse_rows(M, inv);
-            float blo[3], bhi[3];
-            if (!ok) {
-                for (int k = 0; k < 3; ++k) { blo[k] = 0.0f; bhi[k] = 0.0f; }
-            } else if (v0 < v1) {
-                for (int k = 0; k < 3; ++k) {
-                    blo[k] = fminf(fminf(red[k][0], red[k][1]), fminf(red[k][2], red[k][3]));
-                    bhi[k] = fmaxf(fmaxf(red[3 + k][0], red[3 + k][1]), fmaxf(red[3 + k][2], red[3 + k][3]));
-                }
-                widen_world_box(blo, bhi);
-            } else {
-                const float* mb = t.mesh_box + 6 * (size_t)m;
-                instance_world_box(M, mb, mb + 3, blo, bhi);
-            }
-            for (int k = 0; k < 3; ++k) {
-                t.box[6 * (size_t)i + k] = blo[k];
-                t.box[6 * (size_t)i + 3 + k] = bhi[k];
-            }
+        if (threadIdx.x < 3) {
+            const int k = threadIdx.x;
+            atomic_min_f(&t.wbox[6 * (size_t)i + k], fminf(fminf(red[k][0], red[k][1]), fminf(red[k][2], red[k][3])));
+            atomic_max_f(&t.wbox[6 * (size_t)i + 3 + k],
+                         fmaxf(fmaxf(red[3 + k][0], red[3 + k][1]), fmaxf(red[3 + k][2], red[3 + k][3])));
         }
-        __syncthreads();  // red is reused by the next instance
+        __syncthreads();  // red is reused by the next chunk
+    }
+}
+
+// per instance: the widened box of its vertices, a built-in shape's box
+// corners, or an empty box at the origin for an invalid transform (the
+// rebuild is then refused)
+__global__ void k_tlas_world_boxes_done(TlasBuf t) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += gridDim.x * blockDim.x) {
+        const float* M = t.xform + 16 * (size_t)i;
+        const uint32_t m = t.mesh[i];
+        float inv[3][4];
+        const bool ok = M[3] == 0.0f && M[7] == 0.0f && M[11] == 0.0f && M[15] == 1.0f && affine_inverse_rows(M, inv);
+        float lo[3], hi[3];
+        if (!ok) {
+            for (int k = 0; k < 3; ++k) { lo[k] = 0.0f; hi[k] = 0.0f; }
+        } else if (t.vrange[2 * m] < t.vrange[2 * m + 1]) {
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = unmap_f(t.wbox[6 * (size_t)i + k]);
+                hi[k] = unmap_f(t.wbox[6 * (size_t)i + 3 + k]);
+            }
+            widen_world_box(lo, hi);
+        } else {
+            const float* mb = t.mesh_box + 6 * (size_t)m;
+            instance_world_box(M, mb, mb + 3, lo, hi);
+        }
+        for (int k = 0; k < 3; ++k) {
+            t.box[6 * (size_t)i + k] = lo[k];
+            t.box[6 * (size_t)i + 3 + k] = hi[k];
+        }
     }
 }
 
@@ -368,6 +396,7 @@ int tlas_rebuild(pico_hip_ctx* c, uint32_t n, const float* d_transforms, pico_re
     const size_t m = std::max<uint32_t>(n, 2);
     t.inv = (float*)scratch(c, "tlas_inv", m * 48);
     t.box = (float*)scratch(c, "tlas_box", m * 24);
+    t.wbox = (uint32_t*)scratch(c, "tlas_wbox", m * 24);
     t.key = (unsigned long long*)scratch(c, "tlas_key", m * 8);
     t.key_s = (unsigned long long*)scratch(c, "tlas_key_s", m * 8);
     t.child = (uint32_t*)scratch(c, "tlas_child", m * 8);
@@ -380,7 +409,7 @@ int tlas_rebuild(pico_hip_ctx* c, uint32_t n, const float* d_transforms, pico_re
     t.staged = (DNode4*)scratch(c, "tlas_staged", m * sizeof(DNode4));
     t.info = (uint32_t*)scratch(c, "tlas_info", 64);
     uint32_t* ok_flag = (uint32_t*)scratch(c, "tlas_ok", 64);
-    if (!t.inv || !t.box || !t.key || !t.key_s || !t.child || !t.parent || !t.nbox || !t.visits || !t.bnd || !t.flag ||
+    if (!t.inv || !t.box || !t.wbox || !t.key || !t.key_s || !t.child || !t.parent || !t.nbox || !t.visits || !t.bnd || !t.flag ||
         !t.wide || !t.staged || !t.info || !ok_flag)
         return fail(c, PICO_ERR_NOMEM, "device allocation failed");
     hipEvent_t e0, e1;
@@ -390,7 +419,11 @@ int tlas_rebuild(pico_hip_ctx* c, uint32_t n, const float* d_transforms, pico_re
     PICO_HIP(c, hipEventRecord(e0, c->stream));
     PICO_LAUNCH(c, "tlas_init", k_tlas_init, grid_of(n), 256, 0, t);
     PICO_LAUNCH(c, "tlas_instances", k_tlas_instances, grid_of(n), 256, 0, t);
-    PICO_LAUNCH(c, "tlas_world_boxes", k_tlas_world_boxes, std::min<uint32_t>(std::max<uint32_t>(n, 1), 65535), 256, 0, t);
+    const uint32_t chunks = std::max<uint32_t>(1, (c->max_mesh_vertices + kBoxChunk - 1) / kBoxChunk);
+    PICO_LAUNCH(c, "tlas_wbox_init", k_tlas_wbox_init, grid_of(n), 256, 0, t);
+    PICO_LAUNCH(c, "tlas_world_boxes", k_tlas_world_boxes,
+                (uint32_t)std::min<size_t>(std::max<size_t>((size_t)n * chunks, 1), 65535), 256, 0, t, chunks);
+    PICO_LAUNCH(c, "tlas_world_boxes_done", k_tlas_world_boxes_done, grid_of(n), 256, 0, t);
     PICO_LAUNCH(c, "tlas_centroid_bounds", k_tlas_centroid_bounds, std::min<uint32_t>(grid_of(n), 64), 256, 0, t);
     PICO_LAUNCH(c, "tlas_keys", k_tlas_keys, grid_of(n), 256, 0, t);
     size_t tmp = 0;
