@@ -169,6 +169,21 @@ struct TraceStack {
         }
         ++sp;
     }
+    // Pushes e0, ..., e(n-1) (n <= 3) in that order.  While all three slots
+    // above the top are in LDS they are all written, whatever n is (the slots
+    // above the new top are free): no branch per entry.
+    __device__ __forceinline__ void push3(uint32_t n, uint32_t e0, uint32_t e1, uint32_t e2, uint32_t* overflow) {
+        if (sp + 3 <= STACK) {
+            lds[sp * lds_stride] = e0;
+            lds[(sp + 1) * lds_stride] = e1;
+            lds[(sp + 2) * lds_stride] = e2;
+            sp += n;
+        } else {
+            if (n > 0) push(e0, overflow);
+            if (n > 1) push(e1, overflow);
+            if (n > 2) push(e2, overflow);
+        }
+    }
     __device__ __forceinline__ uint32_t pop() {
         --sp;
         if (sp < STACK) return lds[sp * lds_stride];
@@ -346,9 +361,9 @@ __device__ __forceinline__ bool inner_node(const DevScene& s, TraceState& t, Tra
     const int nh = (key[0] != INFINITY) + (key[1] != INFINITY) + (key[2] != INFINITY) + (key[3] != INFINITY);
     if (nh == 0) return false;
     if (COUNT && nh > 1) cnt->spill_push += (uint32_t)max(0, (int)st.sp + nh - 1 - max((int)st.sp, STACK));
-    if (nh > 3) st.push(cref[3], overflow);
-    if (nh > 2) st.push(cref[2], overflow);
-    if (nh > 1) st.push(cref[1], overflow);
+    // the other hit children, farthest first: c3, c2, c1 of those that hit
+    const uint32_t np = (uint32_t)nh - 1u;
+    st.push3(np, np == 3 ? cref[3] : (np == 2 ? cref[2] : cref[1]), np == 3 ? cref[2] : cref[1], cref[1], overflow);
     t.ref = cref[0];
     return true;
 }
