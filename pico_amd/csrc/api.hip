@@ -363,6 +363,11 @@ int pico_hip_upload_scene_ex(pico_hip_ctx* c, const pico_scene_desc* d, uint32_t
     if ((rc = upload_vec(c, hs.positions, &s.pos))) return rc;
     if ((rc = upload_vec(c, hs.normals, &s.nrm))) return rc;
     if ((rc = upload_vec(c, hs.indices, &s.idx))) return rc;
+    {
+        const float* ts = nullptr;
+        if ((rc = upload_vec(c, hs.tshade, &ts))) return rc;
+        s.tshade = reinterpret_cast<const float4*>(ts);
+    }
     PICO_HIP(c, hipStreamSynchronize(c->stream));
     s.tlas_root = hs.tlas_root;
     s.n_lights = (uint32_t)hs.lights.size();

@@ -74,6 +74,8 @@ struct Geo {
     uint32_t* rbox;         // 12 per mesh, mapped: triangle box union, centroid bounds
     uint32_t* err;
     const uint32_t* shape;  // per mesh PICO_SHAPE_*, or null
+    const float* nrm;       // vertex normals
+    float4* tshade;         // per triangle shading record (written)
 };
 
 struct Bins;
@@ -185,6 +187,19 @@ __global__ void k_geo_tris(Geo g) {
                 lo[k] = p[k] < lo[k] ? p[k] : lo[k];
                 hi[k] = hi[k] < p[k] ? p[k] : hi[k];
             }
+        }
+        {
+            // shading record: positions, then normals, of the three vertices
+            float r[20];
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < 3; ++k) {
+                    r[3 * c + k] = g.pos[3 * (size_t)g.gidx[3 * (size_t)t + c] + k];
+                    r[9 + 3 * c + k] = g.nrm[3 * (size_t)g.gidx[3 * (size_t)t + c] + k];
+                }
+            r[18] = 0.0f;
+            r[19] = 0.0f;
+            for (uint32_t q = 0; q < kShadeRec; ++q)
+                g.tshade[kShadeRec * (size_t)t + q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
         }
         for (int k = 0; k < 3; ++k) {
             g.pbox[6 * (size_t)t + k] = lo[k];
@@ -1167,7 +1182,10 @@ static int build_device(pico_hip_ctx* c, const pico_scene_desc& d, pico_build_st
     PICO_HIP(c, hipEventCreate(&e1));
     struct Ev { hipEvent_t a, b; ~Ev() { hipEventDestroy(a); hipEventDestroy(b); } } ev{e0, e1};
     PICO_HIP(c, hipEventRecord(e0, c->stream));
-    Geo g{s.pos, lidx, gidx, vofs, tofs, nm, nt, nv, pbox, pcen, perm[0], mbox, rbox, counters, shapes};
+    float4* tshade = scene_alloc<float4>(c, kShadeRec * (size_t)nt);
+    if (!tshade) return fail(c, PICO_ERR_NOMEM, "device allocation failed");
+    s.tshade = tshade;
+    Geo g{s.pos, lidx, gidx, vofs, tofs, nm, nt, nv, pbox, pcen, perm[0], mbox, rbox, counters, shapes, s.nrm, tshade};
     PICO_LAUNCH(c, "bvh_geo_clear", k_geo_clear, grid_for(nm), 256, 0, g);
     PICO_LAUNCH(c, "bvh_geo_tris", k_geo_tris, grid_for(nt), 256, 0, g);
     PICO_LAUNCH(c, "bvh_tri_bounds", k_tri_bounds, runs_grid(nt), 256, 0, g);

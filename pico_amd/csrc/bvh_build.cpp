@@ -467,6 +467,8 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
         }
     }
     out.blas_nodes2 = (uint32_t)out.nodes.size();
+    out.tshade.resize(4 * kShadeRec * (size_t)nt);
+    shade_records(out.indices.data(), out.positions.data(), out.normals.data(), 0, nt, out.tshade.data());
     e = scene_tables(d, out);
     if (!e.empty()) return e;
     // TLAS over the instance world boxes: a mesh instance's vertices through

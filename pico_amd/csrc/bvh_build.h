@@ -22,6 +22,7 @@ struct HostScene {
     std::vector<float> positions;      // 3 x f32 per vertex (global)
     std::vector<float> normals;        // 3 x f32 per vertex (global)
     std::vector<uint32_t> indices;     // 3 x u32 per triangle, global vertex ids
+    std::vector<float> tshade;         // 4 x kShadeRec f32 per triangle: its vertices' positions and normals
     std::vector<uint32_t> mesh_tri_offset;
     std::vector<float> mesh_box;       // 6 x f32 per mesh: object-space vertex bounds lo, hi
     std::vector<uint32_t> mesh_vrange; // 2 x u32 per mesh: first and end vertex (empty for the built-in shapes)
@@ -62,6 +63,20 @@ uint32_t collapse_bvh4(const std::vector<DNode>& n2, uint32_t root2, std::vector
 std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out);
 // per mesh (first, end) vertex of its triangles; an empty range for a built-in shape
 std::vector<uint32_t> mesh_vertex_ranges(const pico_scene_desc& d);
+// the shading records of triangles [t0, t1) (global indices `idx`, global
+// positions and normals) into out[4 * kShadeRec * t]
+inline void shade_records(const uint32_t* idx, const float* pos, const float* nrm, size_t t0, size_t t1, float* out) {
+    for (size_t t = t0; t < t1; ++t) {
+        float* r = out + 4 * kShadeRec * t;
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 3; ++k) {
+                r[3 * c + k] = pos[3 * (size_t)idx[3 * t + c] + k];
+                r[9 + 3 * c + k] = nrm[3 * (size_t)idx[3 * t + c] + k];
+            }
+        r[18] = 0.0f;
+        r[19] = 0.0f;
+    }
+}
 // the largest range of such (first, end) pairs
 inline uint32_t max_range(const std::vector<uint32_t>& r) {
     uint32_t m = 0;

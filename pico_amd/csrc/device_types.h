@@ -25,6 +25,10 @@ constexpr uint32_t kPrimMask = (1u << 27) - 1u;
 
 inline constexpr uint32_t make_leaf(uint32_t first, uint32_t count) { return kLeafBit | (count << 27) | first; }
 
+// Shading record of a triangle: positions p0, p1, p2 then normals n0, n1, n2
+// of its vertices (18 floats), padded to kShadeRec float4.
+constexpr uint32_t kShadeRec = 5;
+
 // BLAS root of an instance of a built-in implicit shape: kImplicitRoot | PICO_SHAPE_*
 // (no nodes; the traversal intersects the shape directly).  Lights on such a
 // shape have n_tris == 0 and the shape in tri_offset.

@@ -287,8 +287,9 @@ __device__ __forceinline__ f3 mesh_light_point(const DevScene& s, const DLight& 
     const float2 xi = rng.next2d();
     const float su0 = sqrtf(xi.x);
     const float bx = 1 - su0, by = xi.y * su0;
-    const uint32_t t = L.tri_offset + tri;
-    const f3 p0 = ld3(s.pos + 3 * s.idx[3 * t]), p1 = ld3(s.pos + 3 * s.idx[3 * t + 1]), p2 = ld3(s.pos + 3 * s.idx[3 * t + 2]);
+    const float4* r = s.tshade + (size_t)kShadeRec * (L.tri_offset + tri);
+    const float4 a = r[0], b = r[1], c = r[2];
+    const f3 p0 = mk(a.x, a.y, a.z), p1 = mk(a.w, b.x, b.y), p2 = mk(b.z, b.w, c.x);
     return (((1.0f - bx - by) * p0) + (bx * p1)) + (by * p2);
 }
 

@@ -107,6 +107,11 @@ struct DevScene {
     const float* pos;
     const float* nrm;
     const uint32_t* idx;
+    // per triangle (global index, mesh order) its shading record: the three
+    // vertex positions then the three normals, 18 floats padded to 5 float4
+    // (kShadeRec), so a shading point loads one record instead of three
+    // indices and six vertex gathers
+    const float4* tshade;
     uint32_t tlas_root;
     uint32_t n_lights;
     float sky[3];
@@ -533,10 +538,11 @@ __device__ __forceinline__ Frag implicit_frag(const DevScene& s, const Hit& h, f
 __device__ __forceinline__ Frag make_frag(const DevScene& s, const Hit& h) {
     const DInstance& I = s.inst[h.inst];
     const uint32_t t = I.tri_offset + h.prim;
-    const uint32_t i0 = s.idx[3 * t], i1 = s.idx[3 * t + 1], i2 = s.idx[3 * t + 2];
+    const float4* r = s.tshade + (size_t)kShadeRec * t;
+    const float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
     const float w = 1.0f - h.v - h.u;
-    const f3 P = ((w * ld3(s.pos + 3 * i0)) + (h.u * ld3(s.pos + 3 * i1))) + (h.v * ld3(s.pos + 3 * i2));
-    const f3 N = normalize(((w * ld3(s.nrm + 3 * i0)) + (h.u * ld3(s.nrm + 3 * i1))) + (h.v * ld3(s.nrm + 3 * i2)));
+    const f3 P = ((w * mk(a.x, a.y, a.z)) + (h.u * mk(a.w, b.x, b.y))) + (h.v * mk(b.z, b.w, c.x));
+    const f3 N = normalize(((w * mk(c.y, c.z, c.w)) + (h.u * mk(d.x, d.y, d.z))) + (h.v * mk(d.w, e.x, e.y)));
     const float* M = s.xf[h.inst].m;
     Frag f;
     // (c0*x + c1*y) + (c2*z + c3*1)
