@@ -10,8 +10,9 @@
 //             -> hit records, and of the shadow rays of bounce b - 1 -> a light
 //             hit adds radiance
 //   shade     emission, BSDF sample, NEE light sample, RR       -> shadow queue b, extend queue b+1
-//   finalize  per-path radiance (NaN -> pink), ray count
-//   accumulate  the reference's running mean / variance and early-out, in sample order
+//   finalize  (views that return their samples) per-path radiance (NaN -> pink)
+//   accumulate  the reference's running mean / variance and early-out, in
+//             sample order, from each path's radiance (NaN -> pink)
 //   compact   surviving items -> next chunk's active list
 // Every count lives in device memory; the host never synchronises inside a
 // render.  Queue appends are wave-aggregated (ballot + popcount, one atomic per
@@ -100,7 +101,6 @@ struct Pool {
     uint32_t* q0;
     uint32_t* q1;
     uint32_t* shq;
-    float4* samp;    // finalized radiance + rays
     float4* ior;     // index-of-refraction stack ring; its count is in thr.w (bits)
 };
 
@@ -799,6 +799,17 @@ struct BlockViewStats {
     }
 };
 
+// A path's finished radiance and its rays traced: a camera ray that hit and
+// came back NaN or infinite is shown pink (Integrators.cpp:145-146).
+__device__ __forceinline__ float4 path_sample(const Pool& P, size_t i) {
+    const float4 r = P.rad[i];
+    f3 res = xyz(r);
+    if (P.ray_o[i].w == 0.0f && bad(res)) res = mk(1.0f, 0.4f, 0.7f);
+    return make_float4(res.x, res.y, res.z, r.w);
+}
+
+// Views that return their samples (MODE_SAMPLES): every sample to the
+// caller's array.  Accumulating views read their samples in k_accumulate.
 __global__ __launch_bounds__(kBlock) void k_finalize(const DView* __restrict__ views, const uint32_t* __restrict__ active,
                                                      const Ctl* ctl, uint32_t S, uint32_t off, Pool P) {
     __shared__ BlockViewStats bvs;
@@ -814,19 +825,13 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DView* __restrict__ v
             const uint32_t key = active[k];
             const DView& V = views[key >> kViewShift];
             const uint32_t rel = off + j;
-            if (rel < V.spp) {
-                const float4 r = P.rad[i];
-                f3 res = xyz(r);
-                if (P.ray_o[i].w == 0.0f && bad(res)) res = mk(1.0f, 0.4f, 0.7f);  // Integrators.cpp:145-146
-                if (V.mode == MODE_SAMPLES) {
-                    const uint32_t p = key & kPixMask;
-                    float* o = V.samples_out + ((size_t)rel * V.cam.w * V.cam.h + p) * 3;
-                    o[0] = res.x; o[1] = res.y; o[2] = res.z;
-                    vstat = key >> kViewShift;
-                    rays = (unsigned long long)r.w;
-                } else {
-                    P.samp[i] = make_float4(res.x, res.y, res.z, r.w);
-                }
+            if (rel < V.spp && V.mode == MODE_SAMPLES) {
+                const float4 x = path_sample(P, i);
+                const uint32_t p = key & kPixMask;
+                float* o = V.samples_out + ((size_t)rel * V.cam.w * V.cam.h + p) * 3;
+                o[0] = x.x; o[1] = x.y; o[2] = x.z;
+                vstat = key >> kViewShift;
+                rays = (unsigned long long)x.w;
             }
         }
         bvs.add(vstat, rays, vstat != 0xffffffffu ? 1ull : 0ull);
@@ -857,7 +862,7 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const DView* __restrict__
                 if (V.mode == MODE_SUM64) {
                     double sx = V.sum64[3 * p], sy = V.sum64[3 * p + 1], sz = V.sum64[3 * p + 2];
                     for (uint32_t j = 0; j < S && off + j < V.spp; ++j) {
-                        const float4 x = P.samp[(size_t)j * na + k];
+                        const float4 x = path_sample(P, (size_t)j * na + k);
                         sx += (double)x.x; sy += (double)x.y; sz += (double)x.z;
                         rays += (unsigned long long)x.w;
                         ++ns;
@@ -867,7 +872,7 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const DView* __restrict__
                     uint32_t cnt = V.counts[p];
                     f3 pix = ld3(V.pixels + 3 * p), var = ld3(V.variance + 3 * p);
                     for (uint32_t j = 0; j < S && off + j < V.spp; ++j) {
-                        const float4 x4 = P.samp[(size_t)j * na + k];
+                        const float4 x4 = path_sample(P, (size_t)j * na + k);
                         f3 x = xyz(x4);
                         const uint32_t prev = cnt;
                         if (prev < 1) {
@@ -996,6 +1001,8 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     uint32_t cap = tune.queue_paths ? tune.queue_paths : kDefaultQueuePaths;
     cap = std::max<uint32_t>(cap, 64);
     const bool accum_mode = hv[0].mode == MODE_ACCUM;
+    bool samples_mode = false;
+    for (auto& v : hv) samples_mode = samples_mode || v.mode == MODE_SAMPLES;
     // Chunk schedule, fixed on the host: a chunk gives every pixel of the
     // views still short of their sample count S more samples, S = path budget
     // / those pixels (views with fewer samples drop out, so later chunks take
@@ -1015,7 +1022,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         off += S;
     }
     // buffers
-    char* pb = (char*)scratch(c, "pool", paths * (16 * 11 + 4 * 6) + 8192);
+    char* pb = (char*)scratch(c, "pool", paths * (16 * 10 + 4 * 6) + 8192);
     uint32_t* act = (uint32_t*)scratch(c, "active", pix_total * 8 + 64);
     Ctl* ctl = (Ctl*)scratch(c, "ctl", sizeof(Ctl));
     uint32_t* count0 = (uint32_t*)scratch(c, "count0", pix_total * 4 + 64);
@@ -1049,7 +1056,6 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         P.rng = (uint4*)take(paths * 16); P.hit = (float4*)take(paths * 16);
         P.sh_o = (float4*)take(paths * 16); P.sh_d = (float4*)take(paths * 16);
         P.sh_c = (float4*)take(paths * 16);
-        P.samp = (float4*)take(paths * 16);
         P.ior = (float4*)take(paths * 16);
         P.key = (uint32_t*)take(paths * 4); P.hprim = (uint32_t*)take(paths * 4);
         P.q0 = (uint32_t*)take(paths * 4); P.q1 = (uint32_t*)take(paths * 4); P.shq = (uint32_t*)take(paths * 4);
@@ -1127,7 +1133,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         if (count) PICO_LAUNCH(c, "finish", (k_finish<true>), grid_finish, kBlock, 0, c->dev, dv, ctl, P, spill, spill_cap);
         else PICO_LAUNCH(c, "finish", (k_finish<false>), grid_finish, kBlock, 0, c->dev, dv, ctl, P, spill, spill_cap);
         PICO_LAUNCH(c, "sum_rays", k_sum_rays, 1, 64, 0, ctl, pl.max_depth);
-        PICO_LAUNCH(c, "finalize", k_finalize, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
+        if (samples_mode) PICO_LAUNCH(c, "finalize", k_finalize, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         PICO_LAUNCH(c, "accumulate", k_accumulate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         PICO_LAUNCH(c, "compact", k_compact, grid_lin, kBlock, 0, dv, act_a, act_b, ctl);
         std::swap(act_a, act_b);
