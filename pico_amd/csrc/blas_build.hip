@@ -496,7 +496,7 @@ __device__ void finish_decision(const Bins* b, const BuildTask& t, float best_co
         for (int j = 0; j < 3; ++j) { bounds.lo[j] = t.box[j]; bounds.hi[j] = t.box[3 + j]; }
         const float parent = bounds.area();
         const float leaf_cost = (float)n * parent;
-        const float split_cost = parent + best_cost;
+        const float split_cost = kSahTraversal * parent + best_cost;
         if (best_axis < 0) kind = n <= kMaxLeaf ? 0 : 2;
         else kind = (n <= kMaxLeaf && leaf_cost <= split_cost) ? 0 : 1;
     }

@@ -96,7 +96,7 @@ struct Builder {
         }
         const float parent = bounds.area();
         const float leaf_cost = (float)n * parent;
-        const float split_cost = parent + best_cost;  // traversal cost 1, intersection cost 1
+        const float split_cost = kSahTraversal * parent + best_cost;
         if (best_axis < 0) {
             // all centroids coincide: leaf if it fits, else split the index range
             return n <= max_leaf ? e : b + n / 2;
