@@ -519,7 +519,7 @@ std::string build_scene(const pico_scene_desc& d, uint32_t bins, HostScene& out)
         out.tlas_base = (uint32_t)out.nodes4.size();
         out.blas_bound = blas_bound;
         out.tlas_root = collapse_bvh4(out.nodes, out.tlas_root, out.nodes4, tb);
-        out.stack_bound = tb + 1 + blas_bound;  // TLAS entries, the instance sentinel, BLAS entries
+        out.stack_bound = tb + blas_bound;  // TLAS entries, then BLAS entries
     }
     return "";
 }

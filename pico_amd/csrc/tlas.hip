@@ -357,7 +357,7 @@ __global__ void k_tlas_single(TlasBuf t) {
 // step 8: publish when valid
 __global__ void k_tlas_commit(TlasBuf t, uint32_t n_wide_max, uint32_t blas_bound, DNode4* nodes, DInstance* inst,
                               DXform* xf, uint32_t* out_ok) {
-    const uint32_t need = (t.n > 1 ? t.bnd[0] : 0u) + 1u + blas_bound;  // TLAS, instance sentinel, BLAS
+    const uint32_t need = (t.n > 1 ? t.bnd[0] : 0u) + blas_bound;  // TLAS entries, then BLAS entries
     const bool ok = t.info[0] == 0u && need <= 64u;
     const uint32_t n_wide = t.n > 1 ? t.wide[t.n - 2] + t.flag[t.n - 2] : 1u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {

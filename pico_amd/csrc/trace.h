@@ -127,7 +127,8 @@ struct Hit { float t, u, v; uint32_t inst, prim; };
 
 struct TraceCounters { uint32_t nodes, tris, insts, spill_push, spill_pop; };
 
-constexpr uint32_t kSentinel = 0xfffffffeu;
+// TraceState::bsp of a ray in the TLAS (no stack depth is this large)
+constexpr uint32_t kWorld = 0xffffffffu;
 
 __device__ __forceinline__ float4 ldg4(const void* p) { return *reinterpret_cast<const float4*>(p); }
 
@@ -280,15 +281,16 @@ constexpr int kLeafTris = 1;
 // four quantised child boxes (planes origin + q 2^e, which contain the exact
 // boxes) with the slab test, the plane distances taken as
 // fma(q, 2^e inv, origin inv - o inv) per axis, continues with the nearest hit
-// child and pushes the others farthest first; an instance boundary pushes a
-// sentinel that restores the world-space ray when popped.  t is the ray
+// child and pushes the others farthest first; the stack depth at an instance
+// entry is kept, and the world-space ray is restored when the stack is back
+// at that depth.  t is the ray
 // parameter of the world ray (directions are unit length, so t is the hit
 // distance).  Hits at or beyond `tmax` are not reported (infinity for the
 // reference's unbounded query).  Box tests widen the far distance by 2^-21
 // relative so that rounding never culls a triangle whose bound the ray only
 // grazes.
 //
-// Traversal state of one ray, advanced one node, leaf or instance at a time
+// Traversal state of one ray, advanced a few nodes, a leaf or an instance at a time
 // by trace_step, so that a persistent kernel can hand a lane a new ray as
 // soon as its current one is finished.
 struct TraceState {
@@ -298,7 +300,7 @@ struct TraceState {
     f3 winv, woi; // the same of the world ray, restored on leaving an instance
     Hit h;
     uint32_t ref, cur;
-    bool in_blas;
+    uint32_t bsp;  // stack depth at the instance entry (kWorld: in the TLAS)
 };
 
 template <int STACK>
@@ -312,7 +314,7 @@ __device__ __forceinline__ void trace_begin(const DevScene& s, TraceState& t, Tr
     t.h.t = tmax; t.h.u = 0.0f; t.h.v = 0.0f; t.h.inst = 0xffffffffu; t.h.prim = 0xffffffffu;
     t.ref = s.tlas_root;
     t.cur = 0xffffffffu;
-    t.in_blas = false;
+    t.bsp = kWorld;
     st.sp = 0;
 }
 
@@ -421,8 +423,8 @@ __device__ __forceinline__ bool leaf_tris(const DevScene& s, TraceState& t, uint
 }
 
 // TLAS leaf t.ref (one instance): a built-in shape is intersected at once
-// (returns false: nothing to descend into); a mesh instance pushes the
-// instance sentinel, moves the ray into object space and makes the BLAS root
+// (returns false: nothing to descend into); a mesh instance records the stack
+// depth of its entry, moves the ray into object space and makes the BLAS root
 // the next item (returns true).
 template <int STACK, bool COUNT>
 __device__ __forceinline__ bool enter_instance(const DevScene& s, TraceState& t, TraceStack<STACK>& st,
@@ -449,13 +451,11 @@ __device__ __forceinline__ bool enter_instance(const DevScene& s, TraceState& t,
         return false;
     }
     t.cur = cur;
-    if (COUNT && st.sp >= STACK) cnt->spill_push++;
-    st.push(kSentinel, overflow);
+    t.bsp = st.sp;
     t.ro = ro;
     t.rd = rd;
     t.inv = slab_rcp(t.rd);
     t.oi = mk(-t.ro.x * t.inv.x, -t.ro.y * t.inv.y, -t.ro.z * t.inv.z);
-    t.in_blas = true;
     t.ref = root;
     return true;
 }
@@ -464,7 +464,7 @@ __device__ __forceinline__ void leave_instance(TraceState& t) {
     t.ro = t.o; t.rd = t.d;
     t.inv = t.winv;
     t.oi = t.woi;
-    t.in_blas = false;
+    t.bsp = kWorld;
 }
 
 // One traversal step, in stages that a lane passes in order: up to
@@ -492,20 +492,19 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& t, Tra
     }
     if (!done) {
         if (!(t.ref & kLeafBit)) return false;  // kInnerSteps inner nodes taken
-        if (t.in_blas) {
+        if (t.bsp != kWorld) {
             if (leaf_tris<COUNT>(s, t, t.ref, cnt)) return false;
         } else {
             if (enter_instance<STACK, COUNT>(s, t, st, cnt, overflow)) return false;
         }
     }
-    // pop; an instance sentinel restores the world-space ray
-    for (;;) {
-        if (st.sp == 0) return true;
-        if (COUNT && st.sp > STACK) cnt->spill_pop++;
-        t.ref = st.pop();
-        if (t.ref != kSentinel) return false;
-        leave_instance(t);
-    }
+    // pop; back at the stack depth of its entry, the ray leaves the instance
+    // (one pop per step: no sentinel entry marks the instance boundary)
+    if (st.sp == t.bsp) leave_instance(t);
+    if (st.sp == 0) return true;
+    if (COUNT && st.sp > STACK) cnt->spill_pop++;
+    t.ref = st.pop();
+    return false;
 }
 
 // Closest hit of a world-space ray against the TLAS and the instanced
