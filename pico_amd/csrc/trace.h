@@ -273,8 +273,10 @@ __device__ __forceinline__ f3 to_world_point(const float* M, f3 P) {
 // that any of its lanes takes, so the common one (inner nodes) is batched and
 // the leaf loop is short.  Instanced 1080p frame, Mrays/s, with the leaf and
 // instance stages after the inner nodes of the same step: inner 1 / 2 / 3
-// with 1 triangle: 2628 / 2821 / 2790; inner 2 with 2 triangles: 2691.
-constexpr int kInnerSteps = 2;
+// with 1 triangle: 2628 / 2821 / 2790; inner 2 with 2 triangles: 2691.  With
+// a lane whose node misses popping its next entry into the next inner slot:
+// inner 2 / 3 / 4 / 5: 2994 / 3076 / 3045 / 3000.
+constexpr int kInnerSteps = 3;
 constexpr int kLeafTris = 1;
 
 // Two-level closest-hit traversal over 4-wide nodes.  Each node tests its
@@ -469,8 +471,10 @@ __device__ __forceinline__ void leave_instance(TraceState& t) {
 
 // One traversal step, in stages that a lane passes in order: up to
 // kInnerSteps inner nodes (each continues with its nearest hit child, the
-// others pushed farthest first); then, when that reached a leaf (or the step
-// began at one), a BLAS leaf's next triangles or a TLAS leaf's instance entry
+// others pushed farthest first; after a node that no child of hits, the next
+// stack entry is popped into the next slot); then, when that reached a leaf
+// (or the step began at one), a BLAS leaf's next triangles or a TLAS leaf's
+// instance entry
 // (the ray moves into its object space); then, when nothing is left to
 // descend into, the next stack entry is popped.  The wave runs every stage in
 // almost every step whichever lanes need it (the leaf and instance stages in
@@ -486,7 +490,15 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceState& t, Tra
     if (!(t.ref & kLeafBit)) {
 #pragma unroll
         for (int k = 0; k < kInnerSteps; ++k) {
-            if (!inner_node<STACK, COUNT>(s, t, st, cnt, overflow)) { done = true; break; }
+            if (!inner_node<STACK, COUNT>(s, t, st, cnt, overflow)) {
+                if (k + 1 == kInnerSteps) { done = true; break; }
+                // no child hit: the next stack entry takes the next slot of
+                // this step (the pop below, done here)
+                if (st.sp == t.bsp) leave_instance(t);
+                if (st.sp == 0) return true;
+                if (COUNT && st.sp > STACK) cnt->spill_pop++;
+                t.ref = st.pop();
+            }
             if (t.ref & kLeafBit) break;
         }
     }
