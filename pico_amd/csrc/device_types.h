@@ -45,11 +45,11 @@ static_assert(sizeof(DNode) == 64, "node is one 64-byte record");
 
 struct alignas(16) DNode4 {
     float origin[3];        // union lower corner of the children
-    uint32_t exps;          // int8 scale exponents: x | y << 8 | z << 16
+    float scale_x;          // 2^e of the x planes
     uint8_t qx[8];          // lo[4], hi[4]: child box = origin + q * 2^e
     uint8_t qy[8];
     uint8_t qz[8];
-    uint32_t pad[2];
+    float scale_y, scale_z; // 2^e of the y and z planes
     uint32_t ref[4];        // child references (kEmptyRef: no child)
 };
 static_assert(sizeof(DNode4) == 64, "wide node is one 64-byte record");

@@ -96,7 +96,6 @@ PICO_HD void quantise_children(const float (*lo)[3], const float (*hi)[3], int n
     memset(&nd, 0, sizeof(nd));
     for (int i = 0; i < 4; ++i) nd.ref[i] = kEmptyRef;
     uint8_t* q[3] = {nd.qx, nd.qy, nd.qz};
-    uint32_t exps = 0;
     for (int k = 0; k < 3; ++k) {
         float o = INFINITY, m = -INFINITY;
         for (int i = 0; i < nc; ++i) { o = fminf(o, lo[i][k]); m = fmaxf(m, hi[i][k]); }
@@ -126,9 +125,9 @@ PICO_HD void quantise_children(const float (*lo)[3], const float (*hi)[3], int n
             }
             if (ok || e >= 126) break;
         }
-        exps |= (uint32_t)(uint8_t)(int8_t)e << (8 * k);
+        const float sc = ldexpf(1.0f, e);
+        if (k == 0) nd.scale_x = sc; else if (k == 1) nd.scale_y = sc; else nd.scale_z = sc;
     }
-    nd.exps = exps;
 }
 
 }  // namespace pico

@@ -208,11 +208,6 @@ struct TraceStack {
     }
 };
 
-__device__ __forceinline__ int exp_of(uint32_t bits) {
-    // the node's int8 scale exponent e (in [-126, 126]) of one axis
-    return (int)(bits << 24) >> 24;
-}
-
 // The reference's order of std::min / std::max (NaN operands fall through as
 // the reference's comparisons make them).
 
@@ -334,14 +329,14 @@ __device__ __forceinline__ bool inner_node(const DevScene& s, TraceState& t, Tra
     const DNode4* n = s.nodes + t.ref;
     const float4 a = ldg4(&n->origin[0]);
     const uint4 qxy = *reinterpret_cast<const uint4*>(&n->qx[0]);
-    const uint2 qz = *reinterpret_cast<const uint2*>(&n->qz[0]);
+    const uint4 qzs = *reinterpret_cast<const uint4*>(&n->qz[0]);  // qz, scale_y, scale_z
     const uint4 r = *reinterpret_cast<const uint4*>(&n->ref[0]);
     if (COUNT) cnt->nodes++;
-    const uint32_t eb = __float_as_uint(a.w);
+    const uint2 qz = make_uint2(qzs.x, qzs.y);
     float key[4];
     uint32_t cref[4] = {r.x, r.y, r.z, r.w};
-    // 2^e inv per axis: one v_ldexp_f32 each, the same bits as the product
-    const float Ax = ldexpf(t.inv.x, exp_of(eb)), Ay = ldexpf(t.inv.y, exp_of(eb >> 8)), Az = ldexpf(t.inv.z, exp_of(eb >> 16));
+    // 2^e inv per axis: a product by a power of two (exact, as ldexp)
+    const float Ax = t.inv.x * a.w, Ay = t.inv.y * __uint_as_float(qzs.z), Az = t.inv.z * __uint_as_float(qzs.w);
     const float Bx = fmaf(a.x, t.inv.x, t.oi.x), By = fmaf(a.y, t.inv.y, t.oi.y), Bz = fmaf(a.z, t.inv.z, t.oi.z);
     // The plane distance fma(q, A, B) is monotonic in q (one rounding, A of
     // the reciprocal's sign and never 0), so on each axis the near plane is
