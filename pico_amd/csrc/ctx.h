@@ -64,6 +64,9 @@ struct pico_hip_ctx {
     // launch shape of the wavefront and the autotuner's cache
     pico_tuning tuning{kDefaultQueuePaths, 0, 0, 0};  // queue depth, tile, flags
     std::map<uint64_t, TuneEntry> tune_cache;
+    // the first bounce the last render left to its hand-over kernel at the
+    // latest (0: unknown): a render launches the bounces before it only
+    uint32_t finish_hint = 0;
     // timing
     bool timing = false;
     std::map<std::string, KernelTimes> timers;

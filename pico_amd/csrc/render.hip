@@ -81,6 +81,10 @@ struct Ctl {
     unsigned long long nodes_fin, tris_fin, insts_fin;  // k_finish's share of the node / triangle / instance counts
     unsigned long long rays_fin;                        // rays traced by k_finish, every chunk
     uint32_t finish_from;                            // first bounce left to k_finish, or kNoFinish
+    // over the render's chunks: the latest finish_from (kNoFinish once a
+    // chunk ran every bounce as launches, 0 before any), and whether a
+    // hand-over forced by the launch count took kFinishPaths or more paths
+    uint32_t finish_seen, forced_big;
     // k_finish's extension rays, shadow rays (both: those no queue count
     // holds) and every ray it traced: words 0, 1, 2 of a counter slot
     unsigned long long fin_rays[kCtrSlots * kCtrStride];
@@ -186,6 +190,10 @@ __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
         ctl->n_next = 0;
         ctl->gen_head = 0;
         ctl->finish_from = kNoFinish;
+        if (first) {
+            ctl->finish_seen = 0;
+            ctl->forced_big = 0;
+        }
         // bounce 0 uses identity slots: path i sits at queue entry i (dead
         // paths of views with fewer samples are marked kDeadPath)
         ctl->ext_count[0] = na * S;
@@ -616,18 +624,23 @@ constexpr uint32_t kShadeBatches = 4;
 constexpr uint32_t kFinishPaths = 1u << 20;
 
 // `last`: the chunk's last bounce.  With fewer than kFinishPaths paths
-// entering bounce b < last, the pass hands the rest of the chunk over to
-// k_finish: a continuing path's queue entry carries bit 31 when the path also
-// has a shadow ray (k_finish traces it first), and only the shadow rays of
-// paths that end here go to the shadow queue.
+// entering bounce b < last, or at bounce force - 1 (the last bounce the host
+// launched), the pass hands the rest of the chunk over to k_finish: a
+// continuing path's queue entry carries bit 31 when the path also has a
+// shadow ray (k_finish traces it first), and only the shadow rays of paths
+// that end here go to the shadow queue.
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, const DView* __restrict__ views, Ctl* ctl, uint32_t b,
-                                                  uint32_t last, Pool P) {
+                                                  uint32_t last, uint32_t force, Pool P) {
     if (ctl->finish_from <= b) return;
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     uint32_t* qn = (b & 1) ? P.q0 : P.q1;
     const uint32_t n = ctl->ext_count[b];
-    const bool handoff = b < last && n < kFinishPaths;
-    if (handoff && blockIdx.x == 0 && threadIdx.x == 0) ctl->finish_from = b + 1;
+    const bool forced = b + 1 >= force;
+    const bool handoff = b < last && (n < kFinishPaths || forced);
+    if (handoff && blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->finish_from = b + 1;
+        if (n >= kFinishPaths) ctl->forced_big = 1;
+    }
     const uint32_t waves = gridDim.x * (kBlock / 64);
     const uint32_t nb = n / (64u * kShadeBatches) >= 2u * waves ? kShadeBatches : 1u;
     WaveClaim wc(&ctl->shade_head[b], n, 64u * nb);
@@ -934,6 +947,8 @@ __global__ void k_sum_rays(Ctl* ctl, uint32_t bounces) {
         unsigned long long e = 0, sh = 0;
         // queues of the bounces that ran as launches, and k_finish's rays
         const uint32_t last = ctl->finish_from == kNoFinish ? bounces : ctl->finish_from;
+        if (ctl->finish_from == kNoFinish || ctl->finish_seen == kNoFinish) ctl->finish_seen = kNoFinish;
+        else ctl->finish_seen = max(ctl->finish_seen, ctl->finish_from);
         for (uint32_t b = 1; b <= last; ++b) e += ctl->ext_count[b];
         for (uint32_t b = 0; b < last; ++b) sh += ctl->sh_count[b];
         for (uint32_t k = 0; k < kCtrSlots; ++k) {
@@ -958,11 +973,12 @@ struct Plan {
 // b - 1, then the shading of bounce b (which makes the extension rays of
 // bounce b + 1 and the shadow rays of bounce b).  The shading of the last
 // bounce makes no shadow rays.
-int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, uint32_t last, const Pool& P, const DView* dviews,
-                          bool count, uint32_t grid_trace, uint32_t grid_shade, uint32_t* spill, uint32_t spill_cap) {
+int launch_bounce_kernels(pico_hip_ctx* c, Ctl* ctl, uint32_t b, uint32_t last, uint32_t force, const Pool& P,
+                          const DView* dviews, bool count, uint32_t grid_trace, uint32_t grid_shade, uint32_t* spill,
+                          uint32_t spill_cap) {
     if (count) PICO_LAUNCH(c, "trace", (k_trace<true>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
     else PICO_LAUNCH(c, "trace", (k_trace<false>), grid_trace, kBlock, 0, c->dev, ctl, b, P, spill, spill_cap);
-    PICO_LAUNCH(c, "shade", k_shade, grid_shade, kBlock, 0, c->dev, dviews, ctl, b, last, P);
+    PICO_LAUNCH(c, "shade", k_shade, grid_shade, kBlock, 0, c->dev, dviews, ctl, b, last, force, P);
     return PICO_OK;
 }
 
@@ -1112,6 +1128,13 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     // group of a shortest-first batch (its times count from the batch's epoch)
     const bool per_view_times = n_stats > 1 || hooks || (epoch && n_stats > 0);
     std::vector<hipEvent_t> view_render(nv, nullptr), view_done(nv, nullptr);
+    // Bounce launches: up to the hand-over point the last render reached (the
+    // launches after a hand-over would only return), and k_shade hands over
+    // at the last of them; the output does not depend on where the hand-over
+    // happens.  A forced hand-over of kFinishPaths or more paths moves the
+    // point one bounce later for the next render.
+    const uint32_t force = c->finish_hint >= 1 && c->finish_hint <= pl.max_depth ? c->finish_hint : kNoFinish;
+    const uint32_t n_launch = force == kNoFinish ? pl.max_depth + 1 : force;
     bool first = true;
     bool cancelled = false;
     for (const auto& chunk : sched) {
@@ -1125,9 +1148,10 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         PICO_LAUNCH(c, "reset_chunk", k_reset_chunk, 1, 256, 0, ctl, first ? 1u : 0u, S);
         first = false;
         PICO_LAUNCH(c, "generate", k_generate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
-        for (uint32_t b = 0; b <= pl.max_depth; ++b) {
+        for (uint32_t b = 0; b < n_launch; ++b) {
             int rc;
-            rc = launch_bounce_kernels(c, ctl, b, pl.max_depth, P, dv, count, grid_trace, grid_shade, spill, spill_cap);
+            rc = launch_bounce_kernels(c, ctl, b, pl.max_depth, force, P, dv, count, grid_trace, grid_shade, spill,
+                                       spill_cap);
             if (rc) return rc;
         }
         if (count) PICO_LAUNCH(c, "finish", (k_finish<true>), grid_finish, kBlock, 0, c->dev, dv, ctl, P, spill, spill_cap);
@@ -1204,6 +1228,8 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         }
     float ms = 0.0f;
     hipEventElapsedTime(&ms, e0, e1);
+    if (hc.forced_big) c->finish_hint = force + 1 <= pl.max_depth ? force + 1 : 0u;
+    else c->finish_hint = hc.finish_seen == kNoFinish ? 0u : hc.finish_seen;
     if (hc.overflow) return fail(c, PICO_ERR_LIMIT, "traversal stack overflow");
     if (stats) {
         for (uint32_t v = 0; v < n_stats; ++v) {

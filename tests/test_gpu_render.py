@@ -200,3 +200,30 @@ def test_multi_chunk_render_matches_oracle(ctx, w, h, queue):
     assert (gc == cc).mean() >= 0.999, (gc == cc).mean()
     rel = (np.abs(gp - cp) / (np.abs(cp) + 1e-3)).max(axis=1)
     assert (rel < 1e-3).mean() >= 0.99, (rel < 1e-3).mean()
+
+
+def test_bounce_launches_follow_the_last_hand_over(ctx, small_inst):
+    """A render launches the bounces up to the point where the context's last
+    render handed its paths over to the path-by-path kernel, and hands over
+    there at the latest.  The output does not depend on that point: the same
+    render in a fresh context, after itself, after a small render whose
+    hand-over came at bounce 1 (so this one is forced there with a million
+    paths left), and once more after that, gives the same pixels bit for bit."""
+    cam = small_inst.camera(1.0)
+    p = pa.params(1024, 1024, 2, 8, frame_seed=5)
+    tiny = pa.params(16, 16, 1, 8, frame_seed=5)
+    fresh = pa.Context(0)
+    try:
+        fresh.upload_scene(small_inst)
+        want = fresh.render(cam, p)
+    finally:
+        fresh.close()
+    ctx.upload_scene(small_inst)
+    runs = [ctx.render(cam, p), ctx.render(cam, p)]
+    ctx.render(cam, tiny)
+    runs += [ctx.render(cam, p), ctx.render(cam, p)]
+    for gp, gc, gv, st in runs:
+        assert np.array_equal(gc, want[1])
+        assert np.array_equal(gp.view(np.uint32), want[0].view(np.uint32))
+        assert np.array_equal(gv.view(np.uint32), want[2].view(np.uint32))
+        assert st.extension_rays == want[3].extension_rays and st.shadow_rays == want[3].shadow_rays
