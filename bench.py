@@ -648,9 +648,15 @@ def traffic_leg(args):
     for k in ("k_trace",):
         if k in res["FETCH_SIZE"] and k in res["WRITE_SIZE"]:
             fk, wk = res["FETCH_SIZE"][k][1], res["WRITE_SIZE"][k][1]
-            out[k] = {"dispatches": res["FETCH_SIZE"][k][0], "fetch_kib_per_dispatch": round(fk, 1),
-                      "write_kib_per_dispatch": round(wk, 1),
-                      "hbm_bytes_per_dispatch": round(fk * 1024 * f_scale + wk * 1024 * w_scale)}
+            nd = res["FETCH_SIZE"][k][0]
+            per = fk * 1024 * f_scale + wk * 1024 * w_scale
+            # the child's frame is its context's first render, which launches
+            # every bounce (those after the hand-over return at once and move
+            # nothing): the frame's total is what the timed frames' fewer
+            # launches move
+            out[k] = {"dispatches": nd, "fetch_kib_per_dispatch": round(fk, 1),
+                      "write_kib_per_dispatch": round(wk, 1), "hbm_bytes_per_dispatch": round(per),
+                      "hbm_bytes_per_frame": round(per * nd)}
     return out
 
 
@@ -761,7 +767,7 @@ def main():
         if not args.no_traffic:
             try:
                 tdetail = traffic_leg(args)
-                traffic = tdetail["k_trace"]["hbm_bytes_per_dispatch"]
+                traffic = round(tdetail["k_trace"]["hbm_bytes_per_frame"] / launches_per_frame)
                 log(f"[bench] traffic {tdetail}")
             except Exception as e:  # no profiler on this host: no traffic figure
                 tdetail = {"error": str(e)[:300]}
@@ -773,7 +779,8 @@ def main():
                 "traffic_frac": round(real / HBM_PEAK_GBPS, 4) if real else None,
                 "note": "achieved counts the logical bytes of every node, triangle and instance record a ray "
                         "visits; most are served from L2 and the Infinity Cache, so it can exceed the HBM peak. "
-                        "traffic is the measured HBM bytes of one launch, traffic_GBps their rate",
+                        "traffic is the measured HBM bytes of a frame's k_trace launches per launch, "
+                        "traffic_GBps their rate",
                 "measured_copy_GBps": round(copy_gbps, 1),
                 "kernel": "trace", "launches": tr_n, "mean_launch_ms": round(tr_ms, 4),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch),

@@ -23,12 +23,25 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
-// Default path budget of a render pass (paths in flight; 216 B of path state
-// each, 27 GB at 128M).  Samples per chunk = budget / live pixels, so a 1080p
+// Default path budget of a render pass (paths in flight; 184 B of path state
+// each, 23.6 GB at 128M).  Samples per chunk = budget / live pixels, so a 1080p
 // 64-spp frame runs as 2 chunks (6 + 58 samples) instead of 9 at 16M: the
 // last bounces, whose few long rays set each launch's duration, are paid
 // twice instead of nine times (1680 against 1150 Mrays/s).
 constexpr uint32_t kDefaultQueuePaths = 128u << 20;
+
+// Shape of a render chunk (live pixels, first sample, samples, depth): the
+// key of the hand-over point learned from earlier renders.
+struct FinishKey {
+    uint64_t live;
+    uint32_t off, spp, depth;
+    bool operator<(const FinishKey& o) const {
+        if (live != o.live) return live < o.live;
+        if (off != o.off) return off < o.off;
+        if (spp != o.spp) return spp < o.spp;
+        return depth < o.depth;
+    }
+};
 
 struct TuneEntry {
     pico_tuning tuning;
@@ -64,9 +77,10 @@ struct pico_hip_ctx {
     // launch shape of the wavefront and the autotuner's cache
     pico_tuning tuning{kDefaultQueuePaths, 0, 0, 0};  // queue depth, tile, flags
     std::map<uint64_t, TuneEntry> tune_cache;
-    // the first bounce the last render left to its hand-over kernel at the
-    // latest (0: unknown): a render launches the bounces before it only
-    uint32_t finish_hint = 0;
+    // per chunk shape, the first bounce an earlier render left to its
+    // hand-over kernel at the latest: a render launches the bounces before it
+    // only
+    std::map<FinishKey, uint32_t> finish_hints;
     // timing
     bool timing = false;
     std::map<std::string, KernelTimes> timers;

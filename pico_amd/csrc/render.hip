@@ -62,6 +62,8 @@ struct DView {
 // workgroup or wave of a grid adds to at the end are spread over kCtrSlots
 // slots kCtrStride 64-bit words (256 B) apart and summed by their reader.
 constexpr uint32_t kCtrSlots = 16;
+// chunks of a render whose hand-over point is recorded for later renders
+constexpr uint32_t kLogChunks = 64;
 constexpr uint32_t kCtrStride = 32;
 constexpr uint32_t kNoFinish = 0xffffffffu;
 
@@ -81,10 +83,11 @@ struct Ctl {
     unsigned long long nodes_fin, tris_fin, insts_fin;  // k_finish's share of the node / triangle / instance counts
     unsigned long long rays_fin;                        // rays traced by k_finish, every chunk
     uint32_t finish_from;                            // first bounce left to k_finish, or kNoFinish
-    // over the render's chunks: the latest finish_from (kNoFinish once a
-    // chunk ran every bounce as launches, 0 before any), and whether a
-    // hand-over forced by the launch count took kFinishPaths or more paths
-    uint32_t finish_seen, forced_big;
+    // whether this chunk's hand-over, forced by the launch count, took
+    // kFinishPaths or more paths; and per chunk (the first kLogChunks) its
+    // finish_from and that flag, which the host reads after the render
+    uint32_t forced_big;
+    uint32_t chunk_finish[kLogChunks], chunk_forced[kLogChunks];
     // k_finish's extension rays, shadow rays (both: those no queue count
     // holds) and every ray it traced: words 0, 1, 2 of a counter slot
     unsigned long long fin_rays[kCtrSlots * kCtrStride];
@@ -190,10 +193,7 @@ __global__ void k_reset_chunk(Ctl* ctl, uint32_t first, uint32_t S) {
         ctl->n_next = 0;
         ctl->gen_head = 0;
         ctl->finish_from = kNoFinish;
-        if (first) {
-            ctl->finish_seen = 0;
-            ctl->forced_big = 0;
-        }
+        ctl->forced_big = 0;
         // bounce 0 uses identity slots: path i sits at queue entry i (dead
         // paths of views with fewer samples are marked kDeadPath)
         ctl->ext_count[0] = na * S;
@@ -942,13 +942,15 @@ __global__ __launch_bounds__(kBlock) void k_compact(const DView* __restrict__ vi
     }
 }
 
-__global__ void k_sum_rays(Ctl* ctl, uint32_t bounces) {
+__global__ void k_sum_rays(Ctl* ctl, uint32_t bounces, uint32_t chunk) {
     if (threadIdx.x == 0) {
         unsigned long long e = 0, sh = 0;
         // queues of the bounces that ran as launches, and k_finish's rays
         const uint32_t last = ctl->finish_from == kNoFinish ? bounces : ctl->finish_from;
-        if (ctl->finish_from == kNoFinish || ctl->finish_seen == kNoFinish) ctl->finish_seen = kNoFinish;
-        else ctl->finish_seen = max(ctl->finish_seen, ctl->finish_from);
+        if (chunk < kLogChunks) {
+            ctl->chunk_finish[chunk] = ctl->finish_from;
+            ctl->chunk_forced[chunk] = ctl->forced_big;
+        }
         for (uint32_t b = 1; b <= last; ++b) e += ctl->ext_count[b];
         for (uint32_t b = 0; b < last; ++b) sh += ctl->sh_count[b];
         for (uint32_t k = 0; k < kCtrSlots; ++k) {
@@ -1128,17 +1130,28 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
     // group of a shortest-first batch (its times count from the batch's epoch)
     const bool per_view_times = n_stats > 1 || hooks || (epoch && n_stats > 0);
     std::vector<hipEvent_t> view_render(nv, nullptr), view_done(nv, nullptr);
-    // Bounce launches: up to the hand-over point the last render reached (the
-    // launches after a hand-over would only return), and k_shade hands over
-    // at the last of them; the output does not depend on where the hand-over
-    // happens.  A forced hand-over of kFinishPaths or more paths moves the
-    // point one bounce later for the next render.
-    const uint32_t force = c->finish_hint >= 1 && c->finish_hint <= pl.max_depth ? c->finish_hint : kNoFinish;
-    const uint32_t n_launch = force == kNoFinish ? pl.max_depth + 1 : force;
+    // Bounce launches of a chunk: up to the hand-over point that an earlier
+    // render's chunk of the same shape (live pixels, first sample, samples,
+    // depth) reached, with the hand-over forced at the last of them (the
+    // launches after a hand-over would only return); every bounce for a shape
+    // not seen yet, or whose forced hand-over took kFinishPaths or more paths.
+    // The output does not depend on where the hand-over happens.
+    std::vector<FinishKey> chunk_key;
+    std::vector<uint32_t> chunk_force;
     bool first = true;
     bool cancelled = false;
     for (const auto& chunk : sched) {
         const uint32_t off = chunk.first, S = chunk.second;
+        size_t live = 0;
+        for (auto& v : hv)
+            if (v.spp > off) live += (size_t)v.cam.w * v.cam.h;
+        const FinishKey key{(uint64_t)live, off, S, pl.max_depth};
+        const auto hit = c->finish_hints.find(key);
+        const uint32_t force = hit != c->finish_hints.end() ? hit->second : kNoFinish;
+        const uint32_t n_launch = force == kNoFinish ? pl.max_depth + 1 : force;
+        const uint32_t ci = (uint32_t)chunk_key.size();
+        chunk_key.push_back(key);
+        chunk_force.push_back(force);
         if (cancel) {
             // the flag is host memory written by another thread: look at it once
             // the previous chunk has finished on the device
@@ -1156,7 +1169,7 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         }
         if (count) PICO_LAUNCH(c, "finish", (k_finish<true>), grid_finish, kBlock, 0, c->dev, dv, ctl, P, spill, spill_cap);
         else PICO_LAUNCH(c, "finish", (k_finish<false>), grid_finish, kBlock, 0, c->dev, dv, ctl, P, spill, spill_cap);
-        PICO_LAUNCH(c, "sum_rays", k_sum_rays, 1, 64, 0, ctl, pl.max_depth);
+        PICO_LAUNCH(c, "sum_rays", k_sum_rays, 1, 64, 0, ctl, pl.max_depth, ci);
         if (samples_mode) PICO_LAUNCH(c, "finalize", k_finalize, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         PICO_LAUNCH(c, "accumulate", k_accumulate, grid_lin, kBlock, 0, dv, act_a, ctl, S, off, P);
         PICO_LAUNCH(c, "compact", k_compact, grid_lin, kBlock, 0, dv, act_a, act_b, ctl);
@@ -1228,8 +1241,23 @@ int run_views(pico_hip_ctx* c, std::vector<DView>& hv, const pico_tuning& tune, 
         }
     float ms = 0.0f;
     hipEventElapsedTime(&ms, e0, e1);
-    if (hc.forced_big) c->finish_hint = force + 1 <= pl.max_depth ? force + 1 : 0u;
-    else c->finish_hint = hc.finish_seen == kNoFinish ? 0u : hc.finish_seen;
+    {
+        // the latest hand-over of each chunk shape in this render; none when a
+        // chunk of it ran every bounce or was forced with many paths left
+        std::map<FinishKey, uint32_t> seen;
+        for (size_t k = 0; k < chunk_key.size() && k < kLogChunks; ++k) {
+            const uint32_t f = hc.chunk_finish[k];
+            const bool none = f == kNoFinish || hc.chunk_forced[k] != 0;
+            auto it = seen.find(chunk_key[k]);
+            if (it == seen.end()) seen[chunk_key[k]] = none ? kNoFinish : f;
+            else if (it->second != kNoFinish) it->second = none ? kNoFinish : std::max(it->second, f);
+        }
+        if (c->finish_hints.size() > 256) c->finish_hints.clear();
+        for (const auto& kv : seen) {
+            if (kv.second == kNoFinish || kv.second < 1 || kv.second > kv.first.depth) c->finish_hints.erase(kv.first);
+            else c->finish_hints[kv.first] = kv.second;
+        }
+    }
     if (hc.overflow) return fail(c, PICO_ERR_LIMIT, "traversal stack overflow");
     if (stats) {
         for (uint32_t v = 0; v < n_stats; ++v) {

@@ -202,28 +202,30 @@ def test_multi_chunk_render_matches_oracle(ctx, w, h, queue):
     assert (rel < 1e-3).mean() >= 0.99, (rel < 1e-3).mean()
 
 
-def test_bounce_launches_follow_the_last_hand_over(ctx, small_inst):
-    """A render launches the bounces up to the point where the context's last
-    render handed its paths over to the path-by-path kernel, and hands over
-    there at the latest.  The output does not depend on that point: the same
-    render in a fresh context, after itself, after a small render whose
-    hand-over came at bounce 1 (so this one is forced there with a million
-    paths left), and once more after that, gives the same pixels bit for bit."""
-    cam = small_inst.camera(1.0)
+def test_bounce_launches_follow_the_last_hand_over(ctx, small_inst, cornell):
+    """A render launches the bounces of a chunk up to the point where an
+    earlier render's chunk of the same shape (pixels, samples, depth) handed
+    its paths over to the path-by-path kernel, and hands over there at the
+    latest.  The output does not depend on that point: renders of the open
+    instanced scene (it hands over after a few bounces) and of the closed
+    Cornell box (its paths live longer, so the hint learnt on the other scene
+    forces its hand-over with over a million paths left), alternated in one
+    context, give the pixels of a fresh context bit for bit."""
     p = pa.params(1024, 1024, 2, 8, frame_seed=5)
-    tiny = pa.params(16, 16, 1, 8, frame_seed=5)
+    want = {}
     fresh = pa.Context(0)
     try:
-        fresh.upload_scene(small_inst)
-        want = fresh.render(cam, p)
+        for name, sc in (("inst", small_inst), ("cornell", cornell)):
+            fresh.upload_scene(sc)
+            want[name] = fresh.render(sc.camera(1.0), p)
     finally:
         fresh.close()
-    ctx.upload_scene(small_inst)
-    runs = [ctx.render(cam, p), ctx.render(cam, p)]
-    ctx.render(cam, tiny)
-    runs += [ctx.render(cam, p), ctx.render(cam, p)]
-    for gp, gc, gv, st in runs:
-        assert np.array_equal(gc, want[1])
-        assert np.array_equal(gp.view(np.uint32), want[0].view(np.uint32))
-        assert np.array_equal(gv.view(np.uint32), want[2].view(np.uint32))
-        assert st.extension_rays == want[3].extension_rays and st.shadow_rays == want[3].shadow_rays
+    for name, sc, reps in (("inst", small_inst, 2), ("cornell", cornell, 2), ("inst", small_inst, 1)):
+        ctx.upload_scene(sc)
+        for _ in range(reps):
+            gp, gc, gv, st = ctx.render(sc.camera(1.0), p)
+            w = want[name]
+            assert np.array_equal(gc, w[1]), name
+            assert np.array_equal(gp.view(np.uint32), w[0].view(np.uint32)), name
+            assert np.array_equal(gv.view(np.uint32), w[2].view(np.uint32)), name
+            assert st.extension_rays == w[3].extension_rays and st.shadow_rays == w[3].shadow_rays, name
