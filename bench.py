@@ -336,6 +336,7 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3, oracle_spp=
         views.append(v)
         keep.append(b)
     t0 = time.time()
+    saved = ctx.get_tuning()
     tuning, cal_ms, def_ms = ctx.autotune_views(views, calib_spp=4)
     log(f"[bench] autotune {time.time() - t0:.1f} s: {tuning} ({cal_ms:.1f} ms vs default {def_ms:.1f} ms)")
     net = pa.CNN(ctx, 1024, 1024, 1)
@@ -359,6 +360,9 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3, oracle_spp=
     wall, st = timed(0)
     wall2, st2 = timed(pa.TUNE_SIDE_STREAM)
     wall3, st3 = timed(pa.TUNE_SHORTEST_FIRST)
+    # the views one at a time, and the legs after this one, with the
+    # context's own launch shape
+    ctx.set_tuning(saved[0], saved[1:3], saved[3])
     lat3 = np.array([s.done_ms for s in st3])
     rays = sum(s.accumulated_rays for s in st)
     lat = np.array([s.done_ms for s in st])
