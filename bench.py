@@ -337,7 +337,9 @@ def batched_leg(ctx, pa, depth, weights, n_views=64, seed=3, reps=3, oracle_spp=
         keep.append(b)
     t0 = time.time()
     saved = ctx.get_tuning()
-    tuning, cal_ms, def_ms = ctx.autotune_views(views, calib_spp=4)
+    # calibrated on the batch itself: a scaled-down calibration cannot tell
+    # queue depths apart (every depth gives one sample per pixel per chunk)
+    tuning, cal_ms, def_ms = ctx.autotune_views(views, calib_spp=int(spps.max()))
     log(f"[bench] autotune {time.time() - t0:.1f} s: {tuning} ({cal_ms:.1f} ms vs default {def_ms:.1f} ms)")
     net = pa.CNN(ctx, 1024, 1024, 1)
     if weights and os.path.exists(weights):

@@ -1641,25 +1641,15 @@ int pico_hip_autotune_views(pico_hip_ctx* c, const pico_view* views, uint32_t n,
                 pico_tuning x{q, t[0], t[1], 0};
                 if (!(x.queue_paths == defaults.queue_paths && x.tile_w == 0)) cand.push_back(x);
             }
-        // The calibration renders max_spp / calib_spp times fewer samples, so
-        // each candidate's queue depth is scaled down by the same factor
-        // while it is timed: the calibration then runs the chunk schedule
-        // (chunks per view, samples per chunk) that the candidate gives the
-        // full render.
-        const double ratio = std::max(1.0, (double)max_spp / (double)calib_spp);
-        auto scaled = [&](pico_tuning x) {
-            x.queue_paths = (uint32_t)std::max<double>(1u << 20, std::floor((double)x.queue_paths / ratio));
-            return x;
-        };
         // warm-up at the deepest queue: the work buffers reach their largest size
         std::vector<DView> w = hv;
-        if ((rc = run_views(c, w, scaled(pico_tuning{128u << 20, 0, 0, 0}), nullptr, false, nullptr, 0))) return rc;
+        if ((rc = run_views(c, w, pico_tuning{64u << 20, 0, 0, 0}, nullptr, false, nullptr, 0))) return rc;
         std::vector<float> best(cand.size(), 1e30f);
         for (int rep = 0; rep < 2; ++rep)
             for (size_t k = 0; k < cand.size(); ++k) {
                 std::vector<DView> v = hv;
                 pico_stats st;
-                if ((rc = run_views(c, v, scaled(cand[k]), nullptr, false, &st, 1))) return rc;
+                if ((rc = run_views(c, v, cand[k], nullptr, false, &st, 1))) return rc;
                 best[k] = std::min(best[k], (float)st.render_ms);
             }
         size_t kb = 0;
