@@ -503,9 +503,13 @@ __device__ __forceinline__ void conv3r_body(const ConvArgs& a, bf16* lds) {
                         float mx = gl[0];
 #pragma unroll
                         for (int k = 1; k < 25; ++k) mx = fmaxf(mx, gl[k]);
+                        // softmax with the hardware's base-2 exponential of
+                        // the logits scaled by log2(e) (one v_exp_f32 and an
+                        // FMA per logit instead of a full expf)
                         float sum = 0.0f;
+                        const float kL2e = 1.44269504088896340736f, mxs = mx * kL2e;
 #pragma unroll
-                        for (int k = 0; k < 25; ++k) { gl[k] = expf(gl[k] - mx); sum += gl[k]; }
+                        for (int k = 0; k < 25; ++k) { gl[k] = __builtin_amdgcn_exp2f(fmaf(gl[k], kL2e, -mxs)); sum += gl[k]; }
                         const float is = 1.0f / sum;
                         const float* rad = a.rad + (size_t)b * 10 * a.img_h * a.img_w;
                         const size_t plane = (size_t)a.img_h * a.img_w;
